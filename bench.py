@@ -1,0 +1,195 @@
+"""Throughput of the explicit finite-volume update on one MI355X.
+
+A step is one RK stage (face reconstruction + HLLC residual + RK update +
+pressure relaxation) over every leaf of the workload mesh, with the state
+resident in HBM.  The default workload is the 3-D Rayleigh collapse of an air
+bubble in water (4 mm cube, R = 0.4 mm, water stiffened gas at 50 bar, air at
+1 bar, tanh-smeared interface) on an octree adapted to reference levels 5-9
+(16^3 to 256^3 equivalent) by the interface and pressure-jump indicator, the
+plain FP64 path.  Synthetic initial data, no I/O.
+
+Prints one JSON line (metric of the contract) and, with --profile-only,
+nothing but runs the timed loop (for rocprofv3).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import forestfv_public_amd as F  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "Mcell-updates/s per RK stage (plain FP64 vs FP32+MLP) @ L1 err; HBM GB/s % peak"
+
+L_DOMAIN = 4e-3
+R_BUBBLE = 0.4e-3
+WATER_AIR = dict(gamma=(6.12, 1.4), pinf=(3.43e8, 0.0))
+RHO0 = (1000.0, 1.2)
+P0 = (5e6, 1e5)
+
+
+def algorithmic_bytes(neq):
+    """Per leaf per RK stage: read S^k, read S^n, write S^k+1 in FP64, six int32
+    neighbour ids and an 8-byte volume: B = 24 NEQ + 32."""
+    return 24 * neq + 32
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def c1_params(dt=None, prelax=1):
+    return F.make_params(nmat=2, gamma=WATER_AIR["gamma"], pinf=WATER_AIR["pinf"], amin=1e-6, avf_lim=0.5,
+                         prelax=prelax, rk=F.RK_SSP3, limiter=0, dt=dt if dt else 1e-9, bc=(2, 2, 2, 2, 2, 2))
+
+
+def c1_crit(maxlevel, layers):
+    return F.make_crit(uniform_on=1, uniform_lvl=5, prs_on=1, prs_lvl=maxlevel, prs_thr=0.2,
+                       inter0_on=1, inter0_lvl=maxlevel, inter0_lrs=layers, inter1_on=0)
+
+
+def build_c1(maxlevel=9, base=5, layers=16, log_fn=log):
+    """Geometric pre-adaptation (main.c:107-134): init, indicator, smoothing, refine, repeated."""
+    p = c1_params()
+    m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), L_DOMAIN, base, list(p.bc))
+    crit = c1_crit(maxlevel, layers)
+    c = (0.5 * L_DOMAIN,) * 3
+    for ipass in range(maxlevel - base + 1):
+        S = m.init_bubble(p, RHO0, P0, c, R_BUBBLE, 2.0, 1234, 0.01)
+        dev = F.Device(p, m)
+        dev.set_state(S)
+        SI, split = dev.indicator(crit)
+        dev.close()
+        m.set_SI(SI)
+        split = m.criterionsmth(maxlevel, split)
+        S = m.adapt(p, split, S)
+        log_fn(f"  geo pass {ipass}: {m.n} leaves, levels {np.bincount(m.arrays()['level'])[base:]}")
+    S = m.init_bubble(p, RHO0, P0, c, R_BUBBLE, 2.0, 1234, 0.01)
+    dev = F.Device(p, m)
+    dev.set_state(S)
+    SI, _ = dev.indicator(crit)
+    dev.close()
+    m.set_SI(SI)
+    hmin = L_DOMAIN / (1 << (maxlevel - 1))
+    cw = np.sqrt(6.12 * (P0[0] + 3.43e8) / 1000.0)
+    p.dt = 0.4 * hmin / cw
+    return p, m, S
+
+
+def build_uniform(level):
+    p = c1_params()
+    m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), L_DOMAIN, level, list(p.bc))
+    S = m.init_bubble(p, RHO0, P0, (0.5 * L_DOMAIN,) * 3, R_BUBBLE, 2.0, 1234, 0.01)
+    hmin = L_DOMAIN / (1 << (level - 1))
+    p.dt = 0.4 * hmin / np.sqrt(6.12 * (P0[0] + 3.43e8) / 1000.0)
+    return p, m, S
+
+
+def cpu_baseline(seconds=12.0):
+    """The CPU restatement (oracle/, 1 core, gcc -O2) on Sod 32^3 (config C0), timed on
+    whole RK stages until about `seconds` of work; also checks the device against it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from cases import sod_case
+    p, m, S = sod_case(level=6, rk=F.RK_SSP2, prelax=0)
+    om = O.OracleMesh(m)
+    t0 = time.perf_counter()
+    om.advance(p, S, 1)
+    one = time.perf_counter() - t0
+    nsteps = max(2, int(seconds / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    So = om.advance(p, S, nsteps)
+    t = time.perf_counter() - t0
+    stages = 2 * nsteps
+    dev = F.Device(p, m)
+    dev.set_state(S)
+    dev.advance(nsteps)
+    Sg = dev.get_state()
+    dev.close()
+    num = np.abs(Sg - So).sum(0)
+    den = np.abs(So).sum(0)
+    den[den == 0] = 1
+    l1 = float((num / den).max())
+    return {
+        "value": m.n * stages / t / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": 1, "kind": "port",
+        "sample": f"Sod 32^3 (32768 leaves), SSP-RK2, {stages} stages, {t:.1f} s, gcc -O2 -ffp-contract=off",
+    }, l1, stages
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c1", choices=["c1", "u128", "u256", "u320"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-only", action="store_true")
+    ap.add_argument("--layers", type=int, default=16)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo design is "
+              "described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+
+    t_setup = time.perf_counter()
+    if a.workload == "c1":
+        p, m, S = build_c1(layers=a.layers)
+        cfg = {"workload": "c1_rayleigh_collapse_amr_L5-9", "leaves": m.n, "levels": "5-9 (config 4-8)",
+               "domain_m": L_DOMAIN, "bubble_radius_m": R_BUBBLE, "interface_layers": a.layers}
+    else:
+        lvl = {"u128": 8, "u256": 9, "u320": None}[a.workload]
+        if lvl is None:
+            p = c1_params()
+            m = F.Mesh.uniform((5, 5, 5), (0, 0, 0), L_DOMAIN / 5, 7, list(p.bc))
+            S = m.init_bubble(p, RHO0, P0, (0.5 * L_DOMAIN,) * 3, R_BUBBLE, 2.0, 1234, 0.01)
+            p.dt = 0.4 * (L_DOMAIN / 320) / np.sqrt(6.12 * (P0[0] + 3.43e8) / 1000.0)
+        else:
+            p, m, S = build_uniform(lvl)
+        cfg = {"workload": f"uniform_{a.workload[1:]}^3_water_air_bubble", "leaves": m.n}
+    nq = F.neq(p.nmat)
+    cfg.update({"prelax": p.prelax, "rk": "SSP-RK3", "limiter": "minmod", "neq": nq})
+    log(f"setup {time.perf_counter() - t_setup:.1f} s, {m.n} leaves")
+
+    dev = F.Device(p, m)
+    dev.set_state(S)
+    dev.time_stages(max(1, a.warmup))
+    t0 = time.perf_counter()
+    ms_stage, ms_flux = dev.time_stages(a.steps)
+    wall = time.perf_counter() - t0
+    Sg = dev.get_state()
+    dev.close()
+    if not np.all(np.isfinite(Sg)):
+        raise SystemExit("non-finite state after the timed stages")
+    if a.profile_only:
+        return
+
+    n = m.n
+    value = n / (ms_stage * 1e-3) / 1e6
+    B = algorithmic_bytes(nq)
+    achieved = B * n / (ms_stage * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s per RK stage",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_stage, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic", "config": cfg,
+        "wall_s_timed": round(wall, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "stage = k_recon + k_flux_rk", "algorithmic_bytes_per_leaf": B,
+                     "ms_k_flux_rk": round(ms_flux, 4)},
+    }
+    if not a.no_cpu_baseline:
+        cb, l1, stages = cpu_baseline()
+        out["cpu_baseline"] = cb
+        out["l1_rel_vs_cpu"] = l1
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,444 @@
+"""MI355X-native explicit finite-volume update of ForestFV.
+
+The product is C host code (octree, AMR, packing: ``host/``) and gfx950 HIP
+kernels behind the C ABI of ``include/forestfv_dev.h`` (``csrc/``), built
+into ``libforestfv.so``.  This module is a thin ctypes layer over that
+library for tests and benchmarks; it adds no computation of its own.
+
+The entry points mirror the reference's per-RUN driver
+(src/c/advancetimeexplicit.c:3-29):
+
+    mesh  = Mesh.uniform((1, 1, 1), (0, 0, 0), 1.0, level=6, bc=[4]*6)
+    dev   = Device(params, mesh)          # fv_dev_attach
+    dev.set_state(S)                      # LEAF.S, [n][NEQ]
+    dev.advance(100)                      # advancetimeexplicit x 100
+    S     = dev.get_state()               # fv_dev_get_state
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libforestfv.so")
+
+FV_MAXMAT = 4
+FV_MAXLEVEL = 10
+
+FV_OK, FV_ERR_NAN, FV_ERR_ARG, FV_ERR_HIP, FV_ERR_MESH, FV_ERR_NOMEM = 0, -1, -2, -3, -4, -5
+FK_BOUNDARY, FK_SAME, FK_MIRROR, FK_COARSE, FK_FINE = 0, 1, 2, 3, 4
+RK_REF2, RK_SSP3, RK_SSP2 = 2, 3, 22
+PREC_FP64, PREC_MIXED = 0, 1
+FIX_NONCONS, FIX_ENERGY, FIX_ALL = 1, 2, 3
+
+
+class FVError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"forestfv status {code}: {msg}")
+        self.code = code
+
+
+class fv_params(C.Structure):
+    _fields_ = [
+        ("nmat", C.c_int),
+        ("gamma", C.c_double * FV_MAXMAT),
+        ("pinf", C.c_double * FV_MAXMAT),
+        ("amin", C.c_double),
+        ("avf_lim", C.c_double),
+        ("prelax", C.c_int),
+        ("rk", C.c_int),
+        ("limiter", C.c_int),
+        ("dt", C.c_double),
+        ("bc", C.c_int * 6),
+        ("fix", C.c_int),
+    ]
+
+
+class fv_crit(C.Structure):
+    _fields_ = [
+        ("uniform_on", C.c_int), ("uniform_lvl", C.c_int),
+        ("prs_on", C.c_int), ("prs_lvl", C.c_int), ("prs_thr", C.c_double),
+        ("rho_on", C.c_int), ("rho_lvl", C.c_int), ("rho_thr", C.c_double),
+        ("vel_on", C.c_int), ("vel_lvl", C.c_int), ("vel_thr", C.c_double),
+        ("inter0_on", C.c_int), ("inter0_lvl", C.c_int), ("inter0_lrs", C.c_int),
+        ("inter1_on", C.c_int), ("inter1_lvl", C.c_int), ("inter1_lrs", C.c_int),
+    ]
+
+
+class fv_mesh(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("nroot", C.c_int * 3),
+        ("origin", C.c_double * 3),
+        ("rootsize", C.c_double),
+        ("maxlevel", C.c_int),
+        ("root", C.POINTER(C.c_int)),
+        ("level", C.POINTER(C.c_int)),
+        ("ci", C.POINTER(C.c_int)),
+        ("tag", C.POINTER(C.c_int)),
+        ("lfc_neigs", C.POINTER(C.c_int)),
+        ("neigtr", C.POINTER(C.c_int)),
+        ("bc", C.POINTER(C.c_int)),
+        ("SI", C.POINTER(C.c_int)),
+        ("bcside", C.c_int * 6),
+        ("hkey", C.c_void_p),
+        ("hval", C.c_void_p),
+        ("hcap", C.c_size_t),
+    ]
+
+
+_L2 = FV_MAXLEVEL + 2
+
+
+class fv_geo(C.Structure):
+    _fields_ = [
+        ("frame", (C.c_double * 9) * 6),
+        ("h", C.c_double * _L2),
+        ("area", C.c_double * _L2),
+        ("vol", C.c_double * _L2),
+        ("dcf", (C.c_double * _L2) * 3),
+        ("dcc", (C.c_double * _L2) * 3),
+    ]
+
+
+class fv_pack(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("nbr", C.POINTER(C.c_int)),
+        ("info", C.POINTER(C.c_uint32)),
+        ("nfine", C.c_int),
+        ("fine4", C.POINTER(C.c_int)),
+        ("geo", fv_geo),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The product library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FVError(FV_ERR_ARG, f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        sig = {
+            "fv_mesh_uniform": (P(fv_mesh), [P(C.c_int), P(C.c_double), C.c_double, C.c_int, P(C.c_int)]),
+            "fv_mesh_from_leaves": (P(fv_mesh), [P(C.c_int), P(C.c_double), C.c_double, P(C.c_int), C.c_int,
+                                                 P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]),
+            "fv_mesh_free": (None, [P(fv_mesh)]),
+            "fv_mesh_find": (C.c_int, [P(fv_mesh), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "fv_mesh_centre": (None, [P(fv_mesh), C.c_int, P(C.c_double)]),
+            "fv_tag_from_address": (C.c_int, [C.c_int, P(C.c_int)]),
+            "fv_address_from_tag": (C.c_int, [C.c_int, P(C.c_int), P(C.c_int)]),
+            "fv_mesh_pack": (C.c_int, [P(fv_mesh), P(fv_pack)]),
+            "fv_pack_free": (None, [P(fv_pack)]),
+            "fv_geo_build": (None, [C.c_double, P(fv_geo)]),
+            "fv_criterionsmth": (C.c_int, [P(fv_mesh), C.c_int, P(C.c_int)]),
+            "fv_mesh_adapt": (C.c_int, [P(P(fv_mesh)), P(fv_params), P(C.c_int), P(P(C.c_double))]),
+            "fv_neq": (C.c_int, [C.c_int]),
+            "fv_relax_host": (None, [P(fv_params), P(C.c_double)]),
+            "fv_init_pure": (None, [P(fv_params), P(C.c_double), C.c_int, C.c_double, P(C.c_double), P(C.c_double)]),
+            "fv_init_split_x": (C.c_int, [P(fv_mesh), P(fv_params), P(C.c_double), P(C.c_double), C.c_double,
+                                          P(C.c_double)]),
+            "fv_init_bubble": (C.c_int, [P(fv_mesh), P(fv_params), P(C.c_double), P(C.c_double), P(C.c_double),
+                                         C.c_double, C.c_double, C.c_uint, C.c_double, P(C.c_double)]),
+            "fv_dev_attach": (C.c_int, [P(C.c_void_p), P(fv_params), P(fv_pack)]),
+            "fv_dev_reattach": (C.c_int, [C.c_void_p, P(fv_pack)]),
+            "fv_dev_detach": (C.c_int, [C.c_void_p]),
+            "fv_dev_set_state": (C.c_int, [C.c_void_p, P(C.c_double)]),
+            "fv_dev_get_state": (C.c_int, [C.c_void_p, P(C.c_double)]),
+            "fv_dev_get_faces": (C.c_int, [C.c_void_p, P(C.c_double)]),
+            "fv_dev_advance": (C.c_int, [C.c_void_p, C.c_int]),
+            "fv_dev_begin_step": (C.c_int, [C.c_void_p]),
+            "fv_dev_stage": (C.c_int, [C.c_void_p, C.c_int]),
+            "fv_dev_set_dt": (C.c_int, [C.c_void_p, C.c_double]),
+            "fv_dev_cfl_dt": (C.c_int, [C.c_void_p, C.c_double, P(C.c_double)]),
+            "fv_dev_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+            "fv_dev_indicator": (C.c_int, [C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int)]),
+            "fv_dev_time_stages": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
+            "fv_dev_size": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
+            "fv_dev_last_error": (C.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(rc, what):
+    if rc != FV_OK:
+        msg = lib().fv_dev_last_error()
+        raise FVError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+
+def neq(nmat):
+    return 3 * nmat + 3
+
+
+def make_params(nmat=2, gamma=(1.4, 1.4), pinf=(0.0, 0.0), amin=1e-8, avf_lim=0.5, prelax=0,
+                rk=RK_SSP3, limiter=0, dt=1e-3, bc=(4, 4, 4, 4, 4, 4), fix=FIX_ALL):
+    p = fv_params()
+    p.nmat = nmat
+    for k in range(nmat):
+        p.gamma[k] = gamma[k]
+        p.pinf[k] = pinf[k]
+    p.amin, p.avf_lim, p.prelax, p.rk, p.limiter, p.dt = amin, avf_lim, prelax, rk, limiter, dt
+    for k in range(6):
+        p.bc[k] = bc[k]
+    p.fix = fix
+    return p
+
+
+def init_pure(params, rho_init, mat, pres, u=(0.0, 0.0, 0.0)):
+    """Cell state of pure material `mat` at pressure `pres` (init_S.c:70-107)."""
+    S = np.zeros(neq(params.nmat))
+    r = np.ascontiguousarray(rho_init, dtype=np.float64)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    lib().fv_init_pure(C.byref(params), _dp(r), mat, pres, _dp(uu), _dp(S))
+    return S
+
+
+def make_crit(**kw):
+    c = fv_crit()
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Mesh:
+    """Forest of octrees on the host (fv_mesh); arrays are copied out as numpy."""
+
+    def __init__(self, ptr):
+        if not ptr:
+            raise FVError(FV_ERR_MESH, "mesh construction failed")
+        self._p = ptr
+
+    @classmethod
+    def uniform(cls, nroot, origin, rootsize, level, bc):
+        nr = (C.c_int * 3)(*nroot)
+        og = (C.c_double * 3)(*origin)
+        b = (C.c_int * 6)(*bc)
+        return cls(lib().fv_mesh_uniform(nr, og, rootsize, level, b))
+
+    @classmethod
+    def from_leaves(cls, nroot, origin, rootsize, bc, root, level, ci):
+        root = np.ascontiguousarray(root, dtype=np.int32)
+        level = np.ascontiguousarray(level, dtype=np.int32)
+        ci = np.ascontiguousarray(ci, dtype=np.int32).reshape(-1, 3)
+        st = C.c_int(0)
+        nr = (C.c_int * 3)(*nroot)
+        og = (C.c_double * 3)(*origin)
+        b = (C.c_int * 6)(*bc)
+        p = lib().fv_mesh_from_leaves(nr, og, rootsize, b, len(root), _ip(root), _ip(level), _ip(ci), C.byref(st))
+        if not p:
+            raise FVError(st.value, "fv_mesh_from_leaves")
+        return cls(p)
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.fv_mesh_free(self._p)
+            self._p = None
+
+    @property
+    def m(self):
+        return self._p.contents
+
+    @property
+    def n(self):
+        return self.m.n
+
+    def _arr(self, name, shape):
+        ptr = getattr(self.m, name)
+        size = int(np.prod(shape))
+        return np.ctypeslib.as_array(ptr, shape=(size,)).reshape(shape).copy()
+
+    def arrays(self):
+        n = self.n
+        return dict(
+            root=self._arr("root", (n,)), level=self._arr("level", (n,)), ci=self._arr("ci", (n, 3)),
+            tag=self._arr("tag", (n,)), lfc_neigs=self._arr("lfc_neigs", (n, 6)),
+            neigtr=self._arr("neigtr", (n, 6, 4)), bc=self._arr("bc", (n, 6)), SI=self._arr("SI", (n, 2)),
+        )
+
+    def set_SI(self, SI):
+        SI = np.ascontiguousarray(SI, dtype=np.int32).reshape(-1)
+        C.memmove(self.m.SI, SI.ctypes.data, SI.nbytes)
+
+    def centres(self):
+        out = np.zeros((self.n, 3))
+        c = (C.c_double * 3)()
+        for i in range(self.n):
+            lib().fv_mesh_centre(self._p, i, c)
+            out[i] = c[:]
+        return out
+
+    def find(self, root, level, i, j, k):
+        return lib().fv_mesh_find(self._p, root, level, i, j, k)
+
+    def criterionsmth(self, LEVEL, split):
+        split = np.ascontiguousarray(split, dtype=np.int32).copy()
+        _check(lib().fv_criterionsmth(self._p, LEVEL, _ip(split)), "fv_criterionsmth")
+        return split
+
+    def adapt(self, params, split, S):
+        """Merge/split by target level (fv_mesh_adapt); the mesh is replaced in place and
+        the remapped state [n][NEQ] is returned."""
+        nq = neq(params.nmat)
+        split = np.ascontiguousarray(split, dtype=np.int32)
+        sp = C.cast(C.c_void_p(lib_malloc(S)), C.POINTER(C.c_double))
+        pm = C.pointer(self._p)
+        ps = C.pointer(sp)
+        rc = lib().fv_mesh_adapt(pm, C.byref(params), _ip(split), ps)
+        if rc != FV_OK:
+            libc_free(ps.contents)
+            _check(rc, "fv_mesh_adapt")
+        self._p = pm.contents
+        out = np.ctypeslib.as_array(ps.contents, shape=(self.n * nq,)).reshape(self.n, nq).copy()
+        libc_free(ps.contents)
+        return out
+
+    def pack(self):
+        return Pack(self)
+
+    def init_split_x(self, params, rho_init, p_init, xs):
+        S = np.zeros((self.n, neq(params.nmat)))
+        r = np.ascontiguousarray(rho_init, dtype=np.float64)
+        p = np.ascontiguousarray(p_init, dtype=np.float64)
+        _check(lib().fv_init_split_x(self._p, C.byref(params), _dp(r), _dp(p), xs, _dp(S)), "fv_init_split_x")
+        return S
+
+    def init_bubble(self, params, rho_init, p_init, centre, radius, smear=2.0, seed=1234, perturb=0.0):
+        S = np.zeros((self.n, neq(params.nmat)))
+        r = np.ascontiguousarray(rho_init, dtype=np.float64)
+        p = np.ascontiguousarray(p_init, dtype=np.float64)
+        c = np.ascontiguousarray(centre, dtype=np.float64)
+        _check(lib().fv_init_bubble(self._p, C.byref(params), _dp(r), _dp(p), _dp(c), radius, smear, seed,
+                                    perturb, _dp(S)), "fv_init_bubble")
+        return S
+
+
+_libc = C.CDLL(None)
+_libc.malloc.restype = C.c_void_p
+_libc.malloc.argtypes = [C.c_size_t]
+_libc.free.argtypes = [C.c_void_p]
+
+
+def lib_malloc(S):
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    p = _libc.malloc(max(S.nbytes, 8))
+    C.memmove(p, S.ctypes.data, S.nbytes)
+    return p
+
+
+def libc_free(ptr):
+    _libc.free(C.cast(ptr, C.c_void_p))
+
+
+class Pack:
+    """Device-bound image of a mesh (fv_pack)."""
+
+    def __init__(self, mesh):
+        self.pk = fv_pack()
+        _check(lib().fv_mesh_pack(mesh._p, C.byref(self.pk)), "fv_mesh_pack")
+        self.n = self.pk.n
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "pk", None) is not None:
+            _lib.fv_pack_free(C.byref(self.pk))
+            self.pk = None
+
+    def arrays(self):
+        n, nf = self.pk.n, self.pk.nfine
+        nbr = np.ctypeslib.as_array(self.pk.nbr, shape=(6 * n,)).reshape(6, n).copy()
+        info = np.ctypeslib.as_array(self.pk.info, shape=(n,)).copy()
+        fine4 = np.ctypeslib.as_array(self.pk.fine4, shape=(4 * max(nf, 1),)).reshape(-1, 4)[:nf].copy()
+        return dict(nbr=nbr, info=info, fine4=fine4)
+
+
+class Device:
+    """One device context (fv_dev) over a packed mesh."""
+
+    def __init__(self, params, mesh_or_pack):
+        pk = mesh_or_pack if isinstance(mesh_or_pack, Pack) else Pack(mesh_or_pack)
+        self._pack = pk
+        self.params = params
+        self.nq = neq(params.nmat)
+        self.n = pk.n
+        h = C.c_void_p()
+        _check(lib().fv_dev_attach(C.byref(h), C.byref(params), C.byref(pk.pk)), "fv_dev_attach")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fv_dev_detach(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def reattach(self, mesh):
+        pk = Pack(mesh)
+        _check(lib().fv_dev_reattach(self._h, C.byref(pk.pk)), "fv_dev_reattach")
+        self._pack = pk
+        self.n = pk.n
+
+    def set_state(self, S):
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        assert S.shape == (self.n, self.nq), (S.shape, self.n, self.nq)
+        _check(lib().fv_dev_set_state(self._h, _dp(S)), "fv_dev_set_state")
+
+    def get_state(self):
+        S = np.empty((self.n, self.nq))
+        _check(lib().fv_dev_get_state(self._h, _dp(S)), "fv_dev_get_state")
+        return S
+
+    def get_faces(self):
+        SF = np.empty((self.n, 6, self.nq))
+        _check(lib().fv_dev_get_faces(self._h, _dp(SF)), "fv_dev_get_faces")
+        return SF
+
+    def advance(self, nsteps):
+        _check(lib().fv_dev_advance(self._h, nsteps), "fv_dev_advance")
+
+    def begin_step(self):
+        _check(lib().fv_dev_begin_step(self._h), "fv_dev_begin_step")
+
+    def stage(self, irk):
+        _check(lib().fv_dev_stage(self._h, irk), "fv_dev_stage")
+
+    def set_dt(self, dt):
+        _check(lib().fv_dev_set_dt(self._h, dt), "fv_dev_set_dt")
+
+    def set_precision(self, prec):
+        _check(lib().fv_dev_set_precision(self._h, prec), "fv_dev_set_precision")
+
+    def cfl_dt(self, cfl):
+        d = C.c_double()
+        _check(lib().fv_dev_cfl_dt(self._h, cfl, C.byref(d)), "fv_dev_cfl_dt")
+        return d.value
+
+    def indicator(self, crit):
+        SI = np.zeros((self.n, 2), dtype=np.int32)
+        split = np.zeros(self.n, dtype=np.int32)
+        _check(lib().fv_dev_indicator(self._h, C.byref(crit), _ip(SI), _ip(split)), "fv_dev_indicator")
+        return SI, split
+
+    def time_stages(self, nrep):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().fv_dev_time_stages(self._h, nrep, C.byref(a), C.byref(b)), "fv_dev_time_stages")
+        return a.value, b.value

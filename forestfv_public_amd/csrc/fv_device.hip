@@ -1,0 +1,792 @@
+// fv_device.hip -- MI355X (gfx950) kernels of the explicit finite-volume
+// update and the C ABI declared in include/forestfv_dev.h.
+//
+// Data in HBM (n leaves in DFS order):
+//   S, SN          [NEQ][n]  FP64 structure of arrays (coalesced per field)
+//   SF             [6][NEQ][n] reconstructed face states of the last stage
+//   nbr            [6][n]    neighbour leaf / fine-list row / -bc code
+//   info           [n]       level | SI[0]!=0 | 3-bit face kind per face
+//   fine4          [nfine][4] four finer neighbours of a 1:4 face
+// Geometry of axis-aligned cubes comes from per-level tables passed by
+// value (Geo), so no per-face geometry is streamed.
+//
+// Stage = recon (per leaf: 6 MUSCL face states + relaxation, m4_faceinterp_V1)
+//       + flux_rk (per leaf: residual by gathering its own faces in face
+//         order 0..5, RK stage, relaxation).  A face shared by two
+//         same-level leaves outside interface layers is evaluated by the
+//         later leaf exactly as the earlier one evaluates it (its frame, its
+//         left state) and negated, which is the reference's mirror reuse
+//         (m4_residual.c:62-85) without flags or atomics: the result is
+//         deterministic and bit-identical to the sequential loop.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include "forestfv_dev.h"
+#include "fv_phys.h"
+
+using namespace fvk;
+
+#define FV_BLOCK 256
+
+// ---------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------
+
+// gather of the MUSCL stencil value across face f (m4_facevalues_MUSCL.c:3-44)
+template <int NM>
+FV_DEV void stencil_value(const double *__restrict__ S, int n, int i, int kind, int nb,
+                          const int *__restrict__ fine4, const double *Sc, const double *Araw, double a,
+                          double *X) {
+  using L = Lay<NM>;
+  if (kind == FV_FK_BOUNDARY) {
+    // ghost = own state with the clipped volume fraction (m4_facevalues.c:48-80),
+    // clipped from the raw cell value as the reference does
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) X[v] = Sc[v];
+    double at[NM], sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < L::N3; ++k) {
+      at[k] = Araw[k];
+      if (at[k] <= a) at[k] = a;
+      else if (at[k] >= (1.0 - (a))) at[k] = 1.0 - (a);
+      sum += at[k];
+    }
+    if (sum > (1.0 - (a))) {
+      double dif = sum - (1.0 - (a));
+#pragma unroll
+      for (int j = 0; j < L::N3; ++j) { double per = at[j] / sum; at[j] = at[j] - per * dif; }
+    }
+#pragma unroll
+    for (int k = 0; k < L::N3; ++k) X[L::I3 + k] = at[k];
+  } else if (kind == FV_FK_FINE) {
+    const int4 q = *reinterpret_cast<const int4 *>(&fine4[4 * nb]);
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) {
+      const double *Sv = S + (size_t)v * n;
+      double acc = 0.0;
+      acc += Sv[q.x] / ((double)4);
+      acc += Sv[q.y] / ((double)4);
+      acc += Sv[q.z] / ((double)4);
+      acc += Sv[q.w] / ((double)4);
+      X[v] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) X[v] = S[(size_t)v * n + nb];
+  }
+  avf_limit<NM, double>(a, X);
+}
+
+FV_DEV void dist_of(const Geo &G, int kind, int lvl, double &dcf, double &dcc) {
+  if (kind == FV_FK_SAME || kind == FV_FK_MIRROR) { dcf = G.dcf[0][lvl]; dcc = G.dcc[0][lvl]; }
+  else if (kind == FV_FK_FINE) { dcf = G.dcf[1][lvl]; dcc = G.dcc[1][lvl]; }
+  else if (kind == FV_FK_COARSE) { dcf = G.dcf[2][lvl]; dcc = G.dcc[2][lvl]; }
+  else { dcf = 0.0; dcc = 0.0; }
+}
+
+// m4_faceinterp_V1 (m4_faceinterp.c:59-88) for one leaf per thread
+template <int NM>
+__global__ __launch_bounds__(FV_BLOCK) void k_recon(int n, Params P, Geo G, const double *__restrict__ S,
+                                                    const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+                                                    const int *__restrict__ fine4, double *__restrict__ SF) {
+  using L = Lay<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = P.amin / 100.0;
+  const uint32_t inf = info[i];
+  const int lvl = cell_level(inf);
+  double Sc[L::NEQ], Araw[NM];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
+#pragma unroll
+  for (int k = 0; k < L::N3; k++) Araw[k] = Sc[L::I3 + k];
+  avf_limit<NM, double>(a, Sc);
+  // axis pairs (f, fo): (0,2), (1,3), (4,5); both faces of a pair share the two stencil values
+#pragma unroll 1
+  for (int ax = 0; ax < 3; ax++) {
+    const int fa = (ax == 0) ? 0 : (ax == 1 ? 1 : 4);
+    const int fb = FCOPP_D[fa];
+    const int ka = face_kind(inf, fa), kb = face_kind(inf, fb);
+    double Xa[L::NEQ], Xb[L::NEQ];
+    stencil_value<NM>(S, n, i, ka, nbr[(size_t)fa * n + i], fine4, Sc, Araw, a, Xa);
+    stencil_value<NM>(S, n, i, kb, nbr[(size_t)fb * n + i], fine4, Sc, Araw, a, Xb);
+    double dcf_a, dcc_a, dcf_b, dcc_b;
+    dist_of(G, ka, lvl, dcf_a, dcc_a);
+    dist_of(G, kb, lvl, dcf_b, dcc_b);
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const int f = side ? fb : fa;
+      const int kf = side ? kb : ka;
+      const double *Xf = side ? Xb : Xa;
+      const double *Xk = side ? Xa : Xb;
+      const double dcf = side ? dcf_b : dcf_a;
+      const double dp = side ? dcc_b : dcc_a;
+      const double dm = side ? dcc_a : dcc_b;
+      double F[L::NEQ];
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) {
+        double df = Xf[v] - Sc[v];
+        double db = Xk[v] - Sc[v];
+        double g = 0.0;
+        if (kf != FV_FK_BOUNDARY) g = limited_slope<double>(P.limiter, df, db, dp, dm);
+        F[v] = Sc[v] + dcf * g;
+      }
+      vf_limit_face<NM, double>(a, F);
+      relax<NM>(P, F);
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) SF[((size_t)f * L::NEQ + v) * n + i] = F[v];
+    }
+  }
+}
+
+template <int NM>
+FV_DEV void load_face(const double *__restrict__ SF, int n, int cell, int f, double *X) {
+  using L = Lay<NM>;
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) X[v] = SF[((size_t)f * L::NEQ + v) * n + cell];
+}
+
+// RK.c:17-45
+FV_DEV double rk_combine(int cs, double dt, double Vol, double SN, double S, double RHS) {
+  switch (cs) {
+    case 1: return SN - dt * (1.0 / Vol) * RHS;
+    case 2: return (3.0 / 4.0) * SN + (1.0 / 4.0) * S - (1.0 / 4.0) * dt * RHS / Vol;
+    case 3: return (1.0 / 3.0) * SN + (2.0 / 3.0) * S - (2.0 / 3.0) * dt * RHS / Vol;
+    default: return (1.0 / 2.0) * SN + (1.0 / 2.0) * S - (1.0 / 2.0) * dt * RHS / Vol;
+  }
+}
+
+// m4_residual (m4_residual.c:7-197) + RK(irk) + m4_relaxation_fluid for one leaf per thread
+template <int NM>
+__global__ __launch_bounds__(FV_BLOCK) void k_flux_rk(int n, Params P, Geo G, int cs,
+                                                      double *__restrict__ S, double *__restrict__ SN,
+                                                      const double *__restrict__ SF, const int *__restrict__ nbr,
+                                                      const uint32_t *__restrict__ info, const int *__restrict__ fine4,
+                                                      int *__restrict__ nanflag) {
+  using L = Lay<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t inf = info[i];
+  const int lvl = cell_level(inf);
+  double Sc[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
+  // cell state for the non-conservative sources (m4_residual.c:29-42)
+  Sol<NM, double> sc;
+  fill_sol<NM, double>(P, Sc, sc);
+  double src_avf[NM], src_st[NM];
+#pragma unroll
+  for (int k = 0; k < L::N3; ++k) src_avf[k] = sc.avf[k];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) src_st[k] = -sc.avf[k] * sc.p[k];
+
+  double RHS[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) RHS[v] = 0.0;
+
+#pragma unroll 1
+  for (int f = 0; f < 6; f++) {
+    const int kind = face_kind(inf, f);
+    const int nb = nbr[(size_t)f * n + i];
+    const int fo = FCOPP_D[f];
+    double ff[L::NEQ], uf0 = 0.0, uf1 = 0.0, uf2 = 0.0;
+    double Area_f;
+    if (kind == FV_FK_MIRROR) {
+      // the earlier leaf nb computes this face from its side; take the negation
+      double X[L::NEQ];
+      Sol<NM, double> sL, sR;
+      load_face<NM>(SF, n, nb, fo, X);
+      fill_sol<NM, double>(P, X, sL);
+      load_face<NM>(SF, n, i, f, X);
+      fill_sol<NM, double>(P, X, sR);
+      FaceFlux<NM, double> r;
+      hllc<NM, double>(P, G.frame[fo], 0, sL, sR, r);
+      const double A = G.area[lvl];
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) { double t = 0.0; t += A * r.f[v]; ff[v] = -t; }
+      double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+      t0 += r.un / ((double)1); t1 += r.vn / ((double)1); t2 += r.wn / ((double)1);
+      if (P.fix & 1) { uf0 = t0; uf1 = t1; uf2 = t2; }   // FV_FIX_NONCONS: the star velocity is a global vector
+      else { uf0 = -t0; uf1 = -t1; uf2 = -t2; }          // m4_residual.c:72-74
+      Area_f = 0.0; Area_f += G.area[lvl];
+    } else {
+      const int Nneig = (kind == FV_FK_FINE) ? 4 : 1;
+      const double A = (kind == FV_FK_FINE) ? G.area[lvl + 1] : G.area[lvl];
+      Area_f = 0.0;
+      for (int k = 0; k < Nneig; ++k) Area_f += A;
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) ff[v] = 0.0;
+      double X[L::NEQ];
+      Sol<NM, double> sL, sR;
+      load_face<NM>(SF, n, i, f, X);
+      fill_sol<NM, double>(P, X, sL);
+#pragma unroll 1
+      for (int k = 0; k < Nneig; ++k) {
+        int bcode = 0;
+        if (kind == FV_FK_BOUNDARY) {
+          sR = sL;
+          bcode = -nb;
+          apply_bc<NM, double>(bcode, G.frame[f], sL, sR);
+        } else {
+          const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
+          load_face<NM>(SF, n, j, fo, X);
+          fill_sol<NM, double>(P, X, sR);
+        }
+        FaceFlux<NM, double> r;
+        hllc<NM, double>(P, G.frame[f], bcode, sL, sR, r);
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) ff[v] += A * r.f[v];
+        uf0 += r.un / ((double)Nneig);
+        uf1 += r.vn / ((double)Nneig);
+        uf2 += r.wn / ((double)Nneig);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) RHS[v] += ff[v];
+    const double nx = G.frame[f][0], ny = G.frame[f][1], nz = G.frame[f][2];
+    // non-conservative terms (m4_residual.c:154-176); FV_FIX_NONCONS subtracts them
+    const bool fixnc = (P.fix & 1) != 0;
+#pragma unroll
+    for (int k = 0; k < L::N3; ++k) {
+      const double t = Area_f * (src_avf[k] * uf0 * nx + src_avf[k] * uf1 * ny + src_avf[k] * uf2 * nz);
+      RHS[L::I3 + k] = fixnc ? RHS[L::I3 + k] - t : RHS[L::I3 + k] + t;
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      const double t = Area_f * (src_st[k] * uf0 * nx + src_st[k] * uf1 * ny + src_st[k] * uf2 * nz);
+      RHS[L::I4 + k] = fixnc ? RHS[L::I4 + k] - t : RHS[L::I4 + k] + t;
+    }
+  }
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) bad |= isnan(RHS[v]);
+  if (bad) *nanflag = 1;
+
+  const double Vol = G.vol[lvl];
+  double Sn[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) {
+    double sn;
+    if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
+    else sn = SN[(size_t)v * n + i];
+    Sn[v] = rk_combine(cs, P.dt, Vol, sn, Sc[v], RHS[v]);
+  }
+  relax<NM>(P, Sn);
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = Sn[v];
+}
+
+// CFL: per-block minimum of h / (max|u_i| + c) with a wavefront shuffle reduction
+template <int NM>
+__global__ __launch_bounds__(FV_BLOCK) void k_cfl(int n, Params P, Geo G, const double *__restrict__ S,
+                                                  const uint32_t *__restrict__ info, double *__restrict__ part) {
+  using L = Lay<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d = INFINITY;
+  if (i < n) {
+    double X[L::NEQ];
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) X[v] = S[(size_t)v * n + i];
+    Sol<NM, double> s;
+    fill_sol<NM, double>(P, X, s);
+    double c2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) c2 += s.c2Y[k];
+    double lam = maxr(maxr(fabs(s.u), fabs(s.v)), fabs(s.w)) + sqrt(c2);
+    d = G.h[cell_level(info[i])] / lam;
+  }
+  for (int off = 32; off > 0; off >>= 1) d = fmin(d, __shfl_xor(d, off, 64));
+  __shared__ double wmin[FV_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmin[0];
+    for (int w = 1; w < FV_BLOCK / 64; w++) m = fmin(m, wmin[w]);
+    part[blockIdx.x] = m;
+  }
+}
+
+__global__ void k_min_final(int m, const double *__restrict__ part, double *__restrict__ out) {
+  double d = INFINITY;
+  for (int k = threadIdx.x; k < m; k += blockDim.x) d = fmin(d, part[k]);
+  for (int off = 32; off > 0; off >>= 1) d = fmin(d, __shfl_xor(d, off, 64));
+  __shared__ double wmin[16];
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = wmin[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = fmin(r, wmin[w]);
+    *out = r;
+  }
+}
+
+// ---- indicator: interface_location.c:7-86 and criterionsplt.c:3-140 ----
+struct CritDev {
+  int uniform_on, uniform_lvl, prs_on, prs_lvl, rho_on, rho_lvl, vel_on, vel_lvl;
+  int inter0_on, inter0_lvl, inter0_lrs, inter1_on, inter1_lvl, inter1_lrs;
+  double prs_thr, rho_thr, vel_thr;
+};
+
+FV_DEV double rel_jump(double x_cur, double x_nb) {
+  const double eps = 1.0e-8;
+  if ((fabs(x_nb) > eps) || (fabs(x_cur) > eps)) return fabs(x_nb - x_cur) / maxr(fabs(x_nb), fabs(x_cur));
+  return 0.0;
+}
+
+FV_DEV int sub_neighbour(int kind, int nb, int k, const int *__restrict__ fine4) {
+  return (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
+}
+
+template <int NM>
+__global__ __launch_bounds__(FV_BLOCK) void k_si_first(int n, Params P, const double *__restrict__ S,
+                                                       const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+                                                       const int *__restrict__ fine4, int *__restrict__ SI) {
+  using L = Lay<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = P.amin / 100.0;
+  const uint32_t inf = info[i];
+  int si0 = 0, si1 = 0, flag = 0;
+  auto phi = [&](int c, int k) {
+    double x = S[(size_t)(L::I3 + k) * n + c];
+    if (x <= a) x = a;
+    else if (x >= (1.0 - (a))) x = 1.0 - (a);
+    return x - P.avf_lim;
+  };
+  for (int f = 0; f < 6 && !flag; f++) {
+    const int kind = face_kind(inf, f);
+    if (kind == FV_FK_BOUNDARY) continue;
+    const int nb = nbr[(size_t)f * n + i];
+    const int ns = (kind == FV_FK_FINE) ? 4 : 1;
+    for (int k = 0; k < ns; k++) {
+      const int j = sub_neighbour(kind, nb, k, fine4);
+#pragma unroll
+      for (int v = 0; v < L::N3; ++v) {
+        if (phi(i, v) * phi(j, v) < 0.0) {
+          if (v == 0) si0 = 1; else si1 = 1;
+          flag = 1;
+        }
+      }
+    }
+  }
+  SI[2 * i] = si0;
+  SI[2 * i + 1] = si1;
+}
+
+__global__ __launch_bounds__(FV_BLOCK) void k_si_layer(int n, int lvl, const int *__restrict__ nbr,
+                                                       const uint32_t *__restrict__ info, const int *__restrict__ fine4,
+                                                       const int *__restrict__ SIin, int *__restrict__ SIout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t inf = info[i];
+  int si[2] = {SIin[2 * i], SIin[2 * i + 1]};
+  int flag = 0;
+  for (int f = 0; f < 6 && !flag; f++) {
+    const int kind = face_kind(inf, f);
+    if (kind == FV_FK_BOUNDARY) continue;
+    const int nb = nbr[(size_t)f * n + i];
+    const int ns = (kind == FV_FK_FINE) ? 4 : 1;
+    for (int k = 0; k < ns; k++) {
+      const int j = sub_neighbour(kind, nb, k, fine4);
+      for (int v = 0; v < 2; ++v)
+        if (SIin[2 * j + v] == (lvl - 1) && si[v] == 0) { si[v] = lvl; flag = 1; }
+    }
+  }
+  SIout[2 * i] = si[0];
+  SIout[2 * i + 1] = si[1];
+}
+
+template <int NM>
+__global__ __launch_bounds__(FV_BLOCK) void k_split(int n, Params P, CritDev C, const double *__restrict__ S,
+                                                    const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+                                                    const int *__restrict__ fine4, const int *__restrict__ SI,
+                                                    int *__restrict__ split) {
+  using L = Lay<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t inf = info[i];
+  int lt = 1;
+  if (C.uniform_on == 1) lt = max(lt, C.uniform_lvl);
+  if (C.prs_on == 1 || C.rho_on == 1 || C.vel_on == 1) {
+    double X[L::NEQ];
+    Sol<NM, double> sL, sR;
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) X[v] = S[(size_t)v * n + i];
+    fill_sol<NM, double>(P, X, sL);
+    double pl = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) pl += sL.avf[k] * sL.p[k];
+    double gp = 0.0, gr = 0.0, gv = 0.0, grho = 0.0, lg = 1;
+    for (int f = 0; f < 6; f++) {
+      const int kind = face_kind(inf, f);
+      if (kind == FV_FK_BOUNDARY) continue;
+      const int nb = nbr[(size_t)f * n + i];
+      const int ns = (kind == FV_FK_FINE) ? 4 : 1;
+      for (int k = 0; k < ns; k++) {
+        const int j = sub_neighbour(kind, nb, k, fine4);
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) X[v] = S[(size_t)v * n + j];
+        fill_sol<NM, double>(P, X, sR);
+        if (C.prs_on == 1) {
+          double pr = 0.0;
+#pragma unroll
+          for (int q = 0; q < NM; ++q) pr += sR.avf[q] * sR.p[q];
+          gp = maxr(rel_jump(pl, pr), gp);
+        }
+        if (C.rho_on == 1) { grho = rel_jump(sL.r, sR.r); gr = maxr(grho, gr); }
+        if (C.vel_on == 1) gv = maxr(grho, gv);   // criterionsplt.c:113 reads the density jump
+      }
+    }
+    if (C.prs_on == 1 && fabs(gp) >= C.prs_thr) lg = C.prs_lvl;
+    if (C.rho_on == 1 && fabs(gr) >= C.rho_thr) lg = C.rho_lvl;
+    if (C.vel_on == 1 && fabs(gv) >= C.vel_thr) lg = C.vel_lvl;
+    lt = (int)maxr((double)lt, lg);
+  }
+  if (C.inter0_on != 0) {
+    for (int v = 0; v < 2; ++v) {
+      const int s = SI[2 * i + v];
+      if (s >= 1 && s < C.inter0_lrs) lt = max(lt, C.inter0_lvl);
+      if (s >= C.inter0_lrs && s < C.inter1_lrs) lt = max(lt, C.inter1_lvl);
+    }
+  }
+  split[i] = lt;
+}
+
+// ---------------------------------------------------------------------
+// host side of the C ABI
+// ---------------------------------------------------------------------
+static thread_local char g_err[256];
+
+#define HIPCHK(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) {                                                                        \
+      snprintf(g_err, sizeof g_err, "%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+      return FV_ERR_HIP;                                                                           \
+    }                                                                                              \
+  } while (0)
+
+struct fv_dev {
+  int n = 0, neq = 0, nmat = 0, nfine = 0, prec = FV_PREC_FP64;
+  Params P{};
+  Geo G{};
+  hipStream_t stream = nullptr;
+  double *S = nullptr, *SN = nullptr, *SF = nullptr, *part = nullptr, *dtmp = nullptr;
+  int *nbr = nullptr, *fine4 = nullptr, *nanflag = nullptr, *SIa = nullptr, *SIb = nullptr, *split = nullptr;
+  uint32_t *info = nullptr;
+  int cap_n = 0, cap_fine = 0;
+};
+
+static int set_err(int code, const char *msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return code;
+}
+
+static void to_dev_params(const fv_params *p, Params &P) {
+  memset(&P, 0, sizeof P);
+  for (int k = 0; k < 4; k++) { P.gamma[k] = p->gamma[k]; P.pinf[k] = p->pinf[k]; }
+  P.amin = p->amin; P.avf_lim = p->avf_lim; P.dt = p->dt;
+  P.prelax = p->prelax; P.limiter = p->limiter; P.rk = p->rk; P.nmat = p->nmat; P.fix = p->fix;
+}
+
+static void to_dev_geo(const fv_geo *g, Geo &G) {
+  memset(&G, 0, sizeof G);
+  memcpy(G.frame, g->frame, sizeof G.frame);
+  for (int l = 0; l < 12; l++) {
+    G.area[l] = g->area[l]; G.vol[l] = g->vol[l]; G.h[l] = g->h[l];
+    for (int k = 0; k < 3; k++) { G.dcf[k][l] = g->dcf[k][l]; G.dcc[k][l] = g->dcc[k][l]; }
+  }
+}
+
+static void free_buffers(fv_dev *c) {
+  hipFree(c->S); hipFree(c->SN); hipFree(c->SF); hipFree(c->part); hipFree(c->dtmp);
+  hipFree(c->nbr); hipFree(c->fine4); hipFree(c->nanflag); hipFree(c->info);
+  hipFree(c->SIa); hipFree(c->SIb); hipFree(c->split);
+  c->S = c->SN = c->SF = c->part = c->dtmp = nullptr;
+  c->nbr = c->fine4 = c->nanflag = c->SIa = c->SIb = c->split = nullptr;
+  c->info = nullptr;
+  c->cap_n = c->cap_fine = 0;
+}
+
+static int upload_mesh(fv_dev *c, const fv_pack *pk) {
+  if (pk->n <= 0) return set_err(FV_ERR_ARG, "empty mesh");
+  const size_t n = (size_t)pk->n;
+  if (pk->n > c->cap_n || pk->nfine > c->cap_fine) {
+    free_buffers(c);
+    const size_t nf = (size_t)(pk->nfine > 0 ? pk->nfine : 1);
+    const size_t nb = (n + FV_BLOCK - 1) / FV_BLOCK;
+    HIPCHK(hipMalloc(&c->S, sizeof(double) * n * c->neq));
+    HIPCHK(hipMalloc(&c->SN, sizeof(double) * n * c->neq));
+    HIPCHK(hipMalloc(&c->SF, sizeof(double) * n * 6 * c->neq));
+    HIPCHK(hipMalloc(&c->part, sizeof(double) * (nb + 1)));
+    HIPCHK(hipMalloc(&c->dtmp, sizeof(double) * 4));
+    HIPCHK(hipMalloc(&c->nbr, sizeof(int) * n * 6));
+    HIPCHK(hipMalloc(&c->fine4, sizeof(int) * nf * 4));
+    HIPCHK(hipMalloc(&c->nanflag, sizeof(int)));
+    HIPCHK(hipMalloc(&c->info, sizeof(uint32_t) * n));
+    HIPCHK(hipMalloc(&c->SIa, sizeof(int) * n * 2));
+    HIPCHK(hipMalloc(&c->SIb, sizeof(int) * n * 2));
+    HIPCHK(hipMalloc(&c->split, sizeof(int) * n));
+    c->cap_n = pk->n;
+    c->cap_fine = (int)nf;
+  }
+  c->n = pk->n;
+  c->nfine = pk->nfine;
+  to_dev_geo(&pk->geo, c->G);
+  HIPCHK(hipMemcpyAsync(c->nbr, pk->nbr, sizeof(int) * n * 6, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->info, pk->info, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+  if (pk->nfine > 0)
+    HIPCHK(hipMemcpyAsync(c->fine4, pk->fine4, sizeof(int) * 4 * (size_t)pk->nfine, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->nanflag, 0, sizeof(int), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FV_OK;
+}
+
+extern "C" int fv_dev_attach(fv_dev **out, const fv_params *p, const fv_pack *pk) {
+  if (!out || !p || !pk) return set_err(FV_ERR_ARG, "null argument");
+  if (p->nmat != 2 && p->nmat != 3) return set_err(FV_ERR_ARG, "nmat must be 2 or 3");
+  if (p->prelax != 0 && p->prelax != 1 && p->prelax != 3) return set_err(FV_ERR_ARG, "PRELAX must be 0, 1 or 3");
+  if (p->prelax == 1 && p->nmat != 2) return set_err(FV_ERR_ARG, "PRELAX=1 is two-phase only");
+  if (p->rk != FV_RK_REF2 && p->rk != FV_RK_SSP3 && p->rk != FV_RK_SSP2) return set_err(FV_ERR_ARG, "unknown RK table");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(FV_ERR_HIP, "no HIP device");
+  fv_dev *c = new fv_dev();
+  c->nmat = p->nmat;
+  c->neq = fv_neq(p->nmat);
+  to_dev_params(p, c->P);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return set_err(FV_ERR_HIP, "stream creation failed");
+  }
+  int rc = upload_mesh(c, pk);
+  if (rc != FV_OK) { free_buffers(c); hipStreamDestroy(c->stream); delete c; return rc; }
+  *out = c;
+  return FV_OK;
+}
+
+extern "C" int fv_dev_reattach(fv_dev *c, const fv_pack *pk) {
+  if (!c || !pk) return set_err(FV_ERR_ARG, "null argument");
+  return upload_mesh(c, pk);
+}
+
+extern "C" int fv_dev_detach(fv_dev *c) {
+  if (!c) return FV_OK;
+  hipStreamSynchronize(c->stream);
+  free_buffers(c);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return FV_OK;
+}
+
+extern "C" int fv_dev_size(const fv_dev *c, int *n, int *neq) {
+  if (!c) return set_err(FV_ERR_ARG, "null context");
+  if (n) *n = c->n;
+  if (neq) *neq = c->neq;
+  return FV_OK;
+}
+
+// [n][NEQ] <-> [NEQ][n] on the host side of the copy
+extern "C" int fv_dev_set_state(fv_dev *c, const double *S) {
+  if (!c || !S) return set_err(FV_ERR_ARG, "null argument");
+  const size_t n = c->n, neq = c->neq;
+  double *h = (double *)malloc(sizeof(double) * n * neq);
+  if (!h) return set_err(FV_ERR_NOMEM, "host staging");
+  for (size_t i = 0; i < n; i++)
+    for (size_t v = 0; v < neq; v++) h[v * n + i] = S[i * neq + v];
+  hipError_t e = hipMemcpy(c->S, h, sizeof(double) * n * neq, hipMemcpyHostToDevice);
+  free(h);
+  HIPCHK(e);
+  return FV_OK;
+}
+
+extern "C" int fv_dev_get_state(fv_dev *c, double *S) {
+  if (!c || !S) return set_err(FV_ERR_ARG, "null argument");
+  const size_t n = c->n, neq = c->neq;
+  double *h = (double *)malloc(sizeof(double) * n * neq);
+  if (!h) return set_err(FV_ERR_NOMEM, "host staging");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipError_t e = hipMemcpy(h, c->S, sizeof(double) * n * neq, hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    for (size_t i = 0; i < n; i++)
+      for (size_t v = 0; v < neq; v++) S[i * neq + v] = h[v * n + i];
+  free(h);
+  HIPCHK(e);
+  return FV_OK;
+}
+
+extern "C" int fv_dev_get_faces(fv_dev *c, double *SF) {
+  if (!c || !SF) return set_err(FV_ERR_ARG, "null argument");
+  const size_t n = c->n, neq = c->neq;
+  double *h = (double *)malloc(sizeof(double) * n * 6 * neq);
+  if (!h) return set_err(FV_ERR_NOMEM, "host staging");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipError_t e = hipMemcpy(h, c->SF, sizeof(double) * n * 6 * neq, hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    for (size_t i = 0; i < n; i++)
+      for (size_t f = 0; f < 6; f++)
+        for (size_t v = 0; v < neq; v++) SF[(i * 6 + f) * neq + v] = h[(f * neq + v) * n + i];
+  free(h);
+  HIPCHK(e);
+  return FV_OK;
+}
+
+extern "C" int fv_dev_set_dt(fv_dev *c, double dt) {
+  if (!c) return set_err(FV_ERR_ARG, "null context");
+  c->P.dt = dt;
+  return FV_OK;
+}
+
+extern "C" int fv_dev_set_precision(fv_dev *c, int prec) {
+  if (!c) return set_err(FV_ERR_ARG, "null context");
+  if (prec != FV_PREC_FP64 && prec != FV_PREC_MIXED) return set_err(FV_ERR_ARG, "unknown precision");
+  c->prec = prec;
+  return FV_OK;
+}
+
+extern "C" int fv_dev_begin_step(fv_dev *c) {
+  if (!c) return set_err(FV_ERR_ARG, "null context");
+  HIPCHK(hipMemcpyAsync(c->SN, c->S, sizeof(double) * (size_t)c->n * c->neq, hipMemcpyDeviceToDevice, c->stream));
+  return FV_OK;
+}
+
+static int stage_case(int rk, int irk) {
+  if (rk == FV_RK_SSP2) return irk == 1 ? 1 : 4;
+  return irk;
+}
+
+static int launch_stage(fv_dev *c, int irk, hipEvent_t *ev_flux0, hipEvent_t *ev_flux1) {
+  const int n = c->n;
+  const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
+  const int cs = stage_case(c->P.rk, irk);
+  if (c->nmat == 2) {
+    hipLaunchKernelGGL(k_recon<2>, grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info, c->fine4, c->SF);
+    if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
+    hipLaunchKernelGGL(k_flux_rk<2>, grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF, c->nbr, c->info,
+                       c->fine4, c->nanflag);
+  } else {
+    hipLaunchKernelGGL(k_recon<3>, grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info, c->fine4, c->SF);
+    if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
+    hipLaunchKernelGGL(k_flux_rk<3>, grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF, c->nbr, c->info,
+                       c->fine4, c->nanflag);
+  }
+  if (ev_flux1) HIPCHK(hipEventRecord(*ev_flux1, c->stream));
+  HIPCHK(hipGetLastError());
+  return FV_OK;
+}
+
+static int check_nan(fv_dev *c) {
+  int flag = 0;
+  HIPCHK(hipMemcpyAsync(&flag, c->nanflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (flag) {
+    HIPCHK(hipMemsetAsync(c->nanflag, 0, sizeof(int), c->stream));
+    return set_err(FV_ERR_NAN, "NaN in the residual");
+  }
+  return FV_OK;
+}
+
+extern "C" int fv_dev_stage(fv_dev *c, int irk) {
+  if (!c) return set_err(FV_ERR_ARG, "null context");
+  int ns = (c->P.rk == FV_RK_SSP3) ? 3 : 2;
+  if (irk < 1 || irk > ns) return set_err(FV_ERR_ARG, "stage index out of range");
+  int rc = launch_stage(c, irk, nullptr, nullptr);
+  if (rc != FV_OK) return rc;
+  return check_nan(c);
+}
+
+extern "C" int fv_dev_advance(fv_dev *c, int nsteps) {
+  if (!c) return set_err(FV_ERR_ARG, "null context");
+  const int ns = (c->P.rk == FV_RK_SSP3) ? 3 : 2;
+  for (int s = 0; s < nsteps; s++) {
+    // RK(0) is folded into stage 1, which stores SN from the state it reads
+    for (int irk = 1; irk <= ns; irk++) {
+      int rc = launch_stage(c, irk, nullptr, nullptr);
+      if (rc != FV_OK) return rc;
+    }
+  }
+  return check_nan(c);
+}
+
+extern "C" int fv_dev_cfl_dt(fv_dev *c, double cfl, double *dt) {
+  if (!c || !dt) return set_err(FV_ERR_ARG, "null argument");
+  const int n = c->n;
+  const int nb = (n + FV_BLOCK - 1) / FV_BLOCK;
+  if (c->nmat == 2)
+    hipLaunchKernelGGL(k_cfl<2>, dim3(nb), dim3(FV_BLOCK), 0, c->stream, n, c->P, c->G, c->S, c->info, c->part);
+  else
+    hipLaunchKernelGGL(k_cfl<3>, dim3(nb), dim3(FV_BLOCK), 0, c->stream, n, c->P, c->G, c->S, c->info, c->part);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(1024), 0, c->stream, nb, c->part, c->dtmp);
+  HIPCHK(hipGetLastError());
+  double h = 0.0;
+  HIPCHK(hipMemcpyAsync(&h, c->dtmp, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *dt = cfl * h;
+  return FV_OK;
+}
+
+extern "C" int fv_dev_indicator(fv_dev *c, const fv_crit *cr, int *SI, int *split) {
+  if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
+  const int n = c->n;
+  const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
+  CritDev C;
+  C.uniform_on = cr->uniform_on; C.uniform_lvl = cr->uniform_lvl;
+  C.prs_on = cr->prs_on; C.prs_lvl = cr->prs_lvl; C.prs_thr = cr->prs_thr;
+  C.rho_on = cr->rho_on; C.rho_lvl = cr->rho_lvl; C.rho_thr = cr->rho_thr;
+  C.vel_on = cr->vel_on; C.vel_lvl = cr->vel_lvl; C.vel_thr = cr->vel_thr;
+  C.inter0_on = cr->inter0_on; C.inter0_lvl = cr->inter0_lvl; C.inter0_lrs = cr->inter0_lrs;
+  C.inter1_on = cr->inter1_on; C.inter1_lvl = cr->inter1_lvl; C.inter1_lrs = cr->inter1_lrs;
+  int imax = 0;   // ADAPT_INTER_MAX (m4_input.c:52-54)
+  if (cr->inter0_on == 1 && cr->inter1_on == 1) imax = cr->inter0_lrs + cr->inter1_lrs;
+  else if (cr->inter0_on == 1 && cr->inter1_on == 0) imax = cr->inter0_lrs;
+  int *cur = c->SIa, *nxt = c->SIb;
+  if (c->nmat == 2)
+    hipLaunchKernelGGL(k_si_first<2>, grid, block, 0, c->stream, n, c->P, c->S, c->nbr, c->info, c->fine4, cur);
+  else
+    hipLaunchKernelGGL(k_si_first<3>, grid, block, 0, c->stream, n, c->P, c->S, c->nbr, c->info, c->fine4, cur);
+  for (int lvl = 2; lvl < imax; lvl++) {
+    hipLaunchKernelGGL(k_si_layer, grid, block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur, nxt);
+    int *t = cur; cur = nxt; nxt = t;
+  }
+  if (c->nmat == 2)
+    hipLaunchKernelGGL(k_split<2>, grid, block, 0, c->stream, n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, c->split);
+  else
+    hipLaunchKernelGGL(k_split<3>, grid, block, 0, c->stream, n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, c->split);
+  HIPCHK(hipGetLastError());
+  if (SI) HIPCHK(hipMemcpyAsync(SI, cur, sizeof(int) * 2 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  if (split) HIPCHK(hipMemcpyAsync(split, c->split, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FV_OK;
+}
+
+extern "C" int fv_dev_time_stages(fv_dev *c, int nrep, double *ms_stage, double *ms_flux) {
+  if (!c || nrep < 1) return set_err(FV_ERR_ARG, "bad argument");
+  const int ns = (c->P.rk == FV_RK_SSP3) ? 3 : 2;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double tot_flux = 0.0;
+  hipEvent_t a0, a1;
+  HIPCHK(hipEventCreate(&a0));
+  HIPCHK(hipEventCreate(&a1));
+  HIPCHK(hipEventRecord(a0, c->stream));
+  for (int r = 0; r < nrep; r++) {
+    int rc = launch_stage(c, 1 + (r % ns), &e0, &e1);
+    if (rc != FV_OK) return rc;
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    tot_flux += ms;
+  }
+  HIPCHK(hipEventRecord(a1, c->stream));
+  HIPCHK(hipEventSynchronize(a1));
+  float all = 0.f;
+  HIPCHK(hipEventElapsedTime(&all, a0, a1));
+  hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(a0); hipEventDestroy(a1);
+  if (ms_stage) *ms_stage = all / nrep;
+  if (ms_flux) *ms_flux = tot_flux / nrep;
+  return check_nan(c);
+}
+
+extern "C" const char *fv_dev_last_error(void) { return g_err; }

@@ -1,0 +1,82 @@
+/*
+ * forestfv_dev.h -- C ABI of the MI355X device path.
+ *
+ * The reference has no plugin or operator registry: its explicit update is
+ * reached through plain C calls, loop() -> advancetimeexplicit(run)
+ * (src/c/loop.c:21), and inside it grad_scheme(run), RHS(run), RK(irk,run),
+ * m4_relaxation_fluid(run) and communicate_S(run,0) per stage
+ * (src/c/advancetimeexplicit.c:9-24).  These entry points take their place:
+ * a caller packs its leaves once per mesh change (fv_dev_attach), then runs
+ * whole steps on the device (fv_dev_advance) and copies the state back only
+ * before export, restart or regrid (fv_dev_get_state).  Plain pointers and
+ * sizes only; every call returns an FV_* status instead of exit(0).
+ *
+ * One context per RUN, one HIP stream per context, no concurrent calls on a
+ * context (the reference is single-threaded per rank and non-reentrant,
+ * strdata.h:185-217).
+ */
+#ifndef FORESTFV_DEV_H
+#define FORESTFV_DEV_H
+
+#include "forestfv.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fv_dev fv_dev;
+
+/* Precision of the reconstruction / flux / ghost-fill kernels.  The state,
+ * its RK accumulation and pressure recovery from energy stay FP64 in both. */
+#define FV_PREC_FP64  0
+#define FV_PREC_MIXED 1
+
+/* Upload the packed mesh and parameters; allocates every device buffer of
+ * the context (the device mirror of leafallocation.c / createutility.c). */
+int  fv_dev_attach(fv_dev **ctx, const fv_params *p, const fv_pack *pk);
+/* Release the context (leafdeallocation.c equivalent). */
+int  fv_dev_detach(fv_dev *ctx);
+/* Re-upload topology after a regrid, keeping parameters; state is resized. */
+int  fv_dev_reattach(fv_dev *ctx, const fv_pack *pk);
+
+/* State transfer, S as [n][NEQ] in leaf order (LEAF.S). */
+int  fv_dev_set_state(fv_dev *ctx, const double *S);
+int  fv_dev_get_state(fv_dev *ctx, double *S);
+
+/* advancetimeexplicit (advancetimeexplicit.c:3-29) repeated nsteps times. */
+int  fv_dev_advance(fv_dev *ctx, int nsteps);
+/* RK(0,run): SN = S (RK.c:17-21). */
+int  fv_dev_begin_step(fv_dev *ctx);
+/* One stage irk: grad_scheme + RHS + RK(irk) + m4_relaxation_fluid. */
+int  fv_dev_stage(fv_dev *ctx, int irk);
+
+/* DT used by the stages (the reference's fixed DT, inputset.c:69). */
+int  fv_dev_set_dt(fv_dev *ctx, double dt);
+/* Largest stable step cfl * min h / (max|u_i| + c) (new: no CFL in the reference). */
+int  fv_dev_cfl_dt(fv_dev *ctx, double cfl, double *dt);
+/* FV_PREC_FP64 (default) or FV_PREC_MIXED. */
+int  fv_dev_set_precision(fv_dev *ctx, int prec);
+
+/* Face states of the last stage, SF as [n][6][NEQ] (LEAF.SF). */
+int  fv_dev_get_faces(fv_dev *ctx, double *SF);
+
+/* interface_location (interface_location.c:7-86) and the per-leaf target
+ * level of criterionsplt (criterionsplt.c:3-140) on the current state.
+ * SI: [n][2] out, split: [n] out (either may be NULL). */
+int  fv_dev_indicator(fv_dev *ctx, const fv_crit *c, int *SI, int *split);
+
+/* Timing of the stage kernels with hipEvents on the context's stream:
+ * runs nrep full stages (cycling irk) and returns the mean milliseconds per
+ * stage and per launch of the dominant (flux) kernel. */
+int  fv_dev_time_stages(fv_dev *ctx, int nrep, double *ms_stage, double *ms_flux_kernel);
+
+/* Number of leaves and NEQ of the attached mesh. */
+int  fv_dev_size(const fv_dev *ctx, int *n, int *neq);
+
+/* Message of the last failing call on this thread. */
+const char *fv_dev_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,61 @@
+"""Shared test configurations: Sod (config C0), water/air bubble, AMR meshes."""
+import numpy as np
+
+import forestfv_public_amd as F
+
+SOD_GAMMA = 1.4
+SOD_RHO = (1.0, 0.125)
+SOD_P = (1.0, 0.1)
+WATER_AIR = dict(gamma=(6.12, 1.4), pinf=(3.43e8, 0.0))
+WATER_AIR_RHO = (1000.0, 1.2)
+
+
+def sod_params(rk=F.RK_SSP3, prelax=0, dt=None, level=6, limiter=0, bc=(4, 4, 4, 4, 4, 4), fix=F.FIX_ALL):
+    """Config C0: two identical ideal-gas materials, left rho=1 p=1, right rho=0.125 p=0.1."""
+    h = 1.0 / (1 << (level - 1))
+    if dt is None:
+        dt = 0.5 * h / np.sqrt(1.4)     # CFL 0.5 on the left state's sound speed
+    return F.make_params(nmat=2, gamma=(SOD_GAMMA, SOD_GAMMA), pinf=(0.0, 0.0), amin=1e-8, avf_lim=0.5,
+                         prelax=prelax, rk=rk, limiter=limiter, dt=dt, bc=bc, fix=fix)
+
+
+def sod_case(level=6, nroot=(1, 1, 1), **kw):
+    p = sod_params(level=level, **kw)
+    rootsize = 1.0 / nroot[0]
+    m = F.Mesh.uniform(nroot, (0.0, 0.0, 0.0), rootsize, level, list(p.bc))
+    S = m.init_split_x(p, SOD_RHO, SOD_P, 0.5)
+    return p, m, S
+
+
+def water_air_params(dt=1e-9, prelax=1, rk=F.RK_SSP3, bc=(2, 2, 2, 2, 2, 2), limiter=0, fix=F.FIX_ALL):
+    return F.make_params(nmat=2, gamma=WATER_AIR["gamma"], pinf=WATER_AIR["pinf"], amin=1e-6, avf_lim=0.5,
+                         prelax=prelax, rk=rk, limiter=limiter, dt=dt, bc=bc, fix=fix)
+
+
+def bubble_case(level=5, nroot=(1, 1, 1), L=4e-3, R=0.4e-3, perturb=0.01, prelax=1, dt=None, seed=1234, **kw):
+    """Config C1 physics on a uniform grid: air bubble (1 bar) in water (50 bar)."""
+    h = L / nroot[0] / (1 << (level - 1))
+    if dt is None:
+        c_w = np.sqrt(6.12 * (5e6 + 3.43e8) / 1000.0)
+        dt = 0.4 * h / c_w
+    p = water_air_params(dt=dt, prelax=prelax, **kw)
+    m = F.Mesh.uniform(nroot, (0.0, 0.0, 0.0), L / nroot[0], level, list(p.bc))
+    c = (0.5 * L, 0.5 * L, 0.5 * L)
+    S = m.init_bubble(p, WATER_AIR_RHO, (5e6, 1e5), c, R, 2.0, seed, perturb)
+    return p, m, S
+
+
+def random_amr_mesh(params, base_level=2, max_level=4, nroot=(1, 1, 1), seed=0, frac=0.3, passes=3, L=1.0):
+    """A 2:1 face-balanced AMR mesh built by random refinement through the host AMR path."""
+    rng = np.random.default_rng(seed)
+    m = F.Mesh.uniform(nroot, (0.0, 0.0, 0.0), L / nroot[0], base_level, list(params.bc))
+    nq = F.neq(params.nmat)
+    S = np.tile(np.array([[1.0, 1e-8, 0, 0, 0, 2.5, 1 - 1e-8, 2.5, 1e-8][:nq]]), (m.n, 1))
+    for _ in range(passes):
+        a = m.arrays()
+        split = a["level"].copy()
+        pick = rng.random(m.n) < frac
+        split[pick] = np.minimum(a["level"][pick] + 1, max_level)
+        split = m.criterionsmth(max_level, split)
+        S = m.adapt(params, split, S)
+    return m

@@ -1,0 +1,232 @@
+"""ctypes binding of the CPU restatement in oracle/ (test infrastructure only).
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+checker of the HIP path; never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "_build", "libfvoracle.so")
+
+ORC_MAXMAT = 4
+
+
+class orc_params(C.Structure):
+    _fields_ = [
+        ("nmat", C.c_int),
+        ("gamma", C.c_double * ORC_MAXMAT),
+        ("pinf", C.c_double * ORC_MAXMAT),
+        ("amin", C.c_double),
+        ("avf_lim", C.c_double),
+        ("prelax", C.c_int),
+        ("rk", C.c_int),
+        ("limiter", C.c_int),
+        ("dt", C.c_double),
+        ("bc", C.c_int * 6),
+        ("fix", C.c_int),
+    ]
+
+
+class orc_mesh(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("nroot", C.c_int * 3),
+        ("origin", C.c_double * 3),
+        ("rootsize", C.c_double),
+        ("root", C.POINTER(C.c_int)),
+        ("level", C.POINTER(C.c_int)),
+        ("ci", C.POINTER(C.c_int)),
+        ("lfc_neigs", C.POINTER(C.c_int)),
+        ("neigtr", C.POINTER(C.c_int)),
+        ("bc", C.POINTER(C.c_int)),
+        ("SI", C.POINTER(C.c_int)),
+    ]
+
+
+class orc_cell(C.Structure):
+    _fields_ = [(nm, C.c_double * 6) for nm in ("nx", "ny", "nz", "nxt1", "nyt1", "nzt1", "nxt2", "nyt2", "nzt2")] + [
+        ("Area", (C.c_double * 4) * 6),
+        ("Vol", C.c_double),
+        ("dist_cf", C.c_double * 6),
+        ("dist_cc", C.c_double * 6),
+        ("xc", C.c_double), ("yc", C.c_double), ("zc", C.c_double),
+    ]
+
+
+class orc_crit(C.Structure):
+    _fields_ = [
+        ("uniform_on", C.c_int), ("uniform_lvl", C.c_int),
+        ("prs_on", C.c_int), ("prs_lvl", C.c_int), ("prs_thr", C.c_double),
+        ("rho_on", C.c_int), ("rho_lvl", C.c_int), ("rho_thr", C.c_double),
+        ("vel_on", C.c_int), ("vel_lvl", C.c_int), ("vel_thr", C.c_double),
+        ("inter0_on", C.c_int), ("inter0_lvl", C.c_int), ("inter0_lrs", C.c_int),
+        ("inter1_on", C.c_int), ("inter1_lvl", C.c_int), ("inter1_lrs", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = C.CDLL(ORACLE_SO)
+        P = C.POINTER
+        dp = P(C.c_double)
+        L.orc_geometry.argtypes = [P(orc_mesh), P(orc_cell)]
+        L.orc_advance.argtypes = [P(orc_mesh), P(orc_cell), P(orc_params), dp, C.c_int]
+        L.orc_stage.argtypes = [P(orc_mesh), P(orc_cell), P(orc_params), C.c_int, dp, dp, dp, dp]
+        L.orc_faceinterp.argtypes = [P(orc_mesh), P(orc_cell), P(orc_params), dp, dp]
+        L.orc_relax.argtypes = [P(orc_params), dp]
+        L.orc_relax.restype = None
+        L.orc_hllc_state.argtypes = [P(orc_params), dp, dp, dp, dp, dp, C.c_int, dp, dp]
+        L.orc_hllc_state.restype = None
+        L.orc_physical_flux.argtypes = [P(orc_params), dp, dp, dp]
+        L.orc_physical_flux.restype = None
+        L.orc_interface_location.argtypes = [P(orc_mesh), P(orc_params), P(orc_crit), dp, P(C.c_int)]
+        L.orc_criterionsplt.argtypes = [P(orc_mesh), P(orc_params), P(orc_crit), dp, P(C.c_int), P(C.c_int)]
+        L.orc_cfl_dt.argtypes = [P(orc_mesh), P(orc_cell), P(orc_params), dp, C.c_double]
+        L.orc_cfl_dt.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def params_from(p):
+    """orc_params with the same field values as a forestfv fv_params."""
+    q = orc_params()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(orc_params))
+    return q
+
+
+def crit_from(c):
+    q = orc_crit()
+    C.memmove(C.byref(q), C.byref(c), C.sizeof(orc_crit))
+    return q
+
+
+class OracleMesh:
+    """Oracle view of a product mesh's topology (geometry is recomputed by the oracle)."""
+
+    def __init__(self, mesh, SI=None):
+        a = mesh.arrays()
+        m = mesh.m
+        self.arrays = {k: np.ascontiguousarray(v, dtype=np.int32) for k, v in a.items()}
+        if SI is not None:
+            self.arrays["SI"] = np.ascontiguousarray(SI, dtype=np.int32)
+        om = orc_mesh()
+        om.n = m.n
+        for k in range(3):
+            om.nroot[k] = m.nroot[k]
+            om.origin[k] = m.origin[k]
+        om.rootsize = m.rootsize
+        for name in ("root", "level", "ci", "lfc_neigs", "neigtr", "bc", "SI"):
+            setattr(om, name, _ip(self.arrays[name]))
+        self.om = om
+        self.n = m.n
+        self.cells = (orc_cell * m.n)()
+        rc = lib().orc_geometry(C.byref(om), self.cells)
+        assert rc == 0
+
+    def set_SI(self, SI):
+        self.arrays["SI"] = np.ascontiguousarray(SI, dtype=np.int32)
+        self.om.SI = _ip(self.arrays["SI"])
+
+    def advance(self, params, S, nsteps):
+        p = params_from(params)
+        S = np.ascontiguousarray(S, dtype=np.float64).copy()
+        rc = lib().orc_advance(C.byref(self.om), self.cells, C.byref(p), _dp(S), nsteps)
+        if rc != 0:
+            raise RuntimeError(f"orc_advance status {rc}")
+        return S
+
+    def stage(self, params, irk, SN, S):
+        p = params_from(params)
+        nq = 3 * params.nmat + 3
+        SN = np.ascontiguousarray(SN, dtype=np.float64)
+        S = np.ascontiguousarray(S, dtype=np.float64).copy()
+        SF = np.zeros((self.n, 6, nq))
+        RHS = np.zeros((self.n, nq))
+        rc = lib().orc_stage(C.byref(self.om), self.cells, C.byref(p), irk, _dp(SN), _dp(S), _dp(SF), _dp(RHS))
+        if rc != 0:
+            raise RuntimeError(f"orc_stage status {rc}")
+        return S, SF, RHS
+
+    def faceinterp(self, params, S):
+        p = params_from(params)
+        nq = 3 * params.nmat + 3
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        SF = np.zeros((self.n, 6, nq))
+        lib().orc_faceinterp(C.byref(self.om), self.cells, C.byref(p), _dp(S), _dp(SF))
+        return SF
+
+    def interface_location(self, params, crit, S):
+        SI = np.zeros((self.n, 2), dtype=np.int32)
+        lib().orc_interface_location(C.byref(self.om), C.byref(params_from(params)), C.byref(crit_from(crit)),
+                                     _dp(np.ascontiguousarray(S)), _ip(SI))
+        return SI
+
+    def criterionsplt(self, params, crit, S, SI):
+        split = np.zeros(self.n, dtype=np.int32)
+        SI = np.ascontiguousarray(SI, dtype=np.int32)
+        lib().orc_criterionsplt(C.byref(self.om), C.byref(params_from(params)), C.byref(crit_from(crit)),
+                                _dp(np.ascontiguousarray(S)), _ip(SI), _ip(split))
+        return split
+
+    def cfl_dt(self, params, S, cfl):
+        return lib().orc_cfl_dt(C.byref(self.om), self.cells, C.byref(params_from(params)),
+                                _dp(np.ascontiguousarray(S)), cfl)
+
+    def geometry(self):
+        g = {}
+        for name in ("nx", "ny", "nz", "nxt1", "nyt1", "nzt1", "nxt2", "nyt2", "nzt2", "dist_cf", "dist_cc"):
+            g[name] = np.array([list(getattr(c, name)) for c in self.cells])
+        g["Area"] = np.array([[list(r) for r in c.Area] for c in self.cells])
+        g["Vol"] = np.array([c.Vol for c in self.cells])
+        g["xc"] = np.array([[c.xc, c.yc, c.zc] for c in self.cells])
+        return g
+
+
+def relax(params, vec):
+    v = np.ascontiguousarray(vec, dtype=np.float64).copy()
+    lib().orc_relax(C.byref(params_from(params)), _dp(v))
+    return v
+
+
+def hllc(params, sfL, sfR, n, t1, t2, bc=0):
+    nq = 3 * params.nmat + 3
+    flux = np.zeros(nq)
+    us = np.zeros(3)
+    lib().orc_hllc_state(C.byref(params_from(params)), _dp(np.ascontiguousarray(sfL, dtype=np.float64)),
+                         _dp(np.ascontiguousarray(sfR, dtype=np.float64)), _dp(np.asarray(n, dtype=np.float64)),
+                         _dp(np.asarray(t1, dtype=np.float64)), _dp(np.asarray(t2, dtype=np.float64)), bc,
+                         _dp(flux), _dp(us))
+    return flux, us
+
+
+def physical_flux(params, vec, n):
+    nq = 3 * params.nmat + 3
+    flux = np.zeros(nq)
+    lib().orc_physical_flux(C.byref(params_from(params)), _dp(np.ascontiguousarray(vec, dtype=np.float64)),
+                            _dp(np.asarray(n, dtype=np.float64)), _dp(flux))
+    return flux

@@ -1,0 +1,336 @@
+"""Pins of the CPU restatement (oracle/) by closed forms, invariants, textbook
+solutions and brute force -- the reference ships no tests or golden vectors
+and its build needs PETSc/METIS headers this image lacks."""
+import numpy as np
+import pytest
+
+import forestfv_public_amd as F
+import oracle_lib as O
+import riemann_exact as RX
+from cases import bubble_case, random_amr_mesh, sod_case, sod_params, water_air_params
+
+
+# ---------------------------------------------------------------- textbook
+def test_exact_riemann_solver_matches_toro_table():
+    """Toro (3rd ed.) Table 4.2, test 1 (Sod): p* = 0.30313, u* = 0.92745."""
+    ps, us = RX.star_state(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4)
+    assert abs(ps - 0.30313) < 5e-6 and abs(us - 0.92745) < 5e-6
+
+
+# ---------------------------------------------------------------- geometry
+def test_oracle_geometry_matches_closed_form_tables():
+    """Node-based geometry (split.c, normalvector.c, tagvector.c, meshproperties.c,
+    distance.c) on a dyadic AMR mesh equals the product's per-level tables exactly."""
+    p = sod_params()
+    m = random_amr_mesh(p, base_level=2, max_level=4, seed=11, frac=0.3, passes=2)
+    om = O.OracleMesh(m)
+    g = om.geometry()
+    geo = F.fv_geo()
+    F.lib().fv_geo_build(1.0, geo)
+    a = m.arrays()
+    fr = np.array([[geo.frame[f][k] for k in range(9)] for f in range(6)])
+    for i in range(m.n):
+        l = a["level"][i]
+        assert g["Vol"][i] == geo.vol[l]
+        for f in range(6):
+            assert np.array_equal([g["nx"][i, f], g["ny"][i, f], g["nz"][i, f]], fr[f, 0:3])
+            assert np.array_equal([g["nxt1"][i, f], g["nyt1"][i, f], g["nzt1"][i, f]], fr[f, 3:6])
+            assert np.array_equal([g["nxt2"][i, f], g["nyt2"][i, f], g["nzt2"][i, f]], fr[f, 6:9])
+            lfc = a["lfc_neigs"][i, f]
+            if lfc == 4:
+                assert np.all(g["Area"][i, f] == geo.area[l + 1])
+                k = 1
+            else:
+                assert g["Area"][i, f, 0] == geo.area[l]
+                if lfc == 0:
+                    assert g["dist_cf"][i, f] == 0.0 and g["dist_cc"][i, f] == 0.0
+                    continue
+                j = a["neigtr"][i, f, 0]
+                k = 2 if a["level"][j] < l else 0
+            assert g["dist_cf"][i, f] == geo.dcf[k][l] and g["dist_cc"][i, f] == geo.dcc[k][l]
+    # closed forms: h^2 faces, distances h, 0.75 h, 1.5 h, volume = h^3 to rounding
+    for l in range(1, 8):
+        h = 1.0 / (1 << (l - 1))
+        assert geo.area[l] == h * h and geo.dcc[0][l] == h and geo.dcc[1][l] == 0.75 * h and geo.dcc[2][l] == 1.5 * h
+        assert abs(geo.vol[l] - h ** 3) <= 4e-16 * h ** 3
+    # normals are the outward unit axes: f0 -y, f1 +x, f2 +y, f3 -x, f4 -z, f5 +z
+    assert np.array_equal(np.abs(fr[:, 0:3]), np.array([[0, 1, 0], [1, 0, 0], [0, 1, 0], [1, 0, 0], [0, 0, 1], [0, 0, 1]]))
+    assert list(np.sign(fr[:, 0:3].sum(1))) == [-1, 1, 1, -1, -1, 1]
+    for f in range(6):   # orthonormal right-handed frame
+        Mf = fr[f].reshape(3, 3)
+        assert np.allclose(Mf @ Mf.T, np.eye(3)) and np.isclose(np.linalg.det(Mf), 1.0)
+
+
+# ---------------------------------------------------------------- EOS and fluxes
+def rand_state(p, rng, u=True):
+    nm = p.nmat
+    a0 = rng.uniform(0.05, 0.95)
+    avf = np.array([a0, 1 - a0])
+    rho = np.array([rng.uniform(500, 1500), rng.uniform(0.5, 3.0)]) if p.pinf[0] > 0 else rng.uniform(0.1, 2, 2)
+    pr = rng.uniform(1e5, 1e7) if p.pinf[0] > 0 else rng.uniform(0.1, 2)
+    vel = rng.uniform(-1, 1, 3) * (10 if p.pinf[0] > 0 else 0.5) * (1 if u else 0)
+    S = np.zeros(3 * nm + 3)
+    S[0:2] = avf * rho
+    r = S[0:2].sum()
+    S[2:5] = r * vel
+    are = avf * (pr + np.array(p.gamma[:2]) * np.array(p.pinf[:2])) / (np.array(p.gamma[:2]) - 1)
+    S[7:9] = are
+    S[5] = are.sum() + 0.5 * r * (vel ** 2).sum()
+    S[6] = avf[0]
+    return S, pr, vel
+
+
+@pytest.mark.parametrize("water", [False, True])
+def test_physical_flux_closed_form(water):
+    p = water_air_params() if water else sod_params()
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        S, pr, vel = rand_state(p, rng)
+        n = np.array([0.0, 0.0, 1.0]) if _ % 2 else np.array([1.0, 0.0, 0.0])
+        Fx = O.physical_flux(p, S, n)
+        un = vel @ n
+        r = S[0] + S[1]
+        assert np.allclose(Fx[0:2], S[0:2] * un, rtol=1e-13)
+        assert np.allclose(Fx[2:5], r * vel * un + pr * n, rtol=1e-11, atol=1e-9 * pr)
+        assert np.isclose(Fx[5], (S[5] + pr) * un, rtol=1e-10)
+        assert np.isclose(Fx[6], S[6] * un, rtol=1e-13)
+
+
+@pytest.mark.parametrize("water", [False, True])
+def test_hllc_consistency(water):
+    """F_HLLC(U, U) = F(U).n (m4_hllc.c with identical left and right states)."""
+    p = water_air_params() if water else sod_params()
+    rng = np.random.default_rng(3)
+    frames = [([1, 0, 0], [0, 0, 1], [0, -1, 0]), ([0, 1, 0], [0, 0, 1], [1, 0, 0]), ([0, 0, -1], [1, 0, 0], [0, -1, 0])]
+    for t in range(60):
+        S, pr, vel = rand_state(p, rng)
+        n, t1, t2 = frames[t % 3]
+        fl, us = O.hllc(p, S, S, n, t1, t2)
+        fx = O.physical_flux(p, S, n)
+        scale = np.abs(fx).max() + pr
+        assert np.allclose(fl, fx, rtol=1e-10, atol=1e-12 * scale), (fl, fx)
+        assert np.allclose(us, vel, rtol=1e-9, atol=1e-9 * np.abs(vel).max())
+
+
+def test_hllc_literal_energy_deviation_closed_form():
+    """With the published star energy (fix bit FV_FIX_ENERGY off, m4_star_region.c:46) the
+    energy flux of identical states deviates from F(U).n by exactly 2 S_K s11 u / (u* - S_K)
+    in the star regions -- pins the oracle to the literal formula and to its correction."""
+    p = sod_params(fix=F.FIX_NONCONS)
+    rng = np.random.default_rng(12)
+    n, t1, t2 = [1, 0, 0], [0, 0, 1], [0, -1, 0]
+    seen = 0
+    for _ in range(80):
+        S, pr, vel = rand_state(p, rng)
+        fl, us = O.hllc(p, S, S, n, t1, t2)
+        fx = O.physical_flux(p, S, n)
+        c = np.sqrt(1.4 * pr / (S[0] + S[1]))
+        u = vel[0]
+        SL, SR = u - c, u + c
+        if SL >= 0 or SR <= 0:
+            dev = 0.0
+        else:
+            SK = SL if us[0] >= 0 else SR
+            dev = SK * 2 * (-pr) * u / (u - SK)
+            seen += 1
+        assert np.isclose(fl[5] - fx[5], dev, rtol=1e-8, atol=1e-12 * (abs(fx[5]) + pr)), (fl[5] - fx[5], dev)
+        assert np.allclose(np.delete(fl, 5), np.delete(fx, 5), rtol=1e-10, atol=1e-12 * (np.abs(fx).max() + pr))
+    assert seen > 20
+
+
+def test_hllc_stationary_contact_and_reflection():
+    """Equal pressure, zero velocity, different materials: no mass flux, momentum flux p n.
+    Reflective wall (bc 4): zero mass and energy flux for any normal velocity."""
+    p = water_air_params()
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        SL, pr, _v = rand_state(p, rng, u=False)
+        SR, _, _ = rand_state(p, rng, u=False)
+        # same pressure on both sides
+        for S in (SR,):
+            avf = np.array([S[6], 1 - S[6]])
+            are = avf * (pr + np.array(p.gamma[:2]) * np.array(p.pinf[:2])) / (np.array(p.gamma[:2]) - 1)
+            S[7:9] = are
+            S[5] = are.sum()
+        fl, us = O.hllc(p, SL, SR, [1, 0, 0], [0, 0, 1], [0, -1, 0])
+        assert np.allclose(fl[0:2], 0, atol=1e-12 * SL[0:2].sum()) and abs(us[0]) < 1e-9
+        assert np.isclose(fl[2], pr, rtol=1e-9)
+        S, pr2, vel = rand_state(p, rng)
+        fw, usw = O.hllc(p, S, S, [1, 0, 0], [0, 0, 1], [0, -1, 0], bc=4)
+        assert usw[0] == 0.0
+        assert np.allclose(fw[[0, 1, 5]], 0.0, atol=1e-9 * (abs(S[5]) + pr2))
+
+
+def test_relaxation_prelax1_equalises_pressures():
+    """PRELAX=1 (m4_relaxationvec.c:63-127): after relaxation both phasic pressures equal
+    the mixture pressure recovered from total energy; mass, momentum, energy untouched."""
+    p = water_air_params(prelax=1)
+    rng = np.random.default_rng(5)
+    G, Pi = np.array(p.gamma[:2]), np.array(p.pinf[:2])
+    for _ in range(100):
+        S, pr, vel = rand_state(p, rng)
+        S[7] *= 1 + rng.uniform(-0.02, 0.02)       # phasic pressures out of equilibrium
+        S[5] = S[7] + S[8] + 0.5 * (S[0] + S[1]) * (vel ** 2).sum()
+        R = O.relax(p, S)
+        assert np.array_equal(R[0:6], S[0:6])
+        avf = np.array([R[6], 1 - R[6]])
+        pk = (G - 1) * R[7:9] / avf - G * Pi
+        assert abs(pk[0] - pk[1]) <= 1e-9 * (abs(pk[0]) + Pi[0])
+        r = R[0] + R[1]
+        e_hyd = R[5] / r - 0.5 * ((R[2:5] / r) ** 2).sum()
+        assert np.isclose(R[7] + R[8], r * e_hyd, rtol=1e-12)
+        # the phasic densities obey the stiffened-gas isentrope consistency: alpha_k in (0, 1)
+        assert 0 < R[6] < 1
+
+
+def test_relaxation_prelax3_newton_residual():
+    """PRELAX=3: sum_k (alpha rho)_k / rho_k(p) = 1 to the Newton tolerance."""
+    p = water_air_params(prelax=3)
+    rng = np.random.default_rng(6)
+    G, Pi = np.array(p.gamma[:2]), np.array(p.pinf[:2])
+    for _ in range(50):
+        S, pr, vel = rand_state(p, rng)
+        S[7] *= 1 + rng.uniform(-0.01, 0.01)
+        S[5] = S[7] + S[8] + 0.5 * (S[0] + S[1]) * (vel ** 2).sum()
+        R = O.relax(p, S)
+        avf = np.array([R[6], 1 - R[6]])
+        assert abs(avf.sum() - 1) < 1e-15 and np.all(avf > 0)
+        pk = (G - 1) * R[7:9] / avf - G * Pi
+        assert abs(pk[0] - pk[1]) <= 1e-6 * (abs(pk[0]) + Pi[0])
+
+
+# ---------------------------------------------------------------- reconstruction
+def test_muscl_minmod_brute_force_on_a_line():
+    """Face states of a 1-D line of cells (roots 12x1x1) against a direct minmod formula;
+    PRELAX=0 so relaxation only clips the volume fraction."""
+    p = sod_params(prelax=0, bc=(2, 2, 4, 4, 4, 4))
+    m = F.Mesh.uniform((12, 1, 1), (0, 0, 0), 1.0, 1, list(p.bc))
+    rng = np.random.default_rng(8)
+    x = m.centres()[:, 0]
+    S = np.zeros((m.n, 9))
+    rho = 1 + 0.5 * np.sin(x) + 0.3 * rng.random(m.n)
+    S[:, 0] = rho * (1 - 1e-8); S[:, 1] = rho * 1e-8
+    S[:, 2] = rng.normal(size=m.n); S[:, 5] = 3 + rng.random(m.n)
+    S[:, 6] = 1 - 1e-8; S[:, 7] = S[:, 5] - 1e-8; S[:, 8] = 1e-8
+    om = O.OracleMesh(m)
+    SF = om.faceinterp(p, S)
+    for i in range(m.n):
+        for f, sgn in ((1, +1), (3, -1)):
+            exp = S[i].copy()
+            if 0 < i < m.n - 1:
+                fwd = S[i + sgn] - S[i]
+                bwd = S[i - sgn] - S[i]
+                for v in range(9):
+                    if abs(fwd[v]) > 1e-10:
+                        a, b = -bwd[v], fwd[v]         # both per unit distance h = 1
+                        if a * b > 0:
+                            exp[v] = S[i, v] + 0.5 * (b if abs(a) > abs(b) else a)
+            exp[6] = min(max(exp[6], 1e-10), 1 - 2e-10)
+            assert np.allclose(SF[i, f, [0, 1, 2, 5]], exp[[0, 1, 2, 5]], rtol=1e-15, atol=0), (i, f)
+
+
+# ---------------------------------------------------------------- whole stages
+def test_rk_stage_weights_closed_form():
+    """S after stage irk = a SN + b S - c dt RHS / V on the mass and momentum components
+    (RK.c:17-45) for every stage of every table."""
+    p, m, S0 = sod_case(level=3, prelax=0)
+    om = O.OracleMesh(m)
+    V = om.geometry()["Vol"][:, None]
+    W = {1: (0, 1, 1), 2: (0.75, 0.25, 0.25), 3: (1 / 3, 2 / 3, 2 / 3), 4: (0.5, 0.5, 0.5)}
+    rng = np.random.default_rng(9)
+    SN = S0.copy()
+    Scur = S0.copy()
+    Scur[:, 0] *= 1 + 0.01 * rng.random(m.n)
+    for rk, cases in ((F.RK_SSP3, [1, 2, 3]), (F.RK_SSP2, [1, 4]), (F.RK_REF2, [1, 2])):
+        p.rk = rk
+        for irk, cs in enumerate(cases, start=1):
+            Snew, SF, RHS = om.stage(p, irk, SN, Scur)
+            a, b, c = W[cs]
+            if cs == 1:
+                exp = SN - p.dt * RHS / V
+            else:
+                exp = a * SN + b * Scur - c * p.dt * RHS / V
+            assert np.allclose(Snew[:, 0:6], exp[:, 0:6], rtol=1e-14, atol=1e-14)
+
+
+def test_mirror_rule_is_a_pure_reuse():
+    """Interface flags off (flux computed once, negated on the other side) vs on everywhere
+    (both sides compute): identical to rounding on a conforming mesh."""
+    p, m, S = sod_case(level=4, prelax=0)
+    S[:, 2] = 0.1 * np.sin(np.arange(m.n))
+    S[:, 5] += 0.5 * S[:, 2] ** 2 / (S[:, 0] + S[:, 1])
+    om1 = O.OracleMesh(m)
+    om2 = O.OracleMesh(m, SI=np.tile([1, 0], (m.n, 1)))
+    A = om1.advance(p, S, 3)
+    B = om2.advance(p, S, 3)
+    assert not np.array_equal(A, B) or True
+    assert np.allclose(A, B, rtol=1e-12, atol=1e-13)
+
+
+def test_sod_converges_to_exact_solution():
+    """Sod along x on 1-D lines of cells (walls far from the waves): the L1 density error
+    against Toro's exact solution is small and falls with resolution."""
+    errs = []
+    for N in (64, 128):
+        p = sod_params(prelax=0, bc=(4, 4, 4, 4, 4, 4))
+        m = F.Mesh.uniform((N, 1, 1), (0, 0, 0), 1.0 / N, 1, list(p.bc))
+        S = m.init_split_x(p, (1.0, 0.125), (1.0, 0.1), 0.5)
+        h = 1.0 / N
+        p.dt = 0.4 * h / (np.sqrt(1.4) + 1.0)
+        nst = int(round(0.2 / p.dt))
+        p.dt = 0.2 / nst
+        om = O.OracleMesh(m)
+        Sf = om.advance(p, S, nst)
+        x = m.centres()[:, 0]
+        ex = RX.sample(x, 0.2, 0.5, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4)
+        rho = Sf[:, 0] + Sf[:, 1]
+        errs.append(np.mean(np.abs(rho - ex[:, 0])))
+    assert errs[0] < 0.02 and errs[1] < 0.7 * errs[0], errs
+
+
+def test_axis_permutation_symmetry():
+    """The same shock tube along x, y and z gives the same profile (catches transposed
+    face indices and frames)."""
+    outs = []
+    for ax in range(3):
+        nroot = [1, 1, 1]
+        nroot[ax] = 16
+        p = sod_params(prelax=0)
+        m = F.Mesh.uniform(tuple(nroot), (0, 0, 0), 1.0 / 16, 1, list(p.bc))
+        c = m.centres()[:, ax]
+        S = np.zeros((m.n, 9))
+        for i in range(m.n):
+            mat = 1 if c[i] > 0.5 else 0
+            S[i] = F.init_pure(p, (1.0, 0.125), mat, (1.0, 0.1)[mat])
+        p.dt = 0.3 / 16 / 1.2
+        Sf = O.OracleMesh(m).advance(p, S, 10)
+        order = np.argsort(c)
+        prof = Sf[order]
+        mom = prof[:, 2 + ax].copy()
+        prof = np.delete(prof, [2, 3, 4], axis=1)
+        outs.append((prof, mom))
+    for k in (1, 2):
+        assert np.allclose(outs[k][0], outs[0][0], rtol=1e-12, atol=1e-14)
+        assert np.allclose(outs[k][1], outs[0][1], rtol=1e-12, atol=1e-14)
+
+
+def test_conservation_in_a_closed_box():
+    """Reflective walls, conforming mesh: total partial masses and total energy are
+    conserved to rounding (mirror fluxes cancel, wall fluxes vanish)."""
+    p, m, S = bubble_case(level=4, prelax=1, bc=(4, 4, 4, 4, 4, 4))
+    om = O.OracleMesh(m)
+    V = om.geometry()["Vol"]
+    tot0 = (S * V[:, None]).sum(0)
+    Sf = om.advance(p, S, 5)
+    tot1 = (Sf * V[:, None]).sum(0)
+    for v in (0, 1, 5):
+        assert abs(tot1[v] - tot0[v]) <= 1e-12 * abs(tot0[v]), (v, tot0[v], tot1[v])
+
+
+def test_free_stream_on_amr_mesh():
+    """A uniform state at rest stays uniform on a mesh with 2:1 and 1:4 faces."""
+    p = sod_params(prelax=0)
+    m = random_amr_mesh(p, base_level=2, max_level=4, seed=13, frac=0.3, passes=2)
+    S = np.tile([1.0 - 1e-8, 1e-8, 0, 0, 0, 2.5, 1 - 1e-8, 2.5 - 2.5e-8, 2.5e-8], (m.n, 1))
+    Sf = O.OracleMesh(m).advance(p, S, 2)
+    assert np.allclose(Sf, S, rtol=1e-14, atol=1e-15)
