@@ -14,7 +14,8 @@ def sod_params(rk=F.RK_SSP3, prelax=0, dt=None, level=6, limiter=0, bc=(4, 4, 4,
     """Config C0: two identical ideal-gas materials, left rho=1 p=1, right rho=0.125 p=0.1."""
     h = 1.0 / (1 << (level - 1))
     if dt is None:
-        dt = 0.5 * h / np.sqrt(1.4)     # CFL 0.5 on the left state's sound speed
+        # CFL 0.5 against the largest signal speed of the Sod solution: |u| + c <= 1.93 < 2
+        dt = 0.5 * h / 2.0
     return F.make_params(nmat=2, gamma=(SOD_GAMMA, SOD_GAMMA), pinf=(0.0, 0.0), amin=1e-8, avf_lim=0.5,
                          prelax=prelax, rk=rk, limiter=limiter, dt=dt, bc=bc, fix=fix)
 
