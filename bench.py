@@ -54,7 +54,7 @@ def c1_crit(maxlevel, layers):
                        inter0_on=1, inter0_lvl=maxlevel, inter0_lrs=layers, inter1_on=0)
 
 
-def build_c1(maxlevel=9, base=5, layers=16, log_fn=log):
+def build_c1(maxlevel=9, base=5, layers=64, log_fn=log):
     """Geometric pre-adaptation (main.c:107-134): init, indicator, smoothing, refine, repeated."""
     p = c1_params()
     m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), L_DOMAIN, base, list(p.bc))
@@ -130,7 +130,8 @@ def main():
     ap.add_argument("--workload", default="c1", choices=["c1", "u128", "u256", "u320"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true")
-    ap.add_argument("--layers", type=int, default=16)
+    ap.add_argument("--layers", type=int, default=64)
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "exact"])
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo design is "
@@ -153,10 +154,11 @@ def main():
             p, m, S = build_uniform(lvl)
         cfg = {"workload": f"uniform_{a.workload[1:]}^3_water_air_bubble", "leaves": m.n}
     nq = F.neq(p.nmat)
-    cfg.update({"prelax": p.prelax, "rk": "SSP-RK3", "limiter": "minmod", "neq": nq})
+    cfg.update({"prelax": p.prelax, "rk": "SSP-RK3", "limiter": "minmod", "neq": nq, "precision": a.prec})
     log(f"setup {time.perf_counter() - t_setup:.1f} s, {m.n} leaves")
 
-    dev = F.Device(p, m)
+    prec = {"fp64": F.PREC_FP64, "mixed": F.PREC_MIXED, "exact": F.PREC_EXACT}[a.prec]
+    dev = F.Device(p, m, prec=prec)
     dev.set_state(S)
     dev.time_stages(max(1, a.warmup))
     t0 = time.perf_counter()
@@ -166,6 +168,7 @@ def main():
     dev.close()
     if not np.all(np.isfinite(Sg)):
         raise SystemExit("non-finite state after the timed stages")
+    copy_gbps = F.stream_copy_gbps(1 << 30, 10 if a.profile_only else 20)
     if a.profile_only:
         return
 
@@ -176,13 +179,14 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s per RK stage",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_stage, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "exact": "f64", "mixed": "f32 faces / f64 state"}[a.prec],
         "data": "synthetic", "config": cfg,
         "wall_s_timed": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                      "kernel": "stage = k_recon + k_flux_rk", "algorithmic_bytes_per_leaf": B,
-                     "ms_k_flux_rk": round(ms_flux, 4)},
+                     "ms_flux_kernel": round(ms_flux, 4), "stream_copy_GBps": round(copy_gbps, 1)},
     }
     if not a.no_cpu_baseline:
         cb, l1, stages = cpu_baseline()

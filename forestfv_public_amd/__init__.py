@@ -30,7 +30,7 @@ FV_MAXLEVEL = 10
 FV_OK, FV_ERR_NAN, FV_ERR_ARG, FV_ERR_HIP, FV_ERR_MESH, FV_ERR_NOMEM = 0, -1, -2, -3, -4, -5
 FK_BOUNDARY, FK_SAME, FK_MIRROR, FK_COARSE, FK_FINE = 0, 1, 2, 3, 4
 RK_REF2, RK_SSP3, RK_SSP2 = 2, 3, 22
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_EXACT = 0, 1, 2
 FIX_NONCONS, FIX_ENERGY, FIX_ALL = 1, 2, 3
 
 
@@ -162,6 +162,7 @@ def lib():
             "fv_dev_time_stages": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
             "fv_dev_size": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
             "fv_dev_last_error": (C.c_char_p, []),
+            "fv_dev_stream_copy": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -369,10 +370,17 @@ class Pack:
         return dict(nbr=nbr, info=info, fine4=fine4)
 
 
+def stream_copy_gbps(nbytes=1 << 30, reps=20):
+    """Measured device copy bandwidth (read + write GB/s)."""
+    g = C.c_double()
+    _check(lib().fv_dev_stream_copy(nbytes, reps, C.byref(g)), "fv_dev_stream_copy")
+    return g.value
+
+
 class Device:
     """One device context (fv_dev) over a packed mesh."""
 
-    def __init__(self, params, mesh_or_pack):
+    def __init__(self, params, mesh_or_pack, prec=None):
         pk = mesh_or_pack if isinstance(mesh_or_pack, Pack) else Pack(mesh_or_pack)
         self._pack = pk
         self.params = params
@@ -381,6 +389,8 @@ class Device:
         h = C.c_void_p()
         _check(lib().fv_dev_attach(C.byref(h), C.byref(params), C.byref(pk.pk)), "fv_dev_attach")
         self._h = h
+        if prec is not None:
+            self.set_precision(prec)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -23,8 +23,10 @@
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#include <type_traits>
 #include "forestfv_dev.h"
 #include "fv_phys.h"
+#include "fv_fast.h"
 
 using namespace fvk;
 
@@ -278,6 +280,211 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_rk(int n, Params P, Geo G, in
   for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = Sn[v];
 }
 
+// ---------------------------------------------------------------------
+// throughput kernels: plain FP64 (MIXED = false) and mixed precision
+// (MIXED = true: FP32 slopes, face records and HLLC; FP64 relaxation,
+// pressure recovery and RK accumulation)
+// ---------------------------------------------------------------------
+#pragma clang fp contract(fast)
+template <int NM, bool MIXED>
+__global__ __launch_bounds__(FV_BLOCK) void k_recon_fast(int n, Params P, Geo G, const double *__restrict__ S,
+                                                         const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+                                                         const int *__restrict__ fine4, double *__restrict__ SF,
+                                                         float *__restrict__ SFm) {
+  using L = Lay<NM>;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const fvf::Mat<NM, double> M = fvf::make_mat<NM, double>(P);
+  const double a = P.amin / 100.0;
+  const uint32_t inf = info[i];
+  const int lvl = cell_level(inf);
+  double Sc[L::NEQ], Araw[NM];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
+#pragma unroll
+  for (int k = 0; k < L::N3; k++) Araw[k] = Sc[L::I3 + k];
+  avf_limit<NM, double>(a, Sc);
+#pragma unroll 1
+  for (int ax = 0; ax < 3; ax++) {
+    const int fa = (ax == 0) ? 0 : (ax == 1 ? 1 : 4);
+    const int fb = FCOPP_D[fa];
+    const int ka = face_kind(inf, fa), kb = face_kind(inf, fb);
+    double Xa[L::NEQ], Xb[L::NEQ];
+    stencil_value<NM>(S, n, i, ka, nbr[(size_t)fa * n + i], fine4, Sc, Araw, a, Xa);
+    stencil_value<NM>(S, n, i, kb, nbr[(size_t)fb * n + i], fine4, Sc, Araw, a, Xb);
+    double dcf_a, dcc_a, dcf_b, dcc_b;
+    dist_of(G, ka, lvl, dcf_a, dcc_a);
+    dist_of(G, kb, lvl, dcf_b, dcc_b);
+    const bool bnd = (ka == FV_FK_BOUNDARY) || (kb == FV_FK_BOUNDARY);
+    const R ia = bnd ? (R)0 : (R)fvf::rcp(dcc_a), ib = bnd ? (R)0 : (R)fvf::rcp(dcc_b);
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const int f = side ? fb : fa;
+      const double *Xf = side ? Xb : Xa;
+      const double *Xk = side ? Xa : Xb;
+      const double dcf = side ? dcf_b : dcf_a;
+      const R idp = side ? ib : ia, idm = side ? ia : ib;
+      double F[L::NEQ];
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) {
+        const R g = bnd ? (R)0 : fvf::slope<R>(P.limiter, Xf[v] - Sc[v], Xk[v] - Sc[v], idp, idm);
+        F[v] = Sc[v] + dcf * (double)g;
+      }
+      vf_limit_face<NM, double>(a, F);
+      fvf::relax_fast<NM>(P, M, F);
+      if (MIXED) {
+        float rec[fvf::Rec<NM>::N];
+        fvf::to_record<NM>(M, a, F, rec);
+#pragma unroll
+        for (int q = 0; q < fvf::Rec<NM>::N; q++) SFm[((size_t)f * fvf::Rec<NM>::N + q) * n + i] = rec[q];
+      } else {
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) SF[((size_t)f * L::NEQ + v) * n + i] = F[v];
+      }
+    }
+  }
+}
+
+template <int NM, bool MIXED, typename R>
+FV_DEV void load_sol(const fvf::Mat<NM, double> &Md, const fvf::Mat<NM, R> &M, double a, const double *__restrict__ SF,
+                     const float *__restrict__ SFm, int n, int cell, int f, fvf::SolF<NM, R> &s) {
+  if constexpr (MIXED) {
+    float rec[fvf::Rec<NM>::N];
+#pragma unroll
+    for (int q = 0; q < fvf::Rec<NM>::N; q++) rec[q] = SFm[((size_t)f * fvf::Rec<NM>::N + q) * n + cell];
+    fvf::fill_rec<NM>(M, rec, s);
+  } else {
+    double X[Lay<NM>::NEQ];
+    load_face<NM>(SF, n, cell, f, X);
+    fvf::fill_cons<NM, R>(Md, a, X, s);
+  }
+}
+
+template <int NM, bool MIXED>
+__global__ __launch_bounds__(FV_BLOCK) void k_flux_fast(int n, Params P, Geo G, int cs, double *__restrict__ S,
+                                                        double *__restrict__ SN, const double *__restrict__ SF,
+                                                        const float *__restrict__ SFm, const int *__restrict__ nbr,
+                                                        const uint32_t *__restrict__ info, const int *__restrict__ fine4,
+                                                        int *__restrict__ nanflag) {
+  using L = Lay<NM>;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const fvf::Mat<NM, double> Md = fvf::make_mat<NM, double>(P);
+  const fvf::Mat<NM, R> M = fvf::make_mat<NM, R>(P);
+  const double a = P.amin / 100.0;
+  const uint32_t inf = info[i];
+  const int lvl = cell_level(inf);
+  double Sc[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
+  fvf::SolF<NM, double> sc;
+  fvf::fill_cons<NM, double>(Md, a, Sc, sc);
+  double src_avf[NM], src_st[NM];
+#pragma unroll
+  for (int k = 0; k < L::N3; ++k) src_avf[k] = sc.avf[k];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) src_st[k] = -sc.avf[k] * sc.p[k];
+  double RHS[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) RHS[v] = 0.0;
+  const bool fixnc = (P.fix & 1) != 0;
+#pragma unroll 1
+  for (int f = 0; f < 6; f++) {
+    const int kind = face_kind(inf, f);
+    const int nb = nbr[(size_t)f * n + i];
+    const int fo = FCOPP_D[f];
+    double ff[L::NEQ], uf0 = 0.0, uf1 = 0.0, uf2 = 0.0, Area_f;
+    fvf::SolF<NM, R> sL, sR;
+    R fr[9];
+    if (kind == FV_FK_MIRROR) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[fo][q];
+      load_sol<NM, MIXED, R>(Md, M, a, SF, SFm, n, nb, fo, sL);
+      load_sol<NM, MIXED, R>(Md, M, a, SF, SFm, n, i, f, sR);
+      R out[L::NEQ], un, vn, wn;
+      fvf::hllc<NM, R>(M, P.fix, fr, 0, sL, sR, out, un, vn, wn);
+      const double A = G.area[lvl];
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) ff[v] = -(A * (double)out[v]);
+      const double sgn = fixnc ? 1.0 : -1.0;
+      uf0 = sgn * (double)un; uf1 = sgn * (double)vn; uf2 = sgn * (double)wn;
+      Area_f = A;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[f][q];
+      const int Nneig = (kind == FV_FK_FINE) ? 4 : 1;
+      const double A = (kind == FV_FK_FINE) ? G.area[lvl + 1] : G.area[lvl];
+      Area_f = (double)Nneig * A;
+      const double wn_ = (kind == FV_FK_FINE) ? 0.25 : 1.0;
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) ff[v] = 0.0;
+      load_sol<NM, MIXED, R>(Md, M, a, SF, SFm, n, i, f, sL);
+      for (int k = 0; k < Nneig; ++k) {
+        int bcode = 0;
+        if (kind == FV_FK_BOUNDARY) {
+          sR = sL;
+          bcode = -nb;
+          if (bcode == 1 || bcode == 4 || bcode == 5) {
+            const R ul = sL.u * fr[0] + sL.v * fr[1] + sL.w * fr[2];
+            const R vl = sL.u * fr[3] + sL.v * fr[4] + sL.w * fr[5];
+            const R wl = sL.u * fr[6] + sL.v * fr[7] + sL.w * fr[8];
+            const R vr = (bcode == 5) ? -vl : vl, wr = (bcode == 5) ? -wl : wl;
+            sR.u = -ul * fr[0] + vr * fr[3] + wr * fr[6];
+            sR.v = -ul * fr[1] + vr * fr[4] + wr * fr[7];
+            sR.w = -ul * fr[2] + vr * fr[5] + wr * fr[8];
+          }
+        } else {
+          const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
+          load_sol<NM, MIXED, R>(Md, M, a, SF, SFm, n, j, fo, sR);
+        }
+        R out[L::NEQ], un, vn, wn;
+        fvf::hllc<NM, R>(M, P.fix, fr, bcode, sL, sR, out, un, vn, wn);
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) ff[v] += A * (double)out[v];
+        uf0 += wn_ * (double)un;
+        uf1 += wn_ * (double)vn;
+        uf2 += wn_ * (double)wn;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) RHS[v] += ff[v];
+    const double nx = G.frame[f][0], ny = G.frame[f][1], nz = G.frame[f][2];
+    const double und = uf0 * nx + uf1 * ny + uf2 * nz;
+#pragma unroll
+    for (int k = 0; k < L::N3; ++k) {
+      const double tt = Area_f * src_avf[k] * und;
+      RHS[L::I3 + k] = fixnc ? RHS[L::I3 + k] - tt : RHS[L::I3 + k] + tt;
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      const double tt = Area_f * src_st[k] * und;
+      RHS[L::I4 + k] = fixnc ? RHS[L::I4 + k] - tt : RHS[L::I4 + k] + tt;
+    }
+  }
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) bad |= isnan(RHS[v]);
+  if (bad) *nanflag = 1;
+  const double dtv = P.dt * fvf::rcp(G.vol[lvl]);
+  const double wa = (cs == 1) ? 0.0 : (cs == 2 ? 0.75 : (cs == 3 ? 1.0 / 3.0 : 0.5));
+  const double wb = 1.0 - wa;
+  double Sn[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) {
+    double sn;
+    if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
+    else sn = SN[(size_t)v * n + i];
+    Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
+  }
+  fvf::relax_fast<NM>(P, Md, Sn);
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = Sn[v];
+}
+
+#pragma clang fp contract(off)
+
 // CFL: per-block minimum of h / (max|u_i| + c) with a wavefront shuffle reduction
 template <int NM>
 __global__ __launch_bounds__(FV_BLOCK) void k_cfl(int n, Params P, Geo G, const double *__restrict__ S,
@@ -474,6 +681,8 @@ struct fv_dev {
   Geo G{};
   hipStream_t stream = nullptr;
   double *S = nullptr, *SN = nullptr, *SF = nullptr, *part = nullptr, *dtmp = nullptr;
+  float *SFm = nullptr;   // mixed-precision face records, allocated on first use
+  int cap_m = 0;
   int *nbr = nullptr, *fine4 = nullptr, *nanflag = nullptr, *SIa = nullptr, *SIb = nullptr, *split = nullptr;
   uint32_t *info = nullptr;
   int cap_n = 0, cap_fine = 0;
@@ -489,6 +698,13 @@ static void to_dev_params(const fv_params *p, Params &P) {
   for (int k = 0; k < 4; k++) { P.gamma[k] = p->gamma[k]; P.pinf[k] = p->pinf[k]; }
   P.amin = p->amin; P.avf_lim = p->avf_lim; P.dt = p->dt;
   P.prelax = p->prelax; P.limiter = p->limiter; P.rk = p->rk; P.nmat = p->nmat; P.fix = p->fix;
+  for (int k = 0; k < 4; k++) {
+    const double g = p->gamma[k], pi = p->pinf[k];
+    const double gm1 = (g != 1.0) ? g - 1.0 : 1.0, gg = (g != 0.0) ? g : 1.0;
+    P.GP[k] = g * pi; P.Gm1[k] = g - 1.0; P.iGm1[k] = 1.0 / gm1; P.C[k] = g * pi / gm1; P.iG[k] = 1.0 / gg;
+    P.Gf[k] = (float)g; P.Pf[k] = (float)pi; P.GPf[k] = (float)P.GP[k]; P.Gm1f[k] = (float)P.Gm1[k];
+    P.iGm1f[k] = (float)P.iGm1[k]; P.Cf[k] = (float)P.C[k]; P.iGf[k] = (float)P.iG[k];
+  }
 }
 
 static void to_dev_geo(const fv_geo *g, Geo &G) {
@@ -501,7 +717,9 @@ static void to_dev_geo(const fv_geo *g, Geo &G) {
 }
 
 static void free_buffers(fv_dev *c) {
-  hipFree(c->S); hipFree(c->SN); hipFree(c->SF); hipFree(c->part); hipFree(c->dtmp);
+  hipFree(c->S); hipFree(c->SN); hipFree(c->SF); hipFree(c->part); hipFree(c->dtmp); hipFree(c->SFm);
+  c->SFm = nullptr;
+  c->cap_m = 0;
   hipFree(c->nbr); hipFree(c->fine4); hipFree(c->nanflag); hipFree(c->info);
   hipFree(c->SIa); hipFree(c->SIb); hipFree(c->split);
   c->S = c->SN = c->SF = c->part = c->dtmp = nullptr;
@@ -640,7 +858,7 @@ extern "C" int fv_dev_set_dt(fv_dev *c, double dt) {
 
 extern "C" int fv_dev_set_precision(fv_dev *c, int prec) {
   if (!c) return set_err(FV_ERR_ARG, "null context");
-  if (prec != FV_PREC_FP64 && prec != FV_PREC_MIXED) return set_err(FV_ERR_ARG, "unknown precision");
+  if (prec != FV_PREC_FP64 && prec != FV_PREC_MIXED && prec != FV_PREC_EXACT) return set_err(FV_ERR_ARG, "unknown precision");
   c->prec = prec;
   return FV_OK;
 }
@@ -656,21 +874,42 @@ static int stage_case(int rk, int irk) {
   return irk;
 }
 
-static int launch_stage(fv_dev *c, int irk, hipEvent_t *ev_flux0, hipEvent_t *ev_flux1) {
+template <int NM>
+static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
   const int n = c->n;
   const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
-  const int cs = stage_case(c->P.rk, irk);
-  if (c->nmat == 2) {
-    hipLaunchKernelGGL(k_recon<2>, grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info, c->fine4, c->SF);
+  if (c->prec == FV_PREC_EXACT) {
+    hipLaunchKernelGGL(k_recon<NM>, grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info, c->fine4, c->SF);
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
-    hipLaunchKernelGGL(k_flux_rk<2>, grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF, c->nbr, c->info,
-                       c->fine4, c->nanflag);
+    hipLaunchKernelGGL(k_flux_rk<NM>, grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF, c->nbr,
+                       c->info, c->fine4, c->nanflag);
+  } else if (c->prec == FV_PREC_FP64) {
+    hipLaunchKernelGGL((k_recon_fast<NM, false>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
+                       c->fine4, c->SF, c->SFm);
+    if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
+    hipLaunchKernelGGL((k_flux_fast<NM, false>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
+                       c->SFm, c->nbr, c->info, c->fine4, c->nanflag);
   } else {
-    hipLaunchKernelGGL(k_recon<3>, grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info, c->fine4, c->SF);
+    if (c->cap_m < n) {
+      HIPCHK(hipFree(c->SFm));
+      c->SFm = nullptr;
+      c->cap_m = 0;
+      HIPCHK(hipMalloc(&c->SFm, sizeof(float) * (size_t)n * 6 * fvf::Rec<NM>::N));
+      c->cap_m = n;
+    }
+    hipLaunchKernelGGL((k_recon_fast<NM, true>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
+                       c->fine4, c->SF, c->SFm);
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
-    hipLaunchKernelGGL(k_flux_rk<3>, grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF, c->nbr, c->info,
-                       c->fine4, c->nanflag);
+    hipLaunchKernelGGL((k_flux_fast<NM, true>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
+                       c->SFm, c->nbr, c->info, c->fine4, c->nanflag);
   }
+  return FV_OK;
+}
+
+static int launch_stage(fv_dev *c, int irk, hipEvent_t *ev_flux0, hipEvent_t *ev_flux1) {
+  const int cs = stage_case(c->P.rk, irk);
+  int rc = (c->nmat == 2) ? launch_stage_nm<2>(c, cs, ev_flux0) : launch_stage_nm<3>(c, cs, ev_flux0);
+  if (rc != FV_OK) return rc;
   if (ev_flux1) HIPCHK(hipEventRecord(*ev_flux1, c->stream));
   HIPCHK(hipGetLastError());
   return FV_OK;
@@ -787,6 +1026,39 @@ extern "C" int fv_dev_time_stages(fv_dev *c, int nrep, double *ms_stage, double 
   if (ms_stage) *ms_stage = all / nrep;
   if (ms_flux) *ms_flux = tot_flux / nrep;
   return check_nan(c);
+}
+
+// stream copy: measured HBM bandwidth on this device and a known-size dispatch
+// for calibrating the FETCH_SIZE / WRITE_SIZE counters
+__global__ __launch_bounds__(256) void k_copy(size_t n4, const double4 *__restrict__ a, double4 *__restrict__ b) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n4; i += stride) b[i] = a[i];
+}
+
+extern "C" int fv_dev_stream_copy(size_t bytes, int reps, double *gbps) {
+  if (bytes < 4096 || reps < 1 || !gbps) return set_err(FV_ERR_ARG, "bad argument");
+  const size_t n4 = bytes / sizeof(double4);
+  double4 *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, n4 * sizeof(double4)));
+  if (hipMalloc(&b, n4 * sizeof(double4)) != hipSuccess) { (void)hipFree(a); return set_err(FV_ERR_HIP, "alloc"); }
+  HIPCHK(hipMemset(a, 0, n4 * sizeof(double4)));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int blocks = 256 * 16;
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, n4, a, b);
+  HIPCHK(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, n4, a, b);
+  HIPCHK(hipEventRecord(e1, 0));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *gbps = 2.0 * (double)(n4 * sizeof(double4)) * reps / (ms * 1e-3) / 1e9;
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  HIPCHK(hipFree(a));
+  HIPCHK(hipFree(b));
+  return FV_OK;
 }
 
 extern "C" const char *fv_dev_last_error(void) { return g_err; }

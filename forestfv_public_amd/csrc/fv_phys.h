@@ -25,6 +25,10 @@ struct Params {
   double gamma[4], pinf[4];
   double amin, avf_lim, dt;
   int prelax, limiter, rk, nmat, fix;
+  // per-material constants formed once on the host: G p_inf, G - 1, 1/(G-1),
+  // G p_inf/(G-1), 1/G (used by the throughput kernels)
+  double GP[4], Gm1[4], iGm1[4], C[4], iG[4];
+  float Gf[4], Pf[4], GPf[4], Gm1f[4], iGm1f[4], Cf[4], iGf[4];
 };
 
 struct Geo {

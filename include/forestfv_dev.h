@@ -27,9 +27,16 @@ extern "C" {
 typedef struct fv_dev fv_dev;
 
 /* Precision of the reconstruction / flux / ghost-fill kernels.  The state,
- * its RK accumulation and pressure recovery from energy stay FP64 in both. */
+ * its RK accumulation and pressure recovery from energy stay FP64 in all.
+ *  FV_PREC_FP64  (default) plain FP64, reciprocal-multiply form of the
+ *                reference arithmetic; within 1e-10 relative L1 of the CPU
+ *                path after 100 steps;
+ *  FV_PREC_MIXED FP32 slopes, face records and HLLC;
+ *  FV_PREC_EXACT FP64 in the reference's operation order, bit-identical to
+ *                the CPU restatement. */
 #define FV_PREC_FP64  0
 #define FV_PREC_MIXED 1
+#define FV_PREC_EXACT 2
 
 /* Upload the packed mesh and parameters; allocates every device buffer of
  * the context (the device mirror of leafallocation.c / createutility.c). */
@@ -69,6 +76,11 @@ int  fv_dev_indicator(fv_dev *ctx, const fv_crit *c, int *SI, int *split);
  * runs nrep full stages (cycling irk) and returns the mean milliseconds per
  * stage and per launch of the dominant (flux) kernel. */
 int  fv_dev_time_stages(fv_dev *ctx, int nrep, double *ms_stage, double *ms_flux_kernel);
+
+/* Measurement utility with no counterpart in the reference: copies `bytes`
+ * between two device buffers `reps` times and returns read+write GB/s (the
+ * measured HBM stream bandwidth; one dispatch also calibrates byte counters). */
+int  fv_dev_stream_copy(size_t bytes, int reps, double *gbps);
 
 /* Number of leaves and NEQ of the attached mesh. */
 int  fv_dev_size(const fv_dev *ctx, int *n, int *neq);
