@@ -163,6 +163,24 @@ def lib():
             "fv_dev_size": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
             "fv_dev_last_error": (C.c_char_p, []),
             "fv_dev_stream_copy": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
+            "fv_dev_set_limiter_flags": (C.c_int, [C.c_void_p, P(C.c_uint8)]),
+            "fv_dev_get_limiter_flags": (C.c_int, [C.c_void_p, P(C.c_uint8)]),
+            "fv_mlp_create": (C.c_int, [P(C.c_void_p), C.c_uint]),
+            "fv_mlp_destroy": (C.c_int, [C.c_void_p]),
+            "fv_mlp_num_weights": (C.c_int, []),
+            "fv_mlp_get_weights": (C.c_int, [C.c_void_p, P(C.c_float)]),
+            "fv_mlp_set_weights": (C.c_int, [C.c_void_p, P(C.c_float)]),
+            "fv_mlp_set_options": (C.c_int, [C.c_void_p, P(C.c_float), C.c_float]),
+            "fv_mlp_sample": (C.c_int, [C.c_void_p, C.c_void_p, P(fv_crit)]),
+            "fv_mlp_dataset": (C.c_int, [C.c_void_p, P(C.c_long), P(C.c_float), P(C.c_int), P(C.c_float)]),
+            "fv_mlp_clear_dataset": (C.c_int, [C.c_void_p]),
+            "fv_mlp_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint, P(C.c_float)]),
+            "fv_mlp_grad": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_int), P(C.c_float), C.c_int, P(C.c_float),
+                                      P(C.c_float)]),
+            "fv_mlp_forward": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int, P(C.c_float)]),
+            "fv_mlp_infer": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int, P(C.c_float)]),
+            "fv_dev_indicator_mlp": (C.c_int, [C.c_void_p, C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int),
+                                               P(C.c_uint8)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -448,7 +466,107 @@ class Device:
         _check(lib().fv_dev_indicator(self._h, C.byref(crit), _ip(SI), _ip(split)), "fv_dev_indicator")
         return SI, split
 
+    def set_limiter_flags(self, lim):
+        lim = np.ascontiguousarray(lim, dtype=np.uint8)
+        assert lim.shape == (self.n,)
+        _check(lib().fv_dev_set_limiter_flags(self._h, lim.ctypes.data_as(C.POINTER(C.c_uint8))), "limiter flags")
+
+    def get_limiter_flags(self):
+        lim = np.zeros(self.n, dtype=np.uint8)
+        _check(lib().fv_dev_get_limiter_flags(self._h, lim.ctypes.data_as(C.POINTER(C.c_uint8))), "limiter flags")
+        return lim
+
+    def indicator_mlp(self, mlp, crit):
+        SI = np.zeros((self.n, 2), dtype=np.int32)
+        split = np.zeros(self.n, dtype=np.int32)
+        sm = np.zeros(self.n, dtype=np.uint8)
+        _check(lib().fv_dev_indicator_mlp(self._h, mlp._h, C.byref(crit), _ip(SI), _ip(split),
+                                          sm.ctypes.data_as(C.POINTER(C.c_uint8))), "fv_dev_indicator_mlp")
+        return SI, split, sm
+
     def time_stages(self, nrep):
         a, b = C.c_double(), C.c_double()
         _check(lib().fv_dev_time_stages(self._h, nrep, C.byref(a), C.byref(b)), "fv_dev_time_stages")
         return a.value, b.value
+
+
+MLP_K = 112
+MLP_H = 64
+MLP_NO = 4
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class MLP:
+    """Learned refinement / limiter indicator (fv_mlp): weights, dataset and training on the device."""
+
+    def __init__(self, seed=1234):
+        h = C.c_void_p()
+        _check(lib().fv_mlp_create(C.byref(h), seed), "fv_mlp_create")
+        self._h = h
+        self.nw = lib().fv_mlp_num_weights()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fv_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def weights(self):
+        w = np.zeros(self.nw, dtype=np.float32)
+        _check(lib().fv_mlp_get_weights(self._h, _fp(w)), "weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().fv_mlp_set_weights(self._h, _fp(w)), "weights")
+
+    def set_options(self, class_weights=None, smooth_thr=0.0):
+        cw = None if class_weights is None else np.ascontiguousarray(class_weights, dtype=np.float32)
+        _check(lib().fv_mlp_set_options(self._h, None if cw is None else _fp(cw), smooth_thr), "options")
+
+    def sample(self, dev, crit):
+        _check(lib().fv_mlp_sample(dev._h, self._h, C.byref(crit)), "fv_mlp_sample")
+
+    def dataset(self):
+        n = C.c_long()
+        _check(lib().fv_mlp_dataset(self._h, C.byref(n), None, None, None), "dataset")
+        X = np.zeros((n.value, MLP_K), dtype=np.float32)
+        cls = np.zeros(n.value, dtype=np.int32)
+        sm = np.zeros(n.value, dtype=np.float32)
+        _check(lib().fv_mlp_dataset(self._h, C.byref(n), _fp(X), _ip(cls), _fp(sm)), "dataset")
+        return X, cls, sm
+
+    def clear_dataset(self):
+        _check(lib().fv_mlp_clear_dataset(self._h), "clear")
+
+    def train(self, epochs, batch=1024, lr=1e-3, seed=7):
+        loss = C.c_float()
+        _check(lib().fv_mlp_train(self._h, epochs, batch, lr, seed, C.byref(loss)), "fv_mlp_train")
+        return loss.value
+
+    def grad(self, X, cls, smooth):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        cls = np.ascontiguousarray(cls, dtype=np.int32)
+        smooth = np.ascontiguousarray(smooth, dtype=np.float32)
+        g = np.zeros(self.nw, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().fv_mlp_grad(self._h, _fp(X), _ip(cls), _fp(smooth), len(cls), _fp(g), C.byref(loss)), "grad")
+        return g, loss.value
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros((X.shape[0], MLP_NO), dtype=np.float32)
+        _check(lib().fv_mlp_forward(self._h, _fp(X), X.shape[0], _fp(out)), "forward")
+        return out
+
+    def infer(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros((X.shape[0], MLP_NO), dtype=np.float32)
+        _check(lib().fv_mlp_infer(self._h, _fp(X), X.shape[0], _fp(out)), "infer")
+        return out

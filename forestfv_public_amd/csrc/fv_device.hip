@@ -25,7 +25,7 @@
 #include <math.h>
 #include <type_traits>
 #include "forestfv_dev.h"
-#include "fv_phys.h"
+#include "fv_internal.h"
 #include "fv_fast.h"
 
 using namespace fvk;
@@ -92,13 +92,15 @@ FV_DEV void dist_of(const Geo &G, int kind, int lvl, double &dcf, double &dcc) {
 template <int NM>
 __global__ __launch_bounds__(FV_BLOCK) void k_recon(int n, Params P, Geo G, const double *__restrict__ S,
                                                     const int *__restrict__ nbr, const uint32_t *__restrict__ info,
-                                                    const int *__restrict__ fine4, double *__restrict__ SF) {
+                                                    const int *__restrict__ fine4, const uint8_t *__restrict__ lim,
+                                                    double *__restrict__ SF) {
   using L = Lay<NM>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double a = P.amin / 100.0;
   const uint32_t inf = info[i];
   const int lvl = cell_level(inf);
+  const int limiter = (P.limiter == 2) ? (int)lim[i] : P.limiter;
   double Sc[L::NEQ], Araw[NM];
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon(int n, Params P, Geo G, cons
         double df = Xf[v] - Sc[v];
         double db = Xk[v] - Sc[v];
         double g = 0.0;
-        if (kf != FV_FK_BOUNDARY) g = limited_slope<double>(P.limiter, df, db, dp, dm);
+        if (kf != FV_FK_BOUNDARY) g = limited_slope<double>(limiter, df, db, dp, dm);
         F[v] = Sc[v] + dcf * g;
       }
       vf_limit_face<NM, double>(a, F);
@@ -289,8 +291,8 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_rk(int n, Params P, Geo G, in
 template <int NM, bool MIXED>
 __global__ __launch_bounds__(FV_BLOCK) void k_recon_fast(int n, Params P, Geo G, const double *__restrict__ S,
                                                          const int *__restrict__ nbr, const uint32_t *__restrict__ info,
-                                                         const int *__restrict__ fine4, double *__restrict__ SF,
-                                                         float *__restrict__ SFm) {
+                                                         const int *__restrict__ fine4, const uint8_t *__restrict__ lim,
+                                                         double *__restrict__ SF, float *__restrict__ SFm) {
   using L = Lay<NM>;
   using R = typename std::conditional<MIXED, float, double>::type;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -299,6 +301,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon_fast(int n, Params P, Geo G,
   const double a = P.amin / 100.0;
   const uint32_t inf = info[i];
   const int lvl = cell_level(inf);
+  const int limiter = (P.limiter == 2) ? (int)lim[i] : P.limiter;
   double Sc[L::NEQ], Araw[NM];
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
@@ -328,7 +331,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon_fast(int n, Params P, Geo G,
       double F[L::NEQ];
 #pragma unroll
       for (int v = 0; v < L::NEQ; v++) {
-        const R g = bnd ? (R)0 : fvf::slope<R>(P.limiter, Xf[v] - Sc[v], Xk[v] - Sc[v], idp, idm);
+        const R g = bnd ? (R)0 : fvf::slope<R>(limiter, Xf[v] - Sc[v], Xk[v] - Sc[v], idp, idm);
         F[v] = Sc[v] + dcf * (double)g;
       }
       vf_limit_face<NM, double>(a, F);
@@ -664,34 +667,13 @@ __global__ __launch_bounds__(FV_BLOCK) void k_split(int n, Params P, CritDev C, 
 // ---------------------------------------------------------------------
 // host side of the C ABI
 // ---------------------------------------------------------------------
-static thread_local char g_err[256];
+thread_local char fv_g_err[256];
 
-#define HIPCHK(x)                                                                                  \
-  do {                                                                                             \
-    hipError_t e_ = (x);                                                                           \
-    if (e_ != hipSuccess) {                                                                        \
-      snprintf(g_err, sizeof g_err, "%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
-      return FV_ERR_HIP;                                                                           \
-    }                                                                                              \
-  } while (0)
-
-struct fv_dev {
-  int n = 0, neq = 0, nmat = 0, nfine = 0, prec = FV_PREC_FP64;
-  Params P{};
-  Geo G{};
-  hipStream_t stream = nullptr;
-  double *S = nullptr, *SN = nullptr, *SF = nullptr, *part = nullptr, *dtmp = nullptr;
-  float *SFm = nullptr;   // mixed-precision face records, allocated on first use
-  int cap_m = 0;
-  int *nbr = nullptr, *fine4 = nullptr, *nanflag = nullptr, *SIa = nullptr, *SIb = nullptr, *split = nullptr;
-  uint32_t *info = nullptr;
-  int cap_n = 0, cap_fine = 0;
-};
-
-static int set_err(int code, const char *msg) {
-  snprintf(g_err, sizeof g_err, "%s", msg);
+int fv_set_err(int code, const char *msg) {
+  snprintf(fv_g_err, sizeof fv_g_err, "%s", msg);
   return code;
 }
+static int set_err(int code, const char *msg) { return fv_set_err(code, msg); }
 
 static void to_dev_params(const fv_params *p, Params &P) {
   memset(&P, 0, sizeof P);
@@ -721,7 +703,8 @@ static void free_buffers(fv_dev *c) {
   c->SFm = nullptr;
   c->cap_m = 0;
   hipFree(c->nbr); hipFree(c->fine4); hipFree(c->nanflag); hipFree(c->info);
-  hipFree(c->SIa); hipFree(c->SIb); hipFree(c->split);
+  hipFree(c->SIa); hipFree(c->SIb); hipFree(c->split); hipFree(c->lim);
+  c->lim = nullptr;
   c->S = c->SN = c->SF = c->part = c->dtmp = nullptr;
   c->nbr = c->fine4 = c->nanflag = c->SIa = c->SIb = c->split = nullptr;
   c->info = nullptr;
@@ -747,6 +730,7 @@ static int upload_mesh(fv_dev *c, const fv_pack *pk) {
     HIPCHK(hipMalloc(&c->SIa, sizeof(int) * n * 2));
     HIPCHK(hipMalloc(&c->SIb, sizeof(int) * n * 2));
     HIPCHK(hipMalloc(&c->split, sizeof(int) * n));
+    HIPCHK(hipMalloc(&c->lim, n));
     c->cap_n = pk->n;
     c->cap_fine = (int)nf;
   }
@@ -758,6 +742,7 @@ static int upload_mesh(fv_dev *c, const fv_pack *pk) {
   if (pk->nfine > 0)
     HIPCHK(hipMemcpyAsync(c->fine4, pk->fine4, sizeof(int) * 4 * (size_t)pk->nfine, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->nanflag, 0, sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->lim, 0, n, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return FV_OK;
 }
@@ -879,13 +864,14 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
   const int n = c->n;
   const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
   if (c->prec == FV_PREC_EXACT) {
-    hipLaunchKernelGGL(k_recon<NM>, grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info, c->fine4, c->SF);
+    hipLaunchKernelGGL(k_recon<NM>, grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info, c->fine4, c->lim,
+                       c->SF);
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
     hipLaunchKernelGGL(k_flux_rk<NM>, grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF, c->nbr,
                        c->info, c->fine4, c->nanflag);
   } else if (c->prec == FV_PREC_FP64) {
     hipLaunchKernelGGL((k_recon_fast<NM, false>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
-                       c->fine4, c->SF, c->SFm);
+                       c->fine4, c->lim, c->SF, c->SFm);
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
     hipLaunchKernelGGL((k_flux_fast<NM, false>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
                        c->SFm, c->nbr, c->info, c->fine4, c->nanflag);
@@ -898,7 +884,7 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
       c->cap_m = n;
     }
     hipLaunchKernelGGL((k_recon_fast<NM, true>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
-                       c->fine4, c->SF, c->SFm);
+                       c->fine4, c->lim, c->SF, c->SFm);
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
     hipLaunchKernelGGL((k_flux_fast<NM, true>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
                        c->SFm, c->nbr, c->info, c->fine4, c->nanflag);
@@ -965,7 +951,7 @@ extern "C" int fv_dev_cfl_dt(fv_dev *c, double cfl, double *dt) {
   return FV_OK;
 }
 
-extern "C" int fv_dev_indicator(fv_dev *c, const fv_crit *cr, int *SI, int *split) {
+int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev) {
   if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
   const int n = c->n;
   const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
@@ -993,11 +979,35 @@ extern "C" int fv_dev_indicator(fv_dev *c, const fv_crit *cr, int *SI, int *spli
   else
     hipLaunchKernelGGL(k_split<3>, grid, block, 0, c->stream, n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, c->split);
   HIPCHK(hipGetLastError());
-  if (SI) HIPCHK(hipMemcpyAsync(SI, cur, sizeof(int) * 2 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
-  if (split) HIPCHK(hipMemcpyAsync(split, c->split, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  *SIdev = cur;
+  return FV_OK;
+}
+
+extern "C" int fv_dev_indicator(fv_dev *c, const fv_crit *cr, int *SI, int *split) {
+  if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
+  int *cur = nullptr;
+  int rc = fv_dev_indicator_raw(c, cr, &cur);
+  if (rc != FV_OK) return rc;
+  if (SI) HIPCHK(hipMemcpyAsync(SI, cur, sizeof(int) * 2 * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+  if (split) HIPCHK(hipMemcpyAsync(split, c->split, sizeof(int) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return FV_OK;
 }
+
+extern "C" int fv_dev_set_limiter_flags(fv_dev *c, const uint8_t *lim) {
+  if (!c || !lim) return set_err(FV_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpy(c->lim, lim, (size_t)c->n, hipMemcpyHostToDevice));
+  return FV_OK;
+}
+
+extern "C" int fv_dev_get_limiter_flags(fv_dev *c, uint8_t *lim) {
+  if (!c || !lim) return set_err(FV_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(lim, c->lim, (size_t)c->n, hipMemcpyDeviceToHost));
+  return FV_OK;
+}
+
+
 
 extern "C" int fv_dev_time_stages(fv_dev *c, int nrep, double *ms_stage, double *ms_flux) {
   if (!c || nrep < 1) return set_err(FV_ERR_ARG, "bad argument");
@@ -1061,4 +1071,4 @@ extern "C" int fv_dev_stream_copy(size_t bytes, int reps, double *gbps) {
   return FV_OK;
 }
 
-extern "C" const char *fv_dev_last_error(void) { return g_err; }
+extern "C" const char *fv_dev_last_error(void) { return fv_g_err; }

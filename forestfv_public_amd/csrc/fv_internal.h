@@ -1,0 +1,37 @@
+// fv_internal.h -- the device context shared by the solver and MLP units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "forestfv_dev.h"
+#include "fv_phys.h"
+
+extern thread_local char fv_g_err[256];
+int fv_set_err(int code, const char *msg);
+
+#define HIPCHK(x)                                                                                      \
+  do {                                                                                                 \
+    hipError_t e_ = (x);                                                                               \
+    if (e_ != hipSuccess) {                                                                            \
+      snprintf(fv_g_err, sizeof fv_g_err, "%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+      return FV_ERR_HIP;                                                                               \
+    }                                                                                                  \
+  } while (0)
+
+struct fv_dev {
+  int n = 0, neq = 0, nmat = 0, nfine = 0, prec = FV_PREC_FP64;
+  fvk::Params P{};
+  fvk::Geo G{};
+  hipStream_t stream = nullptr;
+  double *S = nullptr, *SN = nullptr, *SF = nullptr, *part = nullptr, *dtmp = nullptr;
+  float *SFm = nullptr;   // mixed-precision face records, allocated on first use
+  int cap_m = 0;
+  int *nbr = nullptr, *fine4 = nullptr, *nanflag = nullptr, *SIa = nullptr, *SIb = nullptr, *split = nullptr;
+  uint32_t *info = nullptr;
+  uint8_t *lim = nullptr;  // per-leaf limiter (0 minmod, 1 MC) read when limiter == 2
+  int cap_n = 0, cap_fine = 0;
+};
+
+// launches the indicator kernels (interface layers, target level) and leaves
+// SI in *SIdev and the targets in c->split; defined in fv_device.hip
+int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev);

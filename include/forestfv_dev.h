@@ -72,6 +72,11 @@ int  fv_dev_get_faces(fv_dev *ctx, double *SF);
  * SI: [n][2] out, split: [n] out (either may be NULL). */
 int  fv_dev_indicator(fv_dev *ctx, const fv_crit *c, int *SI, int *split);
 
+/* Per-leaf limiter when fv_params.limiter == 2: 0 minmod, 1 MC; written by
+ * fv_dev_indicator_mlp from the learned smooth flag, or by the caller. */
+int  fv_dev_set_limiter_flags(fv_dev *ctx, const uint8_t *lim);
+int  fv_dev_get_limiter_flags(fv_dev *ctx, uint8_t *lim);
+
 /* Timing of the stage kernels with hipEvents on the context's stream:
  * runs nrep full stages (cycling irk) and returns the mean milliseconds per
  * stage and per launch of the dominant (flux) kernel. */
@@ -81,6 +86,35 @@ int  fv_dev_time_stages(fv_dev *ctx, int nrep, double *ms_stage, double *ms_flux
  * between two device buffers `reps` times and returns read+write GB/s (the
  * measured HBM stream bandwidth; one dispatch also calibrates byte counters). */
 int  fv_dev_stream_copy(size_t bytes, int reps, double *gbps);
+
+/* ---- learned refinement / limiter indicator (new; no reference counterpart) ----
+ * A 108 -> 64 -> 64 -> 4 SiLU network on a 3x3x3 stencil of normalised
+ * (rho, p, alpha, |u|); outputs refine / keep / coarsen and a smooth flag.
+ * Trained on the device with FP32 forward, backward and Adam kernels against
+ * the plain indicator's labels; inference is a BF16 MFMA GEMM chain.  The
+ * plain indicator's refinement always wins. */
+typedef struct fv_mlp fv_mlp;
+int  fv_mlp_create(fv_mlp **m, unsigned seed);
+int  fv_mlp_destroy(fv_mlp *m);
+int  fv_mlp_num_weights(void);
+/* weights as [W1 112x64 | b1 64 | W2 64x64 | b2 64 | W3 64x4 | b3 4], row-major [in][out] */
+int  fv_mlp_get_weights(fv_mlp *m, float *w);
+int  fv_mlp_set_weights(fv_mlp *m, const float *w);
+/* class weights of refine/keep/coarsen in the loss; smooth-flag threshold on the
+ * largest relative pressure jump to a face neighbour, default 0.1 (<= 0 keeps the current value) */
+int  fv_mlp_set_options(fv_mlp *m, const float *class_weights, float smooth_thr);
+/* append (features, labels) of every leaf of ctx; labels from the plain indicator */
+int  fv_mlp_sample(fv_dev *ctx, fv_mlp *m, const fv_crit *c);
+int  fv_mlp_dataset(fv_mlp *m, long *n, float *X, int *cls, float *smooth);
+int  fv_mlp_clear_dataset(fv_mlp *m);
+int  fv_mlp_train(fv_mlp *m, int epochs, int batch, float lr, unsigned seed, float *loss);
+/* host-batch helpers: FP32 gradient and loss, FP32 forward, BF16 MFMA forward */
+int  fv_mlp_grad(fv_mlp *m, const float *X, const int *cls, const float *smooth, int B, float *grad, float *loss);
+int  fv_mlp_forward(fv_mlp *m, const float *X, int B, float *out);
+int  fv_mlp_infer(fv_mlp *m, const float *X, int B, float *out);
+/* learned indicator on ctx's state: SI [n][2], split [n], smooth [n] (any may be NULL);
+ * also sets the per-leaf limiter used when fv_params.limiter == 2 */
+int  fv_dev_indicator_mlp(fv_dev *ctx, fv_mlp *m, const fv_crit *c, int *SI, int *split, uint8_t *smooth);
 
 /* Number of leaves and NEQ of the attached mesh. */
 int  fv_dev_size(const fv_dev *ctx, int *n, int *neq);

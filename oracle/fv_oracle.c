@@ -617,8 +617,12 @@ static void muscl_neighbour(const layout_t *L, const orc_params *p, const orc_me
   avf_limit(L, p, X);
 }
 
+static const unsigned char *g_cell_limiter = NULL;
+void orc_set_cell_limiter(const unsigned char *lim) { g_cell_limiter = lim; }
+
 static void faceinterp_leaf(const layout_t *L, const orc_params *p, const orc_mesh *m,
                             const orc_cell *cl, const double *S, int i, double *SFi) {
+  const int limiter = (p->limiter == 2 && g_cell_limiter) ? (int)g_cell_limiter[i] : p->limiter;
   double SLc[ORC_MAXNEQ], Xf[ORC_MAXNEQ], Xb[ORC_MAXNEQ], df[ORC_MAXNEQ], db[ORC_MAXNEQ];
   for (int iv = 0; iv < L->neq; iv++) SLc[iv] = S[(size_t)i * L->neq + iv];
   avf_limit(L, p, SLc);
@@ -637,7 +641,7 @@ static void faceinterp_leaf(const layout_t *L, const orc_params *p, const orc_me
       if (bcf == 0) {
         if (fabs(df[iv]) > 1.0e-10) {
           double dp = cl->dist_cc[f], dm = cl->dist_cc[fo];
-          if (p->limiter == 0) {
+          if (limiter == 0) {
             double product = (-db[iv] / dm) * (df[iv] / dp);
             if (product > 0.0) {
               if (fabs(db[iv] / dm) > fabs(df[iv] / dp)) g = 1.0 * df[iv] / dp;

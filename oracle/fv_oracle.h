@@ -36,7 +36,7 @@ typedef struct {
   int    rk;                   /* 2: RK_STEPS=2 as the reference runs it (cases 1,2)
                                   3: SSP-RK3 (cases 1,2,3)
                                   22: SSP-RK2 / Heun (cases 1,4) -- not a reference table */
-  int    limiter;              /* 0: minmod (reference); 1: monotonised central (new) */
+  int    limiter;              /* 0: minmod (reference); 1: monotonised central; 2: per leaf (new) */
   double dt;                   /* DT */
   int    bc[6];                /* BC code of the -x,+x,-y,+y,-z,+z domain sides */
   int    fix;                  /* corrections of reference defects, ORC_FIX_* bits; 0 = literal */
@@ -99,6 +99,9 @@ int  orc_stage(const orc_mesh *m, const orc_cell *cells, const orc_params *p,
 /* Face states only (m4_faceinterp_V1), SF [n][6][NEQ]. */
 int  orc_faceinterp(const orc_mesh *m, const orc_cell *cells, const orc_params *p,
                     const double *S, double *SF);
+
+/* Per-leaf limiter used when p->limiter == 2 (0 minmod, 1 MC); [n], kept by pointer. */
+void orc_set_cell_limiter(const unsigned char *lim);
 
 /* Pointwise pieces, for closed-form tests. */
 void orc_relax(const orc_params *p, double *vec);                       /* m4_relaxationvec */

@@ -100,6 +100,8 @@ def lib():
         L.orc_criterionsplt.argtypes = [P(orc_mesh), P(orc_params), P(orc_crit), dp, P(C.c_int), P(C.c_int)]
         L.orc_cfl_dt.argtypes = [P(orc_mesh), P(orc_cell), P(orc_params), dp, C.c_double]
         L.orc_cfl_dt.restype = C.c_double
+        L.orc_set_cell_limiter.argtypes = [C.c_void_p]
+        L.orc_set_cell_limiter.restype = None
         _lib = L
     return _lib
 
@@ -205,6 +207,20 @@ class OracleMesh:
         g["Vol"] = np.array([c.Vol for c in self.cells])
         g["xc"] = np.array([[c.xc, c.yc, c.zc] for c in self.cells])
         return g
+
+
+_keep = {}
+
+
+def set_cell_limiter(lim):
+    """Per-leaf limiter flags for params.limiter == 2 (None clears)."""
+    if lim is None:
+        _keep.pop("lim", None)
+        lib().orc_set_cell_limiter(None)
+        return
+    a = np.ascontiguousarray(lim, dtype=np.uint8)
+    _keep["lim"] = a
+    lib().orc_set_cell_limiter(a.ctypes.data)
 
 
 def relax(params, vec):
