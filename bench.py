@@ -40,6 +40,18 @@ def algorithmic_bytes(neq):
     return 24 * neq + 32
 
 
+def traffic_bytes(workload, prec, fused):
+    """HBM bytes per stage measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (calibrated against a copy kernel), from the summaries committed under
+    profiles/; None where no such measurement exists."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as fh:
+            t = json.load(fh)
+        return None if fused else t[workload][prec]["bytes_per_stage"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -114,6 +126,10 @@ def cpu_baseline(seconds=12.0):
     dev.close()
     num = np.abs(Sg - So).sum(0)
     den = np.abs(So).sum(0)
+    # the momentum components share the momentum magnitude as scale: the
+    # transverse momentum of the shock tube is zero up to rounding
+    mscale = np.sqrt((So[:, 2:5] ** 2).sum(1)).sum()
+    den[2:5] = np.maximum(den[2:5], mscale)
     den[den == 0] = 1
     l1 = float((num / den).max())
     return {
@@ -127,11 +143,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1", "u128", "u256", "u320"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "u64", "u128", "u256", "u320"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true")
     ap.add_argument("--layers", type=int, default=64)
     ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "exact"])
+    ap.add_argument("--fused", action="store_true", help="one fused stage kernel without stored face states")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo design is "
@@ -144,7 +161,7 @@ def main():
         cfg = {"workload": "c1_rayleigh_collapse_amr_L5-9", "leaves": m.n, "levels": "5-9 (config 4-8)",
                "domain_m": L_DOMAIN, "bubble_radius_m": R_BUBBLE, "interface_layers": a.layers}
     else:
-        lvl = {"u128": 8, "u256": 9, "u320": None}[a.workload]
+        lvl = {"u64": 7, "u128": 8, "u256": 9, "u320": None}[a.workload]
         if lvl is None:
             p = c1_params()
             m = F.Mesh.uniform((5, 5, 5), (0, 0, 0), L_DOMAIN / 5, 7, list(p.bc))
@@ -159,6 +176,7 @@ def main():
 
     prec = {"fp64": F.PREC_FP64, "mixed": F.PREC_MIXED, "exact": F.PREC_EXACT}[a.prec]
     dev = F.Device(p, m, prec=prec)
+    dev.set_fused(a.fused)
     dev.set_state(S)
     dev.time_stages(max(1, a.warmup))
     t0 = time.perf_counter()
@@ -173,6 +191,10 @@ def main():
         return
 
     n = m.n
+    fused = a.fused and a.prec != "exact"
+    kernel = "k_stage_fused" if fused else {"exact": "k_recon + k_flux_rk", "fp64": "k_recon_fast + k_flux_fast",
+                                             "mixed": "k_recon_fast + k_flux_pipe"}[a.prec]
+    cfg["stage_kernels"] = "fused" if fused else "recon+flux"
     value = n / (ms_stage * 1e-3) / 1e6
     B = algorithmic_bytes(nq)
     achieved = B * n / (ms_stage * 1e-3) / 1e9
@@ -184,8 +206,8 @@ def main():
         "data": "synthetic", "config": cfg,
         "wall_s_timed": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "kernel": "stage = k_recon + k_flux_rk", "algorithmic_bytes_per_leaf": B,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_bytes(cfg["workload"], a.prec, fused),
+                     "kernel": kernel, "algorithmic_bytes_per_leaf": B,
                      "ms_flux_kernel": round(ms_flux, 4), "stream_copy_GBps": round(copy_gbps, 1)},
     }
     if not a.no_cpu_baseline:

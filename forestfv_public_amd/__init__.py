@@ -158,6 +158,7 @@ def lib():
             "fv_dev_set_dt": (C.c_int, [C.c_void_p, C.c_double]),
             "fv_dev_cfl_dt": (C.c_int, [C.c_void_p, C.c_double, P(C.c_double)]),
             "fv_dev_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+            "fv_dev_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
             "fv_dev_indicator": (C.c_int, [C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int)]),
             "fv_dev_time_stages": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
             "fv_dev_size": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
@@ -454,6 +455,9 @@ class Device:
 
     def set_precision(self, prec):
         _check(lib().fv_dev_set_precision(self._h, prec), "fv_dev_set_precision")
+
+    def set_fused(self, on):
+        _check(lib().fv_dev_set_fused(self._h, 1 if on else 0), "fv_dev_set_fused")
 
     def cfl_dt(self, cfl):
         d = C.c_double()

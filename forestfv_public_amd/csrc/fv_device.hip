@@ -486,6 +486,439 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_fast(int n, Params P, Geo G, 
   for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = Sn[v];
 }
 
+
+// k_flux_fast with the face loads software-pipelined: the neighbour index of
+// face f+2 and the stored face states of face f+1 are requested while face f
+// is evaluated, so each face waits on no memory round trip of its own (the
+// 1:4 faces still gather their four states in place).  Same arithmetic and
+// order as k_flux_fast.
+template <int NM, bool MIXED>
+struct RawFace {
+  static constexpr int N = MIXED ? fvf::Rec<NM>::N : Lay<NM>::NEQ;
+  typename std::conditional<MIXED, float, double>::type x[N];
+};
+
+template <int NM, bool MIXED>
+FV_DEV void load_raw(const double *__restrict__ SF, const float *__restrict__ SFm, int n, int cell, int f,
+                     RawFace<NM, MIXED> &r) {
+  constexpr int N = RawFace<NM, MIXED>::N;
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    if constexpr (MIXED) r.x[q] = SFm[((size_t)f * N + q) * n + cell];
+    else r.x[q] = SF[((size_t)f * N + q) * n + cell];
+  }
+}
+
+template <int NM, bool MIXED, typename R>
+FV_DEV void raw_sol(const fvf::Mat<NM, double> &Md, const fvf::Mat<NM, R> &M, double a, const RawFace<NM, MIXED> &r,
+                    fvf::SolF<NM, R> &s) {
+  if constexpr (MIXED) fvf::fill_rec<NM>(M, r.x, s);
+  else fvf::fill_cons<NM, R>(Md, a, r.x, s);
+}
+
+FV_DEV bool one_neighbour(int kind) {
+  return kind == FV_FK_SAME || kind == FV_FK_MIRROR || kind == FV_FK_COARSE;
+}
+
+template <int NM, bool MIXED>
+__global__ __launch_bounds__(FV_BLOCK) void k_flux_pipe(int n, Params P, Geo G, int cs, double *__restrict__ S,
+                                                        double *__restrict__ SN, const double *__restrict__ SF,
+                                                        const float *__restrict__ SFm, const int *__restrict__ nbr,
+                                                        const uint32_t *__restrict__ info, const int *__restrict__ fine4,
+                                                        int *__restrict__ nanflag) {
+  using L = Lay<NM>;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const fvf::Mat<NM, double> Md = fvf::make_mat<NM, double>(P);
+  const fvf::Mat<NM, R> M = fvf::make_mat<NM, R>(P);
+  const double a = P.amin / 100.0;
+  const uint32_t inf = info[i];
+  const int lvl = cell_level(inf);
+  int nbA = nbr[i], nbB = nbr[(size_t)n + i];
+  RawFace<NM, MIXED> ownA, nbrA;
+  load_raw<NM, MIXED>(SF, SFm, n, i, 0, ownA);
+  if (one_neighbour(face_kind(inf, 0))) load_raw<NM, MIXED>(SF, SFm, n, nbA, FCOPP_D[0], nbrA);
+  double Sc[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
+  fvf::SolF<NM, double> sc;
+  fvf::fill_cons<NM, double>(Md, a, Sc, sc);
+  double src_avf[NM], src_st[NM];
+#pragma unroll
+  for (int k = 0; k < L::N3; ++k) src_avf[k] = sc.avf[k];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) src_st[k] = -sc.avf[k] * sc.p[k];
+  double RHS[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) RHS[v] = 0.0;
+  const bool fixnc = (P.fix & 1) != 0;
+#pragma unroll 1
+  for (int f = 0; f < 6; f++) {
+    // requests for the faces ahead
+    int nbC = 0;
+    if (f + 2 < 6) nbC = nbr[(size_t)(f + 2) * n + i];
+    RawFace<NM, MIXED> ownB, nbrB;
+    if (f + 1 < 6) {
+      load_raw<NM, MIXED>(SF, SFm, n, i, f + 1, ownB);
+      if (one_neighbour(face_kind(inf, f + 1))) load_raw<NM, MIXED>(SF, SFm, n, nbB, FCOPP_D[f + 1], nbrB);
+    }
+    // face f
+    const int kind = face_kind(inf, f);
+    const int nb = nbA;
+    const int fo = FCOPP_D[f];
+    double ff[L::NEQ], uf0 = 0.0, uf1 = 0.0, uf2 = 0.0, Area_f;
+    fvf::SolF<NM, R> sL, sR;
+    R fr[9];
+    if (kind == FV_FK_MIRROR) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[fo][q];
+      raw_sol<NM, MIXED, R>(Md, M, a, nbrA, sL);
+      raw_sol<NM, MIXED, R>(Md, M, a, ownA, sR);
+      R out[L::NEQ], un, vn, wn;
+      fvf::hllc<NM, R>(M, P.fix, fr, 0, sL, sR, out, un, vn, wn);
+      const double A = G.area[lvl];
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) ff[v] = -(A * (double)out[v]);
+      const double sgn = fixnc ? 1.0 : -1.0;
+      uf0 = sgn * (double)un; uf1 = sgn * (double)vn; uf2 = sgn * (double)wn;
+      Area_f = A;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[f][q];
+      const int Nneig = (kind == FV_FK_FINE) ? 4 : 1;
+      const double A = (kind == FV_FK_FINE) ? G.area[lvl + 1] : G.area[lvl];
+      Area_f = (double)Nneig * A;
+      const double wn_ = (kind == FV_FK_FINE) ? 0.25 : 1.0;
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) ff[v] = 0.0;
+      raw_sol<NM, MIXED, R>(Md, M, a, ownA, sL);
+      for (int k = 0; k < Nneig; ++k) {
+        int bcode = 0;
+        if (kind == FV_FK_BOUNDARY) {
+          sR = sL;
+          bcode = -nb;
+          if (bcode == 1 || bcode == 4 || bcode == 5) {
+            const R ul = sL.u * fr[0] + sL.v * fr[1] + sL.w * fr[2];
+            const R vl = sL.u * fr[3] + sL.v * fr[4] + sL.w * fr[5];
+            const R wl = sL.u * fr[6] + sL.v * fr[7] + sL.w * fr[8];
+            const R vr = (bcode == 5) ? -vl : vl, wr = (bcode == 5) ? -wl : wl;
+            sR.u = -ul * fr[0] + vr * fr[3] + wr * fr[6];
+            sR.v = -ul * fr[1] + vr * fr[4] + wr * fr[7];
+            sR.w = -ul * fr[2] + vr * fr[5] + wr * fr[8];
+          }
+        } else if (kind == FV_FK_FINE) {
+          load_sol<NM, MIXED, R>(Md, M, a, SF, SFm, n, fine4[4 * nb + k], fo, sR);
+        } else {
+          raw_sol<NM, MIXED, R>(Md, M, a, nbrA, sR);
+        }
+        R out[L::NEQ], un, vn, wn;
+        fvf::hllc<NM, R>(M, P.fix, fr, bcode, sL, sR, out, un, vn, wn);
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) ff[v] += A * (double)out[v];
+        uf0 += wn_ * (double)un;
+        uf1 += wn_ * (double)vn;
+        uf2 += wn_ * (double)wn;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) RHS[v] += ff[v];
+    const double nx = G.frame[f][0], ny = G.frame[f][1], nz = G.frame[f][2];
+    const double und = uf0 * nx + uf1 * ny + uf2 * nz;
+#pragma unroll
+    for (int k = 0; k < L::N3; ++k) {
+      const double tt = Area_f * src_avf[k] * und;
+      RHS[L::I3 + k] = fixnc ? RHS[L::I3 + k] - tt : RHS[L::I3 + k] + tt;
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      const double tt = Area_f * src_st[k] * und;
+      RHS[L::I4 + k] = fixnc ? RHS[L::I4 + k] - tt : RHS[L::I4 + k] + tt;
+    }
+    // rotate the pipeline
+    ownA = ownB;
+    nbrA = nbrB;
+    nbA = nbB;
+    nbB = nbC;
+  }
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) bad |= isnan(RHS[v]);
+  if (bad) *nanflag = 1;
+  const double dtv = P.dt * fvf::rcp(G.vol[lvl]);
+  const double wa = (cs == 1) ? 0.0 : (cs == 2 ? 0.75 : (cs == 3 ? 1.0 / 3.0 : 0.5));
+  const double wb = 1.0 - wa;
+  double Sn[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) {
+    double sn;
+    if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
+    else sn = SN[(size_t)v * n + i];
+    Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
+  }
+  fvf::relax_fast<NM>(P, Md, Sn);
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = Sn[v];
+}
+
+// ---------------------------------------------------------------------
+// fused stage (plain FP64 and mixed precision): one leaf per thread, no
+// face-state array.  A leaf reconstructs its own state at each face and the
+// neighbour's state at the shared face from the cell states (the
+// neighbour's two stencil values along the face normal are the leaf's own
+// state and the second neighbour), so HBM traffic per stage is the state
+// read, SN and the new state; neighbour and second-neighbour states come
+// from the caches.  Faces are taken in order 0..5 and every face state is
+// computed by the same device function as in k_recon_fast, so the result
+// equals the two-kernel path.  S_in and S_out are different buffers.
+// Measured on the 2.2 M-leaf bubble mesh it is slower than the two-kernel
+// path (2.4 against 1.0 ms per FP64 stage): the stage is bound by VALU work
+// and latency, not HBM bytes, and the fused form doubles the reconstruction
+// work at 2 waves per SIMD (a launch bound that spills about 40 VGPRs).  It
+// is kept, off by default, as the no-face-state variant.
+// ---------------------------------------------------------------------
+
+// k_recon_fast's face-state body: Sc avf-limited cell state, Xg / Xo the
+// stencil values across face g and across its opposite face
+template <int NM, bool MIXED>
+FV_DEV void recon_one(const Params &P, const fvf::Mat<NM, double> &M, const Geo &G, double a, int limiter, int lvl,
+                      int kg, int ko, const double *Sc, const double *Xg, const double *Xo, double *F) {
+  using L = Lay<NM>;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  double dcf_g, dcc_g, dcf_o, dcc_o;
+  dist_of(G, kg, lvl, dcf_g, dcc_g);
+  dist_of(G, ko, lvl, dcf_o, dcc_o);
+  const bool bnd = (kg == FV_FK_BOUNDARY) || (ko == FV_FK_BOUNDARY);
+  const R idp = bnd ? (R)0 : (R)fvf::rcp(dcc_g), idm = bnd ? (R)0 : (R)fvf::rcp(dcc_o);
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) {
+    const R g = bnd ? (R)0 : fvf::slope<R>(limiter, Xg[v] - Sc[v], Xo[v] - Sc[v], idp, idm);
+    F[v] = Sc[v] + dcf_g * (double)g;
+  }
+  vf_limit_face<NM, double>(a, F);
+  fvf::relax_fast<NM>(P, M, F);
+}
+
+// face state -> solution used by HLLC (through the FP32 record when MIXED)
+template <int NM, bool MIXED, typename R>
+FV_DEV void state_sol(const fvf::Mat<NM, double> &Md, const fvf::Mat<NM, R> &M, double a, const double *F,
+                      fvf::SolF<NM, R> &s) {
+  if constexpr (MIXED) {
+    float rec[fvf::Rec<NM>::N];
+    fvf::to_record<NM>(Md, a, F, rec);
+    fvf::fill_rec<NM>(M, rec, s);
+  } else {
+    fvf::fill_cons<NM, R>(Md, a, F, s);
+  }
+}
+
+FV_DEV int leaf_limiter(const Params &P, const uint8_t *__restrict__ lim, int j) {
+  return (P.limiter == 2) ? (int)lim[j] : P.limiter;
+}
+
+// reconstructed state of any leaf j at its face g, from the cell states
+template <int NM, bool MIXED>
+FV_DEV void face_state_of(const Params &P, const fvf::Mat<NM, double> &M, const Geo &G, double a,
+                          const double *__restrict__ S, int n, const int *__restrict__ nbr,
+                          const uint32_t *__restrict__ info, const int *__restrict__ fine4,
+                          const uint8_t *__restrict__ lim, int j, int g, double *F) {
+  using L = Lay<NM>;
+  double Sc[L::NEQ], Araw[NM], Xg[L::NEQ], Xo[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + j];
+#pragma unroll
+  for (int k = 0; k < L::N3; k++) Araw[k] = Sc[L::I3 + k];
+  avf_limit<NM, double>(a, Sc);
+  const uint32_t inf = info[j];
+  const int go = FCOPP_D[g];
+  const int kg = face_kind(inf, g), ko = face_kind(inf, go);
+  stencil_value<NM>(S, n, j, kg, nbr[(size_t)g * n + j], fine4, Sc, Araw, a, Xg);
+  stencil_value<NM>(S, n, j, ko, nbr[(size_t)go * n + j], fine4, Sc, Araw, a, Xo);
+  recon_one<NM, MIXED>(P, M, G, a, leaf_limiter(P, lim, j), cell_level(inf), kg, ko, Sc, Xg, Xo, F);
+}
+
+// blocks of one XCD (workgroups are dealt to the 8 XCDs round robin) take a
+// contiguous range of leaves, so face neighbours share an L2
+FV_DEV int xcd_leaf(void) {
+  const unsigned per = gridDim.x >> 3;
+  const unsigned b = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  return (int)(b * blockDim.x + threadIdx.x);
+}
+
+template <int NM, bool MIXED>
+__global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, const Geo *__restrict__ Gp, int cs,
+                                                          const double *__restrict__ S, double *__restrict__ Sout,
+                                                          double *__restrict__ SN, const int *__restrict__ nbr,
+                                                          const uint32_t *__restrict__ info,
+                                                          const int *__restrict__ fine4,
+                                                          const uint8_t *__restrict__ lim, int *__restrict__ nanflag) {
+  using L = Lay<NM>;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const int i = xcd_leaf();
+  if (i >= n) return;
+  // the per-level tables are read through a pointer: a by-value copy indexed
+  // by the per-leaf level would be placed in scratch memory
+  const Geo &G = *Gp;
+  const fvf::Mat<NM, double> Md = fvf::make_mat<NM, double>(P);
+  const fvf::Mat<NM, R> M = fvf::make_mat<NM, R>(P);
+  const double a = P.amin / 100.0;
+  const uint32_t inf = info[i];
+  const int lvl = cell_level(inf);
+  const int limiter = leaf_limiter(P, lim, i);
+  double Sc[L::NEQ], Araw[NM];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
+#pragma unroll
+  for (int k = 0; k < L::N3; k++) Araw[k] = Sc[L::I3 + k];
+  double src_avf[NM], src_st[NM];
+  {
+    fvf::SolF<NM, double> sc;
+    fvf::fill_cons<NM, double>(Md, a, Sc, sc);
+#pragma unroll
+    for (int k = 0; k < L::N3; ++k) src_avf[k] = sc.avf[k];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) src_st[k] = -sc.avf[k] * sc.p[k];
+  }
+  avf_limit<NM, double>(a, Sc);
+  double RHS[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) RHS[v] = 0.0;
+  const bool fixnc = (P.fix & 1) != 0;
+#pragma unroll 1
+  for (int f = 0; f < 6; f++) {
+    const int fo = FCOPP_D[f];
+    const int kind = face_kind(inf, f), kopp = face_kind(inf, fo);
+    const int nb = nbr[(size_t)f * n + i];
+    double Xf[L::NEQ];
+    stencil_value<NM>(S, n, i, kind, nb, fine4, Sc, Araw, a, Xf);
+    double ff[L::NEQ], uf0 = 0.0, uf1 = 0.0, uf2 = 0.0, Area_f;
+    R fr[9];
+    if (kind == FV_FK_SAME || kind == FV_FK_MIRROR) {
+      // same-level neighbour j: its cell state is Xf, its stencil value
+      // towards this leaf is Sc, the other one is the second neighbour
+      fvf::SolF<NM, R> sA, sB;
+      {
+        const uint32_t infj = info[nb];
+        const int kj = face_kind(infj, f), kjo = face_kind(infj, fo);
+        double X2[L::NEQ], Arj[NM] = {}, Fnb[L::NEQ];
+        if (kj == FV_FK_BOUNDARY) {
+#pragma unroll
+          for (int k = 0; k < L::N3; k++) Arj[k] = S[(size_t)(L::I3 + k) * n + nb];
+        }
+        stencil_value<NM>(S, n, nb, kj, nbr[(size_t)f * n + nb], fine4, Xf, Arj, a, X2);
+        recon_one<NM, MIXED>(P, Md, G, a, leaf_limiter(P, lim, nb), lvl, kjo, kj, Xf, Sc, X2, Fnb);
+        state_sol<NM, MIXED, R>(Md, M, a, Fnb, sA);
+      }
+      {
+        double Xo[L::NEQ], Fown[L::NEQ];
+        stencil_value<NM>(S, n, i, kopp, nbr[(size_t)fo * n + i], fine4, Sc, Araw, a, Xo);
+        recon_one<NM, MIXED>(P, Md, G, a, limiter, lvl, kind, kopp, Sc, Xf, Xo, Fown);
+        state_sol<NM, MIXED, R>(Md, M, a, Fown, sB);
+      }
+      R out[L::NEQ], un, vn, wn;
+      const double A = G.area[lvl];
+      Area_f = A;
+      if (kind == FV_FK_MIRROR) {
+        // evaluated exactly as the neighbour evaluates it, then negated
+#pragma unroll
+        for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[fo][q];
+        fvf::hllc<NM, R>(M, P.fix, fr, 0, sA, sB, out, un, vn, wn);
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) ff[v] = -(A * (double)out[v]);
+        const double sgn = fixnc ? 1.0 : -1.0;
+        uf0 = sgn * (double)un; uf1 = sgn * (double)vn; uf2 = sgn * (double)wn;
+      } else {
+        // same accumulation form as the one-neighbour case of k_flux_fast
+#pragma unroll
+        for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[f][q];
+        fvf::hllc<NM, R>(M, P.fix, fr, 0, sB, sA, out, un, vn, wn);
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) { ff[v] = 0.0; ff[v] += A * (double)out[v]; }
+        uf0 += 1.0 * (double)un;
+        uf1 += 1.0 * (double)vn;
+        uf2 += 1.0 * (double)wn;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[f][q];
+      const int Nneig = (kind == FV_FK_FINE) ? 4 : 1;
+      const double A = (kind == FV_FK_FINE) ? G.area[lvl + 1] : G.area[lvl];
+      Area_f = (double)Nneig * A;
+      const double wn_ = (kind == FV_FK_FINE) ? 0.25 : 1.0;
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) ff[v] = 0.0;
+      fvf::SolF<NM, R> sL, sR;
+      {
+        double Xo[L::NEQ], Fown[L::NEQ];
+        stencil_value<NM>(S, n, i, kopp, nbr[(size_t)fo * n + i], fine4, Sc, Araw, a, Xo);
+        recon_one<NM, MIXED>(P, Md, G, a, limiter, lvl, kind, kopp, Sc, Xf, Xo, Fown);
+        state_sol<NM, MIXED, R>(Md, M, a, Fown, sL);
+      }
+      for (int k = 0; k < Nneig; ++k) {
+        int bcode = 0;
+        if (kind == FV_FK_BOUNDARY) {
+          sR = sL;
+          bcode = -nb;
+          if (bcode == 1 || bcode == 4 || bcode == 5) {
+            const R ul = sL.u * fr[0] + sL.v * fr[1] + sL.w * fr[2];
+            const R vl = sL.u * fr[3] + sL.v * fr[4] + sL.w * fr[5];
+            const R wl = sL.u * fr[6] + sL.v * fr[7] + sL.w * fr[8];
+            const R vr = (bcode == 5) ? -vl : vl, wr = (bcode == 5) ? -wl : wl;
+            sR.u = -ul * fr[0] + vr * fr[3] + wr * fr[6];
+            sR.v = -ul * fr[1] + vr * fr[4] + wr * fr[7];
+            sR.w = -ul * fr[2] + vr * fr[5] + wr * fr[8];
+          }
+        } else {
+          const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
+          double Fnb[L::NEQ];
+          face_state_of<NM, MIXED>(P, Md, G, a, S, n, nbr, info, fine4, lim, j, fo, Fnb);
+          state_sol<NM, MIXED, R>(Md, M, a, Fnb, sR);
+        }
+        R out[L::NEQ], un, vn, wn;
+        fvf::hllc<NM, R>(M, P.fix, fr, bcode, sL, sR, out, un, vn, wn);
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) ff[v] += A * (double)out[v];
+        uf0 += wn_ * (double)un;
+        uf1 += wn_ * (double)vn;
+        uf2 += wn_ * (double)wn;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) RHS[v] += ff[v];
+    const double nx = G.frame[f][0], ny = G.frame[f][1], nz = G.frame[f][2];
+    const double und = uf0 * nx + uf1 * ny + uf2 * nz;
+#pragma unroll
+    for (int k = 0; k < L::N3; ++k) {
+      const double tt = Area_f * src_avf[k] * und;
+      RHS[L::I3 + k] = fixnc ? RHS[L::I3 + k] - tt : RHS[L::I3 + k] + tt;
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      const double tt = Area_f * src_st[k] * und;
+      RHS[L::I4 + k] = fixnc ? RHS[L::I4 + k] - tt : RHS[L::I4 + k] + tt;
+    }
+  }
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) bad |= isnan(RHS[v]);
+  if (bad) *nanflag = 1;
+  const double dtv = P.dt * fvf::rcp(G.vol[lvl]);
+  const double wa = (cs == 1) ? 0.0 : (cs == 2 ? 0.75 : (cs == 3 ? 1.0 / 3.0 : 0.5));
+  const double wb = 1.0 - wa;
+  double Sn[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) {
+    const double s0 = S[(size_t)v * n + i];
+    double sn;
+    if (cs == 1) { sn = s0; SN[(size_t)v * n + i] = sn; }
+    else sn = SN[(size_t)v * n + i];
+    Sn[v] = wa * sn + wb * (s0 - dtv * RHS[v]);
+  }
+  fvf::relax_fast<NM>(P, Md, Sn);
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sout[(size_t)v * n + i] = Sn[v];
+}
+
 #pragma clang fp contract(off)
 
 // CFL: per-block minimum of h / (max|u_i| + c) with a wavefront shuffle reduction
@@ -699,13 +1132,13 @@ static void to_dev_geo(const fv_geo *g, Geo &G) {
 }
 
 static void free_buffers(fv_dev *c) {
-  hipFree(c->S); hipFree(c->SN); hipFree(c->SF); hipFree(c->part); hipFree(c->dtmp); hipFree(c->SFm);
+  hipFree(c->S); hipFree(c->S2); hipFree(c->SN); hipFree(c->SF); hipFree(c->part); hipFree(c->dtmp); hipFree(c->SFm);
   c->SFm = nullptr;
   c->cap_m = 0;
   hipFree(c->nbr); hipFree(c->fine4); hipFree(c->nanflag); hipFree(c->info);
   hipFree(c->SIa); hipFree(c->SIb); hipFree(c->split); hipFree(c->lim);
   c->lim = nullptr;
-  c->S = c->SN = c->SF = c->part = c->dtmp = nullptr;
+  c->S = c->S2 = c->SN = c->SF = c->part = c->dtmp = nullptr;
   c->nbr = c->fine4 = c->nanflag = c->SIa = c->SIb = c->split = nullptr;
   c->info = nullptr;
   c->cap_n = c->cap_fine = 0;
@@ -719,6 +1152,7 @@ static int upload_mesh(fv_dev *c, const fv_pack *pk) {
     const size_t nf = (size_t)(pk->nfine > 0 ? pk->nfine : 1);
     const size_t nb = (n + FV_BLOCK - 1) / FV_BLOCK;
     HIPCHK(hipMalloc(&c->S, sizeof(double) * n * c->neq));
+    HIPCHK(hipMalloc(&c->S2, sizeof(double) * n * c->neq));
     HIPCHK(hipMalloc(&c->SN, sizeof(double) * n * c->neq));
     HIPCHK(hipMalloc(&c->SF, sizeof(double) * n * 6 * c->neq));
     HIPCHK(hipMalloc(&c->part, sizeof(double) * (nb + 1)));
@@ -737,6 +1171,8 @@ static int upload_mesh(fv_dev *c, const fv_pack *pk) {
   c->n = pk->n;
   c->nfine = pk->nfine;
   to_dev_geo(&pk->geo, c->G);
+  if (!c->Gdev) HIPCHK(hipMalloc(&c->Gdev, sizeof(Geo)));
+  HIPCHK(hipMemcpyAsync(c->Gdev, &c->G, sizeof(Geo), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->nbr, pk->nbr, sizeof(int) * n * 6, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->info, pk->info, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
   if (pk->nfine > 0)
@@ -764,7 +1200,7 @@ extern "C" int fv_dev_attach(fv_dev **out, const fv_params *p, const fv_pack *pk
     return set_err(FV_ERR_HIP, "stream creation failed");
   }
   int rc = upload_mesh(c, pk);
-  if (rc != FV_OK) { free_buffers(c); hipStreamDestroy(c->stream); delete c; return rc; }
+  if (rc != FV_OK) { free_buffers(c); hipFree(c->Gdev); hipStreamDestroy(c->stream); delete c; return rc; }
   *out = c;
   return FV_OK;
 }
@@ -778,6 +1214,7 @@ extern "C" int fv_dev_detach(fv_dev *c) {
   if (!c) return FV_OK;
   hipStreamSynchronize(c->stream);
   free_buffers(c);
+  hipFree(c->Gdev);
   hipStreamDestroy(c->stream);
   delete c;
   return FV_OK;
@@ -848,6 +1285,12 @@ extern "C" int fv_dev_set_precision(fv_dev *c, int prec) {
   return FV_OK;
 }
 
+extern "C" int fv_dev_set_fused(fv_dev *c, int on) {
+  if (!c) return set_err(FV_ERR_ARG, "null context");
+  c->fused = on ? 1 : 0;
+  return FV_OK;
+}
+
 extern "C" int fv_dev_begin_step(fv_dev *c) {
   if (!c) return set_err(FV_ERR_ARG, "null context");
   HIPCHK(hipMemcpyAsync(c->SN, c->S, sizeof(double) * (size_t)c->n * c->neq, hipMemcpyDeviceToDevice, c->stream));
@@ -869,6 +1312,19 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
     hipLaunchKernelGGL(k_flux_rk<NM>, grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF, c->nbr,
                        c->info, c->fine4, c->nanflag);
+  } else if (c->fused) {
+    const unsigned nb = (unsigned)((n + FV_BLOCK - 1) / FV_BLOCK);
+    const dim3 gx((nb + 7u) & ~7u);   // a multiple of 8 for the XCD-contiguous leaf mapping
+    if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
+    if (c->prec == FV_PREC_FP64)
+      hipLaunchKernelGGL((k_stage_fused<NM, false>), gx, block, 0, c->stream, n, c->P, c->Gdev, cs, c->S, c->S2, c->SN,
+                         c->nbr, c->info, c->fine4, c->lim, c->nanflag);
+    else
+      hipLaunchKernelGGL((k_stage_fused<NM, true>), gx, block, 0, c->stream, n, c->P, c->Gdev, cs, c->S, c->S2, c->SN,
+                         c->nbr, c->info, c->fine4, c->lim, c->nanflag);
+    double *t = c->S;
+    c->S = c->S2;
+    c->S2 = t;
   } else if (c->prec == FV_PREC_FP64) {
     hipLaunchKernelGGL((k_recon_fast<NM, false>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
                        c->fine4, c->lim, c->SF, c->SFm);
@@ -886,7 +1342,9 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
     hipLaunchKernelGGL((k_recon_fast<NM, true>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
                        c->fine4, c->lim, c->SF, c->SFm);
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
-    hipLaunchKernelGGL((k_flux_fast<NM, true>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
+    // the pipelined flux needs more registers: 2 waves per SIMD with FP32 face
+    // records (8 % faster), 1 with FP64 ones (35 % slower), so FP64 keeps k_flux_fast
+    hipLaunchKernelGGL((k_flux_pipe<NM, true>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
                        c->SFm, c->nbr, c->info, c->fine4, c->nanflag);
   }
   return FV_OK;

@@ -20,10 +20,12 @@ int fv_set_err(int code, const char *msg);
 
 struct fv_dev {
   int n = 0, neq = 0, nmat = 0, nfine = 0, prec = FV_PREC_FP64;
+  int fused = 0;          // FP64 / MIXED stages: fused kernel (1) or recon + flux with stored face states (0)
   fvk::Params P{};
   fvk::Geo G{};
+  fvk::Geo *Gdev = nullptr;   // device copy of G for kernels that index it per leaf
   hipStream_t stream = nullptr;
-  double *S = nullptr, *SN = nullptr, *SF = nullptr, *part = nullptr, *dtmp = nullptr;
+  double *S = nullptr, *S2 = nullptr, *SN = nullptr, *SF = nullptr, *part = nullptr, *dtmp = nullptr;
   float *SFm = nullptr;   // mixed-precision face records, allocated on first use
   int cap_m = 0;
   int *nbr = nullptr, *fine4 = nullptr, *nanflag = nullptr, *SIa = nullptr, *SIb = nullptr, *split = nullptr;

@@ -61,10 +61,17 @@ int  fv_dev_stage(fv_dev *ctx, int irk);
 int  fv_dev_set_dt(fv_dev *ctx, double dt);
 /* Largest stable step cfl * min h / (max|u_i| + c) (new: no CFL in the reference). */
 int  fv_dev_cfl_dt(fv_dev *ctx, double cfl, double *dt);
-/* FV_PREC_FP64 (default) or FV_PREC_MIXED. */
+/* FV_PREC_FP64 (default), FV_PREC_MIXED or FV_PREC_EXACT. */
 int  fv_dev_set_precision(fv_dev *ctx, int prec);
+/* FV_PREC_FP64 / FV_PREC_MIXED stages: 0 (default) reconstruction and flux
+ * kernels with the face states stored in between; 1 one fused kernel per stage
+ * that keeps no face states and recomputes each neighbour's face state.  Both
+ * give the same state up to FMA contraction.  FV_PREC_EXACT always stores the
+ * face states. */
+int  fv_dev_set_fused(fv_dev *ctx, int on);
 
-/* Face states of the last stage, SF as [n][6][NEQ] (LEAF.SF). */
+/* Face states of the last stage, SF as [n][6][NEQ] (LEAF.SF); written by the
+ * stages of FV_PREC_EXACT and of the unfused throughput path. */
 int  fv_dev_get_faces(fv_dev *ctx, double *SF);
 
 /* interface_location (interface_location.c:7-86) and the per-leaf target
