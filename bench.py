@@ -12,6 +12,7 @@ Prints one JSON line (metric of the contract) and, with --profile-only,
 nothing but runs the timed loop (for rocprofv3).
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -138,6 +139,74 @@ def cpu_baseline(seconds=12.0):
     }, l1, stages
 
 
+def mixture_pressure(p, S):
+    """Sum_k alpha_k p_k = Sum_k (gamma_k - 1) (alpha rho e)_k - alpha_k gamma_k pinf_k (two materials)."""
+    a0 = S[:, 6]
+    g, pi = p.gamma, p.pinf
+    return ((g[0] - 1.0) * S[:, 7] - a0 * g[0] * pi[0]) + ((g[1] - 1.0) * S[:, 8] - (1.0 - a0) * g[1] * pi[1])
+
+
+def rel_l1(a, b):
+    return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+
+
+def fp32_mlp_leg(p, m, S, crit, cycles=4, every=5, log_fn=log):
+    """FP32 faces + learned indicator: the network is trained on the device from
+    the plain indicator's labels on this state, then every `every` steps it
+    proposes targets and the per-leaf limiter (smooth -> MC); the stages run in
+    mixed precision.  Returns cell-updates/s with the indicator's cost included,
+    and the relative L1 in density and pressure against the FP64 path driven the
+    same way."""
+    nstage = 3
+    t0 = time.perf_counter()
+    mlp = F.MLP(seed=7)
+    dev = F.Device(p, m, prec=F.PREC_FP64)
+    dev.set_state(S)
+    mlp.sample(dev, crit)
+    dev.close()
+    loss = mlp.train(2, batch=4096, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    log_fn(f"  learned indicator: {m.n} samples, 2 epochs, loss {loss:.4f}, {t_train:.1f} s")
+    q = F.make_params()
+    C.memmove(C.addressof(q), C.addressof(p), C.sizeof(p))
+    q.limiter = 2
+    out = {}
+    # plain: FP64 stages, plain indicator (minmod); learned: mixed stages, network
+    # indicator (per-leaf limiter); "fp64_learned" runs FP64 stages with the
+    # network and is the reference for the precision error
+    for leg, prec, par, learned in (("plain", F.PREC_FP64, p, False), ("learned", F.PREC_MIXED, q, True),
+                                    ("fp64_learned", F.PREC_FP64, q, True)):
+        dev = F.Device(par, m, prec=prec)
+        SI = np.zeros((m.n, 2), np.int32)
+        split = np.zeros(m.n, np.int32)
+        for rep in range(2):               # the first pass warms up
+            dev.set_state(S)
+            t1 = time.perf_counter()
+            for _ in range(cycles if rep else 1):
+                if learned:
+                    SI, split, _ = dev.indicator_mlp(mlp, crit, smooth=False)
+                else:
+                    SI, split = dev.indicator(crit)
+                dev.advance(every)
+            t = time.perf_counter() - t1
+        out[leg] = (dev.get_state(), t)
+        dev.close()
+    mlp.close()
+    stages = cycles * every * nstage
+    rate = {k: m.n * stages / v[1] / 1e6 for k, v in out.items()}
+    Sm, Sd = out["learned"][0], out["fp64_learned"][0]
+    return {
+        "value": round(rate["learned"], 2), "unit": "Mcell-updates/s per RK stage",
+        "plain_same_driver": round(rate["plain"], 2),
+        "speedup_vs_plain_same_driver": round(rate["learned"] / rate["plain"], 3),
+        "stages": stages, "indicator_every_steps": every,
+        "limiter": "per leaf (learned smooth flag: MC, else minmod)",
+        "l1_rel_rho_vs_fp64": rel_l1(Sm[:, 0] + Sm[:, 1], Sd[:, 0] + Sd[:, 1]),
+        "l1_rel_p_vs_fp64": rel_l1(mixture_pressure(q, Sm), mixture_pressure(q, Sd)),
+        "train_s": round(t_train, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +218,7 @@ def main():
     ap.add_argument("--layers", type=int, default=64)
     ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "exact"])
     ap.add_argument("--fused", action="store_true", help="one fused stage kernel without stored face states")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the FP32 + learned-indicator leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo design is "
@@ -210,6 +280,10 @@ def main():
                      "kernel": kernel, "algorithmic_bytes_per_leaf": B,
                      "ms_flux_kernel": round(ms_flux, 4), "stream_copy_GBps": round(copy_gbps, 1)},
     }
+    if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_mlp:
+        leg = fp32_mlp_leg(p, m, S, c1_crit(9, a.layers))
+        leg["speedup_vs_plain_stage"] = round(leg["value"] / value, 3)
+        out["fp32_mlp"] = leg
     if not a.no_cpu_baseline:
         cb, l1, stages = cpu_baseline()
         out["cpu_baseline"] = cb

@@ -480,12 +480,15 @@ class Device:
         _check(lib().fv_dev_get_limiter_flags(self._h, lim.ctypes.data_as(C.POINTER(C.c_uint8))), "limiter flags")
         return lim
 
-    def indicator_mlp(self, mlp, crit):
+    def indicator_mlp(self, mlp, crit, smooth=True):
+        """Learned indicator: (SI, split, smooth flags); smooth=False leaves the flags
+        on the device only (they are the per-leaf limiter either way)."""
         SI = np.zeros((self.n, 2), dtype=np.int32)
         split = np.zeros(self.n, dtype=np.int32)
-        sm = np.zeros(self.n, dtype=np.uint8)
+        sm = np.zeros(self.n, dtype=np.uint8) if smooth else None
         _check(lib().fv_dev_indicator_mlp(self._h, mlp._h, C.byref(crit), _ip(SI), _ip(split),
-                                          sm.ctypes.data_as(C.POINTER(C.c_uint8))), "fv_dev_indicator_mlp")
+                                          None if sm is None else sm.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "fv_dev_indicator_mlp")
         return SI, split, sm
 
     def time_stages(self, nrep):

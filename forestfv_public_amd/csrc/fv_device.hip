@@ -1215,6 +1215,7 @@ extern "C" int fv_dev_detach(fv_dev *c) {
   hipStreamSynchronize(c->stream);
   free_buffers(c);
   hipFree(c->Gdev);
+  if (c->pinned) hipHostFree(c->pinned);
   hipStreamDestroy(c->stream);
   delete c;
   return FV_OK;
@@ -1441,13 +1442,28 @@ int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev) {
   return FV_OK;
 }
 
+int fv_copy_to_host(fv_dev *c, void *dst, const void *src, size_t bytes) {
+  const size_t chunk = (size_t)32 << 20;
+  if (!c->pinned) {
+    HIPCHK(hipHostMalloc(&c->pinned, chunk, hipHostMallocDefault));
+    c->pinned_cap = chunk;
+  }
+  for (size_t off = 0; off < bytes; off += chunk) {
+    const size_t b = (bytes - off < chunk) ? bytes - off : chunk;
+    HIPCHK(hipMemcpyAsync(c->pinned, (const char *)src + off, b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    memcpy((char *)dst + off, c->pinned, b);
+  }
+  return FV_OK;
+}
+
 extern "C" int fv_dev_indicator(fv_dev *c, const fv_crit *cr, int *SI, int *split) {
   if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
   int *cur = nullptr;
   int rc = fv_dev_indicator_raw(c, cr, &cur);
   if (rc != FV_OK) return rc;
-  if (SI) HIPCHK(hipMemcpyAsync(SI, cur, sizeof(int) * 2 * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
-  if (split) HIPCHK(hipMemcpyAsync(split, c->split, sizeof(int) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+  if (SI && (rc = fv_copy_to_host(c, SI, cur, sizeof(int) * 2 * (size_t)c->n)) != FV_OK) return rc;
+  if (split && (rc = fv_copy_to_host(c, split, c->split, sizeof(int) * (size_t)c->n)) != FV_OK) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return FV_OK;
 }

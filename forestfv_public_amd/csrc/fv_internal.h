@@ -32,7 +32,12 @@ struct fv_dev {
   uint32_t *info = nullptr;
   uint8_t *lim = nullptr;  // per-leaf limiter (0 minmod, 1 MC) read when limiter == 2
   int cap_n = 0, cap_fine = 0;
+  void *pinned = nullptr;  // host staging for device -> host copies of per-leaf outputs
+  size_t pinned_cap = 0;
 };
+
+// device -> host copy through the context's pinned staging buffer (on its stream)
+int fv_copy_to_host(fv_dev *c, void *dst, const void *src, size_t bytes);
 
 // launches the indicator kernels (interface layers, target level) and leaves
 // SI in *SIdev and the targets in c->split; defined in fv_device.hip

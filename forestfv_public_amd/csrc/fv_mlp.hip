@@ -53,8 +53,12 @@ struct fv_mlp {
   float *bx = nullptr, *bs = nullptr, *z1 = nullptr, *h1 = nullptr, *z2 = nullptr, *h2 = nullptr, *o = nullptr;
   float *d3 = nullptr, *d2 = nullptr, *d1 = nullptr, *lossv = nullptr;
   int *bc = nullptr, *bidx = nullptr;
+  // per-leaf stencil fields
+  long fcap = 0;
+  float4 *fld = nullptr;
+  float *snd = nullptr;
   // inference scratch
-  long icap = 0;
+  long icap = 0, xcap = 0;
   float *ix = nullptr, *iout = nullptr;
   uint8_t *ismooth = nullptr;
   float cw[3] = {1.f, 1.f, 1.f};
@@ -95,15 +99,27 @@ FV_DEV int step_face(int j, int f, int n, const int *__restrict__ nbr, const uin
   return nb;
 }
 
-FV_DEV float clampf(double x) { return (float)fmin(8.0, fmax(-8.0, x)); }
+FV_DEV float clampf(float x) { return fminf(8.f, fmaxf(-8.f, x)); }
 
+// per-leaf fields for the stencil features: (rho, p, alpha_0, |u|) and c
 template <int NM>
-__global__ __launch_bounds__(256) void k_mlp_features(int n, Params P, const double *__restrict__ S,
-                                                      const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+__global__ __launch_bounds__(256) void k_mlp_fields(int n, Params P, const double *__restrict__ S,
+                                                    float4 *__restrict__ fld, float *__restrict__ snd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CellF c = cell_fields<NM>(P, S, n, i);
+  fld[i] = make_float4((float)c.rho, (float)c.p, (float)c.alpha, (float)c.umag);
+  snd[i] = (float)c.c;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_features(int n, const float4 *__restrict__ fld,
+                                                      const float *__restrict__ snd, const int *__restrict__ nbr,
+                                                      const uint32_t *__restrict__ info,
                                                       const int *__restrict__ fine4, float *__restrict__ X, long row0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const CellF c0 = cell_fields<NM>(P, S, n, i);
+  const float4 c0 = fld[i];
+  const float ic = 1.f / fmaxf(snd[i], 1e-30f);
   float *x = X + (size_t)(row0 + i) * MLP_K;
   const int fx[2] = {3, 1}, fy[2] = {0, 2}, fz[2] = {4, 5};
   int s = 0;
@@ -114,12 +130,12 @@ __global__ __launch_bounds__(256) void k_mlp_features(int n, Params P, const dou
         if (dx) j = step_face(j, fx[dx > 0], n, nbr, info, fine4);
         if (dy) j = step_face(j, fy[dy > 0], n, nbr, info, fine4);
         if (dz) j = step_face(j, fz[dz > 0], n, nbr, info, fine4);
-        const CellF cj = (j == i) ? c0 : cell_fields<NM>(P, S, n, j);
-        const double pden = fmax(fmax(fabs(cj.p), fabs(c0.p)), 1e-30);
-        x[4 * s + 0] = clampf(cj.rho / c0.rho - 1.0);
-        x[4 * s + 1] = clampf((cj.p - c0.p) / pden);
-        x[4 * s + 2] = clampf(cj.alpha);
-        x[4 * s + 3] = clampf(cj.umag / fmax(c0.c, 1e-30));
+        const float4 cj = (j == i) ? c0 : fld[j];
+        const float pden = fmaxf(fmaxf(fabsf(cj.y), fabsf(c0.y)), 1e-30f);
+        x[4 * s + 0] = clampf(cj.x / c0.x - 1.f);
+        x[4 * s + 1] = clampf((cj.y - c0.y) / pden);
+        x[4 * s + 2] = clampf(cj.z);
+        x[4 * s + 3] = clampf(cj.w * ic);
       }
   for (int k = MLP_NIN; k < MLP_K; k++) x[k] = 0.f;
 }
@@ -268,23 +284,21 @@ FV_DEV bf16x4 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
 
 // one wave = 16 samples; 4 waves per block.  A operand lane l: row l%16,
 // k = 4*(l/16)+0..3; B operand lane l: column l%16, same k; D lane l: rows
-// 4*(l/16)+0..3, column l%16.
-__global__ __launch_bounds__(256) void k_mlp_infer(long n, const float *__restrict__ X, const uint16_t *__restrict__ w1t,
-                                                   const uint16_t *__restrict__ w2t, const uint16_t *__restrict__ w3t,
-                                                   const float *__restrict__ wf, float *__restrict__ out) {
-  __shared__ float hs[4][16][MLP_H + 1];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long r0 = ((long)blockIdx.x * 4 + wv) * 16;
+// 4*(l/16)+0..3, column l%16.  xa(row, k) returns the FP32 input feature.
+template <typename XA>
+FV_DEV void mlp_infer_wave(long n, long r0, int wv, int lane, XA xa, const uint16_t *__restrict__ w1t,
+                           const uint16_t *__restrict__ w2t, const uint16_t *__restrict__ w3t,
+                           const float *__restrict__ wf, float (*hs)[16][MLP_H + 1], float *__restrict__ out) {
   const int row = lane & 15, kq = 4 * (lane >> 4);
-  const long rr = (r0 + row < n) ? r0 + row : n - 1;
   const float *b1 = wf + MLP_OFF_B1, *b2 = wf + MLP_OFF_B2, *b3 = wf + MLP_OFF_B3;
   // layer 1: [16 x 112] x [112 x 64]
   for (int nt = 0; nt < 4; nt++) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int kt = 0; kt < MLP_K / 16; kt++) {
-      const float *xa = X + rr * MLP_K + kt * 16 + kq;
-      const bf16x4 a = pack4(to_bf16(xa[0]), to_bf16(xa[1]), to_bf16(xa[2]), to_bf16(xa[3]));
-      const uint16_t *wb = w1t + (size_t)(nt * 16 + row) * MLP_K + kt * 16 + kq;
+      const int k0 = kt * 16 + kq;
+      const bf16x4 a = pack4(to_bf16(xa(row, k0)), to_bf16(xa(row, k0 + 1)), to_bf16(xa(row, k0 + 2)),
+                             to_bf16(xa(row, k0 + 3)));
+      const uint16_t *wb = w1t + (size_t)(nt * 16 + row) * MLP_K + k0;
       const bf16x4 b = pack4(wb[0], wb[1], wb[2], wb[3]);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
@@ -328,6 +342,82 @@ __global__ __launch_bounds__(256) void k_mlp_infer(long n, const float *__restri
       const long r = r0 + kq + q;
       if (r < n) out[r * MLP_NO + row] = acc[q] + b3[row];
     }
+}
+
+// inference of feature rows stored in HBM (host batches)
+__global__ __launch_bounds__(256) void k_mlp_infer(long n, const float *__restrict__ X, const uint16_t *__restrict__ w1t,
+                                                   const uint16_t *__restrict__ w2t, const uint16_t *__restrict__ w3t,
+                                                   const float *__restrict__ wf, float *__restrict__ out) {
+  __shared__ float hs[4][16][MLP_H + 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long r0 = ((long)blockIdx.x * 4 + wv) * 16;
+  auto xa = [&](int row, int k) {
+    const long rr = (r0 + row < n) ? r0 + row : n - 1;
+    return X[rr * MLP_K + k];
+  };
+  mlp_infer_wave(n, r0, wv, lane, xa, w1t, w2t, w3t, wf, hs, out);
+}
+
+// inference of every leaf with the stencil features gathered straight into LDS
+// (no feature matrix in HBM); same features as k_mlp_features, same MFMA order
+// as k_mlp_infer, so the outputs equal k_mlp_infer on k_mlp_features' rows
+__global__ __launch_bounds__(256) void k_mlp_infer_stencil(int n, const float4 *__restrict__ fld,
+                                                           const float *__restrict__ snd, const int *__restrict__ nbr,
+                                                           const uint32_t *__restrict__ info,
+                                                           const int *__restrict__ fine4,
+                                                           const uint16_t *__restrict__ w1t,
+                                                           const uint16_t *__restrict__ w2t,
+                                                           const uint16_t *__restrict__ w3t,
+                                                           const float *__restrict__ wf, float *__restrict__ out) {
+  __shared__ float hs[4][16][MLP_H + 1];
+  __shared__ float xs[4][16][MLP_K + 4];
+  __shared__ int nix[4][16][28];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long r0 = ((long)blockIdx.x * 4 + wv) * 16;
+  // four lanes per row; the 26 stencil neighbours are found level by level
+  // (faces, then edges from the x/y faces, then corners from the x-y edges),
+  // the same face chains x -> y -> z as k_mlp_features
+  const int row = lane >> 2, q = lane & 3;
+  const int i = (int)((r0 + row < n) ? r0 + row : n - 1);
+  int *nb = nix[wv][row];
+  const int fx[2] = {3, 1}, fy[2] = {0, 2}, fz[2] = {4, 5};
+  auto sidx = [](int dx, int dy, int dz) { return (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1); };
+  if (q == 0) nb[13] = i;
+  for (int e = q; e < 6; e += 4) {
+    const int ax = e >> 1, pm = e & 1, d = pm ? 1 : -1;
+    const int f = (ax == 0) ? fx[pm] : (ax == 1 ? fy[pm] : fz[pm]);
+    nb[sidx(ax == 0 ? d : 0, ax == 1 ? d : 0, ax == 2 ? d : 0)] = step_face(i, f, n, nbr, info, fine4);
+  }
+  __syncthreads();
+  for (int e = q; e < 12; e += 4) {
+    const int kind = e >> 2, a = (e & 1) ? 1 : -1, b = (e & 2) ? 1 : -1;
+    if (kind == 0) nb[sidx(a, b, 0)] = step_face(nb[sidx(a, 0, 0)], fy[b > 0], n, nbr, info, fine4);
+    else if (kind == 1) nb[sidx(a, 0, b)] = step_face(nb[sidx(a, 0, 0)], fz[b > 0], n, nbr, info, fine4);
+    else nb[sidx(0, a, b)] = step_face(nb[sidx(0, a, 0)], fz[b > 0], n, nbr, info, fine4);
+  }
+  __syncthreads();
+  for (int e = q; e < 8; e += 4) {
+    const int a = (e & 1) ? 1 : -1, b = (e & 2) ? 1 : -1, c = (e & 4) ? 1 : -1;
+    nb[sidx(a, b, c)] = step_face(nb[sidx(a, b, 0)], fz[c > 0], n, nbr, info, fine4);
+  }
+  __syncthreads();
+  {
+    const float4 c0 = fld[i];
+    const float ic = 1.f / fmaxf(snd[i], 1e-30f);
+    for (int s = q; s < 27; s += 4) {
+      const int j = nb[s];
+      const float4 cj = (j == i) ? c0 : fld[j];
+      const float pden = fmaxf(fmaxf(fabsf(cj.y), fabsf(c0.y)), 1e-30f);
+      xs[wv][row][4 * s + 0] = clampf(cj.x / c0.x - 1.f);
+      xs[wv][row][4 * s + 1] = clampf((cj.y - c0.y) / pden);
+      xs[wv][row][4 * s + 2] = clampf(cj.z);
+      xs[wv][row][4 * s + 3] = clampf(cj.w * ic);
+    }
+    xs[wv][row][MLP_NIN + q] = 0.f;
+  }
+  __syncthreads();
+  auto xa = [&](int row, int k) { return xs[wv][row][k]; };
+  mlp_infer_wave((long)n, r0, wv, lane, xa, w1t, w2t, w3t, wf, hs, out);
 }
 
 // target level from the network, with the plain indicator's refinement forced
@@ -441,7 +531,7 @@ extern "C" int fv_mlp_create(fv_mlp **out, unsigned seed) {
 extern "C" int fv_mlp_destroy(fv_mlp *m) {
   if (!m) return FV_OK;
   void *ps[] = {m->w, m->g, m->m, m->v, m->w1t, m->w2t, m->w3t, m->X, m->smooth, m->cls, m->bx, m->bs, m->z1, m->h1,
-                m->z2, m->h2, m->o, m->d3, m->d2, m->d1, m->lossv, m->bc, m->bidx, m->ix, m->iout, m->ismooth};
+                m->z2, m->h2, m->o, m->d3, m->d2, m->d1, m->lossv, m->bc, m->bidx, m->ix, m->iout, m->ismooth, m->fld, m->snd};
   for (void *p : ps) (void)hipFree(p);
   delete m;
   return FV_OK;
@@ -487,6 +577,29 @@ static int ensure_data(fv_mlp *m, long need) {
   return FV_OK;
 }
 
+static int ensure_fields(fv_mlp *m, long n) {
+  if (n <= m->fcap) return FV_OK;
+  HIPCHK(hipFree(m->fld)); HIPCHK(hipFree(m->snd));
+  m->fld = nullptr; m->snd = nullptr; m->fcap = 0;
+  HIPCHK(hipMalloc(&m->fld, sizeof(float4) * n));
+  HIPCHK(hipMalloc(&m->snd, sizeof(float) * n));
+  m->fcap = n;
+  return FV_OK;
+}
+
+// stencil features of every leaf of ctx into rows row0.. of X, on stream st
+static int launch_features(fv_dev *c, fv_mlp *m, float *X, long row0, hipStream_t st) {
+  int rc = ensure_fields(m, c->n);
+  if (rc != FV_OK) return rc;
+  if (c->nmat == 2)
+    hipLaunchKernelGGL(k_mlp_fields<2>, g1(c->n), dim3(256), 0, st, c->n, c->P, c->S, m->fld, m->snd);
+  else
+    hipLaunchKernelGGL(k_mlp_fields<3>, g1(c->n), dim3(256), 0, st, c->n, c->P, c->S, m->fld, m->snd);
+  hipLaunchKernelGGL(k_mlp_features, g1(c->n), dim3(256), 0, st, c->n, m->fld, m->snd, c->nbr, c->info, c->fine4, X, row0);
+  HIPCHK(hipGetLastError());
+  return FV_OK;
+}
+
 // append the (features, labels) of every leaf of ctx, labels from the plain indicator
 extern "C" int fv_mlp_sample(fv_dev *c, fv_mlp *m, const fv_crit *cr) {
   if (!c || !m || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
@@ -496,15 +609,14 @@ extern "C" int fv_mlp_sample(fv_dev *c, fv_mlp *m, const fv_crit *cr) {
   HIPCHK(hipStreamSynchronize(c->stream));
   rc = ensure_data(m, m->nd + c->n);
   if (rc != FV_OK) return rc;
-  if (c->nmat == 2) {
-    hipLaunchKernelGGL(k_mlp_features<2>, g1(c->n), dim3(256), 0, 0, c->n, c->P, c->S, c->nbr, c->info, c->fine4, m->X, m->nd);
+  rc = launch_features(c, m, m->X, m->nd, 0);
+  if (rc != FV_OK) return rc;
+  if (c->nmat == 2)
     hipLaunchKernelGGL(k_mlp_labels<2>, g1(c->n), dim3(256), 0, 0, c->n, c->P, c->S, c->nbr, c->info, c->fine4, c->split,
                        m->smooth_thr, m->cls, m->smooth, m->nd);
-  } else {
-    hipLaunchKernelGGL(k_mlp_features<3>, g1(c->n), dim3(256), 0, 0, c->n, c->P, c->S, c->nbr, c->info, c->fine4, m->X, m->nd);
+  else
     hipLaunchKernelGGL(k_mlp_labels<3>, g1(c->n), dim3(256), 0, 0, c->n, c->P, c->S, c->nbr, c->info, c->fine4, c->split,
                        m->smooth_thr, m->cls, m->smooth, m->nd);
-  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   m->nd += c->n;
@@ -604,13 +716,21 @@ static int refresh_bf16(fv_mlp *m) {
   return FV_OK;
 }
 
-static int ensure_infer(fv_mlp *m, long n) {
-  if (n <= m->icap) return FV_OK;
-  HIPCHK(hipFree(m->ix)); HIPCHK(hipFree(m->iout)); HIPCHK(hipFree(m->ismooth));
-  HIPCHK(hipMalloc(&m->ix, sizeof(float) * n * MLP_K));
-  HIPCHK(hipMalloc(&m->iout, sizeof(float) * n * MLP_NO));
-  HIPCHK(hipMalloc(&m->ismooth, n));
-  m->icap = n;
+// outputs of n rows; with_x also the n x 112 input rows of a host batch
+static int ensure_infer(fv_mlp *m, long n, bool with_x) {
+  if (n > m->icap) {
+    HIPCHK(hipFree(m->iout)); HIPCHK(hipFree(m->ismooth));
+    m->iout = nullptr; m->ismooth = nullptr; m->icap = 0;
+    HIPCHK(hipMalloc(&m->iout, sizeof(float) * n * MLP_NO));
+    HIPCHK(hipMalloc(&m->ismooth, n));
+    m->icap = n;
+  }
+  if (with_x && n > m->xcap) {
+    HIPCHK(hipFree(m->ix));
+    m->ix = nullptr; m->xcap = 0;
+    HIPCHK(hipMalloc(&m->ix, sizeof(float) * n * MLP_K));
+    m->xcap = n;
+  }
   return FV_OK;
 }
 
@@ -618,7 +738,7 @@ static int ensure_infer(fv_mlp *m, long n) {
 extern "C" int fv_mlp_infer(fv_mlp *m, const float *X, int B, float *out) {
   if (!m || !X || !out || B < 1) return fv_set_err(FV_ERR_ARG, "bad argument");
   int rc = refresh_bf16(m);
-  if (rc == FV_OK) rc = ensure_infer(m, B);
+  if (rc == FV_OK) rc = ensure_infer(m, B, true);
   if (rc != FV_OK) return rc;
   HIPCHK(hipMemcpy(m->ix, X, sizeof(float) * B * MLP_K, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((B + 63) / 64)), dim3(256), 0, 0, (long)B, m->ix, m->w1t, m->w2t, m->w3t,
@@ -636,21 +756,21 @@ extern "C" int fv_dev_indicator_mlp(fv_dev *c, fv_mlp *m, const fv_crit *cr, int
   int *SIdev = nullptr;
   int rc = fv_dev_indicator_raw(c, cr, &SIdev);
   if (rc == FV_OK) rc = refresh_bf16(m);
-  if (rc == FV_OK) rc = ensure_infer(m, c->n);
+  if (rc == FV_OK) rc = ensure_infer(m, c->n, false);
+  if (rc != FV_OK) return rc;
+  rc = ensure_fields(m, c->n);
   if (rc != FV_OK) return rc;
   if (c->nmat == 2)
-    hipLaunchKernelGGL(k_mlp_features<2>, g1(c->n), dim3(256), 0, c->stream, c->n, c->P, c->S, c->nbr, c->info, c->fine4,
-                       m->ix, 0L);
+    hipLaunchKernelGGL(k_mlp_fields<2>, g1(c->n), dim3(256), 0, c->stream, c->n, c->P, c->S, m->fld, m->snd);
   else
-    hipLaunchKernelGGL(k_mlp_features<3>, g1(c->n), dim3(256), 0, c->stream, c->n, c->P, c->S, c->nbr, c->info, c->fine4,
-                       m->ix, 0L);
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((c->n + 63) / 64)), dim3(256), 0, c->stream, (long)c->n, m->ix, m->w1t,
-                     m->w2t, m->w3t, m->w, m->iout);
+    hipLaunchKernelGGL(k_mlp_fields<3>, g1(c->n), dim3(256), 0, c->stream, c->n, c->P, c->S, m->fld, m->snd);
+  hipLaunchKernelGGL(k_mlp_infer_stencil, dim3((unsigned)((c->n + 63) / 64)), dim3(256), 0, c->stream, c->n, m->fld,
+                     m->snd, c->nbr, c->info, c->fine4, m->w1t, m->w2t, m->w3t, m->w, m->iout);
   hipLaunchKernelGGL(k_mlp_decide, g1(c->n), dim3(256), 0, c->stream, c->n, m->iout, c->info, c->split, c->lim, m->ismooth);
   HIPCHK(hipGetLastError());
-  if (SI) HIPCHK(hipMemcpyAsync(SI, SIdev, sizeof(int) * 2 * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
-  if (split) HIPCHK(hipMemcpyAsync(split, c->split, sizeof(int) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
-  if (smooth) HIPCHK(hipMemcpyAsync(smooth, m->ismooth, (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+  if (SI && (rc = fv_copy_to_host(c, SI, SIdev, sizeof(int) * 2 * (size_t)c->n)) != FV_OK) return rc;
+  if (split && (rc = fv_copy_to_host(c, split, c->split, sizeof(int) * (size_t)c->n)) != FV_OK) return rc;
+  if (smooth && (rc = fv_copy_to_host(c, smooth, m->ismooth, (size_t)c->n)) != FV_OK) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return FV_OK;
 }

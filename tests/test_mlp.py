@@ -232,3 +232,33 @@ def test_per_leaf_limiter_parity_bitwise():
     finally:
         O.set_cell_limiter(None)
         dev.close()
+
+
+@pytest.mark.gpu
+def test_fused_stencil_inference_matches_feature_rows():
+    """fv_dev_indicator_mlp gathers the stencil into LDS inside the MFMA kernel;
+    its decisions equal the ones taken from k_mlp_infer on the feature rows that
+    fv_mlp_sample stores, with the plain refinement forced."""
+    mlp = F.MLP(seed=41)
+    mlp.set_weights(rand_weights(42))
+    p, m, S = bubble_case(level=5, prelax=1)
+    dev = F.Device(p, m)
+    dev.set_state(S)
+    dev.advance(2)
+    crit = F.make_crit(prs_on=1, prs_lvl=7, prs_thr=0.05, inter0_on=1, inter0_lvl=7, inter0_lrs=3, uniform_on=1,
+                       uniform_lvl=4)
+    _, plain = dev.indicator(crit)
+    mlp.sample(dev, crit)
+    X, _, _ = mlp.dataset()
+    out = mlp.infer(X)
+    _, split, sm = dev.indicator_mlp(mlp, crit)
+    dev.close()
+    mlp.close()
+    lvl = m.arrays()["level"]
+    o0, o1, o2 = out[:, 0], out[:, 1], out[:, 2]
+    c = np.where((o0 >= o1) & (o0 >= o2), 0, np.where(o1 >= o2, 1, 2))
+    t = np.maximum(np.where(c == 0, lvl + 1, np.where(c == 2, lvl - 1, lvl)), 1)
+    t = np.where((plain > lvl) & (plain > t), plain, t)
+    say(f"[mlp stencil] classes {np.bincount(c, minlength=3)}, smooth {int((out[:, 3] > 0).sum())}")
+    assert np.array_equal(split, t)
+    assert np.array_equal(sm, (out[:, 3] > 0).astype(np.uint8))
