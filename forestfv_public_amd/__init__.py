@@ -53,6 +53,7 @@ class fv_params(C.Structure):
         ("dt", C.c_double),
         ("bc", C.c_int * 6),
         ("fix", C.c_int),
+        ("primtv", C.c_int),
     ]
 
 
@@ -210,7 +211,7 @@ def neq(nmat):
 
 
 def make_params(nmat=2, gamma=(1.4, 1.4), pinf=(0.0, 0.0), amin=1e-8, avf_lim=0.5, prelax=0,
-                rk=RK_SSP3, limiter=0, dt=1e-3, bc=(4, 4, 4, 4, 4, 4), fix=FIX_ALL):
+                rk=RK_SSP3, limiter=0, dt=1e-3, bc=(4, 4, 4, 4, 4, 4), fix=FIX_ALL, primtv=0):
     p = fv_params()
     p.nmat = nmat
     for k in range(nmat):
@@ -220,6 +221,7 @@ def make_params(nmat=2, gamma=(1.4, 1.4), pinf=(0.0, 0.0), amin=1e-8, avf_lim=0.
     for k in range(6):
         p.bc[k] = bc[k]
     p.fix = fix
+    p.primtv = primtv
     return p
 
 

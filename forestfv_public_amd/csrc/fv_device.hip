@@ -129,15 +129,19 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon(int n, Params P, Geo G, cons
       const double dp = side ? dcc_b : dcc_a;
       const double dm = side ? dcc_a : dcc_b;
       double F[L::NEQ];
+      if (P.primtv) {
+        recon_prim_face<NM>(P, a, limiter, kf == FV_FK_BOUNDARY, Sc, Xf, Xk, dcf, dp, dm, F);
+      } else {
 #pragma unroll
-      for (int v = 0; v < L::NEQ; v++) {
-        double df = Xf[v] - Sc[v];
-        double db = Xk[v] - Sc[v];
-        double g = 0.0;
-        if (kf != FV_FK_BOUNDARY) g = limited_slope<double>(limiter, df, db, dp, dm);
-        F[v] = Sc[v] + dcf * g;
+        for (int v = 0; v < L::NEQ; v++) {
+          double df = Xf[v] - Sc[v];
+          double db = Xk[v] - Sc[v];
+          double g = 0.0;
+          if (kf != FV_FK_BOUNDARY) g = limited_slope<double>(limiter, df, db, dp, dm);
+          F[v] = Sc[v] + dcf * g;
+        }
+        vf_limit_face<NM, double>(a, F);
       }
-      vf_limit_face<NM, double>(a, F);
       relax<NM>(P, F);
 #pragma unroll
       for (int v = 0; v < L::NEQ; v++) SF[((size_t)f * L::NEQ + v) * n + i] = F[v];
@@ -329,12 +333,20 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon_fast(int n, Params P, Geo G,
       const double dcf = side ? dcf_b : dcf_a;
       const R idp = side ? ib : ia, idm = side ? ia : ib;
       double F[L::NEQ];
+      if (P.primtv) {
+        // primitive-variable reconstruction in the reference's arithmetic
+        double dcc_f, dcc_k, dcf_k;
+        dist_of(G, side ? kb : ka, lvl, dcf_k, dcc_f);
+        dist_of(G, side ? ka : kb, lvl, dcf_k, dcc_k);
+        recon_prim_face<NM>(P, a, limiter, (side ? kb : ka) == FV_FK_BOUNDARY, Sc, Xf, Xk, dcf, dcc_f, dcc_k, F);
+      } else {
 #pragma unroll
-      for (int v = 0; v < L::NEQ; v++) {
-        const R g = bnd ? (R)0 : fvf::slope<R>(limiter, Xf[v] - Sc[v], Xk[v] - Sc[v], idp, idm);
-        F[v] = Sc[v] + dcf * (double)g;
+        for (int v = 0; v < L::NEQ; v++) {
+          const R g = bnd ? (R)0 : fvf::slope<R>(limiter, Xf[v] - Sc[v], Xk[v] - Sc[v], idp, idm);
+          F[v] = Sc[v] + dcf * (double)g;
+        }
+        vf_limit_face<NM, double>(a, F);
       }
-      vf_limit_face<NM, double>(a, F);
       fvf::relax_fast<NM>(P, M, F);
       if (MIXED) {
         float rec[fvf::Rec<NM>::N];
@@ -689,13 +701,17 @@ FV_DEV void recon_one(const Params &P, const fvf::Mat<NM, double> &M, const Geo 
   dist_of(G, kg, lvl, dcf_g, dcc_g);
   dist_of(G, ko, lvl, dcf_o, dcc_o);
   const bool bnd = (kg == FV_FK_BOUNDARY) || (ko == FV_FK_BOUNDARY);
-  const R idp = bnd ? (R)0 : (R)fvf::rcp(dcc_g), idm = bnd ? (R)0 : (R)fvf::rcp(dcc_o);
+  if (P.primtv) {
+    recon_prim_face<NM>(P, a, limiter, kg == FV_FK_BOUNDARY, Sc, Xg, Xo, dcf_g, dcc_g, dcc_o, F);
+  } else {
+    const R idp = bnd ? (R)0 : (R)fvf::rcp(dcc_g), idm = bnd ? (R)0 : (R)fvf::rcp(dcc_o);
 #pragma unroll
-  for (int v = 0; v < L::NEQ; v++) {
-    const R g = bnd ? (R)0 : fvf::slope<R>(limiter, Xg[v] - Sc[v], Xo[v] - Sc[v], idp, idm);
-    F[v] = Sc[v] + dcf_g * (double)g;
+    for (int v = 0; v < L::NEQ; v++) {
+      const R g = bnd ? (R)0 : fvf::slope<R>(limiter, Xg[v] - Sc[v], Xo[v] - Sc[v], idp, idm);
+      F[v] = Sc[v] + dcf_g * (double)g;
+    }
+    vf_limit_face<NM, double>(a, F);
   }
-  vf_limit_face<NM, double>(a, F);
   fvf::relax_fast<NM>(P, M, F);
 }
 
@@ -1113,6 +1129,7 @@ static void to_dev_params(const fv_params *p, Params &P) {
   for (int k = 0; k < 4; k++) { P.gamma[k] = p->gamma[k]; P.pinf[k] = p->pinf[k]; }
   P.amin = p->amin; P.avf_lim = p->avf_lim; P.dt = p->dt;
   P.prelax = p->prelax; P.limiter = p->limiter; P.rk = p->rk; P.nmat = p->nmat; P.fix = p->fix;
+  P.primtv = p->primtv;
   for (int k = 0; k < 4; k++) {
     const double g = p->gamma[k], pi = p->pinf[k];
     const double gm1 = (g != 1.0) ? g - 1.0 : 1.0, gg = (g != 0.0) ? g : 1.0;
@@ -1186,7 +1203,8 @@ static int upload_mesh(fv_dev *c, const fv_pack *pk) {
 extern "C" int fv_dev_attach(fv_dev **out, const fv_params *p, const fv_pack *pk) {
   if (!out || !p || !pk) return set_err(FV_ERR_ARG, "null argument");
   if (p->nmat != 2 && p->nmat != 3) return set_err(FV_ERR_ARG, "nmat must be 2 or 3");
-  if (p->prelax != 0 && p->prelax != 1 && p->prelax != 3) return set_err(FV_ERR_ARG, "PRELAX must be 0, 1 or 3");
+  if (p->prelax < 0 || p->prelax > 3) return set_err(FV_ERR_ARG, "PRELAX must be 0, 1, 2 or 3");
+  if (p->primtv != 0 && p->primtv != 1) return set_err(FV_ERR_ARG, "PRIMTV must be 0 or 1");
   if (p->prelax == 1 && p->nmat != 2) return set_err(FV_ERR_ARG, "PRELAX=1 is two-phase only");
   if (p->rk != FV_RK_REF2 && p->rk != FV_RK_SSP3 && p->rk != FV_RK_SSP2) return set_err(FV_ERR_ARG, "unknown RK table");
   int ndev = 0;

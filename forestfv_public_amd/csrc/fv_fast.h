@@ -249,11 +249,11 @@ FV_DEV void hllc(const Mat<NM, R> &M, int fix, const R *F, int bc, const SolF<NM
   un = vel0; vn = vel1; wn = vel2;
 }
 
-// m4_relaxationvec.c for PRELAX 0 and 1 (PRELAX 3 uses fvk::relax)
+// m4_relaxationvec.c for PRELAX 0 and 1 (the Newton forms PRELAX 2 and 3 use fvk::relax)
 template <int NM>
 FV_DEV void relax_fast(const Params &Pr, const Mat<NM, double> &M, double *vec) {
   using L = Lay<NM>;
-  if (Pr.prelax == 3) { fvk::relax<NM>(Pr, vec); return; }
+  if (Pr.prelax == 2 || Pr.prelax == 3) { fvk::relax<NM>(Pr, vec); return; }
   const double a = Pr.amin / 100.0;
   double ar[NM], are[NM], avf[NM + 1], rho = 0.0;
 #pragma unroll

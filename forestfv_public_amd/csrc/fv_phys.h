@@ -24,7 +24,7 @@ struct Lay {
 struct Params {
   double gamma[4], pinf[4];
   double amin, avf_lim, dt;
-  int prelax, limiter, rk, nmat, fix;
+  int prelax, limiter, rk, nmat, fix, primtv;
   // per-material constants formed once on the host: G p_inf, G - 1, 1/(G-1),
   // G p_inf/(G-1), 1/G (used by the throughput kernels)
   double GP[4], Gm1[4], iGm1[4], C[4], iG[4];
@@ -300,6 +300,61 @@ FV_DEV void relax(const Params &P, double *vec) {
     avf[0] = ar[0] / (1.0 / t1_0);
     avf[1] = ar[1] / (1.0 / t1_1);
     vf_limit_relax<NM>(a, avf);
+  } else if (P.prelax == 2) {       // m4_relaxationvec.c:128-297 with F_pe / DF_pe (F_pe.c:18-27, 64-73)
+    // the published branch fills local arrays while F_pe reads never-written
+    // RUN fields of the same names; the iteration uses the local values
+    const double eb = 1.0e-8;
+    double t0[NM], p0[NM], max_avf = -1.0;
+    int principal = 0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      t0[k] = 1.0 / (ar[k] / avf[k]);
+      p0[k] = (are[k] * (P.gamma[k] - 1.0) - avf[k] * P.gamma[k] * P.pinf[k]) / avf[k];
+      if (p0[k] < -(1.0 - eb) * P.pinf[k] + eb) p0[k] = -(1.0 - eb) * P.pinf[k] + 1.0;
+      if (avf[k] > max_avf) { max_avf = avf[k]; principal = k; }
+    }
+    auto Fpe = [&](double pe) {
+      double F = 0.0;
+#pragma unroll
+      for (int k = 0; k < NM; ++k) {
+        const double G = P.gamma[k], Pi = P.pinf[k];
+        F += ar[k] * t0[k] * ((p0[k] + G * Pi + (G - 1.0) * pe) / (pe + G * Pi + (G - 1.0) * pe));
+      }
+      return F + -1.0;
+    };
+    auto DFpe = [&](double pe) {
+      double D = 0.0;
+#pragma unroll
+      for (int k = 0; k < NM; ++k) {
+        const double G = P.gamma[k], Pi = P.pinf[k];
+        const double c1 = ar[k] * t0[k];
+        const double f = (p0[k] + G * Pi + (G - 1.0) * pe);
+        const double g = (pe + G * Pi + (G - 1.0) * pe);
+        D += (c1 * (G - 1.0) * g - c1 * f * G) / sq(g);
+      }
+      return D;
+    };
+    double pe_old = 0.001, pe_new = pe_old;
+    int it = 0;
+    double c1 = fabs(pe_new - pe_old), c2 = fabs(Fpe(pe_new));
+    while (((c2 >= 1.0e-12) || (c1 >= 1.0e-4)) && (it <= 101)) {
+      pe_new = pe_old - 0.8 * (Fpe(pe_old) / DFpe(pe_old));
+      c1 = fabs(pe_new - pe_old);
+      c2 = fabs(Fpe(pe_new));
+      if (it >= 100 && ((fabs(pe_old - pe_new) < 1.0e3) || isnan(pe_new))) { pe_new = NAN; break; }
+      pe_old = pe_new;
+      it++;
+    }
+    const double Pp = P.pinf[principal];
+    if (pe_new < (-(1.0 - eb) * Pp + eb)) pe_new = -(1.0 - eb) * Pp + 1.0;
+    const double pe = pe_new;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      const double G = P.gamma[k], Pi = P.pinf[k];
+      const double t1 = t0[k] * (p0[k] + G * Pi + (G - 1.0) * pe) / (pe + G * Pi + (G - 1.0) * pe);
+      avf[k] = ar[k] / (1.0 / t1);
+    }
+    vf_limit_relax<NM>(a, avf);
   } else if (P.prelax == 3) {       // m4_relaxationvec.c:298-479
     const double es = 1.0e-12, eb = 1.0e-8;
     double rk[NM], pk[NM];
@@ -346,7 +401,7 @@ FV_DEV void relax(const Params &P, double *vec) {
       vf_limit_relax<NM>(a, avf);
     }
   }
-  if ((NM == 2 && P.prelax == 1) || P.prelax == 3) {
+  if ((NM == 2 && P.prelax == 1) || P.prelax == 2 || P.prelax == 3) {
     double e_hyd = rE / rho - 0.5 * (sq((u)) + sq((v)) + sq((w)));
     double num = 0.0, den = 0.0;
 #pragma unroll
@@ -407,6 +462,97 @@ FV_DEV R limited_slope(int limiter, R df, R db, R dp, R dm) {
     }
   }
   return g;
+}
+
+// m4_cons2primtv.c:3-40: [alpha (N3) | alpha rho_k | u v w | p_k]
+template <int NM>
+FV_DEV void cons2prim(const Params &P, const double *c, double *q) {
+  using L = Lay<NM>;
+  double avf[NM], sum = 0.0, rho = 0.0;
+#pragma unroll
+  for (int k = 0; k < L::N3; ++k) { q[k] = c[L::I3 + k]; avf[k] = c[L::I3 + k]; sum += avf[k]; }
+  avf[L::N3] = 1.0 - sum;
+#pragma unroll
+  for (int k = 0; k < NM; ++k) { q[L::N3 + k] = c[k]; rho += c[k]; }
+  constexpr int IU = L::N3 + NM, IP = L::N3 + NM + 3;
+  q[IU + 0] = c[L::I1 + 0] / rho;
+  q[IU + 1] = c[L::I1 + 1] / rho;
+  q[IU + 2] = c[L::I1 + 2] / rho;
+#pragma unroll
+  for (int k = 0; k < NM; ++k) q[IP + k] = ((P.gamma[k] - 1.0) * (c[L::I4 + k]) / avf[k]) - P.gamma[k] * P.pinf[k];
+}
+
+// m4_prim2conc.c:3-74
+template <int NM>
+FV_DEV void prim2cons(const Params &P, const double *q, double *c) {
+  using L = Lay<NM>;
+  const double a10 = P.amin / 10.0, a100 = P.amin / 100.0;
+  double avf[NM], sum = 0.0, rho = 0.0, sumare = 0.0;
+#pragma unroll
+  for (int k = 0; k < L::N3; ++k) {
+    avf[k] = q[k];
+    if (avf[k] <= a10) avf[k] = a10;
+    if (avf[k] >= (1.0 - (2.0 * a10))) avf[k] = 1.0 - (2.0 * a10);
+    sum += avf[k];
+  }
+  if (sum >= (1.0 - a100)) {
+    const double dif = sum - (1.0 - a100);
+#pragma unroll
+    for (int j = 0; j < L::N3; ++j) { double per = avf[j] / sum; avf[j] = avf[j] - per * dif; }
+    sum = 1.0 - a100;
+  }
+  avf[L::N3] = 1.0 - sum;
+#pragma unroll
+  for (int k = 0; k < NM; ++k) { c[k] = q[L::N3 + k]; rho += q[L::N3 + k]; }
+  constexpr int IU = L::N3 + NM, IP = L::N3 + NM + 3;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c[L::I1 + k] = rho * q[IU + k];
+#pragma unroll
+  for (int k = 0; k < L::N3; ++k) c[L::I3 + k] = avf[k];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) {
+    c[L::I4 + k] = (avf[k] * (q[IP + k] + P.gamma[k] * P.pinf[k]) / (P.gamma[k] - 1.0));
+    sumare += c[L::I4 + k];
+  }
+  c[L::I2] = sumare + 0.5 * rho * (sq(q[IU + 0]) + sq(q[IU + 1]) + sq(q[IU + 2]));
+}
+
+// PRIMTV=1 face state before relaxation (m4_reconctruct.c:93-181): deltas and
+// slopes of the primitive variables, the face value, the volume-fraction clip,
+// m4_prim2conc.  Sc, Xf, Xo are the avf-limited cell and stencil values.
+template <int NM>
+FV_DEV void recon_prim_face(const Params &P, double a, int limiter, bool bnd_f, const double *Sc, const double *Xf,
+                            const double *Xo, double dcf, double dp, double dm, double *F) {
+  constexpr int NP = 3 * NM + 2;
+  double qc[NP], qr[NP];
+  cons2prim<NM>(P, Sc, qc);
+  {
+    double qf[NP], qo[NP];
+    cons2prim<NM>(P, Xf, qf);
+    cons2prim<NM>(P, Xo, qo);
+#pragma unroll
+    for (int v = 0; v < NP; ++v) {
+      double g = 0.0;
+      if (!bnd_f) g = limited_slope<double>(limiter, qf[v] - qc[v], qo[v] - qc[v], dp, dm);
+      qr[v] = qc[v] + dcf * g;
+    }
+  }
+  double avf[NM], sum = 0.0;
+#pragma unroll
+  for (int k = 0; k < NM - 1; ++k) {
+    avf[k] = qr[k];
+    if (avf[k] <= a) avf[k] = a;
+    else if (avf[k] >= (1.0 - (2.0 * a))) avf[k] = 1.0 - (2.0 * a);
+    sum += avf[k];
+  }
+  if (sum >= (1.0 - a)) {
+    const double dif = sum - (1.0 - a);
+#pragma unroll
+    for (int j = 0; j < NM - 1; ++j) { double per = avf[j] / sum; avf[j] = avf[j] - per * dif; }
+  }
+#pragma unroll
+  for (int k = 0; k < NM - 1; ++k) qr[k] = avf[k];
+  prim2cons<NM>(P, qr, F);
 }
 
 // VF limit of a reconstructed face state (m4_reconctruct.c:147-171)

@@ -60,12 +60,13 @@ typedef struct {
   double pinf[FV_MAXMAT];     /* MATERPINF  */
   double amin;                /* AMIN       */
   double avf_lim;             /* AVF_LIM    */
-  int    prelax;              /* PRELAX: 0 none, 1 closed-form two-phase, 3 Newton */
+  int    prelax;              /* PRELAX: 0 none, 1 closed-form two-phase, 2 damped Newton on F_pe, 3 Newton */
   int    rk;                  /* FV_RK_* */
   int    limiter;             /* 0 minmod (m4_reconctruct.c:114-119), 1 MC (new), 2 per leaf (new) */
   double dt;                  /* DT */
   int    bc[6];               /* BC code of the -x,+x,-y,+y,-z,+z sides (BCOVERWRT) */
   int    fix;                 /* FV_FIX_* corrections of reference defects; 0 = literal reference */
+  int    primtv;              /* PRIMTV: 0 conservative, 1 primitive reconstruction (m4_reconctruct.c:126-181) */
 } fv_params;
 
 /* Corrections of the published model-4 residual (0 reproduces it literally):

@@ -481,6 +481,26 @@ static void vf_limit_relax(const layout_t *L, const orc_params *p, double *avf) 
   avf[L->n3] = 1.0 - avfsum;
 }
 
+/* F_pe.c:18-27 and :64-73 (i_paper = 1, Favrie): sum_k ar_k tau_k(pe) - 1 and its derivative */
+static double fpe(int nm, const double *G, const double *P, const double *ar, const double *t0, const double *p0,
+                  double pe) {
+  double F = 0.0;
+  for (int iv = 0; iv < nm; ++iv)
+    F += ar[iv] * t0[iv] * ((p0[iv] + G[iv] * P[iv] + (G[iv] - 1.0) * pe) / (pe + G[iv] * P[iv] + (G[iv] - 1.0) * pe));
+  return F + -1.0;
+}
+static double dfpe(int nm, const double *G, const double *P, const double *ar, const double *t0, const double *p0,
+                   double pe) {
+  double D = 0.0;
+  for (int iv = 0; iv < nm; ++iv) {
+    double c1 = ar[iv] * t0[iv];
+    double f = (p0[iv] + G[iv] * P[iv] + (G[iv] - 1.0) * pe);
+    double g = (pe + G[iv] * P[iv] + (G[iv] - 1.0) * pe);
+    D += (c1 * (G[iv] - 1.0) * g - c1 * f * G[iv]) / SQ(g);
+  }
+  return D;
+}
+
 static void relax_vec(const layout_t *L, const orc_params *p, double *vec) {
   double ar[ORC_MAXMAT], are[ORC_MAXMAT], avf[ORC_MAXMAT + 1];
   double rho = 0.0;
@@ -514,6 +534,50 @@ static void relax_vec(const layout_t *L, const orc_params *p, double *vec) {
     }
     double pr = (rho * e_hyd - num) / den;
     for (int iv = 0; iv < L->n0; ++iv) are[iv] = (avf[iv] * (pr + G[iv] * P[iv]) / (G[iv] - 1.0));
+  } else if (p->prelax == 2) {   /* m4_relaxationvec.c:128-297, F_pe.c:3-123 (i_paper = 1) */
+    /* The published branch fills local t0_m / p0_m / ar_R while F_pe and DF_pe
+     * read the RUN fields of the same names, which are allocated
+     * (createutility.c:121-127) and never written; the Newton iteration below
+     * uses the local values, which is what the branch computes them for. */
+    const double eps_big = 1.0e-8;
+    const int nm = L->nmat;
+    double t0[ORC_MAXMAT], p0[ORC_MAXMAT];
+    double max_avf = -1.0;
+    int principal = 0;
+    for (int iv = 0; iv < nm; ++iv) {
+      t0[iv] = 1.0 / (ar[iv] / avf[iv]);
+      p0[iv] = (are[iv] * (G[iv] - 1.0) - avf[iv] * G[iv] * P[iv]) / avf[iv];
+      if (p0[iv] < -(1.0 - eps_big) * P[iv] + eps_big) p0[iv] = -(1.0 - eps_big) * P[iv] + 1.0;
+      if (avf[iv] > max_avf) { max_avf = avf[iv]; principal = iv; }
+    }
+    double pe_old = 0.001, pe_new = pe_old;
+    int iter = 0;
+    double cond_1 = fabs(pe_new - pe_old);
+    double cond_2 = fabs(fpe(nm, G, P, ar, t0, p0, pe_new));
+    while (((cond_2 >= 1.0e-12) || (cond_1 >= 1.0e-4)) && (iter <= 101)) {
+      pe_new = pe_old - 0.8 * (fpe(nm, G, P, ar, t0, p0, pe_old) / dfpe(nm, G, P, ar, t0, p0, pe_old));
+      cond_1 = fabs(pe_new - pe_old);
+      cond_2 = fabs(fpe(nm, G, P, ar, t0, p0, pe_new));
+      if (iter >= 100 && ((fabs(pe_old - pe_new) < 1.0e3) || isnan(pe_new))) { pe_new = NAN; break; }
+      pe_old = pe_new;
+      iter++;
+    }
+    if (pe_new < (-(1.0 - eps_big) * P[principal] + eps_big)) pe_new = -(1.0 - eps_big) * P[principal] + 1.0;
+    const double pe = pe_new;
+    for (int iv = 0; iv < nm; ++iv) {
+      double t1 = t0[iv] * (p0[iv] + G[iv] * P[iv] + (G[iv] - 1.0) * pe) / (pe + G[iv] * P[iv] + (G[iv] - 1.0) * pe);
+      double rm = 1.0 / t1;
+      avf[iv] = ar[iv] / rm;
+    }
+    vf_limit_relax(L, p, avf);
+    double e_hyd = rE / rho - 0.5 * (SQ((u)) + SQ((v)) + SQ((w)));
+    double num = 0.0, den = 0.0;
+    for (int iv = 0; iv < nm; ++iv) {
+      num += avf[iv] * G[iv] * P[iv] / (G[iv] - 1.0);
+      den += avf[iv] / (G[iv] - 1.0);
+    }
+    double pr = (rho * e_hyd - num) / den;
+    for (int iv = 0; iv < nm; ++iv) are[iv] = (avf[iv] * (pr + G[iv] * P[iv]) / (G[iv] - 1.0));
   } else if (p->prelax == 3) {   /* m4_relaxationvec.c:298-479 */
     const double eps_sml = 1.0e-12, eps_big = 1.0e-8;
     double rhoKs[ORC_MAXMAT], pk[ORC_MAXMAT];
@@ -617,6 +681,80 @@ static void muscl_neighbour(const layout_t *L, const orc_params *p, const orc_me
   avf_limit(L, p, X);
 }
 
+/* m4_cons2primtv.c:3-40 */
+static void cons2prim(const layout_t *L, const orc_params *p, const double *cons, double *prim) {
+  double avf[ORC_MAXMAT + 1], avfsum = 0.0, rho = 0.0;
+  for (int i = 0; i < L->n3; ++i) { prim[i] = cons[L->i3 + i]; avf[i] = cons[L->i3 + i]; avfsum += avf[i]; }
+  avf[L->n3] = 1.0 - avfsum;
+  for (int i = 0; i < L->n0; ++i) { prim[L->n3 + i] = cons[i]; rho += cons[i]; }
+  int ip = L->n3 + L->n0;
+  prim[ip + 0] = cons[L->i1 + 0] / rho;
+  prim[ip + 1] = cons[L->i1 + 1] / rho;
+  prim[ip + 2] = cons[L->i1 + 2] / rho;
+  ip = L->n3 + L->n0 + 3;
+  for (int i = 0; i < L->n4; ++i)
+    prim[ip + i] = ((p->gamma[i] - 1.0) * (cons[L->i4 + i]) / avf[i]) - p->gamma[i] * p->pinf[i];
+}
+
+/* m4_prim2conc.c:3-74 */
+static void prim2cons(const layout_t *L, const orc_params *p, const double *prim, double *cons) {
+  double avf[ORC_MAXMAT + 1], avfsum = 0.0, rho = 0.0, sumare = 0.0, a10 = p->amin / 10.0, a100 = p->amin / 100.0;
+  for (int i = 0; i < L->n3; ++i) {
+    avf[i] = prim[i];
+    if (avf[i] <= a10) avf[i] = a10;
+    if (avf[i] >= (1.0 - (2.0 * a10))) avf[i] = 1.0 - (2.0 * a10);
+    avfsum += avf[i];
+  }
+  if (avfsum >= (1.0 - a100)) {
+    double dif = avfsum - (1.0 - a100);
+    for (int j = 0; j < L->n3; ++j) { double per = avf[j] / avfsum; avf[j] = avf[j] - per * dif; }
+    avfsum = 1.0 - a100;
+  }
+  avf[L->n3] = 1.0 - avfsum;
+  for (int i = 0; i < L->n0; ++i) { cons[i] = prim[L->n3 + i]; rho += prim[L->n3 + i]; }
+  int ip = L->n3 + L->n0;
+  for (int i = 0; i < 3; ++i) cons[L->i1 + i] = rho * prim[ip + i];
+  for (int i = 0; i < L->n3; ++i) cons[L->i3 + i] = avf[i];
+  ip = L->n3 + L->n0 + 3;
+  for (int i = 0; i < L->n4; ++i) {
+    cons[L->i4 + i] = (avf[i] * (prim[ip + i] + p->gamma[i] * p->pinf[i]) / (p->gamma[i] - 1.0));
+    sumare += cons[L->i4 + i];
+  }
+  ip = L->n3 + L->n0;
+  cons[L->i2] = sumare + 0.5 * rho * (SQ(prim[ip + 0]) + SQ(prim[ip + 1]) + SQ(prim[ip + 2]));
+}
+
+void orc_cons2prim(const orc_params *p, const double *cons, double *prim) {
+  layout_t L = mk_layout(p->nmat);
+  cons2prim(&L, p, cons, prim);
+}
+void orc_prim2cons(const orc_params *p, const double *prim, double *cons) {
+  layout_t L = mk_layout(p->nmat);
+  prim2cons(&L, p, prim, cons);
+}
+
+/* minmod / MC slope of one variable (m4_reconctruct.c:105-128) */
+static double muscl_slope(int limiter, double df, double db, double dp, double dm) {
+  double g = 0.0;
+  if (limiter == 0) {
+    double product = (-db / dm) * (df / dp);
+    if (product > 0.0) {
+      if (fabs(db / dm) > fabs(df / dp)) g = 1.0 * df / dp;
+      else g = -1.0 * db / dm;
+    } else g = 0.0;
+  } else {
+    /* monotonised central: minmod(2a, 2b, (a+b)/2) with a backward, b forward slope */
+    double a = -db / dm, b = df / dp;
+    if (a * b > 0.0) {
+      double c = 0.5 * (a + b);
+      double m2 = MINF(2.0 * fabs(a), 2.0 * fabs(b));
+      double mag = MINF(m2, fabs(c));
+      g = (c > 0.0) ? mag : -mag;
+    } else g = 0.0;
+  }
+  return g;
+}
+
 static const unsigned char *g_cell_limiter = NULL;
 void orc_set_cell_limiter(const unsigned char *lim) { g_cell_limiter = lim; }
 
@@ -636,33 +774,44 @@ static void faceinterp_leaf(const layout_t *L, const orc_params *p, const orc_me
     /* m4_reconctruct.c:93-183 */
     double *SF = &SFi[f * L->neq];
     int bcf = m->bc[6 * i + f];
+    const double a = p->amin / 100.0;
+    if (p->primtv) {
+      /* deltas, slopes and the face value in primitive variables; the volume
+       * fraction is clipped, m4_prim2conc returns to conservative variables */
+      const int np = 3 * L->nmat + 2;
+      double pc[ORC_MAXNEQ], pf[ORC_MAXNEQ], pb[ORC_MAXNEQ], pr[ORC_MAXNEQ];
+      cons2prim(L, p, SLc, pc);
+      cons2prim(L, p, Xf, pf);
+      cons2prim(L, p, Xb, pb);
+      for (int iv = 0; iv < np; ++iv) {
+        double dfp = pf[iv] - pc[iv], dbp = pb[iv] - pc[iv], g = 0.0;
+        if (bcf == 0 && fabs(dfp) > 1.0e-10) g = muscl_slope(limiter, dfp, dbp, cl->dist_cc[f], cl->dist_cc[fo]);
+        pr[iv] = pc[iv] + cl->dist_cf[f] * g;
+      }
+      double avf[ORC_MAXMAT + 1], avfsum = 0.0;
+      for (int iv = 0; iv < L->n3; ++iv) {
+        avf[iv] = pr[iv];
+        if (avf[iv] <= a) avf[iv] = a;
+        else if (avf[iv] >= (1.0 - (2.0 * a))) avf[iv] = 1.0 - (2.0 * a);
+        avfsum += avf[iv];
+      }
+      if (avfsum >= (1.0 - a)) {
+        double dif = avfsum - (1.0 - a);
+        for (int j = 0; j < L->n3; ++j) { double per = avf[j] / avfsum; avf[j] = avf[j] - per * dif; }
+        avfsum = 1.0 - a;
+      }
+      for (int iv = 0; iv < L->n3; ++iv) pr[iv] = avf[iv];
+      prim2cons(L, p, pr, SF);
+      relax_vec(L, p, SF);
+      continue;
+    }
     for (int iv = 0; iv < L->neq; ++iv) {
       double g = 0.0;
-      if (bcf == 0) {
-        if (fabs(df[iv]) > 1.0e-10) {
-          double dp = cl->dist_cc[f], dm = cl->dist_cc[fo];
-          if (limiter == 0) {
-            double product = (-db[iv] / dm) * (df[iv] / dp);
-            if (product > 0.0) {
-              if (fabs(db[iv] / dm) > fabs(df[iv] / dp)) g = 1.0 * df[iv] / dp;
-              else g = -1.0 * db[iv] / dm;
-            } else g = 0.0;
-          } else {
-            /* monotonised central: minmod(2a, 2b, (a+b)/2) with a backward, b forward slope */
-            double a = -db[iv] / dm, b = df[iv] / dp;
-            if (a * b > 0.0) {
-              double c = 0.5 * (a + b);
-              double m2 = MINF(2.0 * fabs(a), 2.0 * fabs(b));
-              double mag = MINF(m2, fabs(c));
-              g = (c > 0.0) ? mag : -mag;
-            } else g = 0.0;
-          }
-        }
-      }
+      if (bcf == 0 && fabs(df[iv]) > 1.0e-10) g = muscl_slope(limiter, df[iv], db[iv], cl->dist_cc[f], cl->dist_cc[fo]);
       SF[iv] = SLc[iv] + cl->dist_cf[f] * g;
     }
     /* VF limit (m4_reconctruct.c:147-171) */
-    double a = p->amin / 100.0, avf[ORC_MAXMAT + 1], avfsum = 0.0;
+    double avf[ORC_MAXMAT + 1], avfsum = 0.0;
     for (int iv = 0; iv < L->n3; ++iv) {
       avf[iv] = SF[iv + L->i3];
       if (avf[iv] <= a) avf[iv] = a;

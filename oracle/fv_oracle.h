@@ -32,7 +32,7 @@ typedef struct {
   double pinf[ORC_MAXMAT];     /* MATERPINF  */
   double amin;                 /* AMIN       */
   double avf_lim;              /* AVF_LIM    */
-  int    prelax;               /* PRELAX: 0, 1 or 3 */
+  int    prelax;               /* PRELAX: 0, 1, 2 or 3 */
   int    rk;                   /* 2: RK_STEPS=2 as the reference runs it (cases 1,2)
                                   3: SSP-RK3 (cases 1,2,3)
                                   22: SSP-RK2 / Heun (cases 1,4) -- not a reference table */
@@ -40,7 +40,13 @@ typedef struct {
   double dt;                   /* DT */
   int    bc[6];                /* BC code of the -x,+x,-y,+y,-z,+z domain sides */
   int    fix;                  /* corrections of reference defects, ORC_FIX_* bits; 0 = literal */
+  int    primtv;               /* PRIMTV: 0 reconstruct conservative, 1 primitive variables */
 } orc_params;
+
+/* m4_cons2primtv.c:3-40 and m4_prim2conc.c:3-74: primitive vector
+ * [alpha (nmat-1) | alpha rho_k | u v w | p_k], 3 nmat + 2 values */
+void orc_cons2prim(const orc_params *p, const double *cons, double *prim);
+void orc_prim2cons(const orc_params *p, const double *prim, double *cons);
 
 /* The published model-4 residual is inconsistent with its own RK sign
  * convention in two places; each correction is a bit so that the literal

@@ -31,6 +31,7 @@ class orc_params(C.Structure):
         ("dt", C.c_double),
         ("bc", C.c_int * 6),
         ("fix", C.c_int),
+        ("primtv", C.c_int),
     ]
 
 
@@ -102,6 +103,10 @@ def lib():
         L.orc_cfl_dt.restype = C.c_double
         L.orc_set_cell_limiter.argtypes = [C.c_void_p]
         L.orc_set_cell_limiter.restype = None
+        L.orc_cons2prim.argtypes = [P(orc_params), dp, dp]
+        L.orc_cons2prim.restype = None
+        L.orc_prim2cons.argtypes = [P(orc_params), dp, dp]
+        L.orc_prim2cons.restype = None
         _lib = L
     return _lib
 
@@ -221,6 +226,20 @@ def set_cell_limiter(lim):
     a = np.ascontiguousarray(lim, dtype=np.uint8)
     _keep["lim"] = a
     lib().orc_set_cell_limiter(a.ctypes.data)
+
+
+def cons2prim(params, vec):
+    v = np.ascontiguousarray(vec, dtype=np.float64)
+    out = np.zeros(3 * params.nmat + 2)
+    lib().orc_cons2prim(C.byref(params_from(params)), _dp(v), _dp(out))
+    return out
+
+
+def prim2cons(params, prim):
+    v = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.zeros(3 * params.nmat + 3)
+    lib().orc_prim2cons(C.byref(params_from(params)), _dp(v), _dp(out))
+    return out
 
 
 def relax(params, vec):

@@ -44,8 +44,9 @@ def test_attach_without_device_is_a_status():
     assert e.value.code in (F.FV_ERR_HIP,)
 
 
-def test_bad_configuration_rejected_before_the_device():
-    p = F.make_params(nmat=2, prelax=2)
+@pytest.mark.parametrize("bad", [dict(prelax=4), dict(primtv=2), dict(nmat=4, gamma=(1.4,) * 4, pinf=(0.0,) * 4)])
+def test_bad_configuration_rejected_before_the_device(bad):
+    p = F.make_params(**dict(dict(nmat=2), **bad))
     m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), 1.0, 2, [4] * 6)
     with pytest.raises(F.FVError) as e:
         F.Device(p, m)

@@ -86,7 +86,16 @@ STAGE_CASES = {
     "bubble16_prelax1": lambda: bubble_case(level=5, prelax=1, L=2.0 ** -8),
     "bubble16_prelax3": lambda: bubble_case(level=5, prelax=3, L=2.0 ** -8),
     "bubble_mc": lambda: bubble_case(level=4, prelax=1, limiter=1, L=2.0 ** -8),
+    "sod16_primtv": lambda: with_primtv(perturbed_sod(5, prelax=1)),
+    "bubble16_primtv": lambda: with_primtv(bubble_case(level=5, prelax=1, L=2.0 ** -8)),
+    "bubble16_prelax2": lambda: bubble_case(level=5, prelax=2, L=2.0 ** -8),
 }
+
+
+def with_primtv(case):
+    p, m, S = case
+    p.primtv = 1
+    return p, m, S
 
 
 @pytest.mark.parametrize("name", list(STAGE_CASES))
@@ -257,7 +266,7 @@ def run_prec(p, m, S, nsteps, prec, SI=None):
     return Sg
 
 
-@pytest.mark.parametrize("case", ["sod_c0", "bubble_prelax1", "amr"])
+@pytest.mark.parametrize("case", ["sod_c0", "bubble_prelax1", "amr", "bubble_primtv", "bubble_prelax2"])
 def test_plain_fp64_path_100_steps(case):
     """The default plain FP64 kernels (reciprocal-multiply form) against the CPU path:
     relative L1 <= 1e-10 per conserved field after 100 steps."""
@@ -266,6 +275,10 @@ def test_plain_fp64_path_100_steps(case):
         p, m, S = sod_case(level=6, rk=F.RK_SSP2, prelax=0)
     elif case == "bubble_prelax1":
         p, m, S = bubble_case(level=5, prelax=1)
+    elif case == "bubble_primtv":
+        p, m, S = with_primtv(bubble_case(level=5, prelax=1))
+    elif case == "bubble_prelax2":
+        p, m, S = bubble_case(level=5, prelax=2)
     else:
         p, m, S, SI = amr_case(4, si_frac=0.1)
     Sg = run_prec(p, m, S, 100, F.PREC_FP64, SI)

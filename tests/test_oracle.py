@@ -334,3 +334,62 @@ def test_free_stream_on_amr_mesh():
     S = np.tile([1.0 - 1e-8, 1e-8, 0, 0, 0, 2.5, 1 - 1e-8, 2.5 - 2.5e-8, 2.5e-8], (m.n, 1))
     Sf = O.OracleMesh(m).advance(p, S, 2)
     assert np.allclose(Sf, S, rtol=1e-14, atol=1e-15)
+
+
+# ---------------------------------------------------------------- PRIMTV=1 and PRELAX=2
+def test_cons_prim_round_trip_and_closed_form():
+    """m4_cons2primtv / m4_prim2conc: the primitive vector is (alpha, alpha rho_k, u, p_k)
+    with the stiffened-gas p_k, and converting back restores a consistent state."""
+    for p in (water_air_params(prelax=1), sod_params()):
+        rng = np.random.default_rng(21)
+        G, Pi = np.array(p.gamma[:2]), np.array(p.pinf[:2])
+        for _ in range(100):
+            S, pr, vel = rand_state(p, rng)
+            q = O.cons2prim(p, S)
+            assert q[0] == S[6] and np.array_equal(q[1:3], S[0:2])
+            assert np.allclose(q[3:6], vel, rtol=1e-14, atol=1e-15)
+            assert np.allclose(q[6:8], pr, rtol=1e-9 if Pi[0] > 0 else 1e-13)
+            U = O.prim2cons(p, q)
+            assert np.allclose(U, S, rtol=1e-12, atol=1e-12 * np.abs(S).max())
+
+
+def test_relaxation_prelax2_solves_the_prelax1_equation():
+    """PRELAX=2 (damped Newton on F_pe, the Favrie form) and PRELAX=1 (its closed-form
+    root for two phases) solve the same equation sum_k (alpha rho)_k tau_k(p) = 1, so
+    they give the same relaxed state; the Newton exit tolerance is |F| < 1e-12 and
+    |dp| < 1e-4."""
+    p1 = water_air_params(prelax=1)
+    p2 = water_air_params(prelax=2)
+    rng = np.random.default_rng(7)
+    for _ in range(100):
+        S, pr, vel = rand_state(p1, rng)
+        S[7] *= 1 + rng.uniform(-0.02, 0.02)
+        S[5] = S[7] + S[8] + 0.5 * (S[0] + S[1]) * (vel ** 2).sum()
+        R1, R2 = O.relax(p1, S), O.relax(p2, S)
+        assert np.array_equal(R2[0:6], S[0:6])
+        assert np.allclose(R2, R1, rtol=1e-9, atol=0), (R1, R2)
+
+
+def test_sod_primtv_converges_to_exact_solution():
+    """Primitive-variable MUSCL (PRIMTV=1) on the Sod tube: small L1 density error that
+    falls with resolution, and a uniform state is preserved exactly."""
+    errs = []
+    for N in (64, 128):
+        p = sod_params(prelax=1, bc=(4, 4, 4, 4, 4, 4))
+        p.primtv = 1
+        m = F.Mesh.uniform((N, 1, 1), (0, 0, 0), 1.0 / N, 1, list(p.bc))
+        S = m.init_split_x(p, (1.0, 0.125), (1.0, 0.1), 0.5)
+        p.dt = 0.4 * (1.0 / N) / (np.sqrt(1.4) + 1.0)
+        nst = int(round(0.2 / p.dt))
+        p.dt = 0.2 / nst
+        Sf = O.OracleMesh(m).advance(p, S, nst)
+        x = m.centres()[:, 0]
+        ex = RX.sample(x, 0.2, 0.5, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4)
+        errs.append(np.mean(np.abs(Sf[:, 0] + Sf[:, 1] - ex[:, 0])))
+    assert errs[0] < 0.02 and errs[1] < 0.7 * errs[0], errs
+    p = sod_params(prelax=1)
+    p.primtv = 1
+    m = random_amr_mesh(p, base_level=2, max_level=4, seed=13, frac=0.3, passes=2)
+    S = np.tile([1.0 - 1e-8, 1e-8, 0, 0, 0, 2.5, 1 - 1e-8, 2.5 - 2.5e-8, 2.5e-8], (m.n, 1))
+    Sf = O.OracleMesh(m).advance(p, S, 2)
+    assert np.allclose(Sf, S, rtol=1e-14, atol=1e-15)
