@@ -60,3 +60,36 @@ def random_amr_mesh(params, base_level=2, max_level=4, nroot=(1, 1, 1), seed=0, 
         split = m.criterionsmth(max_level, split)
         S = m.adapt(params, split, S)
     return m
+
+
+THREE = dict(gamma=(6.12, 1.4, 2.0), pinf=(3.43e8, 0.0, 1.0e8))
+THREE_RHO = (1000.0, 1.2, 2000.0)
+
+
+def three_material_params(prelax=3, dt=None, level=4, rk=F.RK_SSP3, bc=(2, 2, 2, 2, 2, 2), limiter=0):
+    """Water, air and a stiffened third material (gamma 2, p_inf 1e8 Pa) at 10 bar."""
+    h = 1.0 / (1 << (level - 1))
+    if dt is None:
+        dt = 0.2 * h / 1500.0
+    return F.make_params(nmat=3, gamma=THREE["gamma"], pinf=THREE["pinf"], amin=1e-6, avf_lim=0.5, prelax=prelax,
+                         rk=rk, limiter=limiter, dt=dt, bc=bc)
+
+
+def three_material_case(level=4, prelax=3, **kw):
+    """Three slabs along x (water | air | third material) blended over about two cells,
+    in pressure equilibrium at 1e6 Pa, with a smooth velocity field: every cell is a
+    convex combination of the pure states at its own velocity."""
+    p = three_material_params(prelax=prelax, level=level, **kw)
+    m = F.Mesh.uniform((1, 1, 1), (0.0, 0.0, 0.0), 1.0, level, list(p.bc))
+    c = m.centres()
+    h = 1.0 / (1 << (level - 1))
+    s0 = 0.5 * (1 - np.tanh((c[:, 0] - 0.35) / (1.5 * h)))
+    s2 = 0.5 * (1 + np.tanh((c[:, 0] - 0.65) / (1.5 * h)))
+    w = np.stack([s0, np.clip(1 - s0 - s2, 0, 1), s2], 1)
+    w /= w.sum(1, keepdims=True)
+    S = np.zeros((m.n, F.neq(3)))
+    for i in range(m.n):
+        u = (8.0 * np.sin(2 * np.pi * c[i, 1]), 5.0 * np.cos(2 * np.pi * c[i, 2]), 3.0 * np.sin(2 * np.pi * c[i, 0]))
+        for k in range(3):
+            S[i] += w[i, k] * F.init_pure(p, THREE_RHO, k, 1.0e6, u)
+    return p, m, S

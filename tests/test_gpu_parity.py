@@ -14,7 +14,7 @@ import pytest
 
 import forestfv_public_amd as F
 import oracle_lib as O
-from cases import bubble_case, random_amr_mesh, sod_case, sod_params, water_air_params
+from cases import bubble_case, random_amr_mesh, sod_case, sod_params, three_material_case, water_air_params
 
 pytestmark = pytest.mark.gpu
 
@@ -26,10 +26,13 @@ def say(*a):
     sys.stdout.flush()
 
 
-def rel_l1(a, b, mom=(2, 3, 4)):
+def rel_l1(a, b, mom=None):
     """Per-field relative L1 error of a against b.  The three momentum components share
     the momentum magnitude as their scale, so a component that is zero up to rounding
     (the transverse momentum of a shock tube) is not divided by its own noise."""
+    if mom is None:
+        nm = (a.shape[1] - 3) // 3      # NEQ = 3 nmat + 3, momentum at nmat .. nmat + 2
+        mom = (nm, nm + 1, nm + 2)
     num = np.abs(a - b).sum(0)
     den = np.abs(b).sum(0)
     mscale = np.sqrt((b[:, list(mom)] ** 2).sum(1)).sum()
@@ -89,6 +92,9 @@ STAGE_CASES = {
     "sod16_primtv": lambda: with_primtv(perturbed_sod(5, prelax=1)),
     "bubble16_primtv": lambda: with_primtv(bubble_case(level=5, prelax=1, L=2.0 ** -8)),
     "bubble16_prelax2": lambda: bubble_case(level=5, prelax=2, L=2.0 ** -8),
+    "three_mat_prelax0": lambda: three_material_case(level=5, prelax=0),
+    "three_mat_prelax2": lambda: three_material_case(level=5, prelax=2),
+    "three_mat_prelax3": lambda: three_material_case(level=5, prelax=3),
 }
 
 
@@ -266,7 +272,7 @@ def run_prec(p, m, S, nsteps, prec, SI=None):
     return Sg
 
 
-@pytest.mark.parametrize("case", ["sod_c0", "bubble_prelax1", "amr", "bubble_primtv", "bubble_prelax2"])
+@pytest.mark.parametrize("case", ["sod_c0", "bubble_prelax1", "amr", "bubble_primtv", "bubble_prelax2", "three_mat"])
 def test_plain_fp64_path_100_steps(case):
     """The default plain FP64 kernels (reciprocal-multiply form) against the CPU path:
     relative L1 <= 1e-10 per conserved field after 100 steps."""
@@ -279,6 +285,8 @@ def test_plain_fp64_path_100_steps(case):
         p, m, S = with_primtv(bubble_case(level=5, prelax=1))
     elif case == "bubble_prelax2":
         p, m, S = bubble_case(level=5, prelax=2)
+    elif case == "three_mat":
+        p, m, S = three_material_case(level=5, prelax=3)
     else:
         p, m, S, SI = amr_case(4, si_frac=0.1)
     Sg = run_prec(p, m, S, 100, F.PREC_FP64, SI)
