@@ -11,6 +11,9 @@
  */
 #include "fv_oracle.h"
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -844,9 +847,45 @@ typedef struct {
   int    *flux_flag;   /* [n][6]     */
 } resid_ws;
 
+/* the computed branch of the face loop (m4_residual.c:112-149): flux_face[f] and u_face[f] of leaf i */
+static void face_flux(const layout_t *L, const orc_params *p, const orc_mesh *m, const orc_cell *cells,
+                      const double *S, const double *SF, int i, int f, int Nneig, int bc, double *ff, double *uf) {
+  const orc_cell *cl = &cells[i];
+  const int neq = L->neq;
+  const double *Si = &S[(size_t)i * neq];
+  uf[0] = 0.0; uf[1] = 0.0; uf[2] = 0.0;
+  for (int iv = 0; iv < neq; ++iv) ff[iv] = 0.0;
+  frame_t F = mkframe(cl, f);
+  sol_t sL, sR;
+  fill_sol(L, p, &SF[((size_t)i * 6 + f) * neq], Si, &sL);          /* m4_facercnstr imode 0 */
+  for (int k = 0; k < Nneig; ++k) {
+    if (bc == 0) {
+      int j = m->neigtr[24 * i + 4 * f + k];
+      fill_sol(L, p, &SF[((size_t)j * 6 + FCOPP[f]) * neq], &S[(size_t)j * neq], &sR);
+    } else {
+      fill_sol(L, p, &SF[((size_t)i * 6 + f) * neq], Si, &sR);
+      apply_bc(bc, &F, &sL, &sR);
+    }
+    hllc_ws W;
+    m4_hllc(L, p, &F, bc, &sL, &sR, &W);
+    for (int iv = 0; iv < neq; ++iv) {
+      ff[iv] += (cl->Area[f][k]) *
+                (((W.flux_adv[iv][0] - 0.0) * cl->nx[f] + (W.flux_adv[iv][1] - 0.0) * cl->ny[f] +
+                  (W.flux_adv[iv][2] - 0.0) * cl->nz[f]) + W.flux_hllc[iv]);
+    }
+    uf[0] += W.u_nc / ((double)Nneig);
+    uf[1] += W.v_nc / ((double)Nneig);
+    uf[2] += W.w_nc / ((double)Nneig);
+  }
+}
+
+/* par = 0: the sequential flag protocol of m4_residual.c:62-85.  par = 1: the
+ * same faces, with a mirrored face recomputed exactly as the earlier leaf
+ * computes it instead of read from it, so leaves are independent (OpenMP) and
+ * the result is bit-identical. */
 static int residual_leaf(const layout_t *L, const orc_params *p, const orc_mesh *m,
                          const orc_cell *cells, const double *S, const double *SF, int i,
-                         resid_ws *R, double *RHS) {
+                         resid_ws *R, double *RHS, int par) {
   const orc_cell *cl = &cells[i];
   const int neq = L->neq;
   double *rhs = &RHS[(size_t)i * neq];
@@ -875,7 +914,16 @@ static int residual_leaf(const layout_t *L, const orc_params *p, const orc_mesh 
         int j = m->neigtr[24 * i + 4 * f];
         int fo = FCOPP[f];
         if (m->lfc_neigs[6 * j + fo] == 1) {       /* 1:1 */
-          if (SI0 == 0) {                          /* one rank: parts always equal */
+          if (par) {
+            if (SI0 == 0 && m->SI[2 * j] == 0 && j < i) {
+              double ffj[ORC_MAXNEQ], ufj[3];
+              face_flux(L, p, m, cells, S, SF, j, fo, 1, 0, ffj, ufj);
+              if (p->fix & ORC_FIX_NONCONS) { uf[0] = ufj[0]; uf[1] = ufj[1]; uf[2] = ufj[2]; }
+              else { uf[0] = -ufj[0]; uf[1] = -ufj[1]; uf[2] = -ufj[2]; }
+              for (int iv = 0; iv < neq; ++iv) ff[iv] = -ffj[iv];
+              compute = 0;
+            }
+          } else if (SI0 == 0) {                   /* one rank: parts always equal */
             if (R->flux_flag[6 * i + f] == 1) {
               const double *uo = &R->u_face[((size_t)j * 6 + fo) * 3];
               const double *fo_ = &R->flux_face[((size_t)j * 6 + fo) * neq];
@@ -895,31 +943,7 @@ static int residual_leaf(const layout_t *L, const orc_params *p, const orc_mesh 
     }
     double Area_f = 0.0;
     for (int k = 0; k < Nneig; ++k) Area_f += cl->Area[f][k];
-    if (compute) {
-      for (int iv = 0; iv < neq; ++iv) ff[iv] = 0.0;
-      frame_t F = mkframe(cl, f);
-      sol_t sL, sR;
-      fill_sol(L, p, &SF[((size_t)i * 6 + f) * neq], Si, &sL);          /* m4_facercnstr imode 0 */
-      for (int k = 0; k < Nneig; ++k) {
-        if (bc == 0) {
-          int j = m->neigtr[24 * i + 4 * f + k];
-          fill_sol(L, p, &SF[((size_t)j * 6 + FCOPP[f]) * neq], &S[(size_t)j * neq], &sR);
-        } else {
-          fill_sol(L, p, &SF[((size_t)i * 6 + f) * neq], Si, &sR);
-          apply_bc(bc, &F, &sL, &sR);
-        }
-        hllc_ws W;
-        m4_hllc(L, p, &F, bc, &sL, &sR, &W);
-        for (int iv = 0; iv < neq; ++iv) {
-          ff[iv] += (cl->Area[f][k]) *
-                    (((W.flux_adv[iv][0] - 0.0) * cl->nx[f] + (W.flux_adv[iv][1] - 0.0) * cl->ny[f] +
-                      (W.flux_adv[iv][2] - 0.0) * cl->nz[f]) + W.flux_hllc[iv]);
-        }
-        uf[0] += W.u_nc / ((double)Nneig);
-        uf[1] += W.v_nc / ((double)Nneig);
-        uf[2] += W.w_nc / ((double)Nneig);
-      }
-    }
+    if (compute) face_flux(L, p, m, cells, S, SF, i, f, Nneig, bc, ff, uf);
     for (int iv = 0; iv < neq; ++iv) rhs[iv] += ff[iv];
     double u_nc = uf[0], nx = cl->nx[f];
     double v_nc = uf[1], ny = cl->ny[f];
@@ -942,10 +966,16 @@ static int residual_leaf(const layout_t *L, const orc_params *p, const orc_mesh 
 }
 
 static int rhs_all(const layout_t *L, const orc_params *p, const orc_mesh *m, const orc_cell *cells,
-                   const double *S, const double *SF, double *RHS, resid_ws *R) {
+                   const double *S, const double *SF, double *RHS, resid_ws *R, int par) {
+  if (par) {
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int i = 0; i < m->n; i++) bad |= residual_leaf(L, p, m, cells, S, SF, i, R, RHS, 1) != 0;
+    return bad ? -1 : 0;
+  }
   memset(R->flux_flag, 0, sizeof(int) * 6 * (size_t)m->n);   /* RK.c:11-13 resets flags each stage */
   for (int i = 0; i < m->n; i++) {
-    int rc = residual_leaf(L, p, m, cells, S, SF, i, R, RHS);
+    int rc = residual_leaf(L, p, m, cells, S, SF, i, R, RHS, 0);
     if (rc) return rc;
   }
   return 0;
@@ -970,14 +1000,19 @@ static int stage_case(int rk, int irk) {
 }
 
 static int stage_impl(const layout_t *L, const orc_mesh *m, const orc_cell *cells, const orc_params *p,
-                      int irk, const double *SN, double *S, double *SF, double *RHS, resid_ws *R) {
+                      int irk, const double *SN, double *S, double *SF, double *RHS, resid_ws *R, int par) {
   size_t neq = L->neq;
-  for (int i = 0; i < m->n; i++) faceinterp_leaf(L, p, m, &cells[i], S, i, &SF[(size_t)i * 6 * neq]);
-  int rc = rhs_all(L, p, m, cells, S, SF, RHS, R);
+  const int n = m->n;
+#pragma omp parallel for schedule(static) if (par)
+  for (int i = 0; i < n; i++) faceinterp_leaf(L, p, m, &cells[i], S, i, &SF[(size_t)i * 6 * neq]);
+  int rc = rhs_all(L, p, m, cells, S, SF, RHS, R, par);
   if (rc) return rc;
   int cs = stage_case(p->rk, irk);
-  for (int i = 0; i < m->n; i++) rk_update(L, cs, p->dt, cells[i].Vol, &SN[i * neq], &S[i * neq], &RHS[i * neq]);
-  for (int i = 0; i < m->n; i++) relax_vec(L, p, &S[i * neq]);   /* m4_relaxation_fluid.c:10-18 */
+#pragma omp parallel for schedule(static) if (par)
+  for (int i = 0; i < n; i++) {
+    rk_update(L, cs, p->dt, cells[i].Vol, &SN[i * neq], &S[i * neq], &RHS[i * neq]);
+    relax_vec(L, p, &S[i * neq]);   /* m4_relaxation_fluid.c:10-18 */
+  }
   return 0;
 }
 
@@ -1000,14 +1035,40 @@ int orc_stage(const orc_mesh *m, const orc_cell *cells, const orc_params *p,
   layout_t L = mk_layout(p->nmat);
   double *SF, *RHS; resid_ws R;
   if (alloc_ws(m, L.neq, &SF, &RHS, &R)) return -2;
-  int rc = stage_impl(&L, m, cells, p, irk, SN, S, SF, RHS, &R);
+  int rc = stage_impl(&L, m, cells, p, irk, SN, S, SF, RHS, &R, 0);
   if (SF_out) memcpy(SF_out, SF, sizeof(double) * m->n * 6 * L.neq);
   if (RHS_out) memcpy(RHS_out, RHS, sizeof(double) * m->n * L.neq);
   free_ws(SF, RHS, &R);
   return rc;
 }
 
+static int advance_impl(const orc_mesh *m, const orc_cell *cells, const orc_params *p, double *S, int nsteps,
+                        int par);
+
 int orc_advance(const orc_mesh *m, const orc_cell *cells, const orc_params *p, double *S, int nsteps) {
+  return advance_impl(m, cells, p, S, nsteps, 0);
+}
+
+int orc_advance_par(const orc_mesh *m, const orc_cell *cells, const orc_params *p, double *S, int nsteps,
+                    int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  return advance_impl(m, cells, p, S, nsteps, 1);
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+static int advance_impl(const orc_mesh *m, const orc_cell *cells, const orc_params *p, double *S, int nsteps,
+                        int par) {
   layout_t L = mk_layout(p->nmat);
   size_t nv = (size_t)m->n * L.neq;
   double *SF, *RHS; resid_ws R;
@@ -1017,7 +1078,7 @@ int orc_advance(const orc_mesh *m, const orc_cell *cells, const orc_params *p, d
   for (int step = 0; step < nsteps && !rc; step++) {
     memcpy(SN, S, sizeof(double) * nv);     /* RK(0) */
     for (int irk = 1; irk <= nstages(p->rk) && !rc; irk++)
-      rc = stage_impl(&L, m, cells, p, irk, SN, S, SF, RHS, &R);
+      rc = stage_impl(&L, m, cells, p, irk, SN, S, SF, RHS, &R, par);
   }
   free(SN);
   free_ws(SF, RHS, &R);

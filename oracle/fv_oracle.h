@@ -95,6 +95,12 @@ int  orc_geometry(const orc_mesh *m, orc_cell *cells);
  * status (-1 NaN in RHS, -2 bad input). */
 int  orc_advance(const orc_mesh *m, const orc_cell *cells, const orc_params *p,
                  double *S, int nsteps);
+/* orc_advance with the leaf loops run by OpenMP over nthreads threads (0: the
+ * OpenMP default); mirrored faces are recomputed instead of reused, which gives
+ * the same bits as orc_advance.  orc_num_threads: threads a parallel region uses. */
+int orc_advance_par(const orc_mesh *m, const orc_cell *cells, const orc_params *p, double *S, int nsteps,
+                    int nthreads);
+int orc_num_threads(void);
 
 /* A single RK stage irk (1..) of the chosen table, given SN.  Exposes the
  * intermediate arrays for stage-level tests: SF [n][6][NEQ] and RHS [n][NEQ]

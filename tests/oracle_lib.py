@@ -103,6 +103,8 @@ def lib():
         L.orc_cfl_dt.restype = C.c_double
         L.orc_set_cell_limiter.argtypes = [C.c_void_p]
         L.orc_set_cell_limiter.restype = None
+        L.orc_advance_par.argtypes = [P(orc_mesh), P(orc_cell), P(orc_params), dp, C.c_int, C.c_int]
+        L.orc_num_threads.argtypes = []
         L.orc_cons2prim.argtypes = [P(orc_params), dp, dp]
         L.orc_cons2prim.restype = None
         L.orc_prim2cons.argtypes = [P(orc_params), dp, dp]
@@ -165,6 +167,15 @@ class OracleMesh:
         rc = lib().orc_advance(C.byref(self.om), self.cells, C.byref(p), _dp(S), nsteps)
         if rc != 0:
             raise RuntimeError(f"orc_advance status {rc}")
+        return S
+
+    def advance_par(self, params, S, nsteps, nthreads=0):
+        """orc_advance_par: the same steps with OpenMP over the leaves (same bits)."""
+        p = params_from(params)
+        S = np.ascontiguousarray(S, dtype=np.float64).copy()
+        rc = lib().orc_advance_par(C.byref(self.om), self.cells, C.byref(p), _dp(S), nsteps, nthreads)
+        if rc != 0:
+            raise RuntimeError(f"orc_advance_par status {rc}")
         return S
 
     def stage(self, params, irk, SN, S):
