@@ -312,14 +312,20 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon_fast(int n, Params P, Geo G,
 #pragma unroll
   for (int k = 0; k < L::N3; k++) Araw[k] = Sc[L::I3 + k];
   avf_limit<NM, double>(a, Sc);
+  // the six neighbour ids are requested together with the cell state, so each
+  // axis waits on one round trip (the stencil states) instead of two
+  const int nb0 = nbr[i], nb1 = nbr[(size_t)n + i], nb2 = nbr[(size_t)2 * n + i];
+  const int nb3 = nbr[(size_t)3 * n + i], nb4 = nbr[(size_t)4 * n + i], nb5 = nbr[(size_t)5 * n + i];
 #pragma unroll 1
   for (int ax = 0; ax < 3; ax++) {
     const int fa = (ax == 0) ? 0 : (ax == 1 ? 1 : 4);
     const int fb = FCOPP_D[fa];
     const int ka = face_kind(inf, fa), kb = face_kind(inf, fb);
+    const int na = (ax == 0) ? nb0 : (ax == 1 ? nb1 : nb4);
+    const int nbb = (ax == 0) ? nb2 : (ax == 1 ? nb3 : nb5);
     double Xa[L::NEQ], Xb[L::NEQ];
-    stencil_value<NM>(S, n, i, ka, nbr[(size_t)fa * n + i], fine4, Sc, Araw, a, Xa);
-    stencil_value<NM>(S, n, i, kb, nbr[(size_t)fb * n + i], fine4, Sc, Araw, a, Xb);
+    stencil_value<NM>(S, n, i, ka, na, fine4, Sc, Araw, a, Xa);
+    stencil_value<NM>(S, n, i, kb, nbb, fine4, Sc, Araw, a, Xb);
     double dcf_a, dcc_a, dcf_b, dcc_b;
     dist_of(G, ka, lvl, dcf_a, dcc_a);
     dist_of(G, kb, lvl, dcf_b, dcc_b);
