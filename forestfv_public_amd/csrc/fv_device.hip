@@ -353,13 +353,13 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon_fast(int n, Params P, Geo G,
         }
         vf_limit_face<NM, double>(a, F);
       }
-      fvf::relax_fast<NM>(P, M, F);
       if (MIXED) {
         float rec[fvf::Rec<NM>::N];
-        fvf::to_record<NM>(M, a, F, rec);
+        fvf::relax_to_record<NM>(P, M, a, F, rec);
 #pragma unroll
         for (int q = 0; q < fvf::Rec<NM>::N; q++) SFm[((size_t)f * fvf::Rec<NM>::N + q) * n + i] = rec[q];
       } else {
+        fvf::relax_fast<NM>(P, M, F);
 #pragma unroll
         for (int v = 0; v < L::NEQ; v++) SF[((size_t)f * L::NEQ + v) * n + i] = F[v];
       }
@@ -718,16 +718,18 @@ FV_DEV void recon_one(const Params &P, const fvf::Mat<NM, double> &M, const Geo 
     }
     vf_limit_face<NM, double>(a, F);
   }
-  fvf::relax_fast<NM>(P, M, F);
+  // the mixed path relaxes while it forms the FP32 record (state_sol)
+  if constexpr (!MIXED) fvf::relax_fast<NM>(P, M, F);
 }
 
-// face state -> solution used by HLLC (through the FP32 record when MIXED)
+// face state -> solution used by HLLC (through the FP32 record when MIXED,
+// where the state arrives unrelaxed and is relaxed here, as k_recon_fast does)
 template <int NM, bool MIXED, typename R>
-FV_DEV void state_sol(const fvf::Mat<NM, double> &Md, const fvf::Mat<NM, R> &M, double a, const double *F,
+FV_DEV void state_sol(const Params &P, const fvf::Mat<NM, double> &Md, const fvf::Mat<NM, R> &M, double a, double *F,
                       fvf::SolF<NM, R> &s) {
   if constexpr (MIXED) {
     float rec[fvf::Rec<NM>::N];
-    fvf::to_record<NM>(Md, a, F, rec);
+    fvf::relax_to_record<NM>(P, Md, a, F, rec);
     fvf::fill_rec<NM>(M, rec, s);
   } else {
     fvf::fill_cons<NM, R>(Md, a, F, s);
@@ -829,13 +831,13 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
         }
         stencil_value<NM>(S, n, nb, kj, nbr[(size_t)f * n + nb], fine4, Xf, Arj, a, X2);
         recon_one<NM, MIXED>(P, Md, G, a, leaf_limiter(P, lim, nb), lvl, kjo, kj, Xf, Sc, X2, Fnb);
-        state_sol<NM, MIXED, R>(Md, M, a, Fnb, sA);
+        state_sol<NM, MIXED, R>(P, Md, M, a, Fnb, sA);
       }
       {
         double Xo[L::NEQ], Fown[L::NEQ];
         stencil_value<NM>(S, n, i, kopp, nbr[(size_t)fo * n + i], fine4, Sc, Araw, a, Xo);
         recon_one<NM, MIXED>(P, Md, G, a, limiter, lvl, kind, kopp, Sc, Xf, Xo, Fown);
-        state_sol<NM, MIXED, R>(Md, M, a, Fown, sB);
+        state_sol<NM, MIXED, R>(P, Md, M, a, Fown, sB);
       }
       R out[L::NEQ], un, vn, wn;
       const double A = G.area[lvl];
@@ -874,7 +876,7 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
         double Xo[L::NEQ], Fown[L::NEQ];
         stencil_value<NM>(S, n, i, kopp, nbr[(size_t)fo * n + i], fine4, Sc, Araw, a, Xo);
         recon_one<NM, MIXED>(P, Md, G, a, limiter, lvl, kind, kopp, Sc, Xf, Xo, Fown);
-        state_sol<NM, MIXED, R>(Md, M, a, Fown, sL);
+        state_sol<NM, MIXED, R>(P, Md, M, a, Fown, sL);
       }
       for (int k = 0; k < Nneig; ++k) {
         int bcode = 0;
@@ -894,7 +896,7 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
           const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
           double Fnb[L::NEQ];
           face_state_of<NM, MIXED>(P, Md, G, a, S, n, nbr, info, fine4, lim, j, fo, Fnb);
-          state_sol<NM, MIXED, R>(Md, M, a, Fnb, sR);
+          state_sol<NM, MIXED, R>(P, Md, M, a, Fnb, sR);
         }
         R out[L::NEQ], un, vn, wn;
         fvf::hllc<NM, R>(M, P.fix, fr, bcode, sL, sR, out, un, vn, wn);

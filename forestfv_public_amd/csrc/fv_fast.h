@@ -289,6 +289,66 @@ FV_DEV void relax_fast(const Params &Pr, const Mat<NM, double> &M, double *vec) 
   for (int k = 0; k < NM; ++k) vec[L::I4 + k] = are[k];
 }
 
+// relax_fast for PRELAX=1 (two phases) followed by the FP32 face record,
+// written from the relaxed quantities: both phasic pressures equal the
+// recovered pressure, and rE - Pi uses the Pi of the recovery, so the record
+// needs no second conversion of the face state.  vec is relaxed in place too.
+template <int NM>
+FV_DEV void relax_record(const Params &Pr, const Mat<NM, double> &M, double *vec, float *rec) {
+  using L = Lay<NM>;
+  using Q = Rec<NM>;
+  const double a = Pr.amin / 100.0;
+  double ar[NM], are[NM], avf[NM + 1], rho = 0.0;
+#pragma unroll
+  for (int k = 0; k < NM; ++k) { ar[k] = vec[k]; are[k] = vec[L::I4 + k]; rho += ar[k]; }
+  avf[0] = vec[L::I3];
+  fvk::vf_limit_relax<NM>(a, avf);
+  const double ia0 = rcp(avf[0]), ia1 = rcp(avf[1]);
+  const double p00 = are[0] * M.Gm1[0] * ia0 - M.GP[0];
+  const double p01 = are[1] * M.Gm1[1] * ia1 - M.GP[1];
+  const double w0 = avf[0] * M.iG[0], w1 = avf[1] * M.iG[1];
+  const double iw = rcp(w0 + w1);
+  const double A1 = w0 * (p00 + M.P[0]) * iw, A2 = w1 * (p01 + M.P[1]) * iw;
+  const double d = A2 - A1 - (M.P[1] - M.P[0]);
+  const double pe = 0.5 * (A1 + A2 - (M.P[0] + M.P[1])) + sqrt(0.25 * d * d + A1 * A2);
+  avf[0] = avf[0] * (p00 + M.GP[0] + M.Gm1[0] * pe) * rcp(M.G[0] * pe + M.GP[0]);
+  avf[1] = avf[1] * (p01 + M.GP[1] + M.Gm1[1] * pe) * rcp(M.G[1] * pe + M.GP[1]);
+  fvk::vf_limit_relax<NM>(a, avf);
+  const double irho = rcp(rho);
+  const double u = vec[L::I1] * irho, v = vec[L::I1 + 1] * irho, w = vec[L::I1 + 2] * irho;
+  const double e_hyd = vec[L::I2] * irho - 0.5 * (u * u + v * v + w * w);
+  double num = 0.0, den = 0.0;
+#pragma unroll
+  for (int k = 0; k < NM; ++k) { num += avf[k] * M.C[k]; den += avf[k] * M.iGm1[k]; }
+  const double pr = (rho * e_hyd - num) * rcp(den);
+#pragma unroll
+  for (int k = 0; k < NM; ++k) are[k] = avf[k] * (pr + M.GP[k]) * M.iGm1[k];
+  vec[L::I3] = avf[0];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) vec[L::I4 + k] = are[k];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) {
+    rec[Q::RA + k] = (float)ar[k];
+    rec[Q::P + k] = (float)pr;
+    rec[Q::ARE + k] = (float)are[k];
+  }
+  rec[Q::AL] = (float)avf[0];
+  rec[Q::U] = (float)u;
+  rec[Q::U + 1] = (float)v;
+  rec[Q::U + 2] = (float)w;
+  rec[Q::REP] = (float)(vec[L::I2] - num);
+}
+
+// relaxation of a reconstructed face state and its FP32 record
+template <int NM>
+FV_DEV void relax_to_record(const Params &Pr, const Mat<NM, double> &M, double amin100, double *vec, float *rec) {
+  if constexpr (NM == 2) {
+    if (Pr.prelax == 1) { relax_record<NM>(Pr, M, vec, rec); return; }
+  }
+  relax_fast<NM>(Pr, M, vec);
+  to_record<NM>(M, amin100, vec, rec);
+}
+
 // limited slope (m4_reconctruct.c:105-128) with reciprocal distances; a
 // boundary on either side gives zero slope, as the reference's bc test and
 // its 0/0 on the zero boundary distance do
