@@ -51,7 +51,7 @@ struct fv_mlp {
   // batch scratch
   int bcap = 0;
   float *bx = nullptr, *bs = nullptr, *z1 = nullptr, *h1 = nullptr, *z2 = nullptr, *h2 = nullptr, *o = nullptr;
-  float *d3 = nullptr, *d2 = nullptr, *d1 = nullptr, *lossv = nullptr;
+  float *d3 = nullptr, *d2 = nullptr, *d1 = nullptr, *lossv = nullptr, *wpart = nullptr;
   int *bc = nullptr, *bidx = nullptr;
   // per-leaf stencil fields
   long fcap = 0;
@@ -231,20 +231,39 @@ __global__ void k_bwd_data(int B, int K, int N, const float *__restrict__ dZ, co
   dZp[t] = dh * (s + z * s * (1.f - s));
 }
 
-// dW[k][n] = sum_b X[b][k] dZ[b][n] (rows k < K), dbias[n] = sum_b dZ[b][n] (row K)
-__global__ void k_wgrad(int B, int K, int N, const float *__restrict__ X, int ldx, const float *__restrict__ dZ,
-                        float *__restrict__ dW, float *__restrict__ db) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (K + 1) * N) return;
-  const int k = t / N, nn = t % N;
+// weight gradient in two deterministic steps: partial sums over chunks of
+// MLP_WCHUNK batch rows, then the chunks summed in order.
+// part[c][k][n] = sum_{b in chunk c} X[b][k] dZ[b][n] (rows k < K) and
+// sum_{b in chunk c} dZ[b][n] (row K, the bias)
+#define MLP_WCHUNK 64
+__global__ void k_wgrad_part(int B, int K, int N, const float *__restrict__ X, int ldx, const float *__restrict__ dZ,
+                             float *__restrict__ part) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = (K + 1) * N;
+  const int nch = (B + MLP_WCHUNK - 1) / MLP_WCHUNK;
+  if (t >= (long)per * nch) return;
+  const int c = (int)(t / per), r = (int)(t % per);
+  const int k = r / N, nn = r % N;
+  const int b0 = c * MLP_WCHUNK, b1 = min(B, b0 + MLP_WCHUNK);
   float acc = 0.f;
   if (k < K) {
-    for (int b = 0; b < B; b++) acc += X[(size_t)b * ldx + k] * dZ[b * N + nn];
-    dW[k * N + nn] = acc;
+    for (int b = b0; b < b1; b++) acc += X[(size_t)b * ldx + k] * dZ[b * N + nn];
   } else {
-    for (int b = 0; b < B; b++) acc += dZ[b * N + nn];
-    db[nn] = acc;
+    for (int b = b0; b < b1; b++) acc += dZ[b * N + nn];
   }
+  part[t] = acc;
+}
+
+__global__ void k_wgrad_sum(int B, int K, int N, const float *__restrict__ part, float *__restrict__ dW,
+                            float *__restrict__ db) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = (K + 1) * N;
+  if (t >= per) return;
+  const int nch = (B + MLP_WCHUNK - 1) / MLP_WCHUNK;
+  float acc = 0.f;
+  for (int c = 0; c < nch; c++) acc += part[(size_t)c * per + t];
+  if (t < K * N) dW[t] = acc;
+  else db[t - K * N] = acc;
 }
 
 __global__ void k_adam(int np, float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
@@ -445,7 +464,7 @@ static inline dim3 g1(long n, int b = 256) { return dim3((unsigned)((n + b - 1) 
 
 static int ensure_batch(fv_mlp *m, int B) {
   if (B <= m->bcap) return FV_OK;
-  float **fl[] = {&m->bx, &m->bs, &m->z1, &m->h1, &m->z2, &m->h2, &m->o, &m->d3, &m->d2, &m->d1, &m->lossv};
+  float **fl[] = {&m->bx, &m->bs, &m->z1, &m->h1, &m->z2, &m->h2, &m->o, &m->d3, &m->d2, &m->d1, &m->lossv, &m->wpart};
   for (auto p : fl) { HIPCHK(hipFree(*p)); *p = nullptr; }
   HIPCHK(hipFree(m->bc)); HIPCHK(hipFree(m->bidx));
   HIPCHK(hipMalloc(&m->bx, sizeof(float) * B * MLP_K));
@@ -459,10 +478,18 @@ static int ensure_batch(fv_mlp *m, int B) {
   HIPCHK(hipMalloc(&m->d2, sizeof(float) * B * MLP_H));
   HIPCHK(hipMalloc(&m->d1, sizeof(float) * B * MLP_H));
   HIPCHK(hipMalloc(&m->lossv, sizeof(float) * B));
+  HIPCHK(hipMalloc(&m->wpart, sizeof(float) * (size_t)((B + MLP_WCHUNK - 1) / MLP_WCHUNK) * (MLP_K + 1) * MLP_H));
   HIPCHK(hipMalloc(&m->bc, sizeof(int) * B));
   HIPCHK(hipMalloc(&m->bidx, sizeof(int) * B));
   m->bcap = B;
   return FV_OK;
+}
+
+static void wgrad(fv_mlp *m, int B, int K, int N, const float *X, int ldx, const float *dZ, float *dW, float *db) {
+  const long per = (long)(K + 1) * N;
+  const long nch = (B + MLP_WCHUNK - 1) / MLP_WCHUNK;
+  hipLaunchKernelGGL(k_wgrad_part, g1(per * nch), dim3(256), 0, 0, B, K, N, X, ldx, dZ, m->wpart);
+  hipLaunchKernelGGL(k_wgrad_sum, g1(per), dim3(256), 0, 0, B, K, N, m->wpart, dW, db);
 }
 
 // forward + backward on the batch already in bx/bc/bs; gradient in m->g, loss summed to *loss
@@ -475,14 +502,11 @@ static int fwd_bwd(fv_mlp *m, int B, float *loss, bool backward) {
   hipLaunchKernelGGL(k_loss, g1(B), dim3(256), 0, 0, B, m->o, m->bc, m->bs, m->cw[0], m->cw[1], m->cw[2], m->d3, m->lossv);
   if (backward) {
     float *G = m->g;
-    hipLaunchKernelGGL(k_wgrad, g1((long)(MLP_H + 1) * MLP_NO), dim3(256), 0, 0, B, MLP_H, MLP_NO, m->h2, MLP_H, m->d3,
-                       G + MLP_OFF_W3, G + MLP_OFF_B3);
+    wgrad(m, B, MLP_H, MLP_NO, m->h2, MLP_H, m->d3, G + MLP_OFF_W3, G + MLP_OFF_B3);
     hipLaunchKernelGGL(k_bwd_data, g1((long)B * MLP_H), dim3(256), 0, 0, B, MLP_H, MLP_NO, m->d3, W3, m->z2, m->d2);
-    hipLaunchKernelGGL(k_wgrad, g1((long)(MLP_H + 1) * MLP_H), dim3(256), 0, 0, B, MLP_H, MLP_H, m->h1, MLP_H, m->d2,
-                       G + MLP_OFF_W2, G + MLP_OFF_B2);
+    wgrad(m, B, MLP_H, MLP_H, m->h1, MLP_H, m->d2, G + MLP_OFF_W2, G + MLP_OFF_B2);
     hipLaunchKernelGGL(k_bwd_data, g1((long)B * MLP_H), dim3(256), 0, 0, B, MLP_H, MLP_H, m->d2, W2, m->z1, m->d1);
-    hipLaunchKernelGGL(k_wgrad, g1((long)(MLP_K + 1) * MLP_H), dim3(256), 0, 0, B, MLP_K, MLP_H, m->bx, MLP_K, m->d1,
-                       G, G + MLP_OFF_B1);
+    wgrad(m, B, MLP_K, MLP_H, m->bx, MLP_K, m->d1, G, G + MLP_OFF_B1);
   }
   HIPCHK(hipGetLastError());
   if (loss) {
@@ -531,7 +555,8 @@ extern "C" int fv_mlp_create(fv_mlp **out, unsigned seed) {
 extern "C" int fv_mlp_destroy(fv_mlp *m) {
   if (!m) return FV_OK;
   void *ps[] = {m->w, m->g, m->m, m->v, m->w1t, m->w2t, m->w3t, m->X, m->smooth, m->cls, m->bx, m->bs, m->z1, m->h1,
-                m->z2, m->h2, m->o, m->d3, m->d2, m->d1, m->lossv, m->bc, m->bidx, m->ix, m->iout, m->ismooth, m->fld, m->snd};
+                m->z2, m->h2, m->o, m->d3, m->d2, m->d1, m->lossv, m->bc, m->bidx, m->ix, m->iout, m->ismooth, m->fld, m->snd,
+                m->wpart};
   for (void *p : ps) (void)hipFree(p);
   delete m;
   return FV_OK;
