@@ -359,3 +359,57 @@ def test_fused_stage_matches_two_kernel_path(name, prec):
         err = rel_l1(b, a)
         say(f"[fused] not bit-identical: {describe(b, a)}; rel L1 {err.max():.2e}")
         assert err.max() < (1e-14 if prec == "fp64" else 1e-8)
+
+
+def test_three_material_mixed_and_fused_paths():
+    """The NM = 3 instantiations of the mixed-precision and fused kernels: mixed
+    precision within 1e-4 relative L1 of the CPU path in density and pressure over
+    20 steps, and the fused FP64 stage equal to the two-kernel path."""
+    p, m, S = three_material_case(level=4, prelax=3)
+    So = O.OracleMesh(m).advance(p, S, 20)
+    Sm = run_prec(p, m, S, 20, F.PREC_MIXED)
+    rho_o, rho_m = So[:, 0:3].sum(1), Sm[:, 0:3].sum(1)
+    e_rho = np.abs(rho_m - rho_o).sum() / np.abs(rho_o).sum()
+    e = rel_l1(Sm, So)
+    say(f"[three mixed] rel L1 rho {e_rho:.3e}; per field {e}")
+    assert e_rho <= 1e-4 and np.all(e[6:7] <= 1e-4)
+    outs = []
+    for fused in (False, True):
+        dev = F.Device(p, m, prec=F.PREC_FP64)
+        dev.set_fused(fused)
+        dev.set_state(S)
+        for irk in (1, 2, 3):
+            dev.stage(irk)
+        outs.append(dev.get_state())
+        dev.close()
+    assert rel_l1(outs[1], outs[0]).max() < 1e-14
+
+
+def test_single_leaf_mesh():
+    """One root, one leaf, six boundary faces: stages bit-identical to the CPU path."""
+    p = sod_params(prelax=1, bc=(1, 2, 4, 5, 4, 4))
+    m = F.Mesh.uniform((1, 1, 1), (0.0, 0.0, 0.0), 1.0, 1, list(p.bc))
+    S = np.array([F.init_pure(p, (1.0, 0.125), 0, 1.0, (0.3, -0.2, 0.1))])
+    outs = run_stages(p, m, S, 3)
+    for Sg, So, SFg, SFo in outs:
+        assert np.array_equal(Sg, So) and np.array_equal(SFg, SFo)
+
+
+def test_reattach_to_larger_and_smaller_meshes():
+    """fv_dev_reattach grows and shrinks the device buffers; each mesh then steps
+    exactly as on a fresh context."""
+    cases = [perturbed_sod(3), perturbed_sod(5), perturbed_sod(4)]
+    p0 = cases[0][0]
+    dev = F.Device(p0, cases[0][1], prec=F.PREC_EXACT)
+    for p, m, S in cases:
+        dev.reattach(m)
+        dev.set_dt(p.dt)
+        dev.set_state(S)
+        dev.advance(2)
+        A = dev.get_state()
+        fresh = F.Device(p, m, prec=F.PREC_EXACT)
+        fresh.set_state(S)
+        fresh.advance(2)
+        assert np.array_equal(A, fresh.get_state()), m.n
+        fresh.close()
+    dev.close()

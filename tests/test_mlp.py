@@ -262,3 +262,23 @@ def test_fused_stencil_inference_matches_feature_rows():
     say(f"[mlp stencil] classes {np.bincount(c, minlength=3)}, smooth {int((out[:, 3] > 0).sum())}")
     assert np.array_equal(split, t)
     assert np.array_equal(sm, (out[:, 3] > 0).astype(np.uint8))
+
+
+@pytest.mark.gpu
+def test_learned_indicator_on_a_mesh_smaller_than_a_wave():
+    """Eight leaves (a partial 16-row MFMA tile and a partial block): the fused
+    stencil inference still equals k_mlp_infer on the stored rows."""
+    mlp = F.MLP(seed=43)
+    mlp.set_weights(rand_weights(44))
+    p, m, S = bubble_case(level=2, prelax=1)
+    dev = F.Device(p, m)
+    dev.set_state(S)
+    crit = F.make_crit(uniform_on=1, uniform_lvl=2, prs_on=1, prs_lvl=3, prs_thr=0.05)
+    mlp.sample(dev, crit)
+    X, _, _ = mlp.dataset()
+    out = mlp.infer(X)
+    _, split, sm = dev.indicator_mlp(mlp, crit)
+    dev.close()
+    mlp.close()
+    assert m.n == 8 and X.shape == (8, 112)
+    assert np.array_equal(sm, (out[:, 3] > 0).astype(np.uint8))
