@@ -161,6 +161,7 @@ def lib():
             "fv_dev_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
             "fv_dev_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
             "fv_dev_indicator": (C.c_int, [C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int)]),
+            "fv_dev_interface_location": (C.c_int, [C.c_void_p, P(fv_crit), P(C.c_int)]),
             "fv_dev_time_stages": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
             "fv_dev_size": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
             "fv_dev_last_error": (C.c_char_p, []),
@@ -471,6 +472,12 @@ class Device:
         split = np.zeros(self.n, dtype=np.int32)
         _check(lib().fv_dev_indicator(self._h, C.byref(crit), _ip(SI), _ip(split)), "fv_dev_indicator")
         return SI, split
+
+    def interface_location(self, crit):
+        """SI from the current state; the following stages use it (loop.c:23)."""
+        SI = np.zeros((self.n, 2), dtype=np.int32)
+        _check(lib().fv_dev_interface_location(self._h, C.byref(crit), _ip(SI)), "fv_dev_interface_location")
+        return SI
 
     def set_limiter_flags(self, lim):
         lim = np.ascontiguousarray(lim, dtype=np.uint8)

@@ -1065,6 +1065,25 @@ __global__ __launch_bounds__(FV_BLOCK) void k_si_layer(int n, int lvl, const int
   SIout[2 * i + 1] = si[1];
 }
 
+// New SI: the info word's interface bit and the mirror / same choice of
+// every equal-level face, by the rule of the host packer (a face is computed
+// once, by the later leaf, only when neither leaf is an interface leaf).
+__global__ __launch_bounds__(FV_BLOCK) void k_refresh_kinds(int n, const int *__restrict__ nbr,
+                                                            const int *__restrict__ SI, uint32_t *__restrict__ info) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int si = SI[2 * i];
+  uint32_t inf = (info[i] & ~16u) | (si != 0 ? 16u : 0u);
+  for (int f = 0; f < 6; f++) {
+    const int kind = face_kind(inf, f);
+    if (kind != FV_FK_SAME && kind != FV_FK_MIRROR) continue;
+    const int j = nbr[(size_t)f * n + i];
+    const uint32_t nk = (si == 0 && SI[2 * j] == 0 && j < i) ? FV_FK_MIRROR : FV_FK_SAME;
+    inf = (inf & ~(7u << (8 + 3 * f))) | (nk << (8 + 3 * f));
+  }
+  info[i] = inf;
+}
+
 template <int NM>
 __global__ __launch_bounds__(FV_BLOCK) void k_split(int n, Params P, CritDev C, const double *__restrict__ S,
                                                     const int *__restrict__ nbr, const uint32_t *__restrict__ info,
@@ -1436,17 +1455,12 @@ extern "C" int fv_dev_cfl_dt(fv_dev *c, double cfl, double *dt) {
   return FV_OK;
 }
 
-int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev) {
-  if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
+// interface_location (interface_location.c:7-86): interface leaves, then
+// imax-2 layers grown one face neighbour at a time; returns the buffer that
+// holds the final SI [n][2].
+static int *launch_interface_location(fv_dev *c, const fv_crit *cr) {
   const int n = c->n;
   const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
-  CritDev C;
-  C.uniform_on = cr->uniform_on; C.uniform_lvl = cr->uniform_lvl;
-  C.prs_on = cr->prs_on; C.prs_lvl = cr->prs_lvl; C.prs_thr = cr->prs_thr;
-  C.rho_on = cr->rho_on; C.rho_lvl = cr->rho_lvl; C.rho_thr = cr->rho_thr;
-  C.vel_on = cr->vel_on; C.vel_lvl = cr->vel_lvl; C.vel_thr = cr->vel_thr;
-  C.inter0_on = cr->inter0_on; C.inter0_lvl = cr->inter0_lvl; C.inter0_lrs = cr->inter0_lrs;
-  C.inter1_on = cr->inter1_on; C.inter1_lvl = cr->inter1_lvl; C.inter1_lrs = cr->inter1_lrs;
   int imax = 0;   // ADAPT_INTER_MAX (m4_input.c:52-54)
   if (cr->inter0_on == 1 && cr->inter1_on == 1) imax = cr->inter0_lrs + cr->inter1_lrs;
   else if (cr->inter0_on == 1 && cr->inter1_on == 0) imax = cr->inter0_lrs;
@@ -1459,6 +1473,21 @@ int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev) {
     hipLaunchKernelGGL(k_si_layer, grid, block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur, nxt);
     int *t = cur; cur = nxt; nxt = t;
   }
+  return cur;
+}
+
+int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev) {
+  if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
+  const int n = c->n;
+  const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
+  CritDev C;
+  C.uniform_on = cr->uniform_on; C.uniform_lvl = cr->uniform_lvl;
+  C.prs_on = cr->prs_on; C.prs_lvl = cr->prs_lvl; C.prs_thr = cr->prs_thr;
+  C.rho_on = cr->rho_on; C.rho_lvl = cr->rho_lvl; C.rho_thr = cr->rho_thr;
+  C.vel_on = cr->vel_on; C.vel_lvl = cr->vel_lvl; C.vel_thr = cr->vel_thr;
+  C.inter0_on = cr->inter0_on; C.inter0_lvl = cr->inter0_lvl; C.inter0_lrs = cr->inter0_lrs;
+  C.inter1_on = cr->inter1_on; C.inter1_lvl = cr->inter1_lvl; C.inter1_lrs = cr->inter1_lrs;
+  int *cur = launch_interface_location(c, cr);
   if (c->nmat == 2)
     hipLaunchKernelGGL(k_split<2>, grid, block, 0, c->stream, n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, c->split);
   else
@@ -1490,6 +1519,19 @@ extern "C" int fv_dev_indicator(fv_dev *c, const fv_crit *cr, int *SI, int *spli
   if (rc != FV_OK) return rc;
   if (SI && (rc = fv_copy_to_host(c, SI, cur, sizeof(int) * 2 * (size_t)c->n)) != FV_OK) return rc;
   if (split && (rc = fv_copy_to_host(c, split, c->split, sizeof(int) * (size_t)c->n)) != FV_OK) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FV_OK;
+}
+
+extern "C" int fv_dev_interface_location(fv_dev *c, const fv_crit *cr, int *SI) {
+  if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
+  const int n = c->n;
+  int *cur = launch_interface_location(c, cr);
+  hipLaunchKernelGGL(k_refresh_kinds, dim3((n + FV_BLOCK - 1) / FV_BLOCK), dim3(FV_BLOCK), 0, c->stream, n, c->nbr,
+                     (const int *)cur, c->info);
+  HIPCHK(hipGetLastError());
+  int rc;
+  if (SI && (rc = fv_copy_to_host(c, SI, cur, sizeof(int) * 2 * (size_t)n)) != FV_OK) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return FV_OK;
 }

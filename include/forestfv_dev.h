@@ -79,6 +79,13 @@ int  fv_dev_get_faces(fv_dev *ctx, double *SF);
  * SI: [n][2] out, split: [n] out (either may be NULL). */
 int  fv_dev_indicator(fv_dev *ctx, const fv_crit *c, int *SI, int *split);
 
+/* interface_location as the time loop calls it after every step
+ * (loop.c:23, interface_location.c:7-86): recomputes SI from ctx's state and
+ * makes the following stages use it (the interface bit and the computed-once
+ * choice of equal-level faces, as fv_mesh_pack derives them from the mesh's
+ * SI).  SI: [n][2] out, may be NULL. */
+int  fv_dev_interface_location(fv_dev *ctx, const fv_crit *c, int *SI);
+
 /* Per-leaf limiter when fv_params.limiter == 2: 0 minmod, 1 MC; written by
  * fv_dev_indicator_mlp from the learned smooth flag, or by the caller. */
 int  fv_dev_set_limiter_flags(fv_dev *ctx, const uint8_t *lim);

@@ -220,6 +220,38 @@ def test_indicator_flags_bitwise():
     assert np.array_equal(splitg, splito)
 
 
+@pytest.mark.parametrize("prec", ["exact", "fp64"])
+def test_interface_location_every_step_as_the_time_loop(prec):
+    """loop.c:21-23: a step, then interface_location; the next step's
+    computed-once faces follow the new SI.  The oracle is driven the same way."""
+    # dyadic cube: the oracle's node-based geometry equals the device tables
+    p, m, S = bubble_case(level=5, prelax=1, L=2.0 ** -8)
+    crit = F.make_crit(inter0_on=1, inter0_lvl=6, inter0_lrs=3)
+    dev = F.Device(p, m, prec=F.PREC_EXACT if prec == "exact" else F.PREC_FP64)
+    dev.set_state(S)
+    om = O.OracleMesh(m)
+    So = S.copy()
+    SIo = om.interface_location(p, crit, So)
+    SIg = dev.interface_location(crit)
+    assert np.array_equal(SIg, SIo)
+    om.set_SI(SIo)
+    for step in range(6):
+        dev.advance(1)
+        So = om.advance(p, So, 1)
+        SIg = dev.interface_location(crit)
+        SIo = om.interface_location(p, crit, So)
+        om.set_SI(SIo)
+        assert np.array_equal(SIg, SIo), f"SI after step {step}"
+    Sg = dev.get_state()
+    dev.close()
+    say(f"[interface every step] {prec} SI>0: {(SIo[:, 0] > 0).sum()} of {m.n}")
+    assert (SIo[:, 0] > 0).sum() > 0
+    if prec == "exact":
+        assert np.array_equal(Sg, So), describe(Sg, So)
+    else:
+        assert np.all(rel_l1(Sg, So) <= L1_TOL)
+
+
 def test_nan_is_a_status_not_a_crash():
     p, m, S = sod_case(level=3)
     S[5, 0] = np.nan
