@@ -115,6 +115,19 @@ class fv_pack(C.Structure):
     ]
 
 
+class fv_m5_params(C.Structure):
+    _fields_ = [
+        ("nmat", C.c_int),
+        ("gamma", C.c_double * FV_MAXMAT),
+        ("pinf", C.c_double * FV_MAXMAT),
+        ("mush", C.c_double * FV_MAXMAT),
+        ("amin", C.c_double),
+        ("avf_lim", C.c_double),
+        ("dt", C.c_double),
+        ("zerou", C.c_int * 3),
+    ]
+
+
 _lib = None
 
 
@@ -184,6 +197,12 @@ def lib():
             "fv_mlp_infer": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int, P(C.c_float)]),
             "fv_dev_indicator_mlp": (C.c_int, [C.c_void_p, C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int),
                                                P(C.c_uint8)]),
+            "fv_m5_neq": (C.c_int, [C.c_int]),
+            "fv_m5_gfm_faces": (C.c_int, [P(fv_m5_params), C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
+                                          P(C.c_double), P(C.c_double), P(C.c_double)]),
+            "fv_m5_rk": (C.c_int, [P(fv_m5_params), P(fv_pack), C.c_int, P(C.c_double), P(C.c_double),
+                                   P(C.c_double), P(C.c_uint8)]),
+            "fv_m5_gfm_rel": (C.c_int, [P(fv_m5_params), P(fv_pack), P(C.c_double), P(C.c_double), P(C.c_uint8)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -224,6 +243,58 @@ def make_params(nmat=2, gamma=(1.4, 1.4), pinf=(0.0, 0.0), amin=1e-8, avf_lim=0.
     p.fix = fix
     p.primtv = primtv
     return p
+
+
+def m5_neq(nmat):
+    return 3 * nmat + 12
+
+
+def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush=(2.6e10, 0.0, 0.0), amin=1e-6,
+                   avf_lim=0.5, dt=1e-9, zerou=(0, 0, 0)):
+    """Model-5 material table (MATERGAMA, MATERPINF, MATERMUSH) and stage constants."""
+    p = fv_m5_params()
+    p.nmat = nmat
+    for k in range(nmat):
+        p.gamma[k], p.pinf[k], p.mush[k] = gamma[k], pinf[k], mush[k]
+    p.amin, p.avf_lim, p.dt = amin, avf_lim, dt
+    for k in range(3):
+        p.zerou[k] = zerou[k]
+    return p
+
+
+def m5_gfm_faces(params, UL, UR, frame, area):
+    """GFM flux of interface sub-faces (fv_m5_gfm_faces): returns flux [nf][NEQ], unc [nf][3]."""
+    UL = np.ascontiguousarray(UL, dtype=np.float64)
+    UR = np.ascontiguousarray(UR, dtype=np.float64)
+    fr = np.ascontiguousarray(frame, dtype=np.float64)
+    ar = np.ascontiguousarray(area, dtype=np.float64)
+    nf = UL.shape[0]
+    flux = np.zeros((nf, m5_neq(params.nmat)))
+    unc = np.zeros((nf, 3))
+    _check(lib().fv_m5_gfm_faces(C.byref(params), nf, _dp(UL), _dp(UR), _dp(fr), _dp(ar), _dp(flux), _dp(unc)),
+           "fv_m5_gfm_faces")
+    return flux, unc
+
+
+def m5_rk(params, pack, irk, SN, S, RHS, gfm_face):
+    """Model-5 RK stage (fv_m5_rk): returns (SN, S) after the stage."""
+    SN = np.ascontiguousarray(SN, dtype=np.float64).copy()
+    S = np.ascontiguousarray(S, dtype=np.float64).copy()
+    R = np.ascontiguousarray(RHS, dtype=np.float64)
+    gf = np.ascontiguousarray(gfm_face, dtype=np.uint8)
+    _check(lib().fv_m5_rk(C.byref(params), C.byref(pack.pk), irk, _dp(SN), _dp(S), _dp(R),
+                          gf.ctypes.data_as(C.POINTER(C.c_uint8))), "fv_m5_rk")
+    return SN, S
+
+
+def m5_gfm_rel(params, pack, S_before, S, gfm_face):
+    """Re-projection after relaxation (fv_m5_gfm_rel): returns the new S."""
+    Sb = np.ascontiguousarray(S_before, dtype=np.float64)
+    S = np.ascontiguousarray(S, dtype=np.float64).copy()
+    gf = np.ascontiguousarray(gfm_face, dtype=np.uint8)
+    _check(lib().fv_m5_gfm_rel(C.byref(params), C.byref(pack.pk), _dp(Sb), _dp(S),
+                               gf.ctypes.data_as(C.POINTER(C.c_uint8))), "fv_m5_gfm_rel")
+    return S
 
 
 def init_pure(params, rho_init, mat, pres, u=(0.0, 0.0, 0.0)):

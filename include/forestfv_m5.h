@@ -1,0 +1,72 @@
+/*
+ * forestfv_m5.h -- C ABI of the model-5 ghost-fluid operators on the GPU.
+ *
+ * Model 5 couples an elastic solid with fluids.  At a face where the solid
+ * indicator phi = alpha_solid - AVF_LIM changes sign, the reference replaces
+ * the right state's tangential velocity by the left one (a ghost state) before
+ * its HLLC flux, and after the RK stage or the relaxation that moves a leaf
+ * across the interface it re-projects that leaf's velocity onto the interface
+ * normal.  These entry points replace those pieces one to one; the rest of the
+ * model-5 stage (MUSCL face states, relaxation, the elastic EOS of the cells)
+ * stays with the caller.
+ *
+ * State layout, NEQ = 3 nmat + 12 (m5_input.c): [alpha rho_k (nmat) |
+ * rho u, rho v, rho w | rho E | alpha_k (nmat - 1) | alpha rho e_k (nmat) |
+ * A (9, row-major)].  Host arrays are row-major [count][NEQ]; every call copies
+ * its inputs to the device, runs the kernels and copies the outputs back.
+ * Every call returns FV_OK or an FV_ERR_* status (forestfv.h).
+ */
+#ifndef FORESTFV_M5_H
+#define FORESTFV_M5_H
+
+#include <stdint.h>
+#include "forestfv.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int    nmat;                 /* NMATERIALS: 2 or 3 */
+  double gamma[FV_MAXMAT];     /* MATERGAMA */
+  double pinf[FV_MAXMAT];      /* MATERPINF */
+  double mush[FV_MAXMAT];      /* MATERMUSH: shear modulus, 0 for a fluid */
+  double amin;                 /* AMIN */
+  double avf_lim;              /* AVF_LIM */
+  double dt;                   /* DT */
+  int    zerou[3];             /* ZEROU0, ZEROU1, ZEROU2: freeze a velocity component */
+} fv_m5_params;
+
+int fv_m5_neq(int nmat);
+
+/* GFM flux of nf sub-faces at a solid interface: replaces m5_hllc_gfm
+ * (m5_hllc_gfm.c:3-100, with m5_wave_speeds_gfm.c, m5_flux_gfm.c and
+ * m5_star_region.c) and its accumulation into flux_face
+ * (m5_residual.c:166-170).  UL, UR: [nf][NEQ] face states of the two sides
+ * (the states m5_facercnstr reads); frame: [nf][9] n, t1, t2 of the face;
+ * area: [nf] sub-face area.  Out: flux [nf][NEQ] = area (F.n + F_hllc), and
+ * unc [nf][3] the face velocity of the non-conservative terms (before the
+ * division by the number of sub-faces).  FV_ERR_NAN if a flux is NaN. */
+int fv_m5_gfm_faces(const fv_m5_params *p, int nf, const double *UL, const double *UR, const double *frame,
+                    const double *area, double *flux, double *unc);
+
+/* RK stage irk of model 5 over every leaf of pk: replaces m5_RK
+ * (m5_RK.c:3-126), including the stage-1 interface re-projection m5_gfm
+ * (m5_gfm.c:3-254) of the leaves whose solid fraction crosses AVF_LIM, in leaf
+ * order as the reference runs it.  SN, S, RHS: [n][NEQ] (SN is written by
+ * irk 0, S in place); gfm_face: [n][6], 1 where the stage's residual used the
+ * GFM flux.  FV_ERR_ARG if gfm_face marks a boundary face. */
+int fv_m5_rk(const fv_m5_params *p, const fv_pack *pk, int irk, double *SN, double *S, const double *RHS,
+             const uint8_t *gfm_face);
+
+/* Interface re-projection after relaxation: replaces the m5_gfm_rel step of
+ * m5_relaxation_solid (m5_relaxation_solid.c:16-34, m5_gfm_rel.c:3-246).
+ * S_before: [n][NEQ] state before relaxation; S: the relaxed state, updated
+ * in place. */
+int fv_m5_gfm_rel(const fv_m5_params *p, const fv_pack *pk, const double *S_before, double *S,
+                  const uint8_t *gfm_face);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

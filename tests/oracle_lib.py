@@ -72,6 +72,23 @@ class orc_crit(C.Structure):
     ]
 
 
+class orc_m5_params(C.Structure):
+    _fields_ = [
+        ("nmat", C.c_int),
+        ("gamma", C.c_double * ORC_MAXMAT),
+        ("pinf", C.c_double * ORC_MAXMAT),
+        ("mush", C.c_double * ORC_MAXMAT),
+        ("amin", C.c_double),
+        ("avf_lim", C.c_double),
+        ("dt", C.c_double),
+        ("zerou", C.c_int * 3),
+    ]
+
+
+M5_REC = 39   # orc_m5_record layout (fv_oracle_m5.h)
+M5_REC_FIELDS = {"r": 0, "e": 1, "u": 2, "v": 3, "w": 4, "phi0": 5, "ra": 6, "avf": 9, "are": 12, "c2Y": 15,
+                 "p_hydro": 18, "Amat": 21, "st": 30}
+
 _lib = None
 
 
@@ -109,6 +126,11 @@ def lib():
         L.orc_cons2prim.restype = None
         L.orc_prim2cons.argtypes = [P(orc_params), dp, dp]
         L.orc_prim2cons.restype = None
+        L.orc_m5_record.argtypes = [P(orc_m5_params), dp, dp]
+        L.orc_m5_record.restype = None
+        L.orc_m5_gfm_face.argtypes = [P(orc_m5_params), dp, dp, dp, C.c_double, dp, dp]
+        L.orc_m5_rk.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), C.c_int, dp, dp, dp, P(C.c_uint8)]
+        L.orc_m5_gfm_rel.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), dp, dp, P(C.c_uint8)]
         _lib = L
     return _lib
 
@@ -198,6 +220,28 @@ class OracleMesh:
         lib().orc_faceinterp(C.byref(self.om), self.cells, C.byref(p), _dp(S), _dp(SF))
         return SF
 
+    def m5_rk(self, m5p, irk, SN, S, RHS, gfm_face):
+        """orc_m5_rk: returns (SN, S) after model-5 RK stage irk."""
+        SN = np.ascontiguousarray(SN, dtype=np.float64).copy()
+        S = np.ascontiguousarray(S, dtype=np.float64).copy()
+        R = np.ascontiguousarray(RHS, dtype=np.float64)
+        gf = np.ascontiguousarray(gfm_face, dtype=np.uint8)
+        rc = lib().orc_m5_rk(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), irk, _dp(SN), _dp(S), _dp(R),
+                             gf.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc != 0:
+            raise RuntimeError(f"orc_m5_rk status {rc}")
+        return SN, S
+
+    def m5_gfm_rel(self, m5p, S_before, S, gfm_face):
+        Sb = np.ascontiguousarray(S_before, dtype=np.float64)
+        S = np.ascontiguousarray(S, dtype=np.float64).copy()
+        gf = np.ascontiguousarray(gfm_face, dtype=np.uint8)
+        rc = lib().orc_m5_gfm_rel(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(Sb), _dp(S),
+                                  gf.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc != 0:
+            raise RuntimeError(f"orc_m5_gfm_rel status {rc}")
+        return S
+
     def interface_location(self, params, crit, S):
         SI = np.zeros((self.n, 2), dtype=np.int32)
         lib().orc_interface_location(C.byref(self.om), C.byref(params_from(params)), C.byref(crit_from(crit)),
@@ -276,3 +320,42 @@ def physical_flux(params, vec, n):
     lib().orc_physical_flux(C.byref(params_from(params)), _dp(np.ascontiguousarray(vec, dtype=np.float64)),
                             _dp(np.asarray(n, dtype=np.float64)), _dp(flux))
     return flux
+
+
+def m5_from(p):
+    """orc_m5_params with the same field values as a forestfv fv_m5_params."""
+    q = orc_m5_params()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(orc_m5_params))
+    return q
+
+
+def m5_record(m5p, vec):
+    """orc_m5_record: dict of the face record fields of one model-5 state vector."""
+    out = np.zeros(M5_REC)
+    v = np.ascontiguousarray(vec, dtype=np.float64)
+    lib().orc_m5_record(C.byref(m5_from(m5p)), _dp(v), _dp(out))
+    nm = m5p.nmat
+    rec = {}
+    for k, off in M5_REC_FIELDS.items():
+        if k in ("ra", "avf", "are", "c2Y", "p_hydro"):
+            rec[k] = out[off:off + nm].copy()
+        elif k == "Amat":
+            rec[k] = out[off:off + 9].reshape(3, 3).copy()
+        elif k == "st":
+            rec[k] = out[off:off + 9].reshape(3, 3).copy()
+        else:
+            rec[k] = out[off]
+    return rec
+
+
+def m5_gfm_face(m5p, UL, UR, frame, area):
+    """orc_m5_gfm_face: (flux [NEQ], unc [3]) of one interface sub-face; raises on NaN."""
+    nq = 3 * m5p.nmat + 12
+    flux = np.zeros(nq)
+    unc = np.zeros(3)
+    rc = lib().orc_m5_gfm_face(C.byref(m5_from(m5p)), _dp(np.ascontiguousarray(UL, dtype=np.float64)),
+                               _dp(np.ascontiguousarray(UR, dtype=np.float64)),
+                               _dp(np.ascontiguousarray(frame, dtype=np.float64)), float(area), _dp(flux), _dp(unc))
+    if rc != 0:
+        raise FloatingPointError("NaN in the GFM face flux")
+    return flux, unc
