@@ -197,6 +197,11 @@ def lib():
             "fv_mlp_infer": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int, P(C.c_float)]),
             "fv_dev_indicator_mlp": (C.c_int, [C.c_void_p, C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int),
                                                P(C.c_uint8)]),
+            "fv_dev_set_forest": (C.c_int, [C.c_void_p, P(fv_mesh)]),
+            "fv_dev_get_forest": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]),
+            "fv_dev_get_pack": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_uint32), P(C.c_int), P(C.c_int)]),
+            "fv_dev_criterionsmth": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int)]),
+            "fv_dev_regrid": (C.c_int, [C.c_void_p, P(fv_crit), C.c_int, P(C.c_int)]),
             "fv_m5_neq": (C.c_int, [C.c_int]),
             "fv_m5_gfm_faces": (C.c_int, [P(fv_m5_params), C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
                                           P(C.c_double), P(C.c_double), P(C.c_double)]),
@@ -499,6 +504,44 @@ class Device:
         _check(lib().fv_dev_reattach(self._h, C.byref(pk.pk)), "fv_dev_reattach")
         self._pack = pk
         self.n = pk.n
+
+    def set_forest(self, mesh):
+        """Upload the mesh's leaf addresses and SI for device regrids (fv_dev_set_forest)."""
+        _check(lib().fv_dev_set_forest(self._h, mesh._p), "fv_dev_set_forest")
+
+    def regrid(self, crit, maxlevel):
+        """Device-resident mesh adaptation (fv_dev_regrid); returns the new leaf count."""
+        nn = C.c_int()
+        _check(lib().fv_dev_regrid(self._h, C.byref(crit), maxlevel, C.byref(nn)), "fv_dev_regrid")
+        self.n = nn.value
+        return self.n
+
+    def criterionsmth(self, LEVEL, split):
+        out = np.ascontiguousarray(split, dtype=np.int32).copy()
+        _check(lib().fv_dev_criterionsmth(self._h, LEVEL, _ip(out)), "fv_dev_criterionsmth")
+        return out
+
+    def get_forest(self):
+        """(root, level, ci, SI) of the device mesh."""
+        n = self.n
+        root = np.zeros(n, np.int32)
+        level = np.zeros(n, np.int32)
+        ci = np.zeros((n, 3), np.int32)
+        SI = np.zeros((n, 2), np.int32)
+        _check(lib().fv_dev_get_forest(self._h, _ip(root), _ip(level), _ip(ci), _ip(SI)), "fv_dev_get_forest")
+        return root, level, ci, SI
+
+    def get_pack(self):
+        """(nbr [6][n], info [n], fine4 [nfine][4]) of the device mesh."""
+        n = self.n
+        nbr = np.zeros((6, n), np.int32)
+        info = np.zeros(n, np.uint32)
+        nf = C.c_int()
+        _check(lib().fv_dev_get_pack(self._h, None, None, C.byref(nf), None), "fv_dev_get_pack")
+        fine4 = np.zeros((max(nf.value, 1), 4), np.int32)
+        _check(lib().fv_dev_get_pack(self._h, _ip(nbr), info.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(nf),
+                                     _ip(fine4)), "fv_dev_get_pack")
+        return nbr, info, fine4[:nf.value]
 
     def set_state(self, S):
         S = np.ascontiguousarray(S, dtype=np.float64)

@@ -1188,12 +1188,13 @@ static void free_buffers(fv_dev *c) {
   c->cap_n = c->cap_fine = 0;
 }
 
-static int upload_mesh(fv_dev *c, const fv_pack *pk) {
-  if (pk->n <= 0) return set_err(FV_ERR_ARG, "empty mesh");
-  const size_t n = (size_t)pk->n;
-  if (pk->n > c->cap_n || pk->nfine > c->cap_fine) {
+// buffers for n leaves and nfine 1:4 faces; reallocates everything (contents
+// lost) only when the capacity is too small
+int fv_dev_reserve(fv_dev *c, int n_leaves, int nfine) {
+  const size_t n = (size_t)n_leaves;
+  if (n_leaves > c->cap_n || nfine > c->cap_fine) {
     free_buffers(c);
-    const size_t nf = (size_t)(pk->nfine > 0 ? pk->nfine : 1);
+    const size_t nf = (size_t)(nfine > 0 ? nfine : 1);
     const size_t nb = (n + FV_BLOCK - 1) / FV_BLOCK;
     HIPCHK(hipMalloc(&c->S, sizeof(double) * n * c->neq));
     HIPCHK(hipMalloc(&c->S2, sizeof(double) * n * c->neq));
@@ -1209,9 +1210,18 @@ static int upload_mesh(fv_dev *c, const fv_pack *pk) {
     HIPCHK(hipMalloc(&c->SIb, sizeof(int) * n * 2));
     HIPCHK(hipMalloc(&c->split, sizeof(int) * n));
     HIPCHK(hipMalloc(&c->lim, n));
-    c->cap_n = pk->n;
+    c->cap_n = n_leaves;
     c->cap_fine = (int)nf;
   }
+  return FV_OK;
+}
+
+static int upload_mesh(fv_dev *c, const fv_pack *pk) {
+  if (pk->n <= 0) return set_err(FV_ERR_ARG, "empty mesh");
+  const size_t n = (size_t)pk->n;
+  int rc = fv_dev_reserve(c, pk->n, pk->nfine);
+  if (rc != FV_OK) return rc;
+  c->forest = 0;   // the leaf addresses of an earlier fv_dev_set_forest no longer apply
   c->n = pk->n;
   c->nfine = pk->nfine;
   to_dev_geo(&pk->geo, c->G);
@@ -1259,6 +1269,7 @@ extern "C" int fv_dev_detach(fv_dev *c) {
   if (!c) return FV_OK;
   hipStreamSynchronize(c->stream);
   free_buffers(c);
+  hipFree(c->lroot); hipFree(c->llevel); hipFree(c->lci); hipFree(c->lSI);
   hipFree(c->Gdev);
   if (c->pinned) hipHostFree(c->pinned);
   hipStreamDestroy(c->stream);

@@ -34,7 +34,16 @@ struct fv_dev {
   int cap_n = 0, cap_fine = 0;
   void *pinned = nullptr;  // host staging for device -> host copies of per-leaf outputs
   size_t pinned_cap = 0;
+  // leaf addresses of the forest for device regrids (fv_dev_set_forest): root [n],
+  // level [n], ci [n][3] and the mesh's interface flags SI [n][2]
+  int forest = 0, cap_leaf = 0;
+  int nroot[3] = {1, 1, 1}, bcside[6] = {0, 0, 0, 0, 0, 0};
+  int *lroot = nullptr, *llevel = nullptr, *lci = nullptr, *lSI = nullptr;
 };
+
+// capacity for n leaves and nfine 1:4 faces (reallocates, losing contents, only
+// when too small); defined in fv_device.hip
+int fv_dev_reserve(fv_dev *c, int n, int nfine);
 
 // device -> host copy through the context's pinned staging buffer (on its stream)
 int fv_copy_to_host(fv_dev *c, void *dst, const void *src, size_t bytes);

@@ -54,6 +54,62 @@ void fv_relax_host(const fv_params *p, double *vec) {
       avf[k] = ar[k] / (1.0 / t1);
     }
     VFLIM();
+  } else if (p->prelax == 2) {
+    /* m4_relaxationvec.c:128-297 with F_pe / DF_pe (F_pe.c:18-27, 64-73) on the
+     * branch's local arrays (the published F_pe reads never-written RUN fields) */
+    const double eb = 1.0e-8;
+    double t0[FV_MAXMAT], p0[FV_MAXMAT], max_avf = -1.0;
+    int principal = 0;
+    for (int k = 0; k < nm; k++) {
+      t0[k] = 1.0 / (ar[k] / avf[k]);
+      p0[k] = (are[k] * (G[k] - 1.0) - avf[k] * G[k] * P[k]) / avf[k];
+      if (p0[k] < -(1.0 - eb) * P[k] + eb) p0[k] = -(1.0 - eb) * P[k] + 1.0;
+      if (avf[k] > max_avf) { max_avf = avf[k]; principal = k; }
+    }
+#define FPE(pe_, out_)                                                                      \
+    do {                                                                                    \
+      double F_ = 0.0;                                                                      \
+      for (int k = 0; k < nm; k++)                                                          \
+        F_ += ar[k] * t0[k] * ((p0[k] + G[k] * P[k] + (G[k] - 1.0) * (pe_)) /                \
+                               ((pe_) + G[k] * P[k] + (G[k] - 1.0) * (pe_)));               \
+      (out_) = F_ + -1.0;                                                                   \
+    } while (0)
+#define DFPE(pe_, out_)                                                                     \
+    do {                                                                                    \
+      double D_ = 0.0;                                                                      \
+      for (int k = 0; k < nm; k++) {                                                        \
+        const double c1_ = ar[k] * t0[k];                                                   \
+        const double f_ = (p0[k] + G[k] * P[k] + (G[k] - 1.0) * (pe_));                      \
+        const double g_ = ((pe_) + G[k] * P[k] + (G[k] - 1.0) * (pe_));                      \
+        D_ += (c1_ * (G[k] - 1.0) * g_ - c1_ * f_ * G[k]) / SQ(g_);                         \
+      }                                                                                     \
+      (out_) = D_;                                                                          \
+    } while (0)
+    double pe_old = 0.001, pe_new = pe_old, Fv, Dv;
+    int it = 0;
+    double c1 = fabs(pe_new - pe_old);
+    FPE(pe_new, Fv);
+    double c2 = fabs(Fv);
+    while (((c2 >= 1.0e-12) || (c1 >= 1.0e-4)) && (it <= 101)) {
+      FPE(pe_old, Fv);
+      DFPE(pe_old, Dv);
+      pe_new = pe_old - 0.8 * (Fv / Dv);
+      c1 = fabs(pe_new - pe_old);
+      FPE(pe_new, Fv);
+      c2 = fabs(Fv);
+      if (it >= 100 && ((fabs(pe_old - pe_new) < 1.0e3) || isnan(pe_new))) { pe_new = NAN; break; }
+      pe_old = pe_new;
+      it++;
+    }
+#undef FPE
+#undef DFPE
+    const double Pp = P[principal];
+    if (pe_new < (-(1.0 - eb) * Pp + eb)) pe_new = -(1.0 - eb) * Pp + 1.0;
+    for (int k = 0; k < nm; k++) {
+      double t1 = t0[k] * (p0[k] + G[k] * P[k] + (G[k] - 1.0) * pe_new) / (pe_new + G[k] * P[k] + (G[k] - 1.0) * pe_new);
+      avf[k] = ar[k] / (1.0 / t1);
+    }
+    VFLIM();
   } else if (p->prelax == 3) {
     const double es = 1.0e-12, eb = 1.0e-8;
     double rk[FV_MAXMAT], pk[FV_MAXMAT];
@@ -92,7 +148,7 @@ void fv_relax_host(const fv_params *p, double *vec) {
       VFLIM();
     }
   }
-  if (p->prelax == 1 || p->prelax == 3) {
+  if ((p->prelax == 1 && nm == 2) || p->prelax == 2 || p->prelax == 3) {
     double e_hyd = rE / rho - 0.5 * (SQ((u)) + SQ((v)) + SQ((w)));
     double num = 0.0, den = 0.0;
     for (int k = 0; k < nm; k++) {

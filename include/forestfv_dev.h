@@ -86,6 +86,26 @@ int  fv_dev_indicator(fv_dev *ctx, const fv_crit *c, int *SI, int *split);
  * SI).  SI: [n][2] out, may be NULL. */
 int  fv_dev_interface_location(fv_dev *ctx, const fv_crit *c, int *SI);
 
+/* ---- device-resident mesh adaptation ----
+ * meshadaptation (meshadaptation.c:3-44) without a host round trip: the
+ * indicator, criterionsmth (criterionsmth.c:3-58), merge and split with the
+ * state remap (merge.c:20-55, split.c:425-434), the new DFS order and the
+ * connectivity and face lists of fv_mesh_pack, all on the device.  The result
+ * equals fv_dev_indicator + fv_criterionsmth + fv_mesh_adapt + fv_mesh_pack.
+ * fv_dev_set_forest uploads the leaf addresses (root, level, ci) and SI of the
+ * mesh ctx is attached to; it is needed once after attach or reattach. */
+int  fv_dev_set_forest(fv_dev *ctx, const fv_mesh *m);
+/* *n_out: the new leaf count; the state stays on the device, remapped. */
+int  fv_dev_regrid(fv_dev *ctx, const fv_crit *c, int maxlevel, int *n_out);
+/* criterionsmth alone on the device mesh; split [n] host array, in place. */
+int  fv_dev_criterionsmth(fv_dev *ctx, int LEVEL, int *split);
+/* Leaf addresses of the device mesh: root [n], level [n], ci [n][3], SI [n][2]
+ * (any may be NULL); fv_mesh_from_leaves rebuilds the host forest from them. */
+int  fv_dev_get_forest(fv_dev *ctx, int *root, int *level, int *ci, int *SI);
+/* The device mesh's fv_pack arrays: nbr [6][n], info [n], fine4 [nfine][4]
+ * (any may be NULL). */
+int  fv_dev_get_pack(fv_dev *ctx, int *nbr, uint32_t *info, int *nfine, int *fine4);
+
 /* Per-leaf limiter when fv_params.limiter == 2: 0 minmod, 1 MC; written by
  * fv_dev_indicator_mlp from the learned smooth flag, or by the caller. */
 int  fv_dev_set_limiter_flags(fv_dev *ctx, const uint8_t *lim);

@@ -1,0 +1,155 @@
+"""Device-resident mesh adaptation (fv_dev_regrid) against the host path
+(fv_dev_indicator + fv_criterionsmth + fv_mesh_adapt + fv_mesh_pack), which
+follows meshadaptation.c, criterionsmth.c, merge.c and split.c leaf by leaf.
+Equality is exact: leaf addresses, interface flags, neighbour tables, face
+kinds, 1:4 rows and the remapped state, over several regrid/step cycles."""
+import time
+
+import numpy as np
+import pytest
+
+import forestfv_public_amd as F
+from cases import bubble_case, random_amr_mesh, sod_params, three_material_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def random_targets(m, rng, up=0.15, down=0.4, maxlevel=6):
+    lvl = m.arrays()["level"].copy()
+    r = rng.random(m.n)
+    split = lvl.copy()
+    split[r < up] += 1
+    split[r > 1.0 - down] -= 1
+    return np.clip(split, 1, maxlevel)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_device_criterionsmth_matches_the_sequential_sweep(seed):
+    rng = np.random.default_rng(seed)
+    p = sod_params()
+    m = random_amr_mesh(p, base_level=2, max_level=5, seed=seed, frac=0.3, passes=3)
+    dev = F.Device(p, m)
+    dev.set_forest(m)
+    for trial in range(3):
+        split = random_targets(m, rng, maxlevel=6)
+        LEVEL = 6
+        want = m.criterionsmth(LEVEL, split)
+        got = dev.criterionsmth(LEVEL, split)
+        say(f"[criterionsmth] seed={seed} trial={trial} n={m.n} changed={(want != split).sum()}")
+        assert np.array_equal(got, want)
+    dev.close()
+
+
+def host_regrid(dev, m, p, crit, maxlevel):
+    SI, split = dev.indicator(crit)
+    m.set_SI(SI)
+    split = m.criterionsmth(maxlevel, split)
+    S = m.adapt(p, split, dev.get_state())
+    dev.reattach(m)
+    dev.set_state(S)
+    return S
+
+
+def assert_same_mesh(devd, m, S_host):
+    root, level, ci, SI = devd.get_forest()
+    a = m.arrays()
+    assert np.array_equal(root, a["root"])
+    assert np.array_equal(level, a["level"])
+    assert np.array_equal(ci, a["ci"])
+    assert np.array_equal(SI, a["SI"])
+    nbr, info, fine4 = devd.get_pack()
+    pa = m.pack().arrays()
+    assert np.array_equal(nbr, pa["nbr"])
+    assert np.array_equal(info, pa["info"])
+    assert np.array_equal(fine4, pa["fine4"])
+    assert np.array_equal(devd.get_state(), S_host)
+
+
+CASES = {
+    "bubble_prelax1": lambda: (bubble_case(level=4, prelax=1, L=2.0 ** -8), 6),
+    "bubble_prelax2": lambda: (bubble_case(level=4, prelax=2, L=2.0 ** -8), 6),
+    "three_mat_prelax0": lambda: (three_material_case(level=4, prelax=0), 6),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_device_regrid_matches_host_adaptation(name):
+    (p, m, S), maxlevel = CASES[name]()
+    crit = F.make_crit(uniform_on=1, uniform_lvl=3, prs_on=1, prs_lvl=maxlevel, prs_thr=0.05,
+                       inter0_on=1, inter0_lvl=maxlevel, inter0_lrs=3)
+    mh = F.Mesh.uniform((1, 1, 1), (0.0, 0.0, 0.0), m.m.rootsize, int(m.arrays()["level"][0]), list(p.bc))
+    devh = F.Device(p, mh)
+    devd = F.Device(p, m)
+    devh.set_state(S)
+    devd.set_state(S)
+    devd.set_forest(m)
+    for cycle in range(4):
+        Sh = host_regrid(devh, mh, p, crit, maxlevel)
+        nd = devd.regrid(crit, maxlevel)
+        lv = np.bincount(mh.arrays()["level"])
+        say(f"[regrid] {name} cycle {cycle}: host {mh.n} device {nd} leaves, levels {lv[1:]}")
+        assert nd == mh.n
+        assert_same_mesh(devd, mh, Sh)
+        devh.advance(2)
+        devd.advance(2)
+        assert np.array_equal(devd.get_state(), devh.get_state())
+    devh.close()
+    devd.close()
+
+
+def test_device_regrid_merges_and_splits_random_targets():
+    """Random targets on a random forest: many merges and splits per regrid,
+    through fv_dev_criterionsmth and the adaptation kernels alone."""
+    rng = np.random.default_rng(5)
+    p = sod_params(prelax=1)
+    m = random_amr_mesh(p, base_level=2, max_level=5, seed=5, frac=0.3, passes=3)
+    c = m.centres()
+    S = np.zeros((m.n, F.neq(2)))
+    for i in range(m.n):
+        mat = 1 if c[i, 0] > 0.5 else 0
+        S[i] = F.init_pure(p, (1.0, 0.125), mat, (1.0, 0.1)[mat], (0.1 * c[i, 1], 0.0, 0.05))
+    # a threshold indicator on the density jump gives both refinement and coarsening
+    crit = F.make_crit(uniform_on=1, uniform_lvl=2, rho_on=1, rho_lvl=6, rho_thr=0.05)
+    mh = random_amr_mesh(p, base_level=2, max_level=5, seed=5, frac=0.3, passes=3)
+    devh, devd = F.Device(p, mh), F.Device(p, m)
+    devh.set_state(S)
+    devd.set_state(S)
+    devd.set_forest(m)
+    for cycle in range(3):
+        Sh = host_regrid(devh, mh, p, crit, 6)
+        devd.regrid(crit, 6)
+        say(f"[regrid random] cycle {cycle}: {mh.n} leaves")
+        assert_same_mesh(devd, mh, Sh)
+        devh.advance(1)
+        devd.advance(1)
+    devh.close()
+    devd.close()
+    del rng
+
+
+def test_device_regrid_is_much_faster_than_the_host_path():
+    p = sod_params(level=7)
+    m = F.Mesh.uniform((1, 1, 1), (0.0, 0.0, 0.0), 1.0, 7, list(p.bc))      # 64^3 = 262144 leaves
+    S = m.init_split_x(p, (1.0, 0.125), (1.0, 0.1), 0.5)
+    crit = F.make_crit(uniform_on=1, uniform_lvl=5, rho_on=1, rho_lvl=8, rho_thr=0.05)
+    devd = F.Device(p, m)
+    devd.set_state(S)
+    devd.set_forest(m)
+    t0 = time.perf_counter()
+    devd.regrid(crit, 8)
+    td = time.perf_counter() - t0
+    mh = F.Mesh.uniform((1, 1, 1), (0.0, 0.0, 0.0), 1.0, 7, list(p.bc))
+    devh = F.Device(p, mh)
+    devh.set_state(S)
+    t0 = time.perf_counter()
+    Sh = host_regrid(devh, mh, p, crit, 8)
+    th = time.perf_counter() - t0
+    say(f"[regrid timing] {m.n} -> {mh.n} leaves: device {td * 1e3:.1f} ms, host path {th * 1e3:.1f} ms")
+    assert_same_mesh(devd, mh, Sh)
+    assert td < th
+    devh.close()
+    devd.close()
