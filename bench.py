@@ -221,6 +221,61 @@ def fp32_mlp_leg(p, m, S, crit, cycles=4, every=5, log_fn=log):
     }
 
 
+def regrid_leg(p, m, S, crit, maxlevel=9, every=5, cycles=2, log_fn=log):
+    """Adaptation every `every` steps as loop.c:19-21 runs it, on the C1 mesh:
+    fv_dev_regrid (indicator, criterionsmth, merge/split, connectivity on the
+    device) against the host path (indicator, then fv_criterionsmth +
+    fv_mesh_adapt + fv_mesh_pack on the host and a re-upload).  Reports the
+    regrid times and the loop's cell-updates/s with each."""
+    nstage = 3
+    out = {}
+    for leg in ("device", "host"):
+        if leg == "device":
+            mesh = m
+        else:                                                # the host leg adapts its own copy
+            a = m.arrays()
+            mm = m.m
+            mesh = F.Mesh.from_leaves(tuple(mm.nroot), tuple(mm.origin), mm.rootsize, list(p.bc), a["root"],
+                                      a["level"], a["ci"])
+            mesh.set_SI(a["SI"])
+        dev = F.Device(p, mesh)
+        dev.set_state(S)
+        if leg == "device":
+            dev.set_forest(mesh)
+        t_reg = t_step = 0.0
+        cells = 0
+        sizes = []
+        for c in range(cycles):
+            dev.advance(1)                                   # settle the kernels before timing
+            t0 = time.perf_counter()
+            if leg == "device":
+                dev.regrid(crit, maxlevel)
+            else:
+                SI, split = dev.indicator(crit)
+                mesh.set_SI(SI)
+                split = mesh.criterionsmth(maxlevel, split)
+                S2 = mesh.adapt(p, split, dev.get_state())
+                dev.reattach(mesh)
+                dev.set_state(S2)
+            t_reg += time.perf_counter() - t0
+            sizes.append(dev.n)
+            ms, _ = dev.time_stages(every * nstage)
+            t_step += ms * 1e-3 * every * nstage
+            cells += dev.n * every * nstage
+        dev.close()
+        out[leg] = dict(ms=1e3 * t_reg / cycles, loop=cells / (t_step + t_reg) / 1e6, sizes=sizes)
+        log_fn(f"  regrid {leg}: {out[leg]['ms']:.1f} ms per regrid, leaves {sizes}, "
+               f"loop {out[leg]['loop']:.1f} Mcell-updates/s")
+    return {
+        "device_ms_per_regrid": round(out["device"]["ms"], 2), "host_path_ms_per_regrid": round(out["host"]["ms"], 1),
+        "leaves_after": out["device"]["sizes"], "same_leaf_counts": out["device"]["sizes"] == out["host"]["sizes"],
+        "every_steps": every,
+        "loop_value_device_regrid": round(out["device"]["loop"], 2),
+        "loop_value_host_regrid": round(out["host"]["loop"], 2),
+        "unit": "Mcell-updates/s per RK stage, regrid time included",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +288,7 @@ def main():
     ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "exact"])
     ap.add_argument("--fused", action="store_true", help="one fused stage kernel without stored face states")
     ap.add_argument("--no-mlp", action="store_true", help="skip the FP32 + learned-indicator leg")
+    ap.add_argument("--no-regrid", action="store_true", help="skip the device / host regrid leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo design is "
@@ -298,6 +354,8 @@ def main():
         leg = fp32_mlp_leg(p, m, S, c1_crit(9, a.layers))
         leg["speedup_vs_plain_stage"] = round(leg["value"] / value, 3)
         out["fp32_mlp"] = leg
+    if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_regrid:
+        out["regrid"] = regrid_leg(p, m, S, c1_crit(9, a.layers))
     if not a.no_cpu_baseline:
         cb, l1, stages = cpu_baseline()
         out["cpu_baseline"] = cb

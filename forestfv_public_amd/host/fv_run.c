@@ -3,13 +3,14 @@
  * ADAPTHSTEP steps, geometric pre-adaptation as in main.c:107-134) written
  * against the C ABI of forestfv.h / forestfv_dev.h, with no Python.
  *
- *   fv_run bubble <base> <max> <layers> <steps> <every> [prec] [bc]
+ *   fv_run bubble <base> <max> <layers> <steps> <every> [prec] [bc] [regrid]
  *       air bubble (R = 0.4 mm, 1 bar) in water (50 bar) in a 4 mm cube,
  *       octree levels base..max on the pressure-jump and interface-layer
  *       indicator, regrid before every <every>-th step, interface layers
  *       after every step, CFL 0.4 (dt fixed between regrids), SSP-RK3, PRELAX 1;
  *       prec 0 plain FP64 (default), 1 mixed, 2 bit-identical; bc the code of
- *       all six sides (default 2, outlet; 4 closes the box)
+ *       all six sides (default 2, outlet; 4 closes the box); regrid 1 (default)
+ *       adapts on the device (fv_dev_regrid), 0 through the host mesh
  *   fv_run sod <level> <steps> [prec] [out]
  *       Sod shock tube (config C0) on a uniform octree, SSP-RK3, PRELAX 0,
  *       CFL 0.5, closed box; with <out> the final state is written there as
@@ -77,7 +78,23 @@ static void regrid(fv_dev *dev, fv_mesh **m, const fv_params *p, const fv_crit *
   CHECK(fv_dev_set_state(dev, *S));
 }
 
-static int run_bubble(int base, int maxlevel, int layers, int steps, int every, int prec, int bc) {
+/* the host forest of the device mesh (after device regrids) */
+static fv_mesh *forest_of(fv_dev *dev, const fv_mesh *like) {
+  int n = 0, neq = 0;
+  CHECK(fv_dev_size(dev, &n, &neq));
+  int *root = malloc(sizeof(int) * (size_t)n), *level = malloc(sizeof(int) * (size_t)n);
+  int *ci = malloc(sizeof(int) * 3 * (size_t)n), *SI = malloc(sizeof(int) * 2 * (size_t)n);
+  if (!root || !level || !ci || !SI) { fprintf(stderr, "fv_run: out of memory\n"); exit(1); }
+  CHECK(fv_dev_get_forest(dev, root, level, ci, SI));
+  int st = 0;
+  fv_mesh *m = fv_mesh_from_leaves(like->nroot, like->origin, like->rootsize, like->bcside, n, root, level, ci, &st);
+  if (!m) { fprintf(stderr, "fv_run: fv_mesh_from_leaves -> %d\n", st); exit(1); }
+  memcpy(m->SI, SI, sizeof(int) * 2 * (size_t)n);
+  free(root); free(level); free(ci); free(SI);
+  return m;
+}
+
+static int run_bubble(int base, int maxlevel, int layers, int steps, int every, int prec, int bc, int dev_regrid) {
   const double L = 4e-3, R = 0.4e-3;
   fv_params p;
   memset(&p, 0, sizeof p);
@@ -117,6 +134,7 @@ static int run_bubble(int base, int maxlevel, int layers, int steps, int every, 
   CHECK(fv_init_bubble(m, &p, rho0, p0, c, R, 2.0, 1234u, 0.01, S));
   CHECK(fv_dev_set_state(dev, S));
   CHECK(fv_dev_interface_location(dev, &cr, NULL));
+  if (dev_regrid) CHECK(fv_dev_set_forest(dev, m));
   report_levels("initial mesh", m);
   printf("setup %.2f s\n", now() - t0);
 
@@ -125,14 +143,23 @@ static int run_bubble(int base, int maxlevel, int layers, int steps, int every, 
   long double m0 = total(m, &g, S, neq, 0), m1 = total(m, &g, S, neq, 1), e0 = total(m, &g, S, neq, 5);
   double t_step = 0.0, t_regrid = 0.0, simt = 0.0;
   long long cell_stages = 0;
+  int nleaves = m->n, nregrid = 0;
   for (int step = 0; step < steps; step++) {
     /* loop.c:19: adaptation before the step when (istep+1) % ADAPTHSTEP == 0 */
     const int adapt = (step + 1) % every == 0;
     if (adapt) {
       double ta = now();
-      CHECK(fv_dev_get_state(dev, S));
-      regrid(dev, &m, &p, &cr, maxlevel, &S);
+      if (dev_regrid) {
+        int nn = 0;
+        CHECK(fv_dev_regrid(dev, &cr, maxlevel, &nn));
+        nleaves = nn;
+      } else {
+        CHECK(fv_dev_get_state(dev, S));
+        regrid(dev, &m, &p, &cr, maxlevel, &S);
+        nleaves = m->n;
+      }
       t_regrid += now() - ta;
+      nregrid++;
     }
     double ta = now(), dt;
     if (step == 0 || adapt) {
@@ -145,13 +172,22 @@ static int run_bubble(int base, int maxlevel, int layers, int steps, int every, 
     CHECK(fv_dev_interface_location(dev, &cr, NULL));
     t_step += now() - ta;
     simt += p.dt;
-    cell_stages += 3LL * m->n;
+    cell_stages += 3LL * nleaves;
+  }
+  if (dev_regrid) {
+    fv_mesh *q = forest_of(dev, m);
+    fv_mesh_free(m);
+    m = q;
+    S = realloc(S, sizeof(double) * (size_t)m->n * neq);
+    if (!S) { fprintf(stderr, "fv_run: out of memory\n"); return 1; }
   }
   CHECK(fv_dev_get_state(dev, S));
   report_levels("final mesh", m);
   long double m0f = total(m, &g, S, neq, 0), m1f = total(m, &g, S, neq, 1), e0f = total(m, &g, S, neq, 5);
   printf("steps %d, simulated %.3e s, stepping %.3f s (%.1f Mcell-updates/s per RK stage incl. host calls), "
-         "regrids %.3f s\n", steps, simt, t_step, (double)cell_stages / t_step / 1e6, t_regrid);
+         "%d %s regrids %.3f s (%.2f ms each), loop %.1f Mcell-updates/s\n",
+         steps, simt, t_step, (double)cell_stages / t_step / 1e6, nregrid, dev_regrid ? "device" : "host", t_regrid,
+         nregrid ? 1e3 * t_regrid / nregrid : 0.0, (double)cell_stages / (t_step + t_regrid) / 1e6);
   printf("fv_run bubble: drift water %.3Le air %.3Le energy %.3Le\n", fabsl(m0f - m0) / m0, fabsl(m1f - m1) / m1,
          fabsl(e0f - e0) / e0);
   fv_dev_detach(dev);
@@ -209,15 +245,14 @@ static int run_sod(int level, int steps, int prec, const char *out) {
 }
 
 int main(int argc, char **argv) {
-  if (argc >= 8 && strcmp(argv[1], "bubble") == 0)
-    return run_bubble(atoi(argv[2]), atoi(argv[3]), atoi(argv[4]), atoi(argv[5]), atoi(argv[6]), atoi(argv[7]),
-                      argc > 8 ? atoi(argv[8]) : 2);
   if (argc >= 7 && strcmp(argv[1], "bubble") == 0)
-    return run_bubble(atoi(argv[2]), atoi(argv[3]), atoi(argv[4]), atoi(argv[5]), atoi(argv[6]), FV_PREC_FP64, 2);
+    return run_bubble(atoi(argv[2]), atoi(argv[3]), atoi(argv[4]), atoi(argv[5]), atoi(argv[6]),
+                      argc > 7 ? atoi(argv[7]) : FV_PREC_FP64, argc > 8 ? atoi(argv[8]) : 2,
+                      argc > 9 ? atoi(argv[9]) : 1);
   if (argc >= 4 && strcmp(argv[1], "sod") == 0)
     return run_sod(atoi(argv[2]), atoi(argv[3]), argc > 4 ? atoi(argv[4]) : FV_PREC_FP64, argc > 5 ? argv[5] : NULL);
   fprintf(stderr,
-          "usage: fv_run bubble <base> <max> <layers> <steps> <every> [prec] [bc]\n"
+          "usage: fv_run bubble <base> <max> <layers> <steps> <every> [prec] [bc] [regrid]\n"
           "       fv_run sod <level> <steps> [prec] [out]\n");
   return 2;
 }

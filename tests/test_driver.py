@@ -53,6 +53,17 @@ def test_driver_sod_matches_oracle(tmp_path, prec):
 
 
 @pytest.mark.gpu
+def test_driver_device_and_host_regrids_give_the_same_run():
+    """fv_dev_regrid in the loop leaves the same meshes and states as the host path."""
+    outs = []
+    for regrid in (1, 0):
+        r = run("bubble", 3, 6, 3, 30, 5, F.PREC_FP64, 4, regrid)
+        assert r.returncode == 0
+        outs.append([l for l in r.stdout.splitlines() if l.startswith(("final mesh", "fv_run bubble:"))])
+    assert outs[0] == outs[1] and len(outs[0]) == 2
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("prec", [F.PREC_FP64, F.PREC_MIXED])
 def test_driver_bubble_amr_loop_conserves_in_a_closed_box(prec):
     # levels 3..5, two interface layers, regrid every 5 steps, walls on all sides
