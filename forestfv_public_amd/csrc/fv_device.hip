@@ -1179,7 +1179,8 @@ static void free_buffers(fv_dev *c) {
   hipFree(c->S); hipFree(c->S2); hipFree(c->SN); hipFree(c->SF); hipFree(c->part); hipFree(c->dtmp); hipFree(c->SFm);
   c->SFm = nullptr;
   c->cap_m = 0;
-  hipFree(c->nbr); hipFree(c->fine4); hipFree(c->nanflag); hipFree(c->info);
+  hipFree(c->nbr); hipFree(c->fine4); hipFree(c->nanflag); hipFree(c->info); hipFree(c->iflag); hipFree(c->bfsq);
+  c->iflag = c->bfsq = nullptr;
   hipFree(c->SIa); hipFree(c->SIb); hipFree(c->split); hipFree(c->lim);
   c->lim = nullptr;
   c->S = c->S2 = c->SN = c->SF = c->part = c->dtmp = nullptr;
@@ -1205,6 +1206,8 @@ int fv_dev_reserve(fv_dev *c, int n_leaves, int nfine) {
     HIPCHK(hipMalloc(&c->nbr, sizeof(int) * n * 6));
     HIPCHK(hipMalloc(&c->fine4, sizeof(int) * nf * 4));
     HIPCHK(hipMalloc(&c->nanflag, sizeof(int)));
+    HIPCHK(hipMalloc(&c->iflag, 4 * sizeof(int)));
+    HIPCHK(hipMalloc(&c->bfsq, 2 * n * sizeof(int)));
     HIPCHK(hipMalloc(&c->info, sizeof(uint32_t) * n));
     HIPCHK(hipMalloc(&c->SIa, sizeof(int) * n * 2));
     HIPCHK(hipMalloc(&c->SIb, sizeof(int) * n * 2));
@@ -1466,10 +1469,82 @@ extern "C" int fv_dev_cfl_dt(fv_dev *c, double cfl, double *dt) {
   return FV_OK;
 }
 
+// With two materials only SI[.][0] is ever set, and layer lvl takes every
+// unset leaf with a face neighbour in layer lvl-1: a breadth-first search from
+// the interface leaves, cut at imax-1 layers.  The search keeps the current
+// layer as a queue and pushes from it: each queued leaf claims its unset
+// neighbours with a compare-and-swap and queues them for the next layer.  A
+// leaf gets layer lvl exactly when a neighbour has layer lvl-1, as in the
+// layer-by-layer passes, but each layer touches only its own leaves.  The
+// queue order varies from run to run; the layer values do not.
+constexpr int BFS_INF = 1 << 28;
+constexpr int BFS_GRID = 2048;
+
+// wave-aggregated append of leaf i (when take) to queue Q with counter *cnt
+__device__ __forceinline__ void bfs_append(bool take, int i, int *__restrict__ Q, int *__restrict__ cnt) {
+  const unsigned long long m = __ballot(take);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(cnt, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (take) Q[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+}
+
+__global__ __launch_bounds__(FV_BLOCK) void k_si_bfs_seed(int n, const int *__restrict__ SI, int *__restrict__ X,
+                                                          int *__restrict__ Q, int *__restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  const bool seed = live && SI[2 * i] != 0;
+  if (live) X[i] = seed ? 1 : BFS_INF;
+  bfs_append(seed, i, Q, cnt);
+}
+
+// one layer: leaves of Qin (layer lvl-1) claim their unset neighbours for layer lvl
+__global__ __launch_bounds__(FV_BLOCK) void k_si_bfs_push(int n, int lvl, const int *__restrict__ nbr,
+                                                          const uint32_t *__restrict__ info,
+                                                          const int *__restrict__ fine4, int *__restrict__ X,
+                                                          const int *__restrict__ Qin, const int *__restrict__ cin,
+                                                          int *__restrict__ Qout, int *__restrict__ cout,
+                                                          int *__restrict__ cnext) {
+  const int total = *cin;
+  const int stride = gridDim.x * blockDim.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *cnext = 0;   // the counter two layers on is free again
+  // every lane of a wave runs the same number of rounds, so the ballots see full waves
+  const int rounds = (total + stride - 1) / stride;
+  for (int r = 0; r < rounds; r++) {
+    const int t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < total;
+    const int j = live ? Qin[t] : 0;
+    const uint32_t inf = live ? info[j] : 0u;
+    for (int f = 0; f < 6; f++) {
+      const int kind = face_kind(inf, f);
+      const bool has = live && kind != FV_FK_BOUNDARY;
+      const int nb = has ? nbr[(size_t)f * n + j] : 0;
+      const int ns = (kind == FV_FK_FINE) ? 4 : 1;
+      for (int q = 0; q < 4; q++) {
+        const bool cand = has && q < ns;
+        const int i = cand ? ((kind == FV_FK_FINE) ? fine4[4 * nb + q] : nb) : 0;
+        const bool take = cand && X[i] == BFS_INF && atomicCAS(&X[i], BFS_INF, lvl) == BFS_INF;
+        bfs_append(take, i, Qout, cout);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(FV_BLOCK) void k_si_bfs_final(int n, int dmax, const int *__restrict__ X,
+                                                           int *__restrict__ SI) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = X[i];
+  SI[2 * i] = (x <= dmax) ? x : 0;
+}
+
 // interface_location (interface_location.c:7-86): interface leaves, then
-// imax-2 layers grown one face neighbour at a time; returns the buffer that
+// imax-2 layers grown one face neighbour at a time; *out is the buffer that
 // holds the final SI [n][2].
-static int *launch_interface_location(fv_dev *c, const fv_crit *cr) {
+static int launch_interface_location(fv_dev *c, const fv_crit *cr, int **out) {
   const int n = c->n;
   const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
   int imax = 0;   // ADAPT_INTER_MAX (m4_input.c:52-54)
@@ -1480,11 +1555,28 @@ static int *launch_interface_location(fv_dev *c, const fv_crit *cr) {
     hipLaunchKernelGGL(k_si_first<2>, grid, block, 0, c->stream, n, c->P, c->S, c->nbr, c->info, c->fine4, cur);
   else
     hipLaunchKernelGGL(k_si_first<3>, grid, block, 0, c->stream, n, c->P, c->S, c->nbr, c->info, c->fine4, cur);
-  for (int lvl = 2; lvl < imax; lvl++) {
-    hipLaunchKernelGGL(k_si_layer, grid, block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur, nxt);
-    int *t = cur; cur = nxt; nxt = t;
+  if (c->nmat == 2 && imax > 2) {
+    const int dmax = imax - 1;
+    int *X = nxt;                                   // [n] layer per leaf, in the spare SI buffer
+    int *Q[2] = {c->bfsq, c->bfsq + n};             // current and next layer
+    int *cnt = c->iflag;                            // three rotating queue counters
+    HIPCHK(hipMemsetAsync(cnt, 0, 3 * sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_si_bfs_seed, grid, block, 0, c->stream, n, cur, X, Q[0], cnt);
+    for (int lvl = 2; lvl <= dmax; lvl++) {
+      const int a = (lvl - 2) % 3, b = (lvl - 1) % 3, z = lvl % 3;
+      hipLaunchKernelGGL(k_si_bfs_push, dim3(BFS_GRID), block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, X,
+                         Q[lvl & 1], cnt + a, Q[(lvl + 1) & 1], cnt + b, cnt + z);
+    }
+    hipLaunchKernelGGL(k_si_bfs_final, grid, block, 0, c->stream, n, dmax, X, cur);
+  } else {
+    for (int lvl = 2; lvl < imax; lvl++) {
+      hipLaunchKernelGGL(k_si_layer, grid, block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur, nxt);
+      int *t = cur; cur = nxt; nxt = t;
+    }
   }
-  return cur;
+  HIPCHK(hipGetLastError());
+  *out = cur;
+  return FV_OK;
 }
 
 int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev) {
@@ -1498,7 +1590,9 @@ int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev) {
   C.vel_on = cr->vel_on; C.vel_lvl = cr->vel_lvl; C.vel_thr = cr->vel_thr;
   C.inter0_on = cr->inter0_on; C.inter0_lvl = cr->inter0_lvl; C.inter0_lrs = cr->inter0_lrs;
   C.inter1_on = cr->inter1_on; C.inter1_lvl = cr->inter1_lvl; C.inter1_lrs = cr->inter1_lrs;
-  int *cur = launch_interface_location(c, cr);
+  int *cur = nullptr;
+  int rc = launch_interface_location(c, cr, &cur);
+  if (rc != FV_OK) return rc;
   if (c->nmat == 2)
     hipLaunchKernelGGL(k_split<2>, grid, block, 0, c->stream, n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, c->split);
   else
@@ -1537,11 +1631,12 @@ extern "C" int fv_dev_indicator(fv_dev *c, const fv_crit *cr, int *SI, int *spli
 extern "C" int fv_dev_interface_location(fv_dev *c, const fv_crit *cr, int *SI) {
   if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
   const int n = c->n;
-  int *cur = launch_interface_location(c, cr);
+  int *cur = nullptr;
+  int rc = launch_interface_location(c, cr, &cur);
+  if (rc != FV_OK) return rc;
   hipLaunchKernelGGL(k_refresh_kinds, dim3((n + FV_BLOCK - 1) / FV_BLOCK), dim3(FV_BLOCK), 0, c->stream, n, c->nbr,
                      (const int *)cur, c->info);
   HIPCHK(hipGetLastError());
-  int rc;
   if (SI && (rc = fv_copy_to_host(c, SI, cur, sizeof(int) * 2 * (size_t)n)) != FV_OK) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return FV_OK;

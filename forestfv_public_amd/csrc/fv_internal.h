@@ -29,6 +29,8 @@ struct fv_dev {
   float *SFm = nullptr;   // mixed-precision face records, allocated on first use
   int cap_m = 0;
   int *nbr = nullptr, *fine4 = nullptr, *nanflag = nullptr, *SIa = nullptr, *SIb = nullptr, *split = nullptr;
+  int *iflag = nullptr;   // [4] queue counters of the interface-layer search
+  int *bfsq = nullptr;    // [2][n] current and next queue of the interface-layer search
   uint32_t *info = nullptr;
   uint8_t *lim = nullptr;  // per-leaf limiter (0 minmod, 1 MC) read when limiter == 2
   int cap_n = 0, cap_fine = 0;

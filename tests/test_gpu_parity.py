@@ -220,6 +220,29 @@ def test_indicator_flags_bitwise():
     assert np.array_equal(splitg, splito)
 
 
+@pytest.mark.parametrize("case,lrs", [("bubble32", 24), ("bubble32", 64), ("amr", 12), ("three_mat", 9)])
+def test_interface_layers_match_oracle(case, lrs):
+    """Many interface layers: the two-material relaxation and the three-material
+    layer passes against interface_location.c's passes in the oracle."""
+    if case == "bubble32":
+        p, m, S = bubble_case(level=6, prelax=1)
+    elif case == "amr":
+        p, m, S, _ = amr_case(2, si_frac=0.0)
+        c = m.centres()
+        S[:, 6] = np.where(np.linalg.norm(c - 0.5, axis=1) < 0.3, 1.0 - 1e-8, 1e-8)   # alpha_0: a sphere of material 0
+    else:
+        p, m, S = three_material_case(level=5, prelax=0)
+    crit = F.make_crit(inter0_on=1, inter0_lvl=7, inter0_lrs=lrs)
+    dev = F.Device(p, m)
+    dev.set_state(S)
+    SIg = dev.interface_location(crit)
+    dev.close()
+    SIo = O.OracleMesh(m).interface_location(p, crit, S)
+    say(f"[interface layers] {case} lrs={lrs} n={m.n} layers reached {SIo[:, 0].max()}")
+    assert SIo[:, 0].max() > 2
+    assert np.array_equal(SIg, SIo)
+
+
 @pytest.mark.parametrize("prec", ["exact", "fp64"])
 def test_interface_location_every_step_as_the_time_loop(prec):
     """loop.c:21-23: a step, then interface_location; the next step's
