@@ -1478,7 +1478,7 @@ extern "C" int fv_dev_cfl_dt(fv_dev *c, double cfl, double *dt) {
 // layer-by-layer passes, but each layer touches only its own leaves.  The
 // queue order varies from run to run; the layer values do not.
 constexpr int BFS_INF = 1 << 28;
-constexpr int BFS_GRID = 2048;
+constexpr int BFS_GRID = 256;
 
 // wave-aggregated append of leaf i (when take) to queue Q with counter *cnt
 __device__ __forceinline__ void bfs_append(bool take, int i, int *__restrict__ Q, int *__restrict__ cnt) {
