@@ -276,6 +276,59 @@ def regrid_leg(p, m, S, crit, maxlevel=9, every=5, cycles=2, log_fn=log):
     }
 
 
+def collapse_leg(p, layers=64, log_fn=log):
+    """The bubble to its first collapse with the C driver (forestfv_public_amd/fv_run):
+    the AMR loop in FP64 with a device regrid every 5 steps, then the precision report
+    at that time: FP64 and mixed stages on the fixed C1 mesh with the same fixed dt,
+    per-field relative L1 and Linf of the mixed state against the FP64 one."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "forestfv_public_amd", "fv_run")
+
+    def run(*args):
+        r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise SystemExit(f"fv_run {args} failed: {r.stderr[-500:]}")
+        return json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][-1])
+
+    amr = run("collapse", 5, 9, layers, 5, F.PREC_FP64, 6000)
+    log_fn(f"  collapse (AMR, FP64): {amr['steps']} steps, t = {amr['t_collapse_s']:.3e} s = "
+           f"{amr['t_collapse_over_rayleigh']:.3f} Rayleigh, R_min/R0 {amr['R_min_over_R0']:.3f}, "
+           f"loop {amr['loop_rate_Mcell_s']:.0f} Mcell-updates/s")
+    hmin = L_DOMAIN / 256
+    dt = 0.25 * hmin / np.sqrt(6.12 * (5e6 + 3.43e8) / 1000.0)
+    nsteps = int(np.ceil(amr["t_collapse_s"] / dt))
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for prec, key in ((F.PREC_FP64, "fp64"), (F.PREC_MIXED, "mixed")):
+            path = os.path.join(td, key + ".bin")
+            r = run("collapse", 5, 9, layers, 0, prec, nsteps, repr(dt), path)
+            n, nq = np.fromfile(path, dtype=np.int32, count=2)
+            out[key] = (r, np.fromfile(path, dtype=np.float64, offset=8).reshape(n, nq))
+    (r64, S64), (rmx, Smx) = out["fp64"], out["mixed"]
+    names = ["alpha_rho_water", "alpha_rho_air", "rho_u", "rho_v", "rho_w", "rho_E", "alpha_water",
+             "alpha_rho_e_water", "alpha_rho_e_air"]
+    fields = {}
+    for k, nm in enumerate(names):
+        a, b = Smx[:, k], S64[:, k]
+        fields[nm] = {"l1": rel_l1(a, b), "linf": float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))}
+    for nm, a, b in (("rho", Smx[:, 0] + Smx[:, 1], S64[:, 0] + S64[:, 1]),
+                     ("pressure", mixture_pressure(p, Smx), mixture_pressure(p, S64))):
+        fields[nm] = {"l1": rel_l1(a, b), "linf": float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))}
+    log_fn(f"  collapse time, fixed mesh, {nsteps} steps: mixed vs FP64 rho L1 {fields['rho']['l1']:.2e} "
+           f"Linf {fields['rho']['linf']:.2e}, p L1 {fields['pressure']['l1']:.2e} Linf {fields['pressure']['linf']:.2e}")
+    return {
+        "amr_fp64": {k: amr[k] for k in ("steps", "t_collapse_s", "rayleigh_time_s", "t_collapse_over_rayleigh",
+                                         "R_min_over_R0", "leaves_start", "leaves_min", "leaves_end", "regrids",
+                                         "regrid_ms", "loop_rate_Mcell_s", "water_mass_change", "air_mass_change")},
+        "fixed_mesh_to_collapse": {"steps": nsteps, "dt_s": dt, "leaves": int(S64.shape[0]),
+                                   "stage_rate_fp64": r64["stage_rate_Mcell_s"],
+                                   "stage_rate_mixed": rmx["stage_rate_Mcell_s"],
+                                   "R_min_over_R0_fp64": r64["R_min_over_R0"], "R_min_over_R0_mixed": rmx["R_min_over_R0"],
+                                   "mixed_vs_fp64": fields},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -289,6 +342,7 @@ def main():
     ap.add_argument("--fused", action="store_true", help="one fused stage kernel without stored face states")
     ap.add_argument("--no-mlp", action="store_true", help="skip the FP32 + learned-indicator leg")
     ap.add_argument("--no-regrid", action="store_true", help="skip the device / host regrid leg")
+    ap.add_argument("--no-collapse", action="store_true", help="skip the run to the first collapse")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo design is "
@@ -356,6 +410,8 @@ def main():
         out["fp32_mlp"] = leg
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_regrid:
         out["regrid"] = regrid_leg(p, m, S, c1_crit(9, a.layers))
+    if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_collapse:
+        out["collapse"] = collapse_leg(p, a.layers)
     if not a.no_cpu_baseline:
         cb, l1, stages = cpu_baseline()
         out["cpu_baseline"] = cb

@@ -197,6 +197,7 @@ def lib():
             "fv_mlp_infer": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int, P(C.c_float)]),
             "fv_dev_indicator_mlp": (C.c_int, [C.c_void_p, C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int),
                                                P(C.c_uint8)]),
+            "fv_dev_totals": (C.c_int, [C.c_void_p, P(C.c_double)]),
             "fv_dev_set_forest": (C.c_int, [C.c_void_p, P(fv_mesh)]),
             "fv_dev_get_forest": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]),
             "fv_dev_get_pack": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_uint32), P(C.c_int), P(C.c_int)]),
@@ -504,6 +505,12 @@ class Device:
         _check(lib().fv_dev_reattach(self._h, C.byref(pk.pk)), "fv_dev_reattach")
         self._pack = pk
         self.n = pk.n
+
+    def totals(self):
+        """Volume integral of every state field (fv_dev_totals)."""
+        t = np.zeros(self.nq)
+        _check(lib().fv_dev_totals(self._h, _dp(t)), "fv_dev_totals")
+        return t
 
     def set_forest(self, mesh):
         """Upload the mesh's leaf addresses and SI for device regrids (fv_dev_set_forest)."""

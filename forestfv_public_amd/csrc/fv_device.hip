@@ -1272,7 +1272,7 @@ extern "C" int fv_dev_detach(fv_dev *c) {
   if (!c) return FV_OK;
   hipStreamSynchronize(c->stream);
   free_buffers(c);
-  hipFree(c->lroot); hipFree(c->llevel); hipFree(c->lci); hipFree(c->lSI);
+  hipFree(c->lroot); hipFree(c->llevel); hipFree(c->lci); hipFree(c->lSI); hipFree(c->tpart);
   hipFree(c->Gdev);
   if (c->pinned) hipHostFree(c->pinned);
   hipStreamDestroy(c->stream);
@@ -1450,6 +1450,63 @@ extern "C" int fv_dev_advance(fv_dev *c, int nsteps) {
     }
   }
   return check_nan(c);
+}
+
+// volume integrals of every state field: per-block tree sums, then the blocks in
+// order (the same bits on every run)
+__global__ __launch_bounds__(FV_BLOCK) void k_totals(int n, int neq, const Geo *__restrict__ Gp,
+                                                     const double *__restrict__ S, const uint32_t *__restrict__ info,
+                                                     double *__restrict__ part) {
+  __shared__ double red[FV_BLOCK];
+  const int i = blockIdx.x * FV_BLOCK + threadIdx.x;
+  const double vol = (i < n) ? Gp->vol[cell_level(info[i])] : 0.0;
+  for (int v = 0; v < neq; v++) {
+    red[threadIdx.x] = (i < n) ? S[(size_t)v * n + i] * vol : 0.0;
+    __syncthreads();
+    for (int w = FV_BLOCK / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)v * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_totals_final(int nb, int neq, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double red[1024];
+  for (int v = 0; v < neq; v++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(size_t)v * nb + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[v] = red[0];
+    __syncthreads();
+  }
+}
+
+extern "C" int fv_dev_totals(fv_dev *c, double *tot) {
+  if (!c || !tot) return set_err(FV_ERR_ARG, "null argument");
+  const int n = c->n, neq = c->neq;
+  const int nb = (n + FV_BLOCK - 1) / FV_BLOCK;
+  const size_t need = (size_t)nb * neq + neq;
+  if (need > c->cap_tpart) {
+    hipFree(c->tpart);
+    c->tpart = nullptr;
+    c->cap_tpart = 0;
+    HIPCHK(hipMalloc(&c->tpart, sizeof(double) * need));
+    c->cap_tpart = need;
+  }
+  hipLaunchKernelGGL(k_totals, dim3(nb), dim3(FV_BLOCK), 0, c->stream, n, neq, c->Gdev, c->S, c->info, c->tpart);
+  hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(1024), 0, c->stream, nb, neq, c->tpart,
+                     c->tpart + (size_t)nb * neq);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(tot, c->tpart + (size_t)nb * neq, sizeof(double) * neq, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FV_OK;
 }
 
 extern "C" int fv_dev_cfl_dt(fv_dev *c, double cfl, double *dt) {

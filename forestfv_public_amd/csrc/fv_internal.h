@@ -31,6 +31,8 @@ struct fv_dev {
   int *nbr = nullptr, *fine4 = nullptr, *nanflag = nullptr, *SIa = nullptr, *SIb = nullptr, *split = nullptr;
   int *iflag = nullptr;   // [4] queue counters of the interface-layer search
   int *bfsq = nullptr;    // [2][n] current and next queue of the interface-layer search
+  double *tpart = nullptr; // per-block partial sums of fv_dev_totals
+  size_t cap_tpart = 0;
   uint32_t *info = nullptr;
   uint8_t *lim = nullptr;  // per-leaf limiter (0 minmod, 1 MC) read when limiter == 2
   int cap_n = 0, cap_fine = 0;

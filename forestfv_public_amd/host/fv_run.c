@@ -11,6 +11,8 @@
  *       prec 0 plain FP64 (default), 1 mixed, 2 bit-identical; bc the code of
  *       all six sides (default 2, outlet; 4 closes the box); regrid 1 (default)
  *       adapts on the device (fv_dev_regrid), 0 through the host mesh
+ *   fv_run collapse <base> <max> <layers> <every> <prec> <maxsteps> [fixed_dt] [out]
+ *       the bubble to its first collapse (see run_collapse)
  *   fv_run sod <level> <steps> [prec] [out]
  *       Sod shock tube (config C0) on a uniform octree, SSP-RK3, PRELAX 0,
  *       CFL 0.5, closed box; with <out> the final state is written there as
@@ -94,7 +96,18 @@ static fv_mesh *forest_of(fv_dev *dev, const fv_mesh *like) {
   return m;
 }
 
-static int run_bubble(int base, int maxlevel, int layers, int steps, int every, int prec, int bc, int dev_regrid) {
+typedef struct {
+  fv_params p;
+  fv_crit cr;
+  fv_mesh *m;
+  double *S;
+  fv_dev *dev;
+  double L, R;
+} bubble_t;
+
+/* the water/air bubble of the configs on a forest pre-adapted to levels
+ * base..maxlevel (main.c:107-134: initial data, indicator, smoothing, refine) */
+static void setup_bubble(bubble_t *B, int base, int maxlevel, int layers, int prec, int bc) {
   const double L = 4e-3, R = 0.4e-3;
   fv_params p;
   memset(&p, 0, sizeof p);
@@ -113,13 +126,10 @@ static int run_bubble(int base, int maxlevel, int layers, int steps, int every, 
   const int nroot[3] = {1, 1, 1};
   const double origin[3] = {0.0, 0.0, 0.0};
   const int neq = fv_neq(2);
-
   fv_mesh *m = fv_mesh_uniform(nroot, origin, L, base, p.bc);
-  if (!m) { fprintf(stderr, "fv_run: mesh\n"); return 1; }
+  if (!m) { fprintf(stderr, "fv_run: mesh\n"); exit(1); }
   double *S = malloc(sizeof(double) * (size_t)m->n * neq);
   fv_dev *dev = NULL;
-  double t0 = now();
-  /* geometric pre-adaptation: initial data, indicator, smoothing, refinement */
   for (int pass = 0; pass <= maxlevel - base; pass++) {
     CHECK(fv_init_bubble(m, &p, rho0, p0, c, R, 2.0, 1234u, 0.01, S));
     fv_pack pk;
@@ -134,6 +144,20 @@ static int run_bubble(int base, int maxlevel, int layers, int steps, int every, 
   CHECK(fv_init_bubble(m, &p, rho0, p0, c, R, 2.0, 1234u, 0.01, S));
   CHECK(fv_dev_set_state(dev, S));
   CHECK(fv_dev_interface_location(dev, &cr, NULL));
+  B->p = p; B->cr = cr; B->m = m; B->S = S; B->dev = dev; B->L = L; B->R = R;
+}
+
+static int run_bubble(int base, int maxlevel, int layers, int steps, int every, int prec, int bc, int dev_regrid) {
+  const int neq = fv_neq(2);
+  double t0 = now();
+  bubble_t B;
+  setup_bubble(&B, base, maxlevel, layers, prec, bc);
+  fv_params p = B.p;
+  fv_crit cr = B.cr;
+  fv_mesh *m = B.m;
+  double *S = B.S;
+  fv_dev *dev = B.dev;
+  const double L = B.L;
   if (dev_regrid) CHECK(fv_dev_set_forest(dev, m));
   report_levels("initial mesh", m);
   printf("setup %.2f s\n", now() - t0);
@@ -196,6 +220,86 @@ static int run_bubble(int base, int maxlevel, int layers, int steps, int every, 
   return 0;
 }
 
+/* Run the bubble to its first collapse: the loop of loop.c with a device regrid
+ * every `every` steps (0: fixed mesh, no interface layers), the CFL step
+ * recomputed every 5 steps (or dt fixed when fixed_dt > 0), and the bubble
+ * volume V_domain - integral of alpha_water from fv_dev_totals every 2 steps.
+ * The first collapse is the minimum of the volume, taken once it has fallen
+ * below half its start and rebounded by 2 %.  One JSON line reports it against
+ * the Rayleigh time 0.915 R0 sqrt(rho_l / (p_l - p_g)); with `out` the final
+ * state is written as in the sod mode. */
+static int run_collapse(int base, int maxlevel, int layers, int every, int prec, int maxsteps, double fixed_dt,
+                        const char *out) {
+  const int neq = fv_neq(2);
+  double t0 = now();
+  bubble_t B;
+  setup_bubble(&B, base, maxlevel, layers, prec, 2);
+  const double t_setup = now() - t0;
+  fv_dev *dev = B.dev;
+  if (every > 0) CHECK(fv_dev_set_forest(dev, B.m));
+  const double Vdom = B.L * B.L * B.L;
+  double tot[16];
+  CHECK(fv_dev_totals(dev, tot));
+  const double Vb0 = Vdom - tot[6], mw0 = tot[0], ma0 = tot[1];
+  double Vmin = Vb0, t_min = 0.0, simt = 0.0, dt = fixed_dt;
+  double t_step = 0.0, t_regrid = 0.0;
+  long long cell_stages = 0;
+  int n = B.m->n, nregrid = 0, step = 0, collapsed = 0, nmax = n, nmin = n;
+  for (step = 0; step < maxsteps && !collapsed; step++) {
+    const int adapt = every > 0 && (step + 1) % every == 0;
+    if (adapt) {
+      double ta = now();
+      CHECK(fv_dev_regrid(dev, &B.cr, maxlevel, &n));
+      t_regrid += now() - ta;
+      nregrid++;
+      if (n > nmax) nmax = n;
+      if (n < nmin) nmin = n;
+    }
+    double ta = now();
+    if (fixed_dt <= 0.0 && (step % 5 == 0 || adapt)) CHECK(fv_dev_cfl_dt(dev, 0.4, &dt));
+    CHECK(fv_dev_set_dt(dev, dt));
+    CHECK(fv_dev_advance(dev, 1));
+    if (every > 0) CHECK(fv_dev_interface_location(dev, &B.cr, NULL));
+    t_step += now() - ta;
+    simt += dt;
+    cell_stages += 3LL * n;
+    if (step % 2 == 1) {
+      CHECK(fv_dev_totals(dev, tot));
+      const double Vb = Vdom - tot[6];
+      if (Vb < Vmin) { Vmin = Vb; t_min = simt; }
+      if (Vmin < 0.5 * Vb0 && Vb > 1.02 * Vmin) collapsed = 1;
+    }
+  }
+  CHECK(fv_dev_totals(dev, tot));
+  const double t_ray = 0.915 * B.R * sqrt(1000.0 / (5e6 - 1e5));
+  printf("{\"mode\": \"collapse\", \"collapsed\": %s, \"steps\": %d, \"t_end_s\": %.6e, \"t_collapse_s\": %.6e, "
+         "\"rayleigh_time_s\": %.6e, \"t_collapse_over_rayleigh\": %.4f, \"R_min_over_R0\": %.4f, "
+         "\"leaves_start\": %d, \"leaves_min\": %d, \"leaves_max\": %d, \"leaves_end\": %d, \"regrids\": %d, "
+         "\"regrid_ms\": %.3f, \"stepping_s\": %.3f, \"setup_s\": %.2f, "
+         "\"stage_rate_Mcell_s\": %.1f, \"loop_rate_Mcell_s\": %.1f, \"water_mass_change\": %.3e, "
+         "\"air_mass_change\": %.3e, \"precision\": %d}\n",
+         collapsed ? "true" : "false", step, simt, t_min, t_ray, t_min / t_ray, cbrt(Vmin / Vb0), B.m->n, nmin, nmax, n,
+         nregrid, nregrid ? 1e3 * t_regrid / nregrid : 0.0, t_step, t_setup, (double)cell_stages / t_step / 1e6,
+         (double)cell_stages / (t_step + t_regrid) / 1e6, (tot[0] - mw0) / mw0, (tot[1] - ma0) / ma0, prec);
+  fflush(stdout);
+  if (out) {
+    double *S = malloc(sizeof(double) * (size_t)n * neq);
+    CHECK(fv_dev_get_state(dev, S));
+    FILE *f = fopen(out, "wb");
+    int hdr[2] = {n, neq};
+    if (!S || !f || fwrite(hdr, sizeof(int), 2, f) != 2 || fwrite(S, sizeof(double), (size_t)n * neq, f) != (size_t)n * neq ||
+        fclose(f) != 0) {
+      fprintf(stderr, "fv_run: cannot write %s\n", out);
+      return 1;
+    }
+    free(S);
+  }
+  fv_dev_detach(dev);
+  fv_mesh_free(B.m);
+  free(B.S);
+  return 0;
+}
+
 static int run_sod(int level, int steps, int prec, const char *out) {
   fv_params p;
   memset(&p, 0, sizeof p);
@@ -249,10 +353,14 @@ int main(int argc, char **argv) {
     return run_bubble(atoi(argv[2]), atoi(argv[3]), atoi(argv[4]), atoi(argv[5]), atoi(argv[6]),
                       argc > 7 ? atoi(argv[7]) : FV_PREC_FP64, argc > 8 ? atoi(argv[8]) : 2,
                       argc > 9 ? atoi(argv[9]) : 1);
+  if (argc >= 8 && strcmp(argv[1], "collapse") == 0)
+    return run_collapse(atoi(argv[2]), atoi(argv[3]), atoi(argv[4]), atoi(argv[5]), atoi(argv[6]), atoi(argv[7]),
+                        argc > 8 ? atof(argv[8]) : 0.0, argc > 9 ? argv[9] : NULL);
   if (argc >= 4 && strcmp(argv[1], "sod") == 0)
     return run_sod(atoi(argv[2]), atoi(argv[3]), argc > 4 ? atoi(argv[4]) : FV_PREC_FP64, argc > 5 ? argv[5] : NULL);
   fprintf(stderr,
           "usage: fv_run bubble <base> <max> <layers> <steps> <every> [prec] [bc] [regrid]\n"
+          "       fv_run collapse <base> <max> <layers> <every> <prec> <maxsteps> [fixed_dt] [out]\n"
           "       fv_run sod <level> <steps> [prec] [out]\n");
   return 2;
 }

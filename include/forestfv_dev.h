@@ -150,6 +150,11 @@ int  fv_mlp_infer(fv_mlp *m, const float *X, int B, float *out);
  * also sets the per-leaf limiter used when fv_params.limiter == 2 */
 int  fv_dev_indicator_mlp(fv_dev *ctx, fv_mlp *m, const fv_crit *c, int *SI, int *split, uint8_t *smooth);
 
+/* Volume integral of every state field, tot [NEQ] (the conservation monitor of
+ * conservationcalc.c:3-36): sums per block, then over the blocks in order, so
+ * the result is the same on every run. */
+int  fv_dev_totals(fv_dev *ctx, double *tot);
+
 /* Number of leaves and NEQ of the attached mesh. */
 int  fv_dev_size(const fv_dev *ctx, int *n, int *neq);
 

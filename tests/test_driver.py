@@ -75,3 +75,20 @@ def test_driver_bubble_amr_loop_conserves_in_a_closed_box(prec):
     # merge averages and split distributes conservatively; the stage is a
     # conservative face-flux update with walls on every side
     assert max(d) < (1e-12 if prec == F.PREC_FP64 else 1e-6)
+
+
+@pytest.mark.gpu
+def test_driver_runs_the_bubble_to_its_first_collapse():
+    """The whole loop on the device (stages, interface layers every step, device
+    regrid every 5 steps, CFL step) on a coarse forest (levels 4-7): the bubble
+    collapses near the Rayleigh time 0.915 R0 sqrt(rho / dp), later on this coarse
+    mesh (the bubble radius is six cells), and the air mass stays in the bubble."""
+    import json
+    r = run("collapse", 4, 7, 8, 5, F.PREC_FP64, 3000, timeout=300)
+    assert r.returncode == 0
+    res = json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][-1])
+    assert res["collapsed"] is True
+    assert 1.0 < res["t_collapse_over_rayleigh"] < 1.6
+    assert 0.3 < res["R_min_over_R0"] < 0.8
+    assert abs(res["air_mass_change"]) < 1e-4
+    assert res["regrids"] > 50 and res["leaves_min"] < res["leaves_start"]
