@@ -203,6 +203,12 @@ def lib():
             "fv_dev_get_pack": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_uint32), P(C.c_int), P(C.c_int)]),
             "fv_dev_criterionsmth": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int)]),
             "fv_dev_regrid": (C.c_int, [C.c_void_p, P(fv_crit), C.c_int, P(C.c_int)]),
+            "fv_restart_write": (C.c_int, [C.c_char_p, P(fv_mesh), P(C.c_double), C.c_int, C.c_int, C.c_double,
+                                           C.c_int]),
+            "fv_restart_read": (C.c_int, [C.c_char_p, P(C.c_int), P(C.c_double), C.c_double, P(C.c_int),
+                                          P(P(fv_mesh)), P(P(C.c_double)), P(C.c_int), P(C.c_int), P(C.c_double)]),
+            "fv_dev_get_ntrees": (C.c_int, [C.c_void_p, P(C.c_int)]),
+            "fv_dev_write_restart": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_int]),
             "fv_m5_neq": (C.c_int, [C.c_int]),
             "fv_m5_gfm_faces": (C.c_int, [P(fv_m5_params), C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
                                           P(C.c_double), P(C.c_double), P(C.c_double)]),
@@ -249,6 +255,30 @@ def make_params(nmat=2, gamma=(1.4, 1.4), pinf=(0.0, 0.0), amin=1e-8, avf_lim=0.
     p.fix = fix
     p.primtv = primtv
     return p
+
+
+def restart_write(path, mesh, S, istep=0, time=0.0, binary=False):
+    """restart_save / export_ultra_fast layout (fv_restart_write); binary: lossless."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    assert S.shape[0] == mesh.n
+    _check(lib().fv_restart_write(os.fsencode(path), mesh._p, _dp(S), S.shape[1], istep, time, int(binary)),
+           "fv_restart_write")
+
+
+def restart_read(path, nroot, origin, rootsize, bc):
+    """restart_load (fv_restart_read), either format: (Mesh, S, istep, time)."""
+    pm = C.POINTER(fv_mesh)()
+    ps = C.POINTER(C.c_double)()
+    nq, st, t = C.c_int(), C.c_int(), C.c_double()
+    nr = (C.c_int * 3)(*nroot)
+    og = (C.c_double * 3)(*origin)
+    b = (C.c_int * 6)(*bc)
+    _check(lib().fv_restart_read(os.fsencode(path), nr, og, rootsize, b, C.byref(pm), C.byref(ps), C.byref(nq),
+                                 C.byref(st), C.byref(t)), "fv_restart_read")
+    m = Mesh(pm)
+    S = np.ctypeslib.as_array(ps, shape=(m.n * nq.value,)).reshape(m.n, nq.value).copy()
+    libc_free(ps)
+    return m, S, st.value, t.value
 
 
 def m5_neq(nmat):
@@ -511,6 +541,10 @@ class Device:
         t = np.zeros(self.nq)
         _check(lib().fv_dev_totals(self._h, _dp(t)), "fv_dev_totals")
         return t
+
+    def write_restart(self, path, istep=0, time=0.0, binary=False):
+        """restart file of the device mesh and state (fv_dev_write_restart)."""
+        _check(lib().fv_dev_write_restart(self._h, os.fsencode(path), istep, time, int(binary)), "fv_dev_write_restart")
 
     def set_forest(self, mesh):
         """Upload the mesh's leaf addresses and SI for device regrids (fv_dev_set_forest)."""

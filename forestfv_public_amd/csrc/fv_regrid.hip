@@ -524,6 +524,14 @@ extern "C" int fv_dev_get_forest(fv_dev *c, int *root, int *level, int *ci, int 
   return FV_OK;
 }
 
+extern "C" int fv_dev_get_ntrees(fv_dev *c, int *ntrees) {
+  int rc = need_forest(c);
+  if (rc != FV_OK) return rc;
+  if (!ntrees) return fv_set_err(FV_ERR_ARG, "null argument");
+  *ntrees = c->nroot[0] * c->nroot[1] * c->nroot[2];
+  return FV_OK;
+}
+
 extern "C" int fv_dev_get_pack(fv_dev *c, int *nbr, uint32_t *info, int *nfine, int *fine4) {
   if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
   const size_t n = (size_t)c->n;

@@ -178,6 +178,19 @@ int      fv_init_bubble(const fv_mesh *m, const fv_params *p, const double *rho_
                         const double *p_init, const double centre[3], double radius,
                         double smear, unsigned seed, double perturb, double *S);
 
+/* ---- restart and field files (restart_save.c:12-45, restart_load.c:14-45,
+ * export_ultra_fast in exportfield.c:45-97) ----
+ * Text (binary = 0): header "istep time tleaves ntrees NEQ", then per leaf in
+ * DFS order "root tag S[0..NEQ-1]" at %15.10e, as the reference writes and
+ * reads it.  Binary (binary = 1, new): the same content without rounding,
+ * magic "FVDUMP01".  S is [n][NEQ] in leaf order. */
+int      fv_restart_write(const char *path, const fv_mesh *m, const double *S, int neq, int istep,
+                          double time, int binary);
+/* Either format (told apart by the magic): a new forest over the given brick of
+ * roots, in DFS order, and its state; *S is malloc'd, the caller frees it. */
+int      fv_restart_read(const char *path, const int nroot[3], const double origin[3], double rootsize,
+                         const int bcside[6], fv_mesh **m, double **S, int *neq, int *istep, double *time);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,12 @@ int  fv_dev_criterionsmth(fv_dev *ctx, int LEVEL, int *split);
 /* Leaf addresses of the device mesh: root [n], level [n], ci [n][3], SI [n][2]
  * (any may be NULL); fv_mesh_from_leaves rebuilds the host forest from them. */
 int  fv_dev_get_forest(fv_dev *ctx, int *root, int *level, int *ci, int *SI);
+/* Number of root trees of the forest given to fv_dev_set_forest. */
+int  fv_dev_get_ntrees(fv_dev *ctx, int *ntrees);
+/* restart_save / export_ultra_fast of the device mesh and state (forest set
+ * with fv_dev_set_forest): text (binary = 0) or lossless binary (binary = 1),
+ * the formats of fv_restart_write. */
+int  fv_dev_write_restart(fv_dev *ctx, const char *path, int istep, double time, int binary);
 /* The device mesh's fv_pack arrays: nbr [6][n], info [n], fine4 [nfine][4]
  * (any may be NULL). */
 int  fv_dev_get_pack(fv_dev *ctx, int *nbr, uint32_t *info, int *nfine, int *fine4);
