@@ -1273,6 +1273,8 @@ extern "C" int fv_dev_detach(fv_dev *c) {
   hipStreamSynchronize(c->stream);
   free_buffers(c);
   hipFree(c->lroot); hipFree(c->llevel); hipFree(c->lci); hipFree(c->lSI); hipFree(c->tpart);
+  for (auto &b : c->pool) hipFree(b.first);
+  c->pool.clear();
   hipFree(c->Gdev);
   if (c->pinned) hipHostFree(c->pinned);
   hipStreamDestroy(c->stream);

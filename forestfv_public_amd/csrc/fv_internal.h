@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <vector>
 #include "forestfv_dev.h"
 #include "fv_phys.h"
 
@@ -33,6 +34,8 @@ struct fv_dev {
   int *bfsq = nullptr;    // [2][n] current and next queue of the interface-layer search
   double *tpart = nullptr; // per-block partial sums of fv_dev_totals
   size_t cap_tpart = 0;
+  // device blocks kept between regrids for their temporaries: (pointer, bytes)
+  std::vector<std::pair<void *, size_t>> pool;
   uint32_t *info = nullptr;
   uint8_t *lim = nullptr;  // per-leaf limiter (0 minmod, 1 MC) read when limiter == 2
   int cap_n = 0, cap_fine = 0;

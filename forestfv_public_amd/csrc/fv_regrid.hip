@@ -350,16 +350,48 @@ __global__ __launch_bounds__(BLK) void k_conn_emit(int n, Forest Fo, const uint6
 
 // ---- host side ----
 
+// temporaries of one call, taken from and given back to the context's pool of
+// device blocks, so that a regrid allocates only when the mesh outgrows them
 struct Tmp {
-  std::vector<void *> p;
-  ~Tmp() { for (void *q : p) hipFree(q); }
+  fv_dev *c;
+  std::vector<std::pair<void *, size_t>> p;
+  explicit Tmp(fv_dev *ctx) : c(ctx) {}
+  ~Tmp() {
+    for (auto &b : p)
+      if (b.first) c->pool.push_back(b);
+    // a mesh that keeps growing leaves smaller generations behind: free the smallest
+    while (c->pool.size() > 48) {
+      size_t k = 0;
+      for (size_t j = 1; j < c->pool.size(); j++)
+        if (c->pool[j].second < c->pool[k].second) k = j;
+      hipFree(c->pool[k].first);
+      c->pool.erase(c->pool.begin() + k);
+    }
+  }
   template <class T>
   hipError_t get(T **out, size_t count) {
+    const size_t bytes = (count > 0 ? count : 1) * sizeof(T);
+    size_t best = (size_t)-1;
+    int pick = -1;
+    for (size_t k = 0; k < c->pool.size(); k++)
+      if (c->pool[k].second >= bytes && c->pool[k].second < best) { best = c->pool[k].second; pick = (int)k; }
+    if (pick >= 0) {
+      p.push_back(c->pool[pick]);
+      c->pool.erase(c->pool.begin() + pick);
+      *out = (T *)p.back().first;
+      return hipSuccess;
+    }
     void *q = nullptr;
-    hipError_t e = hipMalloc(&q, (count > 0 ? count : 1) * sizeof(T));
-    if (e == hipSuccess) p.push_back(q);
+    const size_t alloc = bytes + bytes / 4;            // headroom for a slightly larger mesh next time
+    hipError_t e = hipMalloc(&q, alloc);
+    if (e == hipSuccess) p.push_back({q, alloc});
     *out = (T *)q;
     return e;
+  }
+  // hand a block over to the caller (it leaves the pool for good)
+  void keep(const void *q) {
+    for (auto &b : p)
+      if (b.first == q) b.first = nullptr;
   }
 };
 
@@ -390,16 +422,21 @@ int smooth(fv_dev *c, Tmp &T, int LEVEL, int *split) {
     hipLaunchKernelGGL(k_smooth_later, grid_of(n), dim3(BLK), 0, c->stream, n, c->nbr, c->info, c->fine4, x, cv);
     HIPCHK(hipMemcpyAsync(split, cv, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
     int *yin = split, *yout = y2;
+    // sweeps past the fixpoint change nothing, so the host checks only every
+    // third sweep whether the last one still changed a value
     for (int it = 0;; it++) {
       if (it > 64) return fv_set_err(FV_ERR_MESH, "criterionsmth: smoothing sweeps did not settle");
-      HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
+      const bool check = (it % 3) == 2;
+      if (check) HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
       hipLaunchKernelGGL(k_smooth_earlier, grid_of(n), dim3(BLK), 0, c->stream, n, c->nbr, c->info, c->fine4, cv,
                          yin, yout, flag);
-      int h = 0;
-      HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
       int *t = yin; yin = yout; yout = t;
-      if (!h) break;
+      if (check) {
+        int h = 0;
+        HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (!h) break;
+      }
     }
     if (yin != split) HIPCHK(hipMemcpyAsync(split, yin, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
   }
@@ -427,10 +464,8 @@ int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
   const int nn = (int)total;
   int *root2, *level2, *ci2, *SI2;
   double *S2;
-  HIPCHK(hipMalloc(&root2, sizeof(int) * nn)); T.p.push_back(root2);
-  HIPCHK(hipMalloc(&level2, sizeof(int) * nn)); T.p.push_back(level2);
-  HIPCHK(hipMalloc(&ci2, sizeof(int) * 3 * (size_t)nn)); T.p.push_back(ci2);
-  HIPCHK(hipMalloc(&SI2, sizeof(int) * 2 * (size_t)nn)); T.p.push_back(SI2);
+  HIPCHK(T.get(&root2, nn)); HIPCHK(T.get(&level2, nn));
+  HIPCHK(T.get(&ci2, 3 * (size_t)nn)); HIPCHK(T.get(&SI2, 2 * (size_t)nn));
   HIPCHK(T.get(&S2, (size_t)nn * neq));
   if (c->nmat == 2)
     hipLaunchKernelGGL(k_adapt_emit<2>, grid_of(n), dim3(BLK), 0, c->stream, n, nn, c->P, c->lroot, c->llevel, c->lci,
@@ -460,8 +495,11 @@ int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
                      nbr2, info2, fine2, err);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  // install: the device buffers may be reallocated, so the new state waits in S2
-  if ((rc = fv_dev_reserve(c, nn, (int)nfine)) != FV_OK) return rc;
+  // install: the device buffers may be reallocated (with headroom, so that a
+  // slowly growing mesh does not reallocate every time); the new state waits in S2
+  if (nn > c->cap_n || nfine > c->cap_fine) {
+    if ((rc = fv_dev_reserve(c, nn + nn / 4, (int)(nfine + nfine / 4))) != FV_OK) return rc;
+  }
   c->n = nn;
   c->nfine = (int)nfine;
   HIPCHK(hipMemcpyAsync(c->S, S2, sizeof(double) * (size_t)nn * neq, hipMemcpyDeviceToDevice, c->stream));
@@ -471,13 +509,17 @@ int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
     HIPCHK(hipMemcpyAsync(c->fine4, fine2, sizeof(int) * 4 * (size_t)nfine, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->nanflag, 0, sizeof(int), c->stream));
   HIPCHK(hipMemsetAsync(c->lim, 0, (size_t)nn, c->stream));
-  // the forest arrays: keep the new ones (owned by the context from here on)
-  hipFree(c->lroot); hipFree(c->llevel); hipFree(c->lci); hipFree(c->lSI);
+  // the forest arrays: the new ones belong to the context from here on, the old
+  // ones go to the pool once the copies above are done
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t old_n = (size_t)(c->cap_leaf > 0 ? c->cap_leaf : 1);
+  if (c->lroot) c->pool.push_back({c->lroot, old_n * sizeof(int)});
+  if (c->llevel) c->pool.push_back({c->llevel, old_n * sizeof(int)});
+  if (c->lci) c->pool.push_back({c->lci, 3 * old_n * sizeof(int)});
+  if (c->lSI) c->pool.push_back({c->lSI, 2 * old_n * sizeof(int)});
   c->lroot = root2; c->llevel = level2; c->lci = ci2; c->lSI = SI2;
   c->cap_leaf = nn;
-  for (void *&q : T.p)
-    if (q == root2 || q == level2 || q == ci2 || q == SI2) q = nullptr;
-  HIPCHK(hipStreamSynchronize(c->stream));
+  T.keep(root2); T.keep(level2); T.keep(ci2); T.keep(SI2);
   return FV_OK;
 }
 
@@ -548,7 +590,7 @@ extern "C" int fv_dev_criterionsmth(fv_dev *c, int LEVEL, int *split) {
   int rc = need_forest(c);
   if (rc != FV_OK) return rc;
   if (!split) return fv_set_err(FV_ERR_ARG, "null argument");
-  Tmp T;
+  Tmp T(c);
   int *d;
   HIPCHK(T.get(&d, c->n));
   HIPCHK(hipMemcpy(d, split, sizeof(int) * c->n, hipMemcpyHostToDevice));
@@ -560,7 +602,7 @@ extern "C" int fv_dev_regrid(fv_dev *c, const fv_crit *cr, int maxlevel, int *n_
   int rc = need_forest(c);
   if (rc != FV_OK) return rc;
   if (!cr) return fv_set_err(FV_ERR_ARG, "null argument");
-  Tmp T;
+  Tmp T(c);
   int *SIdev = nullptr;
   if ((rc = fv_dev_indicator_raw(c, cr, &SIdev)) != FV_OK) return rc;
   // the new interface flags become the mesh's (interface_location before meshadaptation)
