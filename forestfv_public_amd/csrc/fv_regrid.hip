@@ -423,10 +423,10 @@ int smooth(fv_dev *c, Tmp &T, int LEVEL, int *split) {
     HIPCHK(hipMemcpyAsync(split, cv, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
     int *yin = split, *yout = y2;
     // sweeps past the fixpoint change nothing, so the host checks only every
-    // third sweep whether the last one still changed a value
+    // second sweep whether the last one still changed a value
     for (int it = 0;; it++) {
       if (it > 64) return fv_set_err(FV_ERR_MESH, "criterionsmth: smoothing sweeps did not settle");
-      const bool check = (it % 3) == 2;
+      const bool check = (it % 2) == 1;
       if (check) HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
       hipLaunchKernelGGL(k_smooth_earlier, grid_of(n), dim3(BLK), 0, c->stream, n, c->nbr, c->info, c->fine4, cv,
                          yin, yout, flag);
