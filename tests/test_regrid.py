@@ -131,6 +131,36 @@ def test_device_regrid_merges_and_splits_random_targets():
     del rng
 
 
+@pytest.mark.parametrize("nroot,bc", [((2, 2, 1), (1, 2, 5, 4, 4, 1)), ((3, 1, 2), (4, 4, 2, 2, 5, 5))])
+def test_device_regrid_on_a_brick_of_roots(nroot, bc):
+    """Root trees side by side: neighbours across root faces (root_across in the
+    connectivity), 2:1 faces between roots and every BC code on the outer sides."""
+    p = sod_params(prelax=1, bc=bc)
+    m = random_amr_mesh(p, base_level=2, max_level=4, nroot=nroot, seed=7, frac=0.3, passes=2)
+    mh = random_amr_mesh(p, base_level=2, max_level=4, nroot=nroot, seed=7, frac=0.3, passes=2)
+    c = m.centres()
+    S = np.zeros((m.n, F.neq(2)))
+    for i in range(m.n):
+        mat = 1 if c[i, 0] + 0.3 * c[i, 1] > 0.6 else 0
+        S[i] = F.init_pure(p, (1.0, 0.125), mat, (1.0, 0.1)[mat], (0.1 * c[i, 2], 0.05, 0.0))
+    crit = F.make_crit(uniform_on=1, uniform_lvl=2, rho_on=1, rho_lvl=5, rho_thr=0.05, inter0_on=1, inter0_lvl=5,
+                       inter0_lrs=3)
+    devh, devd = F.Device(p, mh), F.Device(p, m)
+    devh.set_state(S)
+    devd.set_state(S)
+    devd.set_forest(m)
+    for cycle in range(3):
+        Sh = host_regrid(devh, mh, p, crit, 5)
+        devd.regrid(crit, 5)
+        say(f"[regrid brick] nroot={nroot} cycle {cycle}: {mh.n} leaves")
+        assert_same_mesh(devd, mh, Sh)
+        devh.advance(1)
+        devd.advance(1)
+        assert np.array_equal(devd.get_state(), devh.get_state())
+    devh.close()
+    devd.close()
+
+
 def test_device_regrid_is_much_faster_than_the_host_path():
     p = sod_params(level=7)
     m = F.Mesh.uniform((1, 1, 1), (0.0, 0.0, 0.0), 1.0, 7, list(p.bc))      # 64^3 = 262144 leaves
