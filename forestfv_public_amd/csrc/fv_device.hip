@@ -292,8 +292,8 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_rk(int n, Params P, Geo G, in
 // pressure recovery and RK accumulation)
 // ---------------------------------------------------------------------
 #pragma clang fp contract(fast)
-template <int NM, bool MIXED>
-__global__ __launch_bounds__(FV_BLOCK) void k_recon_fast(int n, Params P, Geo G, const double *__restrict__ S,
+template <int NM, bool MIXED, bool XR>
+__global__ __launch_bounds__(FV_BLOCK, 2) void k_recon_fast(int n, Params P, Geo G, const double *__restrict__ S,
                                                          const int *__restrict__ nbr, const uint32_t *__restrict__ info,
                                                          const int *__restrict__ fine4, const uint8_t *__restrict__ lim,
                                                          double *__restrict__ SF, float *__restrict__ SFm) {
@@ -355,11 +355,11 @@ __global__ __launch_bounds__(FV_BLOCK) void k_recon_fast(int n, Params P, Geo G,
       }
       if (MIXED) {
         float rec[fvf::Rec<NM>::N];
-        fvf::relax_to_record<NM>(P, M, a, F, rec);
+        fvf::relax_to_record<NM, XR>(P, M, a, F, rec);
 #pragma unroll
         for (int q = 0; q < fvf::Rec<NM>::N; q++) SFm[((size_t)f * fvf::Rec<NM>::N + q) * n + i] = rec[q];
       } else {
-        fvf::relax_fast<NM>(P, M, F);
+        fvf::relax_fast<NM, XR>(P, M, F);
 #pragma unroll
         for (int v = 0; v < L::NEQ; v++) SF[((size_t)f * L::NEQ + v) * n + i] = F[v];
       }
@@ -382,7 +382,7 @@ FV_DEV void load_sol(const fvf::Mat<NM, double> &Md, const fvf::Mat<NM, R> &M, d
   }
 }
 
-template <int NM, bool MIXED>
+template <int NM, bool MIXED, bool XR>
 __global__ __launch_bounds__(FV_BLOCK) void k_flux_fast(int n, Params P, Geo G, int cs, double *__restrict__ S,
                                                         double *__restrict__ SN, const double *__restrict__ SF,
                                                         const float *__restrict__ SFm, const int *__restrict__ nbr,
@@ -499,7 +499,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_fast(int n, Params P, Geo G, 
     else sn = SN[(size_t)v * n + i];
     Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
   }
-  fvf::relax_fast<NM>(P, Md, Sn);
+  fvf::relax_fast<NM, XR>(P, Md, Sn);
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = Sn[v];
 }
@@ -538,7 +538,7 @@ FV_DEV bool one_neighbour(int kind) {
   return kind == FV_FK_SAME || kind == FV_FK_MIRROR || kind == FV_FK_COARSE;
 }
 
-template <int NM, bool MIXED>
+template <int NM, bool MIXED, bool XR>
 __global__ __launch_bounds__(FV_BLOCK) void k_flux_pipe(int n, Params P, Geo G, int cs, double *__restrict__ S,
                                                         double *__restrict__ SN, const double *__restrict__ SF,
                                                         const float *__restrict__ SFm, const int *__restrict__ nbr,
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_pipe(int n, Params P, Geo G, 
     else sn = SN[(size_t)v * n + i];
     Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
   }
-  fvf::relax_fast<NM>(P, Md, Sn);
+  fvf::relax_fast<NM, XR>(P, Md, Sn);
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = Sn[v];
 }
@@ -698,7 +698,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_pipe(int n, Params P, Geo G, 
 
 // k_recon_fast's face-state body: Sc avf-limited cell state, Xg / Xo the
 // stencil values across face g and across its opposite face
-template <int NM, bool MIXED>
+template <int NM, bool MIXED, bool XR>
 FV_DEV void recon_one(const Params &P, const fvf::Mat<NM, double> &M, const Geo &G, double a, int limiter, int lvl,
                       int kg, int ko, const double *Sc, const double *Xg, const double *Xo, double *F) {
   using L = Lay<NM>;
@@ -719,17 +719,17 @@ FV_DEV void recon_one(const Params &P, const fvf::Mat<NM, double> &M, const Geo 
     vf_limit_face<NM, double>(a, F);
   }
   // the mixed path relaxes while it forms the FP32 record (state_sol)
-  if constexpr (!MIXED) fvf::relax_fast<NM>(P, M, F);
+  if constexpr (!MIXED) fvf::relax_fast<NM, XR>(P, M, F);
 }
 
 // face state -> solution used by HLLC (through the FP32 record when MIXED,
 // where the state arrives unrelaxed and is relaxed here, as k_recon_fast does)
-template <int NM, bool MIXED, typename R>
+template <int NM, bool MIXED, typename R, bool XR>
 FV_DEV void state_sol(const Params &P, const fvf::Mat<NM, double> &Md, const fvf::Mat<NM, R> &M, double a, double *F,
                       fvf::SolF<NM, R> &s) {
   if constexpr (MIXED) {
     float rec[fvf::Rec<NM>::N];
-    fvf::relax_to_record<NM>(P, Md, a, F, rec);
+    fvf::relax_to_record<NM, XR>(P, Md, a, F, rec);
     fvf::fill_rec<NM>(M, rec, s);
   } else {
     fvf::fill_cons<NM, R>(Md, a, F, s);
@@ -741,7 +741,7 @@ FV_DEV int leaf_limiter(const Params &P, const uint8_t *__restrict__ lim, int j)
 }
 
 // reconstructed state of any leaf j at its face g, from the cell states
-template <int NM, bool MIXED>
+template <int NM, bool MIXED, bool XR>
 FV_DEV void face_state_of(const Params &P, const fvf::Mat<NM, double> &M, const Geo &G, double a,
                           const double *__restrict__ S, int n, const int *__restrict__ nbr,
                           const uint32_t *__restrict__ info, const int *__restrict__ fine4,
@@ -758,7 +758,7 @@ FV_DEV void face_state_of(const Params &P, const fvf::Mat<NM, double> &M, const 
   const int kg = face_kind(inf, g), ko = face_kind(inf, go);
   stencil_value<NM>(S, n, j, kg, nbr[(size_t)g * n + j], fine4, Sc, Araw, a, Xg);
   stencil_value<NM>(S, n, j, ko, nbr[(size_t)go * n + j], fine4, Sc, Araw, a, Xo);
-  recon_one<NM, MIXED>(P, M, G, a, leaf_limiter(P, lim, j), cell_level(inf), kg, ko, Sc, Xg, Xo, F);
+  recon_one<NM, MIXED, XR>(P, M, G, a, leaf_limiter(P, lim, j), cell_level(inf), kg, ko, Sc, Xg, Xo, F);
 }
 
 // blocks of one XCD (workgroups are dealt to the 8 XCDs round robin) take a
@@ -769,7 +769,7 @@ FV_DEV int xcd_leaf(void) {
   return (int)(b * blockDim.x + threadIdx.x);
 }
 
-template <int NM, bool MIXED>
+template <int NM, bool MIXED, bool XR>
 __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, const Geo *__restrict__ Gp, int cs,
                                                           const double *__restrict__ S, double *__restrict__ Sout,
                                                           double *__restrict__ SN, const int *__restrict__ nbr,
@@ -830,14 +830,14 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
           for (int k = 0; k < L::N3; k++) Arj[k] = S[(size_t)(L::I3 + k) * n + nb];
         }
         stencil_value<NM>(S, n, nb, kj, nbr[(size_t)f * n + nb], fine4, Xf, Arj, a, X2);
-        recon_one<NM, MIXED>(P, Md, G, a, leaf_limiter(P, lim, nb), lvl, kjo, kj, Xf, Sc, X2, Fnb);
-        state_sol<NM, MIXED, R>(P, Md, M, a, Fnb, sA);
+        recon_one<NM, MIXED, XR>(P, Md, G, a, leaf_limiter(P, lim, nb), lvl, kjo, kj, Xf, Sc, X2, Fnb);
+        state_sol<NM, MIXED, R, XR>(P, Md, M, a, Fnb, sA);
       }
       {
         double Xo[L::NEQ], Fown[L::NEQ];
         stencil_value<NM>(S, n, i, kopp, nbr[(size_t)fo * n + i], fine4, Sc, Araw, a, Xo);
-        recon_one<NM, MIXED>(P, Md, G, a, limiter, lvl, kind, kopp, Sc, Xf, Xo, Fown);
-        state_sol<NM, MIXED, R>(P, Md, M, a, Fown, sB);
+        recon_one<NM, MIXED, XR>(P, Md, G, a, limiter, lvl, kind, kopp, Sc, Xf, Xo, Fown);
+        state_sol<NM, MIXED, R, XR>(P, Md, M, a, Fown, sB);
       }
       R out[L::NEQ], un, vn, wn;
       const double A = G.area[lvl];
@@ -875,8 +875,8 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
       {
         double Xo[L::NEQ], Fown[L::NEQ];
         stencil_value<NM>(S, n, i, kopp, nbr[(size_t)fo * n + i], fine4, Sc, Araw, a, Xo);
-        recon_one<NM, MIXED>(P, Md, G, a, limiter, lvl, kind, kopp, Sc, Xf, Xo, Fown);
-        state_sol<NM, MIXED, R>(P, Md, M, a, Fown, sL);
+        recon_one<NM, MIXED, XR>(P, Md, G, a, limiter, lvl, kind, kopp, Sc, Xf, Xo, Fown);
+        state_sol<NM, MIXED, R, XR>(P, Md, M, a, Fown, sL);
       }
       for (int k = 0; k < Nneig; ++k) {
         int bcode = 0;
@@ -895,8 +895,8 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
         } else {
           const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
           double Fnb[L::NEQ];
-          face_state_of<NM, MIXED>(P, Md, G, a, S, n, nbr, info, fine4, lim, j, fo, Fnb);
-          state_sol<NM, MIXED, R>(P, Md, M, a, Fnb, sR);
+          face_state_of<NM, MIXED, XR>(P, Md, G, a, S, n, nbr, info, fine4, lim, j, fo, Fnb);
+          state_sol<NM, MIXED, R, XR>(P, Md, M, a, Fnb, sR);
         }
         R out[L::NEQ], un, vn, wn;
         fvf::hllc<NM, R>(M, P.fix, fr, bcode, sL, sR, out, un, vn, wn);
@@ -938,7 +938,7 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
     else sn = SN[(size_t)v * n + i];
     Sn[v] = wa * sn + wb * (s0 - dtv * RHS[v]);
   }
-  fvf::relax_fast<NM>(P, Md, Sn);
+  fvf::relax_fast<NM, XR>(P, Md, Sn);
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) Sout[(size_t)v * n + i] = Sn[v];
 }
@@ -1364,7 +1364,7 @@ static int stage_case(int rk, int irk) {
   return irk;
 }
 
-template <int NM>
+template <int NM, bool XR>
 static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
   const int n = c->n;
   const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
@@ -1379,19 +1379,19 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
     const dim3 gx((nb + 7u) & ~7u);   // a multiple of 8 for the XCD-contiguous leaf mapping
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
     if (c->prec == FV_PREC_FP64)
-      hipLaunchKernelGGL((k_stage_fused<NM, false>), gx, block, 0, c->stream, n, c->P, c->Gdev, cs, c->S, c->S2, c->SN,
+      hipLaunchKernelGGL((k_stage_fused<NM, false, XR>), gx, block, 0, c->stream, n, c->P, c->Gdev, cs, c->S, c->S2, c->SN,
                          c->nbr, c->info, c->fine4, c->lim, c->nanflag);
     else
-      hipLaunchKernelGGL((k_stage_fused<NM, true>), gx, block, 0, c->stream, n, c->P, c->Gdev, cs, c->S, c->S2, c->SN,
+      hipLaunchKernelGGL((k_stage_fused<NM, true, XR>), gx, block, 0, c->stream, n, c->P, c->Gdev, cs, c->S, c->S2, c->SN,
                          c->nbr, c->info, c->fine4, c->lim, c->nanflag);
     double *t = c->S;
     c->S = c->S2;
     c->S2 = t;
   } else if (c->prec == FV_PREC_FP64) {
-    hipLaunchKernelGGL((k_recon_fast<NM, false>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
+    hipLaunchKernelGGL((k_recon_fast<NM, false, XR>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
                        c->fine4, c->lim, c->SF, c->SFm);
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
-    hipLaunchKernelGGL((k_flux_fast<NM, false>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
+    hipLaunchKernelGGL((k_flux_fast<NM, false, XR>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
                        c->SFm, c->nbr, c->info, c->fine4, c->nanflag);
   } else {
     if (c->cap_m < n) {
@@ -1401,12 +1401,12 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
       HIPCHK(hipMalloc(&c->SFm, sizeof(float) * (size_t)n * 6 * fvf::Rec<NM>::N));
       c->cap_m = n;
     }
-    hipLaunchKernelGGL((k_recon_fast<NM, true>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
+    hipLaunchKernelGGL((k_recon_fast<NM, true, XR>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
                        c->fine4, c->lim, c->SF, c->SFm);
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
     // the pipelined flux needs more registers: 2 waves per SIMD with FP32 face
     // records (8 % faster), 1 with FP64 ones (35 % slower), so FP64 keeps k_flux_fast
-    hipLaunchKernelGGL((k_flux_pipe<NM, true>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
+    hipLaunchKernelGGL((k_flux_pipe<NM, true, XR>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
                        c->SFm, c->nbr, c->info, c->fine4, c->nanflag);
   }
   return FV_OK;
@@ -1414,7 +1414,11 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
 
 static int launch_stage(fv_dev *c, int irk, hipEvent_t *ev_flux0, hipEvent_t *ev_flux1) {
   const int cs = stage_case(c->P.rk, irk);
-  int rc = (c->nmat == 2) ? launch_stage_nm<2>(c, cs, ev_flux0) : launch_stage_nm<3>(c, cs, ev_flux0);
+  // PRELAX 2 and 3 iterate; their kernels are separate instances so that the
+  // closed-form relaxations do not carry the iterations' code and registers
+  const bool xr = c->P.prelax >= 2;
+  int rc = (c->nmat == 2) ? (xr ? launch_stage_nm<2, true>(c, cs, ev_flux0) : launch_stage_nm<2, false>(c, cs, ev_flux0))
+                          : (xr ? launch_stage_nm<3, true>(c, cs, ev_flux0) : launch_stage_nm<3, false>(c, cs, ev_flux0));
   if (rc != FV_OK) return rc;
   if (ev_flux1) HIPCHK(hipEventRecord(*ev_flux1, c->stream));
   HIPCHK(hipGetLastError());

@@ -250,10 +250,16 @@ FV_DEV void hllc(const Mat<NM, R> &M, int fix, const R *F, int bc, const SolF<NM
 }
 
 // m4_relaxationvec.c for PRELAX 0 and 1 (the Newton forms PRELAX 2 and 3 use fvk::relax)
-template <int NM>
+// XR selects the iterative relaxations (PRELAX 2 and 3), which go to the exact
+// Newton forms; a kernel instantiated without them carries none of their code
+// and registers
+template <int NM, bool XR>
 FV_DEV void relax_fast(const Params &Pr, const Mat<NM, double> &M, double *vec) {
   using L = Lay<NM>;
-  if (Pr.prelax == 2 || Pr.prelax == 3) { fvk::relax<NM>(Pr, vec); return; }
+  if constexpr (XR) {
+    fvk::relax<NM>(Pr, vec);
+    return;
+  }
   const double a = Pr.amin / 100.0;
   double ar[NM], are[NM], avf[NM + 1], rho = 0.0;
 #pragma unroll
@@ -340,12 +346,12 @@ FV_DEV void relax_record(const Params &Pr, const Mat<NM, double> &M, double *vec
 }
 
 // relaxation of a reconstructed face state and its FP32 record
-template <int NM>
+template <int NM, bool XR>
 FV_DEV void relax_to_record(const Params &Pr, const Mat<NM, double> &M, double amin100, double *vec, float *rec) {
-  if constexpr (NM == 2) {
+  if constexpr (NM == 2 && !XR) {
     if (Pr.prelax == 1) { relax_record<NM>(Pr, M, vec, rec); return; }
   }
-  relax_fast<NM>(Pr, M, vec);
+  relax_fast<NM, XR>(Pr, M, vec);
   to_record<NM>(M, amin100, vec, rec);
 }
 
