@@ -125,6 +125,9 @@ class fv_m5_params(C.Structure):
         ("avf_lim", C.c_double),
         ("dt", C.c_double),
         ("zerou", C.c_int * 3),
+        ("prelax", C.c_int),
+        ("order", C.c_int),
+        ("rk_steps", C.c_int),
     ]
 
 
@@ -286,7 +289,7 @@ def m5_neq(nmat):
 
 
 def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush=(2.6e10, 0.0, 0.0), amin=1e-6,
-                   avf_lim=0.5, dt=1e-9, zerou=(0, 0, 0)):
+                   avf_lim=0.5, dt=1e-9, zerou=(0, 0, 0), prelax=1, order=2, rk_steps=3):
     """Model-5 material table (MATERGAMA, MATERPINF, MATERMUSH) and stage constants."""
     p = fv_m5_params()
     p.nmat = nmat
@@ -295,6 +298,7 @@ def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush
     p.amin, p.avf_lim, p.dt = amin, avf_lim, dt
     for k in range(3):
         p.zerou[k] = zerou[k]
+    p.prelax, p.order, p.rk_steps = prelax, order, rk_steps
     return p
 
 

@@ -35,6 +35,9 @@ typedef struct {
   double avf_lim;              /* AVF_LIM */
   double dt;                   /* DT */
   int    zerou[3];             /* ZEROU0, ZEROU1, ZEROU2: freeze a velocity component */
+  int    prelax;               /* PRELAX of m5_relaxationvec: 0, 1, 2 or 3 (stage entry points) */
+  int    order;                /* ORDER: 1 cell states, 2 MUSCL face states (stage entry points) */
+  int    rk_steps;             /* RK_STEPS: stages 1..rk_steps of m5_RK per step (fv_m5_dev_advance) */
 } fv_m5_params;
 
 int fv_m5_neq(int nmat);

@@ -10,6 +10,7 @@
  */
 #include "fv_oracle_m5.h"
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define SQ(x) ((x) * (x))
@@ -42,6 +43,7 @@ typedef struct {
   double r, e, u, v, w, phi0;
   double ra[ORC_MAXMAT], avf[ORC_MAXMAT], are[ORC_MAXMAT], c2Y[ORC_MAXMAT], p_hydro[ORC_MAXMAT];
   double Amat[9], st[3][3];
+  double ast[ORC_MAXMAT][3][3];   /* avf_stress of each material */
   double vec[ORC_M5_MAXNEQ];
   double u_temp, v_temp, w_temp;
 } rec5_t;
@@ -128,10 +130,17 @@ static void eos_stress_v2(double s[3][3], double avf, double pr, const double *G
 }
 
 /* ------------------------------------------------------------------ */
-/* face record (m5_facercnstr.c:3-190; m5_facevalues.c differs only in  */
-/* the order of the two clips of avf_c_temp, which gives the same value) */
+/* face records                                                         */
 /* ------------------------------------------------------------------ */
-static void m5_record(const lay5_t *L, const orc_m5_params *p, const double *S, rec5_t *r) {
+/* face = 0: m5_facevalues.c:3-188, the record of a cell state S (C = S).
+ * face = 1: m5_facercnstr.c:3-190, the record of a face state S whose phi
+ * comes from the cell state C; its pressure and c2Y divide by the re-summed
+ * fraction r->avf[i] where m5_facevalues divides by avf_temp[i] (the two
+ * differ only for the last material after the renormalisation).  The two
+ * clips of avf_c_temp come in opposite orders in the two files, which gives
+ * the same value. */
+static void m5_record_x(const lay5_t *L, const orc_m5_params *p, const double *S, const double *C, int face,
+                        rec5_t *r) {
   double avf_temp[ORC_MAXMAT + 1], avf_c_temp[ORC_MAXMAT + 1];
   const double a = p->amin / 100.0;
   r->r = 0.0;
@@ -142,7 +151,7 @@ static void m5_record(const lay5_t *L, const orc_m5_params *p, const double *S, 
   r->vec[L->i1 + 0] = S[L->i1 + 0]; r->vec[L->i1 + 1] = S[L->i1 + 1]; r->vec[L->i1 + 2] = S[L->i1 + 2];
   r->e = S[L->i2] / r->r;
   r->vec[L->i2] = S[L->i2];
-  for (int i = 0; i < L->n3; ++i) { avf_temp[i] = S[L->i3 + i]; avf_c_temp[i] = S[L->i3 + i]; }
+  for (int i = 0; i < L->n3; ++i) { avf_temp[i] = S[L->i3 + i]; avf_c_temp[i] = C[L->i3 + i]; }
   double avfsum = 0.0;
   for (int i = 0; i < L->n3; ++i) {
     if (avf_temp[i] <= a) avf_temp[i] = a;
@@ -182,14 +191,18 @@ static void m5_record(const lay5_t *L, const orc_m5_params *p, const double *S, 
   for (int i = 0; i < L->n4; ++i) {
     double are_elast = (p->mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, p->mush[i]) : 0.0;
     double g = p->gamma[i], pi = p->pinf[i];
-    double press = ((g - 1.0) * (r->are[i] - are_elast) / avf_temp[i]) - g * pi;
-    r->c2Y[i] = (avf_temp[i] / r->r) * g * (press + pi);
+    const double den = face ? r->avf[i] : avf_temp[i];
+    double press = ((g - 1.0) * (r->are[i] - are_elast) / den) - g * pi;
+    r->c2Y[i] = (den / r->r) * g * (press + pi);
     r->p_hydro[i] = press;
-    double s[3][3];
-    eos_stress_v2(s, r->avf[i], r->p_hydro[i], G, Gmag, p->mush[i]);
+    eos_stress_v2(r->ast[i], r->avf[i], r->p_hydro[i], G, Gmag, p->mush[i]);
     for (int a1 = 0; a1 < 3; ++a1)
-      for (int b = 0; b < 3; ++b) r->st[a1][b] += s[a1][b];
+      for (int b = 0; b < 3; ++b) r->st[a1][b] += r->ast[i][a1][b];
   }
+}
+
+static void m5_record(const lay5_t *L, const orc_m5_params *p, const double *S, rec5_t *r) {
+  m5_record_x(L, p, S, S, 0, r);
 }
 
 void orc_m5_record(const orc_m5_params *p, const double *vec, double *out) {
@@ -646,4 +659,735 @@ int orc_m5_gfm_rel(const orc_mesh *m, const orc_cell *cells, const orc_m5_params
     }
   }
   return 0;
+}
+
+/* ================================================================== */
+/* the whole model-5 stage                                              */
+/* ================================================================== */
+static const int FCOPP5[6] = {2, 3, 0, 1, 5, 4};   /* operatetp.c:268-279 */
+
+static fr5_t frame_of_cell(const orc_cell *cl, int f) {
+  fr5_t F = {cl->nx[f], cl->ny[f], cl->nz[f], cl->nxt1[f], cl->nyt1[f], cl->nzt1[f],
+             cl->nxt2[f], cl->nyt2[f], cl->nzt2[f]};
+  return F;
+}
+
+static void gmat_of(const double *A, double *G, double *Gmag) {
+  G[0] = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+  G[1] = A[0] * A[3] + A[1] * A[4] + A[2] * A[5];
+  G[2] = A[0] * A[6] + A[1] * A[7] + A[2] * A[8];
+  G[3] = A[3] * A[0] + A[4] * A[1] + A[5] * A[2];
+  G[4] = A[3] * A[3] + A[4] * A[4] + A[5] * A[5];
+  G[5] = A[3] * A[6] + A[4] * A[7] + A[5] * A[8];
+  G[6] = A[6] * A[0] + A[7] * A[1] + A[8] * A[2];
+  G[7] = A[6] * A[3] + A[7] * A[4] + A[8] * A[5];
+  G[8] = A[6] * A[6] + A[7] * A[7] + A[8] * A[8];
+  *Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) + G[2] * (G[3] * G[7] - G[4] * G[6]);
+}
+
+/* ------------------------------------------------------------------ */
+/* relaxation (m5_relaxationvec.c)                                      */
+/* ------------------------------------------------------------------ */
+/* the volume-fraction limit after a relaxation (m5_relaxationvec.c:131-150):
+ * two independent clips, renormalised when the sum reaches 1 - AMIN/100 */
+static void vf_limit2(const lay5_t *L, double a, double *avf) {
+  double avfsum = 0.0;
+  for (int iv = 0; iv < L->n3; ++iv) {
+    if (avf[iv] <= a) avf[iv] = a;
+    if (avf[iv] >= (1.0 - a)) avf[iv] = 1.0 - a;
+    avfsum += avf[iv];
+  }
+  if (avfsum >= (1.0 - a)) {
+    double dif = avfsum - (1.0 - a);
+    for (int j = 0; j < L->n3; ++j) { double per = avf[j] / avfsum; avf[j] = avf[j] - per * dif; }
+    avfsum = 1.0 - a;
+  }
+  avf[L->n3] = 1.0 - avfsum;
+}
+
+/* the same limit with else-if clips (m5_relaxationvec.c:44-60 and :512-529) */
+static void vf_limit1(const lay5_t *L, double a, double *avf) {
+  double avfsum = 0.0;
+  for (int iv = 0; iv < L->n3; ++iv) {
+    if (avf[iv] <= a) avf[iv] = a;
+    else if (avf[iv] >= (1.0 - a)) avf[iv] = 1.0 - a;
+    avfsum += avf[iv];
+  }
+  if (avfsum >= (1.0 - a)) {
+    double dif = avfsum - (1.0 - a);
+    for (int j = 0; j < L->n3; ++j) { double per = avf[j] / avfsum; avf[j] = avf[j] - per * dif; }
+    avfsum = 1.0 - a;
+  }
+  avf[L->n3] = 1.0 - avfsum;
+}
+
+/* F_pe.c (i_paper = 1, Favrie) on the branch's own arrays */
+static double m5_fpe(int nm, const double *G, const double *P, const double *ar, const double *t0, const double *p0,
+                     double pe) {
+  double F = 0.0;
+  for (int iv = 0; iv < nm; ++iv)
+    F += ar[iv] * t0[iv] * ((p0[iv] + G[iv] * P[iv] + (G[iv] - 1.0) * pe) / (pe + G[iv] * P[iv] + (G[iv] - 1.0) * pe));
+  return F + -1.0;
+}
+static double m5_dfpe(int nm, const double *G, const double *P, const double *ar, const double *t0, const double *p0,
+                      double pe) {
+  double D = 0.0;
+  for (int iv = 0; iv < nm; ++iv) {
+    double c1 = ar[iv] * t0[iv];
+    double f = (p0[iv] + G[iv] * P[iv] + (G[iv] - 1.0) * pe);
+    double g = (pe + G[iv] * P[iv] + (G[iv] - 1.0) * pe);
+    D += (c1 * (G[iv] - 1.0) * g - c1 * f * G[iv]) / SQ(g);
+  }
+  return D;
+}
+
+static int m5_relaxvec(const lay5_t *L, const orc_m5_params *p, double *vec) {
+  const double a = p->amin / 100.0, eps_sml = 1.0e-12, eps_big = 1.0e-8;
+  const double *Gm = p->gamma, *Pi = p->pinf, *Mu = p->mush;
+  const int nm = L->nmat;
+  double ar[ORC_MAXMAT], are[ORC_MAXMAT], avf[ORC_MAXMAT + 1], avf_init[ORC_MAXMAT + 1];
+  double are_el[ORC_MAXMAT] = {0}, A[9], G[9], Gmag;
+  double rho = 0.0;
+  for (int iv = 0; iv < L->n0; ++iv) { ar[iv] = vec[iv + L->i0]; are[iv] = vec[iv + L->i4]; rho += ar[iv]; }
+  const double u = vec[L->i1 + 0] / rho, v = vec[L->i1 + 1] / rho, w = vec[L->i1 + 2] / rho;
+  const double rE = vec[L->i2];
+  for (int iv = 0; iv < L->n3; ++iv) avf[iv] = vec[iv + L->i3];
+  vf_limit1(L, a, avf);
+  for (int iv = 0; iv <= L->n3; ++iv) avf_init[iv] = avf[iv];
+  for (int k = 0; k < 9; ++k) A[k] = vec[L->i5 + k];
+  gmat_of(A, G, &Gmag);
+  double p_relaxed;
+  if (p->prelax == 1 || p->prelax == 2) {
+    double t0[ORC_MAXMAT], p0[ORC_MAXMAT], pe;
+    if (p->prelax == 1) {   /* m5_relaxationvec.c:89-190 */
+      for (int iv = 0; iv < L->n0; ++iv) {
+        double ael = (Mu[iv] != 0.0) ? elastic_energy_v2(G, Gmag, Mu[iv]) : 0.0;
+        t0[iv] = 1.0 / (ar[iv] / avf[iv]);
+        p0[iv] = ((Gm[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - Gm[iv] * Pi[iv];
+      }
+      double A_1 = (avf[0] / Gm[0]) * (p0[0] + Pi[0]) / ((avf[0] / Gm[0]) + (avf[1] / Gm[1]));
+      double A_2 = (avf[1] / Gm[1]) * (p0[1] + Pi[1]) / ((avf[0] / Gm[0]) + (avf[1] / Gm[1]));
+      pe = 0.5 * (A_1 + A_2 - (Pi[0] + Pi[1])) + sqrt(0.25 * (SQ((A_2 - A_1 - (Pi[1] - Pi[0])))) + A_1 * A_2);
+      if (isnan(pe)) return -1;
+    } else {                /* m5_relaxationvec.c:192-363 */
+      double max_avf = -1.0;
+      int principal = 0;
+      for (int iv = 0; iv < L->n0; ++iv) {
+        double ael = (Mu[iv] != 0.0) ? elastic_energy_v2(G, Gmag, Mu[iv]) : 0.0;
+        t0[iv] = 1.0 / (ar[iv] / avf[iv]);
+        p0[iv] = ((Gm[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - Gm[iv] * Pi[iv];
+        if (p0[iv] < -(1.0 - eps_big) * Pi[iv] + eps_big) p0[iv] = -(1.0 - eps_big) * Pi[iv] + 1.0;
+        if (avf[iv] > max_avf) { max_avf = avf[iv]; principal = iv; }
+      }
+      double pe_old = 0.001, pe_new = pe_old;
+      int iter = 0;
+      double cond_1 = fabs(pe_new - pe_old);
+      double cond_2 = fabs(m5_fpe(nm, Gm, Pi, ar, t0, p0, pe_new));
+      while (((cond_2 >= 1.0e-12) || (cond_1 >= 1.0e-4)) && (iter <= 101)) {
+        pe_new = pe_old - 0.8 * (m5_fpe(nm, Gm, Pi, ar, t0, p0, pe_old) / m5_dfpe(nm, Gm, Pi, ar, t0, p0, pe_old));
+        cond_1 = fabs(pe_new - pe_old);
+        cond_2 = fabs(m5_fpe(nm, Gm, Pi, ar, t0, p0, pe_new));
+        if (iter >= 100 && ((fabs(pe_old - pe_new) < 1.0e3) || isnan(pe_new))) return -3;
+        pe_old = pe_new;
+        iter++;
+      }
+      if (isnan(pe_new)) return -1;
+      if (pe_new < (-(1.0 - eps_big) * Pi[principal] + eps_big)) pe_new = -(1.0 - eps_big) * Pi[principal] + 1.0;
+      pe = pe_new;
+    }
+    for (int iv = 0; iv < L->n0; ++iv) {
+      double t1 = t0[iv] * (p0[iv] + Gm[iv] * Pi[iv] + (Gm[iv] - 1.0) * pe) / (pe + Gm[iv] * Pi[iv] + (Gm[iv] - 1.0) * pe);
+      double r_m = 1.0 / t1;
+      avf[iv] = ar[iv] / r_m;
+    }
+    vf_limit2(L, a, avf);
+    for (int i = 0; i < 9; i++) A[i] = A[i] * (avf[0] / avf_init[0]);
+  } else if (p->prelax == 3) {   /* m5_relaxationvec.c:365-560 */
+    double rhoKs[ORC_MAXMAT] = {0}, pk[ORC_MAXMAT];
+    for (int iv = 0; iv < nm; ++iv) {
+      if (avf[iv] < 0.0 || ar[iv] < 0.0) { avf[iv] = eps_sml; ar[iv] = eps_sml; are[iv] = eps_sml; }
+      if (avf[iv] >= 1.0) avf[iv] = 1.0 - eps_sml;
+    }
+    int irelax = 1;
+    for (int iv = 0; iv < nm; ++iv)
+      if (avf[iv] > 1.0 - eps_sml) irelax = 0;
+    if (irelax == 1) {
+      p_relaxed = 0.0;
+      for (int iv = 0; iv < nm; ++iv) {
+        if (avf[iv] > eps_sml) {
+          double ael = (Mu[iv] != 0.0) ? elastic_energy_v2(G, Gmag, Mu[iv]) : 0.0;
+          pk[iv] = ((Gm[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - Gm[iv] * Pi[iv];
+          if (pk[iv] <= -(1.0 - eps_big) * Pi[iv] + eps_big) pk[iv] = -(1.0 - eps_big) * Pi[iv] + eps_big;
+        } else {
+          pk[iv] = 0.0;
+        }
+        p_relaxed += avf[iv] * pk[iv];
+      }
+      int iter = 0;
+      double F = 1.0e-9, dF = 1.0e9;
+      while ((fabs(F) > 1.0e-12) && (iter < 500)) {
+        p_relaxed = p_relaxed - F / dF;
+        iter = iter + 1;
+        for (int iv = 0; iv < nm; ++iv)
+          if (p_relaxed < -(1.0 - eps_big) * Pi[iv] + eps_big) p_relaxed = -(1.0 - eps_big) * Pi[iv] + 1.0;
+        F = -1.0;
+        dF = 0.0;
+        for (int iv = 0; iv < nm; ++iv) {
+          if (avf[iv] > eps_sml) {
+            rhoKs[iv] = ar[iv] / MAXF(avf[iv], eps_sml) * pow((p_relaxed + Pi[iv]) / (pk[iv] + Pi[iv]), 1.0 / Gm[iv]);
+            F = F + ar[iv] / rhoKs[iv];
+            dF = dF - ar[iv] / (Gm[iv] * rhoKs[iv] * (p_relaxed + Pi[iv]));
+          }
+        }
+        if (isnan(p_relaxed) || isnan(F) || isnan(dF)) return -1;
+      }
+      for (int iv = 0; iv < nm; ++iv)
+        if (avf[iv] > eps_sml) avf[iv] = ar[iv] / rhoKs[iv];
+      vf_limit1(L, a, avf);
+    }
+    int i_as = 0;
+    if ((avf[0] < avf[1]) && (Mu[1] != 0.0)) i_as = 1;
+    for (int i = 0; i < 9; i++) A[i] = A[i] * (avf[i_as] / avf_init[i_as]);
+  }
+  if (p->prelax >= 1 && p->prelax <= 3) {
+    /* re-initialisation: pressure from the total energy (m5_relaxationvec.c:157-189 and the same in each branch) */
+    double e_elast = 0.0;
+    for (int iv = 0; iv < nm; ++iv) {
+      are_el[iv] = (Mu[iv] != 0.0) ? elastic_energy_v2(G, Gmag, Mu[iv]) : 0.0;
+      double e_elastic_K = are_el[iv] / (ar[iv]);
+      double Ys = ar[iv] / rho;
+      e_elast += Ys * e_elastic_K;
+    }
+    double e_hyd = rE / rho - 0.5 * (SQ((u)) + SQ((v)) + SQ((w))) - e_elast;
+    double num = 0.0, den = 0.0;
+    for (int iv = 0; iv < L->n0; ++iv) {
+      num += avf[iv] * Gm[iv] * Pi[iv] / (Gm[iv] - 1.0);
+      den += avf[iv] / (Gm[iv] - 1.0);
+    }
+    p_relaxed = (rho * e_hyd - num) / den;
+    if (isnan(p_relaxed)) return -1;
+    for (int iv = 0; iv < L->n0; ++iv)
+      are[iv] = (avf[iv] * (p_relaxed + Gm[iv] * Pi[iv]) / (Gm[iv] - 1.0)) + are_el[iv];
+  }
+  for (int iv = 0; iv < L->n3; ++iv) {
+    if (isnan(avf[iv])) return -1;
+    vec[iv + L->i3] = avf[iv];
+  }
+  for (int iv = 0; iv < L->n4; iv++) {
+    if (isnan(are[iv])) return -1;
+    vec[iv + L->i4] = are[iv];
+  }
+  for (int iv = 0; iv < 9; ++iv) vec[iv + L->i5] = A[iv];
+  return 0;
+}
+
+int orc_m5_relax(const orc_m5_params *p, double *vec) {
+  lay5_t L = mk_lay5(p->nmat);
+  return m5_relaxvec(&L, p, vec);
+}
+
+/* ------------------------------------------------------------------ */
+/* MUSCL face states (m5_faceinterp_V1)                                 */
+/* ------------------------------------------------------------------ */
+/* avf_limit.c:3-21 with max_limit 100 on the fractions of X */
+static void m5_avf_limit(const lay5_t *L, const orc_m5_params *p, double *X) {
+  double avf[ORC_MAXMAT + 1], a = p->amin / 100.0, avfsum = 0.0;
+  for (int iv = 0; iv < L->n3; ++iv) avf[iv] = X[iv + L->i3];
+  for (int v = 0; v < L->n3; ++v) {
+    if (avf[v] <= a) avf[v] = a;
+    else if (avf[v] >= (1.0 - a)) avf[v] = 1.0 - a;
+    avfsum += avf[v];
+  }
+  if (avfsum > (1.0 - a)) {
+    double dif = avfsum - (1.0 - a);
+    for (int j = 0; j < L->n3; ++j) avf[j] -= (avf[j] / avfsum) * dif;
+  }
+  for (int iv = 0; iv < L->n3; ++iv) X[iv + L->i3] = avf[iv];
+}
+
+/* m5_facevalues_MUSCL.c:3-44 with iside = 1: the neighbour's state, the mean
+ * of four finer neighbours, or at a boundary the own cell's record vector
+ * (m5_bc only moves the record's velocity, not its vector) */
+static void m5_stencil(const lay5_t *L, const orc_m5_params *p, const orc_mesh *m, const double *S, int i, int f,
+                       double *X) {
+  const int neq = L->neq;
+  const int bc = m->bc[6 * i + f];
+  if (bc == 0) {
+    const int lfc = m->lfc_neigs[6 * i + f];
+    if (lfc == 1) {
+      const int j = m->neigtr[24 * i + 4 * f];
+      for (int iv = 0; iv < neq; iv++) X[iv] = S[(size_t)j * neq + iv];
+    } else {
+      for (int iv = 0; iv < neq; iv++) X[iv] = 0.0;
+      for (int k = 0; k < lfc; k++) {
+        const int j = m->neigtr[24 * i + 4 * f + k];
+        for (int iv = 0; iv < neq; iv++) X[iv] += S[(size_t)j * neq + iv] / ((double)lfc);
+      }
+    }
+  } else {
+    rec5_t r;
+    m5_record(L, p, &S[(size_t)i * neq], &r);
+    for (int iv = 0; iv < neq; iv++) X[iv] = r.vec[iv];
+  }
+  m5_avf_limit(L, p, X);
+}
+
+/* m5_faceinterp.c:60-93 for leaf i, with m5_deltavalues_V1 (:20-56) and
+ * m5_reconctruct_V1 (:93-190); g_limitedvars is 0 on the alpha-rho-e block
+ * (m5_input.c:196-204) */
+static int m5_faceinterp_leaf(const lay5_t *L, const orc_m5_params *p, const orc_mesh *m, const orc_cell *cl,
+                              const double *S, int i, double *SFi) {
+  const int neq = L->neq;
+  const double a = p->amin / 100.0;
+  double SLc[ORC_M5_MAXNEQ], Xf[ORC_M5_MAXNEQ], Xb[ORC_M5_MAXNEQ];
+  for (int iv = 0; iv < neq; iv++) SLc[iv] = S[(size_t)i * neq + iv];
+  m5_avf_limit(L, p, SLc);
+  for (int f = 0; f < 6; ++f) {
+    const int fo = FCOPP5[f];
+    m5_stencil(L, p, m, S, i, f, Xf);
+    m5_stencil(L, p, m, S, i, fo, Xb);
+    double *SF = &SFi[f * neq];
+    const int bcf = m->bc[6 * i + f];
+    for (int iv = 0; iv < neq; ++iv) {
+      const double df = Xf[iv] - SLc[iv], db = Xb[iv] - SLc[iv];
+      const double lim = (iv >= L->i4 && iv < L->i5) ? 0.0 : 1.0;
+      double g = 0.0;
+      if (bcf == 0 && fabs(df) > 1.0e-10) {
+        const double dp = cl->dist_cc[f], dm = cl->dist_cc[fo];
+        const double product = (-db / dm) * (df / dp);
+        if (product > 0.0) {
+          if (fabs(-db / dm) > fabs(df / dp)) g = lim * (df / dp);
+          else g = lim * (-db / dm);
+        } else {
+          g = 0.0;
+        }
+      }
+      SF[iv] = SLc[iv] + cl->dist_cf[f] * g;
+    }
+    /* VF limit (m5_reconctruct.c:142-170) */
+    double avf[ORC_MAXMAT + 1], avfsum = 0.0;
+    for (int iv = 0; iv < L->n3; ++iv) {
+      avf[iv] = SF[iv + L->i3];
+      if (avf[iv] <= a) avf[iv] = a;
+      else if (avf[iv] >= (1.0 - (2.0 * a))) avf[iv] = 1.0 - (2.0 * a);
+      avfsum += avf[iv];
+    }
+    if (avfsum >= (1.0 - a)) {
+      double dif = avfsum - (1.0 - a);
+      for (int j = 0; j < L->n3; ++j) { double per = avf[j] / avfsum; avf[j] = avf[j] - per * dif; }
+    }
+    for (int iv = 0; iv < L->n3; ++iv) SF[iv + L->i3] = avf[iv];
+    int rc = m5_relaxvec(L, p, SF);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int orc_m5_faceinterp(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, const double *S, double *SF) {
+  lay5_t L = mk_lay5(p->nmat);
+  for (int i = 0; i < m->n; i++) {
+    int rc = m5_faceinterp_leaf(&L, p, m, &cells[i], S, i, &SF[(size_t)i * 6 * L.neq]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* HLLC face flux (m5_hllc.c, m5_wave_speeds.c, m5_flux.c)              */
+/* ------------------------------------------------------------------ */
+/* m5_bc.c:3-83: 1 symmetry and 4 reflective flip the normal velocity of the
+ * left record into the right one, 5 flips all three components; 2 and 3
+ * leave the right record as it is */
+static void m5_apply_bc(int code, const fr5_t *F, const rec5_t *sL, rec5_t *sR) {
+  if (code != 1 && code != 4 && code != 5) return;
+  const double ul = sL->u * F->nx + sL->v * F->ny + sL->w * F->nz;
+  const double vl = sL->u * F->mx + sL->v * F->my + sL->w * F->mz;
+  const double wl = sL->u * F->lx + sL->v * F->ly + sL->w * F->lz;
+  const double u1 = -ul, v1 = (code == 5) ? -vl : vl, w1 = (code == 5) ? -wl : wl;
+  sR->u = u1 * F->nx + v1 * F->mx + w1 * F->lx;
+  sR->v = u1 * F->ny + v1 * F->my + w1 * F->ly;
+  sR->w = u1 * F->nz + v1 * F->mz + w1 * F->lz;
+}
+
+/* m5_flux.c:3-95: flag 0/3 take the side's own velocity, fractions and -A;
+ * 1/2 keep what m5_star_region set; 4 (symmetry) keeps the face velocity and
+ * takes the side's fractions and +A */
+static void m5_flux(const lay5_t *L, const rec5_t *s, int flag_ifs, ws5_t *W) {
+  for (int v = 0; v < L->n0; v++) {
+    int iv = v + L->i0;
+    W->flux_adv[iv][0] = s->u * s->ra[v];
+    W->flux_adv[iv][1] = s->v * s->ra[v];
+    W->flux_adv[iv][2] = s->w * s->ra[v];
+  }
+  int iv = L->i1;
+  W->flux_adv[iv][0] = s->u * s->u * s->r + s->st[0][0];
+  W->flux_adv[iv][1] = s->v * s->u * s->r + s->st[0][1];
+  W->flux_adv[iv][2] = s->w * s->u * s->r + s->st[0][2];
+  iv = L->i1 + 1;
+  W->flux_adv[iv][0] = s->u * s->v * s->r + s->st[1][0];
+  W->flux_adv[iv][1] = s->v * s->v * s->r + s->st[1][1];
+  W->flux_adv[iv][2] = s->w * s->v * s->r + s->st[1][2];
+  iv = L->i1 + 2;
+  W->flux_adv[iv][0] = s->u * s->w * s->r + s->st[2][0];
+  W->flux_adv[iv][1] = s->v * s->w * s->r + s->st[2][1];
+  W->flux_adv[iv][2] = s->w * s->w * s->r + s->st[2][2];
+  iv = L->i2;
+  W->flux_adv[iv][0] = s->r * s->e * s->u - s->st[0][0] * s->u - s->st[0][1] * s->v - s->st[0][2] * s->w;
+  W->flux_adv[iv][1] = s->r * s->e * s->v - s->st[1][0] * s->u - s->st[1][1] * s->v - s->st[1][2] * s->w;
+  W->flux_adv[iv][2] = s->r * s->e * s->w - s->st[2][0] * s->u - s->st[2][1] * s->v - s->st[2][2] * s->w;
+  if ((flag_ifs == 0) || (flag_ifs == 3)) {
+    W->u_nc = s->u; W->v_nc = s->v; W->w_nc = s->w;
+    for (int v = 0; v < L->n3; ++v) W->avf_S[v] = s->avf[v];
+    for (int v = 0; v < L->n4; ++v) W->are_S[v] = s->are[v];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) W->Amat_S[i][j] = -s->Amat[i * 3 + j];
+  } else if (flag_ifs == 4) {
+    for (int v = 0; v < L->n3; ++v) W->avf_S[v] = s->avf[v];
+    for (int v = 0; v < L->n4; ++v) W->are_S[v] = s->are[v];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) W->Amat_S[i][j] = s->Amat[i * 3 + j];
+  }
+  for (int v = 0; v < L->n3; ++v) {
+    iv = v + L->i3;
+    W->flux_adv[iv][0] = W->u_nc * W->avf_S[v];
+    W->flux_adv[iv][1] = W->v_nc * W->avf_S[v];
+    W->flux_adv[iv][2] = W->w_nc * W->avf_S[v];
+  }
+  for (int v = 0; v < L->n4; ++v) {
+    iv = v + L->i4;
+    W->flux_adv[iv][0] = W->u_nc * W->are_S[v];
+    W->flux_adv[iv][1] = W->v_nc * W->are_S[v];
+    W->flux_adv[iv][2] = W->w_nc * W->are_S[v];
+  }
+  iv = L->i5;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      W->flux_adv[iv][0] = W->u_nc * W->Amat_S[i][j];
+      W->flux_adv[iv][1] = W->v_nc * W->Amat_S[i][j];
+      W->flux_adv[iv][2] = W->w_nc * W->Amat_S[i][j];
+      iv++;
+    }
+}
+
+/* m5_hllc.c:3-94 with m5_wave_speeds.c:3-131.  Xc: the own cell state, for
+ * the symmetry flux that averages the cell record and its mirror. */
+static int m5_hllc(const lay5_t *L, const orc_m5_params *p, const fr5_t *F, int bc, const rec5_t *sL,
+                   const rec5_t *sR, const double *Xc, ws5_t *W) {
+  double u, v, w;
+  u = sL->u; v = sL->v; w = sL->w;
+  double uL = u * F->nx + v * F->ny + w * F->nz;
+  double vL = u * F->mx + v * F->my + w * F->mz;
+  double wL = u * F->lx + v * F->ly + w * F->lz;
+  u = sR->u; v = sR->v; w = sR->w;
+  double uR = u * F->nx + v * F->ny + w * F->nz;
+  double vR = u * F->mx + v * F->my + w * F->mz;
+  double wR = u * F->lx + v * F->ly + w * F->lz;
+  double ruL = sL->r * (uL), ruR = sR->r * (uR);
+  double c2YL = 0.0, c2YR = 0.0;
+  for (int iv = 0; iv < L->nmat; ++iv) { c2YL += sL->c2Y[iv]; c2YR += sR->c2Y[iv]; }
+  double cL = sqrt(c2YL), cR = sqrt(c2YR);
+  double SR = MAXF((uL + cL), (uR + cR));
+  double SL = MINF((uL - cL), (uR - cR));
+  double t[9];
+  rot_tensor(t, (double(*)[3])sL->st, F->nx, F->ny, F->nz, F->mx, F->my, F->mz, F->lx, F->ly, F->lz);
+  double s11L = t[0], s12L = t[1], s13L = t[2];
+  rot_tensor(t, (double(*)[3])sR->st, F->nx, F->ny, F->nz, F->mx, F->my, F->mz, F->lx, F->ly, F->lz);
+  double s11R = t[0], s12R = t[1], s13R = t[2];
+  double u_S = ((sL->r * SQ((uL)) - (s11L)) - (sR->r * SQ((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR)) /
+               (ruL - ruR - ((SL)*sL->r) + ((SR)*sR->r));
+  if (isnan(u_S) || isnan(SR) || isnan(SL)) return -1;
+  if (bc == 4) u_S = 0.0;
+  for (int iv = 0; iv < L->neq; ++iv) W->flux_hllc[iv] = 0.0;
+  int end = 0;
+  if ((0.0 <= SL) && !end) { end = 1; m5_flux(L, sL, 0, W); }
+  if (((SL <= 0.0) && (0.0 <= u_S)) && !end) {
+    end = 1;
+    if (m5_star_region(L, p, F, sL, sR, sL, 1, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R,
+                       s13R, W))
+      return -1;
+    m5_flux(L, sL, 1, W);
+    for (int iv = 0; iv < L->ncons; ++iv) W->flux_hllc[iv] = SL * (W->Us[iv] - sL->vec[iv]);
+  }
+  if (((u_S <= 0.0) && (0.0 <= SR)) && !end) {
+    end = 1;
+    if (m5_star_region(L, p, F, sL, sR, sR, 2, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R,
+                       s13R, W))
+      return -1;
+    m5_flux(L, sR, 2, W);
+    for (int iv = 0; iv < L->ncons; ++iv) W->flux_hllc[iv] = SR * (W->Us[iv] - sR->vec[iv]);
+  }
+  if ((SR <= 0.0) && !end) { end = 1; m5_flux(L, sR, 3, W); }
+  if (bc == 1) {   /* symmetry: mean of the physical fluxes of the cell record and its mirror */
+    double sym[ORC_M5_MAXNEQ][3];
+    for (int iv = 0; iv < L->neq; ++iv) {
+      W->flux_hllc[iv] = 0.0;
+      for (int k = 0; k < 3; k++) sym[iv][k] = 0.0;
+    }
+    rec5_t cL, cR;
+    m5_record(L, p, Xc, &cL);
+    m5_record(L, p, Xc, &cR);
+    m5_apply_bc(1, F, &cL, &cR);
+    W->u_nc = 0.5 * (cL.u + cR.u);
+    W->v_nc = 0.5 * (cL.v + cR.v);
+    W->w_nc = 0.5 * (cL.w + cR.w);
+    m5_flux(L, &cL, 4, W);
+    for (int iv = 0; iv < L->neq; iv++)
+      for (int k = 0; k < 3; k++) sym[iv][k] += 0.5 * W->flux_adv[iv][k];
+    m5_flux(L, &cR, 4, W);
+    for (int iv = 0; iv < L->neq; iv++)
+      for (int k = 0; k < 3; k++) sym[iv][k] += 0.5 * W->flux_adv[iv][k];
+    for (int iv = 0; iv < L->neq; iv++)
+      for (int k = 0; k < 3; k++) W->flux_adv[iv][k] = sym[iv][k];
+  }
+  for (int iv = 0; iv < L->neq; ++iv)
+    if (isnan(W->flux_hllc[iv])) return -1;
+  return 0;
+}
+
+/* one sub-face of m5_residual.c:139-170: GFM where phi changes sign, else
+ * HLLC; returns 1 for GFM, 0 for HLLC, -1 on NaN */
+static int m5_subface(const lay5_t *L, const orc_m5_params *p, const fr5_t *F, int bc, rec5_t *sL, rec5_t *sR,
+                      const double *Xc, ws5_t *W) {
+  if (sL->phi0 * sR->phi0 < 0.0) return m5_hllc_gfm(L, p, F, sL, sR, W) ? -1 : 1;
+  return m5_hllc(L, p, F, bc, sL, sR, Xc, W) ? -1 : 0;
+}
+
+int orc_m5_face(const orc_m5_params *p, const double *XL, const double *CL, const double *XR, const double *CR,
+                const double *frame, int bc, double area, double *flux, double *unc) {
+  lay5_t L = mk_lay5(p->nmat);
+  fr5_t F = {frame[0], frame[1], frame[2], frame[3], frame[4], frame[5], frame[6], frame[7], frame[8]};
+  rec5_t sL, sR;
+  ws5_t W;
+  memset(&sL, 0, sizeof sL); memset(&sR, 0, sizeof sR); memset(&W, 0, sizeof W);
+  m5_record_x(&L, p, XL, CL, 1, &sL);
+  if (bc == 0) {
+    m5_record_x(&L, p, XR, CR, 1, &sR);
+  } else {
+    m5_record_x(&L, p, XL, CL, 1, &sR);
+    m5_apply_bc(bc, &F, &sL, &sR);
+  }
+  int g = m5_subface(&L, p, &F, bc, &sL, &sR, CL, &W);
+  if (g < 0) return -1;
+  for (int iv = 0; iv < L.neq; ++iv)
+    flux[iv] = 0.0 + (area) * (((W.flux_adv[iv][0] - 0.0) * F.nx + (W.flux_adv[iv][1] - 0.0) * F.ny +
+                                (W.flux_adv[iv][2] - 0.0) * F.nz) + W.flux_hllc[iv]);
+  unc[0] = W.u_nc; unc[1] = W.v_nc; unc[2] = W.w_nc;
+  return g;
+}
+
+/* ------------------------------------------------------------------ */
+/* residual (m5_residual.c:7-238)                                       */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  double *flux_face;   /* [n][6][NEQ] */
+  double *u_face;      /* [n][6][3] */
+  int *flux_flag;      /* [n][6] */
+} m5_ws;
+
+static int m5_residual_leaf(const lay5_t *L, const orc_m5_params *p, const orc_mesh *m, const orc_cell *cells,
+                            const double *S, const double *SF, int i, m5_ws *R, double *RHS, unsigned char *gfm) {
+  const orc_cell *cl = &cells[i];
+  const int neq = L->neq;
+  const double *Si = &S[(size_t)i * neq];
+  double *rhs = &RHS[(size_t)i * neq];
+  for (int iv = 0; iv < neq; iv++) rhs[iv] = 0.0;
+  rec5_t sc;
+  m5_record(L, p, Si, &sc);
+  double source_avf[ORC_MAXMAT];
+  for (int iv = 0; iv < L->n3; ++iv) source_avf[iv] = sc.avf[iv];
+  const double A1 = sc.Amat[0], B1 = sc.Amat[3], C1 = sc.Amat[6];
+  const double A2 = sc.Amat[1], B2 = sc.Amat[4], C2 = sc.Amat[7];
+  const double A3 = sc.Amat[2], B3 = sc.Amat[5], C3 = sc.Amat[8];
+  const int SI0 = m->SI[2 * i];
+  for (int f = 0; f < 6; f++) {
+    double *ff = &R->flux_face[((size_t)i * 6 + f) * neq];
+    double *uf = &R->u_face[((size_t)i * 6 + f) * 3];
+    for (int iv = 0; iv < neq; ++iv) ff[iv] = 0.0;
+    gfm[6 * i + f] = 0;
+    uf[0] = 0.0; uf[1] = 0.0; uf[2] = 0.0;
+    const int bc = m->bc[6 * i + f];
+    int Nneig = 1, compute = 1;
+    if (bc == 0) {
+      const int lfc = m->lfc_neigs[6 * i + f];
+      if (lfc == 1) {
+        const int j = m->neigtr[24 * i + 4 * f], fo = FCOPP5[f];
+        if (m->lfc_neigs[6 * j + fo] == 1 && SI0 == 0) {   /* 1:1, one rank */
+          if (R->flux_flag[6 * i + f] == 1) {
+            const double *uo = &R->u_face[((size_t)j * 6 + fo) * 3];
+            const double *fo_ = &R->flux_face[((size_t)j * 6 + fo) * neq];
+            uf[0] = -uo[0]; uf[1] = -uo[1]; uf[2] = -uo[2];
+            for (int iv = 0; iv < neq; ++iv) ff[iv] = -fo_[iv];
+            compute = 0;
+          } else {
+            R->flux_flag[6 * j + fo] = 1;
+          }
+        }
+      } else {
+        Nneig = lfc;
+      }
+    }
+    double Area_f = 0.0;
+    for (int k = 0; k < Nneig; ++k) Area_f += cl->Area[f][k];
+    if (compute) {
+      const fr5_t F = frame_of_cell(cl, f);
+      rec5_t sL, sR;
+      if (p->order == 2) m5_record_x(L, p, &SF[((size_t)i * 6 + f) * neq], Si, 1, &sL);
+      else m5_record(L, p, Si, &sL);
+      for (int k = 0; k < Nneig; ++k) {
+        if (bc == 0) {
+          const int j = m->neigtr[24 * i + 4 * f + k];
+          const double *Sj = &S[(size_t)j * neq];
+          if (p->order == 2) m5_record_x(L, p, &SF[((size_t)j * 6 + FCOPP5[f]) * neq], Sj, 1, &sR);
+          else m5_record(L, p, Sj, &sR);
+        } else {
+          if (p->order == 2) m5_record_x(L, p, &SF[((size_t)i * 6 + f) * neq], Si, 1, &sR);
+          else m5_record(L, p, Si, &sR);
+          m5_apply_bc(bc, &F, &sL, &sR);
+        }
+        ws5_t W;
+        memset(&W, 0, sizeof W);
+        const int g = m5_subface(L, p, &F, bc, &sL, &sR, Si, &W);
+        if (g < 0) return -1;
+        if (g == 1) gfm[6 * i + f] = 1;
+        for (int iv = 0; iv < neq; ++iv)
+          ff[iv] += (cl->Area[f][k]) * (((W.flux_adv[iv][0] - 0.0) * cl->nx[f] + (W.flux_adv[iv][1] - 0.0) * cl->ny[f] +
+                                         (W.flux_adv[iv][2] - 0.0) * cl->nz[f]) + W.flux_hllc[iv]);
+        uf[0] = W.u_nc / ((double)Nneig);   /* overwritten per sub-face (m5_residual.c:163-165) */
+        uf[1] = W.v_nc / ((double)Nneig);
+        uf[2] = W.w_nc / ((double)Nneig);
+      }
+    }
+    for (int iv = 0; iv < neq; ++iv) rhs[iv] += ff[iv];
+    /* non-conservative terms (m5_residual.c:176-214) */
+    const double u_nc = uf[0], nx = cl->nx[f];
+    const double v_nc = uf[1], ny = cl->ny[f];
+    const double w_nc = uf[2], nz = cl->nz[f];
+    int v = L->i3;
+    for (int iv = 0; iv < L->n3; ++iv)
+      rhs[v + iv] += Area_f * (source_avf[iv] * u_nc * nx + source_avf[iv] * v_nc * ny + source_avf[iv] * w_nc * nz);
+    v = L->i4;
+    for (int iv = 0; iv < L->n4; ++iv) {
+      double (*s)[3] = sc.ast[iv];
+      rhs[v + iv] += Area_f * (s[0][0] * u_nc * nx + s[0][1] * v_nc * nx + s[0][2] * w_nc * nx +
+                               s[1][0] * u_nc * ny + s[1][1] * v_nc * ny + s[1][2] * w_nc * ny +
+                               s[2][0] * u_nc * nz + s[2][1] * v_nc * nz + s[2][2] * w_nc * nz);
+    }
+    v = L->i5;
+    rhs[v + 0] += Area_f * ((A1 * v_nc * ny + A1 * w_nc * nz) + B1 * v_nc * nx - C1 * w_nc * nx);
+    rhs[v + 1] += Area_f * ((A2 * v_nc * ny + A2 * w_nc * nz) + B2 * v_nc * nx - C2 * w_nc * nx);
+    rhs[v + 2] += Area_f * ((A3 * v_nc * ny + A3 * w_nc * nz) + B3 * v_nc * nx - C3 * w_nc * nx);
+    rhs[v + 3] += Area_f * ((B1 * u_nc * nx + B1 * w_nc * nz) + A1 * u_nc * ny - C1 * w_nc * ny);
+    rhs[v + 4] += Area_f * ((B2 * u_nc * nx + B2 * w_nc * nz) + A2 * u_nc * ny - C2 * w_nc * ny);
+    rhs[v + 5] += Area_f * ((B3 * u_nc * nx + B3 * w_nc * nz) + A3 * u_nc * ny - C3 * w_nc * ny);
+    rhs[v + 6] += Area_f * ((C1 * u_nc * nx + C1 * v_nc * ny) + A1 * u_nc * nz - B1 * v_nc * nz);
+    rhs[v + 7] += Area_f * ((C2 * u_nc * nx + C2 * v_nc * ny) + A2 * u_nc * nz - B2 * v_nc * nz);
+    rhs[v + 8] += Area_f * ((C3 * u_nc * nx + C3 * v_nc * ny) + A3 * u_nc * nz - B3 * v_nc * nz);
+  }
+  for (int iv = 0; iv < neq; iv++)
+    if (isnan(rhs[iv])) return -1;
+  return 0;
+}
+
+static int m5_ws_alloc(const orc_mesh *m, int neq, m5_ws *R) {
+  R->flux_face = malloc(sizeof(double) * (size_t)m->n * 6 * neq);
+  R->u_face = malloc(sizeof(double) * (size_t)m->n * 18);
+  R->flux_flag = calloc((size_t)m->n * 6, sizeof(int));
+  return (R->flux_face && R->u_face && R->flux_flag) ? 0 : -2;
+}
+static void m5_ws_free(m5_ws *R) { free(R->flux_face); free(R->u_face); free(R->flux_flag); }
+
+static int m5_residual_all(const lay5_t *L, const orc_m5_params *p, const orc_mesh *m, const orc_cell *cells,
+                           const double *S, const double *SF, double *RHS, unsigned char *gfm, m5_ws *R) {
+  memset(R->flux_flag, 0, sizeof(int) * 6 * (size_t)m->n);   /* m5_RK.c:14-16 resets the flags of every leaf */
+  for (int i = 0; i < m->n; i++) {
+    int rc = m5_residual_leaf(L, p, m, cells, S, SF, i, R, RHS, gfm);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int orc_m5_residual(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, const double *S,
+                    const double *SF, double *RHS, unsigned char *gfm_face) {
+  lay5_t L = mk_lay5(p->nmat);
+  m5_ws R;
+  int rc = m5_ws_alloc(m, L.neq, &R);
+  if (!rc) rc = m5_residual_all(&L, p, m, cells, S, SF, RHS, gfm_face, &R);
+  m5_ws_free(&R);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* stage and steps (advancetimeexplicit.c:3-29, m5_relaxation_solid.c)  */
+/* ------------------------------------------------------------------ */
+static int m5_stage_impl(const lay5_t *L, const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, int irk,
+                         double *SN, double *S, double *SF, double *RHS, unsigned char *gfm, double *Sb, m5_ws *R) {
+  const int neq = L->neq;
+  int rc;
+  if (p->order == 2) {
+    for (int i = 0; i < m->n; i++)
+      if ((rc = m5_faceinterp_leaf(L, p, m, &cells[i], S, i, &SF[(size_t)i * 6 * neq])) != 0) return rc;
+  }
+  if ((rc = m5_residual_all(L, p, m, cells, S, SF, RHS, gfm, R)) != 0) return rc;
+  if ((rc = orc_m5_rk(m, cells, p, irk, SN, S, RHS, gfm)) != 0) return rc;
+  /* m5_relaxation_solid.c:12-37: relax, then re-project the leaves that crossed */
+  memcpy(Sb, S, sizeof(double) * (size_t)m->n * neq);
+  for (int i = 0; i < m->n; i++)
+    if ((rc = m5_relaxvec(L, p, &S[(size_t)i * neq])) != 0) return rc;
+  return orc_m5_gfm_rel(m, cells, p, Sb, S, gfm);
+}
+
+typedef struct {
+  double *SF, *RHS, *Sb;
+  unsigned char *gfm;
+  m5_ws R;
+} m5_stage_ws;
+
+static int m5_stage_ws_alloc(const orc_mesh *m, int neq, m5_stage_ws *W) {
+  W->SF = malloc(sizeof(double) * (size_t)m->n * 6 * neq);
+  W->RHS = malloc(sizeof(double) * (size_t)m->n * neq);
+  W->Sb = malloc(sizeof(double) * (size_t)m->n * neq);
+  W->gfm = malloc((size_t)m->n * 6);
+  int rc = m5_ws_alloc(m, neq, &W->R);
+  return (W->SF && W->RHS && W->Sb && W->gfm && !rc) ? 0 : -2;
+}
+static void m5_stage_ws_free(m5_stage_ws *W) {
+  free(W->SF); free(W->RHS); free(W->Sb); free(W->gfm);
+  m5_ws_free(&W->R);
+}
+
+static int m5_check(const orc_m5_params *p) {
+  if (p->nmat != 2 && p->nmat != 3) return -2;
+  if (p->prelax < 0 || p->prelax > 3 || (p->order != 1 && p->order != 2)) return -2;
+  return 0;
+}
+
+int orc_m5_stage(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, int irk, const double *SN,
+                 double *S, double *SF, double *RHS, unsigned char *gfm_face) {
+  if (m5_check(p)) return -2;
+  lay5_t L = mk_lay5(p->nmat);
+  m5_stage_ws W;
+  int rc = m5_stage_ws_alloc(m, L.neq, &W);
+  if (!rc) rc = m5_stage_impl(&L, m, cells, p, irk, (double *)SN, S, W.SF, W.RHS, W.gfm, W.Sb, &W.R);
+  if (!rc && SF) memcpy(SF, W.SF, sizeof(double) * (size_t)m->n * 6 * L.neq);
+  if (!rc && RHS) memcpy(RHS, W.RHS, sizeof(double) * (size_t)m->n * L.neq);
+  if (!rc && gfm_face) memcpy(gfm_face, W.gfm, (size_t)m->n * 6);
+  m5_stage_ws_free(&W);
+  return rc;
+}
+
+int orc_m5_advance(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, double *S, int nsteps) {
+  if (m5_check(p) || p->rk_steps < 1 || p->rk_steps > 3) return -2;
+  lay5_t L = mk_lay5(p->nmat);
+  m5_stage_ws W;
+  double *SN = malloc(sizeof(double) * (size_t)m->n * L.neq);
+  int rc = m5_stage_ws_alloc(m, L.neq, &W);
+  if (!SN) rc = -2;
+  for (int s = 0; s < nsteps && !rc; s++) {
+    memcpy(SN, S, sizeof(double) * (size_t)m->n * L.neq);   /* RK.c case 0 */
+    for (int irk = 1; irk <= p->rk_steps && !rc; irk++)
+      rc = m5_stage_impl(&L, m, cells, p, irk, SN, S, W.SF, W.RHS, W.gfm, W.Sb, &W.R);
+  }
+  free(SN);
+  m5_stage_ws_free(&W);
+  return rc;
 }

@@ -1,10 +1,12 @@
 /*
- * fv_oracle_m5.h -- CPU restatement of ForestFV's model-5 ghost-fluid pieces:
- * the GFM face flux (m5_hllc_gfm.c, m5_wave_speeds_gfm.c, m5_flux_gfm.c,
- * m5_star_region.c), the interface-cell velocity re-projection (m5_gfm.c,
- * m5_gfm_rel.c) and the model-5 RK stage (m5_RK.c), with the first-order
- * face record they read (m5_facevalues.c / m5_facercnstr.c and the elastic
- * EOS of state_equation.c:143-316).
+ * fv_oracle_m5.h -- CPU restatement of ForestFV's model-5 stage: MUSCL face
+ * states (m5_faceinterp.c, m5_reconctruct.c), the residual (m5_residual.c)
+ * with its HLLC (m5_hllc.c, m5_wave_speeds.c, m5_flux.c) and GFM face fluxes
+ * (m5_hllc_gfm.c, m5_wave_speeds_gfm.c, m5_flux_gfm.c, m5_star_region.c),
+ * the RK stage (m5_RK.c) with the interface re-projection (m5_gfm.c), and
+ * the relaxation (m5_relaxationvec.c, m5_relaxation_solid.c, m5_gfm_rel.c),
+ * with the face records they read (m5_facevalues.c / m5_facercnstr.c and the
+ * elastic EOS of state_equation.c:143-316).
  *
  * TEST INFRASTRUCTURE ONLY, like fv_oracle.h: only tests/ may load it.
  *
@@ -32,6 +34,9 @@ typedef struct {
   double avf_lim;              /* AVF_LIM */
   double dt;                   /* DT */
   int    zerou[3];             /* ZEROU0, ZEROU1, ZEROU2 */
+  int    prelax;               /* PRELAX of m5_relaxationvec: 0, 1, 2 or 3 */
+  int    order;                /* ORDER: 1 cell states, 2 MUSCL face states */
+  int    rk_steps;             /* RK_STEPS: stages irk = 1..rk_steps of m5_RK */
 } orc_m5_params;
 
 int orc_m5_neq(int nmat);
@@ -65,6 +70,43 @@ int orc_m5_rk(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, 
  * m5_gfm_rel (m5_gfm_rel.c). */
 int orc_m5_gfm_rel(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, const double *S_before,
                    double *S, const unsigned char *gfm_face);
+
+/* ---- the whole model-5 stage (advancetimeexplicit.c:3-29 with MODEL 5) ----
+ * Arrays are row-major in leaf order: S, SN, RHS [n][NEQ]; SF [n][6][NEQ];
+ * gfm_face [n][6].  Status: 0, -1 NaN (where the reference prints and exits),
+ * -2 bad input, -3 Newton failure of PRELAX 2. */
+
+/* m5_relaxationvec.c:3-620 on one state vector (PRELAX 2 iterates on the
+ * branch's own t0_m / p0_m / ar_R: the published F_pe reads RUN fields that
+ * are never written) */
+int orc_m5_relax(const orc_m5_params *p, double *vec);
+
+/* m5_faceinterp_V1 (m5_faceinterp.c:60-93): MUSCL face states of every leaf
+ * (m5_facevalues_MUSCL.c, m5_deltavalues.c:20-56, m5_reconctruct.c:93-190),
+ * each relaxed by m5_relaxationvec.  PRIMTV = 0. */
+int orc_m5_faceinterp(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, const double *S, double *SF);
+
+/* m5_residual.c:7-238 over all leaves in leaf order, with the flux_flag
+ * mirror protocol; SF is read when p->order == 2. */
+int orc_m5_residual(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, const double *S,
+                    const double *SF, double *RHS, unsigned char *gfm_face);
+
+/* One stage irk: face states, residual, m5_RK(irk), m5_relaxation_solid.
+ * SF, RHS and gfm_face receive the stage's intermediate arrays (may be NULL). */
+int orc_m5_stage(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, int irk, const double *SN,
+                 double *S, double *SF, double *RHS, unsigned char *gfm_face);
+
+/* nsteps explicit steps: SN = S (RK.c case 0), then stages 1..rk_steps. */
+int orc_m5_advance(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, double *S, int nsteps);
+
+/* One face of m5_residual.c:139-170 between two face states (m5_facercnstr
+ * records: face state X with phi from the cell state C), by m5_hllc or, when
+ * phi changes sign, m5_hllc_gfm; bc is the face's BC code (0 interior, the
+ * right state then mirrors the left one).  flux = area (F.n + F_hllc), unc
+ * the non-conservative face velocity.  Returns 1 when the GFM flux was used,
+ * 0 otherwise, -1 on NaN. */
+int orc_m5_face(const orc_m5_params *p, const double *XL, const double *CL, const double *XR, const double *CR,
+                const double *frame, int bc, double area, double *flux, double *unc);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,9 @@ class orc_m5_params(C.Structure):
         ("avf_lim", C.c_double),
         ("dt", C.c_double),
         ("zerou", C.c_int * 3),
+        ("prelax", C.c_int),
+        ("order", C.c_int),
+        ("rk_steps", C.c_int),
     ]
 
 
@@ -131,6 +134,13 @@ def lib():
         L.orc_m5_gfm_face.argtypes = [P(orc_m5_params), dp, dp, dp, C.c_double, dp, dp]
         L.orc_m5_rk.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), C.c_int, dp, dp, dp, P(C.c_uint8)]
         L.orc_m5_gfm_rel.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), dp, dp, P(C.c_uint8)]
+        L.orc_m5_relax.argtypes = [P(orc_m5_params), dp]
+        L.orc_m5_faceinterp.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), dp, dp]
+        L.orc_m5_residual.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), dp, dp, dp, P(C.c_uint8)]
+        L.orc_m5_stage.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), C.c_int, dp, dp, dp, dp,
+                                   P(C.c_uint8)]
+        L.orc_m5_advance.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), dp, C.c_int]
+        L.orc_m5_face.argtypes = [P(orc_m5_params), dp, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp]
         _lib = L
     return _lib
 
@@ -240,6 +250,50 @@ class OracleMesh:
                                   gf.ctypes.data_as(C.POINTER(C.c_uint8)))
         if rc != 0:
             raise RuntimeError(f"orc_m5_gfm_rel status {rc}")
+        return S
+
+    def m5_faceinterp(self, m5p, S):
+        """orc_m5_faceinterp: MUSCL face states [n][6][NEQ]."""
+        nq = 3 * m5p.nmat + 12
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        SF = np.zeros((self.n, 6, nq))
+        rc = lib().orc_m5_faceinterp(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(S), _dp(SF))
+        if rc != 0:
+            raise RuntimeError(f"orc_m5_faceinterp status {rc}")
+        return SF
+
+    def m5_residual(self, m5p, S, SF=None):
+        """orc_m5_residual: (RHS [n][NEQ], gfm_face [n][6])."""
+        nq = 3 * m5p.nmat + 12
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        SF = np.zeros((self.n, 6, nq)) if SF is None else np.ascontiguousarray(SF, dtype=np.float64)
+        RHS = np.zeros((self.n, nq))
+        gf = np.zeros((self.n, 6), np.uint8)
+        rc = lib().orc_m5_residual(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(S), _dp(SF), _dp(RHS),
+                                   gf.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc != 0:
+            raise RuntimeError(f"orc_m5_residual status {rc}")
+        return RHS, gf
+
+    def m5_stage(self, m5p, irk, SN, S):
+        """orc_m5_stage: (S, SF, RHS, gfm_face) after stage irk."""
+        nq = 3 * m5p.nmat + 12
+        SN = np.ascontiguousarray(SN, dtype=np.float64)
+        S = np.ascontiguousarray(S, dtype=np.float64).copy()
+        SF = np.zeros((self.n, 6, nq))
+        RHS = np.zeros((self.n, nq))
+        gf = np.zeros((self.n, 6), np.uint8)
+        rc = lib().orc_m5_stage(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), irk, _dp(SN), _dp(S), _dp(SF),
+                                _dp(RHS), gf.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc != 0:
+            raise RuntimeError(f"orc_m5_stage status {rc}")
+        return S, SF, RHS, gf
+
+    def m5_advance(self, m5p, S, nsteps):
+        S = np.ascontiguousarray(S, dtype=np.float64).copy()
+        rc = lib().orc_m5_advance(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(S), nsteps)
+        if rc != 0:
+            raise RuntimeError(f"orc_m5_advance status {rc}")
         return S
 
     def interface_location(self, params, crit, S):
@@ -359,3 +413,24 @@ def m5_gfm_face(m5p, UL, UR, frame, area):
     if rc != 0:
         raise FloatingPointError("NaN in the GFM face flux")
     return flux, unc
+
+
+def m5_relax(m5p, vec):
+    """orc_m5_relax: the relaxed copy of one model-5 state vector; raises on failure."""
+    v = np.ascontiguousarray(vec, dtype=np.float64).copy()
+    rc = lib().orc_m5_relax(C.byref(m5_from(m5p)), _dp(v))
+    if rc != 0:
+        raise FloatingPointError(f"orc_m5_relax status {rc}")
+    return v
+
+
+def m5_face(m5p, XL, CL, XR, CR, frame, bc, area):
+    """orc_m5_face: (flux [NEQ], unc [3], gfm) of one face between two face states."""
+    nq = 3 * m5p.nmat + 12
+    flux = np.zeros(nq)
+    unc = np.zeros(3)
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (XL, CL, XR, CR, frame)]
+    rc = lib().orc_m5_face(C.byref(m5_from(m5p)), *[_dp(x) for x in a], int(bc), float(area), _dp(flux), _dp(unc))
+    if rc < 0:
+        raise FloatingPointError("NaN in the face flux")
+    return flux, unc, rc
