@@ -218,6 +218,17 @@ def lib():
             "fv_m5_rk": (C.c_int, [P(fv_m5_params), P(fv_pack), C.c_int, P(C.c_double), P(C.c_double),
                                    P(C.c_double), P(C.c_uint8)]),
             "fv_m5_gfm_rel": (C.c_int, [P(fv_m5_params), P(fv_pack), P(C.c_double), P(C.c_double), P(C.c_uint8)]),
+            "fv_m5_dev_attach": (C.c_int, [P(fv_m5_params), P(fv_pack), P(C.c_void_p)]),
+            "fv_m5_dev_detach": (C.c_int, [C.c_void_p]),
+            "fv_m5_dev_set_state": (C.c_int, [C.c_void_p, P(C.c_double)]),
+            "fv_m5_dev_get_state": (C.c_int, [C.c_void_p, P(C.c_double)]),
+            "fv_m5_dev_begin_step": (C.c_int, [C.c_void_p]),
+            "fv_m5_dev_stage": (C.c_int, [C.c_void_p, C.c_int]),
+            "fv_m5_dev_advance": (C.c_int, [C.c_void_p, C.c_int]),
+            "fv_m5_dev_faceinterp": (C.c_int, [C.c_void_p]),
+            "fv_m5_dev_residual": (C.c_int, [C.c_void_p]),
+            "fv_m5_dev_get_stage_arrays": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_uint8)]),
+            "fv_m5_dev_time_steps": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -335,6 +346,71 @@ def m5_gfm_rel(params, pack, S_before, S, gfm_face):
     _check(lib().fv_m5_gfm_rel(C.byref(params), C.byref(pack.pk), _dp(Sb), _dp(S),
                                gf.ctypes.data_as(C.POINTER(C.c_uint8))), "fv_m5_gfm_rel")
     return S
+
+
+class M5Device:
+    """The whole model-5 stage on the device (fv_m5_dev_*): MUSCL face states, the residual with
+    its HLLC and GFM fluxes, m5_RK and the relaxation, on the leaves and face kinds of `mesh`."""
+
+    def __init__(self, params, mesh):
+        self.params = params
+        self.pack = mesh.pack()
+        self.n = mesh.n
+        self.neq = m5_neq(params.nmat)
+        h = C.c_void_p()
+        _check(lib().fv_m5_dev_attach(C.byref(params), C.byref(self.pack.pk), C.byref(h)), "fv_m5_dev_attach")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fv_m5_dev_detach(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, S):
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        assert S.shape == (self.n, self.neq)
+        _check(lib().fv_m5_dev_set_state(self.h, _dp(S)), "fv_m5_dev_set_state")
+
+    def get_state(self):
+        S = np.zeros((self.n, self.neq))
+        _check(lib().fv_m5_dev_get_state(self.h, _dp(S)), "fv_m5_dev_get_state")
+        return S
+
+    def begin_step(self):
+        _check(lib().fv_m5_dev_begin_step(self.h), "fv_m5_dev_begin_step")
+
+    def stage(self, irk):
+        _check(lib().fv_m5_dev_stage(self.h, irk), "fv_m5_dev_stage")
+
+    def advance(self, nsteps):
+        _check(lib().fv_m5_dev_advance(self.h, nsteps), "fv_m5_dev_advance")
+
+    def faceinterp(self):
+        _check(lib().fv_m5_dev_faceinterp(self.h), "fv_m5_dev_faceinterp")
+
+    def residual(self):
+        _check(lib().fv_m5_dev_residual(self.h), "fv_m5_dev_residual")
+
+    def stage_arrays(self):
+        """(SF [n][6][NEQ], RHS [n][NEQ], gfm_face [n][6]) of the last operations."""
+        SF = np.zeros((self.n, 6, self.neq))
+        RHS = np.zeros((self.n, self.neq))
+        gf = np.zeros((self.n, 6), np.uint8)
+        _check(lib().fv_m5_dev_get_stage_arrays(self.h, _dp(SF), _dp(RHS), gf.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "fv_m5_dev_get_stage_arrays")
+        return SF, RHS, gf
+
+    def time_steps(self, nsteps):
+        """(ms per stage, ms per residual kernel launch) over nsteps steps."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib().fv_m5_dev_time_steps(self.h, nsteps, C.byref(a), C.byref(b)), "fv_m5_dev_time_steps")
+        return a.value, b.value
 
 
 def init_pure(params, rho_init, mat, pres, u=(0.0, 0.0, 0.0)):

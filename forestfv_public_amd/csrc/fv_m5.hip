@@ -39,6 +39,7 @@ struct M5P {
   double gamma[3], pinf[3], mush[3];
   double amin, avf_lim, dt;
   int zerou[3];
+  int prelax, order;
 };
 
 template <int NM>
@@ -84,8 +85,8 @@ __device__ double elastic_energy_v1(const double *G, double mu) {
   return 0.0;
 }
 
-// m5_eos_stress_V2, state_equation.c:143-204, accumulated into st
-__device__ void add_stress_v2(double *st, double avf, double pr, const double *G, double Gmag, double mu) {
+// m5_eos_stress_V2, state_equation.c:143-204: the stress s of one material
+__device__ void stress_v2(double *s, double avf, double pr, const double *G, double Gmag, double mu) {
   if ((fabs(Gmag) != 0.0) && (mu != 0.0)) {
     double G2[9], ss[9];
 #pragma unroll
@@ -110,15 +111,34 @@ __device__ void add_stress_v2(double *st, double avf, double pr, const double *G
     ss[8] = G2[8] - J2 / 3.0 - g13 * (G[8] - J1 / 3.0);
     const double k = mu / pow(fabs(Gmag), 1.0 / 6.0);
 #pragma unroll
-    for (int q = 0; q < 9; q++) {
-      const double s = (q == 0 || q == 4 || q == 8) ? (-avf * pr - k * (ss[q])) : (-k * (ss[q]));
-      st[q] += s;
-    }
+    for (int q = 0; q < 9; q++) s[q] = (q == 0 || q == 4 || q == 8) ? (-avf * pr - k * (ss[q])) : (-k * (ss[q]));
   } else {
-    st[0] += -avf * pr; st[1] += 0.0; st[2] += 0.0;
-    st[3] += 0.0; st[4] += -avf * pr; st[5] += 0.0;
-    st[6] += 0.0; st[7] += 0.0; st[8] += -avf * pr;
+    s[0] = -avf * pr; s[1] = 0.0; s[2] = 0.0;
+    s[3] = 0.0; s[4] = -avf * pr; s[5] = 0.0;
+    s[6] = 0.0; s[7] = 0.0; s[8] = -avf * pr;
   }
+}
+
+// the sum of the materials' stresses (m5_facercnstr.c:150-173)
+__device__ void add_stress_v2(double *st, double avf, double pr, const double *G, double Gmag, double mu) {
+  double s[9];
+  stress_v2(s, avf, pr, G, Gmag, mu);
+#pragma unroll
+  for (int q = 0; q < 9; q++) st[q] += s[q];
+}
+
+// G = A A^T and its determinant (m5_facevalues.c:123-136)
+__device__ void gmat(const double *A, double *G, double &Gmag) {
+  G[0] = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+  G[1] = A[0] * A[3] + A[1] * A[4] + A[2] * A[5];
+  G[2] = A[0] * A[6] + A[1] * A[7] + A[2] * A[8];
+  G[3] = A[3] * A[0] + A[4] * A[1] + A[5] * A[2];
+  G[4] = A[3] * A[3] + A[4] * A[4] + A[5] * A[5];
+  G[5] = A[3] * A[6] + A[4] * A[7] + A[5] * A[8];
+  G[6] = A[6] * A[0] + A[7] * A[1] + A[8] * A[2];
+  G[7] = A[6] * A[3] + A[7] * A[4] + A[8] * A[5];
+  G[8] = A[6] * A[6] + A[7] * A[7] + A[8] * A[8];
+  Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) + G[2] * (G[3] * G[7] - G[4] * G[6]);
 }
 
 template <int NM>
@@ -130,9 +150,13 @@ struct Rec5 {
   double ut, vt, wt;       // u_temp, v_temp, w_temp
 };
 
-// face record of one state vector x (m5_facercnstr.c:3-190, n_solids = 0)
-template <int NM>
-__device__ void record(const M5P &P, const double *x, Rec5<NM> &r) {
+// Record of a state vector x, n_solids = 0.  FACE = false: m5_facevalues.c:3-188,
+// x a cell state.  FACE = true: m5_facercnstr.c:3-190, x a face state whose phi
+// comes from the cell's raw solid fraction cavf0, and whose pressure and c2Y
+// divide by the re-summed fraction r.avf[i] (the two forms differ only for the
+// last material after a renormalisation).
+template <int NM, bool FACE = false>
+__device__ void record(const M5P &P, const double *x, Rec5<NM> &r, double cavf0 = 0.0) {
   using L = L5<NM>;
   const double a = P.amin / 100.0;
   double avf_temp[NM], avf_c0 = 0.0;
@@ -148,7 +172,7 @@ __device__ void record(const M5P &P, const double *x, Rec5<NM> &r) {
   double avfsum = 0.0;
 #pragma unroll
   for (int i = 0; i < L::N3; ++i) {
-    double t = x[L::I3 + i], c = x[L::I3 + i];
+    double t = x[L::I3 + i], c = (FACE && i == 0) ? cavf0 : x[L::I3 + i];
     if (t <= a) t = a;
     else if (t >= (1.0 - a)) t = 1.0 - a;
     if (c >= (1.0 - a)) c = 1.0 - a;
@@ -192,8 +216,9 @@ __device__ void record(const M5P &P, const double *x, Rec5<NM> &r) {
   for (int i = 0; i < NM; ++i) {
     const double are_elast = (P.mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[i]) : 0.0;
     const double g = P.gamma[i], pi = P.pinf[i];
-    const double press = ((g - 1.0) * (r.are[i] - are_elast) / avf_temp[i]) - g * pi;
-    r.c2Y[i] = (avf_temp[i] / r.r) * g * (press + pi);
+    const double den = FACE ? r.avf[i] : avf_temp[i];
+    const double press = ((g - 1.0) * (r.are[i] - are_elast) / den) - g * pi;
+    r.c2Y[i] = (den / r.r) * g * (press + pi);
     r.ph[i] = press;
     add_stress_v2(r.st, r.avf[i], r.ph[i], G, Gmag, P.mush[i]);
   }
@@ -309,25 +334,13 @@ __device__ void star_region(const M5P &P, const double *F, const Rec5<NM> &sL, c
   for (int k = 0; k < 9; k++) W.AS[k] = AS[k];
 }
 
-// one interface sub-face: m5_hllc_gfm + m5_residual.c:166-170
+// One GFM sub-face (m5_hllc_gfm.c:3-100 with m5_wave_speeds_gfm.c and
+// m5_flux_gfm.c): out[iv] = F.n + F_hllc as m5_residual.c:166-170 forms it
+// before the area, and the non-conservative face velocity.  sL and sR are
+// rewritten with the ghost.  Returns false where the reference prints NaN and exits.
 template <int NM>
-__global__ __launch_bounds__(256) void k_m5_gfm_faces(int nf, M5P P, const double *__restrict__ UL,
-                                                      const double *__restrict__ UR, const double *__restrict__ FR,
-                                                      const double *__restrict__ area, double *__restrict__ flux,
-                                                      double *__restrict__ unc, int *__restrict__ nanflag) {
+__device__ bool gfm_subface(const M5P &P, const double *F, Rec5<NM> &sL, Rec5<NM> &sR, double *out, double *unc) {
   using L = L5<NM>;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nf) return;
-  double x[L::NEQ], F[9];
-  Rec5<NM> sL, sR;
-#pragma unroll
-  for (int v = 0; v < L::NEQ; v++) x[v] = UL[(size_t)v * nf + q];
-  record<NM>(P, x, sL);
-#pragma unroll
-  for (int v = 0; v < L::NEQ; v++) x[v] = UR[(size_t)v * nf + q];
-  record<NM>(P, x, sR);
-#pragma unroll
-  for (int k = 0; k < 9; k++) F[k] = FR[(size_t)k * nf + q];
   const double *n = F, *m = F + 3, *l = F + 6;
   // face-frame velocities (rotate_vector_gtl_v2) and the ghost: v_R = v_L, w_R = w_L
   const double uL = sL.u * n[0] + sL.v * n[1] + sL.w * n[2];
@@ -356,7 +369,7 @@ __global__ __launch_bounds__(256) void k_m5_gfm_faces(int nf, M5P P, const doubl
   const double s11R = t[0], s12R = t[1], s13R = t[2];
   const double u_S = ((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR)) /
                      (ruL - ruR - ((SL)*sL.r) + ((SR)*sR.r));
-  if (isnan(u_S) || isnan(SR) || isnan(SL)) { atomicOr(nanflag, 1); return; }
+  if (isnan(u_S) || isnan(SR) || isnan(SL)) return false;
   int flag_ifs;
   if (0.0 <= SL) flag_ifs = 0;
   else if ((SL < 0.0) && (0.0 <= u_S)) flag_ifs = 1;
@@ -370,7 +383,7 @@ __global__ __launch_bounds__(256) void k_m5_gfm_faces(int nf, M5P P, const doubl
     star_region<NM>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
 #pragma unroll
     for (int iv = 0; iv < L::NCONS; ++iv)
-      if (isnan(W.Us[iv])) { atomicOr(nanflag, 1); return; }
+      if (isnan(W.Us[iv])) return false;
   } else {
     // m5_flux_gfm.c:43-56: the non-conservative face state of the side itself
     W.unc[0] = s.ut; W.unc[1] = s.vt; W.unc[2] = s.wt;
@@ -382,12 +395,11 @@ __global__ __launch_bounds__(256) void k_m5_gfm_faces(int nf, M5P P, const doubl
     for (int k = 0; k < 9; k++) W.AS[k] = s.A[k];
   }
   const bool star = (flag_ifs == 1 || flag_ifs == 2);
+  bool ok = true;
   auto emit = [&](int iv, double f0, double f1, double f2, double vec_iv) {
     const double h = (star && iv < L::NCONS) ? SK * (W.Us[iv] - vec_iv) : 0.0;
-    if (isnan(h)) atomicOr(nanflag, 1);
-    double fl = 0.0;
-    fl += (area[q]) * (((f0 - 0.0) * n[0] + (f1 - 0.0) * n[1] + (f2 - 0.0) * n[2]) + h);
-    flux[(size_t)iv * nf + q] = fl;
+    if (isnan(h)) ok = false;
+    out[iv] = ((f0 - 0.0) * n[0] + (f1 - 0.0) * n[1] + (f2 - 0.0) * n[2]) + h;
   };
   // m5_flux_gfm.c:10-39: physical flux of side s with its u_temp velocity
 #pragma unroll
@@ -404,9 +416,41 @@ __global__ __launch_bounds__(256) void k_m5_gfm_faces(int nf, M5P P, const doubl
   for (int v = 0; v < NM; ++v) emit(L::I4 + v, W.unc[0] * W.are_S[v], W.unc[1] * W.are_S[v], W.unc[2] * W.are_S[v], 0.0);
 #pragma unroll
   for (int k = 0; k < 9; k++) emit(L::I5 + k, W.unc[0] * W.AS[k], W.unc[1] * W.AS[k], W.unc[2] * W.AS[k], 0.0);
-  unc[(size_t)0 * nf + q] = W.unc[0];
-  unc[(size_t)1 * nf + q] = W.unc[1];
-  unc[(size_t)2 * nf + q] = W.unc[2];
+  unc[0] = W.unc[0]; unc[1] = W.unc[1]; unc[2] = W.unc[2];
+  return ok;
+}
+
+// one interface sub-face of a list: gfm_subface times the sub-face area
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_gfm_faces(int nf, M5P P, const double *__restrict__ UL,
+                                                      const double *__restrict__ UR, const double *__restrict__ FR,
+                                                      const double *__restrict__ area, double *__restrict__ flux,
+                                                      double *__restrict__ unc, int *__restrict__ nanflag) {
+  using L = L5<NM>;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nf) return;
+  double x[L::NEQ], F[9];
+  Rec5<NM> sL, sR;
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) x[v] = UL[(size_t)v * nf + q];
+  record<NM>(P, x, sL);
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) x[v] = UR[(size_t)v * nf + q];
+  record<NM>(P, x, sR);
+#pragma unroll
+  for (int k = 0; k < 9; k++) F[k] = FR[(size_t)k * nf + q];
+  double out[L::NEQ], un[3];
+  if (!gfm_subface<NM>(P, F, sL, sR, out, un)) { atomicOr(nanflag, 1); return; }
+  const double A = area[q];
+#pragma unroll
+  for (int iv = 0; iv < L::NEQ; iv++) {
+    double fl = 0.0;
+    fl += (A) * out[iv];
+    flux[(size_t)iv * nf + q] = fl;
+  }
+  unc[(size_t)0 * nf + q] = un[0];
+  unc[(size_t)1 * nf + q] = un[1];
+  unc[(size_t)2 * nf + q] = un[2];
 }
 
 // ---- m5_RK ----------------------------------------------------------------
@@ -570,13 +614,15 @@ __device__ int gfm_velocity(const M5P &P, const Geo *__restrict__ Gp, int n, int
 // m5_gfm.c:3-254 over the marked leaves in leaf order (one lane: the list is short
 // and each leaf may read the new state of an earlier marked neighbour)
 template <int NM>
-__global__ void k_m5_gfm_seq(int n, int nlist, const int *__restrict__ list, M5P P, const Geo *__restrict__ Gp,
+__global__ void k_m5_gfm_seq(int n, const int *__restrict__ nlist_p, const int *__restrict__ list, M5P P,
+                             const Geo *__restrict__ Gp,
                              const int *__restrict__ nbr, const uint32_t *__restrict__ info,
                              const int *__restrict__ fine4, const uint8_t *__restrict__ gf,
                              const double *__restrict__ SN, const double *__restrict__ Sin, double *__restrict__ S,
                              const double *__restrict__ RHS, int *__restrict__ err) {
   using L = L5<NM>;
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int nlist = *nlist_p;
   for (int t = 0; t < nlist; t++) {
     const int i = list[t];
     auto at = [&](int e) { return (size_t)e * n + i; };
@@ -614,12 +660,14 @@ __global__ void k_m5_gfm_seq(int n, int nlist, const int *__restrict__ list, M5P
 
 // m5_gfm_rel.c:3-246 over the marked leaves in leaf order
 template <int NM>
-__global__ void k_m5_rel_seq(int n, int nlist, const int *__restrict__ list, M5P P, const Geo *__restrict__ Gp,
+__global__ void k_m5_rel_seq(int n, const int *__restrict__ nlist_p, const int *__restrict__ list, M5P P,
+                             const Geo *__restrict__ Gp,
                              const int *__restrict__ nbr, const uint32_t *__restrict__ info,
                              const int *__restrict__ fine4, const uint8_t *__restrict__ gf,
                              const double *__restrict__ Sb, double *__restrict__ S, int *__restrict__ err) {
   using L = L5<NM>;
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int nlist = *nlist_p;
   for (int t = 0; t < nlist; t++) {
     const int i = list[t];
     auto at = [&](int e) { return (size_t)e * n + i; };
@@ -643,6 +691,706 @@ __global__ void k_m5_rel_seq(int n, int nlist, const int *__restrict__ list, M5P
   }
 }
 
+
+// ---- the whole stage: MUSCL face states, residual, relaxation ---------------
+
+__constant__ static const int FCOPP5[6] = {2, 3, 0, 1, 5, 4};   // operatetp.c:268-279
+
+// m5_bc.c:3-83 on the right record: 1 symmetry and 4 reflective flip the
+// normal velocity of the left record, 5 flips all three components
+template <int NM>
+__device__ void apply_bc5(int code, const double *F, const Rec5<NM> &sL, Rec5<NM> &sR) {
+  if (code != 1 && code != 4 && code != 5) return;
+  const double *n = F, *m = F + 3, *l = F + 6;
+  const double ul = sL.u * n[0] + sL.v * n[1] + sL.w * n[2];
+  const double vl = sL.u * m[0] + sL.v * m[1] + sL.w * m[2];
+  const double wl = sL.u * l[0] + sL.v * l[1] + sL.w * l[2];
+  const double u1 = -ul, v1 = (code == 5) ? -vl : vl, w1 = (code == 5) ? -wl : wl;
+  sR.u = u1 * n[0] + v1 * m[0] + w1 * l[0];
+  sR.v = u1 * n[1] + v1 * m[1] + w1 * l[1];
+  sR.w = u1 * n[2] + v1 * m[2] + w1 * l[2];
+}
+
+// m5_flux.c:10-39: the physical flux rows of a record, F.n per equation for
+// the mass, momentum (+ sigma) and energy rows
+template <int NM>
+__device__ void phys_rows(const Rec5<NM> &s, const double *n, double *adv) {
+  using L = L5<NM>;
+  auto dot = [&](double f0, double f1, double f2) { return (f0 - 0.0) * n[0] + (f1 - 0.0) * n[1] + (f2 - 0.0) * n[2]; };
+#pragma unroll
+  for (int v = 0; v < NM; ++v) adv[L::I0 + v] = dot(s.u * s.ra[v], s.v * s.ra[v], s.w * s.ra[v]);
+  adv[L::I1 + 0] = dot(s.u * s.u * s.r + s.st[0], s.v * s.u * s.r + s.st[1], s.w * s.u * s.r + s.st[2]);
+  adv[L::I1 + 1] = dot(s.u * s.v * s.r + s.st[3], s.v * s.v * s.r + s.st[4], s.w * s.v * s.r + s.st[5]);
+  adv[L::I1 + 2] = dot(s.u * s.w * s.r + s.st[6], s.v * s.w * s.r + s.st[7], s.w * s.w * s.r + s.st[8]);
+  adv[L::I2] = dot(s.r * s.e * s.u - s.st[0] * s.u - s.st[1] * s.v - s.st[2] * s.w,
+                   s.r * s.e * s.v - s.st[3] * s.u - s.st[4] * s.v - s.st[5] * s.w,
+                   s.r * s.e * s.w - s.st[6] * s.u - s.st[7] * s.v - s.st[8] * s.w);
+}
+
+// One HLLC sub-face (m5_hllc.c:3-94 with m5_wave_speeds.c:3-131 and
+// m5_flux.c): out[iv] = F.n + F_hllc before the area, and the
+// non-conservative face velocity.  bc 4 forces u* = 0; bc 1 (symmetry)
+// replaces the flux by the mean of the physical fluxes of the cell record of
+// xc and its mirror.  Returns false where the reference prints NaN and exits.
+template <int NM>
+__device__ bool hllc_subface(const M5P &P, const double *F, int bc, const Rec5<NM> &sL, const Rec5<NM> &sR,
+                             const double *__restrict__ S, int n, int cell, double *out, double *unc) {
+  using L = L5<NM>;
+  const double *nv = F, *m = F + 3, *l = F + 6;
+  const double uL = sL.u * nv[0] + sL.v * nv[1] + sL.w * nv[2];
+  const double vL = sL.u * m[0] + sL.v * m[1] + sL.w * m[2];
+  const double wL = sL.u * l[0] + sL.v * l[1] + sL.w * l[2];
+  const double uR = sR.u * nv[0] + sR.v * nv[1] + sR.w * nv[2];
+  const double vR = sR.u * m[0] + sR.v * m[1] + sR.w * m[2];
+  const double wR = sR.u * l[0] + sR.v * l[1] + sR.w * l[2];
+  const double ruL = sL.r * (uL), ruR = sR.r * (uR);
+  double c2YL = 0.0, c2YR = 0.0;
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) { c2YL += sL.c2Y[iv]; c2YR += sR.c2Y[iv]; }
+  const double cL = sqrt(c2YL), cR = sqrt(c2YR);
+  const double SR = maxr((uL + cL), (uR + cR));
+  const double SL = minr((uL - cL), (uR - cR));
+  double t[9];
+  rot_tensor(t, sL.st, nv, m, l);
+  const double s11L = t[0], s12L = t[1], s13L = t[2];
+  rot_tensor(t, sR.st, nv, m, l);
+  const double s11R = t[0], s12R = t[1], s13R = t[2];
+  double u_S = ((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR)) /
+               (ruL - ruR - ((SL)*sL.r) + ((SR)*sR.r));
+  if (isnan(u_S) || isnan(SR) || isnan(SL)) return false;
+  if (bc == 4) u_S = 0.0;
+  int flag_ifs;
+  if (0.0 <= SL) flag_ifs = 0;
+  else if ((SL <= 0.0) && (0.0 <= u_S)) flag_ifs = 1;
+  else if ((u_S <= 0.0) && (0.0 <= SR)) flag_ifs = 2;
+  else flag_ifs = 3;   // SR <= 0.0
+  const Rec5<NM> &s = (flag_ifs <= 1) ? sL : sR;
+  Star5<NM> W;
+  double SK = 0.0;
+  const bool star = (flag_ifs == 1 || flag_ifs == 2);
+  if (star) {
+    SK = (flag_ifs == 1) ? SL : SR;
+    star_region<NM>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
+#pragma unroll
+    for (int iv = 0; iv < L::NCONS; ++iv)
+      if (isnan(W.Us[iv])) return false;
+  } else {
+    // m5_flux.c:44-54: the side's own velocity, fractions and -A
+    W.unc[0] = s.u; W.unc[1] = s.v; W.unc[2] = s.w;
+#pragma unroll
+    for (int v = 0; v < NM - 1; ++v) W.avf_S[v] = s.avf[v];
+#pragma unroll
+    for (int v = 0; v < NM; ++v) W.are_S[v] = s.are[v];
+#pragma unroll
+    for (int k = 0; k < 9; k++) W.AS[k] = -s.A[k];
+  }
+  if (bc == 1) {
+    // m5_hllc.c:66-88: F_hllc = 0, the face velocity and flux are the means of
+    // the cell record and its mirror (flag 4 of m5_flux: +A, the side's fractions).
+    // The region branch before it only matters through its NaN exits.
+    double x[L::NEQ];
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + cell];
+    Rec5<NM> cl, cr;
+    record<NM>(P, x, cl);
+    cr = cl;
+    apply_bc5<NM>(1, F, cl, cr);
+    const double un = 0.5 * (cl.u + cr.u), vn = 0.5 * (cl.v + cr.v), wn = 0.5 * (cl.w + cr.w);
+    double a0[L::NEQ], a1[L::NEQ];
+    auto rows = [&](const Rec5<NM> &s, double *adv, int k) {
+      // the three components of row k of flag 4, k = 0, 1, 2
+      const double uu = (k == 0) ? un : (k == 1 ? vn : wn);
+      const double su = (k == 0) ? s.u : (k == 1 ? s.v : s.w);
+#pragma unroll
+      for (int v = 0; v < NM; ++v) adv[L::I0 + v] = su * s.ra[v];
+      adv[L::I1 + 0] = su * s.u * s.r + s.st[0 + k];
+      adv[L::I1 + 1] = su * s.v * s.r + s.st[3 + k];
+      adv[L::I1 + 2] = su * s.w * s.r + s.st[6 + k];
+      adv[L::I2] = s.r * s.e * su - s.st[3 * k + 0] * s.u - s.st[3 * k + 1] * s.v - s.st[3 * k + 2] * s.w;
+#pragma unroll
+      for (int v = 0; v < NM - 1; ++v) adv[L::I3 + v] = uu * s.avf[v];
+#pragma unroll
+      for (int v = 0; v < NM; ++v) adv[L::I4 + v] = uu * s.are[v];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) adv[L::I5 + q] = uu * s.A[q];
+    };
+    double acc[3][L::NEQ];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      rows(cl, a0, k);
+      rows(cr, a1, k);
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) {
+        double sym = 0.0;
+        sym += 0.5 * a0[v];
+        sym += 0.5 * a1[v];
+        acc[k][v] = sym;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++)
+      out[v] = ((acc[0][v] - 0.0) * nv[0] + (acc[1][v] - 0.0) * nv[1] + (acc[2][v] - 0.0) * nv[2]) + 0.0;
+    unc[0] = un; unc[1] = vn; unc[2] = wn;
+    return true;
+  }
+  double adv[L::NEQ];
+  phys_rows<NM>(s, nv, adv);
+  bool ok = true;
+#pragma unroll
+  for (int iv = 0; iv < L::NCONS; ++iv) {
+    const double vec_iv = (iv < NM) ? s.ra[iv] : (iv < NM + 3 ? s.vm[iv - NM] : s.vE);
+    const double h = star ? SK * (W.Us[iv] - vec_iv) : 0.0;
+    if (isnan(h)) ok = false;
+    out[iv] = adv[iv] + h;
+  }
+  auto dot = [&](double f0, double f1, double f2) { return (f0 - 0.0) * nv[0] + (f1 - 0.0) * nv[1] + (f2 - 0.0) * nv[2]; };
+#pragma unroll
+  for (int v = 0; v < NM - 1; ++v)
+    out[L::I3 + v] = dot(W.unc[0] * W.avf_S[v], W.unc[1] * W.avf_S[v], W.unc[2] * W.avf_S[v]) + 0.0;
+#pragma unroll
+  for (int v = 0; v < NM; ++v)
+    out[L::I4 + v] = dot(W.unc[0] * W.are_S[v], W.unc[1] * W.are_S[v], W.unc[2] * W.are_S[v]) + 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) out[L::I5 + k] = dot(W.unc[0] * W.AS[k], W.unc[1] * W.AS[k], W.unc[2] * W.AS[k]) + 0.0;
+  unc[0] = W.unc[0]; unc[1] = W.unc[1]; unc[2] = W.unc[2];
+  return ok;
+}
+
+// ---- relaxation (m5_relaxationvec.c) ----
+
+template <int NM>
+__device__ void vf_limit2(double a, double *avf) {   // m5_relaxationvec.c:131-150: two independent clips
+  double avfsum = 0.0;
+#pragma unroll
+  for (int iv = 0; iv < NM - 1; ++iv) {
+    if (avf[iv] <= a) avf[iv] = a;
+    if (avf[iv] >= (1.0 - a)) avf[iv] = 1.0 - a;
+    avfsum += avf[iv];
+  }
+  if (avfsum >= (1.0 - a)) {
+    const double dif = avfsum - (1.0 - a);
+#pragma unroll
+    for (int j = 0; j < NM - 1; ++j) { const double per = avf[j] / avfsum; avf[j] = avf[j] - per * dif; }
+    avfsum = 1.0 - a;
+  }
+  avf[NM - 1] = 1.0 - avfsum;
+}
+
+template <int NM>
+__device__ void vf_limit1(double a, double *avf) {   // m5_relaxationvec.c:44-60: else-if clips
+  double avfsum = 0.0;
+#pragma unroll
+  for (int iv = 0; iv < NM - 1; ++iv) {
+    if (avf[iv] <= a) avf[iv] = a;
+    else if (avf[iv] >= (1.0 - a)) avf[iv] = 1.0 - a;
+    avfsum += avf[iv];
+  }
+  if (avfsum >= (1.0 - a)) {
+    const double dif = avfsum - (1.0 - a);
+#pragma unroll
+    for (int j = 0; j < NM - 1; ++j) { const double per = avf[j] / avfsum; avf[j] = avf[j] - per * dif; }
+    avfsum = 1.0 - a;
+  }
+  avf[NM - 1] = 1.0 - avfsum;
+}
+
+// F_pe.c (i_paper = 1) on the PRELAX 2 branch's own arrays: the published F_pe
+// reads RUN fields that are never written
+template <int NM>
+__device__ double fpe5(const M5P &P, const double *ar, const double *t0, const double *p0, double pe) {
+  double F = 0.0;
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) {
+    const double G = P.gamma[iv], Pi = P.pinf[iv];
+    F += ar[iv] * t0[iv] * ((p0[iv] + G * Pi + (G - 1.0) * pe) / (pe + G * Pi + (G - 1.0) * pe));
+  }
+  return F + -1.0;
+}
+template <int NM>
+__device__ double dfpe5(const M5P &P, const double *ar, const double *t0, const double *p0, double pe) {
+  double D = 0.0;
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) {
+    const double G = P.gamma[iv], Pi = P.pinf[iv];
+    const double c1 = ar[iv] * t0[iv];
+    const double f = (p0[iv] + G * Pi + (G - 1.0) * pe);
+    const double g = (pe + G * Pi + (G - 1.0) * pe);
+    D += (c1 * (G - 1.0) * g - c1 * f * G) / sq(g);
+  }
+  return D;
+}
+
+// m5_relaxationvec.c:3-620 on x in place: 0, -1 NaN, -3 PRELAX 2 Newton failure
+template <int NM>
+__device__ int relax5(const M5P &P, double *x) {
+  using L = L5<NM>;
+  const double a = P.amin / 100.0, eps_sml = 1.0e-12, eps_big = 1.0e-8;
+  double ar[NM], are[NM], avf[NM], avf_init[NM], are_el[NM], A[9], G[9], Gmag;
+  double rho = 0.0;
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) { ar[iv] = x[L::I0 + iv]; are[iv] = x[L::I4 + iv]; rho += ar[iv]; are_el[iv] = 0.0; }
+  const double u = x[L::I1 + 0] / rho, v = x[L::I1 + 1] / rho, w = x[L::I1 + 2] / rho;
+  const double rE = x[L::I2];
+#pragma unroll
+  for (int iv = 0; iv < NM - 1; ++iv) avf[iv] = x[L::I3 + iv];
+  vf_limit1<NM>(a, avf);
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) avf_init[iv] = avf[iv];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) A[k] = x[L::I5 + k];
+  gmat(A, G, Gmag);
+  if (P.prelax == 1 || P.prelax == 2) {
+    double t0[NM], p0[NM], pe;
+    if (P.prelax == 1) {   // m5_relaxationvec.c:89-190
+#pragma unroll
+      for (int iv = 0; iv < NM; ++iv) {
+        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[iv]) : 0.0;
+        t0[iv] = 1.0 / (ar[iv] / avf[iv]);
+        p0[iv] = ((P.gamma[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - P.gamma[iv] * P.pinf[iv];
+      }
+      const double G0 = P.gamma[0], G1 = P.gamma[1], P0 = P.pinf[0], P1 = P.pinf[1];
+      const double A_1 = (avf[0] / G0) * (p0[0] + P0) / ((avf[0] / G0) + (avf[1] / G1));
+      const double A_2 = (avf[1] / G1) * (p0[1] + P1) / ((avf[0] / G0) + (avf[1] / G1));
+      pe = 0.5 * (A_1 + A_2 - (P0 + P1)) + sqrt(0.25 * (sq((A_2 - A_1 - (P1 - P0)))) + A_1 * A_2);
+      if (isnan(pe)) return -1;
+    } else {               // m5_relaxationvec.c:192-363
+      double max_avf = -1.0;
+      int principal = 0;
+#pragma unroll
+      for (int iv = 0; iv < NM; ++iv) {
+        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[iv]) : 0.0;
+        t0[iv] = 1.0 / (ar[iv] / avf[iv]);
+        p0[iv] = ((P.gamma[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - P.gamma[iv] * P.pinf[iv];
+        if (p0[iv] < -(1.0 - eps_big) * P.pinf[iv] + eps_big) p0[iv] = -(1.0 - eps_big) * P.pinf[iv] + 1.0;
+        if (avf[iv] > max_avf) { max_avf = avf[iv]; principal = iv; }
+      }
+      double pe_old = 0.001, pe_new = pe_old;
+      int iter = 0;
+      double cond_1 = fabs(pe_new - pe_old);
+      double cond_2 = fabs(fpe5<NM>(P, ar, t0, p0, pe_new));
+      while (((cond_2 >= 1.0e-12) || (cond_1 >= 1.0e-4)) && (iter <= 101)) {
+        pe_new = pe_old - 0.8 * (fpe5<NM>(P, ar, t0, p0, pe_old) / dfpe5<NM>(P, ar, t0, p0, pe_old));
+        cond_1 = fabs(pe_new - pe_old);
+        cond_2 = fabs(fpe5<NM>(P, ar, t0, p0, pe_new));
+        if (iter >= 100 && ((fabs(pe_old - pe_new) < 1.0e3) || isnan(pe_new))) return -3;
+        pe_old = pe_new;
+        iter++;
+      }
+      if (isnan(pe_new)) return -1;
+      double lim = -(1.0 - eps_big) * P.pinf[0];
+#pragma unroll
+      for (int iv = 1; iv < NM; ++iv)
+        if (iv == principal) lim = -(1.0 - eps_big) * P.pinf[iv];
+      if (pe_new < (lim + eps_big)) pe_new = lim + 1.0;
+      pe = pe_new;
+    }
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv) {
+      const double Gk = P.gamma[iv], Pk = P.pinf[iv];
+      const double t1 = t0[iv] * (p0[iv] + Gk * Pk + (Gk - 1.0) * pe) / (pe + Gk * Pk + (Gk - 1.0) * pe);
+      const double r_m = 1.0 / t1;
+      avf[iv] = ar[iv] / r_m;
+    }
+    vf_limit2<NM>(a, avf);
+#pragma unroll
+    for (int i = 0; i < 9; i++) A[i] = A[i] * (avf[0] / avf_init[0]);
+  } else if (P.prelax == 3) {   // m5_relaxationvec.c:365-560
+    double rhoKs[NM], pk[NM];
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv) {
+      rhoKs[iv] = 0.0;
+      if (avf[iv] < 0.0 || ar[iv] < 0.0) { avf[iv] = eps_sml; ar[iv] = eps_sml; are[iv] = eps_sml; }
+      if (avf[iv] >= 1.0) avf[iv] = 1.0 - eps_sml;
+    }
+    int irelax = 1;
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv)
+      if (avf[iv] > 1.0 - eps_sml) irelax = 0;
+    if (irelax == 1) {
+      double pr = 0.0;
+#pragma unroll
+      for (int iv = 0; iv < NM; ++iv) {
+        if (avf[iv] > eps_sml) {
+          const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[iv]) : 0.0;
+          pk[iv] = ((P.gamma[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - P.gamma[iv] * P.pinf[iv];
+          if (pk[iv] <= -(1.0 - eps_big) * P.pinf[iv] + eps_big) pk[iv] = -(1.0 - eps_big) * P.pinf[iv] + eps_big;
+        } else {
+          pk[iv] = 0.0;
+        }
+        pr += avf[iv] * pk[iv];
+      }
+      int iter = 0;
+      double Fv = 1.0e-9, dF = 1.0e9;
+      while ((fabs(Fv) > 1.0e-12) && (iter < 500)) {
+        pr = pr - Fv / dF;
+        iter = iter + 1;
+#pragma unroll
+        for (int iv = 0; iv < NM; ++iv)
+          if (pr < -(1.0 - eps_big) * P.pinf[iv] + eps_big) pr = -(1.0 - eps_big) * P.pinf[iv] + 1.0;
+        Fv = -1.0;
+        dF = 0.0;
+#pragma unroll
+        for (int iv = 0; iv < NM; ++iv) {
+          if (avf[iv] > eps_sml) {
+            rhoKs[iv] = ar[iv] / maxr(avf[iv], eps_sml) * pow((pr + P.pinf[iv]) / (pk[iv] + P.pinf[iv]), 1.0 / P.gamma[iv]);
+            Fv = Fv + ar[iv] / rhoKs[iv];
+            dF = dF - ar[iv] / (P.gamma[iv] * rhoKs[iv] * (pr + P.pinf[iv]));
+          }
+        }
+        if (isnan(pr) || isnan(Fv) || isnan(dF)) return -1;
+      }
+#pragma unroll
+      for (int iv = 0; iv < NM; ++iv)
+        if (avf[iv] > eps_sml) avf[iv] = ar[iv] / rhoKs[iv];
+      vf_limit1<NM>(a, avf);
+    }
+    const int i_as = ((avf[0] < avf[1]) && (P.mush[1] != 0.0)) ? 1 : 0;
+    const double ratio = (i_as == 1) ? (avf[1] / avf_init[1]) : (avf[0] / avf_init[0]);
+#pragma unroll
+    for (int i = 0; i < 9; i++) A[i] = A[i] * ratio;
+  }
+  if (P.prelax >= 1 && P.prelax <= 3) {
+    // re-initialisation: the pressure from the total energy
+    double e_elast = 0.0;
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv) {
+      are_el[iv] = (P.mush[iv] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[iv]) : 0.0;
+      const double e_elastic_K = are_el[iv] / (ar[iv]);
+      const double Ys = ar[iv] / rho;
+      e_elast += Ys * e_elastic_K;
+    }
+    const double e_hyd = rE / rho - 0.5 * (sq((u)) + sq((v)) + sq((w))) - e_elast;
+    double num = 0.0, den = 0.0;
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv) {
+      num += avf[iv] * P.gamma[iv] * P.pinf[iv] / (P.gamma[iv] - 1.0);
+      den += avf[iv] / (P.gamma[iv] - 1.0);
+    }
+    const double p_relaxed = (rho * e_hyd - num) / den;
+    if (isnan(p_relaxed)) return -1;
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv)
+      are[iv] = (avf[iv] * (p_relaxed + P.gamma[iv] * P.pinf[iv]) / (P.gamma[iv] - 1.0)) + are_el[iv];
+  }
+  bool bad = false;
+#pragma unroll
+  for (int iv = 0; iv < NM - 1; ++iv) { bad |= isnan(avf[iv]); x[L::I3 + iv] = avf[iv]; }
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) { bad |= isnan(are[iv]); x[L::I4 + iv] = are[iv]; }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) x[L::I5 + k] = A[k];
+  return bad ? -1 : 0;
+}
+
+// avf_limit.c:3-21 with max_limit 100 on the fractions of X
+template <int NM>
+__device__ void avf_limit5(double a, double *X) {
+  using L = L5<NM>;
+  double avf[NM], avfsum = 0.0;
+#pragma unroll
+  for (int v = 0; v < NM - 1; ++v) {
+    avf[v] = X[L::I3 + v];
+    if (avf[v] <= a) avf[v] = a;
+    else if (avf[v] >= (1.0 - a)) avf[v] = 1.0 - a;
+    avfsum += avf[v];
+  }
+  if (avfsum > (1.0 - a)) {
+    const double dif = avfsum - (1.0 - a);
+#pragma unroll
+    for (int j = 0; j < NM - 1; ++j) avf[j] -= (avf[j] / avfsum) * dif;
+  }
+#pragma unroll
+  for (int v = 0; v < NM - 1; ++v) X[L::I3 + v] = avf[v];
+}
+
+// m5_facevalues_MUSCL.c:3-44, iside = 1: the value across face f of leaf i
+template <int NM>
+__device__ void stencil5(const M5P &P, const double *__restrict__ S, int n, int i, int kind, int nb,
+                         const int *__restrict__ fine4, double *X) {
+  using L = L5<NM>;
+  if (kind == FV_FK_BOUNDARY) {
+    // the own cell's record vector: raw state with the clipped fractions (m5_bc moves only u, v, w)
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) X[v] = S[(size_t)v * n + i];
+    const double a = P.amin / 100.0;
+    double at[NM], sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM - 1; ++k) {
+      at[k] = X[L::I3 + k];
+      if (at[k] <= a) at[k] = a;
+      else if (at[k] >= (1.0 - a)) at[k] = 1.0 - a;
+      sum += at[k];
+    }
+    if (sum > (1.0 - a)) {
+      const double dif = sum - (1.0 - a);
+#pragma unroll
+      for (int j = 0; j < NM - 1; ++j) { const double per = at[j] / sum; at[j] = at[j] - per * dif; }
+    }
+#pragma unroll
+    for (int k = 0; k < NM - 1; ++k) X[L::I3 + k] = at[k];
+  } else if (kind == FV_FK_FINE) {
+    const int4 q = *reinterpret_cast<const int4 *>(&fine4[4 * nb]);
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) {
+      const double *Sv = S + (size_t)v * n;
+      double acc = 0.0;
+      acc += Sv[q.x] / ((double)4);
+      acc += Sv[q.y] / ((double)4);
+      acc += Sv[q.z] / ((double)4);
+      acc += Sv[q.w] / ((double)4);
+      X[v] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) X[v] = S[(size_t)v * n + nb];
+  }
+  avf_limit5<NM>(P.amin / 100.0, X);
+}
+
+__device__ void dist5(const Geo &G, int kind, int lvl, double &dcf, double &dcc) {
+  if (kind == FV_FK_SAME || kind == FV_FK_MIRROR) { dcf = G.dcf[0][lvl]; dcc = G.dcc[0][lvl]; }
+  else if (kind == FV_FK_FINE) { dcf = G.dcf[1][lvl]; dcc = G.dcc[1][lvl]; }
+  else if (kind == FV_FK_COARSE) { dcf = G.dcf[2][lvl]; dcc = G.dcc[2][lvl]; }
+  else { dcf = 0.0; dcc = 0.0; }
+}
+
+// m5_faceinterp_V1 (m5_faceinterp.c:60-93) for one leaf per thread: MUSCL face
+// states with minmod (m5_reconctruct.c:93-190; g_limitedvars is 0 on the
+// alpha-rho-e block, m5_input.c:196-204), the VF limit and m5_relaxationvec
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__restrict__ Gp,
+                                                  const double *__restrict__ S, const int *__restrict__ nbr,
+                                                  const uint32_t *__restrict__ info, const int *__restrict__ fine4,
+                                                  double *__restrict__ SF, int *__restrict__ flag) {
+  using L = L5<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = P.amin / 100.0;
+  const uint32_t inf = info[i];
+  const int lvl = fvk::cell_level(inf);
+  double Sc[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
+  avf_limit5<NM>(a, Sc);
+#pragma unroll 1
+  for (int f = 0; f < 6; f++) {
+    const int fo = FCOPP5[f];
+    const int kf = fvk::face_kind(inf, f), kb = fvk::face_kind(inf, fo);
+    double Xf[L::NEQ], Xb[L::NEQ];
+    stencil5<NM>(P, S, n, i, kf, nbr[(size_t)f * n + i], fine4, Xf);
+    stencil5<NM>(P, S, n, i, kb, nbr[(size_t)fo * n + i], fine4, Xb);
+    double dcf, dp, dcf_b, dm;
+    dist5(*Gp, kf, lvl, dcf, dp);
+    dist5(*Gp, kb, lvl, dcf_b, dm);
+    double Fv[L::NEQ];
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) {
+      const double df = Xf[v] - Sc[v], db = Xb[v] - Sc[v];
+      const double lim = (v >= L::I4 && v < L::I5) ? 0.0 : 1.0;
+      double g = 0.0;
+      if (kf != FV_FK_BOUNDARY && fabs(df) > 1.0e-10) {
+        const double product = (-db / dm) * (df / dp);
+        if (product > 0.0) {
+          if (fabs(-db / dm) > fabs(df / dp)) g = lim * (df / dp);
+          else g = lim * (-db / dm);
+        } else {
+          g = 0.0;
+        }
+      }
+      Fv[v] = Sc[v] + dcf * g;
+    }
+    // VF limit (m5_reconctruct.c:142-170)
+    double avf[NM], avfsum = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM - 1; ++k) {
+      avf[k] = Fv[L::I3 + k];
+      if (avf[k] <= a) avf[k] = a;
+      else if (avf[k] >= (1.0 - (2.0 * a))) avf[k] = 1.0 - (2.0 * a);
+      avfsum += avf[k];
+    }
+    if (avfsum >= (1.0 - a)) {
+      const double dif = avfsum - (1.0 - a);
+#pragma unroll
+      for (int j = 0; j < NM - 1; ++j) { const double per = avf[j] / avfsum; avf[j] = avf[j] - per * dif; }
+    }
+#pragma unroll
+    for (int k = 0; k < NM - 1; ++k) Fv[L::I3 + k] = avf[k];
+    const int rc = relax5<NM>(P, Fv);
+    if (rc) atomicOr(flag, rc == -3 ? 4 : 1);
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) SF[((size_t)f * L::NEQ + v) * n + i] = Fv[v];
+  }
+}
+
+// the record a face flux reads: the face state (ORDER 2, m5_facercnstr) or
+// the cell state (ORDER 1, m5_facevalues) of leaf j across its face fj
+template <int NM>
+__device__ void side_record(const M5P &P, const double *__restrict__ S, const double *__restrict__ SF, int n, int j,
+                            int fj, Rec5<NM> &r) {
+  using L = L5<NM>;
+  double x[L::NEQ];
+  if (P.order == 2) {
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) x[v] = SF[((size_t)fj * L::NEQ + v) * n + j];
+    record<NM, true>(P, x, r, S[(size_t)L::I3 * n + j]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + j];
+    record<NM>(P, x, r);
+  }
+}
+
+// m5_residual.c:7-238 for one leaf per thread.  A mirrored face (the earlier
+// leaf's flux_flag protocol, a face kind of the pack) recomputes the earlier
+// leaf's evaluation of that face and negates it, as the reference reuses it.
+// gfm bit f: the face used the GFM flux.
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__restrict__ Gp,
+                                                     const double *__restrict__ S, const double *__restrict__ SF,
+                                                     const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+                                                     const int *__restrict__ fine4, double *__restrict__ RHS,
+                                                     uint8_t *__restrict__ gfm, int *__restrict__ flag) {
+  using L = L5<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Geo &G = *Gp;
+  const uint32_t inf = info[i];
+  const int lvl = fvk::cell_level(inf);
+  // sources of the cell record (m5_residual.c:31-59)
+  double src_avf[NM - 1], src_st[NM][9], Am[9];
+  {
+    double x[L::NEQ];
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + i];
+    Rec5<NM> sc;
+    record<NM>(P, x, sc);
+#pragma unroll
+    for (int k = 0; k < NM - 1; ++k) src_avf[k] = sc.avf[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Am[k] = sc.A[k];
+    double Gm[9], Gmag;
+    gmat(sc.A, Gm, Gmag);
+#pragma unroll
+    for (int k = 0; k < NM; ++k) stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, P.mush[k]);
+  }
+  const double A1 = Am[0], B1 = Am[3], C1 = Am[6];
+  const double A2 = Am[1], B2 = Am[4], C2 = Am[7];
+  const double A3 = Am[2], B3 = Am[5], C3 = Am[8];
+  double rhs[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) rhs[v] = 0.0;
+  uint8_t gbits = 0;
+  bool ok = true;
+#pragma unroll 1
+  for (int f = 0; f < 6; f++) {
+    const int kind = fvk::face_kind(inf, f);
+    const int nb = nbr[(size_t)f * n + i];
+    const int fo = FCOPP5[f];
+    double ff[L::NEQ], uf0 = 0.0, uf1 = 0.0, uf2 = 0.0, Area_f = 0.0;
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) ff[v] = 0.0;
+    if (kind == FV_FK_MIRROR) {
+      Rec5<NM> sL, sR;
+      side_record<NM>(P, S, SF, n, nb, fo, sL);
+      side_record<NM>(P, S, SF, n, i, f, sR);
+      double out[L::NEQ], un[3];
+      const bool g = sL.phi0 * sR.phi0 < 0.0;
+      ok &= g ? gfm_subface<NM>(P, G.frame[fo], sL, sR, out, un)
+              : hllc_subface<NM>(P, G.frame[fo], 0, sL, sR, S, n, nb, out, un);
+      const double A = G.area[lvl];
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) {
+        double t = 0.0;
+        t += (A) * out[v];
+        ff[v] = -t;
+      }
+      uf0 = -(un[0] / ((double)1)); uf1 = -(un[1] / ((double)1)); uf2 = -(un[2] / ((double)1));
+      Area_f += A;
+    } else {
+      const int Nneig = (kind == FV_FK_FINE) ? 4 : 1;
+      const double A = (kind == FV_FK_FINE) ? G.area[lvl + 1] : G.area[lvl];
+      for (int k = 0; k < Nneig; ++k) Area_f += A;
+      Rec5<NM> sL;
+      side_record<NM>(P, S, SF, n, i, f, sL);
+      for (int k = 0; k < Nneig; ++k) {
+        Rec5<NM> sR;
+        int bcode = 0;
+        if (kind == FV_FK_BOUNDARY) {
+          bcode = -nb;
+          side_record<NM>(P, S, SF, n, i, f, sR);
+          apply_bc5<NM>(bcode, G.frame[f], sL, sR);
+        } else {
+          const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
+          side_record<NM>(P, S, SF, n, j, fo, sR);
+        }
+        double out[L::NEQ], un[3];
+        if (sL.phi0 * sR.phi0 < 0.0) {
+          gbits |= (uint8_t)(1u << f);
+          Rec5<NM> gL = sL;   // the ghost rewrites only this sub-face's copies
+          ok &= gfm_subface<NM>(P, G.frame[f], gL, sR, out, un);
+        } else {
+          ok &= hllc_subface<NM>(P, G.frame[f], bcode, sL, sR, S, n, i, out, un);
+        }
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) ff[v] += (A) * out[v];
+        uf0 = un[0] / ((double)Nneig);   // overwritten per sub-face (m5_residual.c:163-165)
+        uf1 = un[1] / ((double)Nneig);
+        uf2 = un[2] / ((double)Nneig);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) rhs[v] += ff[v];
+    // non-conservative terms (m5_residual.c:176-214)
+    const double u_nc = uf0, nx = G.frame[f][0];
+    const double v_nc = uf1, ny = G.frame[f][1];
+    const double w_nc = uf2, nz = G.frame[f][2];
+#pragma unroll
+    for (int k = 0; k < NM - 1; ++k)
+      rhs[L::I3 + k] += Area_f * (src_avf[k] * u_nc * nx + src_avf[k] * v_nc * ny + src_avf[k] * w_nc * nz);
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      const double *s = src_st[k];
+      rhs[L::I4 + k] += Area_f * (s[0] * u_nc * nx + s[1] * v_nc * nx + s[2] * w_nc * nx + s[3] * u_nc * ny +
+                                  s[4] * v_nc * ny + s[5] * w_nc * ny + s[6] * u_nc * nz + s[7] * v_nc * nz +
+                                  s[8] * w_nc * nz);
+    }
+    const int v5 = L::I5;
+    rhs[v5 + 0] += Area_f * ((A1 * v_nc * ny + A1 * w_nc * nz) + B1 * v_nc * nx - C1 * w_nc * nx);
+    rhs[v5 + 1] += Area_f * ((A2 * v_nc * ny + A2 * w_nc * nz) + B2 * v_nc * nx - C2 * w_nc * nx);
+    rhs[v5 + 2] += Area_f * ((A3 * v_nc * ny + A3 * w_nc * nz) + B3 * v_nc * nx - C3 * w_nc * nx);
+    rhs[v5 + 3] += Area_f * ((B1 * u_nc * nx + B1 * w_nc * nz) + A1 * u_nc * ny - C1 * w_nc * ny);
+    rhs[v5 + 4] += Area_f * ((B2 * u_nc * nx + B2 * w_nc * nz) + A2 * u_nc * ny - C2 * w_nc * ny);
+    rhs[v5 + 5] += Area_f * ((B3 * u_nc * nx + B3 * w_nc * nz) + A3 * u_nc * ny - C3 * w_nc * ny);
+    rhs[v5 + 6] += Area_f * ((C1 * u_nc * nx + C1 * v_nc * ny) + A1 * u_nc * nz - B1 * v_nc * nz);
+    rhs[v5 + 7] += Area_f * ((C2 * u_nc * nx + C2 * v_nc * ny) + A2 * u_nc * nz - B2 * v_nc * nz);
+    rhs[v5 + 8] += Area_f * ((C3 * u_nc * nx + C3 * v_nc * ny) + A3 * u_nc * nz - B3 * v_nc * nz);
+  }
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) {
+    ok &= !isnan(rhs[v]);
+    RHS[(size_t)v * n + i] = rhs[v];
+  }
+#pragma unroll
+  for (int f = 0; f < 6; f++) gfm[6 * i + f] = (gbits >> f) & 1u;
+  if (!ok) atomicOr(flag, 1);
+}
+
+// m5_relaxation_solid.c:18-24: Sb keeps the state before relaxation for the re-projection trigger
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_relax(int n, M5P P, double *__restrict__ S, double *__restrict__ Sb,
+                                                  int *__restrict__ flag) {
+  using L = L5<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) { x[v] = S[(size_t)v * n + i]; Sb[(size_t)v * n + i] = x[v]; }
+  const int rc = relax5<NM>(P, x);
+  if (rc) atomicOr(flag, rc == -3 ? 4 : 1);
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = x[v];
+}
+
 // ---- host side --------------------------------------------------------------
 
 int m5_params(const fv_m5_params *p, M5P &P) {
@@ -657,6 +1405,7 @@ int m5_params(const fv_m5_params *p, M5P &P) {
     P.zerou[k] = p->zerou[k];
   }
   P.amin = p->amin; P.avf_lim = p->avf_lim; P.dt = p->dt;
+  P.prelax = p->prelax; P.order = p->order;
   return FV_OK;
 }
 
@@ -712,8 +1461,8 @@ int upload_pack(Bufs &B, const fv_pack *pk, MeshDev &M) {
   return FV_OK;
 }
 
-// leaves with mark[i] != 0 in leaf order; returns their count in *count
-int compact_marked(Bufs &B, int n, const uint8_t *mark, int **list, int *count) {
+// leaves with mark[i] != 0 in leaf order; returns their count in *count and on the device in *dcount
+int compact_marked(Bufs &B, int n, const uint8_t *mark, int **list, int *count, int **dcount) {
   int *d_list = nullptr, *d_num = nullptr;
   HIPCHK(B.alloc(&d_list, (size_t)n));
   HIPCHK(B.alloc(&d_num, 1));
@@ -725,6 +1474,7 @@ int compact_marked(Bufs &B, int n, const uint8_t *mark, int **list, int *count) 
   HIPCHK(hipcub::DeviceSelect::Flagged(tmp, tmp_bytes, idx, mark, d_list, d_num, n));
   HIPCHK(hipMemcpy(count, d_num, sizeof(int), hipMemcpyDeviceToHost));
   *list = d_list;
+  *dcount = d_num;
   return FV_OK;
 }
 
@@ -789,10 +1539,10 @@ int rk_t(const M5P &P, const fv_pack *pk, int irk, double *SN, double *S, const 
                      dmark);
   HIPCHK(hipGetLastError());
   if (irk == 1) {
-    int *list = nullptr, count = 0;
-    if ((rc = compact_marked(B, n, dmark, &list, &count)) != FV_OK) return rc;
+    int *list = nullptr, count = 0, *dcount = nullptr;
+    if ((rc = compact_marked(B, n, dmark, &list, &count, &dcount)) != FV_OK) return rc;
     if (count > 0) {
-      hipLaunchKernelGGL(k_m5_gfm_seq<NM>, dim3(1), dim3(64), 0, 0, n, count, list, P, M.G, M.nbr, M.info, M.fine4,
+      hipLaunchKernelGGL(k_m5_gfm_seq<NM>, dim3(1), dim3(64), 0, 0, n, dcount, list, P, M.G, M.nbr, M.info, M.fine4,
                          dgf, dSN, dSin, dS, dR, derr);
       HIPCHK(hipGetLastError());
       int err = 0;
@@ -832,11 +1582,11 @@ int rel_t(const M5P &P, const fv_pack *pk, const double *Sb, double *S, const ui
   HIPCHK(hipMemset(derr, 0, sizeof(int)));
   hipLaunchKernelGGL(k_m5_rel_mark<NM>, dim3((n + 255) / 256), dim3(256), 0, 0, n, P, dSb, dS, dmark);
   HIPCHK(hipGetLastError());
-  int *list = nullptr, count = 0;
-  if ((rc = compact_marked(B, n, dmark, &list, &count)) != FV_OK) return rc;
+  int *list = nullptr, count = 0, *dcount = nullptr;
+  if ((rc = compact_marked(B, n, dmark, &list, &count, &dcount)) != FV_OK) return rc;
   if (count > 0) {
-    hipLaunchKernelGGL(k_m5_rel_seq<NM>, dim3(1), dim3(64), 0, 0, n, count, list, P, M.G, M.nbr, M.info, M.fine4, dgf,
-                       dSb, dS, derr);
+    hipLaunchKernelGGL(k_m5_rel_seq<NM>, dim3(1), dim3(64), 0, 0, n, dcount, list, P, M.G, M.nbr, M.info, M.fine4,
+                       dgf, dSb, dS, derr);
     HIPCHK(hipGetLastError());
     int err = 0;
     HIPCHK(hipMemcpy(&err, derr, sizeof(int), hipMemcpyDeviceToHost));
@@ -847,7 +1597,293 @@ int rel_t(const M5P &P, const fv_pack *pk, const double *Sb, double *S, const ui
   return FV_OK;
 }
 
+
+// ---- device-resident model-5 stage (fv_m5_dev_*) -----------------------------
+
+template <class T>
+static hipError_t dalloc(T **p, size_t count) {
+  return hipMalloc((void **)p, (count > 0 ? count : 1) * sizeof(T));
+}
+
 }  // namespace
+
+struct fv_m5_dev {
+  M5P P{};
+  int n = 0, nfine = 0, neq = 0, rk_steps = 3;
+  hipStream_t stream = nullptr;
+  Geo *G = nullptr;
+  int *nbr = nullptr, *fine4 = nullptr;
+  uint32_t *info = nullptr;
+  double *S = nullptr, *S2 = nullptr, *SN = nullptr, *SF = nullptr, *RHS = nullptr, *Sb = nullptr;
+  uint8_t *gfm = nullptr, *mark = nullptr;
+  int *flag = nullptr;        // [0] NaN (1) / Newton failure (4) bits, [1] GFM flag on a boundary face
+  int *list = nullptr, *count = nullptr;
+  void *cub_tmp = nullptr;
+  size_t cub_bytes = 0;
+};
+
+namespace {
+
+void m5_free(fv_m5_dev *c) {
+  void *ptrs[] = {c->G, c->nbr, c->fine4, c->info, c->S, c->S2, c->SN, c->SF, c->RHS, c->Sb,
+                  c->gfm, c->mark, c->flag, c->list, c->count, c->cub_tmp};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// leaves with mark != 0 in leaf order into c->list, their number into c->count (device side only)
+int m5_compact(fv_m5_dev *c) {
+  hipcub::CountingInputIterator<int> idx(0);
+  size_t bytes = c->cub_bytes;
+  HIPCHK(hipcub::DeviceSelect::Flagged(c->cub_tmp, bytes, idx, c->mark, c->list, c->count, c->n, c->stream));
+  return FV_OK;
+}
+
+dim3 grid_of(int n) { return dim3((unsigned)((n + 255) / 256)); }
+
+template <int NM>
+int m5_faceinterp_t(fv_m5_dev *c) {
+  hipLaunchKernelGGL(k_m5_recon<NM>, grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S, c->nbr,
+                     c->info, c->fine4, c->SF, c->flag);
+  HIPCHK(hipGetLastError());
+  return FV_OK;
+}
+
+template <int NM>
+int m5_residual_t(fv_m5_dev *c) {
+  hipLaunchKernelGGL(k_m5_residual<NM>, grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S, c->SF,
+                     c->nbr, c->info, c->fine4, c->RHS, c->gfm, c->flag);
+  HIPCHK(hipGetLastError());
+  return FV_OK;
+}
+
+// one stage irk (advancetimeexplicit.c:11-24 with MODEL 5), no host synchronisation;
+// ev brackets the residual kernel when given
+template <int NM>
+int m5_stage_t(fv_m5_dev *c, int irk, hipEvent_t *ev) {
+  int rc;
+  const dim3 g = grid_of(c->n), b(256);
+  if (c->P.order == 2 && (rc = m5_faceinterp_t<NM>(c)) != FV_OK) return rc;
+  if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
+  if ((rc = m5_residual_t<NM>(c)) != FV_OK) return rc;
+  if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
+  // m5_RK(irk): S2 from S, SN and RHS; stage 1 re-projects the crossing leaves in leaf order
+  hipLaunchKernelGGL(k_m5_rk<NM>, g, b, 0, c->stream, c->n, c->P, irk, c->G, c->info, c->SN, c->S, c->S2, c->RHS,
+                     c->mark);
+  HIPCHK(hipGetLastError());
+  if (irk == 1) {
+    if ((rc = m5_compact(c)) != FV_OK) return rc;
+    hipLaunchKernelGGL(k_m5_gfm_seq<NM>, dim3(1), dim3(64), 0, c->stream, c->n, c->count, c->list, c->P, c->G,
+                       c->nbr, c->info, c->fine4, c->gfm, c->SN, c->S, c->S2, c->RHS, c->flag + 1);
+    HIPCHK(hipGetLastError());
+  }
+  std::swap(c->S, c->S2);
+  // m5_relaxation_solid: relax every leaf, then re-project the leaves whose solid fraction crossed
+  hipLaunchKernelGGL(k_m5_relax<NM>, g, b, 0, c->stream, c->n, c->P, c->S, c->Sb, c->flag);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_m5_rel_mark<NM>, g, b, 0, c->stream, c->n, c->P, c->Sb, c->S, c->mark);
+  HIPCHK(hipGetLastError());
+  if ((rc = m5_compact(c)) != FV_OK) return rc;
+  hipLaunchKernelGGL(k_m5_rel_seq<NM>, dim3(1), dim3(64), 0, c->stream, c->n, c->count, c->list, c->P, c->G, c->nbr,
+                     c->info, c->fine4, c->gfm, c->Sb, c->S, c->flag + 1);
+  HIPCHK(hipGetLastError());
+  return FV_OK;
+}
+
+int m5_stage(fv_m5_dev *c, int irk, hipEvent_t *ev) {
+  return c->P.nmat == 2 ? m5_stage_t<2>(c, irk, ev) : m5_stage_t<3>(c, irk, ev);
+}
+
+// the flags the kernels raised since the last check, as a status (cleared)
+int m5_check(fv_m5_dev *c) {
+  int h[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(h, c->flag, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (h[0] || h[1]) HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
+  if (h[1]) return fv_set_err(FV_ERR_ARG, "model 5: a GFM face flag on a boundary face");
+  if (h[0] & 4) return fv_set_err(FV_ERR_NAN, "model 5: PRELAX 2 Newton failed (m5_relaxationvec.c:244-272)");
+  if (h[0]) return fv_set_err(FV_ERR_NAN, "model 5: NaN in a face state, flux or relaxation");
+  return FV_OK;
+}
+
+}  // namespace
+
+extern "C" int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_dev **out) {
+  if (!out) return fv_set_err(FV_ERR_ARG, "null argument");
+  *out = nullptr;
+  M5P P;
+  int rc = m5_params(p, P);
+  if (rc != FV_OK) return rc;
+  if (p->prelax < 0 || p->prelax > 3) return fv_set_err(FV_ERR_ARG, "model 5: prelax must be 0..3");
+  if (p->order != 1 && p->order != 2) return fv_set_err(FV_ERR_ARG, "model 5: order must be 1 or 2");
+  if (p->rk_steps < 1 || p->rk_steps > 3) return fv_set_err(FV_ERR_ARG, "model 5: rk_steps must be 1..3");
+  if (!pk || pk->n <= 0 || !pk->nbr || !pk->info) return fv_set_err(FV_ERR_ARG, "model 5: empty mesh");
+  fv_m5_dev *c = new fv_m5_dev();
+  c->P = P;
+  c->n = pk->n;
+  c->nfine = pk->nfine;
+  c->neq = 3 * P.nmat + 12;
+  c->rk_steps = p->rk_steps;
+  const size_t n = (size_t)c->n, nf = (size_t)(pk->nfine > 0 ? pk->nfine : 1), sz = n * c->neq;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  chk(hipStreamCreate(&c->stream));
+  chk(dalloc(&c->G, 1));
+  chk(dalloc(&c->nbr, 6 * n));
+  chk(dalloc(&c->fine4, 4 * nf));
+  chk(dalloc(&c->info, n));
+  chk(dalloc(&c->S, sz)); chk(dalloc(&c->S2, sz)); chk(dalloc(&c->SN, sz)); chk(dalloc(&c->Sb, sz));
+  chk(dalloc(&c->RHS, sz));
+  chk(dalloc(&c->SF, 6 * sz));
+  chk(dalloc(&c->gfm, 6 * n)); chk(dalloc(&c->mark, n));
+  chk(dalloc(&c->flag, 2)); chk(dalloc(&c->list, n)); chk(dalloc(&c->count, 1));
+  if (e == hipSuccess) {
+    hipcub::CountingInputIterator<int> idx(0);
+    chk(hipcub::DeviceSelect::Flagged(nullptr, c->cub_bytes, idx, c->mark, c->list, c->count, c->n, c->stream));
+    chk(dalloc((char **)&c->cub_tmp, c->cub_bytes));
+  }
+  if (e == hipSuccess) {
+    Geo G;
+    for (int f = 0; f < 6; f++)
+      for (int k = 0; k < 9; k++) G.frame[f][k] = pk->geo.frame[f][k];
+    for (int l = 0; l < FV_MAXLEVEL + 2; l++) {
+      G.h[l] = pk->geo.h[l]; G.area[l] = pk->geo.area[l]; G.vol[l] = pk->geo.vol[l];
+      for (int k = 0; k < 3; k++) { G.dcf[k][l] = pk->geo.dcf[k][l]; G.dcc[k][l] = pk->geo.dcc[k][l]; }
+    }
+    chk(hipMemcpy(c->G, &G, sizeof(Geo), hipMemcpyHostToDevice));
+    chk(hipMemcpy(c->nbr, pk->nbr, sizeof(int) * 6 * n, hipMemcpyHostToDevice));
+    chk(hipMemcpy(c->info, pk->info, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    if (pk->nfine > 0) chk(hipMemcpy(c->fine4, pk->fine4, sizeof(int) * 4 * (size_t)pk->nfine, hipMemcpyHostToDevice));
+    chk(hipMemset(c->flag, 0, 2 * sizeof(int)));
+    chk(hipMemset(c->S, 0, sizeof(double) * sz));
+    chk(hipMemset(c->SN, 0, sizeof(double) * sz));
+    chk(hipMemset(c->SF, 0, sizeof(double) * 6 * sz));
+  }
+  if (e != hipSuccess) {
+    m5_free(c);
+    return fv_set_err(e == hipErrorOutOfMemory ? FV_ERR_NOMEM : FV_ERR_HIP, hipGetErrorString(e));
+  }
+  *out = c;
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_detach(fv_m5_dev *c) {
+  if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
+  m5_free(c);
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_set_state(fv_m5_dev *c, const double *S) {
+  if (!c || !S) return fv_set_err(FV_ERR_ARG, "null argument");
+  std::vector<double> h;
+  to_soa(S, c->n, c->neq, h);
+  HIPCHK(hipMemcpyAsync(c->S, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_get_state(fv_m5_dev *c, double *S) {
+  if (!c || !S) return fv_set_err(FV_ERR_ARG, "null argument");
+  std::vector<double> h((size_t)c->n * c->neq);
+  HIPCHK(hipMemcpyAsync(h.data(), c->S, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  from_soa(h, c->n, c->neq, S);
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_begin_step(fv_m5_dev *c) {
+  if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(c->SN, c->S, sizeof(double) * (size_t)c->n * c->neq, hipMemcpyDeviceToDevice, c->stream));
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_stage(fv_m5_dev *c, int irk) {
+  if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (irk < 1 || irk > 4) return fv_set_err(FV_ERR_ARG, "model 5: irk must be 1..4 (m5_RK.c cases)");
+  int rc = m5_stage(c, irk, nullptr);
+  return rc != FV_OK ? rc : m5_check(c);
+}
+
+extern "C" int fv_m5_dev_advance(fv_m5_dev *c, int nsteps) {
+  if (!c || nsteps < 0) return fv_set_err(FV_ERR_ARG, "bad argument");
+  for (int s = 0; s < nsteps; s++) {
+    int rc = fv_m5_dev_begin_step(c);
+    for (int irk = 1; irk <= c->rk_steps && rc == FV_OK; irk++) rc = m5_stage(c, irk, nullptr);
+    if (rc != FV_OK) return rc;
+    if ((rc = m5_check(c)) != FV_OK) return rc;
+  }
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_faceinterp(fv_m5_dev *c) {
+  if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
+  int rc = c->P.nmat == 2 ? m5_faceinterp_t<2>(c) : m5_faceinterp_t<3>(c);
+  return rc != FV_OK ? rc : m5_check(c);
+}
+
+extern "C" int fv_m5_dev_residual(fv_m5_dev *c) {
+  if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
+  int rc = c->P.nmat == 2 ? m5_residual_t<2>(c) : m5_residual_t<3>(c);
+  return rc != FV_OK ? rc : m5_check(c);
+}
+
+extern "C" int fv_m5_dev_get_stage_arrays(fv_m5_dev *c, double *SF, double *RHS, uint8_t *gfm_face) {
+  if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
+  const size_t n = (size_t)c->n, neq = (size_t)c->neq;
+  if (SF) {
+    std::vector<double> h(6 * n * neq);
+    HIPCHK(hipMemcpyAsync(h.data(), c->SF, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++)
+      for (int f = 0; f < 6; f++)
+        for (size_t v = 0; v < neq; v++) SF[(i * 6 + f) * neq + v] = h[((size_t)f * neq + v) * n + i];
+  }
+  if (RHS) {
+    std::vector<double> h(n * neq);
+    HIPCHK(hipMemcpyAsync(h.data(), c->RHS, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    from_soa(h, c->n, c->neq, RHS);
+  }
+  if (gfm_face) {
+    HIPCHK(hipMemcpyAsync(gfm_face, c->gfm, 6 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_time_steps(fv_m5_dev *c, int nsteps, double *ms_stage, double *ms_residual) {
+  if (!c || nsteps < 1) return fv_set_err(FV_ERR_ARG, "bad argument");
+  hipEvent_t a0, a1, ev[2];
+  HIPCHK(hipEventCreate(&a0)); HIPCHK(hipEventCreate(&a1));
+  HIPCHK(hipEventCreate(&ev[0])); HIPCHK(hipEventCreate(&ev[1]));
+  double tot_res = 0.0;
+  int rc = FV_OK;
+  HIPCHK(hipEventRecord(a0, c->stream));
+  for (int s = 0; s < nsteps && rc == FV_OK; s++) {
+    rc = fv_m5_dev_begin_step(c);
+    for (int irk = 1; irk <= c->rk_steps && rc == FV_OK; irk++) {
+      rc = m5_stage(c, irk, ev);
+      if (rc != FV_OK) break;
+      HIPCHK(hipEventSynchronize(ev[1]));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      tot_res += ms;
+    }
+  }
+  HIPCHK(hipEventRecord(a1, c->stream));
+  HIPCHK(hipEventSynchronize(a1));
+  float all = 0.f;
+  HIPCHK(hipEventElapsedTime(&all, a0, a1));
+  hipEventDestroy(a0); hipEventDestroy(a1); hipEventDestroy(ev[0]); hipEventDestroy(ev[1]);
+  if (rc != FV_OK) return rc;
+  const int nst = nsteps * c->rk_steps;
+  if (ms_stage) *ms_stage = all / nst;
+  if (ms_residual) *ms_residual = tot_res / nst;
+  return m5_check(c);
+}
+
 
 extern "C" int fv_m5_neq(int nmat) { return 3 * nmat + 12; }
 

@@ -6,9 +6,8 @@
  * the right state's tangential velocity by the left one (a ghost state) before
  * its HLLC flux, and after the RK stage or the relaxation that moves a leaf
  * across the interface it re-projects that leaf's velocity onto the interface
- * normal.  These entry points replace those pieces one to one; the rest of the
- * model-5 stage (MUSCL face states, relaxation, the elastic EOS of the cells)
- * stays with the caller.
+ * normal.  The operator entry points replace those pieces one to one; the
+ * fv_m5_dev_* entry points run the whole model-5 stage on the device.
  *
  * State layout, NEQ = 3 nmat + 12 (m5_input.c): [alpha rho_k (nmat) |
  * rho u, rho v, rho w | rho E | alpha_k (nmat - 1) | alpha rho e_k (nmat) |
@@ -68,6 +67,41 @@ int fv_m5_rk(const fv_m5_params *p, const fv_pack *pk, int irk, double *SN, doub
  * in place. */
 int fv_m5_gfm_rel(const fv_m5_params *p, const fv_pack *pk, const double *S_before, double *S,
                   const uint8_t *gfm_face);
+
+/* ---- the whole model-5 stage, device resident ----
+ * advancetimeexplicit (advancetimeexplicit.c:3-29) with MODEL 5: per stage
+ * m5_faceinterp_V1 (m5_faceinterp.c:60-93, ORDER 2), m5_residual
+ * (m5_residual.c:7-238, via RHS.c:57-66), m5_RK (m5_RK.c:3-126) and
+ * m5_relaxation_solid (m5_relaxation_solid.c:3-40 with m5_relaxationvec.c).
+ * The leaves and face kinds are those of pk (its SI decides which faces reuse
+ * the earlier leaf's flux, m5_residual.c:62-103); p->prelax, p->order and
+ * p->rk_steps select the branches.  PRELAX 2 iterates on its own arrays (the
+ * published F_pe reads RUN fields that are never written).  Every call
+ * returns FV_OK or FV_ERR_*: FV_ERR_NAN where the reference prints NaN and
+ * exits, FV_ERR_ARG for a GFM flag on a boundary face. */
+typedef struct fv_m5_dev fv_m5_dev;
+int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_dev **out);
+int fv_m5_dev_detach(fv_m5_dev *c);
+/* LEAF.S of every leaf, row-major [n][NEQ] in leaf order */
+int fv_m5_dev_set_state(fv_m5_dev *c, const double *S);
+int fv_m5_dev_get_state(fv_m5_dev *c, double *S);
+/* RK(0) (RK.c:6-15): SN = S */
+int fv_m5_dev_begin_step(fv_m5_dev *c);
+/* one stage irk (1..4, the m5_RK cases) after fv_m5_dev_begin_step */
+int fv_m5_dev_stage(fv_m5_dev *c, int irk);
+/* nsteps steps of stages 1..rk_steps (loop.c:21) */
+int fv_m5_dev_advance(fv_m5_dev *c, int nsteps);
+/* the stage's pieces alone: face states of the current S (m5_faceinterp_V1),
+ * and the residual of S and those face states (m5_residual) */
+int fv_m5_dev_faceinterp(fv_m5_dev *c);
+int fv_m5_dev_residual(fv_m5_dev *c);
+/* the last face states SF [n][6][NEQ], residual RHS [n][NEQ] and GFM flags
+ * gfm_face [n][6] (LEAF.SF, LEAF.RHS, LEAF.gfm_face); any may be NULL */
+int fv_m5_dev_get_stage_arrays(fv_m5_dev *c, double *SF, double *RHS, uint8_t *gfm_face);
+/* Measurement utility with no counterpart in the reference: runs nsteps
+ * steps and returns the mean milliseconds per stage and per residual kernel
+ * launch (hipEvents on the context's stream). */
+int fv_m5_dev_time_steps(fv_m5_dev *c, int nsteps, double *ms_stage, double *ms_residual);
 
 #ifdef __cplusplus
 }

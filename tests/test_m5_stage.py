@@ -479,3 +479,169 @@ def test_m5_advance_runs_steps_of_stages():
     for irk in (1, 2, 3):
         Sx, _, _, _ = om.m5_stage(p, irk, SN, Sx)
     assert np.array_equal(S2, Sx)
+
+
+# ---------------------------------------------------------------- device parity (GPU)
+
+# (table, nmat, prelax, bitwise): without a solid and without PRELAX 3 no pow is evaluated, so the
+# device and the restatement agree bit for bit; the elastic EOS and PRELAX 3 go through pow, where
+# the device and glibc may differ in the last bit
+DEVICE_CASES = {
+    "fluids2_prelax1": (FLUIDS, 2, 1, True),
+    "fluids3_prelax0": (FLUIDS, 3, 0, True),
+    "fluids3_prelax2": (FLUIDS, 3, 2, True),
+    "solid3_prelax3": (SOLID_WATER_AIR, 3, 3, False),
+    "solid2_prelax0": (SOLID_WATER_AIR, 2, 0, False),
+}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def assert_close(got, want, bitwise, what, rel=1e-11):
+    if bitwise:
+        assert np.array_equal(got, want), f"{what}: max diff {np.abs(got - want).max()}"
+        return
+    g = got.reshape(got.shape[0], -1)
+    w = want.reshape(want.shape[0], -1)
+    scale = np.abs(w).max(axis=0) + 1e-300
+    err = (np.abs(g - w).max(axis=0) / scale).max()
+    assert err <= rel, f"{what}: relative error {err}"
+
+
+def rel_l1(got, want):
+    num = np.abs(got - want).sum(axis=0)
+    den = np.abs(want).sum(axis=0) + 1e-300
+    return (num / den).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("amr", [False, True])
+@pytest.mark.parametrize("name", list(DEVICE_CASES))
+def test_m5_device_face_states_and_residual_match_oracle(name, amr):
+    table, nmat, prelax, bitwise = DEVICE_CASES[name]
+    p, m, om, S = m5_case(100 + nmat + prelax, amr=amr, nmat=nmat, table=table, prelax=prelax)
+    say(f"[m5 dev] {name} amr={amr} n={m.n}")
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    dev.faceinterp()
+    dev.residual()
+    SF, RHS, gf = dev.stage_arrays()
+    SFo = om.m5_faceinterp(p, S)
+    assert_close(SF, SFo, bitwise, "face states")
+    RHSo, gfo = om.m5_residual(p, S, SFo)
+    assert np.array_equal(gf, gfo)
+    assert gfo.any()
+    assert_close(RHS, RHSo, bitwise, "residual")
+    dev.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(DEVICE_CASES))
+def test_m5_device_stages_match_oracle(name):
+    table, nmat, prelax, bitwise = DEVICE_CASES[name]
+    p, m, om, S = m5_case(110 + nmat + prelax, nmat=nmat, table=table, prelax=prelax)
+    p.dt = 2e-9
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    dev.begin_step()
+    SN = S.copy()
+    So = S.copy()
+    for irk in (1, 2, 3):
+        say(f"[m5 dev stage] {name} n={m.n} irk={irk}")
+        dev.stage(irk)
+        So, SFo, RHSo, gfo = om.m5_stage(p, irk, SN, So)
+        _, RHS, gf = dev.stage_arrays()
+        assert np.array_equal(gf, gfo)
+        assert_close(RHS, RHSo, bitwise, f"residual of stage {irk}", rel=1e-10)
+        Sd = dev.get_state()
+        if bitwise:
+            assert np.array_equal(Sd, So), f"stage {irk}"
+        else:
+            assert rel_l1(Sd, So) <= 1e-12, f"stage {irk}: {rel_l1(Sd, So)}"
+            So = Sd.copy()         # continue both from the same state
+            dev.set_state(So)
+    dev.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["fluids3_prelax0", "fluids3_prelax2", "solid3_prelax3"])
+def test_m5_device_steps_match_oracle(name):
+    table, nmat, prelax, bitwise = DEVICE_CASES[name]
+    p, m, om, S = m5_case(120 + nmat + prelax, nmat=nmat, table=table, prelax=prelax)
+    p.dt = 2e-9
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    dev.advance(3)
+    Sd = dev.get_state()
+    So = om.m5_advance(p, S, 3)
+    say(f"[m5 dev steps] {name} n={m.n} rel L1 {rel_l1(Sd, So):.2e}")
+    if bitwise:
+        assert np.array_equal(Sd, So)
+    else:
+        assert rel_l1(Sd, So) <= 1e-10
+    dev.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bc", [(1, 2, 5, 4, 1, 5), (2, 1, 1, 5, 4, 2)])
+def test_m5_device_boundary_codes_match_oracle(bc):
+    """Symmetry faces average the mirrored cell fluxes (m5_hllc.c:66-88), walls force u* = 0 or
+    mirror all velocity components, outlets copy the face state."""
+    p, m, om, S = m5_case(130 + bc[0], nmat=3, table=FLUIDS, prelax=0, bc=bc)
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    dev.faceinterp()
+    dev.residual()
+    SF, RHS, gf = dev.stage_arrays()
+    SFo = om.m5_faceinterp(p, S)
+    RHSo, gfo = om.m5_residual(p, S, SFo)
+    say(f"[m5 dev bc] {bc} n={m.n}")
+    assert np.array_equal(SF, SFo) and np.array_equal(RHS, RHSo) and np.array_equal(gf, gfo)
+    dev.close()
+
+
+@pytest.mark.gpu
+def test_m5_device_first_order_matches_oracle():
+    p, m, om, S = m5_case(140, nmat=3, table=FLUIDS, prelax=0, order=1)
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    dev.residual()
+    _, RHS, gf = dev.stage_arrays()
+    RHSo, gfo = om.m5_residual(p, S, np.zeros((m.n, 6, F.m5_neq(3))))
+    assert np.array_equal(RHS, RHSo) and np.array_equal(gf, gfo)
+    dev.close()
+
+
+@pytest.mark.gpu
+def test_m5_device_and_oracle_fail_on_the_same_run():
+    """The literal model-5 update is anti-diffusive (+dt RHS / V with the outward flux) and its star
+    energy grows at rest; with the two-phase PRELAX 1 this case reaches NaN within three steps.
+    The reference would print and exit; the oracle and the device both report it."""
+    table, nmat, prelax, _ = DEVICE_CASES["fluids2_prelax1"]
+    p, m, om, S = m5_case(120 + nmat + prelax, nmat=nmat, table=table, prelax=prelax)
+    p.dt = 2e-9
+    with pytest.raises(RuntimeError):
+        om.m5_advance(p, S, 3)
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    with pytest.raises(F.FVError) as e:
+        dev.advance(3)
+    assert e.value.code == F.FV_ERR_NAN
+    dev.close()
+
+
+@pytest.mark.gpu
+def test_m5_device_reports_nan_as_a_status():
+    """A leaf with zero density gives 0/0 velocities; the reference prints and exits, the device
+    returns FV_ERR_NAN."""
+    p, m, om, S = m5_case(141, nmat=3, table=FLUIDS, prelax=0)
+    S[5, :3] = 0.0
+    S[5, 3:6] = 0.0
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    with pytest.raises(F.FVError) as e:
+        dev.advance(1)
+    assert e.value.code == F.FV_ERR_NAN
+    dev.close()
