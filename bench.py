@@ -414,12 +414,113 @@ def collapse_leg(p, layers=64, log_fn=log):
     }
 
 
+# config 2 materials (solid slab, water, air) for the model-5 stage
+C2_TABLE = dict(gamma=(3.0, 6.12, 1.4), pinf=(2.0e10, 3.43e8, 0.0), mush=(26e9, 0.0, 0.0))
+C2_RHO = np.array([2700.0, 1000.0, 1.2])
+
+
+def c2_state(p, m, L=L_DOMAIN):
+    """Config 2 initial data on mesh m: a solid slab filling the bottom 25 % of the cube, an air
+    bubble of radius 0.4 mm whose centre stands 1.2 radii above the slab, water at 100 bar, air at
+    1 bar, both interfaces smeared with tanh over the cell size; A = I (undeformed solid)."""
+    c = m.centres()
+    n = m.n
+    h = L / (1 << (int(m.arrays()["level"].max()) - 1))
+    z_slab = 0.25 * L
+    centre = np.array([0.5 * L, 0.5 * L, z_slab + 1.2 * R_BUBBLE])
+    a_s = 0.5 * (1.0 - np.tanh((c[:, 2] - z_slab) / h))
+    bub = 0.5 * (1.0 - np.tanh((np.linalg.norm(c - centre, axis=1) - R_BUBBLE) / h))
+    a_a = bub * (1.0 - a_s)
+    alpha = np.stack([a_s, 1.0 - a_s - a_a, a_a], axis=1)
+    alpha = np.clip(alpha, 1e-6, None)
+    alpha /= alpha.sum(axis=1, keepdims=True)
+    pres = 1e5 + (1e7 - 1e5) * (1.0 - bub)
+    # a seeded smooth velocity of about 1 m/s: at rest the star velocity of every face is zero up to
+    # rounding, where the published star energy (denominator u* + S_K) changes sign with u*
+    rng = np.random.default_rng(1234)
+    k = rng.uniform(1.0, 3.0, size=(3, 3)) * (2.0 * np.pi / L)
+    ph = rng.uniform(0.0, 2.0 * np.pi, size=3)
+    u = np.stack([np.sin(c @ k[q] + ph[q]) for q in range(3)], axis=1)
+    g, pi = np.array(p.gamma[:3]), np.array(p.pinf[:3])
+    nq = F.m5_neq(3)
+    S = np.zeros((n, nq))
+    ra = alpha * C2_RHO
+    are = alpha * (pres[:, None] + g * pi) / (g - 1.0)
+    r = ra.sum(axis=1)
+    S[:, 0:3] = ra
+    S[:, 3:6] = r[:, None] * u
+    S[:, 6] = are.sum(axis=1) + 0.5 * r * (u * u).sum(axis=1)
+    S[:, 7:9] = alpha[:, :2]
+    S[:, 9:12] = are
+    S[:, 12:] = np.eye(3).ravel()
+    return S
+
+
+def m5_leg(level=8, steps=3, check=True, log_fn=log):
+    """The whole model-5 stage (MUSCL, HLLC/GFM residual with A transport, m5_RK, PRELAX 3
+    relaxation and the interface re-projections) with config 2's materials on a uniform
+    128^3 grid, with the oracle timed on 32^3 and the device checked against it there."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    out = {}
+    for lvl, what in ((level, "device"), (6, "check")) if check else ((level, "device"),):
+        p = F.make_m5_params(nmat=3, gamma=C2_TABLE["gamma"], pinf=C2_TABLE["pinf"], mush=C2_TABLE["mush"],
+                             amin=1e-6, avf_lim=0.5, prelax=3, order=2, rk_steps=3)
+        m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), L_DOMAIN, lvl, [4] * 6)
+        h = L_DOMAIN / (1 << (lvl - 1))
+        # 1 % of the CFL step on the solid sound speed: the published model-5 update is not stable
+        # on this state (see below) and reaches NaN within one step at the CFL step itself
+        p.dt = 0.01 * 0.4 * h / np.sqrt(3.0 * (1e7 + 2.0e10) / 2700.0)
+        S = c2_state(p, m)
+        dev = F.M5Device(p, m)
+        dev.set_state(S)
+        if what == "device":
+            # The published model-5 update reaches NaN in its second step on this state (the GFM
+            # faces flux rho u u - sigma, the others rho u u + sigma, and the unlimited alpha-rho-e
+            # face states put pressures of order gamma pinf into sigma at the slab), so each timed
+            # step starts again from the initial state.
+            dev.time_steps(1)
+            ms = []
+            for _ in range(steps):
+                dev.set_state(S)
+                ms.append(dev.time_steps(1))
+            ms_stage = float(np.mean([x[0] for x in ms]))
+            ms_res = float(np.mean([x[1] for x in ms]))
+            Sg = dev.get_state()
+            out.update({"leaves": m.n, "ms_per_stage": round(ms_stage, 3), "ms_residual_kernel": round(ms_res, 3),
+                        "value": round(m.n / (ms_stage * 1e-3) / 1e6, 2), "finite": bool(np.all(np.isfinite(Sg)))})
+            B = algorithmic_bytes(F.m5_neq(3))
+            out["roofline"] = {"bound": "hbm", "algorithmic_bytes_per_leaf": B,
+                               "achieved": round(B * m.n / (ms_stage * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS}
+            out["roofline"]["frac"] = round(out["roofline"]["achieved"] / HBM_PEAK_GBPS, 4)
+            log_fn(f"[m5] {m.n} leaves: {ms_stage:.3f} ms/stage, residual {ms_res:.3f} ms")
+        else:
+            dev.advance(1)
+            Sg = dev.get_state()
+            om = O.OracleMesh(m)
+            t0 = time.perf_counter()
+            So = om.m5_advance(p, S, 1)
+            t_cpu = time.perf_counter() - t0
+            num = np.abs(Sg - So).sum(axis=0)
+            den = np.abs(So).sum(axis=0) + 1e-300
+            out["rel_l1_vs_oracle_1step_32^3"] = float((num / den).max())
+            out["cpu_value_1core"] = round(m.n * 3 / t_cpu / 1e6, 4)
+            out["cpu_sample"] = f"32^3 config-2 state, 1 step (3 stages), {t_cpu:.1f} s, 1 thread, gcc -O2"
+        dev.close()
+    out["workload"] = ("config-2 materials (solid slab, water, air bubble), uniform 128^3, PRELAX 3, SSP-RK3, "
+                       "dt = 1 % of the CFL-0.4 step on the solid sound speed, each timed step from the initial state")
+    if check:
+        out["speedup_vs_cpu_1core"] = round(out["value"] / out["cpu_value_1core"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1", "u64", "u128", "u256", "u320"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "u64", "u128", "u256", "u320", "c2"],
+                    help="c2: the model-5 stage alone (config-2 materials, uniform 128^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true")
     ap.add_argument("--layers", type=int, default=64)
@@ -428,11 +529,22 @@ def main():
     ap.add_argument("--no-mlp", action="store_true", help="skip the FP32 + learned-indicator leg")
     ap.add_argument("--no-regrid", action="store_true", help="skip the device / host regrid leg")
     ap.add_argument("--no-collapse", action="store_true", help="skip the run to the first collapse")
+    ap.add_argument("--no-m5", action="store_true", help="skip the model-5 (solid, GFM) stage leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo design is "
               "described in DESIGN.md but not built", file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "c2":
+        leg = m5_leg(check=not a.no_cpu_baseline)
+        print(json.dumps({"metric": "Mcell-updates/s per RK stage (model 5: solid, GFM)", "value": leg["value"],
+                          "unit": "Mcell-updates/s per RK stage", "n_gpus": 1, "steps": 3 * 3, "warmup": 3,
+                          "ms_per_step": leg["ms_per_stage"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": leg["workload"], "leaves": leg["leaves"]}, "m5_stage": leg}),
+              flush=True)
+        return
 
     t_setup = time.perf_counter()
     if a.workload == "c1":
@@ -497,6 +609,8 @@ def main():
         out["regrid"] = regrid_leg(p, m, S, c1_crit(9, a.layers))
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_collapse:
         out["collapse"] = collapse_leg(p, a.layers)
+    if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_m5:
+        out["m5_stage"] = m5_leg()
     if not a.no_cpu_baseline:
         cb, l1, stages = cpu_baseline()
         out["cpu_baseline"] = cb

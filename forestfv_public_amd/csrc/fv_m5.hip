@@ -920,37 +920,186 @@ __device__ double dfpe5(const M5P &P, const double *ar, const double *t0, const 
   return D;
 }
 
+// m5_relaxationvec.c in pieces.  RelaxIn holds what every branch reads
+// (m5_relaxationvec.c:29-87); PRELAX 3's Newton iteration (:365-560) can be
+// stopped after a number of iterations and resumed from (pr, F, dF, iter) with
+// the same arithmetic, so the few slow states can be finished apart from the rest.
+template <int NM>
+struct RelaxIn {
+  double ar[NM], are[NM], avf[NM], avf_init[NM], A[9], G[9], Gmag, rho, u, v, w, rE;
+};
+
+template <int NM>
+__device__ void relax_in(const M5P &P, const double *x, RelaxIn<NM> &R) {
+  using L = L5<NM>;
+  R.rho = 0.0;
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) { R.ar[iv] = x[L::I0 + iv]; R.are[iv] = x[L::I4 + iv]; R.rho += R.ar[iv]; }
+  R.u = x[L::I1 + 0] / R.rho; R.v = x[L::I1 + 1] / R.rho; R.w = x[L::I1 + 2] / R.rho;
+  R.rE = x[L::I2];
+#pragma unroll
+  for (int iv = 0; iv < NM - 1; ++iv) R.avf[iv] = x[L::I3 + iv];
+  vf_limit1<NM>(P.amin / 100.0, R.avf);
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) R.avf_init[iv] = R.avf[iv];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R.A[k] = x[L::I5 + k];
+  gmat(R.A, R.G, R.Gmag);
+}
+
+// the re-initialisation after PRELAX 1, 2, 3 (m5_relaxationvec.c:157-189 and
+// the same in the other branches) and the write-back (:580-620)
+template <int NM>
+__device__ int relax_out(const M5P &P, const RelaxIn<NM> &R, double *are, double *x) {
+  using L = L5<NM>;
+  if (P.prelax >= 1 && P.prelax <= 3) {
+    double e_elast = 0.0, are_el[NM];
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv) {
+      are_el[iv] = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, P.mush[iv]) : 0.0;
+      const double e_elastic_K = are_el[iv] / (R.ar[iv]);
+      const double Ys = R.ar[iv] / R.rho;
+      e_elast += Ys * e_elastic_K;
+    }
+    const double e_hyd = R.rE / R.rho - 0.5 * (sq((R.u)) + sq((R.v)) + sq((R.w))) - e_elast;
+    double num = 0.0, den = 0.0;
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv) {
+      num += R.avf[iv] * P.gamma[iv] * P.pinf[iv] / (P.gamma[iv] - 1.0);
+      den += R.avf[iv] / (P.gamma[iv] - 1.0);
+    }
+    const double p_relaxed = (R.rho * e_hyd - num) / den;
+    if (isnan(p_relaxed)) return -1;
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv)
+      are[iv] = (R.avf[iv] * (p_relaxed + P.gamma[iv] * P.pinf[iv]) / (P.gamma[iv] - 1.0)) + are_el[iv];
+  }
+  bool bad = false;
+#pragma unroll
+  for (int iv = 0; iv < NM - 1; ++iv) { bad |= isnan(R.avf[iv]); x[L::I3 + iv] = R.avf[iv]; }
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) { bad |= isnan(are[iv]); x[L::I4 + iv] = are[iv]; }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) x[L::I5 + k] = R.A[k];
+  return bad ? -1 : 0;
+}
+
+// PRELAX 3 set-up (m5_relaxationvec.c:365-400): clamps, the phasic pressures
+// pk and the starting pressure; returns irelax
+template <int NM>
+__device__ int r3_setup(const M5P &P, RelaxIn<NM> &R, double *pk, double &pr) {
+  const double eps_sml = 1.0e-12, eps_big = 1.0e-8;
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) {
+    if (R.avf[iv] < 0.0 || R.ar[iv] < 0.0) { R.avf[iv] = eps_sml; R.ar[iv] = eps_sml; R.are[iv] = eps_sml; }
+    if (R.avf[iv] >= 1.0) R.avf[iv] = 1.0 - eps_sml;
+  }
+  int irelax = 1;
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv)
+    if (R.avf[iv] > 1.0 - eps_sml) irelax = 0;
+  pr = 0.0;
+  if (irelax == 1) {
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv) {
+      if (R.avf[iv] > eps_sml) {
+        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, P.mush[iv]) : 0.0;
+        pk[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
+        if (pk[iv] <= -(1.0 - eps_big) * P.pinf[iv] + eps_big) pk[iv] = -(1.0 - eps_big) * P.pinf[iv] + eps_big;
+      } else {
+        pk[iv] = 0.0;
+      }
+      pr += R.avf[iv] * pk[iv];
+    }
+  }
+  return irelax;
+}
+
+template <int NM>
+__device__ double r3_rhoK(const M5P &P, const RelaxIn<NM> &R, const double *pk, double pr, int iv) {
+  return R.ar[iv] / maxr(R.avf[iv], 1.0e-12) * pow((pr + P.pinf[iv]) / (pk[iv] + P.pinf[iv]), 1.0 / P.gamma[iv]);
+}
+
+// at most cap iterations of the Newton loop (m5_relaxationvec.c:409-470)
+// from (pr, F, dF, iter): 1 when the loop ended, 0 when cap was reached first,
+// -1 on NaN
+template <int NM>
+__device__ int r3_newton(const M5P &P, const RelaxIn<NM> &R, const double *pk, double &pr, double &Fv, double &dF,
+                         int &iter, int cap) {
+  const double eps_sml = 1.0e-12, eps_big = 1.0e-8;
+  for (int k = 0; k < cap; k++) {
+    if (!((fabs(Fv) > 1.0e-12) && (iter < 500))) return 1;
+    const double pr0 = pr, F0 = Fv, dF0 = dF;
+    pr = pr - Fv / dF;
+    iter = iter + 1;
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv)
+      if (pr < -(1.0 - eps_big) * P.pinf[iv] + eps_big) pr = -(1.0 - eps_big) * P.pinf[iv] + 1.0;
+    Fv = -1.0;
+    dF = 0.0;
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv) {
+      if (R.avf[iv] > eps_sml) {
+        const double rk = r3_rhoK<NM>(P, R, pk, pr, iv);
+        Fv = Fv + R.ar[iv] / rk;
+        dF = dF - R.ar[iv] / (P.gamma[iv] * rk * (pr + P.pinf[iv]));
+      }
+    }
+    if (isnan(pr) || isnan(Fv) || isnan(dF)) return -1;
+    // an iteration that reproduces its own input (the clamp holding pr at the same
+    // value) repeats itself up to the limit of 500: end there with the same values
+    if (pr == pr0 && Fv == F0 && dF == dF0) { iter = 500; return 1; }
+  }
+  return ((fabs(Fv) > 1.0e-12) && (iter < 500)) ? 0 : 1;
+}
+
+// after the Newton loop (m5_relaxationvec.c:472-578)
+template <int NM>
+__device__ int r3_finish(const M5P &P, RelaxIn<NM> &R, const double *pk, int irelax, double pr, double *x) {
+  if (irelax == 1) {
+#pragma unroll
+    for (int iv = 0; iv < NM; ++iv)
+      if (R.avf[iv] > 1.0e-12) R.avf[iv] = R.ar[iv] / r3_rhoK<NM>(P, R, pk, pr, iv);
+    vf_limit1<NM>(P.amin / 100.0, R.avf);
+  }
+  const int i_as = ((R.avf[0] < R.avf[1]) && (P.mush[1] != 0.0)) ? 1 : 0;
+  const double ratio = (i_as == 1) ? (R.avf[1] / R.avf_init[1]) : (R.avf[0] / R.avf_init[0]);
+#pragma unroll
+  for (int i = 0; i < 9; i++) R.A[i] = R.A[i] * ratio;
+  double are[NM];
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) are[iv] = R.are[iv];
+  return relax_out<NM>(P, R, are, x);
+}
+
 // m5_relaxationvec.c:3-620 on x in place: 0, -1 NaN, -3 PRELAX 2 Newton failure
 template <int NM>
 __device__ int relax5(const M5P &P, double *x) {
-  using L = L5<NM>;
-  const double a = P.amin / 100.0, eps_sml = 1.0e-12, eps_big = 1.0e-8;
-  double ar[NM], are[NM], avf[NM], avf_init[NM], are_el[NM], A[9], G[9], Gmag;
-  double rho = 0.0;
+  const double a = P.amin / 100.0, eps_big = 1.0e-8;
+  RelaxIn<NM> R;
+  relax_in<NM>(P, x, R);
+  if (P.prelax == 3) {
+    double pk[NM], pr, Fv = 1.0e-9, dF = 1.0e9;
+    int iter = 0;
+    const int irelax = r3_setup<NM>(P, R, pk, pr);
+    if (irelax == 1 && r3_newton<NM>(P, R, pk, pr, Fv, dF, iter, 1 << 30) < 0) return -1;
+    return r3_finish<NM>(P, R, pk, irelax, pr, x);
+  }
+  double are[NM];
 #pragma unroll
-  for (int iv = 0; iv < NM; ++iv) { ar[iv] = x[L::I0 + iv]; are[iv] = x[L::I4 + iv]; rho += ar[iv]; are_el[iv] = 0.0; }
-  const double u = x[L::I1 + 0] / rho, v = x[L::I1 + 1] / rho, w = x[L::I1 + 2] / rho;
-  const double rE = x[L::I2];
-#pragma unroll
-  for (int iv = 0; iv < NM - 1; ++iv) avf[iv] = x[L::I3 + iv];
-  vf_limit1<NM>(a, avf);
-#pragma unroll
-  for (int iv = 0; iv < NM; ++iv) avf_init[iv] = avf[iv];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) A[k] = x[L::I5 + k];
-  gmat(A, G, Gmag);
+  for (int iv = 0; iv < NM; ++iv) are[iv] = R.are[iv];
   if (P.prelax == 1 || P.prelax == 2) {
     double t0[NM], p0[NM], pe;
     if (P.prelax == 1) {   // m5_relaxationvec.c:89-190
 #pragma unroll
       for (int iv = 0; iv < NM; ++iv) {
-        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[iv]) : 0.0;
-        t0[iv] = 1.0 / (ar[iv] / avf[iv]);
-        p0[iv] = ((P.gamma[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - P.gamma[iv] * P.pinf[iv];
+        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, P.mush[iv]) : 0.0;
+        t0[iv] = 1.0 / (R.ar[iv] / R.avf[iv]);
+        p0[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
       }
       const double G0 = P.gamma[0], G1 = P.gamma[1], P0 = P.pinf[0], P1 = P.pinf[1];
-      const double A_1 = (avf[0] / G0) * (p0[0] + P0) / ((avf[0] / G0) + (avf[1] / G1));
-      const double A_2 = (avf[1] / G1) * (p0[1] + P1) / ((avf[0] / G0) + (avf[1] / G1));
+      const double A_1 = (R.avf[0] / G0) * (p0[0] + P0) / ((R.avf[0] / G0) + (R.avf[1] / G1));
+      const double A_2 = (R.avf[1] / G1) * (p0[1] + P1) / ((R.avf[0] / G0) + (R.avf[1] / G1));
       pe = 0.5 * (A_1 + A_2 - (P0 + P1)) + sqrt(0.25 * (sq((A_2 - A_1 - (P1 - P0)))) + A_1 * A_2);
       if (isnan(pe)) return -1;
     } else {               // m5_relaxationvec.c:192-363
@@ -958,20 +1107,20 @@ __device__ int relax5(const M5P &P, double *x) {
       int principal = 0;
 #pragma unroll
       for (int iv = 0; iv < NM; ++iv) {
-        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[iv]) : 0.0;
-        t0[iv] = 1.0 / (ar[iv] / avf[iv]);
-        p0[iv] = ((P.gamma[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - P.gamma[iv] * P.pinf[iv];
+        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, P.mush[iv]) : 0.0;
+        t0[iv] = 1.0 / (R.ar[iv] / R.avf[iv]);
+        p0[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
         if (p0[iv] < -(1.0 - eps_big) * P.pinf[iv] + eps_big) p0[iv] = -(1.0 - eps_big) * P.pinf[iv] + 1.0;
-        if (avf[iv] > max_avf) { max_avf = avf[iv]; principal = iv; }
+        if (R.avf[iv] > max_avf) { max_avf = R.avf[iv]; principal = iv; }
       }
       double pe_old = 0.001, pe_new = pe_old;
       int iter = 0;
       double cond_1 = fabs(pe_new - pe_old);
-      double cond_2 = fabs(fpe5<NM>(P, ar, t0, p0, pe_new));
+      double cond_2 = fabs(fpe5<NM>(P, R.ar, t0, p0, pe_new));
       while (((cond_2 >= 1.0e-12) || (cond_1 >= 1.0e-4)) && (iter <= 101)) {
-        pe_new = pe_old - 0.8 * (fpe5<NM>(P, ar, t0, p0, pe_old) / dfpe5<NM>(P, ar, t0, p0, pe_old));
+        pe_new = pe_old - 0.8 * (fpe5<NM>(P, R.ar, t0, p0, pe_old) / dfpe5<NM>(P, R.ar, t0, p0, pe_old));
         cond_1 = fabs(pe_new - pe_old);
-        cond_2 = fabs(fpe5<NM>(P, ar, t0, p0, pe_new));
+        cond_2 = fabs(fpe5<NM>(P, R.ar, t0, p0, pe_new));
         if (iter >= 100 && ((fabs(pe_old - pe_new) < 1.0e3) || isnan(pe_new))) return -3;
         pe_old = pe_new;
         iter++;
@@ -989,97 +1138,102 @@ __device__ int relax5(const M5P &P, double *x) {
       const double Gk = P.gamma[iv], Pk = P.pinf[iv];
       const double t1 = t0[iv] * (p0[iv] + Gk * Pk + (Gk - 1.0) * pe) / (pe + Gk * Pk + (Gk - 1.0) * pe);
       const double r_m = 1.0 / t1;
-      avf[iv] = ar[iv] / r_m;
+      R.avf[iv] = R.ar[iv] / r_m;
     }
-    vf_limit2<NM>(a, avf);
+    vf_limit2<NM>(a, R.avf);
 #pragma unroll
-    for (int i = 0; i < 9; i++) A[i] = A[i] * (avf[0] / avf_init[0]);
-  } else if (P.prelax == 3) {   // m5_relaxationvec.c:365-560
-    double rhoKs[NM], pk[NM];
-#pragma unroll
-    for (int iv = 0; iv < NM; ++iv) {
-      rhoKs[iv] = 0.0;
-      if (avf[iv] < 0.0 || ar[iv] < 0.0) { avf[iv] = eps_sml; ar[iv] = eps_sml; are[iv] = eps_sml; }
-      if (avf[iv] >= 1.0) avf[iv] = 1.0 - eps_sml;
-    }
-    int irelax = 1;
-#pragma unroll
-    for (int iv = 0; iv < NM; ++iv)
-      if (avf[iv] > 1.0 - eps_sml) irelax = 0;
-    if (irelax == 1) {
-      double pr = 0.0;
-#pragma unroll
-      for (int iv = 0; iv < NM; ++iv) {
-        if (avf[iv] > eps_sml) {
-          const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[iv]) : 0.0;
-          pk[iv] = ((P.gamma[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - P.gamma[iv] * P.pinf[iv];
-          if (pk[iv] <= -(1.0 - eps_big) * P.pinf[iv] + eps_big) pk[iv] = -(1.0 - eps_big) * P.pinf[iv] + eps_big;
-        } else {
-          pk[iv] = 0.0;
-        }
-        pr += avf[iv] * pk[iv];
-      }
-      int iter = 0;
-      double Fv = 1.0e-9, dF = 1.0e9;
-      while ((fabs(Fv) > 1.0e-12) && (iter < 500)) {
-        pr = pr - Fv / dF;
-        iter = iter + 1;
-#pragma unroll
-        for (int iv = 0; iv < NM; ++iv)
-          if (pr < -(1.0 - eps_big) * P.pinf[iv] + eps_big) pr = -(1.0 - eps_big) * P.pinf[iv] + 1.0;
-        Fv = -1.0;
-        dF = 0.0;
-#pragma unroll
-        for (int iv = 0; iv < NM; ++iv) {
-          if (avf[iv] > eps_sml) {
-            rhoKs[iv] = ar[iv] / maxr(avf[iv], eps_sml) * pow((pr + P.pinf[iv]) / (pk[iv] + P.pinf[iv]), 1.0 / P.gamma[iv]);
-            Fv = Fv + ar[iv] / rhoKs[iv];
-            dF = dF - ar[iv] / (P.gamma[iv] * rhoKs[iv] * (pr + P.pinf[iv]));
-          }
-        }
-        if (isnan(pr) || isnan(Fv) || isnan(dF)) return -1;
-      }
-#pragma unroll
-      for (int iv = 0; iv < NM; ++iv)
-        if (avf[iv] > eps_sml) avf[iv] = ar[iv] / rhoKs[iv];
-      vf_limit1<NM>(a, avf);
-    }
-    const int i_as = ((avf[0] < avf[1]) && (P.mush[1] != 0.0)) ? 1 : 0;
-    const double ratio = (i_as == 1) ? (avf[1] / avf_init[1]) : (avf[0] / avf_init[0]);
-#pragma unroll
-    for (int i = 0; i < 9; i++) A[i] = A[i] * ratio;
+    for (int i = 0; i < 9; i++) R.A[i] = R.A[i] * (R.avf[0] / R.avf_init[0]);
   }
-  if (P.prelax >= 1 && P.prelax <= 3) {
-    // re-initialisation: the pressure from the total energy
-    double e_elast = 0.0;
+  return relax_out<NM>(P, R, are, x);
+}
+
+// PRELAX 3 over many states (face states or cell states) with the slow ones
+// finished apart: state q of `count` is element v at X[(q / n) * NEQ * n + v * n + q % n]
+// (SF [6][NEQ][n] for face states, S [NEQ][n] for cell states).  The start
+// pass gives every state R3_CAP iterations; a state still iterating keeps its
+// unrelaxed vector in X, saves (pr, F, dF, iter) and is marked; resume
+// passes continue the marked ones from there.  Sb, when given, receives the
+// state before relaxation (m5_relaxation_solid.c:18-24).
+constexpr int R3_CAP = 8;
+
+template <int NM>
+__device__ void load_state(const double *X, int n, long q, double *x) {
+  using L = L5<NM>;
+  const long f = q / n, i = q - f * n;
 #pragma unroll
-    for (int iv = 0; iv < NM; ++iv) {
-      are_el[iv] = (P.mush[iv] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[iv]) : 0.0;
-      const double e_elastic_K = are_el[iv] / (ar[iv]);
-      const double Ys = ar[iv] / rho;
-      e_elast += Ys * e_elastic_K;
-    }
-    const double e_hyd = rE / rho - 0.5 * (sq((u)) + sq((v)) + sq((w))) - e_elast;
-    double num = 0.0, den = 0.0;
+  for (int v = 0; v < L::NEQ; v++) x[v] = X[(size_t)f * L::NEQ * n + (size_t)v * n + i];
+}
+template <int NM>
+__device__ void store_state(double *X, int n, long q, const double *x) {
+  using L = L5<NM>;
+  const long f = q / n, i = q - f * n;
 #pragma unroll
-    for (int iv = 0; iv < NM; ++iv) {
-      num += avf[iv] * P.gamma[iv] * P.pinf[iv] / (P.gamma[iv] - 1.0);
-      den += avf[iv] / (P.gamma[iv] - 1.0);
-    }
-    const double p_relaxed = (rho * e_hyd - num) / den;
-    if (isnan(p_relaxed)) return -1;
-#pragma unroll
-    for (int iv = 0; iv < NM; ++iv)
-      are[iv] = (avf[iv] * (p_relaxed + P.gamma[iv] * P.pinf[iv]) / (P.gamma[iv] - 1.0)) + are_el[iv];
+  for (int v = 0; v < L::NEQ; v++) X[(size_t)f * L::NEQ * n + (size_t)v * n + i] = x[v];
+}
+
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_relax3_start(long count, int n, M5P P, double *__restrict__ X,
+                                                         double *__restrict__ Sb, double *__restrict__ nst,
+                                                         int *__restrict__ iters, uint8_t *__restrict__ mark,
+                                                         int *__restrict__ flag) {
+  using L = L5<NM>;
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  double x[L::NEQ];
+  load_state<NM>(X, n, q, x);
+  if (Sb) store_state<NM>(Sb, n, q, x);
+  RelaxIn<NM> R;
+  relax_in<NM>(P, x, R);
+  double pk[NM], pr, Fv = 1.0e-9, dF = 1.0e9;
+  int iter = 0;
+  const int irelax = r3_setup<NM>(P, R, pk, pr);
+  int rc = 1;
+  if (irelax == 1) rc = r3_newton<NM>(P, R, pk, pr, Fv, dF, iter, R3_CAP);
+  if (rc < 0) { atomicOr(flag, 1); mark[q] = 0; return; }
+  if (rc == 0) {
+    nst[3 * q + 0] = pr; nst[3 * q + 1] = Fv; nst[3 * q + 2] = dF;
+    iters[q] = iter;
+    mark[q] = 1;
+    return;
   }
-  bool bad = false;
-#pragma unroll
-  for (int iv = 0; iv < NM - 1; ++iv) { bad |= isnan(avf[iv]); x[L::I3 + iv] = avf[iv]; }
-#pragma unroll
-  for (int iv = 0; iv < NM; ++iv) { bad |= isnan(are[iv]); x[L::I4 + iv] = are[iv]; }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) x[L::I5 + k] = A[k];
-  return bad ? -1 : 0;
+  mark[q] = 0;
+  if (r3_finish<NM>(P, R, pk, irelax, pr, x) < 0) atomicOr(flag, 1);
+  store_state<NM>(X, n, q, x);
+}
+
+// resumes the listed states for at most cap more iterations; those still
+// iterating save their Newton state again and are marked (mark != nullptr)
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_relax3_resume(const int *__restrict__ nlist_p,
+                                                          const int *__restrict__ list, int n, M5P P,
+                                                          double *__restrict__ X, double *__restrict__ nst,
+                                                          int *__restrict__ iters, uint8_t *__restrict__ mark, int cap,
+                                                          int *__restrict__ flag) {
+  using L = L5<NM>;
+  const int nlist = *nlist_p;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist; t += gridDim.x * blockDim.x) {
+    const long q = list[t];
+    double x[L::NEQ];
+    load_state<NM>(X, n, q, x);
+    RelaxIn<NM> R;
+    relax_in<NM>(P, x, R);
+    double pk[NM], pr;
+    const int irelax = r3_setup<NM>(P, R, pk, pr);
+    pr = nst[3 * q + 0];
+    double Fv = nst[3 * q + 1], dF = nst[3 * q + 2];
+    int iter = iters[q];
+    const int rc = r3_newton<NM>(P, R, pk, pr, Fv, dF, iter, cap);
+    if (rc < 0) { atomicOr(flag, 1); if (mark) mark[q] = 0; continue; }
+    if (rc == 0) {
+      nst[3 * q + 0] = pr; nst[3 * q + 1] = Fv; nst[3 * q + 2] = dF;
+      iters[q] = iter;
+      if (mark) mark[q] = 1;
+      continue;
+    }
+    if (mark) mark[q] = 0;
+    if (r3_finish<NM>(P, R, pk, irelax, pr, x) < 0) atomicOr(flag, 1);
+    store_state<NM>(X, n, q, x);
+  }
 }
 
 // avf_limit.c:3-21 with max_limit 100 on the fractions of X
@@ -1157,6 +1311,7 @@ __device__ void dist5(const Geo &G, int kind, int lvl, double &dcf, double &dcc)
 // m5_faceinterp_V1 (m5_faceinterp.c:60-93) for one leaf per thread: MUSCL face
 // states with minmod (m5_reconctruct.c:93-190; g_limitedvars is 0 on the
 // alpha-rho-e block, m5_input.c:196-204), the VF limit and m5_relaxationvec
+// (PRELAX 3: left to k_m5_relax3_start / _finish)
 template <int NM>
 __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__restrict__ Gp,
                                                   const double *__restrict__ S, const int *__restrict__ nbr,
@@ -1215,7 +1370,8 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
     }
 #pragma unroll
     for (int k = 0; k < NM - 1; ++k) Fv[L::I3 + k] = avf[k];
-    const int rc = relax5<NM>(P, Fv);
+    // PRELAX 3 relaxes the face states in k_m5_relax3_start / _finish
+    const int rc = (P.prelax == 3) ? 0 : relax5<NM>(P, Fv);
     if (rc) atomicOr(flag, rc == -3 ? 4 : 1);
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) SF[((size_t)f * L::NEQ + v) * n + i] = Fv[v];
@@ -1620,24 +1776,51 @@ struct fv_m5_dev {
   int *list = nullptr, *count = nullptr;
   void *cub_tmp = nullptr;
   size_t cub_bytes = 0;
+  // PRELAX 3 over the 6n face states (and the n cell states): saved Newton state of the slow ones
+  double *nst = nullptr;      // [6n][3] pr, F, dF
+  int *iters = nullptr;       // [6n]
+  uint8_t *mark6 = nullptr;   // [6n]
+  int *list6 = nullptr;       // [6n]
 };
 
 namespace {
 
 void m5_free(fv_m5_dev *c) {
   void *ptrs[] = {c->G, c->nbr, c->fine4, c->info, c->S, c->S2, c->SN, c->SF, c->RHS, c->Sb,
-                  c->gfm, c->mark, c->flag, c->list, c->count, c->cub_tmp};
+                  c->gfm, c->mark, c->flag, c->list, c->count, c->cub_tmp, c->nst, c->iters, c->mark6, c->list6};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
 
-// leaves with mark != 0 in leaf order into c->list, their number into c->count (device side only)
-int m5_compact(fv_m5_dev *c) {
+// items with mark != 0, in order, into list, their number into c->count (device side only)
+int m5_compact(fv_m5_dev *c, const uint8_t *mark, int *list, int items) {
   hipcub::CountingInputIterator<int> idx(0);
   size_t bytes = c->cub_bytes;
-  HIPCHK(hipcub::DeviceSelect::Flagged(c->cub_tmp, bytes, idx, c->mark, c->list, c->count, c->n, c->stream));
+  HIPCHK(hipcub::DeviceSelect::Flagged(c->cub_tmp, bytes, idx, mark, list, c->count, items, c->stream));
+  return FV_OK;
+}
+
+// PRELAX 3 on `count` states of X (see k_m5_relax3_start)
+template <int NM>
+int m5_relax3(fv_m5_dev *c, double *X, long count, double *Sb) {
+  const dim3 g((unsigned)((count + 255) / 256)), b(256);
+  hipLaunchKernelGGL(k_m5_relax3_start<NM>, g, b, 0, c->stream, count, c->n, c->P, X, Sb, c->nst, c->iters, c->mark6,
+                     c->flag);
+  HIPCHK(hipGetLastError());
+  // the states still iterating: 24 more iterations for all of them, then the
+  // few that stagnate run to the reference's limit of 500 in full waves
+  const dim3 gr((unsigned)std::min<long>(4096, (count + 255) / 256));
+  int rc = m5_compact(c, c->mark6, c->list6, (int)count);
+  if (rc != FV_OK) return rc;
+  hipLaunchKernelGGL(k_m5_relax3_resume<NM>, gr, b, 0, c->stream, c->count, c->list6, c->n, c->P, X, c->nst,
+                     c->iters, c->mark6, 24, c->flag);
+  HIPCHK(hipGetLastError());
+  if ((rc = m5_compact(c, c->mark6, c->list6, (int)count)) != FV_OK) return rc;
+  hipLaunchKernelGGL(k_m5_relax3_resume<NM>, gr, b, 0, c->stream, c->count, c->list6, c->n, c->P, X, c->nst,
+                     c->iters, (uint8_t *)nullptr, 1 << 30, c->flag);
+  HIPCHK(hipGetLastError());
   return FV_OK;
 }
 
@@ -1648,6 +1831,7 @@ int m5_faceinterp_t(fv_m5_dev *c) {
   hipLaunchKernelGGL(k_m5_recon<NM>, grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S, c->nbr,
                      c->info, c->fine4, c->SF, c->flag);
   HIPCHK(hipGetLastError());
+  if (c->P.prelax == 3) return m5_relax3<NM>(c, c->SF, 6L * c->n, nullptr);
   return FV_OK;
 }
 
@@ -1674,18 +1858,22 @@ int m5_stage_t(fv_m5_dev *c, int irk, hipEvent_t *ev) {
                      c->mark);
   HIPCHK(hipGetLastError());
   if (irk == 1) {
-    if ((rc = m5_compact(c)) != FV_OK) return rc;
+    if ((rc = m5_compact(c, c->mark, c->list, c->n)) != FV_OK) return rc;
     hipLaunchKernelGGL(k_m5_gfm_seq<NM>, dim3(1), dim3(64), 0, c->stream, c->n, c->count, c->list, c->P, c->G,
                        c->nbr, c->info, c->fine4, c->gfm, c->SN, c->S, c->S2, c->RHS, c->flag + 1);
     HIPCHK(hipGetLastError());
   }
   std::swap(c->S, c->S2);
   // m5_relaxation_solid: relax every leaf, then re-project the leaves whose solid fraction crossed
-  hipLaunchKernelGGL(k_m5_relax<NM>, g, b, 0, c->stream, c->n, c->P, c->S, c->Sb, c->flag);
-  HIPCHK(hipGetLastError());
+  if (c->P.prelax == 3) {
+    if ((rc = m5_relax3<NM>(c, c->S, c->n, c->Sb)) != FV_OK) return rc;
+  } else {
+    hipLaunchKernelGGL(k_m5_relax<NM>, g, b, 0, c->stream, c->n, c->P, c->S, c->Sb, c->flag);
+    HIPCHK(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_m5_rel_mark<NM>, g, b, 0, c->stream, c->n, c->P, c->Sb, c->S, c->mark);
   HIPCHK(hipGetLastError());
-  if ((rc = m5_compact(c)) != FV_OK) return rc;
+  if ((rc = m5_compact(c, c->mark, c->list, c->n)) != FV_OK) return rc;
   hipLaunchKernelGGL(k_m5_rel_seq<NM>, dim3(1), dim3(64), 0, c->stream, c->n, c->count, c->list, c->P, c->G, c->nbr,
                      c->info, c->fine4, c->gfm, c->Sb, c->S, c->flag + 1);
   HIPCHK(hipGetLastError());
@@ -1739,9 +1927,15 @@ extern "C" int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_
   chk(dalloc(&c->SF, 6 * sz));
   chk(dalloc(&c->gfm, 6 * n)); chk(dalloc(&c->mark, n));
   chk(dalloc(&c->flag, 2)); chk(dalloc(&c->list, n)); chk(dalloc(&c->count, 1));
+  if (p->prelax == 3) {
+    chk(dalloc(&c->nst, 18 * n)); chk(dalloc(&c->iters, 6 * n));
+    chk(dalloc(&c->mark6, 6 * n)); chk(dalloc(&c->list6, 6 * n));
+  }
   if (e == hipSuccess) {
+    // temporary storage of the largest compaction (6n face states under PRELAX 3, else n leaves)
     hipcub::CountingInputIterator<int> idx(0);
-    chk(hipcub::DeviceSelect::Flagged(nullptr, c->cub_bytes, idx, c->mark, c->list, c->count, c->n, c->stream));
+    const int items = (p->prelax == 3) ? 6 * c->n : c->n;
+    chk(hipcub::DeviceSelect::Flagged(nullptr, c->cub_bytes, idx, c->mark, c->list, c->count, items, c->stream));
     chk(dalloc((char **)&c->cub_tmp, c->cub_bytes));
   }
   if (e == hipSuccess) {
