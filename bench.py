@@ -106,9 +106,10 @@ def build_uniform(level):
 
 def cpu_baseline(seconds=8.0):
     """The CPU restatement (oracle/, gcc -O2) timed on the host on whole RK stages:
-    single core on Sod 32^3, and OpenMP over all the threads the process may use on
-    Sod 64^3 (mirrored faces recomputed, bit-identical to the sequential run);
-    about `seconds` of work each.  Also checks the device against the CPU path."""
+    single core on Sod 32^3 (config 0), and OpenMP over all the threads the process
+    may use on Sod 128^3, 2.1 M leaves (mirrored faces recomputed, bit-identical to
+    the sequential run); about `seconds` of work each.  Also checks the device
+    against the CPU path."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from cases import sod_case
@@ -141,12 +142,13 @@ def cpu_baseline(seconds=8.0):
     den[den == 0] = 1
     l1 = float((num / den).max())
     nthr = int(O.lib().orc_num_threads())
-    p2, m2, S2 = sod_case(level=7, rk=F.RK_SSP2, prelax=0)
+    # the 2.1 M-leaf grid of the GPU runs (Sod 128^3)
+    p2, m2, S2 = sod_case(level=8, rk=F.RK_SSP2, prelax=0)
     _, nsteps2, t2 = timed(O.OracleMesh(m2), p2, S2, True, nthr)
     stages2 = 2 * nsteps2
     return {
         "value": m2.n * stages2 / t2 / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": nthr, "kind": "port",
-        "sample": f"Sod 64^3 (262144 leaves), SSP-RK2, {stages2} stages, {t2:.1f} s, gcc -O2 -ffp-contract=off, "
+        "sample": f"Sod 128^3 ({m2.n} leaves), SSP-RK2, {stages2} stages, {t2:.1f} s, gcc -O2 -ffp-contract=off, "
                   f"OpenMP {nthr} threads (os.cpu_count {os.cpu_count()})",
         "value_1core": m.n * stages1 / t1 / 1e6,
         "sample_1core": f"Sod 32^3 (32768 leaves), SSP-RK2, {stages1} stages, {t1:.1f} s, 1 thread",
