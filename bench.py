@@ -274,16 +274,15 @@ def fp32_mlp_leg(p, m, S, crit, cycles=4, every=5, log_fn=log):
     for leg, prec, par, learned in (("plain", F.PREC_FP64, p, False), ("learned", F.PREC_MIXED, q, True),
                                     ("fp64_learned", F.PREC_FP64, q, True)):
         dev = F.Device(par, m, prec=prec)
-        SI = np.zeros((m.n, 2), np.int32)
-        split = np.zeros(m.n, np.int32)
         for rep in range(2):               # the first pass warms up
             dev.set_state(S)
             t1 = time.perf_counter()
             for _ in range(cycles if rep else 1):
+                # SI and the targets stay on the device, as the device regrid reads them
                 if learned:
-                    SI, split, _ = dev.indicator_mlp(mlp, crit, smooth=False)
+                    dev.indicator_mlp(mlp, crit, smooth=False, to_host=False)
                 else:
-                    SI, split = dev.indicator(crit)
+                    dev.indicator(crit, to_host=False)
                 dev.advance(every)
             t = time.perf_counter() - t1
         out[leg] = (dev.get_state(), t)

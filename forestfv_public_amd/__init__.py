@@ -702,7 +702,11 @@ class Device:
         _check(lib().fv_dev_cfl_dt(self._h, cfl, C.byref(d)), "fv_dev_cfl_dt")
         return d.value
 
-    def indicator(self, crit):
+    def indicator(self, crit, to_host=True):
+        """(SI, split); to_host=False leaves both on the device, as the device regrid reads them."""
+        if not to_host:
+            _check(lib().fv_dev_indicator(self._h, C.byref(crit), None, None), "fv_dev_indicator")
+            return None, None
         SI = np.zeros((self.n, 2), dtype=np.int32)
         split = np.zeros(self.n, dtype=np.int32)
         _check(lib().fv_dev_indicator(self._h, C.byref(crit), _ip(SI), _ip(split)), "fv_dev_indicator")
@@ -724,9 +728,13 @@ class Device:
         _check(lib().fv_dev_get_limiter_flags(self._h, lim.ctypes.data_as(C.POINTER(C.c_uint8))), "limiter flags")
         return lim
 
-    def indicator_mlp(self, mlp, crit, smooth=True):
+    def indicator_mlp(self, mlp, crit, smooth=True, to_host=True):
         """Learned indicator: (SI, split, smooth flags); smooth=False leaves the flags
-        on the device only (they are the per-leaf limiter either way)."""
+        on the device only (they are the per-leaf limiter either way), to_host=False
+        SI and split too."""
+        if not to_host:
+            _check(lib().fv_dev_indicator_mlp(self._h, mlp._h, C.byref(crit), None, None, None), "fv_dev_indicator_mlp")
+            return None, None, None
         SI = np.zeros((self.n, 2), dtype=np.int32)
         split = np.zeros(self.n, dtype=np.int32)
         sm = np.zeros(self.n, dtype=np.uint8) if smooth else None

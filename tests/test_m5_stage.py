@@ -645,3 +645,32 @@ def test_m5_device_reports_nan_as_a_status():
         dev.advance(1)
     assert e.value.code == F.FV_ERR_NAN
     dev.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nroot,level", [((1, 1, 1), 1), ((2, 1, 1), 1), ((2, 2, 1), 3)])
+def test_m5_device_single_leaf_and_root_bricks_match_oracle(nroot, level):
+    """One leaf with six boundary faces, two single-leaf roots side by side, and a brick of level-3
+    roots: the root-crossing faces and a stencil made only of ghost values."""
+    p = m5p(FLUIDS, nmat=3, prelax=0)
+    bc = (1, 2, 5, 4, 4, 1)
+    m = F.Mesh.uniform(nroot, (0.0, 0.0, 0.0), 2.0 ** -6, level, list(bc))
+    rng = np.random.default_rng(150 + level)
+    S = smooth_field(p, m, rng, 2.0 ** -6 * nroot[0])
+    om = O.OracleMesh(m)
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    dev.faceinterp()
+    dev.residual()
+    SF, RHS, gf = dev.stage_arrays()
+    SFo = om.m5_faceinterp(p, S)
+    RHSo, gfo = om.m5_residual(p, S, SFo)
+    say(f"[m5 dev small] nroot={nroot} level={level} n={m.n}")
+    assert np.array_equal(SF, SFo) and np.array_equal(RHS, RHSo) and np.array_equal(gf, gfo)
+    p.dt = 1e-10
+    dev2 = F.M5Device(p, m)
+    dev2.set_state(S)
+    dev2.advance(1)
+    assert np.array_equal(dev2.get_state(), om.m5_advance(p, S, 1))
+    dev.close()
+    dev2.close()
