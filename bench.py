@@ -515,13 +515,66 @@ def m5_leg(level=8, steps=3, check=True, log_fn=log):
     return out
 
 
+def c3_run(steps, warmup, layers, nsteps_cmp=300, log_fn=log):
+    """Config 3 at its size: the C1 bubble pre-adapted to reference levels 5-10 (config
+    levels 4-9); FP64 and mixed-precision stage rates, and the mixed state against the FP64
+    one after nsteps_cmp steps on the fixed mesh (per-field relative L1 and Linf)."""
+    t0 = time.perf_counter()
+    p, m, S = build_c1(maxlevel=10, layers=layers, log_fn=log_fn)
+    log_fn(f"setup {time.perf_counter() - t0:.1f} s, {m.n} leaves")
+    nq = F.neq(p.nmat)
+    B = algorithmic_bytes(nq)
+    out = {"metric": "Mcell-updates/s per RK stage (config 3: FP64 and mixed precision)", "unit":
+           "Mcell-updates/s per RK stage", "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+           "config": {"workload": "c3_rayleigh_collapse_amr_L5-10", "leaves": m.n, "levels": "5-10 (config 4-9)",
+                      "interface_layers": layers, "prelax": p.prelax, "rk": "SSP-RK3"}}
+    states = {}
+    for prec, name in ((F.PREC_FP64, "fp64"), (F.PREC_MIXED, "mixed")):
+        dev = F.Device(p, m, prec=prec)
+        dev.set_state(S)
+        dev.time_stages(max(1, warmup))
+        ms, msf = dev.time_stages(steps)
+        dev.set_state(S)
+        t1 = time.perf_counter()
+        dev.advance(nsteps_cmp)
+        wall = time.perf_counter() - t1
+        states[name] = dev.get_state()
+        dev.close()
+        out[name] = {"value": round(m.n / (ms * 1e-3) / 1e6, 2), "ms_per_stage": round(ms, 4),
+                     "ms_flux_kernel": round(msf, 4),
+                     "achieved_GBps": round(B * m.n / (ms * 1e-3) / 1e9, 1),
+                     "frac_of_8TBps": round(B * m.n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                     "run_s": round(wall, 2)}
+        log_fn(f"[c3] {name}: {ms:.3f} ms/stage")
+    a, b = states["mixed"], states["fp64"]
+    names = ["alpha_rho_water", "alpha_rho_air", "rho_u", "rho_v", "rho_w", "rho_E", "alpha_water",
+             "alpha_rho_e_water", "alpha_rho_e_air"]
+    err = {}
+    for k, nm in enumerate(names):
+        den = np.abs(b[:, k]).sum() if k not in (2, 3, 4) else np.sqrt((b[:, 2:5] ** 2).sum(axis=1)).sum()
+        dmax = np.abs(b[:, k]).max() if k not in (2, 3, 4) else np.sqrt((b[:, 2:5] ** 2).sum(axis=1)).max()
+        err[nm] = {"l1": float(np.abs(a[:, k] - b[:, k]).sum() / den),
+                   "linf": float(np.abs(a[:, k] - b[:, k]).max() / dmax)}
+    pa, pb = mixture_pressure(p, a), mixture_pressure(p, b)
+    err["pressure"] = {"l1": float(np.abs(pa - pb).sum() / np.abs(pb).sum()),
+                       "linf": float(np.abs(pa - pb).max() / np.abs(pb).max())}
+    out["mixed_vs_fp64"] = {"steps": nsteps_cmp, "fields": err}
+    out["value"] = out["fp64"]["value"]
+    out["ms_per_step"] = out["fp64"]["ms_per_stage"]
+    out["dtype"] = "f64"
+    out["speedup_mixed_vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c1", "u64", "u128", "u256", "u320", "c2"],
-                    help="c2: the model-5 stage alone (config-2 materials, uniform 128^3)")
+    ap.add_argument("--workload", default="c1", choices=["c1", "u64", "u128", "u256", "u320", "c2", "c3"],
+                    help="c2: the model-5 stage alone (config-2 materials, uniform 128^3); c3: config 3, the "
+                         "bubble adapted to levels 5-10, FP64 against mixed precision")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true")
     ap.add_argument("--layers", type=int, default=64)
@@ -545,6 +598,10 @@ def main():
                           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
                           "config": {"workload": leg["workload"], "leaves": leg["leaves"]}, "m5_stage": leg}),
               flush=True)
+        return
+
+    if a.workload == "c3":
+        print(json.dumps(c3_run(a.steps, a.warmup, a.layers)), flush=True)
         return
 
     t_setup = time.perf_counter()
