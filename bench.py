@@ -586,8 +586,8 @@ def main():
     ap.add_argument("--no-m5", action="store_true", help="skip the model-5 (solid, GFM) stage leg")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo design is "
-              "described in DESIGN.md but not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo exchange is "
+              "designed but not built", file=sys.stderr)
         sys.exit(2)
 
     if a.workload == "c2":

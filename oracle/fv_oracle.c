@@ -1,7 +1,7 @@
 /*
  * fv_oracle.c -- CPU restatement of ForestFV's model-4 explicit update.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path (see fv_oracle.h).
+ * TEST INFRASTRUCTURE ONLY: the oracle the HIP path is tested against (see fv_oracle.h).
  * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).
  *
  * Every function names the reference lines it restates.  pow(x,2.0) in the

@@ -1,7 +1,7 @@
 /*
  * fv_oracle.h -- CPU restatement of ForestFV's model-4 explicit update.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle the HIP path is tested against:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it, and never as the thing that is measured or shipped.
  *

@@ -1,8 +1,8 @@
 /*
- * fv_oracle_m5.c -- CPU restatement of ForestFV's model-5 ghost-fluid pieces.
+ * fv_oracle_m5.c -- CPU restatement of ForestFV's model-5 stage.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP model-5 operators (see
- * fv_oracle_m5.h).  Built into libfvoracle.so with gcc -O2 -ffp-contract=off.
+ * TEST INFRASTRUCTURE ONLY: the oracle the HIP model-5 kernels are tested
+ * against (see fv_oracle_m5.h).  Built into libfvoracle.so with gcc -O2 -ffp-contract=off.
  *
  * Every function names the reference lines it restates and keeps their
  * operation order.  pow(x,2.0) is written x*x as in fv_oracle.c; the

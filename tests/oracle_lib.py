@@ -1,7 +1,7 @@
 """ctypes binding of the CPU restatement in oracle/ (test infrastructure only).
 
 Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
-checker of the HIP path; never as the thing measured or shipped.
+oracle the HIP path is tested against; never as the thing measured or shipped.
 """
 from __future__ import annotations
 
