@@ -128,6 +128,7 @@ class fv_m5_params(C.Structure):
         ("prelax", C.c_int),
         ("order", C.c_int),
         ("rk_steps", C.c_int),
+        ("yield_", C.c_double),
     ]
 
 
@@ -300,7 +301,7 @@ def m5_neq(nmat):
 
 
 def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush=(2.6e10, 0.0, 0.0), amin=1e-6,
-                   avf_lim=0.5, dt=1e-9, zerou=(0, 0, 0), prelax=1, order=2, rk_steps=3):
+                   avf_lim=0.5, dt=1e-9, zerou=(0, 0, 0), prelax=1, order=2, rk_steps=3, yield_stress=0.0):
     """Model-5 material table (MATERGAMA, MATERPINF, MATERMUSH) and stage constants."""
     p = fv_m5_params()
     p.nmat = nmat
@@ -310,6 +311,7 @@ def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush
     for k in range(3):
         p.zerou[k] = zerou[k]
     p.prelax, p.order, p.rk_steps = prelax, order, rk_steps
+    p.yield_ = yield_stress
     return p
 
 

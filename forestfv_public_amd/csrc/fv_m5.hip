@@ -40,6 +40,7 @@ struct M5P {
   double amin, avf_lim, dt;
   int zerou[3];
   int prelax, order;
+  double yield;   // von Mises yield stress of perfect plasticity, 0 = elastic
 };
 
 template <int NM>
@@ -1147,6 +1148,143 @@ __device__ int relax5(const M5P &P, double *x) {
   return relax_out<NM>(P, R, are, x);
 }
 
+// ---- perfect plasticity ----------------------------------------------------
+// New physics: the reference's plastic_rlx (m5_relaxation_solid.c:14) is
+// commented out.  With G = A A^T = Q diag(l) Q^T, d = det G and log strains
+// k_i = ln l_i - ln(d)/3 (sum 0), the state moves along l_i(t) = d^(1/3)
+// exp(t k_i), which keeps det G, from t = 1 towards the stress-free t = 0, to
+// the largest t whose von Mises stress of the deviatoric stress of
+// state_equation.c:143-204 (coefficient sum_k mu_k) is at most the yield
+// stress: A becomes Q diag(exp((t - 1) k_i / 2)) Q^T A.  The energies stay, so
+// the released elastic energy reappears as pressure.
+
+// cyclic Jacobi on the symmetric 3x3 a (destroyed): a = Q diag(l) Q^T
+__device__ void sym3_eigen(double a[9], double l[3], double Q[9]) {
+#pragma unroll
+  for (int k = 0; k < 9; k++) Q[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 12; sweep++) {
+    const double off = fabs(a[1]) + fabs(a[2]) + fabs(a[5]);
+    const double dia = fabs(a[0]) + fabs(a[4]) + fabs(a[8]);
+    if (off <= 1e-300 || off <= 1e-18 * dia) break;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const int P = (r == 2) ? 1 : 0, q = (r == 0) ? 1 : 2;
+      const double apq = a[P * 3 + q];
+      if (apq == 0.0) continue;
+      const double th = (a[q * 3 + q] - a[P * 3 + P]) / (2.0 * apq);
+      const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double akp = a[k * 3 + P], akq = a[k * 3 + q];
+        a[k * 3 + P] = c * akp - s * akq;
+        a[k * 3 + q] = s * akp + c * akq;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double apk = a[P * 3 + k], aqk = a[q * 3 + k];
+        a[P * 3 + k] = c * apk - s * aqk;
+        a[q * 3 + k] = s * apk + c * aqk;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double qkp = Q[k * 3 + P], qkq = Q[k * 3 + q];
+        Q[k * 3 + P] = c * qkp - s * qkq;
+        Q[k * 3 + q] = s * qkp + c * qkq;
+      }
+    }
+  }
+  l[0] = a[0]; l[1] = a[4]; l[2] = a[8];
+}
+
+// von Mises stress along the path, divided by mu d^(1/2)
+__device__ double vm_path(const double k[3], double t) {
+  double e1[3], e2[3], s1 = 0.0, s2 = 0.0, q = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) { e1[i] = exp(t * k[i]); e2[i] = e1[i] * e1[i]; s1 += e1[i]; s2 += e2[i]; }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const double si = (e2[i] - s2 / 3.0) - (e1[i] - s1 / 3.0);
+    q += si * si;
+  }
+  return sqrt(1.5 * q);
+}
+
+// the return map on A (9 values, row-major) in place: 1 changed, 0 not, -1 NaN
+__device__ int plastic_A(const M5P &P, double mu, double *A) {
+  double G[9], d;
+  gmat(A, G, d);
+  if (!(d > 0.0)) return 0;
+  double G2[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) G2[i * 3 + j] = G[i * 3 + 0] * G[0 * 3 + j] + G[i * 3 + 1] * G[1 * 3 + j] + G[i * 3 + 2] * G[2 * 3 + j];
+  const double J1 = G[0] + G[4] + G[8], J2 = G2[0] + G2[4] + G2[8], d13 = cbrt(d);
+  double q = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    double ssk = G2[k] - d13 * G[k];
+    if (k % 4 == 0) ssk -= J2 / 3.0 - d13 * J1 / 3.0;
+    q += ssk * ssk;
+  }
+  const double vm = mu / pow(d, 1.0 / 6.0) * sqrt(1.5 * q);
+  if (!(vm > P.yield)) return 0;
+  double l[3], Q[9], kap[3];
+  sym3_eigen(G, l, Q);
+  const double lnd3 = (log(l[0]) + log(l[1]) + log(l[2])) / 3.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) kap[i] = log(l[i]) - lnd3;
+  const double target = P.yield / (mu * sqrt(d));
+  double lo = 0.0, hi = 1.0;
+  for (int it = 0; it < 64; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (vm_path(kap, mid) > target) hi = mid; else lo = mid;
+  }
+  double f[3], M[9], An[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++) f[i] = exp(0.5 * (lo - 1.0) * kap[i]);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      M[i * 3 + j] = Q[i * 3 + 0] * f[0] * Q[j * 3 + 0] + Q[i * 3 + 1] * f[1] * Q[j * 3 + 1] + Q[i * 3 + 2] * f[2] * Q[j * 3 + 2];
+  int bad = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      An[i * 3 + j] = M[i * 3 + 0] * A[0 * 3 + j] + M[i * 3 + 1] * A[1 * 3 + j] + M[i * 3 + 2] * A[2 * 3 + j];
+      bad |= isnan(An[i * 3 + j]);
+    }
+  if (bad) return -1;
+#pragma unroll
+  for (int k = 0; k < 9; k++) A[k] = An[k];
+  return 1;
+}
+
+// the return map on the A rows of every leaf of S [NEQ][n], run between m5_RK
+// and the relaxation (where m5_relaxation_solid.c:14 would call plastic_rlx);
+// the launch is skipped when the yield stress is 0
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_plastic(int n, M5P P, double *__restrict__ S, int *__restrict__ flag) {
+  using L = L5<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double mu = 0.0;
+#pragma unroll
+  for (int k = 0; k < NM; k++) mu += P.mush[k];
+  if (mu == 0.0) return;
+  double A[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) A[k] = S[(size_t)(L::I5 + k) * n + i];
+  const int rc = plastic_A(P, mu, A);
+  if (rc < 0) { atomicOr(flag, 1); return; }
+  if (rc == 0) return;
+#pragma unroll
+  for (int k = 0; k < 9; k++) S[(size_t)(L::I5 + k) * n + i] = A[k];
+}
+
 // PRELAX 3 over many states (face states or cell states) with the slow ones
 // finished apart: state q of `count` is element v at X[(q / n) * NEQ * n + v * n + q % n]
 // (SF [6][NEQ][n] for face states, S [NEQ][n] for cell states).  The start
@@ -1562,6 +1700,8 @@ int m5_params(const fv_m5_params *p, M5P &P) {
   }
   P.amin = p->amin; P.avf_lim = p->avf_lim; P.dt = p->dt;
   P.prelax = p->prelax; P.order = p->order;
+  if (!(p->yield >= 0.0)) return fv_set_err(FV_ERR_ARG, "model 5: the yield stress must be >= 0");
+  P.yield = p->yield;
   return FV_OK;
 }
 
@@ -1864,7 +2004,14 @@ int m5_stage_t(fv_m5_dev *c, int irk, hipEvent_t *ev) {
     HIPCHK(hipGetLastError());
   }
   std::swap(c->S, c->S2);
-  // m5_relaxation_solid: relax every leaf, then re-project the leaves whose solid fraction crossed
+  // m5_relaxation_solid: the plastic return map (new), relax every leaf, then
+  // re-project the leaves whose solid fraction crossed.  The relaxation saves
+  // the state it starts from in Sb; the re-projection reads only the fractions
+  // and velocities of Sb, which the map does not change.
+  if (c->P.yield > 0.0) {
+    hipLaunchKernelGGL(k_m5_plastic<NM>, g, b, 0, c->stream, c->n, c->P, c->S, c->flag);
+    HIPCHK(hipGetLastError());
+  }
   if (c->P.prelax == 3) {
     if ((rc = m5_relax3<NM>(c, c->S, c->n, c->Sb)) != FV_OK) return rc;
   } else {

@@ -37,6 +37,9 @@ typedef struct {
   int    prelax;               /* PRELAX of m5_relaxationvec: 0, 1, 2 or 3 (stage entry points) */
   int    order;                /* ORDER: 1 cell states, 2 MUSCL face states (stage entry points) */
   int    rk_steps;             /* RK_STEPS: stages 1..rk_steps of m5_RK per step (fv_m5_dev_advance) */
+  double yield;                /* von Mises yield stress of the solid (material 0) for perfect
+                                  plasticity, 0 = elastic; new: the reference's plastic_rlx is
+                                  commented out (m5_relaxation_solid.c:14) */
 } fv_m5_params;
 
 int fv_m5_neq(int nmat);

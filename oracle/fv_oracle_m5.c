@@ -887,6 +887,114 @@ int orc_m5_relax(const orc_m5_params *p, double *vec) {
 }
 
 /* ------------------------------------------------------------------ */
+/* perfect plasticity (new: plastic_rlx, m5_relaxation_solid.c:14, is    */
+/* commented out in the reference)                                      */
+/* ------------------------------------------------------------------ */
+/* cyclic Jacobi on the symmetric 3x3 matrix a (row-major, destroyed):
+ * l = eigenvalues, Q columns = eigenvectors, a = Q diag(l) Q^T */
+static void sym3_eigen(double a[9], double l[3], double Q[9]) {
+  static const int PQ[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int k = 0; k < 9; k++) Q[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 12; sweep++) {
+    double off = fabs(a[1]) + fabs(a[2]) + fabs(a[5]);
+    double dia = fabs(a[0]) + fabs(a[4]) + fabs(a[8]);
+    if (off <= 1e-300 || off <= 1e-18 * dia) break;
+    for (int r = 0; r < 3; r++) {
+      const int P = PQ[r][0], q = PQ[r][1];
+      const double apq = a[P * 3 + q];
+      if (apq == 0.0) continue;
+      const double th = (a[q * 3 + q] - a[P * 3 + P]) / (2.0 * apq);
+      const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+      for (int k = 0; k < 3; k++) {   /* a = a J (columns P, q) */
+        const double akp = a[k * 3 + P], akq = a[k * 3 + q];
+        a[k * 3 + P] = c * akp - s * akq;
+        a[k * 3 + q] = s * akp + c * akq;
+      }
+      for (int k = 0; k < 3; k++) {   /* a = J^T a (rows P, q) */
+        const double apk = a[P * 3 + k], aqk = a[q * 3 + k];
+        a[P * 3 + k] = c * apk - s * aqk;
+        a[q * 3 + k] = s * apk + c * aqk;
+      }
+      for (int k = 0; k < 3; k++) {   /* Q = Q J */
+        const double qkp = Q[k * 3 + P], qkq = Q[k * 3 + q];
+        Q[k * 3 + P] = c * qkp - s * qkq;
+        Q[k * 3 + q] = s * qkp + c * qkq;
+      }
+    }
+  }
+  l[0] = a[0]; l[1] = a[4]; l[2] = a[8];
+}
+
+/* von Mises stress of the deviatoric stress -mu d^(-1/6) (G^2 - tr(G^2)/3 I
+ * - d^(1/3) (G - tr(G)/3 I)) of state_equation.c:143-204, for G with
+ * eigenvalues d^(1/3) exp(t k_i) (sum k_i = 0), divided by mu d^(1/2) */
+static double vm_path(const double k[3], double t) {
+  double e1[3], e2[3], s1 = 0.0, s2 = 0.0, q = 0.0;
+  for (int i = 0; i < 3; i++) { e1[i] = exp(t * k[i]); e2[i] = e1[i] * e1[i]; s1 += e1[i]; s2 += e2[i]; }
+  for (int i = 0; i < 3; i++) {
+    const double si = (e2[i] - s2 / 3.0) - (e1[i] - s1 / 3.0);
+    q += si * si;
+  }
+  return sqrt(1.5 * q);
+}
+
+static int m5_plastic(const orc_m5_params *p, const lay5_t *L, double *vec) {
+  double mu = 0.0;
+  for (int i = 0; i < p->nmat; i++) mu += p->mush[i];
+  if (!(p->yield > 0.0) || mu == 0.0) return 0;
+  double A[9], G[9], d;
+  for (int k = 0; k < 9; k++) A[k] = vec[L->i5 + k];
+  gmat_of(A, G, &d);
+  if (!(d > 0.0)) return 0;
+  /* the von Mises stress of the current state, matrix form */
+  double G2[9] = {0}, ss[9];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k)
+      for (int j = 0; j < 3; ++j) G2[i * 3 + j] += G[i * 3 + k] * G[k * 3 + j];
+  const double J1 = G[0] + G[4] + G[8], J2 = G2[0] + G2[4] + G2[8], d13 = cbrt(d);
+  double q = 0.0;
+  for (int k = 0; k < 9; k++) {
+    ss[k] = G2[k] - d13 * G[k];
+    if (k % 4 == 0) ss[k] -= J2 / 3.0 - d13 * J1 / 3.0;
+    q += ss[k] * ss[k];
+  }
+  const double vm = mu / pow(d, 1.0 / 6.0) * sqrt(1.5 * q);
+  if (!(vm > p->yield)) return 0;
+  /* log strains k_i = ln l_i - ln(d)/3 of G = Q diag(l) Q^T */
+  double a[9], l[3], Q[9], kap[3];
+  memcpy(a, G, sizeof a);
+  sym3_eigen(a, l, Q);
+  const double lnd3 = (log(l[0]) + log(l[1]) + log(l[2])) / 3.0;
+  for (int i = 0; i < 3; i++) kap[i] = log(l[i]) - lnd3;
+  /* the largest t in [0, 1] with von Mises <= yield along l_i(t) = d^(1/3) exp(t k_i) */
+  const double target = p->yield / (mu * sqrt(d));
+  double lo = 0.0, hi = 1.0;
+  for (int it = 0; it < 64; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (vm_path(kap, mid) > target) hi = mid; else lo = mid;
+  }
+  /* A = Q diag(exp((t - 1) k_i / 2)) Q^T A */
+  double f[3], M[9], An[9];
+  for (int i = 0; i < 3; i++) f[i] = exp(0.5 * (lo - 1.0) * kap[i]);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      M[i * 3 + j] = Q[i * 3 + 0] * f[0] * Q[j * 3 + 0] + Q[i * 3 + 1] * f[1] * Q[j * 3 + 1] + Q[i * 3 + 2] * f[2] * Q[j * 3 + 2];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) An[i * 3 + j] = M[i * 3 + 0] * A[0 * 3 + j] + M[i * 3 + 1] * A[1 * 3 + j] + M[i * 3 + 2] * A[2 * 3 + j];
+  for (int k = 0; k < 9; k++) {
+    if (isnan(An[k])) return -1;
+    vec[L->i5 + k] = An[k];
+  }
+  return 1;
+}
+
+int orc_m5_plastic(const orc_m5_params *p, double *vec) {
+  lay5_t L = mk_lay5(p->nmat);
+  return m5_plastic(p, &L, vec);
+}
+
+/* ------------------------------------------------------------------ */
 /* MUSCL face states (m5_faceinterp_V1)                                 */
 /* ------------------------------------------------------------------ */
 /* avf_limit.c:3-21 with max_limit 100 on the fractions of X */
@@ -1331,8 +1439,10 @@ static int m5_stage_impl(const lay5_t *L, const orc_mesh *m, const orc_cell *cel
   if ((rc = orc_m5_rk(m, cells, p, irk, SN, S, RHS, gfm)) != 0) return rc;
   /* m5_relaxation_solid.c:12-37: relax, then re-project the leaves that crossed */
   memcpy(Sb, S, sizeof(double) * (size_t)m->n * neq);
-  for (int i = 0; i < m->n; i++)
+  for (int i = 0; i < m->n; i++) {
+    if (p->yield > 0.0 && (rc = m5_plastic(p, L, &S[(size_t)i * neq])) < 0) return rc;   /* where :14 calls plastic_rlx */
     if ((rc = m5_relaxvec(L, p, &S[(size_t)i * neq])) != 0) return rc;
+  }
   return orc_m5_gfm_rel(m, cells, p, Sb, S, gfm);
 }
 
@@ -1358,6 +1468,7 @@ static void m5_stage_ws_free(m5_stage_ws *W) {
 static int m5_check(const orc_m5_params *p) {
   if (p->nmat != 2 && p->nmat != 3) return -2;
   if (p->prelax < 0 || p->prelax > 3 || (p->order != 1 && p->order != 2)) return -2;
+  if (!(p->yield >= 0.0)) return -2;
   return 0;
 }
 

@@ -37,6 +37,7 @@ typedef struct {
   int    prelax;               /* PRELAX of m5_relaxationvec: 0, 1, 2 or 3 */
   int    order;                /* ORDER: 1 cell states, 2 MUSCL face states */
   int    rk_steps;             /* RK_STEPS: stages irk = 1..rk_steps of m5_RK */
+  double yield;                /* von Mises yield stress of the solid, 0 = elastic (new) */
 } orc_m5_params;
 
 int orc_m5_neq(int nmat);
@@ -95,6 +96,17 @@ int orc_m5_residual(const orc_mesh *m, const orc_cell *cells, const orc_m5_param
  * SF, RHS and gfm_face receive the stage's intermediate arrays (may be NULL). */
 int orc_m5_stage(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, int irk, const double *SN,
                  double *S, double *SF, double *RHS, unsigned char *gfm_face);
+
+/* Perfect plasticity of the solid (material 0), new: the reference's
+ * plastic_rlx (m5_relaxation_solid.c:14) is commented out and undefined.
+ * With G = A A^T = Q diag(l) Q^T and d = det G, the elastic state is moved
+ * along l_i(t) = d^(1/3) (l_i / d^(1/3))^t, t in [0, 1] (volume preserving:
+ * det G stays d; t = 0 is stress free), to the t where the von Mises stress of
+ * the deviatoric stress of state_equation.c:143-204 equals the yield stress;
+ * A becomes Q diag(sqrt(l(t) / l)) Q^T A.  Below yield nothing changes.  The
+ * energies are kept, so the released elastic energy becomes heat through the
+ * pressure recovery.  Returns 1 when A changed, 0 when not. */
+int orc_m5_plastic(const orc_m5_params *p, double *vec);
 
 /* nsteps explicit steps: SN = S (RK.c case 0), then stages 1..rk_steps. */
 int orc_m5_advance(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, double *S, int nsteps);

@@ -85,6 +85,7 @@ class orc_m5_params(C.Structure):
         ("prelax", C.c_int),
         ("order", C.c_int),
         ("rk_steps", C.c_int),
+        ("yield_", C.c_double),
     ]
 
 
@@ -141,6 +142,7 @@ def lib():
                                    P(C.c_uint8)]
         L.orc_m5_advance.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), dp, C.c_int]
         L.orc_m5_face.argtypes = [P(orc_m5_params), dp, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp]
+        L.orc_m5_plastic.argtypes = [P(orc_m5_params), dp]
         _lib = L
     return _lib
 
@@ -434,3 +436,10 @@ def m5_face(m5p, XL, CL, XR, CR, frame, bc, area):
     if rc < 0:
         raise FloatingPointError("NaN in the face flux")
     return flux, unc, rc
+
+
+def m5_plastic(m5p, vec):
+    """orc_m5_plastic: (the state after the plastic step, whether A changed)."""
+    v = np.ascontiguousarray(vec, dtype=np.float64).copy()
+    changed = lib().orc_m5_plastic(C.byref(m5_from(m5p)), _dp(v))
+    return v, bool(changed)
