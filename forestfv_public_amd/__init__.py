@@ -54,6 +54,9 @@ class fv_params(C.Structure):
         ("bc", C.c_int * 6),
         ("fix", C.c_int),
         ("primtv", C.c_int),
+        ("bc_fixed_p", C.c_int),
+        ("p_inlet", C.c_double),
+        ("p_outlet", C.c_double),
     ]
 
 
@@ -258,7 +261,9 @@ def neq(nmat):
 
 
 def make_params(nmat=2, gamma=(1.4, 1.4), pinf=(0.0, 0.0), amin=1e-8, avf_lim=0.5, prelax=0,
-                rk=RK_SSP3, limiter=0, dt=1e-3, bc=(4, 4, 4, 4, 4, 4), fix=FIX_ALL, primtv=0):
+                rk=RK_SSP3, limiter=0, dt=1e-3, bc=(4, 4, 4, 4, 4, 4), fix=FIX_ALL, primtv=0, fixed_p=None):
+    """fixed_p = (p_inlet, p_outlet): fixed-pressure ghost states at inlet (code 3) and outlet
+    (code 2) faces, as the reference's CASE 3400/3410 with (150e6, 1e6); None: plain copies."""
     p = fv_params()
     p.nmat = nmat
     for k in range(nmat):
@@ -269,6 +274,9 @@ def make_params(nmat=2, gamma=(1.4, 1.4), pinf=(0.0, 0.0), amin=1e-8, avf_lim=0.
         p.bc[k] = bc[k]
     p.fix = fix
     p.primtv = primtv
+    if fixed_p is not None:
+        p.bc_fixed_p = 1
+        p.p_inlet, p.p_outlet = fixed_p
     return p
 
 

@@ -236,7 +236,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_rk(int n, Params P, Geo G, in
         if (kind == FV_FK_BOUNDARY) {
           sR = sL;
           bcode = -nb;
-          apply_bc<NM, double>(bcode, G.frame[f], sL, sR);
+          apply_bc<NM, double>(P, bcode, G.frame[f], sL, sR);
         } else {
           const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
           load_face<NM>(SF, n, j, fo, X);
@@ -447,15 +447,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_fast(int n, Params P, Geo G, 
         if (kind == FV_FK_BOUNDARY) {
           sR = sL;
           bcode = -nb;
-          if (bcode == 1 || bcode == 4 || bcode == 5) {
-            const R ul = sL.u * fr[0] + sL.v * fr[1] + sL.w * fr[2];
-            const R vl = sL.u * fr[3] + sL.v * fr[4] + sL.w * fr[5];
-            const R wl = sL.u * fr[6] + sL.v * fr[7] + sL.w * fr[8];
-            const R vr = (bcode == 5) ? -vl : vl, wr = (bcode == 5) ? -wl : wl;
-            sR.u = -ul * fr[0] + vr * fr[3] + wr * fr[6];
-            sR.v = -ul * fr[1] + vr * fr[4] + wr * fr[7];
-            sR.w = -ul * fr[2] + vr * fr[5] + wr * fr[8];
-          }
+          fvf::ghost<NM, R, XR>(M, P, bcode, fr, sL, sR);
         } else {
           const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
           load_sol<NM, MIXED, R>(Md, M, a, SF, SFm, n, j, fo, sR);
@@ -616,15 +608,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_pipe(int n, Params P, Geo G, 
         if (kind == FV_FK_BOUNDARY) {
           sR = sL;
           bcode = -nb;
-          if (bcode == 1 || bcode == 4 || bcode == 5) {
-            const R ul = sL.u * fr[0] + sL.v * fr[1] + sL.w * fr[2];
-            const R vl = sL.u * fr[3] + sL.v * fr[4] + sL.w * fr[5];
-            const R wl = sL.u * fr[6] + sL.v * fr[7] + sL.w * fr[8];
-            const R vr = (bcode == 5) ? -vl : vl, wr = (bcode == 5) ? -wl : wl;
-            sR.u = -ul * fr[0] + vr * fr[3] + wr * fr[6];
-            sR.v = -ul * fr[1] + vr * fr[4] + wr * fr[7];
-            sR.w = -ul * fr[2] + vr * fr[5] + wr * fr[8];
-          }
+          fvf::ghost<NM, R, XR>(M, P, bcode, fr, sL, sR);
         } else if (kind == FV_FK_FINE) {
           load_sol<NM, MIXED, R>(Md, M, a, SF, SFm, n, fine4[4 * nb + k], fo, sR);
         } else {
@@ -883,15 +867,7 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
         if (kind == FV_FK_BOUNDARY) {
           sR = sL;
           bcode = -nb;
-          if (bcode == 1 || bcode == 4 || bcode == 5) {
-            const R ul = sL.u * fr[0] + sL.v * fr[1] + sL.w * fr[2];
-            const R vl = sL.u * fr[3] + sL.v * fr[4] + sL.w * fr[5];
-            const R wl = sL.u * fr[6] + sL.v * fr[7] + sL.w * fr[8];
-            const R vr = (bcode == 5) ? -vl : vl, wr = (bcode == 5) ? -wl : wl;
-            sR.u = -ul * fr[0] + vr * fr[3] + wr * fr[6];
-            sR.v = -ul * fr[1] + vr * fr[4] + wr * fr[7];
-            sR.w = -ul * fr[2] + vr * fr[5] + wr * fr[8];
-          }
+          fvf::ghost<NM, R, XR>(M, P, bcode, fr, sL, sR);
         } else {
           const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
           double Fnb[L::NEQ];
@@ -1157,6 +1133,7 @@ static void to_dev_params(const fv_params *p, Params &P) {
   P.amin = p->amin; P.avf_lim = p->avf_lim; P.dt = p->dt;
   P.prelax = p->prelax; P.limiter = p->limiter; P.rk = p->rk; P.nmat = p->nmat; P.fix = p->fix;
   P.primtv = p->primtv;
+  P.bc_fixed_p = p->bc_fixed_p; P.p_inlet = p->p_inlet; P.p_outlet = p->p_outlet;
   for (int k = 0; k < 4; k++) {
     const double g = p->gamma[k], pi = p->pinf[k];
     const double gm1 = (g != 1.0) ? g - 1.0 : 1.0, gg = (g != 0.0) ? g : 1.0;
@@ -1245,6 +1222,9 @@ extern "C" int fv_dev_attach(fv_dev **out, const fv_params *p, const fv_pack *pk
   if (p->nmat != 2 && p->nmat != 3) return set_err(FV_ERR_ARG, "nmat must be 2 or 3");
   if (p->prelax < 0 || p->prelax > 3) return set_err(FV_ERR_ARG, "PRELAX must be 0, 1, 2 or 3");
   if (p->primtv != 0 && p->primtv != 1) return set_err(FV_ERR_ARG, "PRIMTV must be 0 or 1");
+  if (p->bc_fixed_p != 0 && p->bc_fixed_p != 1) return set_err(FV_ERR_ARG, "bc_fixed_p must be 0 or 1");
+  if (p->bc_fixed_p && !(isfinite(p->p_inlet) && isfinite(p->p_outlet)))
+    return set_err(FV_ERR_ARG, "the inlet and outlet pressures must be finite");
   if (p->prelax == 1 && p->nmat != 2) return set_err(FV_ERR_ARG, "PRELAX=1 is two-phase only");
   if (p->rk != FV_RK_REF2 && p->rk != FV_RK_SSP3 && p->rk != FV_RK_SSP2) return set_err(FV_ERR_ARG, "unknown RK table");
   int ndev = 0;
@@ -1415,8 +1395,9 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
 static int launch_stage(fv_dev *c, int irk, hipEvent_t *ev_flux0, hipEvent_t *ev_flux1) {
   const int cs = stage_case(c->P.rk, irk);
   // PRELAX 2 and 3 iterate; their kernels are separate instances so that the
-  // closed-form relaxations do not carry the iterations' code and registers
-  const bool xr = c->P.prelax >= 2;
+  // closed-form relaxations do not carry the iterations' code and registers.
+  // The fixed-pressure inlet and outlet ghosts live in those instances too.
+  const bool xr = c->P.prelax >= 2 || c->P.bc_fixed_p;
   int rc = (c->nmat == 2) ? (xr ? launch_stage_nm<2, true>(c, cs, ev_flux0) : launch_stage_nm<2, false>(c, cs, ev_flux0))
                           : (xr ? launch_stage_nm<3, true>(c, cs, ev_flux0) : launch_stage_nm<3, false>(c, cs, ev_flux0));
   if (rc != FV_OK) return rc;

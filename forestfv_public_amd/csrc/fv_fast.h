@@ -170,6 +170,44 @@ FV_DEV void fill_rec(const Mat<NM, float> &M, const float *rec, SolF<NM, float> 
   s.rEp = rec[Q::REP];
 }
 
+// The boundary ghost sR (a copy of sL on entry): the mirrored velocity of
+// codes 1, 4, 5 (bc_symmetry.c, bc_reflective.c:12-28, bc_wall.c:12-26) and,
+// with bc_fixed_p, every material at the outlet (2) / inlet (3) pressure of
+// m4_bc_outlet / m4_bc_inlet for CASE 3400/3410 (m4_bc.c:44-113, 121-186).
+// Only the XR kernel instances carry the fixed-pressure branch: compiled into
+// the others it costs their flux kernels 14 VGPRs and a spill.
+template <int NM, typename R, bool XR>
+FV_DEV void ghost(const Mat<NM, R> &M, const Params &P, int bcode, const R *fr, const SolF<NM, R> &sL,
+                  SolF<NM, R> &sR) {
+  if (bcode == 1 || bcode == 4 || bcode == 5) {
+    const R ul = sL.u * fr[0] + sL.v * fr[1] + sL.w * fr[2];
+    const R vl = sL.u * fr[3] + sL.v * fr[4] + sL.w * fr[5];
+    const R wl = sL.u * fr[6] + sL.v * fr[7] + sL.w * fr[8];
+    const R vr = (bcode == 5) ? -vl : vl, wr = (bcode == 5) ? -wl : wl;
+    sR.u = -ul * fr[0] + vr * fr[3] + wr * fr[6];
+    sR.v = -ul * fr[1] + vr * fr[4] + wr * fr[7];
+    sR.w = -ul * fr[2] + vr * fr[5] + wr * fr[8];
+    return;
+  }
+  if constexpr (XR) {
+    if (!(P.bc_fixed_p && (bcode == 2 || bcode == 3))) return;
+    const R pb = (R)(bcode == 3 ? P.p_inlet : P.p_outlet);
+    const R ir = rcp(sR.r);
+    R re = (R)0, c2 = (R)0;
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+      sR.p[k] = pb;
+      sR.are[k] = sR.avf[k] * (pb + M.GP[k]) * M.iGm1[k];
+      c2 += sR.avf[k] * ir * M.G[k] * (pb + M.P[k]);
+      re += sR.are[k];
+    }
+    re += (R)0.5 * sR.r * (sR.u * sR.u + sR.v * sR.v + sR.w * sR.w);
+    sR.st = -pb;
+    sR.c2 = c2;
+    sR.rEp = re - sR.Pi;
+  }
+}
+
 // HLLC n-flux of one face (m4_hllc.c, m4_wave_speeds.c, m4_star_region.c, m4_flux.c)
 template <int NM, typename R>
 FV_DEV void hllc(const Mat<NM, R> &M, int fix, const R *F, int bc, const SolF<NM, R> &sL,

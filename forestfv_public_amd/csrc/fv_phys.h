@@ -25,6 +25,8 @@ struct Params {
   double gamma[4], pinf[4];
   double amin, avf_lim, dt;
   int prelax, limiter, rk, nmat, fix, primtv;
+  int bc_fixed_p;               // fixed-pressure ghost at outlet (2) / inlet (3) faces
+  double p_inlet, p_outlet;
   // per-material constants formed once on the host: G p_inf, G - 1, 1/(G-1),
   // G p_inf/(G-1), 1/G (used by the throughput kernels)
   double GP[4], Gm1[4], iGm1[4], C[4], iG[4];
@@ -105,9 +107,30 @@ FV_DEV void fill_sol(const Params &P, const R *X, Sol<NM, R> &s) {
   }
 }
 
+// m4_bc_inlet / m4_bc_outlet with CASE 3400 or 3410 (m4_bc.c:44-113, 121-186):
+// every material of the ghost at pressure pb, with the ghost's own fractions,
+// densities and velocity; are, c2Y, e and rho E follow (the reference writes
+// materials 0 and 1, its injector cases having two; this covers all)
+template <int NM, typename R>
+FV_DEV void fixed_pressure_ghost(const Params &P, R pb, Sol<NM, R> &s) {
+  s.st = -pb;
+  R re = (R)0.0;
+#pragma unroll
+  for (int k = 0; k < NM; ++k) {
+    const R g = (R)P.gamma[k], pi = (R)P.pinf[k];
+    s.p[k] = pb;
+    s.are[k] = s.avf[k] * (pb + g * pi) / (g - (R)1.0);
+    s.c2Y[k] = (s.avf[k] / s.r) * g * (pb + pi);
+    re += s.are[k];
+  }
+  re = re + (R)0.5 * s.r * (s.u * s.u + s.v * s.v + s.w * s.w);
+  s.e = re / s.r;
+  s.rE = re;
+}
+
 // bc_symmetry.c / bc_reflective.c:12-28 / bc_wall.c:12-26: only sol_R->u,v,w move
 template <int NM, typename R>
-FV_DEV void apply_bc(int code, const R *F, const Sol<NM, R> &sL, Sol<NM, R> &sR) {
+FV_DEV void apply_bc(const Params &P, int code, const R *F, const Sol<NM, R> &sL, Sol<NM, R> &sR) {
   if (code == 1 || code == 4 || code == 5) {
     R u = sL.u, v = sL.v, w = sL.w;
     R ul = u * F[0] + v * F[1] + w * F[2];
@@ -119,6 +142,8 @@ FV_DEV void apply_bc(int code, const R *F, const Sol<NM, R> &sL, Sol<NM, R> &sR)
     sR.v = ur * F[1] + vr * F[4] + wr * F[7];
     sR.w = ur * F[2] + vr * F[5] + wr * F[8];
   }
+  if (P.bc_fixed_p && (code == 2 || code == 3))
+    fixed_pressure_ghost<NM, R>(P, (R)(code == 3 ? P.p_inlet : P.p_outlet), sR);
 }
 
 // Output of one Riemann problem: n-flux (F.n + F_HLLC) and the star velocity.

@@ -67,6 +67,11 @@ typedef struct {
   int    bc[6];               /* BC code of the -x,+x,-y,+y,-z,+z sides (BCOVERWRT) */
   int    fix;                 /* FV_FIX_* corrections of reference defects; 0 = literal reference */
   int    primtv;              /* PRIMTV: 0 conservative, 1 primitive reconstruction (m4_reconctruct.c:126-181) */
+  int    bc_fixed_p;          /* 1: the ghost state of an outlet (code 2) / inlet (code 3) face holds every
+                                 material at pressure p_outlet / p_inlet, as m4_bc_outlet / m4_bc_inlet do for
+                                 CASE 3400 and 3410 (m4_bc.c:44-113, 121-186; 1e6 and 150e6 Pa there);
+                                 0: those faces copy the own face state (every other CASE) */
+  double p_inlet, p_outlet;
 } fv_params;
 
 /* Corrections of the published model-4 residual (0 reproduces it literally):

@@ -293,9 +293,29 @@ static frame_t mkframe(const orc_cell *c, int f) {
   return F;
 }
 
+/* m4_bc_inlet / m4_bc_outlet with CASE 3400 or 3410 (m4_bc.c:44-113, 121-186):
+ * every material of the ghost at pressure pb, with the ghost's own fractions,
+ * densities and velocity; are, c2Y, e and rho E follow.  The reference writes
+ * materials 0 and 1 (its injector cases have two); this covers all of them. */
+static void fixed_pressure_ghost(const layout_t *L, const orc_params *p, double pb, sol_t *s) {
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) s->st[a][b] = (a == b) ? -pb : 0.0;
+  double re = 0.0;
+  for (int k = 0; k < L->n4; k++) {
+    s->p_hydro[k] = pb;
+    s->are[k] = s->avf[k] * (pb + p->gamma[k] * p->pinf[k]) / (p->gamma[k] - 1.0);
+    s->vec[L->i4 + k] = s->are[k];
+    s->c2Y[k] = (s->avf[k] / s->r) * p->gamma[k] * (pb + p->pinf[k]);
+    re += s->are[k];
+  }
+  re = re + 0.5 * s->r * (SQ(s->u) + SQ(s->v) + SQ(s->w));
+  s->e = re / s->r;
+  s->vec[L->i2] = re;
+}
+
 /* boundary ghost velocity (bc_symmetry.c, bc_reflective.c:12-28, bc_wall.c:12-26,
  * m4_bc.c:3-34).  Only sol_R->u,v,w change; vec keeps the unmirrored momentum. */
-static void apply_bc(int code, const frame_t *F, const sol_t *sL, sol_t *sR) {
+static void apply_bc(const layout_t *L, const orc_params *p, int code, const frame_t *F, const sol_t *sL, sol_t *sR) {
   if (code == 1 || code == 4 || code == 5) {
     double u = sL->u, v = sL->v, w = sL->w;
     double ul = u * F->nx + v * F->ny + w * F->nz;
@@ -307,7 +327,8 @@ static void apply_bc(int code, const frame_t *F, const sol_t *sL, sol_t *sR) {
     sR->v = ur * F->ny + vr * F->my + wr * F->ly;
     sR->w = ur * F->nz + vr * F->mz + wr * F->lz;
   }
-  /* codes 2 and 3 only act for the reference's CASE 3400/3410: a plain copy otherwise */
+  /* codes 2 and 3 act only for the reference's CASE 3400/3410: a plain copy otherwise */
+  if (p->bc_fixed_p && (code == 2 || code == 3)) fixed_pressure_ghost(L, p, code == 3 ? p->p_inlet : p->p_outlet, sR);
 }
 
 /* rotate_vectors.c:128-188 */
@@ -864,7 +885,7 @@ static void face_flux(const layout_t *L, const orc_params *p, const orc_mesh *m,
       fill_sol(L, p, &SF[((size_t)j * 6 + FCOPP[f]) * neq], &S[(size_t)j * neq], &sR);
     } else {
       fill_sol(L, p, &SF[((size_t)i * 6 + f) * neq], Si, &sR);
-      apply_bc(bc, &F, &sL, &sR);
+      apply_bc(L, p, bc, &F, &sL, &sR);
     }
     hllc_ws W;
     m4_hllc(L, p, &F, bc, &sL, &sR, &W);
@@ -1096,7 +1117,7 @@ void orc_hllc_state(const orc_params *p, const double *sfL, const double *sfR,
   sol_t sL, sR;
   fill_sol(&L, p, sfL, sfL, &sL);
   fill_sol(&L, p, sfR, sfR, &sR);
-  if (bc) apply_bc(bc, &F, &sL, &sR);
+  if (bc) apply_bc(&L, p, bc, &F, &sL, &sR);
   hllc_ws W;
   m4_hllc(&L, p, &F, bc, &sL, &sR, &W);
   for (int iv = 0; iv < L.neq; ++iv)

@@ -41,6 +41,8 @@ typedef struct {
   int    bc[6];                /* BC code of the -x,+x,-y,+y,-z,+z domain sides */
   int    fix;                  /* corrections of reference defects, ORC_FIX_* bits; 0 = literal */
   int    primtv;               /* PRIMTV: 0 reconstruct conservative, 1 primitive variables */
+  int    bc_fixed_p;           /* 1: fixed-pressure ghost at codes 2 / 3 (m4_bc.c:44-113, 121-186) */
+  double p_inlet, p_outlet;
 } orc_params;
 
 /* m4_cons2primtv.c:3-40 and m4_prim2conc.c:3-74: primitive vector

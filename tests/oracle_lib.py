@@ -32,6 +32,9 @@ class orc_params(C.Structure):
         ("bc", C.c_int * 6),
         ("fix", C.c_int),
         ("primtv", C.c_int),
+        ("bc_fixed_p", C.c_int),
+        ("p_inlet", C.c_double),
+        ("p_outlet", C.c_double),
     ]
 
 
