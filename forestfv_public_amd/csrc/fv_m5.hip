@@ -51,15 +51,23 @@ struct L5 {
 };
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
+__device__ __forceinline__ bool has_solid(const M5P &P) { return P.mush[0] != 0.0 || P.mush[1] != 0.0 || P.mush[2] != 0.0; }
 __device__ __forceinline__ double maxr(double a, double b) { return (a > b) ? a : b; }   // strdata.h:27
 __device__ __forceinline__ double minr(double a, double b) { return (a < b) ? a : b; }   // strdata.h:31
 
+// pow(|G|, 1/3) and pow(|G|, 1/6), which the elastic energy and the stress of
+// every solid material of one state use: evaluated once per state
+struct GPow {
+  double g13, g16;
+};
+__device__ GPow gpow(double Gmag) { return GPow{pow(fabs(Gmag), 1.0 / 3.0), pow(fabs(Gmag), 1.0 / 6.0)}; }
+
 // m5_elastic_energy_solid_V2, state_equation.c:269-316
-__device__ double elastic_energy_v2(const double *G, double Gmag, double mu) {
+__device__ double elastic_energy_v2(const double *G, double Gmag, const GPow &gp, double mu) {
   double eints;
   if (fabs(Gmag) != 0.0) {
     double M1[9], M2[9];
-    double Gmag13 = pow(fabs(Gmag), 1.0 / 3.0);
+    double Gmag13 = gp.g13;
     if (isnan(Gmag13)) Gmag13 = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; k++) { M1[k] = G[k]; M2[k] = 0.0; }
@@ -71,7 +79,7 @@ __device__ double elastic_energy_v2(const double *G, double Gmag, double mu) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) M2[i * 3 + j] += M1[i * 3 + k] * M1[k * 3 + j];
     const double tr1 = M2[0] + M2[4] + M2[8];
-    eints = mu / (4.0 * pow(fabs(Gmag), 1.0 / 6.0)) * tr1;
+    eints = mu / (4.0 * gp.g16) * tr1;
   } else {
     eints = 0.0;
   }
@@ -82,12 +90,12 @@ __device__ double elastic_energy_v2(const double *G, double Gmag, double mu) {
 __device__ double elastic_energy_v1(const double *G, double mu) {
   const double Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
                       G[2] * (G[3] * G[7] - G[4] * G[6]);
-  if ((fabs(Gmag) != 0.0) && (mu != 0.0)) return elastic_energy_v2(G, Gmag, mu);
+  if ((fabs(Gmag) != 0.0) && (mu != 0.0)) return elastic_energy_v2(G, Gmag, gpow(Gmag), mu);
   return 0.0;
 }
 
 // m5_eos_stress_V2, state_equation.c:143-204: the stress s of one material
-__device__ void stress_v2(double *s, double avf, double pr, const double *G, double Gmag, double mu) {
+__device__ void stress_v2(double *s, double avf, double pr, const double *G, double Gmag, const GPow &gp, double mu) {
   if ((fabs(Gmag) != 0.0) && (mu != 0.0)) {
     double G2[9], ss[9];
 #pragma unroll
@@ -100,7 +108,7 @@ __device__ void stress_v2(double *s, double avf, double pr, const double *G, dou
         for (int j = 0; j < 3; ++j) G2[i * 3 + j] += G[i * 3 + k] * G[k * 3 + j];
     const double J1 = G[0] + G[4] + G[8];
     const double J2 = G2[0] + G2[4] + G2[8];
-    const double g13 = pow(fabs(Gmag), 1.0 / 3.0);
+    const double g13 = gp.g13;
     ss[0] = G2[0] - J2 / 3.0 - g13 * (G[0] - J1 / 3.0);
     ss[1] = G2[1] - g13 * (G[1]);
     ss[2] = G2[2] - g13 * (G[2]);
@@ -110,7 +118,7 @@ __device__ void stress_v2(double *s, double avf, double pr, const double *G, dou
     ss[6] = G2[6] - g13 * (G[6]);
     ss[7] = G2[7] - g13 * (G[7]);
     ss[8] = G2[8] - J2 / 3.0 - g13 * (G[8] - J1 / 3.0);
-    const double k = mu / pow(fabs(Gmag), 1.0 / 6.0);
+    const double k = mu / gp.g16;
 #pragma unroll
     for (int q = 0; q < 9; q++) s[q] = (q == 0 || q == 4 || q == 8) ? (-avf * pr - k * (ss[q])) : (-k * (ss[q]));
   } else {
@@ -121,9 +129,10 @@ __device__ void stress_v2(double *s, double avf, double pr, const double *G, dou
 }
 
 // the sum of the materials' stresses (m5_facercnstr.c:150-173)
-__device__ void add_stress_v2(double *st, double avf, double pr, const double *G, double Gmag, double mu) {
+__device__ void add_stress_v2(double *st, double avf, double pr, const double *G, double Gmag, const GPow &gp,
+                              double mu) {
   double s[9];
-  stress_v2(s, avf, pr, G, Gmag, mu);
+  stress_v2(s, avf, pr, G, Gmag, gp, mu);
 #pragma unroll
   for (int q = 0; q < 9; q++) st[q] += s[q];
 }
@@ -213,15 +222,16 @@ __device__ void record(const M5P &P, const double *x, Rec5<NM> &r, double cavf0 
   G[8] = A[6] * A[6] + A[7] * A[7] + A[8] * A[8];
   const double Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
                       G[2] * (G[3] * G[7] - G[4] * G[6]);
+  const GPow gp = has_solid(P) ? gpow(Gmag) : GPow{0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < NM; ++i) {
-    const double are_elast = (P.mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, P.mush[i]) : 0.0;
+    const double are_elast = (P.mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, gp, P.mush[i]) : 0.0;
     const double g = P.gamma[i], pi = P.pinf[i];
     const double den = FACE ? r.avf[i] : avf_temp[i];
     const double press = ((g - 1.0) * (r.are[i] - are_elast) / den) - g * pi;
     r.c2Y[i] = (den / r.r) * g * (press + pi);
     r.ph[i] = press;
-    add_stress_v2(r.st, r.avf[i], r.ph[i], G, Gmag, P.mush[i]);
+    add_stress_v2(r.st, r.avf[i], r.ph[i], G, Gmag, gp, P.mush[i]);
   }
 }
 
@@ -928,6 +938,7 @@ __device__ double dfpe5(const M5P &P, const double *ar, const double *t0, const 
 template <int NM>
 struct RelaxIn {
   double ar[NM], are[NM], avf[NM], avf_init[NM], A[9], G[9], Gmag, rho, u, v, w, rE;
+  GPow gp;
 };
 
 template <int NM>
@@ -946,6 +957,7 @@ __device__ void relax_in(const M5P &P, const double *x, RelaxIn<NM> &R) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) R.A[k] = x[L::I5 + k];
   gmat(R.A, R.G, R.Gmag);
+  R.gp = has_solid(P) ? gpow(R.Gmag) : GPow{0.0, 0.0};
 }
 
 // the re-initialisation after PRELAX 1, 2, 3 (m5_relaxationvec.c:157-189 and
@@ -957,7 +969,7 @@ __device__ int relax_out(const M5P &P, const RelaxIn<NM> &R, double *are, double
     double e_elast = 0.0, are_el[NM];
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv) {
-      are_el[iv] = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, P.mush[iv]) : 0.0;
+      are_el[iv] = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0;
       const double e_elastic_K = are_el[iv] / (R.ar[iv]);
       const double Ys = R.ar[iv] / R.rho;
       e_elast += Ys * e_elastic_K;
@@ -1004,7 +1016,7 @@ __device__ int r3_setup(const M5P &P, RelaxIn<NM> &R, double *pk, double &pr) {
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv) {
       if (R.avf[iv] > eps_sml) {
-        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, P.mush[iv]) : 0.0;
+        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0;
         pk[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
         if (pk[iv] <= -(1.0 - eps_big) * P.pinf[iv] + eps_big) pk[iv] = -(1.0 - eps_big) * P.pinf[iv] + eps_big;
       } else {
@@ -1094,7 +1106,7 @@ __device__ int relax5(const M5P &P, double *x) {
     if (P.prelax == 1) {   // m5_relaxationvec.c:89-190
 #pragma unroll
       for (int iv = 0; iv < NM; ++iv) {
-        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, P.mush[iv]) : 0.0;
+        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0;
         t0[iv] = 1.0 / (R.ar[iv] / R.avf[iv]);
         p0[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
       }
@@ -1108,7 +1120,7 @@ __device__ int relax5(const M5P &P, double *x) {
       int principal = 0;
 #pragma unroll
       for (int iv = 0; iv < NM; ++iv) {
-        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, P.mush[iv]) : 0.0;
+        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0;
         t0[iv] = 1.0 / (R.ar[iv] / R.avf[iv]);
         p0[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
         if (p0[iv] < -(1.0 - eps_big) * P.pinf[iv] + eps_big) p0[iv] = -(1.0 - eps_big) * P.pinf[iv] + 1.0;
@@ -1564,8 +1576,9 @@ __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__
     for (int k = 0; k < 9; ++k) Am[k] = sc.A[k];
     double Gm[9], Gmag;
     gmat(sc.A, Gm, Gmag);
+    const GPow gps = has_solid(P) ? gpow(Gmag) : GPow{0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < NM; ++k) stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, P.mush[k]);
+    for (int k = 0; k < NM; ++k) stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mush[k]);
   }
   const double A1 = Am[0], B1 = Am[3], C1 = Am[6];
   const double A2 = Am[1], B2 = Am[4], C2 = Am[7];
