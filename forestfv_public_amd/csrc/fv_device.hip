@@ -1545,7 +1545,13 @@ __global__ __launch_bounds__(FV_BLOCK) void k_si_bfs_seed(int n, const int *__re
   bfs_append(seed, i, Q, cnt);
 }
 
-// one layer: leaves of Qin (layer lvl-1) claim their unset neighbours for layer lvl
+// one layer: leaves of Qin (layer lvl-1) claim their unset neighbours for layer lvl.
+// A lane gathers its up to 24 neighbour indices, then loads their layers, then
+// claims the unset ones with compare-and-swap, so the memory operations of its
+// candidates overlap instead of forming one chain; the claimed leaves are
+// appended with one queue reservation per wave (an exclusive scan of the
+// lanes' counts).  A leaf's layer is the level of the pass that claims it
+// whichever lane wins, so the result does not depend on the queue order.
 __global__ __launch_bounds__(FV_BLOCK) void k_si_bfs_push(int n, int lvl, const int *__restrict__ nbr,
                                                           const uint32_t *__restrict__ info,
                                                           const int *__restrict__ fine4, int *__restrict__ X,
@@ -1555,25 +1561,49 @@ __global__ __launch_bounds__(FV_BLOCK) void k_si_bfs_push(int n, int lvl, const 
   const int total = *cin;
   const int stride = gridDim.x * blockDim.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) *cnext = 0;   // the counter two layers on is free again
-  // every lane of a wave runs the same number of rounds, so the ballots see full waves
+  const int lane = threadIdx.x & 63;
+  // every lane of a wave runs the same number of rounds, so the scans see full waves
   const int rounds = (total + stride - 1) / stride;
   for (int r = 0; r < rounds; r++) {
     const int t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = t < total;
     const int j = live ? Qin[t] : 0;
     const uint32_t inf = live ? info[j] : 0u;
+    int nb[6];
+#pragma unroll
+    for (int f = 0; f < 6; f++) nb[f] = (live && face_kind(inf, f) != FV_FK_BOUNDARY) ? nbr[(size_t)f * n + j] : -1;
+    int ci[24];
+#pragma unroll
     for (int f = 0; f < 6; f++) {
-      const int kind = face_kind(inf, f);
-      const bool has = live && kind != FV_FK_BOUNDARY;
-      const int nb = has ? nbr[(size_t)f * n + j] : 0;
-      const int ns = (kind == FV_FK_FINE) ? 4 : 1;
+      const bool fine = face_kind(inf, f) == FV_FK_FINE;
+#pragma unroll
       for (int q = 0; q < 4; q++) {
-        const bool cand = has && q < ns;
-        const int i = cand ? ((kind == FV_FK_FINE) ? fine4[4 * nb + q] : nb) : 0;
-        const bool take = cand && X[i] == BFS_INF && atomicCAS(&X[i], BFS_INF, lvl) == BFS_INF;
-        bfs_append(take, i, Qout, cout);
+        if (nb[f] < 0) ci[4 * f + q] = -1;
+        else if (fine) ci[4 * f + q] = fine4[4 * nb[f] + q];
+        else ci[4 * f + q] = (q == 0) ? nb[f] : -1;
       }
     }
+    int xv[24];
+#pragma unroll
+    for (int s = 0; s < 24; s++) xv[s] = (ci[s] >= 0) ? X[ci[s]] : 0;
+    unsigned took = 0u;
+#pragma unroll
+    for (int s = 0; s < 24; s++)
+      if (ci[s] >= 0 && xv[s] == BFS_INF && atomicCAS(&X[ci[s]], BFS_INF, lvl) == BFS_INF) took |= 1u << s;
+    const int cnt = __popc(took);
+    int incl = cnt;   // inclusive scan of the counts over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    const int wsum = __shfl(incl, 63, 64);
+    int base = 0;
+    if (lane == 63 && wsum > 0) base = atomicAdd(cout, wsum);
+    base = __shfl(base, 63, 64) + incl - cnt;
+#pragma unroll
+    for (int s = 0; s < 24; s++)
+      if (took & (1u << s)) Qout[base++] = ci[s];
   }
 }
 
