@@ -29,6 +29,7 @@ using namespace fvk;
 #define MLP_K 112
 #define MLP_H 64
 #define MLP_NO 4
+#define MLP_HP (MLP_H + 2)   // row pitch of the BF16 activations in LDS
 #define MLP_NW (MLP_K * MLP_H + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H * MLP_NO + MLP_NO)
 #define MLP_OFF_B1 (MLP_K * MLP_H)
 #define MLP_OFF_W2 (MLP_OFF_B1 + MLP_H)
@@ -307,7 +308,7 @@ FV_DEV bf16x4 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
 template <typename XA>
 FV_DEV void mlp_infer_wave(long n, long r0, int wv, int lane, XA xa, const uint16_t *__restrict__ w1t,
                            const uint16_t *__restrict__ w2t, const uint16_t *__restrict__ w3t,
-                           const float *__restrict__ wf, float (*hs)[16][MLP_H + 1], float *__restrict__ out) {
+                           const float *__restrict__ wf, uint16_t (*hs)[16][MLP_HP], float *__restrict__ out) {
   const int row = lane & 15, kq = 4 * (lane >> 4);
   const float *b1 = wf + MLP_OFF_B1, *b2 = wf + MLP_OFF_B2, *b3 = wf + MLP_OFF_B3;
   // layer 1: [16 x 112] x [112 x 64]
@@ -315,15 +316,14 @@ FV_DEV void mlp_infer_wave(long n, long r0, int wv, int lane, XA xa, const uint1
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int kt = 0; kt < MLP_K / 16; kt++) {
       const int k0 = kt * 16 + kq;
-      const bf16x4 a = pack4(to_bf16(xa(row, k0)), to_bf16(xa(row, k0 + 1)), to_bf16(xa(row, k0 + 2)),
-                             to_bf16(xa(row, k0 + 3)));
+      const bf16x4 a = pack4(xa(row, k0), xa(row, k0 + 1), xa(row, k0 + 2), xa(row, k0 + 3));
       const uint16_t *wb = w1t + (size_t)(nt * 16 + row) * MLP_K + k0;
       const bf16x4 b = pack4(wb[0], wb[1], wb[2], wb[3]);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
     for (int q = 0; q < 4; q++) {
       const float z = acc[q] + b1[nt * 16 + row];
-      hs[wv][kq + q][nt * 16 + row] = z / (1.f + expf(-z));
+      hs[wv][kq + q][nt * 16 + row] = to_bf16(z / (1.f + expf(-z)));
     }
   }
   __syncthreads();
@@ -332,8 +332,8 @@ FV_DEV void mlp_infer_wave(long n, long r0, int wv, int lane, XA xa, const uint1
   for (int nt = 0; nt < 4; nt++) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int kt = 0; kt < MLP_H / 16; kt++) {
-      const float *ha = &hs[wv][row][kt * 16 + kq];
-      const bf16x4 a = pack4(to_bf16(ha[0]), to_bf16(ha[1]), to_bf16(ha[2]), to_bf16(ha[3]));
+      const uint16_t *ha = &hs[wv][row][kt * 16 + kq];
+      const bf16x4 a = pack4(ha[0], ha[1], ha[2], ha[3]);
       const uint16_t *wb = w2t + (size_t)(nt * 16 + row) * MLP_H + kt * 16 + kq;
       const bf16x4 b = pack4(wb[0], wb[1], wb[2], wb[3]);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
@@ -344,14 +344,14 @@ FV_DEV void mlp_infer_wave(long n, long r0, int wv, int lane, XA xa, const uint1
   for (int nt = 0; nt < 4; nt++)
     for (int q = 0; q < 4; q++) {
       const float z = acc2[nt][q] + b2[nt * 16 + row];
-      hs[wv][kq + q][nt * 16 + row] = z / (1.f + expf(-z));
+      hs[wv][kq + q][nt * 16 + row] = to_bf16(z / (1.f + expf(-z)));
     }
   __syncthreads();
   // layer 3: [16 x 64] x [64 x 16] (4 outputs used)
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
   for (int kt = 0; kt < MLP_H / 16; kt++) {
-    const float *ha = &hs[wv][row][kt * 16 + kq];
-    const bf16x4 a = pack4(to_bf16(ha[0]), to_bf16(ha[1]), to_bf16(ha[2]), to_bf16(ha[3]));
+    const uint16_t *ha = &hs[wv][row][kt * 16 + kq];
+    const bf16x4 a = pack4(ha[0], ha[1], ha[2], ha[3]);
     const uint16_t *wb = w3t + (size_t)row * MLP_H + kt * 16 + kq;
     const bf16x4 b = pack4(wb[0], wb[1], wb[2], wb[3]);
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
@@ -367,12 +367,12 @@ FV_DEV void mlp_infer_wave(long n, long r0, int wv, int lane, XA xa, const uint1
 __global__ __launch_bounds__(256) void k_mlp_infer(long n, const float *__restrict__ X, const uint16_t *__restrict__ w1t,
                                                    const uint16_t *__restrict__ w2t, const uint16_t *__restrict__ w3t,
                                                    const float *__restrict__ wf, float *__restrict__ out) {
-  __shared__ float hs[4][16][MLP_H + 1];
+  __shared__ uint16_t hs[4][16][MLP_HP];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long r0 = ((long)blockIdx.x * 4 + wv) * 16;
   auto xa = [&](int row, int k) {
     const long rr = (r0 + row < n) ? r0 + row : n - 1;
-    return X[rr * MLP_K + k];
+    return to_bf16(X[rr * MLP_K + k]);
   };
   mlp_infer_wave(n, r0, wv, lane, xa, w1t, w2t, w3t, wf, hs, out);
 }
@@ -388,9 +388,14 @@ __global__ __launch_bounds__(256) void k_mlp_infer_stencil(int n, const float4 *
                                                            const uint16_t *__restrict__ w2t,
                                                            const uint16_t *__restrict__ w3t,
                                                            const float *__restrict__ wf, float *__restrict__ out) {
-  __shared__ float hs[4][16][MLP_H + 1];
-  __shared__ float xs[4][16][MLP_K + 4];
-  __shared__ int nix[4][16][28];
+  // the features and hidden activations are kept as the BF16 bits the MFMAs
+  // read, and the neighbour table shares its memory with the activations
+  // (it is dead once the features are written): 23 KB per block
+  __shared__ uint16_t xs[4][16][MLP_K + 8];
+  __shared__ __align__(16) unsigned char nh[(sizeof(int) * 4 * 16 * 28 > sizeof(uint16_t) * 4 * 16 * MLP_HP)
+                                                ? sizeof(int) * 4 * 16 * 28 : sizeof(uint16_t) * 4 * 16 * MLP_HP];
+  int (*nix)[16][28] = reinterpret_cast<int (*)[16][28]>(nh);
+  uint16_t (*hs)[16][MLP_HP] = reinterpret_cast<uint16_t (*)[16][MLP_HP]>(nh);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long r0 = ((long)blockIdx.x * 4 + wv) * 16;
   // four lanes per row; the 26 stencil neighbours are found level by level
@@ -427,12 +432,12 @@ __global__ __launch_bounds__(256) void k_mlp_infer_stencil(int n, const float4 *
       const int j = nb[s];
       const float4 cj = (j == i) ? c0 : fld[j];
       const float pden = fmaxf(fmaxf(fabsf(cj.y), fabsf(c0.y)), 1e-30f);
-      xs[wv][row][4 * s + 0] = clampf(cj.x / c0.x - 1.f);
-      xs[wv][row][4 * s + 1] = clampf((cj.y - c0.y) / pden);
-      xs[wv][row][4 * s + 2] = clampf(cj.z);
-      xs[wv][row][4 * s + 3] = clampf(cj.w * ic);
+      xs[wv][row][4 * s + 0] = to_bf16(clampf(cj.x / c0.x - 1.f));
+      xs[wv][row][4 * s + 1] = to_bf16(clampf((cj.y - c0.y) / pden));
+      xs[wv][row][4 * s + 2] = to_bf16(clampf(cj.z));
+      xs[wv][row][4 * s + 3] = to_bf16(clampf(cj.w * ic));
     }
-    xs[wv][row][MLP_NIN + q] = 0.f;
+    xs[wv][row][MLP_NIN + q] = to_bf16(0.f);
   }
   __syncthreads();
   auto xa = [&](int row, int k) { return xs[wv][row][k]; };
