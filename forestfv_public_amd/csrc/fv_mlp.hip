@@ -407,28 +407,56 @@ __global__ __launch_bounds__(256) void k_mlp_infer_stencil(int n, const float4 *
   const int fx[2] = {3, 1}, fy[2] = {0, 2}, fz[2] = {4, 5};
   auto sidx = [](int dx, int dy, int dz) { return (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1); };
   if (q == 0) nb[13] = i;
-  for (int e = q; e < 6; e += 4) {
+  // fixed trip counts, so that a lane's independent neighbour chains overlap
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const int e = q + 4 * t;
+    if (e >= 6) break;
     const int ax = e >> 1, pm = e & 1, d = pm ? 1 : -1;
     const int f = (ax == 0) ? fx[pm] : (ax == 1 ? fy[pm] : fz[pm]);
     nb[sidx(ax == 0 ? d : 0, ax == 1 ? d : 0, ax == 2 ? d : 0)] = step_face(i, f, n, nbr, info, fine4);
   }
   __syncthreads();
-  for (int e = q; e < 12; e += 4) {
-    const int kind = e >> 2, a = (e & 1) ? 1 : -1, b = (e & 2) ? 1 : -1;
-    if (kind == 0) nb[sidx(a, b, 0)] = step_face(nb[sidx(a, 0, 0)], fy[b > 0], n, nbr, info, fine4);
-    else if (kind == 1) nb[sidx(a, 0, b)] = step_face(nb[sidx(a, 0, 0)], fz[b > 0], n, nbr, info, fine4);
-    else nb[sidx(0, a, b)] = step_face(nb[sidx(0, a, 0)], fz[b > 0], n, nbr, info, fine4);
+  {
+    // the table reads first, then the chains, then the writes
+    int src[3], fc[3], dst[3], res[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      const int e = q + 4 * t;
+      const int kind = e >> 2, a = (e & 1) ? 1 : -1, b = (e & 2) ? 1 : -1;
+      src[t] = (kind == 2) ? nb[sidx(0, a, 0)] : nb[sidx(a, 0, 0)];
+      fc[t] = (kind == 0) ? fy[b > 0] : fz[b > 0];
+      dst[t] = (kind == 0) ? sidx(a, b, 0) : (kind == 1 ? sidx(a, 0, b) : sidx(0, a, b));
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) res[t] = step_face(src[t], fc[t], n, nbr, info, fine4);
+#pragma unroll
+    for (int t = 0; t < 3; t++) nb[dst[t]] = res[t];
   }
   __syncthreads();
-  for (int e = q; e < 8; e += 4) {
-    const int a = (e & 1) ? 1 : -1, b = (e & 2) ? 1 : -1, c = (e & 4) ? 1 : -1;
-    nb[sidx(a, b, c)] = step_face(nb[sidx(a, b, 0)], fz[c > 0], n, nbr, info, fine4);
+  {
+    int src[2], fc[2], dst[2], res[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int e = q + 4 * t;
+      const int a = (e & 1) ? 1 : -1, b = (e & 2) ? 1 : -1, c = (e & 4) ? 1 : -1;
+      src[t] = nb[sidx(a, b, 0)];
+      fc[t] = fz[c > 0];
+      dst[t] = sidx(a, b, c);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) res[t] = step_face(src[t], fc[t], n, nbr, info, fine4);
+#pragma unroll
+    for (int t = 0; t < 2; t++) nb[dst[t]] = res[t];
   }
   __syncthreads();
   {
     const float4 c0 = fld[i];
     const float ic = 1.f / fmaxf(snd[i], 1e-30f);
-    for (int s = q; s < 27; s += 4) {
+#pragma unroll
+    for (int t = 0; t < 7; t++) {
+      const int s = q + 4 * t;
+      if (s >= 27) break;
       const int j = nb[s];
       const float4 cj = (j == i) ? c0 : fld[j];
       const float pden = fmaxf(fmaxf(fabsf(cj.y), fabsf(c0.y)), 1e-30f);
