@@ -243,6 +243,27 @@ def test_interface_layers_match_oracle(case, lrs):
     assert np.array_equal(SIg, SIo)
 
 
+def test_interface_layers_with_frontiers_larger_than_one_push_round():
+    """Blocks of 8^3 cells of alternating fractions on a 64^3 grid: the first
+    layers hold over 100 000 leaves, more than one round of the breadth-first
+    push kernel's fixed grid (65 536 lanes), so the rounds and the per-wave
+    queue reservations of several rounds are exercised."""
+    p, m, S = sod_case(level=7, prelax=1)
+    c = m.centres()
+    blk = np.floor(c * 8).astype(int).sum(axis=1) % 2
+    S[:, 6] = np.where(blk == 0, 0.9, 0.1)   # alpha_0
+    crit = F.make_crit(inter0_on=1, inter0_lvl=7, inter0_lrs=6)
+    SIo = O.OracleMesh(m).interface_location(p, crit, S)
+    sizes = np.bincount(SIo[:, 0])
+    say(f"[interface layers, large frontiers] n={m.n} layer sizes {sizes}")
+    assert sizes[1] > 65536 and sizes[2] > 0
+    dev = F.Device(p, m)
+    dev.set_state(S)
+    for _ in range(3):   # the queues and counters are reused from call to call
+        assert np.array_equal(dev.interface_location(crit), SIo)
+    dev.close()
+
+
 @pytest.mark.parametrize("prec", ["exact", "fp64"])
 def test_interface_location_every_step_as_the_time_loop(prec):
     """loop.c:21-23: a step, then interface_location; the next step's
