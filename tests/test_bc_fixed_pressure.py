@@ -26,7 +26,6 @@ import forestfv_public_amd as F
 import oracle_lib as O
 from cases import sod_case
 
-FRAME = (np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0]), np.array([0.0, 0.0, 1.0]))
 WATER_AIR = dict(gamma=(6.12, 1.4, 1.4), pinf=(3.43e8, 0.0, 0.0))
 
 
@@ -170,3 +169,15 @@ def test_device_fast_paths_match_oracle_over_steps(prec):
     q, _, _ = inlet_outlet_case(level=5)
     q.bc_fixed_p = 0
     assert rel_l1(O.OracleMesh(m).advance(q, S, 20), So).max() > 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bad", ["flag", "pressure"])
+def test_device_rejects_bad_fixed_pressure_settings(bad):
+    p, m, S = inlet_outlet_case(level=3)
+    if bad == "flag":
+        p.bc_fixed_p = 2
+    else:
+        p.p_inlet = float("nan")
+    with pytest.raises(F.FVError):
+        F.Device(p, m)
