@@ -16,8 +16,10 @@ The entry points mirror the reference's per-RUN driver
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -137,6 +139,30 @@ class fv_m5_params(C.Structure):
 
 _lib = None
 
+# Device contexts still open (Device, M5Device, MLP).  They are closed by an
+# atexit hook registered when the library is loaded: Python runs it before the
+# interpreter and the C runtime tear down, so every fv_*_detach / hipFree runs
+# while the HIP runtime (and a profiler's exit handlers) are still alive.  After
+# the hook has run, __del__ does nothing.
+_live = weakref.WeakSet()
+_shutdown = False
+_atexit_registered = False
+
+
+def _close_all():
+    """Close every open device context; registered with atexit by lib()."""
+    global _shutdown
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    _shutdown = True
+
+
+def _track(obj):
+    _live.add(obj)
+
 
 def lib():
     """The product library; raises if it has not been built."""
@@ -185,6 +211,7 @@ def lib():
             "fv_dev_time_stages": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
             "fv_dev_size": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
             "fv_dev_last_error": (C.c_char_p, []),
+            "fv_dev_selftest_guard": (C.c_int, []),
             "fv_dev_stream_copy": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
             "fv_dev_set_limiter_flags": (C.c_int, [C.c_void_p, P(C.c_uint8)]),
             "fv_dev_get_limiter_flags": (C.c_int, [C.c_void_p, P(C.c_uint8)]),
@@ -239,6 +266,10 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        global _atexit_registered
+        if not _atexit_registered:
+            atexit.register(_close_all)
+            _atexit_registered = True
     return _lib
 
 
@@ -370,17 +401,19 @@ class M5Device:
         h = C.c_void_p()
         _check(lib().fv_m5_dev_attach(C.byref(params), C.byref(self.pack.pk), C.byref(h)), "fv_m5_dev_attach")
         self.h = h
+        _track(self)
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
             lib().fv_m5_dev_detach(self.h)
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        if _lib is not None and not _shutdown:
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def set_state(self, S):
         S = np.ascontiguousarray(S, dtype=np.float64)
@@ -608,6 +641,7 @@ class Device:
         h = C.c_void_p()
         _check(lib().fv_dev_attach(C.byref(h), C.byref(params), C.byref(pk.pk)), "fv_dev_attach")
         self._h = h
+        _track(self)
         if prec is not None:
             self.set_precision(prec)
 
@@ -617,7 +651,7 @@ class Device:
             self._h = None
 
     def __del__(self):
-        if _lib is not None:
+        if _lib is not None and not _shutdown:
             self.close()
 
     def reattach(self, mesh):
@@ -775,6 +809,7 @@ class MLP:
         h = C.c_void_p()
         _check(lib().fv_mlp_create(C.byref(h), seed), "fv_mlp_create")
         self._h = h
+        _track(self)
         self.nw = lib().fv_mlp_num_weights()
 
     def close(self):
@@ -783,7 +818,7 @@ class MLP:
             self._h = None
 
     def __del__(self):
-        if _lib is not None:
+        if _lib is not None and not _shutdown:
             self.close()
 
     def weights(self):

@@ -1167,32 +1167,41 @@ static void free_buffers(fv_dev *c) {
 }
 
 // buffers for n leaves and nfine 1:4 faces; reallocates everything (contents
-// lost) only when the capacity is too small
+// lost) only when the capacity is too small.  The new set is allocated beside
+// the old one and swapped in only when every allocation succeeded, so a failed
+// reservation leaves the context with its old buffers, size and state.
 int fv_dev_reserve(fv_dev *c, int n_leaves, int nfine) {
+  if (n_leaves <= c->cap_n && nfine <= c->cap_fine) return FV_OK;
   const size_t n = (size_t)n_leaves;
-  if (n_leaves > c->cap_n || nfine > c->cap_fine) {
-    free_buffers(c);
-    const size_t nf = (size_t)(nfine > 0 ? nfine : 1);
-    const size_t nb = (n + FV_BLOCK - 1) / FV_BLOCK;
-    HIPCHK(hipMalloc(&c->S, sizeof(double) * n * c->neq));
-    HIPCHK(hipMalloc(&c->S2, sizeof(double) * n * c->neq));
-    HIPCHK(hipMalloc(&c->SN, sizeof(double) * n * c->neq));
-    HIPCHK(hipMalloc(&c->SF, sizeof(double) * n * 6 * c->neq));
-    HIPCHK(hipMalloc(&c->part, sizeof(double) * (nb + 1)));
-    HIPCHK(hipMalloc(&c->dtmp, sizeof(double) * 4));
-    HIPCHK(hipMalloc(&c->nbr, sizeof(int) * n * 6));
-    HIPCHK(hipMalloc(&c->fine4, sizeof(int) * nf * 4));
-    HIPCHK(hipMalloc(&c->nanflag, sizeof(int)));
-    HIPCHK(hipMalloc(&c->iflag, 4 * sizeof(int)));
-    HIPCHK(hipMalloc(&c->bfsq, 2 * n * sizeof(int)));
-    HIPCHK(hipMalloc(&c->info, sizeof(uint32_t) * n));
-    HIPCHK(hipMalloc(&c->SIa, sizeof(int) * n * 2));
-    HIPCHK(hipMalloc(&c->SIb, sizeof(int) * n * 2));
-    HIPCHK(hipMalloc(&c->split, sizeof(int) * n));
-    HIPCHK(hipMalloc(&c->lim, n));
-    c->cap_n = n_leaves;
-    c->cap_fine = (int)nf;
+  const size_t nf = (size_t)(nfine > 0 ? nfine : 1);
+  const size_t nb = (n + FV_BLOCK - 1) / FV_BLOCK;
+  fv_dev t;   // only its buffer pointers are used
+  struct { void **p; size_t bytes; } want[] = {
+      {(void **)&t.S, sizeof(double) * n * c->neq},      {(void **)&t.S2, sizeof(double) * n * c->neq},
+      {(void **)&t.SN, sizeof(double) * n * c->neq},     {(void **)&t.SF, sizeof(double) * n * 6 * c->neq},
+      {(void **)&t.part, sizeof(double) * (nb + 1)},     {(void **)&t.dtmp, sizeof(double) * 4},
+      {(void **)&t.nbr, sizeof(int) * n * 6},            {(void **)&t.fine4, sizeof(int) * nf * 4},
+      {(void **)&t.nanflag, sizeof(int)},                {(void **)&t.iflag, 4 * sizeof(int)},
+      {(void **)&t.bfsq, 2 * n * sizeof(int)},           {(void **)&t.info, sizeof(uint32_t) * n},
+      {(void **)&t.SIa, sizeof(int) * n * 2},            {(void **)&t.SIb, sizeof(int) * n * 2},
+      {(void **)&t.split, sizeof(int) * n},              {(void **)&t.lim, n},
+  };
+  const int nw = (int)(sizeof want / sizeof want[0]);
+  for (int k = 0; k < nw; k++) {
+    const hipError_t e = hipMalloc(want[k].p, want[k].bytes);
+    if (e != hipSuccess) {
+      for (int j = 0; j < k; j++) hipFree(*want[j].p);
+      snprintf(fv_g_err, sizeof fv_g_err, "reserve for %d leaves: hipMalloc of %zu bytes: %s (the context keeps its "
+               "%d leaves)", n_leaves, want[k].bytes, hipGetErrorString(e), c->n);
+      return e == hipErrorOutOfMemory ? FV_ERR_NOMEM : FV_ERR_HIP;
+    }
   }
+  free_buffers(c);
+  c->S = t.S; c->S2 = t.S2; c->SN = t.SN; c->SF = t.SF; c->part = t.part; c->dtmp = t.dtmp;
+  c->nbr = t.nbr; c->fine4 = t.fine4; c->nanflag = t.nanflag; c->iflag = t.iflag; c->bfsq = t.bfsq;
+  c->info = t.info; c->SIa = t.SIa; c->SIb = t.SIb; c->split = t.split; c->lim = t.lim;
+  c->cap_n = n_leaves;
+  c->cap_fine = (int)nf;
   return FV_OK;
 }
 
@@ -1201,6 +1210,7 @@ static int upload_mesh(fv_dev *c, const fv_pack *pk) {
   const size_t n = (size_t)pk->n;
   int rc = fv_dev_reserve(c, pk->n, pk->nfine);
   if (rc != FV_OK) return rc;
+  c->broken = 1;   // until every upload below has succeeded
   c->forest = 0;   // the leaf addresses of an earlier fv_dev_set_forest no longer apply
   c->n = pk->n;
   c->nfine = pk->nfine;
@@ -1214,6 +1224,7 @@ static int upload_mesh(fv_dev *c, const fv_pack *pk) {
   HIPCHK(hipMemsetAsync(c->nanflag, 0, sizeof(int), c->stream));
   HIPCHK(hipMemsetAsync(c->lim, 0, n, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  c->broken = 0;
   return FV_OK;
 }
 
@@ -1244,6 +1255,7 @@ extern "C" int fv_dev_attach(fv_dev **out, const fv_params *p, const fv_pack *pk
 }
 
 extern "C" int fv_dev_reattach(fv_dev *c, const fv_pack *pk) {
+  FV_GUARD(c);
   if (!c || !pk) return set_err(FV_ERR_ARG, "null argument");
   return upload_mesh(c, pk);
 }
@@ -1271,6 +1283,7 @@ extern "C" int fv_dev_size(const fv_dev *c, int *n, int *neq) {
 
 // [n][NEQ] <-> [NEQ][n] on the host side of the copy
 extern "C" int fv_dev_set_state(fv_dev *c, const double *S) {
+  FV_GUARD(c);
   if (!c || !S) return set_err(FV_ERR_ARG, "null argument");
   const size_t n = c->n, neq = c->neq;
   double *h = (double *)malloc(sizeof(double) * n * neq);
@@ -1284,6 +1297,7 @@ extern "C" int fv_dev_set_state(fv_dev *c, const double *S) {
 }
 
 extern "C" int fv_dev_get_state(fv_dev *c, double *S) {
+  FV_GUARD(c);
   if (!c || !S) return set_err(FV_ERR_ARG, "null argument");
   const size_t n = c->n, neq = c->neq;
   double *h = (double *)malloc(sizeof(double) * n * neq);
@@ -1299,6 +1313,7 @@ extern "C" int fv_dev_get_state(fv_dev *c, double *S) {
 }
 
 extern "C" int fv_dev_get_faces(fv_dev *c, double *SF) {
+  FV_GUARD(c);
   if (!c || !SF) return set_err(FV_ERR_ARG, "null argument");
   const size_t n = c->n, neq = c->neq;
   double *h = (double *)malloc(sizeof(double) * n * 6 * neq);
@@ -1315,12 +1330,14 @@ extern "C" int fv_dev_get_faces(fv_dev *c, double *SF) {
 }
 
 extern "C" int fv_dev_set_dt(fv_dev *c, double dt) {
+  FV_GUARD(c);
   if (!c) return set_err(FV_ERR_ARG, "null context");
   c->P.dt = dt;
   return FV_OK;
 }
 
 extern "C" int fv_dev_set_precision(fv_dev *c, int prec) {
+  FV_GUARD(c);
   if (!c) return set_err(FV_ERR_ARG, "null context");
   if (prec != FV_PREC_FP64 && prec != FV_PREC_MIXED && prec != FV_PREC_EXACT) return set_err(FV_ERR_ARG, "unknown precision");
   c->prec = prec;
@@ -1328,12 +1345,14 @@ extern "C" int fv_dev_set_precision(fv_dev *c, int prec) {
 }
 
 extern "C" int fv_dev_set_fused(fv_dev *c, int on) {
+  FV_GUARD(c);
   if (!c) return set_err(FV_ERR_ARG, "null context");
   c->fused = on ? 1 : 0;
   return FV_OK;
 }
 
 extern "C" int fv_dev_begin_step(fv_dev *c) {
+  FV_GUARD(c);
   if (!c) return set_err(FV_ERR_ARG, "null context");
   HIPCHK(hipMemcpyAsync(c->SN, c->S, sizeof(double) * (size_t)c->n * c->neq, hipMemcpyDeviceToDevice, c->stream));
   return FV_OK;
@@ -1418,6 +1437,7 @@ static int check_nan(fv_dev *c) {
 }
 
 extern "C" int fv_dev_stage(fv_dev *c, int irk) {
+  FV_GUARD(c);
   if (!c) return set_err(FV_ERR_ARG, "null context");
   int ns = (c->P.rk == FV_RK_SSP3) ? 3 : 2;
   if (irk < 1 || irk > ns) return set_err(FV_ERR_ARG, "stage index out of range");
@@ -1427,6 +1447,7 @@ extern "C" int fv_dev_stage(fv_dev *c, int irk) {
 }
 
 extern "C" int fv_dev_advance(fv_dev *c, int nsteps) {
+  FV_GUARD(c);
   if (!c) return set_err(FV_ERR_ARG, "null context");
   const int ns = (c->P.rk == FV_RK_SSP3) ? 3 : 2;
   for (int s = 0; s < nsteps; s++) {
@@ -1476,6 +1497,7 @@ __global__ void k_totals_final(int nb, int neq, const double *__restrict__ part,
 }
 
 extern "C" int fv_dev_totals(fv_dev *c, double *tot) {
+  FV_GUARD(c);
   if (!c || !tot) return set_err(FV_ERR_ARG, "null argument");
   const int n = c->n, neq = c->neq;
   const int nb = (n + FV_BLOCK - 1) / FV_BLOCK;
@@ -1497,6 +1519,7 @@ extern "C" int fv_dev_totals(fv_dev *c, double *tot) {
 }
 
 extern "C" int fv_dev_cfl_dt(fv_dev *c, double cfl, double *dt) {
+  FV_GUARD(c);
   if (!c || !dt) return set_err(FV_ERR_ARG, "null argument");
   const int n = c->n;
   const int nb = (n + FV_BLOCK - 1) / FV_BLOCK;
@@ -1692,6 +1715,7 @@ int fv_copy_to_host(fv_dev *c, void *dst, const void *src, size_t bytes) {
 }
 
 extern "C" int fv_dev_indicator(fv_dev *c, const fv_crit *cr, int *SI, int *split) {
+  FV_GUARD(c);
   if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
   int *cur = nullptr;
   int rc = fv_dev_indicator_raw(c, cr, &cur);
@@ -1703,6 +1727,7 @@ extern "C" int fv_dev_indicator(fv_dev *c, const fv_crit *cr, int *SI, int *spli
 }
 
 extern "C" int fv_dev_interface_location(fv_dev *c, const fv_crit *cr, int *SI) {
+  FV_GUARD(c);
   if (!c || !cr) return set_err(FV_ERR_ARG, "null argument");
   const int n = c->n;
   int *cur = nullptr;
@@ -1717,12 +1742,14 @@ extern "C" int fv_dev_interface_location(fv_dev *c, const fv_crit *cr, int *SI) 
 }
 
 extern "C" int fv_dev_set_limiter_flags(fv_dev *c, const uint8_t *lim) {
+  FV_GUARD(c);
   if (!c || !lim) return set_err(FV_ERR_ARG, "null argument");
   HIPCHK(hipMemcpy(c->lim, lim, (size_t)c->n, hipMemcpyHostToDevice));
   return FV_OK;
 }
 
 extern "C" int fv_dev_get_limiter_flags(fv_dev *c, uint8_t *lim) {
+  FV_GUARD(c);
   if (!c || !lim) return set_err(FV_ERR_ARG, "null argument");
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(lim, c->lim, (size_t)c->n, hipMemcpyDeviceToHost));
@@ -1731,30 +1758,33 @@ extern "C" int fv_dev_get_limiter_flags(fv_dev *c, uint8_t *lim) {
 
 
 
+// nrep stages back to back (stage 1, 2, .. of the table, cycling); every stage's
+// flux kernel is bracketed by its own event pair, and the host waits only
+// once, after the last stage, so the overall window holds no host sync
 extern "C" int fv_dev_time_stages(fv_dev *c, int nrep, double *ms_stage, double *ms_flux) {
-  if (!c || nrep < 1) return set_err(FV_ERR_ARG, "bad argument");
+  FV_GUARD(c);
+  if (nrep < 1) return set_err(FV_ERR_ARG, "bad argument");
   const int ns = (c->P.rk == FV_RK_SSP3) ? 3 : 2;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  std::vector<hipEvent_t> ev(2 * (size_t)nrep + 2, nullptr);
+  int rc = FV_OK;
+  for (auto &e : ev) {
+    if (hipEventCreate(&e) != hipSuccess) { rc = set_err(FV_ERR_HIP, "hipEventCreate failed"); break; }
+  }
   double tot_flux = 0.0;
-  hipEvent_t a0, a1;
-  HIPCHK(hipEventCreate(&a0));
-  HIPCHK(hipEventCreate(&a1));
-  HIPCHK(hipEventRecord(a0, c->stream));
-  for (int r = 0; r < nrep; r++) {
-    int rc = launch_stage(c, 1 + (r % ns), &e0, &e1);
-    if (rc != FV_OK) return rc;
-    HIPCHK(hipEventSynchronize(e1));
+  float all = 0.f;
+  if (rc == FV_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = set_err(FV_ERR_HIP, "hipEventRecord failed");
+  for (int r = 0; r < nrep && rc == FV_OK; r++) rc = launch_stage(c, 1 + (r % ns), &ev[2 + 2 * r], &ev[3 + 2 * r]);
+  if (rc == FV_OK && hipEventRecord(ev[1], c->stream) != hipSuccess) rc = set_err(FV_ERR_HIP, "hipEventRecord failed");
+  if (rc == FV_OK && hipEventSynchronize(ev[1]) != hipSuccess) rc = set_err(FV_ERR_HIP, "hipEventSynchronize failed");
+  for (int r = 0; r < nrep && rc == FV_OK; r++) {
     float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (hipEventElapsedTime(&ms, ev[2 + 2 * r], ev[3 + 2 * r]) != hipSuccess) rc = set_err(FV_ERR_HIP, "event timing failed");
     tot_flux += ms;
   }
-  HIPCHK(hipEventRecord(a1, c->stream));
-  HIPCHK(hipEventSynchronize(a1));
-  float all = 0.f;
-  HIPCHK(hipEventElapsedTime(&all, a0, a1));
-  hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(a0); hipEventDestroy(a1);
+  if (rc == FV_OK && hipEventElapsedTime(&all, ev[0], ev[1]) != hipSuccess) rc = set_err(FV_ERR_HIP, "event timing failed");
+  for (auto &e : ev)
+    if (e) hipEventDestroy(e);
+  if (rc != FV_OK) return rc;
   if (ms_stage) *ms_stage = all / nrep;
   if (ms_flux) *ms_flux = tot_flux / nrep;
   return check_nan(c);
@@ -1794,3 +1824,44 @@ extern "C" int fv_dev_stream_copy(size_t bytes, int reps, double *gbps) {
 }
 
 extern "C" const char *fv_dev_last_error(void) { return fv_g_err; }
+
+// Host-only self-check of the failure handling of a context (no counterpart in
+// the reference).  1) A context marked broken is refused by the entry points
+// that launch kernels or copy state, before any HIP call.  2) A reservation
+// that cannot be satisfied leaves the context's buffers, capacity and size as
+// they were.  Returns FV_OK when both hold, else the number of the failed check.
+extern "C" int fv_dev_selftest_guard(void) {
+  fv_dev *c = new fv_dev();
+  c->n = 4; c->neq = 9; c->nmat = 2; c->cap_n = 4; c->cap_fine = 1;
+  double buf[64] = {0};
+  int SI[8], split[4], nout = 0;
+  fv_crit cr;
+  memset(&cr, 0, sizeof cr);
+  double ms0 = 0.0, ms1 = 0.0, dt = 0.0;
+  int bad = 0;
+  c->broken = 1;
+  if (fv_dev_stage(c, 1) != FV_ERR_HIP) bad = 1;
+  else if (fv_dev_advance(c, 1) != FV_ERR_HIP) bad = 2;
+  else if (fv_dev_begin_step(c) != FV_ERR_HIP) bad = 3;
+  else if (fv_dev_set_state(c, buf) != FV_ERR_HIP) bad = 4;
+  else if (fv_dev_get_state(c, buf) != FV_ERR_HIP) bad = 5;
+  else if (fv_dev_cfl_dt(c, 0.4, &dt) != FV_ERR_HIP) bad = 6;
+  else if (fv_dev_totals(c, buf) != FV_ERR_HIP) bad = 7;
+  else if (fv_dev_indicator(c, &cr, SI, split) != FV_ERR_HIP) bad = 8;
+  else if (fv_dev_interface_location(c, &cr, SI) != FV_ERR_HIP) bad = 9;
+  else if (fv_dev_regrid(c, &cr, 5, &nout) != FV_ERR_HIP) bad = 10;
+  else if (fv_dev_time_stages(c, 3, &ms0, &ms1) != FV_ERR_HIP) bad = 11;
+  else if (fv_dev_criterionsmth(c, 5, split) != FV_ERR_HIP) bad = 12;
+  if (!bad) {
+    c->broken = 0;
+    double *sentinel = reinterpret_cast<double *>(buf);
+    c->S = sentinel;
+    // INT_MAX leaves need far more than any device holds, so some hipMalloc fails
+    const int rc = fv_dev_reserve(c, 0x7fffffff, 1);
+    if (rc == FV_OK) bad = 13;
+    else if (c->S != sentinel || c->cap_n != 4 || c->n != 4 || c->broken) bad = 14;
+    c->S = nullptr;
+  }
+  delete c;
+  return bad;
+}

@@ -46,7 +46,19 @@ struct fv_dev {
   int forest = 0, cap_leaf = 0;
   int nroot[3] = {1, 1, 1}, bcside[6] = {0, 0, 0, 0, 0, 0};
   int *lroot = nullptr, *llevel = nullptr, *lci = nullptr, *lSI = nullptr;
+  // set while a mesh is being installed (attach, reattach, regrid) and left set
+  // when the install fails part way: the buffers and n may then disagree, so
+  // every later entry point refuses the context instead of launching on it
+  int broken = 0;
 };
+
+// entry-point guard: a null or broken context is an error status, never a launch
+#define FV_GUARD(c)                                                                                    \
+  do {                                                                                                 \
+    if (!(c)) return fv_set_err(FV_ERR_ARG, "null context");                                          \
+    if ((c)->broken)                                                                                   \
+      return fv_set_err(FV_ERR_HIP, "context unusable: an earlier mesh install failed part way");     \
+  } while (0)
 
 // capacity for n leaves and nfine 1:4 faces (reallocates, losing contents, only
 // when too small); defined in fv_device.hip

@@ -660,6 +660,7 @@ static int launch_features(fv_dev *c, fv_mlp *m, float *X, long row0, hipStream_
 
 // append the (features, labels) of every leaf of ctx, labels from the plain indicator
 extern "C" int fv_mlp_sample(fv_dev *c, fv_mlp *m, const fv_crit *cr) {
+  FV_GUARD(c);
   if (!c || !m || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
   int *SIdev = nullptr;
   int rc = fv_dev_indicator_raw(c, cr, &SIdev);
@@ -810,6 +811,7 @@ extern "C" int fv_mlp_infer(fv_mlp *m, const float *X, int B, float *out) {
 // targets as fv_dev_indicator, then the network's targets with the plain
 // refinement forced; the smooth flag becomes the per-leaf limiter.
 extern "C" int fv_dev_indicator_mlp(fv_dev *c, fv_mlp *m, const fv_crit *cr, int *SI, int *split, uint8_t *smooth) {
+  FV_GUARD(c);
   if (!c || !m || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
   int *SIdev = nullptr;
   int rc = fv_dev_indicator_raw(c, cr, &SIdev);

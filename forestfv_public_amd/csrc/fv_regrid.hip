@@ -500,6 +500,7 @@ int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
   if (nn > c->cap_n || nfine > c->cap_fine) {
     if ((rc = fv_dev_reserve(c, nn + nn / 4, (int)(nfine + nfine / 4))) != FV_OK) return rc;
   }
+  c->broken = 1;   // until the new mesh and state are installed
   c->n = nn;
   c->nfine = (int)nfine;
   HIPCHK(hipMemcpyAsync(c->S, S2, sizeof(double) * (size_t)nn * neq, hipMemcpyDeviceToDevice, c->stream));
@@ -520,6 +521,7 @@ int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
   c->lroot = root2; c->llevel = level2; c->lci = ci2; c->lSI = SI2;
   c->cap_leaf = nn;
   T.keep(root2); T.keep(level2); T.keep(ci2); T.keep(SI2);
+  c->broken = 0;
   return FV_OK;
 }
 
@@ -532,6 +534,7 @@ int need_forest(fv_dev *c) {
 }  // namespace
 
 extern "C" int fv_dev_set_forest(fv_dev *c, const fv_mesh *m) {
+  FV_GUARD(c);
   if (!c || !m) return fv_set_err(FV_ERR_ARG, "null argument");
   if (m->n != c->n) return fv_set_err(FV_ERR_ARG, "fv_dev_set_forest: mesh and context differ in leaf count");
   const size_t n = (size_t)m->n;
@@ -556,6 +559,7 @@ extern "C" int fv_dev_set_forest(fv_dev *c, const fv_mesh *m) {
 }
 
 extern "C" int fv_dev_get_forest(fv_dev *c, int *root, int *level, int *ci, int *SI) {
+  FV_GUARD(c);
   int rc = need_forest(c);
   if (rc != FV_OK) return rc;
   const size_t n = (size_t)c->n;
@@ -567,6 +571,7 @@ extern "C" int fv_dev_get_forest(fv_dev *c, int *root, int *level, int *ci, int 
 }
 
 extern "C" int fv_dev_get_ntrees(fv_dev *c, int *ntrees) {
+  FV_GUARD(c);
   int rc = need_forest(c);
   if (rc != FV_OK) return rc;
   if (!ntrees) return fv_set_err(FV_ERR_ARG, "null argument");
@@ -575,6 +580,7 @@ extern "C" int fv_dev_get_ntrees(fv_dev *c, int *ntrees) {
 }
 
 extern "C" int fv_dev_get_pack(fv_dev *c, int *nbr, uint32_t *info, int *nfine, int *fine4) {
+  FV_GUARD(c);
   if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
   const size_t n = (size_t)c->n;
   int rc;
@@ -587,6 +593,7 @@ extern "C" int fv_dev_get_pack(fv_dev *c, int *nbr, uint32_t *info, int *nfine, 
 }
 
 extern "C" int fv_dev_criterionsmth(fv_dev *c, int LEVEL, int *split) {
+  FV_GUARD(c);
   int rc = need_forest(c);
   if (rc != FV_OK) return rc;
   if (!split) return fv_set_err(FV_ERR_ARG, "null argument");
@@ -599,6 +606,7 @@ extern "C" int fv_dev_criterionsmth(fv_dev *c, int LEVEL, int *split) {
 }
 
 extern "C" int fv_dev_regrid(fv_dev *c, const fv_crit *cr, int maxlevel, int *n_out) {
+  FV_GUARD(c);
   int rc = need_forest(c);
   if (rc != FV_OK) return rc;
   if (!cr) return fv_set_err(FV_ERR_ARG, "null argument");

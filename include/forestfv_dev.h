@@ -164,6 +164,13 @@ int  fv_dev_totals(fv_dev *ctx, double *tot);
 /* Number of leaves and NEQ of the attached mesh. */
 int  fv_dev_size(const fv_dev *ctx, int *n, int *neq);
 
+/* Host-only self-check with no counterpart in the reference: a context left
+ * unusable by a failed mesh install is refused by the entry points (status
+ * FV_ERR_HIP, no kernel launched), and a device reservation that cannot be
+ * satisfied leaves the context's buffers and size unchanged.  FV_OK when both
+ * hold. */
+int  fv_dev_selftest_guard(void);
+
 /* Message of the last failing call on this thread. */
 const char *fv_dev_last_error(void);
 

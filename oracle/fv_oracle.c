@@ -1125,6 +1125,18 @@ void orc_hllc_state(const orc_params *p, const double *sfL, const double *sfR,
   ustar[0] = W.u_nc; ustar[1] = W.v_nc; ustar[2] = W.w_nc;
 }
 
+void orc_bc_ghost(const orc_params *p, const double *sf, const double *n, const double *t1, const double *t2,
+                  int code, double *uvw, double *vec) {
+  layout_t L = mk_layout(p->nmat);
+  frame_t F = {n[0], n[1], n[2], t1[0], t1[1], t1[2], t2[0], t2[1], t2[2]};
+  sol_t sL, sR;
+  fill_sol(&L, p, sf, sf, &sL);
+  sR = sL;
+  apply_bc(&L, p, code, &F, &sL, &sR);
+  uvw[0] = sR.u; uvw[1] = sR.v; uvw[2] = sR.w;
+  for (int iv = 0; iv < L.neq; ++iv) vec[iv] = sR.vec[iv];
+}
+
 void orc_physical_flux(const orc_params *p, const double *vec, const double *n, double *flux) {
   layout_t L = mk_layout(p->nmat);
   sol_t s;

@@ -140,6 +140,23 @@ int  orc_interface_location(const orc_mesh *m, const orc_params *p, const orc_cr
 int  orc_criterionsplt(const orc_mesh *m, const orc_params *p, const orc_crit *c,
                        const double *S, const int *SI, int *split);
 
+/* criterionsmth (criterionsmth.c:13-56): LEVEL + 1 passes of sibling consensus,
+ * the one-level refinement cap and 2:1 smoothing, in place on split [n], in
+ * leaf order as the reference's list walk updates it. */
+int  orc_criterionsmth(const orc_mesh *m, int LEVEL, int *split);
+/* refineh (refineh.c:3-61; merge.c:20-55; split.c:425-434): merge the families
+ * whose eight leaf kids all coarsen, then split the leaves with split > level;
+ * the new leaves in DFS order with their state [NEQ], SI [2] and split.  The
+ * outputs hold up to 8 n leaves; *nout receives the count. */
+int  orc_refineh(const orc_mesh *m, const orc_params *p, const int *split, const double *S, int *nout,
+                 int *root_o, int *level_o, int *ci_o, double *S_o, int *SI_o, int *split_o);
+/* The boundary ghost of a face state (m4_bc.c:3-34 with bc_symmetry.c,
+ * bc_reflective.c, bc_wall.c): the ghost velocity u,v,w [3] and the ghost's
+ * conservative vector [NEQ] (its vec, which the published code leaves
+ * unmirrored) for BC code `code` on a face with frame n, t1, t2. */
+void orc_bc_ghost(const orc_params *p, const double *sf, const double *n, const double *t1, const double *t2,
+                  int code, double *uvw, double *vec);
+
 /* Largest stable time step dt = cfl * min_cells h / max_faces(|u.n| + c). */
 double orc_cfl_dt(const orc_mesh *m, const orc_cell *cells, const orc_params *p,
                   const double *S, double cfl);

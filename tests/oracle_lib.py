@@ -146,6 +146,11 @@ def lib():
         L.orc_m5_advance.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), dp, C.c_int]
         L.orc_m5_face.argtypes = [P(orc_m5_params), dp, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp]
         L.orc_m5_plastic.argtypes = [P(orc_m5_params), dp]
+        ip = P(C.c_int)
+        L.orc_criterionsmth.argtypes = [P(orc_mesh), C.c_int, ip]
+        L.orc_refineh.argtypes = [P(orc_mesh), P(orc_params), ip, dp, ip, ip, ip, ip, dp, ip, ip]
+        L.orc_bc_ghost.argtypes = [P(orc_params), dp, dp, dp, dp, C.c_int, dp, dp]
+        L.orc_bc_ghost.restype = None
         _lib = L
     return _lib
 
@@ -314,6 +319,30 @@ class OracleMesh:
                                 _dp(np.ascontiguousarray(S)), _ip(SI), _ip(split))
         return split
 
+    def criterionsmth(self, LEVEL, split):
+        """orc_criterionsmth: the smoothed target levels (a new array)."""
+        sp = np.ascontiguousarray(split, dtype=np.int32).copy()
+        rc = lib().orc_criterionsmth(C.byref(self.om), int(LEVEL), _ip(sp))
+        if rc != 0:
+            raise RuntimeError(f"orc_criterionsmth status {rc}")
+        return sp
+
+    def refineh(self, params, split, S):
+        """orc_refineh: dict of the new leaves (root, level, ci, S, SI, split) in DFS order."""
+        nq = 3 * params.nmat + 3
+        cap = 8 * self.n
+        sp = np.ascontiguousarray(split, dtype=np.int32)
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        out = {"root": np.zeros(cap, np.int32), "level": np.zeros(cap, np.int32), "ci": np.zeros((cap, 3), np.int32),
+               "S": np.zeros((cap, nq)), "SI": np.zeros((cap, 2), np.int32), "split": np.zeros(cap, np.int32)}
+        nout = C.c_int()
+        rc = lib().orc_refineh(C.byref(self.om), C.byref(params_from(params)), _ip(sp), _dp(S), C.byref(nout),
+                               _ip(out["root"]), _ip(out["level"]), _ip(out["ci"]), _dp(out["S"]), _ip(out["SI"]),
+                               _ip(out["split"]))
+        if rc != 0:
+            raise RuntimeError(f"orc_refineh status {rc}")
+        return {k: v[:nout.value].copy() for k, v in out.items()}
+
     def cfl_dt(self, params, S, cfl):
         return lib().orc_cfl_dt(C.byref(self.om), self.cells, C.byref(params_from(params)),
                                 _dp(np.ascontiguousarray(S)), cfl)
@@ -367,17 +396,28 @@ def hllc(params, sfL, sfR, n, t1, t2, bc=0):
     flux = np.zeros(nq)
     us = np.zeros(3)
     lib().orc_hllc_state(C.byref(params_from(params)), _dp(np.ascontiguousarray(sfL, dtype=np.float64)),
-                         _dp(np.ascontiguousarray(sfR, dtype=np.float64)), _dp(np.asarray(n, dtype=np.float64)),
-                         _dp(np.asarray(t1, dtype=np.float64)), _dp(np.asarray(t2, dtype=np.float64)), bc,
+                         _dp(np.ascontiguousarray(sfR, dtype=np.float64)), _dp(np.ascontiguousarray(n, dtype=np.float64)),
+                         _dp(np.ascontiguousarray(t1, dtype=np.float64)), _dp(np.ascontiguousarray(t2, dtype=np.float64)), bc,
                          _dp(flux), _dp(us))
     return flux, us
+
+
+def bc_ghost(params, sf, n, t1, t2, code):
+    """orc_bc_ghost: (ghost velocity [3], ghost conservative vector [NEQ])."""
+    nq = 3 * params.nmat + 3
+    uvw = np.zeros(3)
+    vec = np.zeros(nq)
+    lib().orc_bc_ghost(C.byref(params_from(params)), _dp(np.ascontiguousarray(sf, dtype=np.float64)),
+                       _dp(np.ascontiguousarray(n, dtype=np.float64)), _dp(np.ascontiguousarray(t1, dtype=np.float64)),
+                       _dp(np.ascontiguousarray(t2, dtype=np.float64)), int(code), _dp(uvw), _dp(vec))
+    return uvw, vec
 
 
 def physical_flux(params, vec, n):
     nq = 3 * params.nmat + 3
     flux = np.zeros(nq)
     lib().orc_physical_flux(C.byref(params_from(params)), _dp(np.ascontiguousarray(vec, dtype=np.float64)),
-                            _dp(np.asarray(n, dtype=np.float64)), _dp(flux))
+                            _dp(np.ascontiguousarray(n, dtype=np.float64)), _dp(flux))
     return flux
 
 

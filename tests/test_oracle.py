@@ -449,3 +449,319 @@ def test_openmp_variant_is_bit_identical(case):
     A = om.advance(p, S, 3)
     B = om.advance_par(p, S, 3, 4)
     assert np.array_equal(A, B)
+
+
+# ---------------------------------------------------------------- MC limiter (new)
+def mc_phi(r):
+    """Van Leer's monotonised central limiter in its ratio form, phi(r) = max(0, min(2r, (1+r)/2, 2))."""
+    return np.maximum(0.0, np.minimum(np.minimum(2.0 * r, 0.5 * (1.0 + r)), 2.0))
+
+
+def test_muscl_mc_brute_force_on_a_line():
+    """limiter = 1: face states of a 1-D line of cells against phi(r) b with r = a / b, the
+    ratio of the backward to the forward difference -- a different form from the oracle's
+    minmod(2a, 2b, (a+b)/2).  Every branch (2r, (1+r)/2, 2, 0) occurs on the data."""
+    p = sod_params(prelax=0, bc=(2, 2, 4, 4, 4, 4), limiter=1)
+    m = F.Mesh.uniform((40, 1, 1), (0, 0, 0), 1.0, 1, list(p.bc))
+    rng = np.random.default_rng(81)
+    x = m.centres()[:, 0]
+    S = np.zeros((m.n, 9))
+    rho = 1 + 0.5 * np.sin(0.7 * x) + 0.3 * rng.random(m.n)
+    S[:, 0] = rho * (1 - 1e-8); S[:, 1] = rho * 1e-8
+    S[:, 2] = rng.normal(size=m.n); S[:, 5] = 3 + rng.random(m.n)
+    S[:, 6] = 1 - 1e-8; S[:, 7] = S[:, 5] - 1e-8; S[:, 8] = 1e-8
+    SF = O.OracleMesh(m).faceinterp(p, S)
+    branches = set()
+    for i in range(1, m.n - 1):
+        for f, sgn in ((1, +1), (3, -1)):
+            fwd = S[i + sgn] - S[i]
+            bwd = S[i - sgn] - S[i]
+            for v in (0, 2, 5):
+                a, b = -bwd[v], fwd[v]
+                r = a / b
+                phi = float(mc_phi(r))
+                exp = S[i, v] + 0.5 * phi * b
+                branches.add(int(np.argmin([2 * r, 0.5 * (1 + r), 2.0])) if r > 0 else 3)
+                assert abs(SF[i, f, v] - exp) <= 1e-14 * (abs(exp) + abs(b)), (i, f, v, r)
+    assert branches == {0, 1, 2, 3}
+
+
+@pytest.mark.parametrize("limiter", [0, 1])
+def test_muscl_reproduces_linear_data(limiter):
+    """Both limiters are exact on linear data: interior face states of a linear density and
+    energy profile are the profile's values at the faces."""
+    p = sod_params(prelax=0, bc=(2, 2, 4, 4, 4, 4), limiter=limiter)
+    m = F.Mesh.uniform((16, 1, 1), (0, 0, 0), 1.0, 1, list(p.bc))
+    x = m.centres()[:, 0]
+    S = np.zeros((m.n, 9))
+    rho = 1.0 + 0.1 * x
+    S[:, 0] = rho * (1 - 1e-8); S[:, 1] = rho * 1e-8
+    S[:, 5] = 2.0 + 0.25 * x
+    S[:, 6] = 1 - 1e-8; S[:, 7] = S[:, 5] - 1e-8; S[:, 8] = 1e-8
+    SF = O.OracleMesh(m).faceinterp(p, S)
+    for i in range(1, m.n - 1):
+        for f, dx in ((1, 0.5), (3, -0.5)):
+            assert SF[i, f, 0] == pytest.approx((1.0 + 0.1 * (x[i] + dx)) * (1 - 1e-8), rel=1e-14)
+            assert SF[i, f, 5] == pytest.approx(2.0 + 0.25 * (x[i] + dx), rel=1e-14)
+
+
+# ---------------------------------------------------------------- boundary codes 1..5
+def random_frame(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 2] = -q[:, 2]
+    return q[:, 0].copy(), q[:, 1].copy(), q[:, 2].copy()
+
+
+def prim_state(p, alpha, rho_k, u, pres):
+    """Conservative two-material vector from primitives, in closed form."""
+    G, Pi = np.array(p.gamma[:2]), np.array(p.pinf[:2])
+    a = np.array([alpha, 1.0 - alpha])
+    ar = a * np.array(rho_k)
+    r = ar.sum()
+    are = a * (pres + G * Pi) / (G - 1.0)
+    u = np.asarray(u, dtype=float)
+    return np.concatenate([ar, r * u, [are.sum() + 0.5 * r * (u @ u)], [alpha], are])
+
+
+def test_bc_ghost_velocity_closed_forms():
+    """Ghost velocity of each BC code (bc_symmetry.c:19-29, bc_reflective.c:12-28,
+    bc_wall.c:17-25): the reflection u - 2 (u.n) n for symmetry (1) and reflective (4)
+    faces, -u for the no-slip wall (5), u itself for the outlet (2) and inlet (3); the
+    ghost's conservative vector is the own one, unmirrored (bc_reflective.c:26-28)."""
+    p = water_air_params(prelax=0)
+    rng = np.random.default_rng(12)
+    for _ in range(40):
+        n, t1, t2 = random_frame(rng)
+        u = rng.normal(size=3) * 30.0
+        sf = prim_state(p, rng.uniform(0.1, 0.9), (1000.0, 1.2), u, rng.uniform(1e5, 1e7))
+        for code in (1, 2, 3, 4, 5):
+            g, vec = O.bc_ghost(p, sf, n, t1, t2, code)
+            exp = {1: u - 2 * (u @ n) * n, 4: u - 2 * (u @ n) * n, 5: -u, 2: u, 3: u}[code]
+            assert np.allclose(g, exp, rtol=0, atol=1e-12 * np.abs(u).max()), code
+            assert np.array_equal(vec, sf)
+
+
+@pytest.mark.parametrize("code", [1, 4, 5])
+def test_wall_faces_carry_no_mass_and_the_resting_pressure(code):
+    """Through symmetry, reflective and no-slip faces: the star velocity and the mass and
+    volume-fraction fluxes vanish for any moving state, the energy flux too with the
+    consistent star energy; a resting state gives the momentum flux p n.  The tangential
+    momentum flux vanishes at symmetry and reflective faces, whose ghost keeps the tangential
+    velocity.  At a no-slip face it depends on which star region the rounding-level u*
+    selects, since the ghost's conservative momentum stays unmirrored (bc_wall.c:17-25,
+    m4_hllc.c:30-48), so it is not pinned there."""
+    p = water_air_params(prelax=0)
+    rng = np.random.default_rng(13 + code)
+    for _ in range(30):
+        n, t1, t2 = random_frame(rng)
+        pres = rng.uniform(1e5, 1e7)
+        u = rng.normal(size=3) * 20.0
+        sf = prim_state(p, rng.uniform(0.1, 0.9), (1000.0, 1.2), u, pres)
+        r = sf[0] + sf[1]
+        flux, us = O.hllc(p, sf, sf, n, t1, t2, bc=code)
+        c = np.sqrt(6.12 * (pres + 3.43e8) / 1000.0)
+        scale = r * c
+        assert np.abs(us @ n) <= 1e-12 * c
+        assert abs(flux[0]) + abs(flux[1]) <= 1e-10 * scale
+        mom = flux[2:5]
+        if code != 5:
+            assert abs(mom @ t1) + abs(mom @ t2) <= 1e-10 * scale * np.abs(u).max()
+        assert abs(flux[6]) <= 1e-12 * c
+        assert abs(flux[5]) <= 1e-9 * scale * c * np.abs(u).max()
+        rest = prim_state(p, 0.5, (1000.0, 1.2), (0.0, 0.0, 0.0), pres)
+        fr, _ = O.hllc(p, rest, rest, n, t1, t2, bc=code)
+        assert np.allclose(fr[2:5], pres * n, rtol=1e-13, atol=1e-13 * pres)
+        # zero up to the rounding of rho* = rho S_K / S_K and of rE* (one ulp times S_K)
+        assert abs(fr[0]) + abs(fr[1]) <= 1e-13 * rest[0:2].sum() * c
+        assert abs(fr[5]) <= 1e-13 * rest[5] * c
+
+
+# ---------------------------------------------------------------- CFL (new)
+def closed_form_dt(p, m, states, cfl):
+    lv = m.arrays()["level"]
+    h = m.m.rootsize / (2.0 ** (lv - 1))
+    lam = []
+    for st in states:
+        alpha, rk, u, pres = st
+        a = np.array([alpha, 1 - alpha])
+        r = (a * np.array(rk)).sum()
+        c = np.sqrt((a * np.array(p.gamma[:2]) * (pres + np.array(p.pinf[:2]))).sum() / r)
+        lam.append(np.abs(u).max() + c)
+    return cfl * np.min(h / np.array(lam))
+
+
+def test_cfl_closed_form_uniform_and_two_level():
+    """dt = CFL min_i h_i / (max_k |u_k| + c_i) with the frozen mixture sound speed
+    c^2 = sum_k alpha_k gamma_k (p + p_inf_k) / rho, on a uniform mesh with one state and on a
+    two-level forest where every leaf has its own state."""
+    p = water_air_params(prelax=1)
+    m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), 4e-3, 4, list(p.bc))
+    st = (0.3, (1000.0, 1.2), np.array([12.0, -40.0, 3.0]), 5e6)
+    S = np.tile(prim_state(p, *st), (m.n, 1))
+    dt = O.OracleMesh(m).cfl_dt(p, S, 0.4)
+    h = 4e-3 / 8
+    c = np.sqrt((0.3 * 6.12 * (5e6 + 3.43e8) + 0.7 * 1.4 * 5e6) / (0.3 * 1000 + 0.7 * 1.2))
+    assert dt == pytest.approx(0.4 * h / (40.0 + c), rel=1e-14)
+    m2 = random_amr_mesh(p, base_level=2, max_level=3, seed=5, frac=0.4, passes=1, L=4e-3)
+    assert set(m2.arrays()["level"].tolist()) == {2, 3}
+    rng = np.random.default_rng(14)
+    states = []
+    for i in range(m2.n):
+        states.append((rng.uniform(0.05, 0.95), (1000.0, 1.2), rng.normal(size=3) * 100.0, rng.uniform(1e5, 1e7)))
+    S2 = np.array([prim_state(p, *s) for s in states])
+    dt2 = O.OracleMesh(m2).cfl_dt(p, S2, 0.5)
+    assert dt2 == pytest.approx(closed_form_dt(p, m2, states, 0.5), rel=1e-13)
+
+
+# ---------------------------------------------------------------- criterionsmth / refineh
+def test_criterionsmth_hand_worked_4cube():
+    """A 4^3 block of level-3 leaves (one root), LEVEL 4.  Leaf A = (1,0,0) asks for level 5,
+    every other leaf for level 2.  Worked through criterionsmth.c:13-56 by hand:
+      - A is capped at one level finer: 4;
+      - A's own family (the level-2 block (0,0,0)) cannot merge, so its seven other leaves keep 3;
+      - (2,0,0) faces A's level-4 kids, so the 2:1 step lifts it to 3 in pass 0; its family
+        (block (1,0,0)) then no longer agrees and keeps 3 in pass 1;
+      - the six other level-2 blocks merge: 2."""
+    p = sod_params()
+    m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), 1.0, 3, list(p.bc))
+    a = m.arrays()
+    ci = a["ci"]
+    split = np.full(m.n, 2, np.int32)
+    iA = [i for i in range(m.n) if tuple(ci[i]) == (1, 0, 0)][0]
+    split[iA] = 5
+    out = O.OracleMesh(m).criterionsmth(4, split)
+    blk = ci >> 1
+    for i in range(m.n):
+        b = tuple(blk[i])
+        if i == iA:
+            exp = 4
+        elif b in ((0, 0, 0), (1, 0, 0)):
+            exp = 3
+        else:
+            exp = 2
+        assert out[i] == exp, (tuple(ci[i]), out[i], exp)
+
+
+def brute_balanced(root, level, ci, nroot):
+    """No two face-adjacent leaves differ by more than one level (integer boxes at the finest level)."""
+    D = 1 << (int(level.max()) - 1)
+    rx = np.stack([root % nroot[0], (root // nroot[0]) % nroot[1], root // (nroot[0] * nroot[1])], 1)
+    size = D >> (level - 1)
+    lo = rx * D + ci * size[:, None]
+    hi = lo + size[:, None]
+    n = len(level)
+    for i in range(n):
+        for ax in range(3):
+            o = [k for k in range(3) if k != ax]
+            touch = (lo[:, ax] == hi[i, ax]) | (hi[:, ax] == lo[i, ax])
+            ov = np.ones(n, dtype=bool)
+            for k in o:
+                ov &= (lo[:, k] < hi[i, k]) & (hi[:, k] > lo[i, k])
+            nb = np.nonzero(touch & ov)[0]
+            if len(nb) and np.abs(level[nb] - level[i]).max() > 1:
+                return False
+    return True
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_criterionsmth_properties_on_random_forests(seed):
+    """On random AMR forests with targets from 3 levels coarser to 3 finer:
+      - no target ends more than one level finer, nor more than one coarser;
+      - a leaf ends coarser only inside a family of eight leaves that all end coarser;
+      - the mesh that refineh builds from the result is 2:1 face balanced (brute force);
+      - the host product's fv_criterionsmth gives the same targets, bit for bit."""
+    p = sod_params()
+    m = random_amr_mesh(p, base_level=2, max_level=5, seed=seed, frac=0.3, passes=3)
+    a = m.arrays()
+    lvl = a["level"]
+    rng = np.random.default_rng(100 + seed)
+    om = O.OracleMesh(m)
+    for trial in range(3):
+        split = np.clip(lvl + rng.integers(-3, 4, size=m.n), 1, 8).astype(np.int32)
+        out = om.criterionsmth(6, split)
+        assert np.all(out <= lvl + 1) and np.all(out >= lvl - 1)
+        fam = {}
+        for i in range(m.n):
+            if lvl[i] > 1:
+                fam.setdefault((a["root"][i], lvl[i], *(a["ci"][i] >> 1)), []).append(i)
+        for i in np.nonzero(out < lvl)[0]:
+            f = fam[(a["root"][i], lvl[i], *(a["ci"][i] >> 1))]
+            assert len(f) == 8 and np.all(out[f] < lvl[f])
+        new = om.refineh(p, out, np.tile([1.0, 1e-8, 0, 0, 0, 2.5, 1 - 1e-8, 2.5, 1e-8], (m.n, 1)))
+        assert brute_balanced(new["root"], new["level"], new["ci"], (1, 1, 1))
+        assert np.array_equal(m.criterionsmth(6, split), out)
+
+
+def dfs_key(root, level, ci, L):
+    """Sort key of the DFS leaf order: root, then the Morton code (x fastest) of the
+    leaf's first cell at level L."""
+    sh = L - level
+    keys = []
+    for r, s, c in zip(root, sh, ci):
+        x, y, z = (int(v) << int(s) for v in c)
+        k = 0
+        for b in range(L):
+            k |= ((x >> b) & 1) << (3 * b) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b + 2)
+        keys.append((int(r), k))
+    return keys
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_refineh_order_tiling_state_and_host_equality(seed):
+    """orc_refineh on random forests and targets smoothed by criterionsmth:
+      - the new leaves are in DFS order (strictly increasing (root, Morton) key);
+      - they tile the domain: volumes sum to the domain's, with no overlap (a 2:1 check
+        over integer boxes would fail on overlaps);
+      - a split kid's state is its parent's, bit for bit; a merged leaf's partial masses,
+        momentum and energy are the mean of its kids (relaxation leaves them alone);
+      - the host product's fv_mesh_adapt gives the same leaves, state and SI, bit for bit."""
+    p = sod_params(prelax=1)
+    m = random_amr_mesh(p, base_level=2, max_level=5, nroot=(2, 1, 1), seed=seed, frac=0.3, passes=2, L=2.0)
+    a = m.arrays()
+    lvl = a["level"]
+    rng = np.random.default_rng(200 + seed)
+    S = np.tile([1.0, 1e-8, 0.0, 0.0, 0.0, 2.5, 1 - 1e-8, 2.5, 1e-8], (m.n, 1))
+    S[:, 0] *= 1 + 0.2 * rng.random(m.n)
+    S[:, 2] = rng.normal(size=m.n)
+    S[:, 5] = 2.5 + 0.5 * rng.random(m.n) + 0.5 * S[:, 2] ** 2 / S[:, 0]
+    S[:, 7] = S[:, 5] - 0.5 * S[:, 2] ** 2 / S[:, 0] - 1e-8
+    SI = np.zeros((m.n, 2), np.int32)
+    SI[:, 0] = rng.integers(0, 3, size=m.n)
+    m.set_SI(SI)
+    om = O.OracleMesh(m, SI=SI)
+    # random targets in the first root, coarsening everywhere in the second (whole families agree)
+    want = np.where(a["root"] == 0, lvl + rng.integers(-2, 3, size=m.n), lvl - 1)
+    split = om.criterionsmth(6, np.clip(want, 1, 6).astype(np.int32))
+    new = om.refineh(p, split, S)
+    L = int(new["level"].max())
+    keys = dfs_key(new["root"], new["level"], new["ci"], L)
+    assert all(keys[k] < keys[k + 1] for k in range(len(keys) - 1))
+    vol = (0.5 ** (3 * (new["level"] - 1))).sum()
+    assert vol == 2.0
+    assert brute_balanced(new["root"], new["level"], new["ci"], (2, 1, 1))
+    # parents of the new leaves, and the old leaves by address
+    old = {(a["root"][i], lvl[i], *a["ci"][i]): i for i in range(m.n)}
+    nmerged = nsplit = 0
+    for k in range(len(new["level"])):
+        key = (new["root"][k], new["level"][k], *new["ci"][k])
+        if key in old:
+            assert np.array_equal(new["S"][k], S[old[key]])
+            continue
+        par = (new["root"][k], new["level"][k] - 1, *(new["ci"][k] >> 1))
+        if par in old:
+            nsplit += 1
+            assert np.array_equal(new["S"][k], S[old[par]])
+            assert new["SI"][k, 0] == SI[old[par], 0] and new["SI"][k, 1] == 0
+            continue
+        kids = [old[(new["root"][k], new["level"][k] + 1, *(2 * new["ci"][k] + np.array([q & 1, (q >> 1) & 1, q >> 2])))]
+                for q in range(8)]
+        nmerged += 1
+        assert np.allclose(new["S"][k, :6], S[kids, :6].mean(0), rtol=1e-15, atol=0)
+        assert np.array_equal(new["SI"][k], [0, 0])
+    assert nmerged > 0 and nsplit > 0
+    S_host = m.adapt(p, split, S)
+    b = m.arrays()
+    assert np.array_equal(b["root"], new["root"]) and np.array_equal(b["level"], new["level"])
+    assert np.array_equal(b["ci"], new["ci"]) and np.array_equal(b["SI"], new["SI"])
+    assert np.array_equal(S_host, new["S"])

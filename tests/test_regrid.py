@@ -183,3 +183,57 @@ def test_device_regrid_is_much_faster_than_the_host_path():
     assert td < th
     devh.close()
     devd.close()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_device_criterionsmth_matches_oracle(seed):
+    """fv_dev_criterionsmth against orc_criterionsmth (criterionsmth.c:13-56 restated in
+    oracle/, pinned by a hand-worked case and by its cap, consensus and 2:1 properties)."""
+    import oracle_lib as O
+    rng = np.random.default_rng(50 + seed)
+    p = sod_params()
+    m = random_amr_mesh(p, base_level=2, max_level=5, seed=seed, frac=0.3, passes=3)
+    lvl = m.arrays()["level"]
+    om = O.OracleMesh(m)
+    dev = F.Device(p, m)
+    dev.set_forest(m)
+    for trial in range(3):
+        split = np.clip(lvl + rng.integers(-3, 4, size=m.n), 1, 8).astype(np.int32)
+        want = om.criterionsmth(6, split)
+        got = dev.criterionsmth(6, split)
+        say(f"[criterionsmth vs oracle] seed={seed} trial={trial} n={m.n} changed={(want != split).sum()}")
+        assert np.array_equal(got, want)
+    dev.close()
+
+
+@pytest.mark.parametrize("name", ["bubble_prelax1", "three_mat_prelax0"])
+def test_device_regrid_matches_oracle(name):
+    """fv_dev_regrid against the oracle's meshadaptation chain: orc_interface_location,
+    orc_criterionsplt, orc_criterionsmth and orc_refineh (refineh.c, merge.c, split.c).
+    Leaf addresses, SI and the remapped state are equal, bit for bit, over three cycles."""
+    import oracle_lib as O
+    (p, m, S), maxlevel = CASES[name]()
+    crit = F.make_crit(uniform_on=1, uniform_lvl=3, prs_on=1, prs_lvl=maxlevel, prs_thr=0.05,
+                       inter0_on=1, inter0_lvl=maxlevel, inter0_lrs=3)
+    dev = F.Device(p, m)
+    dev.set_state(S)
+    dev.set_forest(m)
+    mo = m
+    for cycle in range(3):
+        Sd = dev.get_state()
+        om = O.OracleMesh(mo)
+        SI = om.interface_location(p, crit, Sd)
+        om.set_SI(SI)
+        split = om.criterionsplt(p, crit, Sd, SI)
+        split = om.criterionsmth(maxlevel, split)
+        new = om.refineh(p, split, Sd)
+        nd = dev.regrid(crit, maxlevel)
+        root, level, ci, SId = dev.get_forest()
+        say(f"[regrid vs oracle] {name} cycle {cycle}: oracle {len(new['level'])} device {nd} leaves")
+        assert nd == len(new["level"])
+        assert np.array_equal(root, new["root"]) and np.array_equal(level, new["level"])
+        assert np.array_equal(ci, new["ci"]) and np.array_equal(SId, new["SI"])
+        assert np.array_equal(dev.get_state(), new["S"])
+        mo = F.Mesh.from_leaves((1, 1, 1), (0.0, 0.0, 0.0), m.m.rootsize, list(p.bc), root, level, ci)
+        dev.advance(2)
+    dev.close()
