@@ -212,6 +212,8 @@ def lib():
             "fv_dev_size": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
             "fv_dev_last_error": (C.c_char_p, []),
             "fv_dev_selftest_guard": (C.c_int, []),
+            "fv_cfl_dt_noalloc": (C.c_int, [P(fv_params), P(fv_geo), C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                            C.c_void_p, C.c_void_p]),
             "fv_dev_stream_copy": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
             "fv_dev_set_limiter_flags": (C.c_int, [C.c_void_p, P(C.c_uint8)]),
             "fv_dev_get_limiter_flags": (C.c_int, [C.c_void_p, P(C.c_uint8)]),

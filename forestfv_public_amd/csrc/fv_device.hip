@@ -951,7 +951,8 @@ __global__ __launch_bounds__(FV_BLOCK) void k_cfl(int n, Params P, Geo G, const 
   }
 }
 
-__global__ void k_min_final(int m, const double *__restrict__ part, double *__restrict__ out) {
+// minimum of m partial minima, times scale, into *out (one block of 1024 lanes)
+__global__ void k_min_final(int m, const double *__restrict__ part, double *__restrict__ out, double scale) {
   double d = INFINITY;
   for (int k = threadIdx.x; k < m; k += blockDim.x) d = fmin(d, part[k]);
   for (int off = 32; off > 0; off >>= 1) d = fmin(d, __shfl_xor(d, off, 64));
@@ -961,7 +962,7 @@ __global__ void k_min_final(int m, const double *__restrict__ part, double *__re
   if (threadIdx.x == 0) {
     double r = wmin[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = fmin(r, wmin[w]);
-    *out = r;
+    *out = scale * r;
   }
 }
 
@@ -1527,12 +1528,12 @@ extern "C" int fv_dev_cfl_dt(fv_dev *c, double cfl, double *dt) {
     hipLaunchKernelGGL(k_cfl<2>, dim3(nb), dim3(FV_BLOCK), 0, c->stream, n, c->P, c->G, c->S, c->info, c->part);
   else
     hipLaunchKernelGGL(k_cfl<3>, dim3(nb), dim3(FV_BLOCK), 0, c->stream, n, c->P, c->G, c->S, c->info, c->part);
-  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(1024), 0, c->stream, nb, c->part, c->dtmp);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(1024), 0, c->stream, nb, c->part, c->dtmp, cfl);
   HIPCHK(hipGetLastError());
   double h = 0.0;
   HIPCHK(hipMemcpyAsync(&h, c->dtmp, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  *dt = cfl * h;
+  *dt = h;
   return FV_OK;
 }
 
@@ -1864,4 +1865,68 @@ extern "C" int fv_dev_selftest_guard(void) {
   }
   delete c;
   return bad;
+}
+
+// ---------------------------------------------------------------------
+// fv_cfl_dt_noalloc: fv_dev_cfl_dt's kernels on caller-owned device arrays,
+// with the per-block partial minima in a fixed device array instead of a
+// context buffer, so that the call allocates nothing.
+// ---------------------------------------------------------------------
+#include <mutex>
+
+// per-block partial minima of k_cfl: FV_CFL_NOALLOC_MAX_LEAVES / FV_BLOCK doubles
+// (2 MiB), held by every device the library runs on until the process ends
+static __device__ __align__(256) double scratch[FV_CFL_NOALLOC_MAX_LEAVES / FV_BLOCK];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// the address of this device's copy of `scratch`, looked up once per device
+// with that device current (the array's name is only a host-side stand-in)
+int scratch_of_current_device(double **out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_err(FV_ERR_ARG, "fv_cfl_dt_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return set_err(FV_ERR_HIP, "fv_cfl_dt_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return FV_OK;
+}
+}  // namespace
+
+extern "C" int fv_cfl_dt_noalloc(const fv_params *p, const fv_geo *g, int n, const double *S, const uint32_t *info,
+                                 double cfl, double *dt, void *stream) {
+  if (!p || !g || !S || !info || !dt) return set_err(FV_ERR_ARG, "fv_cfl_dt_noalloc: null argument");
+  if (p->nmat != 2 && p->nmat != 3) return set_err(FV_ERR_ARG, "fv_cfl_dt_noalloc: nmat must be 2 or 3");
+  if (n < 1) return set_err(FV_ERR_ARG, "fv_cfl_dt_noalloc: no leaves");
+  if (n > FV_CFL_NOALLOC_MAX_LEAVES) {
+    snprintf(fv_g_err, sizeof fv_g_err, "fv_cfl_dt_noalloc: %d leaves exceed the scratch array's %d", n,
+             FV_CFL_NOALLOC_MAX_LEAVES);
+    return FV_ERR_ARG;
+  }
+  double *part = nullptr;
+  int rc = scratch_of_current_device(&part);
+  if (rc != FV_OK) return rc;
+  Params P;
+  Geo G;
+  to_dev_params(p, P);
+  to_dev_geo(g, G);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nb = (n + FV_BLOCK - 1) / FV_BLOCK;
+  // every one of the nb partials is written by k_cfl before k_min_final reads it,
+  // so the contents left by an earlier call need no clearing
+  if (p->nmat == 2)
+    hipLaunchKernelGGL(k_cfl<2>, dim3(nb), dim3(FV_BLOCK), 0, st, n, P, G, S, info, part);
+  else
+    hipLaunchKernelGGL(k_cfl<3>, dim3(nb), dim3(FV_BLOCK), 0, st, n, P, G, S, info, part);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(1024), 0, st, nb, part, dt, cfl);
+  HIPCHK(hipGetLastError());
+  return FV_OK;
 }

@@ -164,6 +164,22 @@ int  fv_dev_totals(fv_dev *ctx, double *tot);
 /* Number of leaves and NEQ of the attached mesh. */
 int  fv_dev_size(const fv_dev *ctx, int *n, int *neq);
 
+/* Allocation-free CFL step, with no counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It is fv_dev_cfl_dt's
+ * computation, cfl * min h / (max|u_i| + c), on caller-owned device arrays:
+ * S [NEQ][n] (structure of arrays, as a context holds it) and info [n] (the
+ * fv_pack info words), with the geometry tables g of fv_pack.  The result is
+ * written to the device double *dt, in stream order on `stream` (a
+ * hipStream_t; NULL is the null stream); the call does not wait for it.  The
+ * per-block partial minima live in a fixed device array of
+ * FV_CFL_NOALLOC_MAX_LEAVES / 256 doubles (2 MiB per device, held until the
+ * process ends), so n above FV_CFL_NOALLOC_MAX_LEAVES is rejected with
+ * FV_ERR_ARG.  Calls that overlap on one device share that array and race. */
+#define FV_CFL_NOALLOC_MAX_LEAVES (1 << 26)
+int  fv_cfl_dt_noalloc(const fv_params *p, const fv_geo *g, int n, const double *S, const uint32_t *info,
+                       double cfl, double *dt, void *stream);
+
 /* Host-only self-check with no counterpart in the reference: a context left
  * unusable by a failed mesh install is refused by the entry points (status
  * FV_ERR_HIP, no kernel launched), and a device reservation that cannot be
