@@ -212,6 +212,8 @@ def lib():
             "fv_dev_size": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
             "fv_dev_last_error": (C.c_char_p, []),
             "fv_dev_selftest_guard": (C.c_int, []),
+            "fv_dev_set_bricks": (C.c_int, [C.c_void_p, C.c_int]),
+            "fv_dev_brick_count": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
             "fv_cfl_dt_noalloc": (C.c_int, [P(fv_params), P(fv_geo), C.c_int, C.c_void_p, C.c_void_p, C.c_double,
                                             C.c_void_p, C.c_void_p]),
             "fv_dev_stream_copy": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
@@ -742,6 +744,16 @@ class Device:
 
     def set_fused(self, on):
         _check(lib().fv_dev_set_fused(self._h, 1 if on else 0), "fv_dev_set_fused")
+
+    def set_bricks(self, on):
+        """4x4x4 same-level blocks through the LDS block kernel (default on) or every leaf per leaf."""
+        _check(lib().fv_dev_set_bricks(self._h, 1 if on else 0), "fv_dev_set_bricks")
+
+    def brick_count(self):
+        """(blocks, leaves outside blocks) of the current mesh."""
+        a, b = C.c_int(), C.c_int()
+        _check(lib().fv_dev_brick_count(self._h, C.byref(a), C.byref(b)), "fv_dev_brick_count")
+        return a.value, b.value
 
     def cfl_dt(self, cfl):
         d = C.c_double()

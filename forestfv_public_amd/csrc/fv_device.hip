@@ -27,6 +27,7 @@
 #include "forestfv_dev.h"
 #include "fv_internal.h"
 #include "fv_fast.h"
+#include <hipcub/hipcub.hpp>
 
 using namespace fvk;
 
@@ -296,11 +297,13 @@ template <int NM, bool MIXED, bool XR>
 __global__ __launch_bounds__(FV_BLOCK, 2) void k_recon_fast(int n, Params P, Geo G, const double *__restrict__ S,
                                                          const int *__restrict__ nbr, const uint32_t *__restrict__ info,
                                                          const int *__restrict__ fine4, const uint8_t *__restrict__ lim,
-                                                         double *__restrict__ SF, float *__restrict__ SFm) {
+                                                         double *__restrict__ SF, float *__restrict__ SFm,
+                                                         const int *__restrict__ list, int cnt) {
   using L = Lay<NM>;
   using R = typename std::conditional<MIXED, float, double>::type;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = list ? list[t] : t;   // every leaf, or the leaves of a list
   const fvf::Mat<NM, double> M = fvf::make_mat<NM, double>(P);
   const double a = P.amin / 100.0;
   const uint32_t inf = info[i];
@@ -383,15 +386,17 @@ FV_DEV void load_sol(const fvf::Mat<NM, double> &Md, const fvf::Mat<NM, R> &M, d
 }
 
 template <int NM, bool MIXED, bool XR>
-__global__ __launch_bounds__(FV_BLOCK) void k_flux_fast(int n, Params P, Geo G, int cs, double *__restrict__ S,
+__global__ __launch_bounds__(FV_BLOCK) void k_flux_fast(int n, Params P, Geo G, int cs, const double *__restrict__ S,
+                                                        double *__restrict__ Sout,
                                                         double *__restrict__ SN, const double *__restrict__ SF,
                                                         const float *__restrict__ SFm, const int *__restrict__ nbr,
                                                         const uint32_t *__restrict__ info, const int *__restrict__ fine4,
-                                                        int *__restrict__ nanflag) {
+                                                        int *__restrict__ nanflag, const int *__restrict__ list, int cnt) {
   using L = Lay<NM>;
   using R = typename std::conditional<MIXED, float, double>::type;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = list ? list[t] : t;
   const fvf::Mat<NM, double> Md = fvf::make_mat<NM, double>(P);
   const fvf::Mat<NM, R> M = fvf::make_mat<NM, R>(P);
   const double a = P.amin / 100.0;
@@ -493,7 +498,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_fast(int n, Params P, Geo G, 
   }
   fvf::relax_fast<NM, XR>(P, Md, Sn);
 #pragma unroll
-  for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = Sn[v];
+  for (int v = 0; v < L::NEQ; v++) Sout[(size_t)v * n + i] = Sn[v];
 }
 
 
@@ -531,15 +536,17 @@ FV_DEV bool one_neighbour(int kind) {
 }
 
 template <int NM, bool MIXED, bool XR>
-__global__ __launch_bounds__(FV_BLOCK) void k_flux_pipe(int n, Params P, Geo G, int cs, double *__restrict__ S,
+__global__ __launch_bounds__(FV_BLOCK) void k_flux_pipe(int n, Params P, Geo G, int cs, const double *__restrict__ S,
+                                                        double *__restrict__ Sout,
                                                         double *__restrict__ SN, const double *__restrict__ SF,
                                                         const float *__restrict__ SFm, const int *__restrict__ nbr,
                                                         const uint32_t *__restrict__ info, const int *__restrict__ fine4,
-                                                        int *__restrict__ nanflag) {
+                                                        int *__restrict__ nanflag, const int *__restrict__ list, int cnt) {
   using L = Lay<NM>;
   using R = typename std::conditional<MIXED, float, double>::type;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = list ? list[t] : t;
   const fvf::Mat<NM, double> Md = fvf::make_mat<NM, double>(P);
   const fvf::Mat<NM, R> M = fvf::make_mat<NM, R>(P);
   const double a = P.amin / 100.0;
@@ -660,7 +667,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_pipe(int n, Params P, Geo G, 
   }
   fvf::relax_fast<NM, XR>(P, Md, Sn);
 #pragma unroll
-  for (int v = 0; v < L::NEQ; v++) S[(size_t)v * n + i] = Sn[v];
+  for (int v = 0; v < L::NEQ; v++) Sout[(size_t)v * n + i] = Sn[v];
 }
 
 // ---------------------------------------------------------------------
@@ -917,6 +924,421 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
   fvf::relax_fast<NM, XR>(P, Md, Sn);
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) Sout[(size_t)v * n + i] = Sn[v];
+}
+
+// ---------------------------------------------------------------------
+// block stage (plain FP64 and mixed precision): one workgroup per brick,
+// 64 consecutive leaves in DFS order that form a 4x4x4 block of one level
+// and whose faces on the block surface have a same-level neighbour or a
+// domain boundary.  The cell states, the states across the surface and the
+// stencil values beyond those are staged in LDS once.  Then one lane per face
+// (144 inside the brick, 96 on its surface) reconstructs both face states of
+// its face from the staged states and runs HLLC, so every face is evaluated
+// once and no face state goes through HBM except the ones a non-brick
+// neighbour's flux reads.  Each cell then gathers its six faces in face
+// order and applies the RK stage and the relaxation.  A face takes one
+// evaluation, from the side with the lower position along the face normal
+// (the brick cell's side on the surface), which equals the reference's
+// evaluation by either side (computed once by the earlier leaf, or by both
+// for interface leaves, m4_residual.c:62-103) to rounding: the HLLC flux is
+// antisymmetric under exchange of the sides with the normal reversed.
+// ---------------------------------------------------------------------
+constexpr int BR_THREADS = 256;
+constexpr int BR_SLOTS = 96;              // surface slots: 6 faces x 16 cells
+constexpr int BR_TASKS = BR_SLOTS + 144;  // faces: one per surface slot + 3 x 48 inside the brick
+
+// a position in the brick; components are selected with compares rather than
+// array indexing, so that no private array is left in scratch memory
+struct BrPos {
+  int x, y, z;
+};
+FV_DEV int br_get(const BrPos &p, int ax) { return ax == 0 ? p.x : (ax == 1 ? p.y : p.z); }
+FV_DEV void br_add(BrPos &p, int ax, int d) {
+  p.x += (ax == 0) ? d : 0;
+  p.y += (ax == 1) ? d : 0;
+  p.z += (ax == 2) ? d : 0;
+}
+// local index of a position: Morton, x fastest (the DFS order)
+FV_DEV int br_index(const BrPos &p) {
+  return (p.x & 1) | ((p.y & 1) << 1) | ((p.z & 1) << 2) | ((p.x >> 1) << 3) | ((p.y >> 1) << 4) | ((p.z >> 1) << 5);
+}
+FV_DEV BrPos br_pos(int l) {
+  return BrPos{(l & 1) | (((l >> 3) & 1) << 1), ((l >> 1) & 1) | (((l >> 4) & 1) << 1),
+               ((l >> 2) & 1) | (((l >> 5) & 1) << 1)};
+}
+// axis and direction of face f (f0 y-, f1 x+, f2 y+, f3 x-, f4 z-, f5 z+)
+FV_DEV int br_axis(int f) { return (f == 1 || f == 3) ? 0 : ((f == 0 || f == 2) ? 1 : 2); }
+FV_DEV int br_plus(int f) { return f == 1 || f == 2 || f == 5; }
+FV_DEV int br_plus_face(int ax) { return ax == 0 ? 1 : (ax == 1 ? 2 : 5); }
+// position of a cell in the 4x4 sheet across the axis: the other two coordinates
+FV_DEV int br_sheet(int ax, const BrPos &p) {
+  return ax == 0 ? p.y + 4 * p.z : (ax == 1 ? p.x + 4 * p.z : p.x + 4 * p.y);
+}
+FV_DEV BrPos br_sheet_pos(int ax, int k, int along) {
+  const int u = k & 3, w = k >> 2;
+  return ax == 0 ? BrPos{along, u, w} : (ax == 1 ? BrPos{u, along, w} : BrPos{u, w, along});
+}
+// cell of surface slot s = 16 f + k
+FV_DEV int br_slot_cell(int s) {
+  const int f = s >> 4;
+  return br_index(br_sheet_pos(br_axis(f), s & 15, br_plus(f) ? 3 : 0));
+}
+
+// XCD-aware brick order: workgroups are dealt to the 8 XCDs round robin, so
+// workgroup w takes brick (w mod 8) * per + w / 8 and each XCD walks a
+// contiguous range of bricks, whose surface neighbours then share its L2
+FV_DEV int br_of_block(int nb) {
+  const unsigned per = gridDim.x >> 3;
+  const unsigned b = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  return (int)b < nb ? (int)b : -1;
+}
+
+template <int NM>
+struct BrickLds {
+  using L = Lay<NM>;
+  // staged cell states [v][c], avf-limited as stencil values and the cell's
+  // own reconstruction base use them: own cells c < 64, the neighbour across
+  // surface slot s at 64 + s, that neighbour's stencil value beyond it at 160 + s
+  static constexpr int NC = 64 + 2 * BR_SLOTS;
+  double st[L::NEQ * NC];
+  double araw[L::N3 * 64];              // the own cells' volume fractions before the limit
+  double ff[(L::NEQ + 1) * BR_TASKS];   // face fluxes A F and u*.n per face task
+  uint32_t info[64];
+  uint32_t infoh[BR_SLOTS];             // info word of the neighbour across each slot
+  int nb[BR_SLOTS];                     // that neighbour (leaf index, or -BC code)
+  uint8_t need[BR_SLOTS];               // it is no brick leaf: its flux reads this face state from HBM
+  uint8_t lim[64 + BR_SLOTS];           // per-leaf limiter of the own cells and the slot neighbours
+  double idmh[BR_SLOTS];                // 1/dcc of the slot neighbour's face beyond it (any kind)
+};
+
+template <int NM>
+FV_DEV void br_state(const BrickLds<NM> &B, int c, double *X) {
+#pragma unroll
+  for (int v = 0; v < Lay<NM>::NEQ; v++) X[v] = B.st[v * BrickLds<NM>::NC + c];
+}
+
+// face state (FP64 vector, relaxed) or FP32 record of a same-level cell at a
+// face, from its staged state (column c) and the stencil value columns across
+// the face (cg) and across the opposite face (co): k_recon_fast's arithmetic
+// with the brick's one level (dcf, 1/dcc), and zero slope when either face is
+// a domain boundary (bnd)
+template <int NM, bool MIXED, bool XR>
+FV_DEV void br_face_state(const Params &P, const fvf::Mat<NM, double> &Md, double a, const BrickLds<NM> &B, int c,
+                          int cg, int co, int limiter, bool bnd, double dcf, double idp, double idm, double *F,
+                          float *rec) {
+  using L = Lay<NM>;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  br_state<NM>(B, c, F);
+  if (!bnd) {
+    const R ip = (R)idp, im = (R)idm;
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) {
+      const double sc = F[v];
+      const double xg = B.st[v * BrickLds<NM>::NC + cg], xo = B.st[v * BrickLds<NM>::NC + co];
+      const R g = fvf::slope<R>(limiter, xg - sc, xo - sc, ip, im);
+      F[v] = sc + dcf * (double)g;
+    }
+  }
+  vf_limit_face<NM, double>(a, F);
+  if constexpr (MIXED) fvf::relax_to_record<NM, XR>(P, Md, a, F, rec);
+  else fvf::relax_fast<NM, XR>(P, Md, F);
+}
+
+template <int NM, bool MIXED, typename R>
+FV_DEV void br_sol(const fvf::Mat<NM, double> &Md, const fvf::Mat<NM, R> &M, double a, const double *F,
+                   const float *rec, fvf::SolF<NM, R> &s) {
+  if constexpr (MIXED) fvf::fill_rec<NM>(M, rec, s);
+  else fvf::fill_cons<NM, R>(Md, a, F, s);
+}
+
+template <int NM, bool MIXED>
+FV_DEV void br_write_sf(int n, int i, int f, const double *F, const float *rec, double *__restrict__ SF,
+                        float *__restrict__ SFm) {
+  if constexpr (MIXED) {
+#pragma unroll
+    for (int q = 0; q < fvf::Rec<NM>::N; q++) SFm[((size_t)f * fvf::Rec<NM>::N + q) * n + i] = rec[q];
+  } else {
+#pragma unroll
+    for (int v = 0; v < Lay<NM>::NEQ; v++) SF[((size_t)f * Lay<NM>::NEQ + v) * n + i] = F[v];
+  }
+}
+
+template <int NM, bool MIXED, bool XR>
+__global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, const Geo *__restrict__ Gp, int cs,
+                                                               const int *__restrict__ bricks, int nbricks,
+                                                               const double *__restrict__ S, double *__restrict__ Sout,
+                                                               double *__restrict__ SN, const int *__restrict__ nbr,
+                                                               const uint32_t *__restrict__ info,
+                                                               const int *__restrict__ fine4,
+                                                               const uint8_t *__restrict__ lim,
+                                                               const uint8_t *__restrict__ inbrick,
+                                                               double *__restrict__ SF, float *__restrict__ SFm,
+                                                               int *__restrict__ nanflag) {
+  using L = Lay<NM>;
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using LD = BrickLds<NM>;
+  constexpr int NC = LD::NC;
+  __shared__ LD B;
+  const int br = br_of_block(nbricks);
+  if (br < 0) return;   // whole workgroups only: no barrier is reached by part of one
+  const int b0 = bricks[br];
+  const int tid = threadIdx.x;
+  const Geo &G = *Gp;
+  const fvf::Mat<NM, double> Md = fvf::make_mat<NM, double>(P);
+  const double a = P.amin / 100.0;
+
+  // ---- stage the cell states, the info words, the neighbour across each
+  // surface slot and that neighbour's stencil value beyond it
+  for (int q = tid; q < L::NEQ * 64; q += BR_THREADS) {
+    const int v = q >> 6, l = q & 63;
+    B.st[v * NC + l] = S[(size_t)v * n + b0 + l];
+  }
+  if (tid < 64) {
+    B.info[tid] = info[b0 + tid];
+    B.lim[tid] = (uint8_t)leaf_limiter(P, lim, b0 + tid);
+  }
+  __syncthreads();
+  if (tid < 64) {
+    // the own states avf-limited in place; the raw fractions kept for the cell update
+    double X[L::NEQ];
+    br_state<NM>(B, tid, X);
+#pragma unroll
+    for (int k = 0; k < L::N3; k++) B.araw[k * 64 + tid] = X[L::I3 + k];
+    avf_limit<NM, double>(a, X);
+#pragma unroll
+    for (int k = 0; k < L::N3; k++) B.st[(L::I3 + k) * NC + tid] = X[L::I3 + k];
+  } else if (tid < 64 + BR_SLOTS) {
+    const int s = tid - 64, f = s >> 4, i = b0 + br_slot_cell(s);
+    const int nb = nbr[(size_t)f * n + i];
+    B.nb[s] = nb;
+    if (face_kind(info[i], f) == FV_FK_BOUNDARY) {
+      // no state across a boundary face: its slope is zero and its ghost comes from the face state
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) { B.st[v * NC + 64 + s] = 0.0; B.st[v * NC + 160 + s] = 0.0; }
+      B.need[s] = 0;
+      B.infoh[s] = 0;
+      B.lim[64 + s] = 0;
+      B.idmh[s] = 0.0;
+    } else {
+      const uint32_t infj = info[nb];
+      B.infoh[s] = infj;
+      B.need[s] = inbrick[nb] ? 0 : 1;
+      B.lim[64 + s] = (uint8_t)leaf_limiter(P, lim, nb);
+      double Xj[L::NEQ], X2[L::NEQ], Araw[NM] = {};
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) Xj[v] = S[(size_t)v * n + nb];
+      const int kj = face_kind(infj, f);
+      if (kj == FV_FK_BOUNDARY) {
+#pragma unroll
+        for (int k = 0; k < L::N3; k++) Araw[k] = Xj[L::I3 + k];
+      }
+      {
+        double dcf_j, dcc_j;
+        dist_of(G, kj, cell_level(infj), dcf_j, dcc_j);
+        B.idmh[s] = (kj == FV_FK_BOUNDARY) ? 0.0 : fvf::rcp(dcc_j);
+      }
+      avf_limit<NM, double>(a, Xj);
+      stencil_value<NM>(S, n, nb, kj, nbr[(size_t)f * n + nb], fine4, Xj, Araw, a, X2);
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) {
+        B.st[v * NC + 64 + s] = Xj[v];
+        B.st[v * NC + 160 + s] = X2[v];
+      }
+    }
+  }
+  __syncthreads();
+  // the brick's one level: distances and areas are the same for every face
+  const int lvl = cell_level(B.info[0]);
+  const double dcf = G.dcf[0][lvl], idcc = fvf::rcp(G.dcc[0][lvl]), A = G.area[lvl];
+
+  // ---- faces: one lane per face, 96 across the surface slots and 144 inside
+  // the brick; a lane reconstructs both face states of its face and runs HLLC
+  if (tid < BR_TASKS) {
+    const fvf::Mat<NM, R> M = fvf::make_mat<NM, R>(P);
+    int fL, bcode = 0;
+    fvf::SolF<NM, R> sL, sR;
+    if (tid < BR_SLOTS) {
+      const int s = tid, f = s >> 4, fo = FCOPP_D[f], ax = br_axis(f);
+      const int l = br_slot_cell(s), i = b0 + l;
+      const uint32_t inf = B.info[l];
+      fL = f;
+      BrPos pos = br_pos(l);
+      br_add(pos, ax, br_plus(f) ? -1 : 1);   // the brick cell's neighbour on the other side, inside the brick
+      const bool bndf = face_kind(inf, f) == FV_FK_BOUNDARY;
+      const bool bnd = bndf || face_kind(inf, fo) == FV_FK_BOUNDARY;
+      double F[L::NEQ];
+      float rec[MIXED ? fvf::Rec<NM>::N : 1];
+      br_face_state<NM, MIXED, XR>(P, Md, a, B, l, 64 + s, br_index(pos), B.lim[l], bnd, dcf, idcc, idcc, F, rec);
+      if (B.need[s]) br_write_sf<NM, MIXED>(n, i, f, F, rec, SF, SFm);
+      br_sol<NM, MIXED, R>(Md, M, a, F, rec, sL);
+      if (bndf) {
+        bcode = -B.nb[s];
+        sR = sL;
+      } else {
+        // the neighbour's face state towards the brick: its stencil value across
+        // that face is the brick cell, beyond it the staged second layer
+        const uint32_t infj = B.infoh[s];
+        const bool bj = face_kind(infj, fo) == FV_FK_BOUNDARY || face_kind(infj, f) == FV_FK_BOUNDARY;
+        br_face_state<NM, MIXED, XR>(P, Md, a, B, 64 + s, l, 160 + s, B.lim[64 + s], bj, dcf, idcc, B.idmh[s], F,
+                                     rec);
+        br_sol<NM, MIXED, R>(Md, M, a, F, rec, sR);
+      }
+    } else {
+      const int q = tid - BR_SLOTS, ax = q / 48, p = (q % 48) >> 4, k = q & 15;
+      const int f = br_plus_face(ax), fo = FCOPP_D[f];
+      const int lL = br_index(br_sheet_pos(ax, k, p)), lR = br_index(br_sheet_pos(ax, k, p + 1));
+      // the cells before the left one and after the right one along the axis:
+      // inside the brick, or the staged neighbour of the - / + surface slot
+      const int cm = (p > 0) ? br_index(br_sheet_pos(ax, k, p - 1)) : 64 + 16 * fo + k;
+      const int cp = (p < 2) ? br_index(br_sheet_pos(ax, k, p + 2)) : 64 + 16 * f + k;
+      // inside the brick only the far faces of the two cells can be domain boundaries
+      const bool bl = face_kind(B.info[lL], fo) == FV_FK_BOUNDARY;
+      const bool brr = face_kind(B.info[lR], f) == FV_FK_BOUNDARY;
+      fL = f;
+      double F[L::NEQ];
+      float rec[MIXED ? fvf::Rec<NM>::N : 1];
+      br_face_state<NM, MIXED, XR>(P, Md, a, B, lL, lR, cm, B.lim[lL], bl, dcf, idcc, idcc, F, rec);
+      br_sol<NM, MIXED, R>(Md, M, a, F, rec, sL);
+      br_face_state<NM, MIXED, XR>(P, Md, a, B, lR, lL, cp, B.lim[lR], brr, dcf, idcc, idcc, F, rec);
+      br_sol<NM, MIXED, R>(Md, M, a, F, rec, sR);
+    }
+    R fr[9];
+#pragma unroll
+    for (int qq = 0; qq < 9; qq++) fr[qq] = (R)G.frame[fL][qq];
+    if (bcode != 0) fvf::ghost<NM, R, XR>(M, P, bcode, fr, sL, sR);
+    R out[L::NEQ], un, vn, wn;
+    fvf::hllc<NM, R>(M, P.fix, fr, bcode, sL, sR, out, un, vn, wn);
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) B.ff[v * BR_TASKS + tid] = A * (double)out[v];
+    B.ff[L::NEQ * BR_TASKS + tid] =
+        (double)un * G.frame[fL][0] + (double)vn * G.frame[fL][1] + (double)wn * G.frame[fL][2];
+  }
+  __syncthreads();
+
+  // ---- per cell: gather the six faces in face order, RK stage, relaxation
+  if (tid >= 64) return;
+  const int l = tid, i = b0 + l;
+  const uint32_t inf = B.info[l];
+  double Sc[L::NEQ];
+  br_state<NM>(B, l, Sc);
+#pragma unroll
+  for (int k = 0; k < L::N3; k++) Sc[L::I3 + k] = B.araw[k * 64 + l];   // the raw state
+  fvf::SolF<NM, double> sc;
+  fvf::fill_cons<NM, double>(Md, a, Sc, sc);
+  double src_avf[NM], src_st[NM];
+#pragma unroll
+  for (int k = 0; k < L::N3; ++k) src_avf[k] = sc.avf[k];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) src_st[k] = -sc.avf[k] * sc.p[k];
+  const bool fixnc = (P.fix & 1) != 0;
+  const double Area_f = A;
+  const BrPos pos = br_pos(l);
+  double RHS[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) RHS[v] = 0.0;
+#pragma unroll 1
+  for (int f = 0; f < 6; f++) {
+    const int ax = br_axis(f), along = br_get(pos, ax), k = br_sheet(ax, pos);
+    const int kind = face_kind(inf, f);
+    int t;
+    bool right = false;
+    if (br_plus(f)) t = (along == 3) ? 16 * f + k : BR_SLOTS + 48 * ax + 16 * along + k;
+    else if (along == 0) t = 16 * f + k;
+    else { t = BR_SLOTS + 48 * ax + 16 * (along - 1) + k; right = true; }
+    const double sgn = right ? -1.0 : 1.0;
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) RHS[v] += sgn * B.ff[v * BR_TASKS + t];
+    // u*.n of this leaf's face: the star velocity is shared by both sides; the
+    // published code negates the reused global-frame vector on the mirrored side
+    // (m4_residual.c:72-74), which FV_FIX_NONCONS leaves out
+    const double un = B.ff[L::NEQ * BR_TASKS + t];
+    double und;
+    if (right) und = (!fixnc && kind == FV_FK_MIRROR) ? un : -un;
+    else und = (!fixnc && kind == FV_FK_MIRROR) ? -un : un;
+#pragma unroll
+    for (int q = 0; q < L::N3; ++q) {
+      const double tt = Area_f * src_avf[q] * und;
+      RHS[L::I3 + q] = fixnc ? RHS[L::I3 + q] - tt : RHS[L::I3 + q] + tt;
+    }
+#pragma unroll
+    for (int q = 0; q < NM; ++q) {
+      const double tt = Area_f * src_st[q] * und;
+      RHS[L::I4 + q] = fixnc ? RHS[L::I4 + q] - tt : RHS[L::I4 + q] + tt;
+    }
+  }
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) bad |= isnan(RHS[v]);
+  if (bad) *nanflag = 1;
+  const double dtv = P.dt * fvf::rcp(G.vol[lvl]);
+  const double wa = (cs == 1) ? 0.0 : (cs == 2 ? 0.75 : (cs == 3 ? 1.0 / 3.0 : 0.5));
+  const double wb = 1.0 - wa;
+  double Sn[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) {
+    double sn;
+    if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
+    else sn = SN[(size_t)v * n + i];
+    Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
+  }
+  fvf::relax_fast<NM, XR>(P, Md, Sn);
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) Sout[(size_t)v * n + i] = Sn[v];
+}
+
+// A leaf i starts a brick when leaves i .. i+63 are of one level and connected
+// as a 4x4x4 block in DFS (Morton) order, and each face of the block surface
+// has a same-level neighbour or is a domain boundary.  First a cheap test per
+// leaf: one level at both ends, and the links a block's first cells have to
+// their +x, +y, +z neighbours and to the next octants (cells 1, 2, 4 to the
+// octants at 8, 16, 32), which only a block's first leaf satisfies; then one
+// wave per candidate checks all 64 cells' faces.
+__global__ __launch_bounds__(FV_BLOCK) void k_brick_quick(int n, const int *__restrict__ nbr,
+                                                          const uint32_t *__restrict__ info, int *__restrict__ mark) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int ok = 0;
+  if (i + 63 < n) {
+    const size_t N = (size_t)n;
+    ok = cell_level(info[i + 63]) == cell_level(info[i]) && nbr[1 * N + i] == i + 1 && nbr[2 * N + i] == i + 2 &&
+         nbr[5 * N + i] == i + 4 && nbr[1 * N + i + 1] == i + 8 && nbr[2 * N + i + 2] == i + 16 &&
+         nbr[5 * N + i + 4] == i + 32;
+  }
+  mark[i] = ok;
+}
+
+__global__ __launch_bounds__(64) void k_brick_full(int n, const int *__restrict__ cand, const int *__restrict__ nbr,
+                                                   const uint32_t *__restrict__ info, int *__restrict__ mark) {
+  const int i = cand[blockIdx.x];
+  const int l = threadIdx.x;
+  const uint32_t il = info[i + l];
+  bool ok = cell_level(il) == cell_level(info[i]);
+  const BrPos pos = br_pos(l);
+  for (int f = 0; f < 6; f++) {
+    const int k = face_kind(il, f), ax = br_axis(f);
+    BrPos q = pos;
+    br_add(q, ax, br_plus(f) ? 1 : -1);
+    const int along = br_get(q, ax);
+    if (along >= 0 && along <= 3)
+      ok = ok && (k == FV_FK_SAME || k == FV_FK_MIRROR) && nbr[(size_t)f * n + i + l] == i + br_index(q);
+    else
+      ok = ok && (k == FV_FK_SAME || k == FV_FK_MIRROR || k == FV_FK_BOUNDARY);
+  }
+  const bool all = __all(ok);
+  if (l == 0 && all) mark[i] = 1;
+}
+
+// leaves of the listed bricks: inbrick = 1 (0 elsewhere, set beforehand); rest = !inbrick
+__global__ __launch_bounds__(FV_BLOCK) void k_brick_fill(int nbricks, const int *__restrict__ bricks,
+                                                         uint8_t *__restrict__ inbrick) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbricks * 64) return;
+  inbrick[bricks[t >> 6] + (t & 63)] = 1;
+}
+
+__global__ __launch_bounds__(FV_BLOCK) void k_not(int n, const uint8_t *__restrict__ a, uint8_t *__restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i] ? 0 : 1;
 }
 
 #pragma clang fp contract(off)
@@ -1212,6 +1634,7 @@ static int upload_mesh(fv_dev *c, const fv_pack *pk) {
   int rc = fv_dev_reserve(c, pk->n, pk->nfine);
   if (rc != FV_OK) return rc;
   c->broken = 1;   // until every upload below has succeeded
+  c->brick_ok = 0;
   c->forest = 0;   // the leaf addresses of an earlier fv_dev_set_forest no longer apply
   c->n = pk->n;
   c->nfine = pk->nfine;
@@ -1266,6 +1689,8 @@ extern "C" int fv_dev_detach(fv_dev *c) {
   hipStreamSynchronize(c->stream);
   free_buffers(c);
   hipFree(c->lroot); hipFree(c->llevel); hipFree(c->lci); hipFree(c->lSI); hipFree(c->tpart);
+  hipFree(c->bricks); hipFree(c->rest); hipFree(c->inbrick); hipFree(c->notbrick); hipFree(c->bmark);
+  hipFree(c->bcnt); hipFree(c->bcub);
   for (auto &b : c->pool) hipFree(b.first);
   c->pool.clear();
   hipFree(c->Gdev);
@@ -1345,6 +1770,26 @@ extern "C" int fv_dev_set_precision(fv_dev *c, int prec) {
   return FV_OK;
 }
 
+static int classify_bricks(fv_dev *c);
+
+extern "C" int fv_dev_set_bricks(fv_dev *c, int on) {
+  FV_GUARD(c);
+  if (on != 0 && on != 1) return set_err(FV_ERR_ARG, "fv_dev_set_bricks: on must be 0 or 1");
+  c->use_bricks = on;
+  return FV_OK;
+}
+
+extern "C" int fv_dev_brick_count(fv_dev *c, int *nbricks, int *nrest) {
+  FV_GUARD(c);
+  if (!c->brick_ok) {
+    int rc = classify_bricks(c);
+    if (rc != FV_OK) return rc;
+  }
+  if (nbricks) *nbricks = c->nbricks;
+  if (nrest) *nrest = c->nrest;
+  return FV_OK;
+}
+
 extern "C" int fv_dev_set_fused(fv_dev *c, int on) {
   FV_GUARD(c);
   if (!c) return set_err(FV_ERR_ARG, "null context");
@@ -1362,6 +1807,70 @@ extern "C" int fv_dev_begin_step(fv_dev *c) {
 static int stage_case(int rk, int irk) {
   if (rk == FV_RK_SSP2) return irk == 1 ? 1 : 4;
   return irk;
+}
+
+// Bricks of the current mesh (k_brick_mark), their leaves (inbrick) and the
+// list of the other leaves, which run the per-leaf kernels.  Computed at the
+// first stage after a mesh change; face-kind refreshes between steps keep it
+// valid, since a brick accepts same-level faces of either kind.
+static int classify_bricks(fv_dev *c) {
+  const int n = c->n;
+  if (c->brick_cap < n) {
+    hipFree(c->bricks); hipFree(c->rest); hipFree(c->inbrick); hipFree(c->notbrick); hipFree(c->bmark);
+    hipFree(c->bcnt); hipFree(c->bcub);
+    c->bricks = c->rest = c->bmark = c->bcnt = nullptr;
+    c->inbrick = c->notbrick = nullptr;
+    c->bcub = nullptr;
+    c->brick_cap = 0;
+    size_t b1 = 0, b2 = 0;
+    hipcub::CountingInputIterator<int> idx(0);
+    HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, b1, idx, (const int *)nullptr, (int *)nullptr, (int *)nullptr, n,
+                                         c->stream));
+    HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, b2, idx, (const uint8_t *)nullptr, (int *)nullptr, (int *)nullptr,
+                                         n, c->stream));
+    c->bcub_bytes = b1 > b2 ? b1 : b2;
+    HIPCHK(hipMalloc(&c->bricks, sizeof(int) * ((size_t)n / 64 + 1)));
+    HIPCHK(hipMalloc(&c->rest, sizeof(int) * (size_t)n));
+    HIPCHK(hipMalloc(&c->bmark, sizeof(int) * (size_t)n));
+    HIPCHK(hipMalloc(&c->inbrick, (size_t)n));
+    HIPCHK(hipMalloc(&c->notbrick, (size_t)n));
+    HIPCHK(hipMalloc(&c->bcnt, 2 * sizeof(int)));
+    HIPCHK(hipMalloc(&c->bcub, c->bcub_bytes));
+    c->brick_cap = n;
+  }
+  const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
+  // candidates (into the rest list's buffer, which holds n), then the full check
+  hipLaunchKernelGGL(k_brick_quick, grid, block, 0, c->stream, n, c->nbr, c->info, c->bmark);
+  hipcub::CountingInputIterator<int> idx(0);
+  size_t bytes = c->bcub_bytes;
+  HIPCHK(hipcub::DeviceSelect::Flagged(c->bcub, bytes, idx, c->bmark, c->rest, c->bcnt, n, c->stream));
+  int ncand = 0;
+  HIPCHK(hipMemcpyAsync(&ncand, c->bcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemsetAsync(c->bmark, 0, sizeof(int) * (size_t)n, c->stream));
+  if (ncand > 0) hipLaunchKernelGGL(k_brick_full, dim3(ncand), dim3(64), 0, c->stream, n, c->rest, c->nbr, c->info, c->bmark);
+  bytes = c->bcub_bytes;
+  HIPCHK(hipcub::DeviceSelect::Flagged(c->bcub, bytes, idx, c->bmark, c->bricks, c->bcnt, n, c->stream));
+  int nb = 0;
+  HIPCHK(hipMemcpyAsync(&nb, c->bcnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (nb > n / 64) return set_err(FV_ERR_MESH, "brick classification: more blocks than the leaves allow");
+  HIPCHK(hipMemsetAsync(c->inbrick, 0, (size_t)n, c->stream));
+  if (nb > 0)
+    hipLaunchKernelGGL(k_brick_fill, dim3((64 * nb + FV_BLOCK - 1) / FV_BLOCK), block, 0, c->stream, nb, c->bricks,
+                       c->inbrick);
+  hipLaunchKernelGGL(k_not, grid, block, 0, c->stream, n, c->inbrick, c->notbrick);
+  bytes = c->bcub_bytes;
+  HIPCHK(hipcub::DeviceSelect::Flagged(c->bcub, bytes, idx, c->notbrick, c->rest, c->bcnt + 1, n, c->stream));
+  int nr = 0;
+  HIPCHK(hipMemcpyAsync(&nr, c->bcnt + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipGetLastError());
+  if (nr + 64 * nb != n) return set_err(FV_ERR_MESH, "brick classification: leaves counted twice or not at all");
+  c->nbricks = nb;
+  c->nrest = nr;
+  c->brick_ok = 1;
+  return FV_OK;
 }
 
 template <int NM, bool XR>
@@ -1387,27 +1896,59 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
     double *t = c->S;
     c->S = c->S2;
     c->S2 = t;
-  } else if (c->prec == FV_PREC_FP64) {
-    hipLaunchKernelGGL((k_recon_fast<NM, false, XR>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
-                       c->fine4, c->lim, c->SF, c->SFm);
-    if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
-    hipLaunchKernelGGL((k_flux_fast<NM, false, XR>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
-                       c->SFm, c->nbr, c->info, c->fine4, c->nanflag);
   } else {
-    if (c->cap_m < n) {
+    const bool mixed = c->prec == FV_PREC_MIXED;
+    if (mixed && c->cap_m < n) {
       HIPCHK(hipFree(c->SFm));
       c->SFm = nullptr;
       c->cap_m = 0;
       HIPCHK(hipMalloc(&c->SFm, sizeof(float) * (size_t)n * 6 * fvf::Rec<NM>::N));
       c->cap_m = n;
     }
-    hipLaunchKernelGGL((k_recon_fast<NM, true, XR>), grid, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
-                       c->fine4, c->lim, c->SF, c->SFm);
+    const bool bricks = c->use_bricks && !c->P.primtv && n >= 64;
+    if (bricks && !c->brick_ok) {
+      int rc = classify_bricks(c);
+      if (rc != FV_OK) return rc;
+    }
+    // per-leaf kernels: every leaf, or the leaves outside bricks
+    const int cnt = bricks ? c->nrest : n;
+    const int *list = bricks ? c->rest : nullptr;
+    const dim3 gl((cnt + FV_BLOCK - 1) / FV_BLOCK);
+    if (cnt > 0) {
+      if (mixed)
+        hipLaunchKernelGGL((k_recon_fast<NM, true, XR>), gl, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
+                           c->fine4, c->lim, c->SF, c->SFm, list, cnt);
+      else
+        hipLaunchKernelGGL((k_recon_fast<NM, false, XR>), gl, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
+                           c->fine4, c->lim, c->SF, c->SFm, list, cnt);
+    }
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
-    // the pipelined flux needs more registers: 2 waves per SIMD with FP32 face
-    // records (8 % faster), 1 with FP64 ones (35 % slower), so FP64 keeps k_flux_fast
-    hipLaunchKernelGGL((k_flux_pipe<NM, true, XR>), grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF,
-                       c->SFm, c->nbr, c->info, c->fine4, c->nanflag);
+    if (bricks && c->nbricks > 0) {
+      const dim3 gb(((unsigned)c->nbricks + 7u) & ~7u);   // a multiple of 8 for the XCD-aware brick order
+      if (mixed)
+        hipLaunchKernelGGL((k_stage_brick<NM, true, XR>), gb, dim3(BR_THREADS), 0, c->stream, n, c->P, c->Gdev, cs,
+                           c->bricks, c->nbricks, c->S, c->S2, c->SN, c->nbr, c->info, c->fine4, c->lim, c->inbrick,
+                           c->SF, c->SFm, c->nanflag);
+      else
+        hipLaunchKernelGGL((k_stage_brick<NM, false, XR>), gb, dim3(BR_THREADS), 0, c->stream, n, c->P, c->Gdev, cs,
+                           c->bricks, c->nbricks, c->S, c->S2, c->SN, c->nbr, c->info, c->fine4, c->lim, c->inbrick,
+                           c->SF, c->SFm, c->nanflag);
+    }
+    if (cnt > 0) {
+      // the pipelined flux needs more registers: 2 waves per SIMD with FP32 face
+      // records (8 % faster), 1 with FP64 ones (35 % slower), so FP64 keeps k_flux_fast
+      if (mixed)
+        hipLaunchKernelGGL((k_flux_pipe<NM, true, XR>), gl, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->S2, c->SN,
+                           c->SF, c->SFm, c->nbr, c->info, c->fine4, c->nanflag, list, cnt);
+      else
+        hipLaunchKernelGGL((k_flux_fast<NM, false, XR>), gl, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->S2, c->SN,
+                           c->SF, c->SFm, c->nbr, c->info, c->fine4, c->nanflag, list, cnt);
+    }
+    // the new state is in S2; the old one stays intact until every kernel of the
+    // stage has read its neighbours from it
+    double *t = c->S;
+    c->S = c->S2;
+    c->S2 = t;
   }
   return FV_OK;
 }

@@ -50,6 +50,13 @@ struct fv_dev {
   // when the install fails part way: the buffers and n may then disagree, so
   // every later entry point refuses the context instead of launching on it
   int broken = 0;
+  // 4x4x4 same-level blocks of the mesh for the block stage kernel (FP64 and
+  // mixed stages): their first leaves, the leaves outside them, a per-leaf flag
+  int use_bricks = 1, brick_ok = 0, brick_cap = 0, nbricks = 0, nrest = 0;
+  int *bricks = nullptr, *rest = nullptr, *bmark = nullptr, *bcnt = nullptr;
+  uint8_t *inbrick = nullptr, *notbrick = nullptr;
+  void *bcub = nullptr;
+  size_t bcub_bytes = 0;
 };
 
 // entry-point guard: a null or broken context is an error status, never a launch

@@ -501,6 +501,7 @@ int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
     if ((rc = fv_dev_reserve(c, nn + nn / 4, (int)(nfine + nfine / 4))) != FV_OK) return rc;
   }
   c->broken = 1;   // until the new mesh and state are installed
+  c->brick_ok = 0;
   c->n = nn;
   c->nfine = (int)nfine;
   HIPCHK(hipMemcpyAsync(c->S, S2, sizeof(double) * (size_t)nn * neq, hipMemcpyDeviceToDevice, c->stream));

@@ -93,3 +93,20 @@ def three_material_case(level=4, prelax=3, **kw):
         for k in range(3):
             S[i] += w[i, k] * F.init_pure(p, THREE_RHO, k, 1.0e6, u)
     return p, m, S
+
+
+def sphere_amr_mesh(params, base_level=4, max_level=6, nroot=(1, 1, 1), radius=0.3, L=1.0, centre=None):
+    """An AMR mesh refined towards a sphere: leaves within `radius` of the centre go to
+    max_level, the smoothing adds the 2:1 halo, so the mesh has large same-level regions
+    (many complete 4x4x4 blocks) and 2:1 faces between them."""
+    m = F.Mesh.uniform(nroot, (0.0, 0.0, 0.0), L / nroot[0], base_level, list(params.bc))
+    nq = F.neq(params.nmat)
+    c0 = np.array(centre if centre is not None else (0.5 * L * nroot[0] / nroot[0],) * 3)
+    S = np.tile(np.array([[1.0, 1e-8, 0, 0, 0, 2.5, 1 - 1e-8, 2.5, 1e-8][:nq]]), (m.n, 1))
+    for _ in range(max_level - base_level):
+        c = m.centres()
+        lv = m.arrays()["level"]
+        split = np.where(np.linalg.norm(c - c0, axis=1) < radius, max_level, lv).astype(np.int32)
+        split = m.criterionsmth(max_level, split)
+        S = m.adapt(params, split, S)
+    return m

@@ -447,6 +447,7 @@ def test_fused_stage_matches_two_kernel_path(name, prec):
     for fused in (False, True):
         dev = F.Device(p, m, prec=pr)
         dev.set_fused(fused)
+        dev.set_bricks(False)   # the per-leaf two-kernel path is the reference here
         dev.set_state(S)
         dev.set_limiter_flags(flags)
         for irk in (1, 2, 3):
@@ -476,6 +477,7 @@ def test_three_material_mixed_and_fused_paths():
     for fused in (False, True):
         dev = F.Device(p, m, prec=F.PREC_FP64)
         dev.set_fused(fused)
+        dev.set_bricks(False)
         dev.set_state(S)
         for irk in (1, 2, 3):
             dev.stage(irk)
