@@ -104,55 +104,63 @@ def build_uniform(level):
     return p, m, S
 
 
-def cpu_baseline(seconds=8.0):
-    """The CPU restatement (oracle/, gcc -O2) timed on the host on whole RK stages:
-    single core on Sod 32^3 (config 0), and OpenMP over all the threads the process
-    may use on Sod 128^3, 2.1 M leaves (mirrored faces recomputed, bit-identical to
-    the sequential run); about `seconds` of work each.  Also checks the device
-    against the CPU path."""
+def cpu_baseline(p, m, S, workload, seconds=6.0):
+    """The CPU restatement (oracle/, gcc -ffp-contract=off) timed on the host on the
+    headline workload itself (the same mesh, parameters and state as the GPU line),
+    on whole RK stages: the -O2 build with OpenMP over all the threads the process
+    may use (mirrored faces recomputed, bit-identical to the sequential run) for
+    about `seconds`, the -O2 build on one core for one stage, and the -O0 build (the
+    reference makefile's flags, src/makefile:39) on one core for one stage.  Also
+    checks one device stage against the CPU stage on that workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    from cases import sod_case
 
-    def timed(om, p, S, par, nthreads=0):
-        run = (lambda k: om.advance_par(p, S, k, nthreads)) if par else (lambda k: om.advance(p, S, k))
-        t0 = time.perf_counter()
-        run(1)
-        one = time.perf_counter() - t0
-        nsteps = max(2, int(seconds / max(one, 1e-3)))
-        t0 = time.perf_counter()
-        out = run(nsteps)
-        return out, nsteps, time.perf_counter() - t0
-
-    p, m, S = sod_case(level=6, rk=F.RK_SSP2, prelax=0)
+    t0 = time.perf_counter()
     om = O.OracleMesh(m)
-    So, nsteps, t1 = timed(om, p, S, False)
-    stages1 = 2 * nsteps
+    t_geo = time.perf_counter() - t0
+    nthr = int(O.lib().orc_num_threads())
+    # OpenMP, whole SSP-RK3 steps
+    t0 = time.perf_counter()
+    om.advance_par(p, S, 1, nthr)
+    one = time.perf_counter() - t0
+    nsteps = max(1, int(seconds / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    om.advance_par(p, S, nsteps, nthr)
+    t_par = time.perf_counter() - t0
+    stages_par = 3 * nsteps
+    # one core, one stage (stage 1 from SN = S), -O2 and -O0
+    t0 = time.perf_counter()
+    S1, _, _ = om.stage(p, 1, S, S)
+    t_1 = time.perf_counter() - t0
+    om0 = O.OracleMesh(m, variant="O0", geometry_from=om)
+    t0 = time.perf_counter()
+    S1_O0, _, _ = om0.stage(p, 1, S, S)
+    t_0 = time.perf_counter() - t0
+    # the device's stage 1 on the same state against the CPU stage
     dev = F.Device(p, m)
     dev.set_state(S)
-    dev.advance(nsteps)
+    dev.stage(1)
     Sg = dev.get_state()
     dev.close()
-    num = np.abs(Sg - So).sum(0)
-    den = np.abs(So).sum(0)
-    # the momentum components share the momentum magnitude as scale: the
-    # transverse momentum of the shock tube is zero up to rounding
-    mscale = np.sqrt((So[:, 2:5] ** 2).sum(1)).sum()
+    num = np.abs(Sg - S1).sum(0)
+    den = np.abs(S1).sum(0)
+    mscale = np.sqrt((S1[:, 2:5] ** 2).sum(1)).sum()
     den[2:5] = np.maximum(den[2:5], mscale)
     den[den == 0] = 1
     l1 = float((num / den).max())
-    nthr = int(O.lib().orc_num_threads())
-    # the 2.1 M-leaf grid of the GPU runs (Sod 128^3)
-    p2, m2, S2 = sod_case(level=8, rk=F.RK_SSP2, prelax=0)
-    _, nsteps2, t2 = timed(O.OracleMesh(m2), p2, S2, True, nthr)
-    stages2 = 2 * nsteps2
+    same_O0 = bool(np.array_equal(S1, S1_O0))
+    n = m.n
     return {
-        "value": m2.n * stages2 / t2 / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": nthr, "kind": "port",
-        "sample": f"Sod 128^3 ({m2.n} leaves), SSP-RK2, {stages2} stages, {t2:.1f} s, gcc -O2 -ffp-contract=off, "
-                  f"OpenMP {nthr} threads (os.cpu_count {os.cpu_count()})",
-        "value_1core": m.n * stages1 / t1 / 1e6,
-        "sample_1core": f"Sod 32^3 (32768 leaves), SSP-RK2, {stages1} stages, {t1:.1f} s, 1 thread",
-    }, l1, stages1
+        "value": n * stages_par / t_par / 1e6, "unit": "Mcell-updates/s per RK stage", "cores": nthr, "kind": "port",
+        "sample": f"{workload} ({n} leaves), SSP-RK3, PRELAX {p.prelax}, {stages_par} stages in {t_par:.1f} s, "
+                  f"gcc -O2 -ffp-contract=off, OpenMP {nthr} threads (os.cpu_count {os.cpu_count()}); geometry "
+                  f"{t_geo:.1f} s not timed",
+        "value_1core": n / t_1 / 1e6,
+        "sample_1core": f"{workload}, RK stage 1, {t_1:.1f} s, gcc -O2, 1 thread",
+        "value_O0": n / t_0 / 1e6,
+        "sample_O0": f"{workload}, RK stage 1, {t_0:.1f} s, gcc -O0 (the reference makefile's flags), 1 thread; "
+                     f"bit-identical to -O2: {same_O0}",
+    }, l1, 1
 
 
 def mixture_pressure(p, S):
@@ -677,9 +685,10 @@ def main():
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_m5:
         out["m5_stage"] = m5_leg()
     if not a.no_cpu_baseline:
-        cb, l1, stages = cpu_baseline()
+        cb, l1, stages = cpu_baseline(p, m, S, cfg["workload"])
         out["cpu_baseline"] = cb
         out["l1_rel_vs_cpu"] = l1
+        out["l1_rel_vs_cpu_case"] = "the workload above, RK stage 1, device FP64 path against the CPU restatement"
     print(json.dumps(out), flush=True)
 
 

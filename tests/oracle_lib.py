@@ -96,19 +96,21 @@ M5_REC = 39   # orc_m5_record layout (fv_oracle_m5.h)
 M5_REC_FIELDS = {"r": 0, "e": 1, "u": 2, "v": 3, "w": 4, "phi0": 5, "ra": 6, "avf": 9, "are": 12, "c2Y": 15,
                  "p_hydro": 18, "Amat": 21, "st": 30}
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_SO):
+def lib(variant="O2"):
+    """The oracle library: the -O2 build (the checker), or "O0", the same sources built
+    without optimisation as the reference makefile does (CPU baseline only)."""
+    if variant not in _libs:
+        path = ORACLE_SO if variant == "O2" else os.path.join(ORACLE_DIR, "_build", "libfvoracle_O0.so")
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(ORACLE_SO)
+        L = C.CDLL(path)
         P = C.POINTER
         dp = P(C.c_double)
         L.orc_geometry.argtypes = [P(orc_mesh), P(orc_cell)]
@@ -151,8 +153,8 @@ def lib():
         L.orc_refineh.argtypes = [P(orc_mesh), P(orc_params), ip, dp, ip, ip, ip, ip, dp, ip, ip]
         L.orc_bc_ghost.argtypes = [P(orc_params), dp, dp, dp, dp, C.c_int, dp, dp]
         L.orc_bc_ghost.restype = None
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _dp(a):
@@ -179,7 +181,8 @@ def crit_from(c):
 class OracleMesh:
     """Oracle view of a product mesh's topology (geometry is recomputed by the oracle)."""
 
-    def __init__(self, mesh, SI=None):
+    def __init__(self, mesh, SI=None, variant="O2", geometry_from=None):
+        self.L = lib(variant)
         a = mesh.arrays()
         m = mesh.m
         self.arrays = {k: np.ascontiguousarray(v, dtype=np.int32) for k, v in a.items()}
@@ -195,9 +198,14 @@ class OracleMesh:
             setattr(om, name, _ip(self.arrays[name]))
         self.om = om
         self.n = m.n
-        self.cells = (orc_cell * m.n)()
-        rc = lib().orc_geometry(C.byref(om), self.cells)
-        assert rc == 0
+        if geometry_from is not None:
+            # the same mesh's geometry, computed by another build of the same sources
+            assert geometry_from.n == m.n
+            self.cells = geometry_from.cells
+        else:
+            self.cells = (orc_cell * m.n)()
+            rc = self.L.orc_geometry(C.byref(om), self.cells)
+            assert rc == 0
 
     def set_SI(self, SI):
         self.arrays["SI"] = np.ascontiguousarray(SI, dtype=np.int32)
@@ -206,7 +214,7 @@ class OracleMesh:
     def advance(self, params, S, nsteps):
         p = params_from(params)
         S = np.ascontiguousarray(S, dtype=np.float64).copy()
-        rc = lib().orc_advance(C.byref(self.om), self.cells, C.byref(p), _dp(S), nsteps)
+        rc = self.L.orc_advance(C.byref(self.om), self.cells, C.byref(p), _dp(S), nsteps)
         if rc != 0:
             raise RuntimeError(f"orc_advance status {rc}")
         return S
@@ -215,7 +223,7 @@ class OracleMesh:
         """orc_advance_par: the same steps with OpenMP over the leaves (same bits)."""
         p = params_from(params)
         S = np.ascontiguousarray(S, dtype=np.float64).copy()
-        rc = lib().orc_advance_par(C.byref(self.om), self.cells, C.byref(p), _dp(S), nsteps, nthreads)
+        rc = self.L.orc_advance_par(C.byref(self.om), self.cells, C.byref(p), _dp(S), nsteps, nthreads)
         if rc != 0:
             raise RuntimeError(f"orc_advance_par status {rc}")
         return S
@@ -227,7 +235,7 @@ class OracleMesh:
         S = np.ascontiguousarray(S, dtype=np.float64).copy()
         SF = np.zeros((self.n, 6, nq))
         RHS = np.zeros((self.n, nq))
-        rc = lib().orc_stage(C.byref(self.om), self.cells, C.byref(p), irk, _dp(SN), _dp(S), _dp(SF), _dp(RHS))
+        rc = self.L.orc_stage(C.byref(self.om), self.cells, C.byref(p), irk, _dp(SN), _dp(S), _dp(SF), _dp(RHS))
         if rc != 0:
             raise RuntimeError(f"orc_stage status {rc}")
         return S, SF, RHS
@@ -237,7 +245,7 @@ class OracleMesh:
         nq = 3 * params.nmat + 3
         S = np.ascontiguousarray(S, dtype=np.float64)
         SF = np.zeros((self.n, 6, nq))
-        lib().orc_faceinterp(C.byref(self.om), self.cells, C.byref(p), _dp(S), _dp(SF))
+        self.L.orc_faceinterp(C.byref(self.om), self.cells, C.byref(p), _dp(S), _dp(SF))
         return SF
 
     def m5_rk(self, m5p, irk, SN, S, RHS, gfm_face):
@@ -246,7 +254,7 @@ class OracleMesh:
         S = np.ascontiguousarray(S, dtype=np.float64).copy()
         R = np.ascontiguousarray(RHS, dtype=np.float64)
         gf = np.ascontiguousarray(gfm_face, dtype=np.uint8)
-        rc = lib().orc_m5_rk(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), irk, _dp(SN), _dp(S), _dp(R),
+        rc = self.L.orc_m5_rk(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), irk, _dp(SN), _dp(S), _dp(R),
                              gf.ctypes.data_as(C.POINTER(C.c_uint8)))
         if rc != 0:
             raise RuntimeError(f"orc_m5_rk status {rc}")
@@ -256,7 +264,7 @@ class OracleMesh:
         Sb = np.ascontiguousarray(S_before, dtype=np.float64)
         S = np.ascontiguousarray(S, dtype=np.float64).copy()
         gf = np.ascontiguousarray(gfm_face, dtype=np.uint8)
-        rc = lib().orc_m5_gfm_rel(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(Sb), _dp(S),
+        rc = self.L.orc_m5_gfm_rel(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(Sb), _dp(S),
                                   gf.ctypes.data_as(C.POINTER(C.c_uint8)))
         if rc != 0:
             raise RuntimeError(f"orc_m5_gfm_rel status {rc}")
@@ -267,7 +275,7 @@ class OracleMesh:
         nq = 3 * m5p.nmat + 12
         S = np.ascontiguousarray(S, dtype=np.float64)
         SF = np.zeros((self.n, 6, nq))
-        rc = lib().orc_m5_faceinterp(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(S), _dp(SF))
+        rc = self.L.orc_m5_faceinterp(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(S), _dp(SF))
         if rc != 0:
             raise RuntimeError(f"orc_m5_faceinterp status {rc}")
         return SF
@@ -279,7 +287,7 @@ class OracleMesh:
         SF = np.zeros((self.n, 6, nq)) if SF is None else np.ascontiguousarray(SF, dtype=np.float64)
         RHS = np.zeros((self.n, nq))
         gf = np.zeros((self.n, 6), np.uint8)
-        rc = lib().orc_m5_residual(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(S), _dp(SF), _dp(RHS),
+        rc = self.L.orc_m5_residual(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(S), _dp(SF), _dp(RHS),
                                    gf.ctypes.data_as(C.POINTER(C.c_uint8)))
         if rc != 0:
             raise RuntimeError(f"orc_m5_residual status {rc}")
@@ -293,7 +301,7 @@ class OracleMesh:
         SF = np.zeros((self.n, 6, nq))
         RHS = np.zeros((self.n, nq))
         gf = np.zeros((self.n, 6), np.uint8)
-        rc = lib().orc_m5_stage(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), irk, _dp(SN), _dp(S), _dp(SF),
+        rc = self.L.orc_m5_stage(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), irk, _dp(SN), _dp(S), _dp(SF),
                                 _dp(RHS), gf.ctypes.data_as(C.POINTER(C.c_uint8)))
         if rc != 0:
             raise RuntimeError(f"orc_m5_stage status {rc}")
@@ -301,28 +309,28 @@ class OracleMesh:
 
     def m5_advance(self, m5p, S, nsteps):
         S = np.ascontiguousarray(S, dtype=np.float64).copy()
-        rc = lib().orc_m5_advance(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(S), nsteps)
+        rc = self.L.orc_m5_advance(C.byref(self.om), self.cells, C.byref(m5_from(m5p)), _dp(S), nsteps)
         if rc != 0:
             raise RuntimeError(f"orc_m5_advance status {rc}")
         return S
 
     def interface_location(self, params, crit, S):
         SI = np.zeros((self.n, 2), dtype=np.int32)
-        lib().orc_interface_location(C.byref(self.om), C.byref(params_from(params)), C.byref(crit_from(crit)),
+        self.L.orc_interface_location(C.byref(self.om), C.byref(params_from(params)), C.byref(crit_from(crit)),
                                      _dp(np.ascontiguousarray(S)), _ip(SI))
         return SI
 
     def criterionsplt(self, params, crit, S, SI):
         split = np.zeros(self.n, dtype=np.int32)
         SI = np.ascontiguousarray(SI, dtype=np.int32)
-        lib().orc_criterionsplt(C.byref(self.om), C.byref(params_from(params)), C.byref(crit_from(crit)),
+        self.L.orc_criterionsplt(C.byref(self.om), C.byref(params_from(params)), C.byref(crit_from(crit)),
                                 _dp(np.ascontiguousarray(S)), _ip(SI), _ip(split))
         return split
 
     def criterionsmth(self, LEVEL, split):
         """orc_criterionsmth: the smoothed target levels (a new array)."""
         sp = np.ascontiguousarray(split, dtype=np.int32).copy()
-        rc = lib().orc_criterionsmth(C.byref(self.om), int(LEVEL), _ip(sp))
+        rc = self.L.orc_criterionsmth(C.byref(self.om), int(LEVEL), _ip(sp))
         if rc != 0:
             raise RuntimeError(f"orc_criterionsmth status {rc}")
         return sp
@@ -336,7 +344,7 @@ class OracleMesh:
         out = {"root": np.zeros(cap, np.int32), "level": np.zeros(cap, np.int32), "ci": np.zeros((cap, 3), np.int32),
                "S": np.zeros((cap, nq)), "SI": np.zeros((cap, 2), np.int32), "split": np.zeros(cap, np.int32)}
         nout = C.c_int()
-        rc = lib().orc_refineh(C.byref(self.om), C.byref(params_from(params)), _ip(sp), _dp(S), C.byref(nout),
+        rc = self.L.orc_refineh(C.byref(self.om), C.byref(params_from(params)), _ip(sp), _dp(S), C.byref(nout),
                                _ip(out["root"]), _ip(out["level"]), _ip(out["ci"]), _dp(out["S"]), _ip(out["SI"]),
                                _ip(out["split"]))
         if rc != 0:
@@ -344,7 +352,7 @@ class OracleMesh:
         return {k: v[:nout.value].copy() for k, v in out.items()}
 
     def cfl_dt(self, params, S, cfl):
-        return lib().orc_cfl_dt(C.byref(self.om), self.cells, C.byref(params_from(params)),
+        return self.L.orc_cfl_dt(C.byref(self.om), self.cells, C.byref(params_from(params)),
                                 _dp(np.ascontiguousarray(S)), cfl)
 
     def geometry(self):
