@@ -134,6 +134,7 @@ class fv_m5_params(C.Structure):
         ("order", C.c_int),
         ("rk_steps", C.c_int),
         ("yield_", C.c_double),
+        ("fix", C.c_int),
     ]
 
 
@@ -343,8 +344,12 @@ def m5_neq(nmat):
     return 3 * nmat + 12
 
 
+# fv_m5_params.fix: the consistent model-5 update (FV_M5_FIX_SIGNS of forestfv_m5.h)
+M5_FIX_SIGNS = 1
+
+
 def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush=(2.6e10, 0.0, 0.0), amin=1e-6,
-                   avf_lim=0.5, dt=1e-9, zerou=(0, 0, 0), prelax=1, order=2, rk_steps=3, yield_stress=0.0):
+                   avf_lim=0.5, dt=1e-9, zerou=(0, 0, 0), prelax=1, order=2, rk_steps=3, yield_stress=0.0, fix=0):
     """Model-5 material table (MATERGAMA, MATERPINF, MATERMUSH) and stage constants."""
     p = fv_m5_params()
     p.nmat = nmat
@@ -355,6 +360,7 @@ def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush
         p.zerou[k] = zerou[k]
     p.prelax, p.order, p.rk_steps = prelax, order, rk_steps
     p.yield_ = yield_stress
+    p.fix = fix
     return p
 
 

@@ -41,6 +41,9 @@ struct M5P {
   int zerou[3];
   int prelax, order;
   double yield;   // von Mises yield stress of perfect plasticity, 0 = elastic
+  int fix;        // 1: the consistent update (FV_M5_FIX_SIGNS), 0: the published one
+  double dtu;     // signed step of the update: S = SN + dtu RHS / V, +dt published, -dt consistent
+  double sg;      // sign of sigma in the momentum flux: +1 published (rho u u + sigma), -1 consistent
 };
 
 template <int NM>
@@ -298,14 +301,19 @@ __device__ void star_region(const M5P &P, const double *F, const Rec5<NM> &sL, c
   vel[1] = u_S * n[1] + v_S * m[1] + w_S * l[1];
   vel[2] = u_S * n[2] + v_S * m[2] + w_S * l[2];
   const double re = s.r * s.e;
-  const double Em_S = (re * (uK - SK) + s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
-                      (r_S * (u_S + SK));
+  // consistent: - s11K uK over r* (u* - S_K); published (m5_star_region.c:73): + s11K uK over r* (u* + S_K)
+  const double Em_S = P.fix ? (re * (uK - SK) - s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
+                                  (r_S * (u_S - SK))
+                            : (re * (uK - SK) + s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
+                                  (r_S * (u_S + SK));
   double A[9], t[9], AS[9];
   rot_tensor(t, s.A, n, m, l);
 #pragma unroll
   for (int k = 0; k < 9; k++) A[k] = t[k];
 #pragma unroll
-  for (int j = 0; j < 3; ++j) AS[j] = (A[j] * (uK - SK) - A[3 + j] * (vK - v_S) + A[6 + j] * (wK - w_S)) / (u_S - SK);
+  for (int j = 0; j < 3; ++j)
+    AS[j] = P.fix ? (A[j] * (uK - SK) + A[3 + j] * (vK - v_S) + A[6 + j] * (wK - w_S)) / (u_S - SK)
+                  : (A[j] * (uK - SK) - A[3 + j] * (vK - v_S) + A[6 + j] * (wK - w_S)) / (u_S - SK);
 #pragma unroll
   for (int j = 0; j < 3; ++j) { AS[3 + j] = A[3 + j]; AS[6 + j] = A[6 + j]; }
   {
@@ -325,11 +333,11 @@ __device__ void star_region(const M5P &P, const double *F, const Rec5<NM> &sL, c
     const double g = P.gamma[v], pi = P.pinf[v];
     const double rm = (s.ra[v] / s.avf[v]);
     const double rm_S = (s.ra[v] / s.avf[v]) * (uK - SK) / (u_S - SK);
-    const double p_k = -s.ph[v];
+    const double p_k = P.fix ? s.ph[v] : -s.ph[v];   // published: -p_hydro
     const double p_S = (p_k + pi) * (((g - 1.0) * rm + (g + 1.0) * rm_S) / ((g - 1.0) * rm_S + (g + 1.0) * rm)) - pi;
     const double are_el = elastic_energy_v1(G, P.mush[v]);
     const double e_el = are_el / (s.avf[v] * rm_S);
-    e_S[v] = (p_S + g * pi) / (rm_S * (g + 1.0)) + e_el;
+    e_S[v] = (p_S + g * pi) / (rm_S * (P.fix ? (g - 1.0) : (g + 1.0))) + e_el;   // published: gamma + 1
   }
 #pragma unroll
   for (int v = 0; v < NM; ++v) W.Us[v] = ar_S[v];
@@ -504,7 +512,7 @@ __global__ __launch_bounds__(256) void k_m5_rk(int n, M5P P, int irk, const Geo 
           const int e = L::I3 + v;
           const double phi_old = Sin[at(e)] - P.avf_lim;
           const double buf = (1.0 / Vol) * RHS[at(e)];
-          const double as_new = SN[at(e)] + P.dt * buf;
+          const double as_new = SN[at(e)] + P.dtu * buf;
           const double phi_new = as_new - P.avf_lim;
           if (phi_old * phi_new < 0.0) flag_rk = 1;
         }
@@ -513,24 +521,24 @@ __global__ __launch_bounds__(256) void k_m5_rk(int n, M5P P, int irk, const Geo 
       for (int e = 0; e < L::NEQ; e++) {
         if (flag_rk && e < L::I2) { Sout[at(e)] = Sin[at(e)]; continue; }   // set by m5_gfm
         const double buf = (1.0 / Vol) * RHS[at(e)];
-        Sout[at(e)] = SN[at(e)] + P.dt * buf;
+        Sout[at(e)] = SN[at(e)] + P.dtu * buf;
       }
       break;
     }
     case 2:
 #pragma unroll
       for (int e = 0; e < L::NEQ; e++)
-        Sout[at(e)] = (3.0 / 4.0) * SN[at(e)] + (1.0 / 4.0) * Sin[at(e)] + (1.0 / 4.0) * P.dt * RHS[at(e)] / Vol;
+        Sout[at(e)] = (3.0 / 4.0) * SN[at(e)] + (1.0 / 4.0) * Sin[at(e)] + (1.0 / 4.0) * P.dtu * RHS[at(e)] / Vol;
       break;
     case 3:
 #pragma unroll
       for (int e = 0; e < L::NEQ; e++)
-        Sout[at(e)] = (1.0 / 3.0) * SN[at(e)] + (2.0 / 3.0) * Sin[at(e)] + (2.0 / 3.0) * P.dt * RHS[at(e)] / Vol;
+        Sout[at(e)] = (1.0 / 3.0) * SN[at(e)] + (2.0 / 3.0) * Sin[at(e)] + (2.0 / 3.0) * P.dtu * RHS[at(e)] / Vol;
       break;
     case 4:
 #pragma unroll
       for (int e = 0; e < L::NEQ; e++)
-        Sout[at(e)] = (1.0 / 2.0) * SN[at(e)] + (1.0 / 2.0) * Sin[at(e)] + (1.0 / 2.0) * P.dt * RHS[at(e)] / Vol;
+        Sout[at(e)] = (1.0 / 2.0) * SN[at(e)] + (1.0 / 2.0) * Sin[at(e)] + (1.0 / 2.0) * P.dtu * RHS[at(e)] / Vol;
       break;
     default:
 #pragma unroll
@@ -637,7 +645,7 @@ __global__ void k_m5_gfm_seq(int n, const int *__restrict__ nlist_p, const int *
   for (int t = 0; t < nlist; t++) {
     const int i = list[t];
     auto at = [&](int e) { return (size_t)e * n + i; };
-    const double dV = P.dt / Gp->vol[fvk::cell_level(info[i])];
+    const double dV = P.dtu / Gp->vol[fvk::cell_level(info[i])];
     double rho_new = 0.0;
 #pragma unroll
     for (int v = 0; v < NM; ++v) rho_new += S[at(L::I0 + v)] + (dV)*RHS[at(L::I0 + v)];
@@ -723,16 +731,16 @@ __device__ void apply_bc5(int code, const double *F, const Rec5<NM> &sL, Rec5<NM
 }
 
 // m5_flux.c:10-39: the physical flux rows of a record, F.n per equation for
-// the mass, momentum (+ sigma) and energy rows
+// the mass, momentum (sg sigma: + published, - consistent) and energy rows
 template <int NM>
-__device__ void phys_rows(const Rec5<NM> &s, const double *n, double *adv) {
+__device__ void phys_rows(const Rec5<NM> &s, const double *n, double sg, double *adv) {
   using L = L5<NM>;
   auto dot = [&](double f0, double f1, double f2) { return (f0 - 0.0) * n[0] + (f1 - 0.0) * n[1] + (f2 - 0.0) * n[2]; };
 #pragma unroll
   for (int v = 0; v < NM; ++v) adv[L::I0 + v] = dot(s.u * s.ra[v], s.v * s.ra[v], s.w * s.ra[v]);
-  adv[L::I1 + 0] = dot(s.u * s.u * s.r + s.st[0], s.v * s.u * s.r + s.st[1], s.w * s.u * s.r + s.st[2]);
-  adv[L::I1 + 1] = dot(s.u * s.v * s.r + s.st[3], s.v * s.v * s.r + s.st[4], s.w * s.v * s.r + s.st[5]);
-  adv[L::I1 + 2] = dot(s.u * s.w * s.r + s.st[6], s.v * s.w * s.r + s.st[7], s.w * s.w * s.r + s.st[8]);
+  adv[L::I1 + 0] = dot(s.u * s.u * s.r + sg * s.st[0], s.v * s.u * s.r + sg * s.st[1], s.w * s.u * s.r + sg * s.st[2]);
+  adv[L::I1 + 1] = dot(s.u * s.v * s.r + sg * s.st[3], s.v * s.v * s.r + sg * s.st[4], s.w * s.v * s.r + sg * s.st[5]);
+  adv[L::I1 + 2] = dot(s.u * s.w * s.r + sg * s.st[6], s.v * s.w * s.r + sg * s.st[7], s.w * s.w * s.r + sg * s.st[8]);
   adv[L::I2] = dot(s.r * s.e * s.u - s.st[0] * s.u - s.st[1] * s.v - s.st[2] * s.w,
                    s.r * s.e * s.v - s.st[3] * s.u - s.st[4] * s.v - s.st[5] * s.w,
                    s.r * s.e * s.w - s.st[6] * s.u - s.st[7] * s.v - s.st[8] * s.w);
@@ -786,14 +794,14 @@ __device__ bool hllc_subface(const M5P &P, const double *F, int bc, const Rec5<N
     for (int iv = 0; iv < L::NCONS; ++iv)
       if (isnan(W.Us[iv])) return false;
   } else {
-    // m5_flux.c:44-54: the side's own velocity, fractions and -A
+    // m5_flux.c:44-54: the side's own velocity, fractions and -A (+A consistent)
     W.unc[0] = s.u; W.unc[1] = s.v; W.unc[2] = s.w;
 #pragma unroll
     for (int v = 0; v < NM - 1; ++v) W.avf_S[v] = s.avf[v];
 #pragma unroll
     for (int v = 0; v < NM; ++v) W.are_S[v] = s.are[v];
 #pragma unroll
-    for (int k = 0; k < 9; k++) W.AS[k] = -s.A[k];
+    for (int k = 0; k < 9; k++) W.AS[k] = P.fix ? s.A[k] : -s.A[k];
   }
   if (bc == 1) {
     // m5_hllc.c:66-88: F_hllc = 0, the face velocity and flux are the means of
@@ -814,9 +822,9 @@ __device__ bool hllc_subface(const M5P &P, const double *F, int bc, const Rec5<N
       const double su = (k == 0) ? s.u : (k == 1 ? s.v : s.w);
 #pragma unroll
       for (int v = 0; v < NM; ++v) adv[L::I0 + v] = su * s.ra[v];
-      adv[L::I1 + 0] = su * s.u * s.r + s.st[0 + k];
-      adv[L::I1 + 1] = su * s.v * s.r + s.st[3 + k];
-      adv[L::I1 + 2] = su * s.w * s.r + s.st[6 + k];
+      adv[L::I1 + 0] = su * s.u * s.r + P.sg * s.st[0 + k];
+      adv[L::I1 + 1] = su * s.v * s.r + P.sg * s.st[3 + k];
+      adv[L::I1 + 2] = su * s.w * s.r + P.sg * s.st[6 + k];
       adv[L::I2] = s.r * s.e * su - s.st[3 * k + 0] * s.u - s.st[3 * k + 1] * s.v - s.st[3 * k + 2] * s.w;
 #pragma unroll
       for (int v = 0; v < NM - 1; ++v) adv[L::I3 + v] = uu * s.avf[v];
@@ -845,7 +853,7 @@ __device__ bool hllc_subface(const M5P &P, const double *F, int bc, const Rec5<N
     return true;
   }
   double adv[L::NEQ];
-  phys_rows<NM>(s, nv, adv);
+  phys_rows<NM>(s, nv, P.sg, adv);
   bool ok = true;
 #pragma unroll
   for (int iv = 0; iv < L::NCONS; ++iv) {
@@ -1077,8 +1085,9 @@ __device__ int r3_finish(const M5P &P, RelaxIn<NM> &R, const double *pk, int ire
   }
   const int i_as = ((R.avf[0] < R.avf[1]) && (P.mush[1] != 0.0)) ? 1 : 0;
   const double ratio = (i_as == 1) ? (R.avf[1] / R.avf_init[1]) : (R.avf[0] / R.avf_init[0]);
+  const double sA = P.fix ? cbrt(1.0 / ratio) : ratio;   // consistent: det A follows 1 / alpha
 #pragma unroll
-  for (int i = 0; i < 9; i++) R.A[i] = R.A[i] * ratio;
+  for (int i = 0; i < 9; i++) R.A[i] = R.A[i] * sA;
   double are[NM];
 #pragma unroll
   for (int iv = 0; iv < NM; ++iv) are[iv] = R.are[iv];
@@ -1154,8 +1163,10 @@ __device__ int relax5(const M5P &P, double *x) {
       R.avf[iv] = R.ar[iv] / r_m;
     }
     vf_limit2<NM>(a, R.avf);
+    const double ratio = R.avf[0] / R.avf_init[0];
+    const double sA = P.fix ? cbrt(1.0 / ratio) : ratio;   // consistent: det A follows 1 / alpha
 #pragma unroll
-    for (int i = 0; i < 9; i++) R.A[i] = R.A[i] * (R.avf[0] / R.avf_init[0]);
+    for (int i = 0; i < 9; i++) R.A[i] = R.A[i] * sA;
   }
   return relax_out<NM>(P, R, are, x);
 }
@@ -1520,6 +1531,19 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
     }
 #pragma unroll
     for (int k = 0; k < NM - 1; ++k) Fv[L::I3 + k] = avf[k];
+    if (P.fix) {
+      // consistent: each phase's alpha-rho-e follows its reconstructed fraction, which
+      // keeps the phase's cell pressure at the face
+      double sf = 0.0, sc = 0.0;
+#pragma unroll
+      for (int k = 0; k < NM - 1; ++k) { sf += Fv[L::I3 + k]; sc += Sc[L::I3 + k]; }
+#pragma unroll
+      for (int k = 0; k < NM; ++k) {
+        const double af = (k < NM - 1) ? Fv[L::I3 + k] : 1.0 - sf;
+        const double ac = (k < NM - 1) ? Sc[L::I3 + k] : 1.0 - sc;
+        Fv[L::I4 + k] = Sc[L::I4 + k] * (af / ac);
+      }
+    }
     // PRELAX 3 relaxes the face states in k_m5_relax3_start / _finish
     const int rc = (P.prelax == 3) ? 0 : relax5<NM>(P, Fv);
     if (rc) atomicOr(flag, rc == -3 ? 4 : 1);
@@ -1611,7 +1635,9 @@ __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__
         t += (A) * out[v];
         ff[v] = -t;
       }
-      uf0 = -(un[0] / ((double)1)); uf1 = -(un[1] / ((double)1)); uf2 = -(un[2] / ((double)1));
+      // the star velocity is a global vector: the published code negates it with the flux
+      const double us = P.fix ? 1.0 : -1.0;
+      uf0 = us * (un[0] / ((double)1)); uf1 = us * (un[1] / ((double)1)); uf2 = us * (un[2] / ((double)1));
       Area_f += A;
     } else {
       const int Nneig = (kind == FV_FK_FINE) ? 4 : 1;
@@ -1651,6 +1677,29 @@ __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__
     const double u_nc = uf0, nx = G.frame[f][0];
     const double v_nc = uf1, ny = G.frame[f][1];
     const double w_nc = uf2, nz = G.frame[f][2];
+    if (P.fix) {
+      // consistent sources (the residual is subtracted in this mode): the published alpha and
+      // alpha-rho-e terms negated, and for A_ij n_i (A_kj u_k) - A_ij (u.n), all from the cell
+#pragma unroll
+      for (int k = 0; k < NM - 1; ++k)
+        rhs[L::I3 + k] -= Area_f * (src_avf[k] * u_nc * nx + src_avf[k] * v_nc * ny + src_avf[k] * w_nc * nz);
+#pragma unroll
+      for (int k = 0; k < NM; ++k) {
+        const double *s = src_st[k];
+        rhs[L::I4 + k] -= Area_f * (s[0] * u_nc * nx + s[1] * v_nc * nx + s[2] * w_nc * nx + s[3] * u_nc * ny +
+                                    s[4] * v_nc * ny + s[5] * w_nc * ny + s[6] * u_nc * nz + s[7] * v_nc * nz +
+                                    s[8] * w_nc * nz);
+      }
+      const double nn[3] = {nx, ny, nz}, uu[3] = {u_nc, v_nc, w_nc};
+      const double un = u_nc * nx + v_nc * ny + w_nc * nz;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double cu = Am[0 * 3 + j] * uu[0] + Am[1 * 3 + j] * uu[1] + Am[2 * 3 + j] * uu[2];
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) rhs[L::I5 + 3 * ii + j] += Area_f * (nn[ii] * cu - Am[ii * 3 + j] * un);
+      }
+      continue;
+    }
 #pragma unroll
     for (int k = 0; k < NM - 1; ++k)
       rhs[L::I3 + k] += Area_f * (src_avf[k] * u_nc * nx + src_avf[k] * v_nc * ny + src_avf[k] * w_nc * nz);
@@ -1715,6 +1764,10 @@ int m5_params(const fv_m5_params *p, M5P &P) {
   P.prelax = p->prelax; P.order = p->order;
   if (!(p->yield >= 0.0)) return fv_set_err(FV_ERR_ARG, "model 5: the yield stress must be >= 0");
   P.yield = p->yield;
+  if (p->fix != 0 && p->fix != FV_M5_FIX_SIGNS) return fv_set_err(FV_ERR_ARG, "model 5: fix must be 0 or FV_M5_FIX_SIGNS");
+  P.fix = p->fix ? 1 : 0;
+  P.dtu = P.fix ? -p->dt : p->dt;
+  P.sg = P.fix ? -1.0 : 1.0;
   return FV_OK;
 }
 

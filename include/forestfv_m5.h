@@ -40,7 +40,25 @@ typedef struct {
   double yield;                /* von Mises yield stress of the solid (material 0) for perfect
                                   plasticity, 0 = elastic; new: the reference's plastic_rlx is
                                   commented out (m5_relaxation_solid.c:14) */
+  int    fix;                  /* FV_M5_FIX_SIGNS: the consistent model-5 update (new, below); 0 = the
+                                  published code */
 } fv_m5_params;
+
+/* Corrections of the published model-5 update, one mode (0 reproduces it):
+ * the momentum flux rho u u - sigma (m5_flux.c:21-33 adds sigma); S = SN -
+ * dt RHS / V (m5_RK.c:49,58-79 add), also in the stage-1 re-projection of
+ * m5_gfm.c; the mirrored side of a face reuses the star velocity as it is
+ * (m5_residual.c:62-103 negates it); the non-conservative sources with the
+ * sign of that update, and for the deformation tensor n_i (A_kj u_k) - A_ij
+ * (u.n) per face (m5_residual.c:176-209); +A in the supersonic advective A
+ * flux (m5_flux.c:54-56); MUSCL face states whose alpha-rho-e follow their
+ * reconstructed fractions (m5_input.c:196-204 keep the cell value); in
+ * m5_star_region.c the star energy with -s11K uK over r* (u* - S_K) (:73),
+ * + A_1j (vK - v*) in the star A (:84) and the phasic star energy from +p_k
+ * with gamma - 1 (:118-125); and the relaxation scales A by cbrt(alpha0 /
+ * alpha), so that det A follows the solid's density (m5_relaxationvec.c
+ * scales by alpha / alpha0). */
+#define FV_M5_FIX_SIGNS 1
 
 int fv_m5_neq(int nmat);
 

@@ -329,8 +329,14 @@ static int m5_star_region(const lay5_t *L, const orc_m5_params *p, const fr5_t *
   vel[1] = u_S * F->ny + v_S * F->my + w_S * F->ly;
   vel[2] = u_S * F->nz + v_S * F->mz + w_S * F->lz;
   double re = s->r * s->e;
-  double Em_S = (re * (uK - SK) + s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
-                (r_S * (u_S + SK));
+  const int fix = p->fix & ORC_M5_FIX_SIGNS;
+  double Em_S;
+  if (fix)
+    Em_S = (re * (uK - SK) - s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
+           (r_S * (u_S - SK));
+  else
+    Em_S = (re * (uK - SK) + s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
+           (r_S * (u_S + SK));
   double A[3][3], t[9], AS[3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) A[i][j] = s->Amat[i * 3 + j];
@@ -338,7 +344,8 @@ static int m5_star_region(const lay5_t *L, const orc_m5_params *p, const fr5_t *
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) A[i][j] = t[i * 3 + j];
   for (int j = 0; j < 3; ++j)
-    AS[0][j] = (A[0][j] * (uK - SK) - A[1][j] * (vK - v_S) + A[2][j] * (wK - w_S)) / (u_S - SK);
+    AS[0][j] = fix ? (A[0][j] * (uK - SK) + A[1][j] * (vK - v_S) + A[2][j] * (wK - w_S)) / (u_S - SK)
+                   : (A[0][j] * (uK - SK) - A[1][j] * (vK - v_S) + A[2][j] * (wK - w_S)) / (u_S - SK);
   for (int j = 0; j < 3; ++j) { AS[1][j] = A[1][j]; AS[2][j] = A[2][j]; }
   rot_tensor(t, AS, F->nx, F->mx, F->lx, F->ny, F->my, F->ly, F->nz, F->mz, F->lz);
   for (int i = 0; i < 3; ++i)
@@ -351,11 +358,11 @@ static int m5_star_region(const lay5_t *L, const orc_m5_params *p, const fr5_t *
     double g = p->gamma[v], pi = p->pinf[v];
     double rm = (s->ra[v] / s->avf[v]);
     double rm_S = (s->ra[v] / s->avf[v]) * (uK - SK) / (u_S - SK);
-    double p_k = -s->p_hydro[v];
+    double p_k = fix ? s->p_hydro[v] : -s->p_hydro[v];
     double p_S = (p_k + pi) * (((g - 1.0) * rm + (g + 1.0) * rm_S) / ((g - 1.0) * rm_S + (g + 1.0) * rm)) - pi;
     double are_el = elastic_energy_v1(G, p->mush[v]);
     double e_el = are_el / (s->avf[v] * rm_S);
-    e_S[v] = (p_S + g * pi) / (rm_S * (g + 1.0)) + e_el;
+    e_S[v] = (p_S + g * pi) / (rm_S * (fix ? (g - 1.0) : (g + 1.0))) + e_el;
   }
   for (int v = 0; v < L->n0; ++v) W->Us[L->i0 + v] = ar_S[v];
   for (int v = 0; v < 3; ++v) W->Us[L->i1 + v] = r_S * vel[v];
@@ -515,7 +522,7 @@ static int m5_gfm(const lay5_t *L, const orc_m5_params *p, const orc_mesh *m, co
   const int neq = L->neq;
   double *Sc = S + (size_t)c * neq;
   const double *R = RHS + (size_t)c * neq;
-  const double dV = p->dt / cl->Vol;
+  const double dV = ((p->fix & ORC_M5_FIX_SIGNS) ? -p->dt : p->dt) / cl->Vol;
   double rho_new = 0.0;
   for (int v = 0; v < L->n0; ++v) rho_new += Sc[L->i0 + v] + (dV) * R[L->i0 + v];
   double ut = Sc[L->i1 + 0] + (dV) * R[L->i1 + 0];
@@ -551,6 +558,8 @@ int orc_m5_rk(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, 
               double *S, const double *RHS, const unsigned char *gfm_face) {
   lay5_t L = mk_lay5(p->nmat);
   const int neq = L.neq;
+  /* published S = SN + dt RHS / V; consistent S = SN - dt RHS / V */
+  const double DT = (p->fix & ORC_M5_FIX_SIGNS) ? -p->dt : p->dt;
   double *SN = (double *)SN_in;   /* written only by case 0 (SN = S) */
   for (int c = 0; c < m->n; c++) {
     double *Sc = S + (size_t)c * neq, *SNc = SN + (size_t)c * neq;
@@ -568,7 +577,7 @@ int orc_m5_rk(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, 
           if (p->mush[v] != 0.0) {
             int e = L.i3 + v;
             double buf = (1.0 / Vol) * R[e];
-            double as_new = SNc[e] + p->dt * buf;
+            double as_new = SNc[e] + DT * buf;
             phi_new[v] = as_new - p->avf_lim;
             if (phi_old[v] * phi_new[v] < 0.0) flag_rk = 1;
           }
@@ -577,7 +586,7 @@ int orc_m5_rk(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, 
           for (int e = 0; e < neq; e++) {
             if (e >= L.i2) {
               double buf = (1.0 / Vol) * R[e];
-              Sc[e] = SNc[e] + p->dt * buf;
+              Sc[e] = SNc[e] + DT * buf;
             }
           }
           int rc = m5_gfm(&L, p, m, &cells[c], c, S, RHS, gfm_face);
@@ -585,19 +594,19 @@ int orc_m5_rk(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, 
         } else {
           for (int e = 0; e < neq; e++) {
             double buf = (1.0 / Vol) * R[e];
-            Sc[e] = SNc[e] + p->dt * buf;
+            Sc[e] = SNc[e] + DT * buf;
           }
         }
         break;
       }
       case 2:
-        for (int e = 0; e < neq; e++) Sc[e] = (3.0 / 4.0) * SNc[e] + (1.0 / 4.0) * Sc[e] + (1.0 / 4.0) * p->dt * R[e] / Vol;
+        for (int e = 0; e < neq; e++) Sc[e] = (3.0 / 4.0) * SNc[e] + (1.0 / 4.0) * Sc[e] + (1.0 / 4.0) * DT * R[e] / Vol;
         break;
       case 3:
-        for (int e = 0; e < neq; e++) Sc[e] = (1.0 / 3.0) * SNc[e] + (2.0 / 3.0) * Sc[e] + (2.0 / 3.0) * p->dt * R[e] / Vol;
+        for (int e = 0; e < neq; e++) Sc[e] = (1.0 / 3.0) * SNc[e] + (2.0 / 3.0) * Sc[e] + (2.0 / 3.0) * DT * R[e] / Vol;
         break;
       case 4:
-        for (int e = 0; e < neq; e++) Sc[e] = (1.0 / 2.0) * SNc[e] + (1.0 / 2.0) * Sc[e] + (1.0 / 2.0) * p->dt * R[e] / Vol;
+        for (int e = 0; e < neq; e++) Sc[e] = (1.0 / 2.0) * SNc[e] + (1.0 / 2.0) * Sc[e] + (1.0 / 2.0) * DT * R[e] / Vol;
         break;
       default:
         break;
@@ -801,7 +810,8 @@ static int m5_relaxvec(const lay5_t *L, const orc_m5_params *p, double *vec) {
       avf[iv] = ar[iv] / r_m;
     }
     vf_limit2(L, a, avf);
-    for (int i = 0; i < 9; i++) A[i] = A[i] * (avf[0] / avf_init[0]);
+    const double ratio = avf[0] / avf_init[0];
+    for (int i = 0; i < 9; i++) A[i] = A[i] * ((p->fix & ORC_M5_FIX_SIGNS) ? cbrt(1.0 / ratio) : ratio);
   } else if (p->prelax == 3) {   /* m5_relaxationvec.c:365-560 */
     double rhoKs[ORC_MAXMAT] = {0}, pk[ORC_MAXMAT];
     for (int iv = 0; iv < nm; ++iv) {
@@ -847,7 +857,8 @@ static int m5_relaxvec(const lay5_t *L, const orc_m5_params *p, double *vec) {
     }
     int i_as = 0;
     if ((avf[0] < avf[1]) && (Mu[1] != 0.0)) i_as = 1;
-    for (int i = 0; i < 9; i++) A[i] = A[i] * (avf[i_as] / avf_init[i_as]);
+    const double ratio = avf[i_as] / avf_init[i_as];
+    for (int i = 0; i < 9; i++) A[i] = A[i] * ((p->fix & ORC_M5_FIX_SIGNS) ? cbrt(1.0 / ratio) : ratio);
   }
   if (p->prelax >= 1 && p->prelax <= 3) {
     /* re-initialisation: pressure from the total energy (m5_relaxationvec.c:157-189 and the same in each branch) */
@@ -1085,6 +1096,18 @@ static int m5_faceinterp_leaf(const lay5_t *L, const orc_m5_params *p, const orc
       for (int j = 0; j < L->n3; ++j) { double per = avf[j] / avfsum; avf[j] = avf[j] - per * dif; }
     }
     for (int iv = 0; iv < L->n3; ++iv) SF[iv + L->i3] = avf[iv];
+    if (p->fix & ORC_M5_FIX_SIGNS) {
+      /* the phasic energies follow their fractions, so that each phase keeps its cell
+       * pressure at the face (the published block keeps the cell's alpha-rho-e with
+       * reconstructed fractions, a pressure error of order gamma p_inf d(alpha)/alpha) */
+      double sf = 0.0, sc = 0.0;
+      for (int k = 0; k < L->n3; ++k) { sf += SF[L->i3 + k]; sc += SLc[L->i3 + k]; }
+      for (int k = 0; k < L->n4; ++k) {
+        const double af = (k < L->n3) ? SF[L->i3 + k] : 1.0 - sf;
+        const double ac = (k < L->n3) ? SLc[L->i3 + k] : 1.0 - sc;
+        SF[L->i4 + k] = SLc[L->i4 + k] * (af / ac);
+      }
+    }
     int rc = m5_relaxvec(L, p, SF);
     if (rc) return rc;
   }
@@ -1120,7 +1143,7 @@ static void m5_apply_bc(int code, const fr5_t *F, const rec5_t *sL, rec5_t *sR) 
 /* m5_flux.c:3-95: flag 0/3 take the side's own velocity, fractions and -A;
  * 1/2 keep what m5_star_region set; 4 (symmetry) keeps the face velocity and
  * takes the side's fractions and +A */
-static void m5_flux(const lay5_t *L, const rec5_t *s, int flag_ifs, ws5_t *W) {
+static void m5_flux(const lay5_t *L, const rec5_t *s, int flag_ifs, int fix, ws5_t *W) {
   for (int v = 0; v < L->n0; v++) {
     int iv = v + L->i0;
     W->flux_adv[iv][0] = s->u * s->ra[v];
@@ -1128,17 +1151,19 @@ static void m5_flux(const lay5_t *L, const rec5_t *s, int flag_ifs, ws5_t *W) {
     W->flux_adv[iv][2] = s->w * s->ra[v];
   }
   int iv = L->i1;
-  W->flux_adv[iv][0] = s->u * s->u * s->r + s->st[0][0];
-  W->flux_adv[iv][1] = s->v * s->u * s->r + s->st[0][1];
-  W->flux_adv[iv][2] = s->w * s->u * s->r + s->st[0][2];
+  /* published: rho u u + sigma; consistent: rho u u - sigma */
+  const double sg = (fix & ORC_M5_FIX_SIGNS) ? -1.0 : 1.0;
+  W->flux_adv[iv][0] = s->u * s->u * s->r + sg * s->st[0][0];
+  W->flux_adv[iv][1] = s->v * s->u * s->r + sg * s->st[0][1];
+  W->flux_adv[iv][2] = s->w * s->u * s->r + sg * s->st[0][2];
   iv = L->i1 + 1;
-  W->flux_adv[iv][0] = s->u * s->v * s->r + s->st[1][0];
-  W->flux_adv[iv][1] = s->v * s->v * s->r + s->st[1][1];
-  W->flux_adv[iv][2] = s->w * s->v * s->r + s->st[1][2];
+  W->flux_adv[iv][0] = s->u * s->v * s->r + sg * s->st[1][0];
+  W->flux_adv[iv][1] = s->v * s->v * s->r + sg * s->st[1][1];
+  W->flux_adv[iv][2] = s->w * s->v * s->r + sg * s->st[1][2];
   iv = L->i1 + 2;
-  W->flux_adv[iv][0] = s->u * s->w * s->r + s->st[2][0];
-  W->flux_adv[iv][1] = s->v * s->w * s->r + s->st[2][1];
-  W->flux_adv[iv][2] = s->w * s->w * s->r + s->st[2][2];
+  W->flux_adv[iv][0] = s->u * s->w * s->r + sg * s->st[2][0];
+  W->flux_adv[iv][1] = s->v * s->w * s->r + sg * s->st[2][1];
+  W->flux_adv[iv][2] = s->w * s->w * s->r + sg * s->st[2][2];
   iv = L->i2;
   W->flux_adv[iv][0] = s->r * s->e * s->u - s->st[0][0] * s->u - s->st[0][1] * s->v - s->st[0][2] * s->w;
   W->flux_adv[iv][1] = s->r * s->e * s->v - s->st[1][0] * s->u - s->st[1][1] * s->v - s->st[1][2] * s->w;
@@ -1148,7 +1173,7 @@ static void m5_flux(const lay5_t *L, const rec5_t *s, int flag_ifs, ws5_t *W) {
     for (int v = 0; v < L->n3; ++v) W->avf_S[v] = s->avf[v];
     for (int v = 0; v < L->n4; ++v) W->are_S[v] = s->are[v];
     for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) W->Amat_S[i][j] = -s->Amat[i * 3 + j];
+      for (int j = 0; j < 3; ++j) W->Amat_S[i][j] = (fix & ORC_M5_FIX_SIGNS) ? s->Amat[i * 3 + j] : -s->Amat[i * 3 + j];
   } else if (flag_ifs == 4) {
     for (int v = 0; v < L->n3; ++v) W->avf_S[v] = s->avf[v];
     for (int v = 0; v < L->n4; ++v) W->are_S[v] = s->are[v];
@@ -1207,13 +1232,13 @@ static int m5_hllc(const lay5_t *L, const orc_m5_params *p, const fr5_t *F, int 
   if (bc == 4) u_S = 0.0;
   for (int iv = 0; iv < L->neq; ++iv) W->flux_hllc[iv] = 0.0;
   int end = 0;
-  if ((0.0 <= SL) && !end) { end = 1; m5_flux(L, sL, 0, W); }
+  if ((0.0 <= SL) && !end) { end = 1; m5_flux(L, sL, 0, p->fix, W); }
   if (((SL <= 0.0) && (0.0 <= u_S)) && !end) {
     end = 1;
     if (m5_star_region(L, p, F, sL, sR, sL, 1, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R,
                        s13R, W))
       return -1;
-    m5_flux(L, sL, 1, W);
+    m5_flux(L, sL, 1, p->fix, W);
     for (int iv = 0; iv < L->ncons; ++iv) W->flux_hllc[iv] = SL * (W->Us[iv] - sL->vec[iv]);
   }
   if (((u_S <= 0.0) && (0.0 <= SR)) && !end) {
@@ -1221,10 +1246,10 @@ static int m5_hllc(const lay5_t *L, const orc_m5_params *p, const fr5_t *F, int 
     if (m5_star_region(L, p, F, sL, sR, sR, 2, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R,
                        s13R, W))
       return -1;
-    m5_flux(L, sR, 2, W);
+    m5_flux(L, sR, 2, p->fix, W);
     for (int iv = 0; iv < L->ncons; ++iv) W->flux_hllc[iv] = SR * (W->Us[iv] - sR->vec[iv]);
   }
-  if ((SR <= 0.0) && !end) { end = 1; m5_flux(L, sR, 3, W); }
+  if ((SR <= 0.0) && !end) { end = 1; m5_flux(L, sR, 3, p->fix, W); }
   if (bc == 1) {   /* symmetry: mean of the physical fluxes of the cell record and its mirror */
     double sym[ORC_M5_MAXNEQ][3];
     for (int iv = 0; iv < L->neq; ++iv) {
@@ -1238,10 +1263,10 @@ static int m5_hllc(const lay5_t *L, const orc_m5_params *p, const fr5_t *F, int 
     W->u_nc = 0.5 * (cL.u + cR.u);
     W->v_nc = 0.5 * (cL.v + cR.v);
     W->w_nc = 0.5 * (cL.w + cR.w);
-    m5_flux(L, &cL, 4, W);
+    m5_flux(L, &cL, 4, p->fix, W);
     for (int iv = 0; iv < L->neq; iv++)
       for (int k = 0; k < 3; k++) sym[iv][k] += 0.5 * W->flux_adv[iv][k];
-    m5_flux(L, &cR, 4, W);
+    m5_flux(L, &cR, 4, p->fix, W);
     for (int iv = 0; iv < L->neq; iv++)
       for (int k = 0; k < 3; k++) sym[iv][k] += 0.5 * W->flux_adv[iv][k];
     for (int iv = 0; iv < L->neq; iv++)
@@ -1323,7 +1348,10 @@ static int m5_residual_leaf(const lay5_t *L, const orc_m5_params *p, const orc_m
           if (R->flux_flag[6 * i + f] == 1) {
             const double *uo = &R->u_face[((size_t)j * 6 + fo) * 3];
             const double *fo_ = &R->flux_face[((size_t)j * 6 + fo) * neq];
-            uf[0] = -uo[0]; uf[1] = -uo[1]; uf[2] = -uo[2];
+            /* the star velocity is a global vector: the published code negates it on
+             * the mirrored side (m5_residual.c:62-103), the consistent mode reuses it */
+            const double us = (p->fix & ORC_M5_FIX_SIGNS) ? 1.0 : -1.0;
+            uf[0] = us * uo[0]; uf[1] = us * uo[1]; uf[2] = us * uo[2];
             for (int iv = 0; iv < neq; ++iv) ff[iv] = -fo_[iv];
             compute = 0;
           } else {
@@ -1371,6 +1399,27 @@ static int m5_residual_leaf(const lay5_t *L, const orc_m5_params *p, const orc_m
     const double v_nc = uf[1], ny = cl->ny[f];
     const double w_nc = uf[2], nz = cl->nz[f];
     int v = L->i3;
+    if (p->fix & ORC_M5_FIX_SIGNS) {
+      /* the consistent sources: -alpha (u.n), +n.(alpha_k sigma_k).u, and for A_ij
+       * n_i (A_kj u_k) - A_ij (u.n), all with the cell's own values */
+      for (int iv = 0; iv < L->n3; ++iv)
+        rhs[v + iv] -= Area_f * (source_avf[iv] * u_nc * nx + source_avf[iv] * v_nc * ny + source_avf[iv] * w_nc * nz);
+      v = L->i4;
+      for (int iv = 0; iv < L->n4; ++iv) {
+        double (*s)[3] = sc.ast[iv];
+        rhs[v + iv] -= Area_f * (s[0][0] * u_nc * nx + s[0][1] * v_nc * nx + s[0][2] * w_nc * nx +
+                                 s[1][0] * u_nc * ny + s[1][1] * v_nc * ny + s[1][2] * w_nc * ny +
+                                 s[2][0] * u_nc * nz + s[2][1] * v_nc * nz + s[2][2] * w_nc * nz);
+      }
+      v = L->i5;
+      const double nn[3] = {nx, ny, nz}, uu[3] = {u_nc, v_nc, w_nc};
+      const double un = u_nc * nx + v_nc * ny + w_nc * nz;
+      for (int j = 0; j < 3; ++j) {
+        const double cu = sc.Amat[0 * 3 + j] * uu[0] + sc.Amat[1 * 3 + j] * uu[1] + sc.Amat[2 * 3 + j] * uu[2];
+        for (int ii = 0; ii < 3; ++ii) rhs[v + 3 * ii + j] += Area_f * (nn[ii] * cu - sc.Amat[ii * 3 + j] * un);
+      }
+      continue;
+    }
     for (int iv = 0; iv < L->n3; ++iv)
       rhs[v + iv] += Area_f * (source_avf[iv] * u_nc * nx + source_avf[iv] * v_nc * ny + source_avf[iv] * w_nc * nz);
     v = L->i4;

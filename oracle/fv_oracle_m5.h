@@ -38,7 +38,39 @@ typedef struct {
   int    order;                /* ORDER: 1 cell states, 2 MUSCL face states */
   int    rk_steps;             /* RK_STEPS: stages irk = 1..rk_steps of m5_RK */
   double yield;                /* von Mises yield stress of the solid, 0 = elastic (new) */
+  int    fix;                  /* ORC_M5_FIX_SIGNS: the consistent model-5 update below; 0 = literal */
 } orc_m5_params;
+
+/* The published model-5 update is inconsistent with itself in several places;
+ * ORC_M5_FIX_SIGNS takes the consistent form of each, all as one mode (0
+ * reproduces the published code):
+ *  - m5_flux.c:21-33 forms the momentum flux rho u u + sigma: rho u u - sigma,
+ *    as m4_flux.c:21, m5_flux_gfm.c:22-34 and the contact speed use;
+ *  - m5_RK.c:49,58-79 update S = SN + dt RHS / V while the residual sums the
+ *    outward face fluxes (m5_residual.c:156-161): S = SN - dt RHS / V, also in
+ *    the stage-1 re-projection of m5_gfm.c;
+ *  - the mirrored side of a face reuses the star velocity vector as it is
+ *    (m5_residual.c:62-103 negates the global-frame vector with the flux);
+ *  - the non-conservative sources follow that sign: -alpha div u for the
+ *    fractions and +alpha_k sigma_k : grad u in the phasic energies (the
+ *    published terms negated), and for the deformation gradient
+ *    A_ij: n_i (A_kj u_k) - A_ij (u.n) per face from the cell's A, the
+ *    remainder of the flux A_kj u_k n_i of dA_ij/dt + u.grad A_ij +
+ *    A_kj d_i u_k = 0 after the advective part u* A_S (m5_residual.c:197-209
+ *    has a sign that differs from this in one term of each row);
+ *  - the MUSCL face states scale each phase's alpha-rho-e with its
+ *    reconstructed fraction, which keeps the phase's pressure at the face
+ *    (m5_input.c:196-204 leave the block at the cell value while the
+ *    fractions are reconstructed);
+ *  - the advective A flux takes +A on the supersonic sides, as the star
+ *    states do (m5_flux.c:54-56 negates it there only);
+ *  - m5_star_region.c: the star energy with -s11K uK over r* (u* - S_K)
+ *    (:73 has + s11K uK over r* (u* + S_K)), the first row of the star A with
+ *    + A_1j (vK - v*) (:84 has -), and the phasic star energy from +p_k with
+ *    (gamma - 1) (:118-125 take -p_hydro and gamma + 1);
+ *  - the relaxation scales A by cbrt(alpha0 / alpha) of the solid, so that
+ *    det A follows its density (m5_relaxationvec.c scales A by alpha / alpha0). */
+#define ORC_M5_FIX_SIGNS 1
 
 int orc_m5_neq(int nmat);
 

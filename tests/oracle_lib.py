@@ -89,6 +89,7 @@ class orc_m5_params(C.Structure):
         ("order", C.c_int),
         ("rk_steps", C.c_int),
         ("yield_", C.c_double),
+        ("fix", C.c_int),
     ]
 
 

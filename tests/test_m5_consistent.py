@@ -1,0 +1,240 @@
+"""The consistent model-5 update (fv_m5_params.fix = FV_M5_FIX_SIGNS) in the oracle,
+pinned by physics rather than by the published code, which it departs from:
+
+  - face consistency: the HLLC flux between two equal states is the physical flux
+    (rho u.n, rho u (u.n) - sigma.n, (rho E u - sigma u).n, and u.n times the
+    fractions, phasic energies and A), computed here from the face record;
+  - the longitudinal wave speed of an elastic solid: a pressure pulse in config 2's
+    slab material travels at c_L^2 = (gamma (p + p_inf) + 4 mu / 3) / rho, and at the
+    hydrodynamic speed without shear;
+  - mirror symmetry of a symmetric pulse, conservation of every partial mass and of
+    the total energy in a closed box, and a state at rest that stays at rest;
+  - the published mode (fix = 0) is left as it was: it does not propagate the pulse
+    (it grows where it starts and ends in NaN).
+
+GPU tests run the same mode through fv_m5_dev_* and compare it with the oracle: face
+states and residuals on the mixed solid / water / air sphere of test_m5_stage, ten steps
+of it, and thirty steps of the solid pulse at CFL 0.3."""
+import numpy as np
+import pytest
+
+import forestfv_public_amd as F
+import oracle_lib as O
+from test_m5 import deviatoric_stress, elastic_energy, m5_state, near_identity, random_rotation
+
+C2 = dict(gamma=(3.0, 6.12, 1.4), pinf=(2.0e10, 3.43e8, 0.0), mush=(26e9, 0.0, 0.0))
+RHO = np.array([2700.0, 1000.0, 1.2])
+
+
+def params(mu=26e9, prelax=0, fix=F.M5_FIX_SIGNS, order=2, dt=1e-9):
+    return F.make_m5_params(nmat=3, gamma=C2["gamma"], pinf=C2["pinf"], mush=(mu, 0.0, 0.0), amin=1e-6, avf_lim=0.5,
+                            prelax=prelax, order=order, rk_steps=3, fix=fix, dt=dt)
+
+
+def line(N=128, L=1e-2):
+    return F.Mesh.uniform((N, 1, 1), (0, 0, 0), L / N, 1, [4] * 6)
+
+
+def pulse(p, m, L=1e-2, width=0.03):
+    x = m.centres()[:, 0]
+    return np.array([m5_state(p, [1 - 2e-6, 1e-6, 1e-6], RHO, 1e7 + 1e6 * np.exp(-((xi - 0.5 * L) / (width * L)) ** 2),
+                              [0, 0, 0], np.eye(3)) for xi in x])
+
+
+@pytest.mark.parametrize("mu", [26e9, 0.0])
+def test_solid_pulse_travels_at_the_longitudinal_speed(mu):
+    p = params(mu=mu)
+    m = line()
+    L, N = 1e-2, m.n
+    h = L / N
+    cL = np.sqrt((3.0 * (1e7 + 2e10) + 4.0 / 3.0 * mu) / 2700.0)
+    ch = np.sqrt(3.0 * (1e7 + 2e10) / 2700.0)
+    p.dt = 0.3 * h / cL
+    S = O.OracleMesh(m).m5_advance(p, pulse(p, m), 100)
+    x = m.centres()[:, 0]
+    sxx = np.array([O.m5_record(p, s)["st"][0, 0] for s in S])
+    w = np.clip(-(sxx[N // 2:] + 1e7), 0, None)     # compressive stress above the ambient, right half
+    c = ((x[N // 2:] * w).sum() / w.sum() - 0.5 * L) / (100 * p.dt)
+    assert abs(c / cL - 1.0) < 0.03, (c, cL)
+    if mu > 0:
+        assert c / ch > 1.15            # the shear modulus is in the speed, not just the fluid part
+
+
+def test_symmetric_pulse_stays_symmetric_and_published_mode_does_not_propagate():
+    p = params()
+    m = line(N=64)
+    h = 1e-2 / 64
+    p.dt = 0.3 * h / 5922.0
+    S0 = pulse(p, m, width=0.05)
+    S = O.OracleMesh(m).m5_advance(p, S0, 40)
+    u = S[:, 3] / S[:, 0:3].sum(1)
+    assert np.abs(u).max() < 0.1                    # a 1e6 Pa pulse in a 2e10 Pa material: u ~ dp / (rho c)
+    assert np.abs(u + u[::-1]).max() <= 1e-6 * np.abs(u).max()
+    q = params(fix=0)
+    q.dt = p.dt
+    with pytest.raises(RuntimeError):
+        O.OracleMesh(m).m5_advance(q, S0, 40)
+
+
+def physical_flux(p, x, n):
+    r = O.m5_record(p, x)
+    u = np.array([r["u"], r["v"], r["w"]])
+    un = u @ n
+    st = r["st"]
+    nm = 3
+    f = np.zeros(F.m5_neq(nm))
+    f[0:nm] = r["ra"] * un
+    f[nm:nm + 3] = r["r"] * u * un - st @ n
+    f[nm + 3] = r["r"] * r["e"] * un - (st @ u) @ n
+    f[nm + 4:nm + 6] = r["avf"][:2] * un
+    f[nm + 6:nm + 9] = r["are"] * un
+    f[nm + 9:] = r["Amat"].ravel() * un
+    return f
+
+
+def test_face_flux_between_equal_states_is_the_physical_flux():
+    p = params()
+    rng = np.random.default_rng(31)
+    for _ in range(40):
+        R = random_rotation(rng)
+        n, t1, t2 = R[:, 0].copy(), R[:, 1].copy(), R[:, 2].copy()
+        a_s = rng.uniform(0.6, 0.99)
+        alpha = np.array([a_s, 0.7 * (1 - a_s), 0.3 * (1 - a_s)])
+        A = near_identity(rng, 0.01)
+        x = m5_state(p, alpha, RHO, rng.uniform(1e6, 1e8), rng.normal(size=3) * 20.0, A)
+        fr = np.concatenate([n, t1, t2])
+        flux, unc, g = O.m5_face(p, x, x, x, x, fr, 0, 1.0)
+        ref = physical_flux(p, x, n)
+        scale = np.abs(ref).max()
+        assert g == 0
+        assert np.allclose(flux, ref, rtol=1e-9, atol=1e-9 * scale), np.abs(flux - ref).max() / scale
+
+
+def test_closed_box_conserves_partial_masses_and_energy():
+    """Reflective walls all round a line of cells, a pulse and a flow along it: the transverse
+    velocity stays zero, the end walls hold u* = 0, so every partial mass and the total energy
+    stay what they were to rounding.  (A wall's ghost keeps the shear stress of the solid, so a
+    3-D shear flow would do work on the walls.)"""
+    p = params(prelax=0)
+    m = line(N=64)
+    p.dt = 0.3 * (1e-2 / 64) / 5922.0
+    S = pulse(p, m, width=0.05)
+    S[:, 3] = S[:, 0:3].sum(1) * 2.0 * np.sin(np.pi * m.centres()[:, 0] / 1e-2)   # and a flow towards the +x wall
+    S[:, 6] += 0.5 * S[:, 3] ** 2 / S[:, 0:3].sum(1)
+    om = O.OracleMesh(m)
+    V = om.geometry()["Vol"]
+    S2 = om.m5_advance(p, S, 30)
+    tot0 = (S * V[:, None]).sum(0)
+    tot1 = (S2 * V[:, None]).sum(0)
+    for k in (0, 1, 2, 6):               # partial masses and total energy
+        assert abs(tot1[k] - tot0[k]) <= 1e-13 * abs(tot0[k]), (k, tot1[k] - tot0[k])
+    assert np.abs(S2[:, 4:6]).max() == 0.0
+
+
+def test_state_at_rest_stays_at_rest():
+    p = params(prelax=0)
+    m = line(N=32)
+    S = np.tile(m5_state(p, [0.3, 0.5, 0.2], RHO, 1e7, [0, 0, 0], np.eye(3)), (m.n, 1))
+    p.dt = 0.3 * (1e-2 / 32) / 6000.0
+    S2 = O.OracleMesh(m).m5_advance(p, S, 20)
+    assert np.abs(S2[:, 3:6]).max() <= 1e-9
+    assert np.allclose(S2, S, rtol=1e-12, atol=0)
+
+
+# ---------------------------------------------------------------- device parity
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel_l1(got, want):
+    num = np.abs(got - want).sum(axis=0)
+    den = np.abs(want).sum(axis=0) + 1e-300
+    return (num / den).max()
+
+
+def rel_max(got, want):
+    g = got.reshape(got.shape[0], -1)
+    w = want.reshape(want.shape[0], -1)
+    return (np.abs(g - w).max(axis=0) / (np.abs(w).max(axis=0) + 1e-300)).max()
+
+
+def fixed_case(seed, amr, prelax, table_name):
+    from test_m5 import FLUIDS, SOLID_WATER_AIR
+    from test_m5_stage import m5_case
+    table = SOLID_WATER_AIR if table_name == "solid" else FLUIDS
+    p, m, om, S = m5_case(seed, amr=amr, nmat=3, table=table, prelax=prelax)
+    p.fix = F.M5_FIX_SIGNS
+    return p, m, om, S
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("amr", [False, True])
+@pytest.mark.parametrize("table,prelax", [("fluids", 0), ("solid", 1), ("solid", 3)])
+def test_m5_device_consistent_face_states_and_residual_match_oracle(table, prelax, amr):
+    p, m, om, S = fixed_case(140 + prelax, amr, prelax, table)
+    say(f"[m5 fix dev] {table} prelax={prelax} amr={amr} n={m.n}")
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    dev.faceinterp()
+    dev.residual()
+    SF, RHS, gf = dev.stage_arrays()
+    SFo = om.m5_faceinterp(p, S)
+    RHSo, gfo = om.m5_residual(p, S, SFo)
+    dev.close()
+    assert np.array_equal(gf, gfo)
+    assert gfo.any()
+    if table == "fluids":                 # no relaxation, no cube roots: the same operations
+        assert np.array_equal(SF, SFo)
+        assert np.array_equal(RHS, RHSo)
+    else:
+        assert rel_max(SF, SFo) <= 1e-11
+        assert rel_max(RHS, RHSo) <= 1e-10
+    # and the consistent residual is not the published one
+    q = F.make_m5_params(nmat=3, gamma=tuple(p.gamma[:3]), pinf=tuple(p.pinf[:3]), mush=tuple(p.mush[:3]),
+                         amin=p.amin, avf_lim=p.avf_lim, dt=p.dt, prelax=p.prelax, order=p.order, fix=0)
+    assert rel_max(om.m5_residual(q, S, om.m5_faceinterp(q, S))[0], RHSo) > 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prelax", [0, 3])
+def test_m5_device_consistent_steps_match_oracle(prelax):
+    p, m, om, S = fixed_case(150 + prelax, True, prelax, "solid")
+    p.dt = 2e-9
+    say(f"[m5 fix dev steps] prelax={prelax} n={m.n}")
+    dev = F.M5Device(p, m)
+    dev.set_state(S)
+    dev.advance(10)
+    Sd = dev.get_state()
+    dev.close()
+    So = om.m5_advance(p, S, 10)
+    err = rel_l1(Sd, So)
+    say(f"[m5 fix dev steps] prelax={prelax} rel L1 {err:.2e}")
+    assert err <= 1e-10
+
+
+@pytest.mark.gpu
+def test_m5_device_consistent_solid_pulse_matches_oracle():
+    """Thirty steps at CFL 0.3 of the 1-D solid pulse: the device and the oracle agree, and the
+    device's pulse has moved at the longitudinal speed of the solid."""
+    p = params(prelax=0)
+    m = line()
+    L, N = 1e-2, m.n
+    cL = np.sqrt((3.0 * (1e7 + 2e10) + 4.0 / 3.0 * 26e9) / 2700.0)
+    p.dt = 0.3 * (L / N) / cL
+    S0 = pulse(p, m)
+    say(f"[m5 fix dev pulse] n={m.n} dt={p.dt:.3e}")
+    dev = F.M5Device(p, m)
+    dev.set_state(S0)
+    dev.advance(30)
+    Sd = dev.get_state()
+    dev.close()
+    So = O.OracleMesh(m).m5_advance(p, S0, 30)
+    err = rel_l1(Sd, So)
+    say(f"[m5 fix dev pulse] rel L1 {err:.2e}")
+    assert err <= 1e-11
+    x = m.centres()[:, 0]
+    sxx = np.array([O.m5_record(p, s)["st"][0, 0] for s in Sd])
+    w = np.clip(-(sxx[N // 2:] + 1e7), 0, None)
+    c = ((x[N // 2:] * w).sum() / w.sum() - 0.5 * L) / (30 * p.dt)
+    assert abs(c / cL - 1.0) < 0.1, (c, cL)
