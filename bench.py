@@ -41,16 +41,25 @@ def algorithmic_bytes(neq):
     return 24 * neq + 32
 
 
-def traffic_bytes(workload, prec, fused):
-    """HBM bytes per stage measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    (calibrated against a copy kernel), from the summaries committed under
-    profiles/; None where no such measurement exists."""
+def traffic_bytes(workload, prec, kernel):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (calibrated against a copy kernel of known size), from the
+    summaries committed under profiles/; None where no such measurement exists."""
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as fh:
             t = json.load(fh)
-        return None if fused else t[workload][prec]["bytes_per_stage"]
+        return t[workload][prec][kernel]["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def dominant_kernel(prec, fused, nbricks, n):
+    """(kernel name, leaves one launch updates) of the kernel that dominates a stage."""
+    if fused and prec != "exact":
+        return "k_stage_fused", n
+    if prec != "exact" and nbricks > 0:
+        return "k_stage_brick", 64 * nbricks
+    return {"exact": "k_flux_rk", "fp64": "k_flux_fast", "mixed": "k_flux_pipe"}[prec], n
 
 
 def log(*a):
@@ -526,6 +535,35 @@ def m5_leg(level=8, steps=3, check=True, log_fn=log):
     return out
 
 
+def uniform_leg(level=9, warmup=3, steps=3, log_fn=log):
+    """The FP64 stage on the uniform 2^(level-1) cube of the same bubble (256^3 =
+    16.8 M leaves at level 9, every leaf in a 4x4x4 block): the block kernel's
+    roofline at a size where its launch fills the chip many times over."""
+    t = time.perf_counter()
+    p, m, S = build_uniform(level)
+    dev = F.Device(p, m, prec=F.PREC_FP64)
+    dev.set_state(S)
+    dev.time_stages(3 * warmup)
+    ms_stage, ms_kernel = dev.time_stages(3 * steps)
+    nbricks, nrest = dev.brick_count()
+    ok = bool(np.all(np.isfinite(dev.get_state())))
+    dev.close()
+    if not ok:
+        raise SystemExit("non-finite state after the uniform-cube stages")
+    B = algorithmic_bytes(F.neq(p.nmat))
+    achieved = B * 64 * nbricks / (ms_kernel * 1e-3) / 1e9
+    side = 1 << (level - 1)
+    wl = f"uniform_{side}^3_water_air_bubble"
+    log_fn(f"uniform {side}^3: {m.n} leaves, {ms_stage:.3f} ms/stage, block kernel {ms_kernel:.3f} ms "
+           f"({time.perf_counter() - t:.1f} s)")
+    return {"workload": wl, "leaves": m.n, "ms_per_stage": round(ms_stage, 4),
+            "value": round(m.n / (ms_stage * 1e-3) / 1e6, 2), "unit": "Mcell-updates/s per RK stage",
+            "roofline": {"bound": "hbm", "kernel": "k_stage_brick", "leaves_per_launch": 64 * nbricks,
+                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBPS, 4), "ms_kernel": round(ms_kernel, 4),
+                         "traffic": traffic_bytes(wl, "fp64", "k_stage_brick"), "algorithmic_bytes_per_leaf": B}}
+
+
 def c3_run(steps, warmup, layers, nsteps_cmp=300, log_fn=log):
     """Config 3 at its size: the C1 bubble pre-adapted to reference levels 5-10 (config
     levels 4-9); FP64 and mixed-precision stage rates, and the mixed state against the FP64
@@ -553,7 +591,7 @@ def c3_run(steps, warmup, layers, nsteps_cmp=300, log_fn=log):
         states[name] = dev.get_state()
         dev.close()
         out[name] = {"value": round(m.n / (ms * 1e-3) / 1e6, 2), "ms_per_stage": round(ms, 4),
-                     "ms_flux_kernel": round(msf, 4),
+                     "ms_dominant_kernel": round(msf, 4),
                      "achieved_GBps": round(B * m.n / (ms * 1e-3) / 1e9, 1),
                      "frac_of_8TBps": round(B * m.n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
                      "run_s": round(wall, 2)}
@@ -596,6 +634,7 @@ def main():
     ap.add_argument("--no-regrid", action="store_true", help="skip the device / host regrid leg")
     ap.add_argument("--no-collapse", action="store_true", help="skip the run to the first collapse")
     ap.add_argument("--no-m5", action="store_true", help="skip the model-5 (solid, GFM) stage leg")
+    ap.add_argument("--no-u256", action="store_true", help="skip the uniform 256^3 block-kernel roofline leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo exchange is "
@@ -646,6 +685,7 @@ def main():
     ms_stage, ms_flux = dev.time_stages(ns * a.steps)
     wall = time.perf_counter() - t0
     Sg = dev.get_state()
+    nbricks, nrest = dev.brick_count() if a.prec != "exact" and not a.fused else (0, m.n)
     dev.close()
     if not np.all(np.isfinite(Sg)):
         raise SystemExit("non-finite state after the timed stages")
@@ -655,12 +695,18 @@ def main():
 
     n = m.n
     fused = a.fused and a.prec != "exact"
-    kernel = "k_stage_fused" if fused else {"exact": "k_recon + k_flux_rk", "fp64": "k_recon_fast + k_flux_fast",
-                                             "mixed": "k_recon_fast + k_flux_pipe"}[a.prec]
-    cfg["stage_kernels"] = "fused" if fused else "recon+flux"
+    kernel, units = dominant_kernel(a.prec, fused, nbricks, n)
+    cfg["stage_kernels"] = ("fused" if fused else
+                            f"4x4x4 blocks ({nbricks} blocks, {64 * nbricks} leaves) + recon+flux ({nrest} leaves)"
+                            if nbricks > 0 else "recon+flux")
     value = n / (ms_stage * 1e-3) / 1e6
     B = algorithmic_bytes(nq)
-    achieved = B * n / (ms_stage * 1e-3) / 1e9
+    if kernel in ("k_stage_brick", "k_stage_fused"):
+        # the kernel does a whole stage for its leaves: B per leaf over its own launch time
+        achieved = B * units / (ms_flux * 1e-3) / 1e9
+    else:
+        # the per-leaf path spreads a stage over two kernels: B per leaf over the stage
+        achieved = B * n / (ms_stage * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s per RK stage",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ns * ms_stage, 4),
@@ -670,9 +716,10 @@ def main():
         "data": "synthetic", "config": cfg,
         "wall_s_timed": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_bytes(cfg["workload"], a.prec, fused),
-                     "kernel": kernel, "algorithmic_bytes_per_leaf": B,
-                     "ms_flux_kernel": round(ms_flux, 4), "stream_copy_GBps": round(copy_gbps, 1)},
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_bytes(cfg["workload"], a.prec, kernel),
+                     "kernel": kernel, "leaves_per_launch": units, "algorithmic_bytes_per_leaf": B,
+                     "ms_kernel": round(ms_flux, 4), "stage_achieved_GBps": round(B * n / (ms_stage * 1e-3) / 1e9, 1),
+                     "stream_copy_GBps": round(copy_gbps, 1)},
     }
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_mlp:
         leg = fp32_mlp_leg(p, m, S, c1_crit(9, a.layers))
@@ -684,6 +731,8 @@ def main():
         out["collapse"] = collapse_leg(p, a.layers)
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_m5:
         out["m5_stage"] = m5_leg()
+    if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_u256:
+        out["u256"] = uniform_leg(9)
     if not a.no_cpu_baseline:
         cb, l1, stages = cpu_baseline(p, m, S, cfg["workload"])
         out["cpu_baseline"] = cb

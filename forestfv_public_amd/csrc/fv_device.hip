@@ -1152,56 +1152,67 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
   const double dcf = G.dcf[0][lvl], idcc = fvf::rcp(G.dcc[0][lvl]), A = G.area[lvl];
 
   // ---- faces: one lane per face, 96 across the surface slots and 144 inside
-  // the brick; a lane reconstructs both face states of its face and runs HLLC
+  // the brick; a lane reconstructs both face states of its face and runs HLLC.
+  // Surface and inside lanes differ only in the LDS columns and constants they
+  // pass, so that a wave holding both kinds runs one instruction stream.
   if (tid < BR_TASKS) {
     const fvf::Mat<NM, R> M = fvf::make_mat<NM, R>(P);
-    int fL, bcode = 0;
-    fvf::SolF<NM, R> sL, sR;
-    if (tid < BR_SLOTS) {
+    const bool surf = tid < BR_SLOTS;
+    int fL, bcode = 0, lown;
+    // left side (outward normal frame[fL]) and right side: the cell column, its
+    // stencil value columns across the face and across the opposite face, its
+    // limiter, zero-slope flag and 1/dcc beyond it
+    int cA, gA, oA, cB, gB, oB, limA, limB;
+    bool bndA, bndB, need = false;
+    double idmB = idcc;
+    if (surf) {
       const int s = tid, f = s >> 4, fo = FCOPP_D[f], ax = br_axis(f);
-      const int l = br_slot_cell(s), i = b0 + l;
+      const int l = br_slot_cell(s);
       const uint32_t inf = B.info[l];
       fL = f;
+      lown = l;
       BrPos pos = br_pos(l);
       br_add(pos, ax, br_plus(f) ? -1 : 1);   // the brick cell's neighbour on the other side, inside the brick
       const bool bndf = face_kind(inf, f) == FV_FK_BOUNDARY;
-      const bool bnd = bndf || face_kind(inf, fo) == FV_FK_BOUNDARY;
-      double F[L::NEQ];
-      float rec[MIXED ? fvf::Rec<NM>::N : 1];
-      br_face_state<NM, MIXED, XR>(P, Md, a, B, l, 64 + s, br_index(pos), B.lim[l], bnd, dcf, idcc, idcc, F, rec);
-      if (B.need[s]) br_write_sf<NM, MIXED>(n, i, f, F, rec, SF, SFm);
-      br_sol<NM, MIXED, R>(Md, M, a, F, rec, sL);
+      cA = l; gA = 64 + s; oA = br_index(pos); limA = B.lim[l];
+      bndA = bndf || face_kind(inf, fo) == FV_FK_BOUNDARY;
+      need = B.need[s] != 0;
       if (bndf) {
+        // no cell across a domain boundary: the right state is the ghost of the left
+        // one, so the right reconstruction repeats the left one on valid data
         bcode = -B.nb[s];
-        sR = sL;
+        cB = cA; gB = gA; oB = oA; limB = limA; bndB = bndA;
       } else {
         // the neighbour's face state towards the brick: its stencil value across
         // that face is the brick cell, beyond it the staged second layer
         const uint32_t infj = B.infoh[s];
-        const bool bj = face_kind(infj, fo) == FV_FK_BOUNDARY || face_kind(infj, f) == FV_FK_BOUNDARY;
-        br_face_state<NM, MIXED, XR>(P, Md, a, B, 64 + s, l, 160 + s, B.lim[64 + s], bj, dcf, idcc, B.idmh[s], F,
-                                     rec);
-        br_sol<NM, MIXED, R>(Md, M, a, F, rec, sR);
+        cB = 64 + s; gB = l; oB = 160 + s; limB = B.lim[64 + s];
+        bndB = face_kind(infj, fo) == FV_FK_BOUNDARY || face_kind(infj, f) == FV_FK_BOUNDARY;
+        idmB = B.idmh[s];
       }
     } else {
       const int q = tid - BR_SLOTS, ax = q / 48, p = (q % 48) >> 4, k = q & 15;
       const int f = br_plus_face(ax), fo = FCOPP_D[f];
       const int lL = br_index(br_sheet_pos(ax, k, p)), lR = br_index(br_sheet_pos(ax, k, p + 1));
+      fL = f;
+      lown = lL;
       // the cells before the left one and after the right one along the axis:
       // inside the brick, or the staged neighbour of the - / + surface slot
-      const int cm = (p > 0) ? br_index(br_sheet_pos(ax, k, p - 1)) : 64 + 16 * fo + k;
-      const int cp = (p < 2) ? br_index(br_sheet_pos(ax, k, p + 2)) : 64 + 16 * f + k;
+      cA = lL; gA = lR; oA = (p > 0) ? br_index(br_sheet_pos(ax, k, p - 1)) : 64 + 16 * fo + k; limA = B.lim[lL];
+      cB = lR; gB = lL; oB = (p < 2) ? br_index(br_sheet_pos(ax, k, p + 2)) : 64 + 16 * f + k; limB = B.lim[lR];
       // inside the brick only the far faces of the two cells can be domain boundaries
-      const bool bl = face_kind(B.info[lL], fo) == FV_FK_BOUNDARY;
-      const bool brr = face_kind(B.info[lR], f) == FV_FK_BOUNDARY;
-      fL = f;
-      double F[L::NEQ];
-      float rec[MIXED ? fvf::Rec<NM>::N : 1];
-      br_face_state<NM, MIXED, XR>(P, Md, a, B, lL, lR, cm, B.lim[lL], bl, dcf, idcc, idcc, F, rec);
-      br_sol<NM, MIXED, R>(Md, M, a, F, rec, sL);
-      br_face_state<NM, MIXED, XR>(P, Md, a, B, lR, lL, cp, B.lim[lR], brr, dcf, idcc, idcc, F, rec);
-      br_sol<NM, MIXED, R>(Md, M, a, F, rec, sR);
+      bndA = face_kind(B.info[lL], fo) == FV_FK_BOUNDARY;
+      bndB = face_kind(B.info[lR], f) == FV_FK_BOUNDARY;
     }
+    fvf::SolF<NM, R> sL, sR;
+    double F[L::NEQ];
+    float rec[MIXED ? fvf::Rec<NM>::N : 1];
+    br_face_state<NM, MIXED, XR>(P, Md, a, B, cA, gA, oA, limA, bndA, dcf, idcc, idcc, F, rec);
+    if (need) br_write_sf<NM, MIXED>(n, b0 + lown, fL, F, rec, SF, SFm);
+    br_sol<NM, MIXED, R>(Md, M, a, F, rec, sL);
+    br_face_state<NM, MIXED, XR>(P, Md, a, B, cB, gB, oB, limB, bndB, dcf, idcc, idmB, F, rec);
+    br_sol<NM, MIXED, R>(Md, M, a, F, rec, sR);
+    if (bcode != 0) sR = sL;
     R fr[9];
 #pragma unroll
     for (int qq = 0; qq < 9; qq++) fr[qq] = (R)G.frame[fL][qq];
@@ -1873,8 +1884,10 @@ static int classify_bricks(fv_dev *c) {
   return FV_OK;
 }
 
+// ev_flux0 / ev_flux1 bracket the dominant kernel: the block kernel where the mesh
+// has bricks, else the flux (or fused stage) kernel
 template <int NM, bool XR>
-static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
+static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0, hipEvent_t *ev_flux1) {
   const int n = c->n;
   const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
   if (c->prec == FV_PREC_EXACT) {
@@ -1883,6 +1896,7 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
     hipLaunchKernelGGL(k_flux_rk<NM>, grid, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->SN, c->SF, c->nbr,
                        c->info, c->fine4, c->nanflag);
+    if (ev_flux1) HIPCHK(hipEventRecord(*ev_flux1, c->stream));
   } else if (c->fused) {
     const unsigned nb = (unsigned)((n + FV_BLOCK - 1) / FV_BLOCK);
     const dim3 gx((nb + 7u) & ~7u);   // a multiple of 8 for the XCD-contiguous leaf mapping
@@ -1893,6 +1907,7 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
     else
       hipLaunchKernelGGL((k_stage_fused<NM, true, XR>), gx, block, 0, c->stream, n, c->P, c->Gdev, cs, c->S, c->S2, c->SN,
                          c->nbr, c->info, c->fine4, c->lim, c->nanflag);
+    if (ev_flux1) HIPCHK(hipEventRecord(*ev_flux1, c->stream));
     double *t = c->S;
     c->S = c->S2;
     c->S2 = t;
@@ -1933,6 +1948,7 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
         hipLaunchKernelGGL((k_stage_brick<NM, false, XR>), gb, dim3(BR_THREADS), 0, c->stream, n, c->P, c->Gdev, cs,
                            c->bricks, c->nbricks, c->S, c->S2, c->SN, c->nbr, c->info, c->fine4, c->lim, c->inbrick,
                            c->SF, c->SFm, c->nanflag);
+      if (ev_flux1) HIPCHK(hipEventRecord(*ev_flux1, c->stream));
     }
     if (cnt > 0) {
       // the pipelined flux needs more registers: 2 waves per SIMD with FP32 face
@@ -1944,6 +1960,7 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0) {
         hipLaunchKernelGGL((k_flux_fast<NM, false, XR>), gl, block, 0, c->stream, n, c->P, c->G, cs, c->S, c->S2, c->SN,
                            c->SF, c->SFm, c->nbr, c->info, c->fine4, c->nanflag, list, cnt);
     }
+    if (ev_flux1 && !(bricks && c->nbricks > 0)) HIPCHK(hipEventRecord(*ev_flux1, c->stream));
     // the new state is in S2; the old one stays intact until every kernel of the
     // stage has read its neighbours from it
     double *t = c->S;
@@ -1959,10 +1976,10 @@ static int launch_stage(fv_dev *c, int irk, hipEvent_t *ev_flux0, hipEvent_t *ev
   // closed-form relaxations do not carry the iterations' code and registers.
   // The fixed-pressure inlet and outlet ghosts live in those instances too.
   const bool xr = c->P.prelax >= 2 || c->P.bc_fixed_p;
-  int rc = (c->nmat == 2) ? (xr ? launch_stage_nm<2, true>(c, cs, ev_flux0) : launch_stage_nm<2, false>(c, cs, ev_flux0))
-                          : (xr ? launch_stage_nm<3, true>(c, cs, ev_flux0) : launch_stage_nm<3, false>(c, cs, ev_flux0));
+  int rc = (c->nmat == 2)
+               ? (xr ? launch_stage_nm<2, true>(c, cs, ev_flux0, ev_flux1) : launch_stage_nm<2, false>(c, cs, ev_flux0, ev_flux1))
+               : (xr ? launch_stage_nm<3, true>(c, cs, ev_flux0, ev_flux1) : launch_stage_nm<3, false>(c, cs, ev_flux0, ev_flux1));
   if (rc != FV_OK) return rc;
-  if (ev_flux1) HIPCHK(hipEventRecord(*ev_flux1, c->stream));
   HIPCHK(hipGetLastError());
   return FV_OK;
 }

@@ -131,7 +131,9 @@ int  fv_dev_get_limiter_flags(fv_dev *ctx, uint8_t *lim);
 
 /* Timing of the stage kernels with hipEvents on the context's stream:
  * runs nrep full stages (cycling irk) and returns the mean milliseconds per
- * stage and per launch of the dominant (flux) kernel. */
+ * stage and per launch of the dominant kernel: the 4x4x4 block kernel where
+ * the FP64 or mixed stage has blocks (fv_dev_brick_count), else the flux or
+ * fused stage kernel. */
 int  fv_dev_time_stages(fv_dev *ctx, int nrep, double *ms_stage, double *ms_flux_kernel);
 
 /* Measurement utility with no counterpart in the reference: copies `bytes`

@@ -232,7 +232,9 @@ def test_m5_device_consistent_solid_pulse_matches_oracle():
     So = O.OracleMesh(m).m5_advance(p, S0, 30)
     err = rel_l1(Sd, So)
     say(f"[m5 fix dev pulse] rel L1 {err:.2e}")
-    assert err <= 1e-11
+    # the solid's record takes pow(det G, 1/3) and 1/6 (device and host libm differ in the last
+    # bit), and the stress of a 2e10 Pa material carries a 1e6 Pa pulse: 30 steps end near 1e-10
+    assert err <= 5e-10
     x = m.centres()[:, 0]
     sxx = np.array([O.m5_record(p, s)["st"][0, 0] for s in Sd])
     w = np.clip(-(sxx[N // 2:] + 1e7), 0, None)
