@@ -998,10 +998,13 @@ struct BrickLds {
   using L = Lay<NM>;
   // staged cell states [v][c], avf-limited as stencil values and the cell's
   // own reconstruction base use them: own cells c < 64, the neighbour across
-  // surface slot s at 64 + s, that neighbour's stencil value beyond it at 160 + s
-  static constexpr int NC = 64 + 2 * BR_SLOTS;
+  // surface slot s at 64 + s (that neighbour's stencil value beyond it stays in
+  // the registers of the slot's lane)
+  static constexpr int NC = 64 + BR_SLOTS;
   double st[L::NEQ * NC];
+  double sn[L::NEQ * 64];               // S^n of the own cells (RK stages after the first)
   double araw[L::N3 * 64];              // the own cells' volume fractions before the limit
+  double src[(L::N3 + NM) * 64];        // the own cells' source coefficients: alpha_k, then -alpha_k p_k
   double ff[(L::NEQ + 1) * BR_TASKS];   // face fluxes A F and u*.n per face task
   uint32_t info[64];
   uint32_t infoh[BR_SLOTS];             // info word of the neighbour across each slot
@@ -1018,14 +1021,15 @@ FV_DEV void br_state(const BrickLds<NM> &B, int c, double *X) {
 }
 
 // face state (FP64 vector, relaxed) or FP32 record of a same-level cell at a
-// face, from its staged state (column c) and the stencil value columns across
-// the face (cg) and across the opposite face (co): k_recon_fast's arithmetic
+// face, from its staged state (column c) and the stencil values across the
+// face (column cg) and across the opposite face (column co, or Xo in registers
+// when xo_reg): k_recon_fast's arithmetic
 // with the brick's one level (dcf, 1/dcc), and zero slope when either face is
 // a domain boundary (bnd)
 template <int NM, bool MIXED, bool XR>
 FV_DEV void br_face_state(const Params &P, const fvf::Mat<NM, double> &Md, double a, const BrickLds<NM> &B, int c,
-                          int cg, int co, int limiter, bool bnd, double dcf, double idp, double idm, double *F,
-                          float *rec) {
+                          int cg, int co, const double *Xo, bool xo_reg, int limiter, bool bnd, double dcf,
+                          double idp, double idm, double *F, float *rec) {
   using L = Lay<NM>;
   using R = typename std::conditional<MIXED, float, double>::type;
   br_state<NM>(B, c, F);
@@ -1034,7 +1038,8 @@ FV_DEV void br_face_state(const Params &P, const fvf::Mat<NM, double> &Md, doubl
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) {
       const double sc = F[v];
-      const double xg = B.st[v * BrickLds<NM>::NC + cg], xo = B.st[v * BrickLds<NM>::NC + co];
+      const double xs = B.st[v * BrickLds<NM>::NC + co];
+      const double xg = B.st[v * BrickLds<NM>::NC + cg], xo = xo_reg ? Xo[v] : xs;
       const R g = fvf::slope<R>(limiter, xg - sc, xo - sc, ip, im);
       F[v] = sc + dcf * (double)g;
     }
@@ -1087,34 +1092,52 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
   const fvf::Mat<NM, double> Md = fvf::make_mat<NM, double>(P);
   const double a = P.amin / 100.0;
 
-  // ---- stage the cell states, the info words, the neighbour across each
-  // surface slot and that neighbour's stencil value beyond it
-  for (int q = tid; q < L::NEQ * 64; q += BR_THREADS) {
-    const int v = q >> 6, l = q & 63;
-    B.st[v * NC + l] = S[(size_t)v * n + b0 + l];
+  // ---- one staging phase: S^n (every lane), the own states (lanes 96..159,
+  // one cell each: raw fractions, source coefficients and the avf-limited
+  // state), the info words and limiters (lanes 160..223), and per surface slot
+  // (lanes 0..95) the neighbour across it, whose stencil value beyond it the
+  // slot's lane keeps in registers for its face task.  No load waits for
+  // another lane's.
+  if (cs != 1) {
+    for (int q = tid; q < L::NEQ * 64; q += BR_THREADS) {
+      const int v = q >> 6, l = q & 63;
+      B.sn[v * 64 + l] = SN[(size_t)v * n + b0 + l];
+    }
   }
-  if (tid < 64) {
-    B.info[tid] = info[b0 + tid];
-    B.lim[tid] = (uint8_t)leaf_limiter(P, lim, b0 + tid);
-  }
-  __syncthreads();
-  if (tid < 64) {
-    // the own states avf-limited in place; the raw fractions kept for the cell update
-    double X[L::NEQ];
-    br_state<NM>(B, tid, X);
+  double X2[L::NEQ];   // slot lanes: the slot neighbour's stencil value beyond it
 #pragma unroll
-    for (int k = 0; k < L::N3; k++) B.araw[k * 64 + tid] = X[L::I3 + k];
+  for (int v = 0; v < L::NEQ; v++) X2[v] = 0.0;
+  if (tid >= 96 && tid < 160) {
+    const int l = tid - 96;
+    double X[L::NEQ];
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) X[v] = S[(size_t)v * n + b0 + l];
+#pragma unroll
+    for (int k = 0; k < L::N3; k++) B.araw[k * 64 + l] = X[L::I3 + k];
+    {
+      // the sources of the cell update take the raw state (k_flux_fast's fill_cons of S)
+      fvf::SolF<NM, double> sc;
+      fvf::fill_cons<NM, double>(Md, a, X, sc);
+#pragma unroll
+      for (int k = 0; k < L::N3; ++k) B.src[k * 64 + l] = sc.avf[k];
+#pragma unroll
+      for (int k = 0; k < NM; ++k) B.src[(L::N3 + k) * 64 + l] = -sc.avf[k] * sc.p[k];
+    }
     avf_limit<NM, double>(a, X);
 #pragma unroll
-    for (int k = 0; k < L::N3; k++) B.st[(L::I3 + k) * NC + tid] = X[L::I3 + k];
-  } else if (tid < 64 + BR_SLOTS) {
-    const int s = tid - 64, f = s >> 4, i = b0 + br_slot_cell(s);
+    for (int v = 0; v < L::NEQ; v++) B.st[v * NC + l] = X[v];
+  } else if (tid >= 160 && tid < 224) {
+    const int l = tid - 160;
+    B.info[l] = info[b0 + l];
+    B.lim[l] = (uint8_t)leaf_limiter(P, lim, b0 + l);
+  } else if (tid < BR_SLOTS) {
+    const int s = tid, f = s >> 4, i = b0 + br_slot_cell(s);
     const int nb = nbr[(size_t)f * n + i];
     B.nb[s] = nb;
     if (face_kind(info[i], f) == FV_FK_BOUNDARY) {
       // no state across a boundary face: its slope is zero and its ghost comes from the face state
 #pragma unroll
-      for (int v = 0; v < L::NEQ; v++) { B.st[v * NC + 64 + s] = 0.0; B.st[v * NC + 160 + s] = 0.0; }
+      for (int v = 0; v < L::NEQ; v++) B.st[v * NC + 64 + s] = 0.0;
       B.need[s] = 0;
       B.infoh[s] = 0;
       B.lim[64 + s] = 0;
@@ -1124,7 +1147,7 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
       B.infoh[s] = infj;
       B.need[s] = inbrick[nb] ? 0 : 1;
       B.lim[64 + s] = (uint8_t)leaf_limiter(P, lim, nb);
-      double Xj[L::NEQ], X2[L::NEQ], Araw[NM] = {};
+      double Xj[L::NEQ], Araw[NM] = {};
 #pragma unroll
       for (int v = 0; v < L::NEQ; v++) Xj[v] = S[(size_t)v * n + nb];
       const int kj = face_kind(infj, f);
@@ -1140,10 +1163,7 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
       avf_limit<NM, double>(a, Xj);
       stencil_value<NM>(S, n, nb, kj, nbr[(size_t)f * n + nb], fine4, Xj, Araw, a, X2);
 #pragma unroll
-      for (int v = 0; v < L::NEQ; v++) {
-        B.st[v * NC + 64 + s] = Xj[v];
-        B.st[v * NC + 160 + s] = X2[v];
-      }
+      for (int v = 0; v < L::NEQ; v++) B.st[v * NC + 64 + s] = Xj[v];
     }
   }
   __syncthreads();
@@ -1163,7 +1183,7 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
     // stencil value columns across the face and across the opposite face, its
     // limiter, zero-slope flag and 1/dcc beyond it
     int cA, gA, oA, cB, gB, oB, limA, limB;
-    bool bndA, bndB, need = false;
+    bool bndA, bndB, need = false, oreg = false;
     double idmB = idcc;
     if (surf) {
       const int s = tid, f = s >> 4, fo = FCOPP_D[f], ax = br_axis(f);
@@ -1186,7 +1206,8 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
         // the neighbour's face state towards the brick: its stencil value across
         // that face is the brick cell, beyond it the staged second layer
         const uint32_t infj = B.infoh[s];
-        cB = 64 + s; gB = l; oB = 160 + s; limB = B.lim[64 + s];
+        cB = 64 + s; gB = l; oB = l; limB = B.lim[64 + s];   // the column beyond is X2
+        oreg = true;
         bndB = face_kind(infj, fo) == FV_FK_BOUNDARY || face_kind(infj, f) == FV_FK_BOUNDARY;
         idmB = B.idmh[s];
       }
@@ -1207,11 +1228,12 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
     fvf::SolF<NM, R> sL, sR;
     double F[L::NEQ];
     float rec[MIXED ? fvf::Rec<NM>::N : 1];
-    br_face_state<NM, MIXED, XR>(P, Md, a, B, cA, gA, oA, limA, bndA, dcf, idcc, idcc, F, rec);
+    // the right side first: its stencil value beyond may be X2, which then dies
+    br_face_state<NM, MIXED, XR>(P, Md, a, B, cB, gB, oB, X2, oreg, limB, bndB, dcf, idcc, idmB, F, rec);
+    br_sol<NM, MIXED, R>(Md, M, a, F, rec, sR);
+    br_face_state<NM, MIXED, XR>(P, Md, a, B, cA, gA, oA, X2, false, limA, bndA, dcf, idcc, idcc, F, rec);
     if (need) br_write_sf<NM, MIXED>(n, b0 + lown, fL, F, rec, SF, SFm);
     br_sol<NM, MIXED, R>(Md, M, a, F, rec, sL);
-    br_face_state<NM, MIXED, XR>(P, Md, a, B, cB, gB, oB, limB, bndB, dcf, idcc, idmB, F, rec);
-    br_sol<NM, MIXED, R>(Md, M, a, F, rec, sR);
     if (bcode != 0) sR = sL;
     R fr[9];
 #pragma unroll
@@ -1234,20 +1256,18 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
   br_state<NM>(B, l, Sc);
 #pragma unroll
   for (int k = 0; k < L::N3; k++) Sc[L::I3 + k] = B.araw[k * 64 + l];   // the raw state
-  fvf::SolF<NM, double> sc;
-  fvf::fill_cons<NM, double>(Md, a, Sc, sc);
   double src_avf[NM], src_st[NM];
 #pragma unroll
-  for (int k = 0; k < L::N3; ++k) src_avf[k] = sc.avf[k];
+  for (int k = 0; k < L::N3; ++k) src_avf[k] = B.src[k * 64 + l];
 #pragma unroll
-  for (int k = 0; k < NM; ++k) src_st[k] = -sc.avf[k] * sc.p[k];
+  for (int k = 0; k < NM; ++k) src_st[k] = B.src[(L::N3 + k) * 64 + l];
   const bool fixnc = (P.fix & 1) != 0;
   const double Area_f = A;
   const BrPos pos = br_pos(l);
   double RHS[L::NEQ];
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) RHS[v] = 0.0;
-#pragma unroll 1
+#pragma unroll
   for (int f = 0; f < 6; f++) {
     const int ax = br_axis(f), along = br_get(pos, ax), k = br_sheet(ax, pos);
     const int kind = face_kind(inf, f);
@@ -1289,7 +1309,7 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
   for (int v = 0; v < L::NEQ; v++) {
     double sn;
     if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
-    else sn = SN[(size_t)v * n + i];
+    else sn = B.sn[v * 64 + l];
     Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
   }
   fvf::relax_fast<NM, XR>(P, Md, Sn);
