@@ -514,6 +514,39 @@ def m5_leg(level=8, steps=3, check=True, log_fn=log):
                                "achieved": round(B * m.n / (ms_stage * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS}
             out["roofline"]["frac"] = round(out["roofline"]["achieved"] / HBM_PEAK_GBPS, 4)
             log_fn(f"[m5] {m.n} leaves: {ms_stage:.3f} ms/stage, residual {ms_res:.3f} ms")
+            # mixed precision: FP32 slopes, face records, HLLC and GFM flux; one step from the
+            # same state against the FP64 step, per conserved field
+            dev.set_precision(F.PREC_MIXED)
+            dev.set_state(S)
+            dev.time_steps(1)
+            mm = []
+            for _ in range(steps):
+                dev.set_state(S)
+                mm.append(dev.time_steps(1))
+            Sm = dev.get_state()
+            ms_mx = float(np.mean([x[0] for x in mm]))
+            nm = 3
+            names = [f"alpha_rho_{k}" for k in range(nm)] + ["rho_u", "rho_v", "rho_w", "rho_E"] + \
+                [f"alpha_{k}" for k in range(nm - 1)] + [f"alpha_rho_e_{k}" for k in range(nm)] + \
+                [f"A_{a}{b}" for a in range(3) for b in range(3)]
+            # momentum on the momentum magnitude, the deformation tensor on its Frobenius norm
+            # (its off-diagonal entries start at zero)
+            scale1 = np.abs(Sg).sum(0)
+            scalei = np.abs(Sg).max(0)
+            mom = np.sqrt((Sg[:, nm:nm + 3] ** 2).sum(1))
+            an = np.sqrt((Sg[:, -9:] ** 2).sum(1))
+            scale1[nm:nm + 3] = mom.sum(); scalei[nm:nm + 3] = mom.max()
+            scale1[-9:] = an.sum(); scalei[-9:] = an.max()
+            l1 = np.abs(Sm - Sg).sum(0) / np.maximum(scale1, 1e-300)
+            li = np.abs(Sm - Sg).max(0) / np.maximum(scalei, 1e-300)
+            out["mixed"] = {"ms_per_stage": round(ms_mx, 3), "ms_residual_kernel": round(float(np.mean([x[1] for x in mm])), 3),
+                            "value": round(m.n / (ms_mx * 1e-3) / 1e6, 2),
+                            "speedup_vs_fp64_stage": round(ms_stage / ms_mx, 3),
+                            "rel_l1_vs_fp64_1step": {k: float(v) for k, v in zip(names, l1)},
+                            "rel_linf_vs_fp64_1step": {k: float(v) for k, v in zip(names, li)},
+                            "rel_l1_vs_fp64_1step_max_field": float(l1.max()),
+                            "finite": bool(np.all(np.isfinite(Sm)))}
+            log_fn(f"[m5] mixed: {ms_mx:.3f} ms/stage, max field L1 vs FP64 {l1.max():.2e} ({names[int(l1.argmax())]})")
         else:
             dev.advance(1)
             Sg = dev.get_state()

@@ -265,6 +265,7 @@ def lib():
             "fv_m5_dev_residual": (C.c_int, [C.c_void_p]),
             "fv_m5_dev_get_stage_arrays": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_uint8)]),
             "fv_m5_dev_time_steps": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
+            "fv_m5_dev_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -437,6 +438,10 @@ class M5Device:
 
     def begin_step(self):
         _check(lib().fv_m5_dev_begin_step(self.h), "fv_m5_dev_begin_step")
+
+    def set_precision(self, prec):
+        """PREC_FP64 (default) or PREC_MIXED: FP32 slopes, face records, HLLC and GFM flux."""
+        _check(lib().fv_m5_dev_set_precision(self.h, prec), "fv_m5_dev_set_precision")
 
     def stage(self, irk):
         _check(lib().fv_m5_dev_stage(self.h, irk), "fv_m5_dev_stage")

@@ -53,27 +53,33 @@ struct L5 {
   static constexpr int I0 = 0, I1 = NM, I2 = NM + 3, I3 = NM + 4, N3 = NM - 1, I4 = NM + 4 + NM - 1, I5 = I4 + NM;
 };
 
-__device__ __forceinline__ double sq(double x) { return x * x; }
+template <typename R>
+__device__ __forceinline__ R sq(R x) { return x * x; }
 __device__ __forceinline__ bool has_solid(const M5P &P) { return P.mush[0] != 0.0 || P.mush[1] != 0.0 || P.mush[2] != 0.0; }
-__device__ __forceinline__ double maxr(double a, double b) { return (a > b) ? a : b; }   // strdata.h:27
-__device__ __forceinline__ double minr(double a, double b) { return (a < b) ? a : b; }   // strdata.h:31
+template <typename R>
+__device__ __forceinline__ R maxr(R a, R b) { return (a > b) ? a : b; }   // strdata.h:27
+template <typename R>
+__device__ __forceinline__ R minr(R a, R b) { return (a < b) ? a : b; }   // strdata.h:31
 
 // pow(|G|, 1/3) and pow(|G|, 1/6), which the elastic energy and the stress of
 // every solid material of one state use: evaluated once per state
+template <typename R = double>
 struct GPow {
-  double g13, g16;
+  R g13, g16;
 };
-__device__ GPow gpow(double Gmag) { return GPow{pow(fabs(Gmag), 1.0 / 3.0), pow(fabs(Gmag), 1.0 / 6.0)}; }
+template <typename R>
+__device__ GPow<R> gpow(R Gmag) { return GPow<R>{pow(fabs(Gmag), R(1.0) / R(3.0)), pow(fabs(Gmag), R(1.0) / R(6.0))}; }
 
 // m5_elastic_energy_solid_V2, state_equation.c:269-316
-__device__ double elastic_energy_v2(const double *G, double Gmag, const GPow &gp, double mu) {
-  double eints;
-  if (fabs(Gmag) != 0.0) {
-    double M1[9], M2[9];
-    double Gmag13 = gp.g13;
-    if (isnan(Gmag13)) Gmag13 = 0.0;
+template <typename R>
+__device__ R elastic_energy_v2(const R *G, R Gmag, const GPow<R> &gp, R mu) {
+  R eints;
+  if (fabs(Gmag) != R(0.0)) {
+    R M1[9], M2[9];
+    R Gmag13 = gp.g13;
+    if (isnan(Gmag13)) Gmag13 = R(0.0);
 #pragma unroll
-    for (int k = 0; k < 9; k++) { M1[k] = G[k]; M2[k] = 0.0; }
+    for (int k = 0; k < 9; k++) { M1[k] = G[k]; M2[k] = R(0.0); }
     M1[0] = G[0] - Gmag13; M1[4] = G[4] - Gmag13; M1[8] = G[8] - Gmag13;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -81,60 +87,63 @@ __device__ double elastic_energy_v2(const double *G, double Gmag, const GPow &gp
       for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int j = 0; j < 3; ++j) M2[i * 3 + j] += M1[i * 3 + k] * M1[k * 3 + j];
-    const double tr1 = M2[0] + M2[4] + M2[8];
-    eints = mu / (4.0 * gp.g16) * tr1;
+    const R tr1 = M2[0] + M2[4] + M2[8];
+    eints = mu / (R(4.0) * gp.g16) * tr1;
   } else {
-    eints = 0.0;
+    eints = R(0.0);
   }
   return eints;
 }
 
 // G_det (state_equation.c:206-215) and m5_elastic_energy_solid (:217-267)
-__device__ double elastic_energy_v1(const double *G, double mu) {
-  const double Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
+template <typename R>
+__device__ R elastic_energy_v1(const R *G, R mu) {
+  const R Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
                       G[2] * (G[3] * G[7] - G[4] * G[6]);
-  if ((fabs(Gmag) != 0.0) && (mu != 0.0)) return elastic_energy_v2(G, Gmag, gpow(Gmag), mu);
-  return 0.0;
+  if ((fabs(Gmag) != R(0.0)) && (mu != R(0.0))) return elastic_energy_v2(G, Gmag, gpow(Gmag), mu);
+  return R(0.0);
 }
 
 // m5_eos_stress_V2, state_equation.c:143-204: the stress s of one material
-__device__ void stress_v2(double *s, double avf, double pr, const double *G, double Gmag, const GPow &gp, double mu) {
-  if ((fabs(Gmag) != 0.0) && (mu != 0.0)) {
-    double G2[9], ss[9];
+template <typename R>
+__device__ void stress_v2(R *s, R avf, R pr, const R *G, R Gmag, const GPow<R> &gp, R mu) {
+  if ((fabs(Gmag) != R(0.0)) && (mu != R(0.0))) {
+    R G2[9], ss[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) G2[k] = 0.0;
+    for (int k = 0; k < 9; k++) G2[k] = R(0.0);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int j = 0; j < 3; ++j) G2[i * 3 + j] += G[i * 3 + k] * G[k * 3 + j];
-    const double J1 = G[0] + G[4] + G[8];
-    const double J2 = G2[0] + G2[4] + G2[8];
-    const double g13 = gp.g13;
-    ss[0] = G2[0] - J2 / 3.0 - g13 * (G[0] - J1 / 3.0);
+    const R J1 = G[0] + G[4] + G[8];
+    const R J2 = G2[0] + G2[4] + G2[8];
+    const R g13 = gp.g13;
+    ss[0] = G2[0] - J2 / R(3.0) - g13 * (G[0] - J1 / R(3.0));
     ss[1] = G2[1] - g13 * (G[1]);
     ss[2] = G2[2] - g13 * (G[2]);
     ss[3] = G2[3] - g13 * (G[3]);
-    ss[4] = G2[4] - J2 / 3.0 - g13 * (G[4] - J1 / 3.0);
+    ss[4] = G2[4] - J2 / R(3.0) - g13 * (G[4] - J1 / R(3.0));
     ss[5] = G2[5] - g13 * (G[5]);
     ss[6] = G2[6] - g13 * (G[6]);
     ss[7] = G2[7] - g13 * (G[7]);
-    ss[8] = G2[8] - J2 / 3.0 - g13 * (G[8] - J1 / 3.0);
-    const double k = mu / gp.g16;
+    ss[8] = G2[8] - J2 / R(3.0) - g13 * (G[8] - J1 / R(3.0));
+    const R k = mu / gp.g16;
 #pragma unroll
     for (int q = 0; q < 9; q++) s[q] = (q == 0 || q == 4 || q == 8) ? (-avf * pr - k * (ss[q])) : (-k * (ss[q]));
   } else {
-    s[0] = -avf * pr; s[1] = 0.0; s[2] = 0.0;
-    s[3] = 0.0; s[4] = -avf * pr; s[5] = 0.0;
-    s[6] = 0.0; s[7] = 0.0; s[8] = -avf * pr;
+    s[0] = -avf * pr; s[1] = R(0.0); s[2] = R(0.0);
+    s[3] = R(0.0); s[4] = -avf * pr; s[5] = R(0.0);
+    s[6] = R(0.0); s[7] = R(0.0); s[8] = -avf * pr;
   }
 }
 
 // the sum of the materials' stresses (m5_facercnstr.c:150-173)
-__device__ void add_stress_v2(double *st, double avf, double pr, const double *G, double Gmag, const GPow &gp,
-                              double mu) {
-  double s[9];
+template <typename R>
+__device__ void add_stress_v2(R *st, R avf, R pr, const R *G, R Gmag, const GPow<R> &gp,
+                              R mu) {
+  R s[9];
   stress_v2(s, avf, pr, G, Gmag, gp, mu);
 #pragma unroll
   for (int q = 0; q < 9; q++) st[q] += s[q];
@@ -154,14 +163,47 @@ __device__ void gmat(const double *A, double *G, double &Gmag) {
   Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) + G[2] * (G[3] * G[7] - G[4] * G[6]);
 }
 
-template <int NM>
+template <int NM, typename R = double>
 struct Rec5 {
-  double r, e, u, v, w, phi0;
-  double ra[NM], avf[NM], are[NM], c2Y[NM], ph[NM];
-  double A[9], st[9];
-  double vm[3], vE;        // vec momentum and energy (rewritten for a ghost)
-  double ut, vt, wt;       // u_temp, v_temp, w_temp
+  R r, e, u, v, w, phi0;
+  R ra[NM], avf[NM], are[NM], c2Y[NM], ph[NM];
+  R A[9], st[9];
+  R vm[3], vE;        // vec momentum and energy (rewritten for a ghost)
+  R ut, vt, wt;       // u_temp, v_temp, w_temp
 };
+
+// s = left ? a : b, field by field (selects in registers)
+template <int NM, typename R>
+__device__ __forceinline__ void sel_rec(bool left, const Rec5<NM, R> &a, const Rec5<NM, R> &b, Rec5<NM, R> &s) {
+  s.r = left ? a.r : b.r; s.e = left ? a.e : b.e; s.u = left ? a.u : b.u; s.v = left ? a.v : b.v;
+  s.w = left ? a.w : b.w; s.phi0 = left ? a.phi0 : b.phi0;
+#pragma unroll
+  for (int k = 0; k < NM; k++) {
+    s.ra[k] = left ? a.ra[k] : b.ra[k]; s.avf[k] = left ? a.avf[k] : b.avf[k]; s.are[k] = left ? a.are[k] : b.are[k];
+    s.c2Y[k] = left ? a.c2Y[k] : b.c2Y[k]; s.ph[k] = left ? a.ph[k] : b.ph[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) { s.A[k] = left ? a.A[k] : b.A[k]; s.st[k] = left ? a.st[k] : b.st[k]; }
+#pragma unroll
+  for (int k = 0; k < 3; k++) s.vm[k] = left ? a.vm[k] : b.vm[k];
+  s.vE = left ? a.vE : b.vE; s.ut = left ? a.ut : b.ut; s.vt = left ? a.vt : b.vt; s.wt = left ? a.wt : b.wt;
+}
+
+// the record in the face arithmetic's precision (FP64 records are built first:
+// the pressures carry p_inf of up to 2e10 Pa and are formed in FP64)
+template <int NM, typename R>
+__device__ void narrow(const Rec5<NM, double> &a, Rec5<NM, R> &b) {
+  b.r = (R)a.r; b.e = (R)a.e; b.u = (R)a.u; b.v = (R)a.v; b.w = (R)a.w; b.phi0 = (R)a.phi0;
+#pragma unroll
+  for (int k = 0; k < NM; k++) {
+    b.ra[k] = (R)a.ra[k]; b.avf[k] = (R)a.avf[k]; b.are[k] = (R)a.are[k]; b.c2Y[k] = (R)a.c2Y[k]; b.ph[k] = (R)a.ph[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) { b.A[k] = (R)a.A[k]; b.st[k] = (R)a.st[k]; }
+#pragma unroll
+  for (int k = 0; k < 3; k++) b.vm[k] = (R)a.vm[k];
+  b.vE = (R)a.vE; b.ut = (R)a.ut; b.vt = (R)a.vt; b.wt = (R)a.wt;
+}
 
 // Record of a state vector x, n_solids = 0.  FACE = false: m5_facevalues.c:3-188,
 // x a cell state.  FACE = true: m5_facercnstr.c:3-190, x a face state whose phi
@@ -225,7 +267,7 @@ __device__ void record(const M5P &P, const double *x, Rec5<NM> &r, double cavf0 
   G[8] = A[6] * A[6] + A[7] * A[7] + A[8] * A[8];
   const double Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
                       G[2] * (G[3] * G[7] - G[4] * G[6]);
-  const GPow gp = has_solid(P) ? gpow(Gmag) : GPow{0.0, 0.0};
+  const GPow<> gp = has_solid(P) ? gpow(Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < NM; ++i) {
     const double are_elast = (P.mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, gp, P.mush[i]) : 0.0;
@@ -239,14 +281,15 @@ __device__ void record(const M5P &P, const double *x, Rec5<NM> &r, double cavf0 
 }
 
 // rotate_tensor (rotate_vectors.c:128-188): t = V st V^T, V rows a, b, c
-__device__ void rot_tensor(double *t, const double *st, const double *a, const double *b, const double *c) {
-  double V[9] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]};
-  double M1[9];
+template <typename R>
+__device__ void rot_tensor(R *t, const R *st, const R *a, const R *b, const R *c) {
+  R V[9] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]};
+  R M1[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      double s = 0.0;
+      R s = R(0.0);
 #pragma unroll
       for (int k = 0; k < 3; ++k) s += V[i * 3 + k] * st[k * 3 + j];
       M1[i * 3 + j] = s;
@@ -255,58 +298,58 @@ __device__ void rot_tensor(double *t, const double *st, const double *a, const d
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      double s = 0.0;
+      R s = R(0.0);
 #pragma unroll
       for (int k = 0; k < 3; ++k) s += M1[i * 3 + k] * V[j * 3 + k];   // vecT[k][j] = V[j][k]
       t[i * 3 + j] = s;
     }
 }
 
-template <int NM>
+template <int NM, typename R = double>
 struct Star5 {
-  double Us[NM + 4];
-  double unc[3];
-  double avf_S[NM], are_S[NM], AS[9];
+  R Us[NM + 4];
+  R unc[3];
+  R avf_S[NM], are_S[NM], AS[9];
 };
 
 // m5_star_region.c:3-157 for side s (flag 1: left, 2: right)
-template <int NM>
-__device__ void star_region(const M5P &P, const double *F, const Rec5<NM> &sL, const Rec5<NM> &sR,
-                            const Rec5<NM> &s, int flag_ifs, double u_S, double SL, double SR, double uL,
-                            double vL, double wL, double uR, double vR, double wR, double s11L, double s12L,
-                            double s13L, double s11R, double s12R, double s13R, Star5<NM> &W) {
-  const double *n = F, *m = F + 3, *l = F + 6;
-  double SK, uK, vK, wK, s11K, s12K, s13K;
+template <int NM, typename R>
+__device__ void star_region(const M5P &P, const R *F, const Rec5<NM, R> &sL, const Rec5<NM, R> &sR,
+                            const Rec5<NM, R> &s, int flag_ifs, R u_S, R SL, R SR, R uL,
+                            R vL, R wL, R uR, R vR, R wR, R s11L, R s12L,
+                            R s13L, R s11R, R s12R, R s13R, Star5<NM, R> &W) {
+  const R *n = F, *m = F + 3, *l = F + 6;
+  R SK, uK, vK, wK, s11K, s12K, s13K;
   if (flag_ifs == 1) { SK = SL; uK = uL; vK = vL; wK = wL; s11K = s11L; s12K = s12L; s13K = s13L; }
   else { SK = SR; uK = uR; vK = vR; wK = wR; s11K = s11R; s12K = s12R; s13K = s13R; }
-  double ar_S[NM], r_S = 0.0;
+  R ar_S[NM], r_S = R(0.0);
 #pragma unroll
   for (int v = 0; v < NM; ++v) { ar_S[v] = s.ra[v] * (SK - uK) / (SK - u_S); r_S += ar_S[v]; }
-  const double s11_S = ((uR - SR) * sR.r * (s11L) - (uL - SL) * sL.r * (s11R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (uR - uL)) /
+  const R s11_S = ((uR - SR) * sR.r * (s11L) - (uL - SL) * sL.r * (s11R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (uR - uL)) /
                        ((uR - SR) * sR.r - (uL - SL) * sL.r);
-  double s12_S, s13_S;
-  if ((sL.phi0 > 0.0) && (sR.phi0 > 0.0)) {
+  R s12_S, s13_S;
+  if ((sL.phi0 > R(0.0)) && (sR.phi0 > R(0.0))) {
     s12_S = ((uR - SR) * sR.r * (s12L) - (uL - SL) * sL.r * (s12R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (vR - vL)) /
             ((uR - SR) * sR.r - (uL - SL) * sL.r);
     s13_S = ((uR - SR) * sR.r * (s13L) - (uL - SL) * sL.r * (s13R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (wR - wL)) /
             ((uR - SR) * sR.r - (uL - SL) * sL.r);
   } else {
-    s12_S = 0.0;
-    s13_S = 0.0;
+    s12_S = R(0.0);
+    s13_S = R(0.0);
   }
-  const double v_S = vK + (s12_S - s12K) / ((uK - SK) * s.r);
-  const double w_S = wK + (s13_S - s13K) / ((uK - SK) * s.r);
-  double vel[3];
+  const R v_S = vK + (s12_S - s12K) / ((uK - SK) * s.r);
+  const R w_S = wK + (s13_S - s13K) / ((uK - SK) * s.r);
+  R vel[3];
   vel[0] = u_S * n[0] + v_S * m[0] + w_S * l[0];   // rotate_vector_ltg_v2
   vel[1] = u_S * n[1] + v_S * m[1] + w_S * l[1];
   vel[2] = u_S * n[2] + v_S * m[2] + w_S * l[2];
-  const double re = s.r * s.e;
+  const R re = s.r * s.e;
   // consistent: - s11K uK over r* (u* - S_K); published (m5_star_region.c:73): + s11K uK over r* (u* + S_K)
-  const double Em_S = P.fix ? (re * (uK - SK) - s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
+  const R Em_S = P.fix ? (re * (uK - SK) - s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
                                   (r_S * (u_S - SK))
                             : (re * (uK - SK) + s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
                                   (r_S * (u_S + SK));
-  double A[9], t[9], AS[9];
+  R A[9], t[9], AS[9];
   rot_tensor(t, s.A, n, m, l);
 #pragma unroll
   for (int k = 0; k < 9; k++) A[k] = t[k];
@@ -317,27 +360,27 @@ __device__ void star_region(const M5P &P, const double *F, const Rec5<NM> &sL, c
 #pragma unroll
   for (int j = 0; j < 3; ++j) { AS[3 + j] = A[3 + j]; AS[6 + j] = A[6 + j]; }
   {
-    const double c0[3] = {n[0], m[0], l[0]}, c1[3] = {n[1], m[1], l[1]}, c2[3] = {n[2], m[2], l[2]};
+    const R c0[3] = {n[0], m[0], l[0]}, c1[3] = {n[1], m[1], l[1]}, c2[3] = {n[2], m[2], l[2]};
     rot_tensor(t, AS, c0, c1, c2);
   }
 #pragma unroll
   for (int k = 0; k < 9; k++) AS[k] = t[k];
-  double G[9];
+  R G[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) G[i * 3 + j] = AS[i * 3 + 0] * AS[j * 3 + 0] + AS[i * 3 + 1] * AS[j * 3 + 1] + AS[i * 3 + 2] * AS[j * 3 + 2];
-  double e_S[NM];
+  R e_S[NM];
 #pragma unroll
   for (int v = 0; v < NM; ++v) {
-    const double g = P.gamma[v], pi = P.pinf[v];
-    const double rm = (s.ra[v] / s.avf[v]);
-    const double rm_S = (s.ra[v] / s.avf[v]) * (uK - SK) / (u_S - SK);
-    const double p_k = P.fix ? s.ph[v] : -s.ph[v];   // published: -p_hydro
-    const double p_S = (p_k + pi) * (((g - 1.0) * rm + (g + 1.0) * rm_S) / ((g - 1.0) * rm_S + (g + 1.0) * rm)) - pi;
-    const double are_el = elastic_energy_v1(G, P.mush[v]);
-    const double e_el = are_el / (s.avf[v] * rm_S);
-    e_S[v] = (p_S + g * pi) / (rm_S * (P.fix ? (g - 1.0) : (g + 1.0))) + e_el;   // published: gamma + 1
+    const R g = (R)P.gamma[v], pi = (R)P.pinf[v];
+    const R rm = (s.ra[v] / s.avf[v]);
+    const R rm_S = (s.ra[v] / s.avf[v]) * (uK - SK) / (u_S - SK);
+    const R p_k = P.fix ? s.ph[v] : -s.ph[v];   // published: -p_hydro
+    const R p_S = (p_k + pi) * (((g - R(1.0)) * rm + (g + R(1.0)) * rm_S) / ((g - R(1.0)) * rm_S + (g + R(1.0)) * rm)) - pi;
+    const R are_el = elastic_energy_v1(G, (R)P.mush[v]);
+    const R e_el = are_el / (s.avf[v] * rm_S);
+    e_S[v] = (p_S + g * pi) / (rm_S * (P.fix ? (g - R(1.0)) : (g + R(1.0)))) + e_el;   // published: gamma + 1
   }
 #pragma unroll
   for (int v = 0; v < NM; ++v) W.Us[v] = ar_S[v];
@@ -357,16 +400,16 @@ __device__ void star_region(const M5P &P, const double *F, const Rec5<NM> &sL, c
 // m5_flux_gfm.c): out[iv] = F.n + F_hllc as m5_residual.c:166-170 forms it
 // before the area, and the non-conservative face velocity.  sL and sR are
 // rewritten with the ghost.  Returns false where the reference prints NaN and exits.
-template <int NM>
-__device__ bool gfm_subface(const M5P &P, const double *F, Rec5<NM> &sL, Rec5<NM> &sR, double *out, double *unc) {
+template <int NM, typename R>
+__device__ bool gfm_subface(const M5P &P, const R *F, Rec5<NM, R> &sL, Rec5<NM, R> &sR, R *out, R *unc) {
   using L = L5<NM>;
-  const double *n = F, *m = F + 3, *l = F + 6;
+  const R *n = F, *m = F + 3, *l = F + 6;
   // face-frame velocities (rotate_vector_gtl_v2) and the ghost: v_R = v_L, w_R = w_L
-  const double uL = sL.u * n[0] + sL.v * n[1] + sL.w * n[2];
-  const double vL = sL.u * m[0] + sL.v * m[1] + sL.w * m[2];
-  const double wL = sL.u * l[0] + sL.v * l[1] + sL.w * l[2];
-  const double uR = sR.u * n[0] + sR.v * n[1] + sR.w * n[2];
-  const double vR = vL, wR = wL;
+  const R uL = sL.u * n[0] + sL.v * n[1] + sL.w * n[2];
+  const R vL = sL.u * m[0] + sL.v * m[1] + sL.w * m[2];
+  const R wL = sL.u * l[0] + sL.v * l[1] + sL.w * l[2];
+  const R uR = sR.u * n[0] + sR.v * n[1] + sR.w * n[2];
+  const R vR = vL, wR = wL;
   sL.ut = sL.u; sL.vt = sL.v; sL.wt = sL.w;
   sR.ut = uR * n[0] + vR * m[0] + wR * l[0];
   sR.vt = uR * n[1] + vR * m[1] + wR * l[1];
@@ -374,32 +417,35 @@ __device__ bool gfm_subface(const M5P &P, const double *F, Rec5<NM> &sL, Rec5<NM
   sR.vm[0] = sR.ut * sR.r; sR.vm[1] = sR.vt * sR.r; sR.vm[2] = sR.wt * sR.r;
   sR.vE = sR.r * sR.e;
   // m5_wave_speeds_gfm.c:16-58
-  const double ruL = sL.r * (uL), ruR = sR.r * (uR);
-  double c2YL = 0.0, c2YR = 0.0;
+  const R ruL = sL.r * (uL), ruR = sR.r * (uR);
+  R c2YL = R(0.0), c2YR = R(0.0);
 #pragma unroll
   for (int iv = 0; iv < NM; ++iv) { c2YL += sL.c2Y[iv]; c2YR += sR.c2Y[iv]; }
-  const double cL = sqrt(c2YL), cR = sqrt(c2YR);
-  const double SR = maxr((uL + cL), (uR + cR));
-  const double SL = minr((uL - cL), (uR - cR));
-  double t[9];
+  const R cL = sqrt(c2YL), cR = sqrt(c2YR);
+  const R SR = maxr((uL + cL), (uR + cR));
+  const R SL = minr((uL - cL), (uR - cR));
+  R t[9];
   rot_tensor(t, sL.st, n, m, l);
-  const double s11L = t[0], s12L = t[1], s13L = t[2];
+  const R s11L = t[0], s12L = t[1], s13L = t[2];
   rot_tensor(t, sR.st, n, m, l);
-  const double s11R = t[0], s12R = t[1], s13R = t[2];
-  const double u_S = ((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR)) /
+  const R s11R = t[0], s12R = t[1], s13R = t[2];
+  const R u_S = ((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR)) /
                      (ruL - ruR - ((SL)*sL.r) + ((SR)*sR.r));
   if (isnan(u_S) || isnan(SR) || isnan(SL)) return false;
   int flag_ifs;
-  if (0.0 <= SL) flag_ifs = 0;
-  else if ((SL < 0.0) && (0.0 <= u_S)) flag_ifs = 1;
-  else if ((u_S < 0.0) && (0.0 < SR)) flag_ifs = 2;
-  else flag_ifs = 3;   // SR <= 0.0, the only case left once no speed is NaN
-  const Rec5<NM> &s = (flag_ifs <= 1) ? sL : sR;
-  Star5<NM> W;
-  double SK = 0.0;
+  if (R(0.0) <= SL) flag_ifs = 0;
+  else if ((SL < R(0.0)) && (R(0.0) <= u_S)) flag_ifs = 1;
+  else if ((u_S < R(0.0)) && (R(0.0) < SR)) flag_ifs = 2;
+  else flag_ifs = 3;   // SR <= 0, the only case left once no speed is NaN
+  // the side's record by value, selected field by field: a reference chosen at run
+  // time would place both records in scratch memory
+  Rec5<NM, R> s;
+  sel_rec<NM, R>(flag_ifs <= 1, sL, sR, s);
+  Star5<NM, R> W;
+  R SK = R(0.0);
   if (flag_ifs == 1 || flag_ifs == 2) {
     SK = (flag_ifs == 1) ? SL : SR;
-    star_region<NM>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
+    star_region<NM, R>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
 #pragma unroll
     for (int iv = 0; iv < L::NCONS; ++iv)
       if (isnan(W.Us[iv])) return false;
@@ -415,10 +461,10 @@ __device__ bool gfm_subface(const M5P &P, const double *F, Rec5<NM> &sL, Rec5<NM
   }
   const bool star = (flag_ifs == 1 || flag_ifs == 2);
   bool ok = true;
-  auto emit = [&](int iv, double f0, double f1, double f2, double vec_iv) {
-    const double h = (star && iv < L::NCONS) ? SK * (W.Us[iv] - vec_iv) : 0.0;
+  auto emit = [&](int iv, R f0, R f1, R f2, R vec_iv) {
+    const R h = (star && iv < L::NCONS) ? SK * (W.Us[iv] - vec_iv) : R(0.0);
     if (isnan(h)) ok = false;
-    out[iv] = ((f0 - 0.0) * n[0] + (f1 - 0.0) * n[1] + (f2 - 0.0) * n[2]) + h;
+    out[iv] = ((f0 - R(0.0)) * n[0] + (f1 - R(0.0)) * n[1] + (f2 - R(0.0)) * n[2]) + h;
   };
   // m5_flux_gfm.c:10-39: physical flux of side s with its u_temp velocity
 #pragma unroll
@@ -430,11 +476,11 @@ __device__ bool gfm_subface(const M5P &P, const double *F, Rec5<NM> &sL, Rec5<NM
        s.r * s.e * s.vt - s.st[3] * s.ut - s.st[4] * s.vt - s.st[5] * s.wt,
        s.r * s.e * s.wt - s.st[6] * s.ut - s.st[7] * s.vt - s.st[8] * s.wt, s.vE);
 #pragma unroll
-  for (int v = 0; v < NM - 1; ++v) emit(L::I3 + v, W.unc[0] * W.avf_S[v], W.unc[1] * W.avf_S[v], W.unc[2] * W.avf_S[v], 0.0);
+  for (int v = 0; v < NM - 1; ++v) emit(L::I3 + v, W.unc[0] * W.avf_S[v], W.unc[1] * W.avf_S[v], W.unc[2] * W.avf_S[v], R(0.0));
 #pragma unroll
-  for (int v = 0; v < NM; ++v) emit(L::I4 + v, W.unc[0] * W.are_S[v], W.unc[1] * W.are_S[v], W.unc[2] * W.are_S[v], 0.0);
+  for (int v = 0; v < NM; ++v) emit(L::I4 + v, W.unc[0] * W.are_S[v], W.unc[1] * W.are_S[v], W.unc[2] * W.are_S[v], R(0.0));
 #pragma unroll
-  for (int k = 0; k < 9; k++) emit(L::I5 + k, W.unc[0] * W.AS[k], W.unc[1] * W.AS[k], W.unc[2] * W.AS[k], 0.0);
+  for (int k = 0; k < 9; k++) emit(L::I5 + k, W.unc[0] * W.AS[k], W.unc[1] * W.AS[k], W.unc[2] * W.AS[k], R(0.0));
   unc[0] = W.unc[0]; unc[1] = W.unc[1]; unc[2] = W.unc[2];
   return ok;
 }
@@ -459,7 +505,7 @@ __global__ __launch_bounds__(256) void k_m5_gfm_faces(int nf, M5P P, const doubl
 #pragma unroll
   for (int k = 0; k < 9; k++) F[k] = FR[(size_t)k * nf + q];
   double out[L::NEQ], un[3];
-  if (!gfm_subface<NM>(P, F, sL, sR, out, un)) { atomicOr(nanflag, 1); return; }
+  if (!gfm_subface<NM, double>(P, F, sL, sR, out, un)) { atomicOr(nanflag, 1); return; }
   const double A = area[q];
 #pragma unroll
   for (int iv = 0; iv < L::NEQ; iv++) {
@@ -717,14 +763,14 @@ __constant__ static const int FCOPP5[6] = {2, 3, 0, 1, 5, 4};   // operatetp.c:2
 
 // m5_bc.c:3-83 on the right record: 1 symmetry and 4 reflective flip the
 // normal velocity of the left record, 5 flips all three components
-template <int NM>
-__device__ void apply_bc5(int code, const double *F, const Rec5<NM> &sL, Rec5<NM> &sR) {
+template <int NM, typename R>
+__device__ void apply_bc5(int code, const R *F, const Rec5<NM, R> &sL, Rec5<NM, R> &sR) {
   if (code != 1 && code != 4 && code != 5) return;
-  const double *n = F, *m = F + 3, *l = F + 6;
-  const double ul = sL.u * n[0] + sL.v * n[1] + sL.w * n[2];
-  const double vl = sL.u * m[0] + sL.v * m[1] + sL.w * m[2];
-  const double wl = sL.u * l[0] + sL.v * l[1] + sL.w * l[2];
-  const double u1 = -ul, v1 = (code == 5) ? -vl : vl, w1 = (code == 5) ? -wl : wl;
+  const R *n = F, *m = F + 3, *l = F + 6;
+  const R ul = sL.u * n[0] + sL.v * n[1] + sL.w * n[2];
+  const R vl = sL.u * m[0] + sL.v * m[1] + sL.w * m[2];
+  const R wl = sL.u * l[0] + sL.v * l[1] + sL.w * l[2];
+  const R u1 = -ul, v1 = (code == 5) ? -vl : vl, w1 = (code == 5) ? -wl : wl;
   sR.u = u1 * n[0] + v1 * m[0] + w1 * l[0];
   sR.v = u1 * n[1] + v1 * m[1] + w1 * l[1];
   sR.w = u1 * n[2] + v1 * m[2] + w1 * l[2];
@@ -732,10 +778,10 @@ __device__ void apply_bc5(int code, const double *F, const Rec5<NM> &sL, Rec5<NM
 
 // m5_flux.c:10-39: the physical flux rows of a record, F.n per equation for
 // the mass, momentum (sg sigma: + published, - consistent) and energy rows
-template <int NM>
-__device__ void phys_rows(const Rec5<NM> &s, const double *n, double sg, double *adv) {
+template <int NM, typename R>
+__device__ void phys_rows(const Rec5<NM, R> &s, const R *n, R sg, R *adv) {
   using L = L5<NM>;
-  auto dot = [&](double f0, double f1, double f2) { return (f0 - 0.0) * n[0] + (f1 - 0.0) * n[1] + (f2 - 0.0) * n[2]; };
+  auto dot = [&](R f0, R f1, R f2) { return (f0 - R(0.0)) * n[0] + (f1 - R(0.0)) * n[1] + (f2 - R(0.0)) * n[2]; };
 #pragma unroll
   for (int v = 0; v < NM; ++v) adv[L::I0 + v] = dot(s.u * s.ra[v], s.v * s.ra[v], s.w * s.ra[v]);
   adv[L::I1 + 0] = dot(s.u * s.u * s.r + sg * s.st[0], s.v * s.u * s.r + sg * s.st[1], s.w * s.u * s.r + sg * s.st[2]);
@@ -751,45 +797,48 @@ __device__ void phys_rows(const Rec5<NM> &s, const double *n, double sg, double 
 // non-conservative face velocity.  bc 4 forces u* = 0; bc 1 (symmetry)
 // replaces the flux by the mean of the physical fluxes of the cell record of
 // xc and its mirror.  Returns false where the reference prints NaN and exits.
-template <int NM>
-__device__ bool hllc_subface(const M5P &P, const double *F, int bc, const Rec5<NM> &sL, const Rec5<NM> &sR,
-                             const double *__restrict__ S, int n, int cell, double *out, double *unc) {
+template <int NM, typename R>
+__device__ bool hllc_subface(const M5P &P, const R *F, int bc, const Rec5<NM, R> &sL, const Rec5<NM, R> &sR,
+                             const double *__restrict__ S, int n, int cell, R *out, R *unc) {
   using L = L5<NM>;
-  const double *nv = F, *m = F + 3, *l = F + 6;
-  const double uL = sL.u * nv[0] + sL.v * nv[1] + sL.w * nv[2];
-  const double vL = sL.u * m[0] + sL.v * m[1] + sL.w * m[2];
-  const double wL = sL.u * l[0] + sL.v * l[1] + sL.w * l[2];
-  const double uR = sR.u * nv[0] + sR.v * nv[1] + sR.w * nv[2];
-  const double vR = sR.u * m[0] + sR.v * m[1] + sR.w * m[2];
-  const double wR = sR.u * l[0] + sR.v * l[1] + sR.w * l[2];
-  const double ruL = sL.r * (uL), ruR = sR.r * (uR);
-  double c2YL = 0.0, c2YR = 0.0;
+  const R *nv = F, *m = F + 3, *l = F + 6;
+  const R uL = sL.u * nv[0] + sL.v * nv[1] + sL.w * nv[2];
+  const R vL = sL.u * m[0] + sL.v * m[1] + sL.w * m[2];
+  const R wL = sL.u * l[0] + sL.v * l[1] + sL.w * l[2];
+  const R uR = sR.u * nv[0] + sR.v * nv[1] + sR.w * nv[2];
+  const R vR = sR.u * m[0] + sR.v * m[1] + sR.w * m[2];
+  const R wR = sR.u * l[0] + sR.v * l[1] + sR.w * l[2];
+  const R ruL = sL.r * (uL), ruR = sR.r * (uR);
+  R c2YL = R(0.0), c2YR = R(0.0);
 #pragma unroll
   for (int iv = 0; iv < NM; ++iv) { c2YL += sL.c2Y[iv]; c2YR += sR.c2Y[iv]; }
-  const double cL = sqrt(c2YL), cR = sqrt(c2YR);
-  const double SR = maxr((uL + cL), (uR + cR));
-  const double SL = minr((uL - cL), (uR - cR));
-  double t[9];
+  const R cL = sqrt(c2YL), cR = sqrt(c2YR);
+  const R SR = maxr((uL + cL), (uR + cR));
+  const R SL = minr((uL - cL), (uR - cR));
+  R t[9];
   rot_tensor(t, sL.st, nv, m, l);
-  const double s11L = t[0], s12L = t[1], s13L = t[2];
+  const R s11L = t[0], s12L = t[1], s13L = t[2];
   rot_tensor(t, sR.st, nv, m, l);
-  const double s11R = t[0], s12R = t[1], s13R = t[2];
-  double u_S = ((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR)) /
+  const R s11R = t[0], s12R = t[1], s13R = t[2];
+  R u_S = ((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR)) /
                (ruL - ruR - ((SL)*sL.r) + ((SR)*sR.r));
   if (isnan(u_S) || isnan(SR) || isnan(SL)) return false;
-  if (bc == 4) u_S = 0.0;
+  if (bc == 4) u_S = R(0.0);
   int flag_ifs;
-  if (0.0 <= SL) flag_ifs = 0;
-  else if ((SL <= 0.0) && (0.0 <= u_S)) flag_ifs = 1;
-  else if ((u_S <= 0.0) && (0.0 <= SR)) flag_ifs = 2;
-  else flag_ifs = 3;   // SR <= 0.0
-  const Rec5<NM> &s = (flag_ifs <= 1) ? sL : sR;
-  Star5<NM> W;
-  double SK = 0.0;
+  if (R(0.0) <= SL) flag_ifs = 0;
+  else if ((SL <= R(0.0)) && (R(0.0) <= u_S)) flag_ifs = 1;
+  else if ((u_S <= R(0.0)) && (R(0.0) <= SR)) flag_ifs = 2;
+  else flag_ifs = 3;   // SR <= 0
+  // the side's record by value, selected field by field: a reference chosen at run
+  // time would place both records in scratch memory
+  Rec5<NM, R> s;
+  sel_rec<NM, R>(flag_ifs <= 1, sL, sR, s);
+  Star5<NM, R> W;
+  R SK = R(0.0);
   const bool star = (flag_ifs == 1 || flag_ifs == 2);
   if (star) {
     SK = (flag_ifs == 1) ? SL : SR;
-    star_region<NM>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
+    star_region<NM, R>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
 #pragma unroll
     for (int iv = 0; iv < L::NCONS; ++iv)
       if (isnan(W.Us[iv])) return false;
@@ -810,21 +859,25 @@ __device__ bool hllc_subface(const M5P &P, const double *F, int bc, const Rec5<N
     double x[L::NEQ];
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + cell];
-    Rec5<NM> cl, cr;
-    record<NM>(P, x, cl);
+    Rec5<NM, R> cl, cr;
+    {
+      Rec5<NM> c64;
+      record<NM>(P, x, c64);
+      narrow<NM, R>(c64, cl);
+    }
     cr = cl;
-    apply_bc5<NM>(1, F, cl, cr);
-    const double un = 0.5 * (cl.u + cr.u), vn = 0.5 * (cl.v + cr.v), wn = 0.5 * (cl.w + cr.w);
-    double a0[L::NEQ], a1[L::NEQ];
-    auto rows = [&](const Rec5<NM> &s, double *adv, int k) {
+    apply_bc5<NM, R>(1, F, cl, cr);
+    const R un = R(0.5) * (cl.u + cr.u), vn = R(0.5) * (cl.v + cr.v), wn = R(0.5) * (cl.w + cr.w);
+    R a0[L::NEQ], a1[L::NEQ];
+    auto rows = [&](const Rec5<NM, R> &s, R *adv, int k) {
       // the three components of row k of flag 4, k = 0, 1, 2
-      const double uu = (k == 0) ? un : (k == 1 ? vn : wn);
-      const double su = (k == 0) ? s.u : (k == 1 ? s.v : s.w);
+      const R uu = (k == 0) ? un : (k == 1 ? vn : wn);
+      const R su = (k == 0) ? s.u : (k == 1 ? s.v : s.w);
 #pragma unroll
       for (int v = 0; v < NM; ++v) adv[L::I0 + v] = su * s.ra[v];
-      adv[L::I1 + 0] = su * s.u * s.r + P.sg * s.st[0 + k];
-      adv[L::I1 + 1] = su * s.v * s.r + P.sg * s.st[3 + k];
-      adv[L::I1 + 2] = su * s.w * s.r + P.sg * s.st[6 + k];
+      adv[L::I1 + 0] = su * s.u * s.r + (R)P.sg * s.st[0 + k];
+      adv[L::I1 + 1] = su * s.v * s.r + (R)P.sg * s.st[3 + k];
+      adv[L::I1 + 2] = su * s.w * s.r + (R)P.sg * s.st[6 + k];
       adv[L::I2] = s.r * s.e * su - s.st[3 * k + 0] * s.u - s.st[3 * k + 1] * s.v - s.st[3 * k + 2] * s.w;
 #pragma unroll
       for (int v = 0; v < NM - 1; ++v) adv[L::I3 + v] = uu * s.avf[v];
@@ -833,44 +886,44 @@ __device__ bool hllc_subface(const M5P &P, const double *F, int bc, const Rec5<N
 #pragma unroll
       for (int q = 0; q < 9; ++q) adv[L::I5 + q] = uu * s.A[q];
     };
-    double acc[3][L::NEQ];
+    R acc[3][L::NEQ];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       rows(cl, a0, k);
       rows(cr, a1, k);
 #pragma unroll
       for (int v = 0; v < L::NEQ; v++) {
-        double sym = 0.0;
-        sym += 0.5 * a0[v];
-        sym += 0.5 * a1[v];
+        R sym = R(0.0);
+        sym += R(0.5) * a0[v];
+        sym += R(0.5) * a1[v];
         acc[k][v] = sym;
       }
     }
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++)
-      out[v] = ((acc[0][v] - 0.0) * nv[0] + (acc[1][v] - 0.0) * nv[1] + (acc[2][v] - 0.0) * nv[2]) + 0.0;
+      out[v] = ((acc[0][v] - R(0.0)) * nv[0] + (acc[1][v] - R(0.0)) * nv[1] + (acc[2][v] - R(0.0)) * nv[2]) + R(0.0);
     unc[0] = un; unc[1] = vn; unc[2] = wn;
     return true;
   }
-  double adv[L::NEQ];
-  phys_rows<NM>(s, nv, P.sg, adv);
+  R adv[L::NEQ];
+  phys_rows<NM, R>(s, nv, (R)P.sg, adv);
   bool ok = true;
 #pragma unroll
   for (int iv = 0; iv < L::NCONS; ++iv) {
-    const double vec_iv = (iv < NM) ? s.ra[iv] : (iv < NM + 3 ? s.vm[iv - NM] : s.vE);
-    const double h = star ? SK * (W.Us[iv] - vec_iv) : 0.0;
+    const R vec_iv = (iv < NM) ? s.ra[iv] : (iv < NM + 3 ? s.vm[iv - NM] : s.vE);
+    const R h = star ? SK * (W.Us[iv] - vec_iv) : R(0.0);
     if (isnan(h)) ok = false;
     out[iv] = adv[iv] + h;
   }
-  auto dot = [&](double f0, double f1, double f2) { return (f0 - 0.0) * nv[0] + (f1 - 0.0) * nv[1] + (f2 - 0.0) * nv[2]; };
+  auto dot = [&](R f0, R f1, R f2) { return (f0 - R(0.0)) * nv[0] + (f1 - R(0.0)) * nv[1] + (f2 - R(0.0)) * nv[2]; };
 #pragma unroll
   for (int v = 0; v < NM - 1; ++v)
-    out[L::I3 + v] = dot(W.unc[0] * W.avf_S[v], W.unc[1] * W.avf_S[v], W.unc[2] * W.avf_S[v]) + 0.0;
+    out[L::I3 + v] = dot(W.unc[0] * W.avf_S[v], W.unc[1] * W.avf_S[v], W.unc[2] * W.avf_S[v]) + R(0.0);
 #pragma unroll
   for (int v = 0; v < NM; ++v)
-    out[L::I4 + v] = dot(W.unc[0] * W.are_S[v], W.unc[1] * W.are_S[v], W.unc[2] * W.are_S[v]) + 0.0;
+    out[L::I4 + v] = dot(W.unc[0] * W.are_S[v], W.unc[1] * W.are_S[v], W.unc[2] * W.are_S[v]) + R(0.0);
 #pragma unroll
-  for (int k = 0; k < 9; k++) out[L::I5 + k] = dot(W.unc[0] * W.AS[k], W.unc[1] * W.AS[k], W.unc[2] * W.AS[k]) + 0.0;
+  for (int k = 0; k < 9; k++) out[L::I5 + k] = dot(W.unc[0] * W.AS[k], W.unc[1] * W.AS[k], W.unc[2] * W.AS[k]) + R(0.0);
   unc[0] = W.unc[0]; unc[1] = W.unc[1]; unc[2] = W.unc[2];
   return ok;
 }
@@ -946,7 +999,7 @@ __device__ double dfpe5(const M5P &P, const double *ar, const double *t0, const 
 template <int NM>
 struct RelaxIn {
   double ar[NM], are[NM], avf[NM], avf_init[NM], A[9], G[9], Gmag, rho, u, v, w, rE;
-  GPow gp;
+  GPow<> gp;
 };
 
 template <int NM>
@@ -965,7 +1018,7 @@ __device__ void relax_in(const M5P &P, const double *x, RelaxIn<NM> &R) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) R.A[k] = x[L::I5 + k];
   gmat(R.A, R.G, R.Gmag);
-  R.gp = has_solid(P) ? gpow(R.Gmag) : GPow{0.0, 0.0};
+  R.gp = has_solid(P) ? gpow(R.Gmag) : GPow<>{0.0, 0.0};
 }
 
 // the re-initialisation after PRELAX 1, 2, 3 (m5_relaxationvec.c:157-189 and
@@ -1472,8 +1525,10 @@ __device__ void dist5(const Geo &G, int kind, int lvl, double &dcf, double &dcc)
 // m5_faceinterp_V1 (m5_faceinterp.c:60-93) for one leaf per thread: MUSCL face
 // states with minmod (m5_reconctruct.c:93-190; g_limitedvars is 0 on the
 // alpha-rho-e block, m5_input.c:196-204), the VF limit and m5_relaxationvec
-// (PRELAX 3: left to k_m5_relax3_start / _finish)
-template <int NM>
+// (PRELAX 3: left to k_m5_relax3_start / _finish).  R = float (mixed precision)
+// forms the limited slopes in FP32 from the FP64 differences and adds them to
+// the FP64 cell state; the relaxation stays FP64.
+template <int NM, typename R>
 __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__restrict__ Gp,
                                                   const double *__restrict__ S, const int *__restrict__ nbr,
                                                   const uint32_t *__restrict__ info, const int *__restrict__ fine4,
@@ -1501,19 +1556,20 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
     double Fv[L::NEQ];
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) {
-      const double df = Xf[v] - Sc[v], db = Xb[v] - Sc[v];
-      const double lim = (v >= L::I4 && v < L::I5) ? 0.0 : 1.0;
-      double g = 0.0;
-      if (kf != FV_FK_BOUNDARY && fabs(df) > 1.0e-10) {
-        const double product = (-db / dm) * (df / dp);
-        if (product > 0.0) {
-          if (fabs(-db / dm) > fabs(df / dp)) g = lim * (df / dp);
-          else g = lim * (-db / dm);
+      const R df = (R)(Xf[v] - Sc[v]), db = (R)(Xb[v] - Sc[v]);
+      const R lim = (v >= L::I4 && v < L::I5) ? R(0.0) : R(1.0);
+      const R dpr = (R)dp, dmr = (R)dm;
+      R g = R(0.0);
+      if (kf != FV_FK_BOUNDARY && fabs(df) > R(1.0e-10)) {
+        const R product = (-db / dmr) * (df / dpr);
+        if (product > R(0.0)) {
+          if (fabs(-db / dmr) > fabs(df / dpr)) g = lim * (df / dpr);
+          else g = lim * (-db / dmr);
         } else {
-          g = 0.0;
+          g = R(0.0);
         }
       }
-      Fv[v] = Sc[v] + dcf * g;
+      Fv[v] = Sc[v] + dcf * (double)g;
     }
     // VF limit (m5_reconctruct.c:142-170)
     double avf[NM], avfsum = 0.0;
@@ -1554,27 +1610,29 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
 
 // the record a face flux reads: the face state (ORDER 2, m5_facercnstr) or
 // the cell state (ORDER 1, m5_facevalues) of leaf j across its face fj
-template <int NM>
+template <int NM, typename R>
 __device__ void side_record(const M5P &P, const double *__restrict__ S, const double *__restrict__ SF, int n, int j,
-                            int fj, Rec5<NM> &r) {
+                            int fj, Rec5<NM, R> &r) {
   using L = L5<NM>;
   double x[L::NEQ];
+  Rec5<NM> r64;
   if (P.order == 2) {
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) x[v] = SF[((size_t)fj * L::NEQ + v) * n + j];
-    record<NM, true>(P, x, r, S[(size_t)L::I3 * n + j]);
+    record<NM, true>(P, x, r64, S[(size_t)L::I3 * n + j]);
   } else {
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + j];
-    record<NM>(P, x, r);
+    record<NM>(P, x, r64);
   }
+  narrow<NM, R>(r64, r);
 }
 
 // m5_residual.c:7-238 for one leaf per thread.  A mirrored face (the earlier
 // leaf's flux_flag protocol, a face kind of the pack) recomputes the earlier
 // leaf's evaluation of that face and negates it, as the reference reuses it.
 // gfm bit f: the face used the GFM flux.
-template <int NM>
+template <int NM, typename R>
 __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__restrict__ Gp,
                                                      const double *__restrict__ S, const double *__restrict__ SF,
                                                      const int *__restrict__ nbr, const uint32_t *__restrict__ info,
@@ -1600,7 +1658,7 @@ __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__
     for (int k = 0; k < 9; ++k) Am[k] = sc.A[k];
     double Gm[9], Gmag;
     gmat(sc.A, Gm, Gmag);
-    const GPow gps = has_solid(P) ? gpow(Gmag) : GPow{0.0, 0.0};
+    const GPow<> gps = has_solid(P) ? gpow(Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < NM; ++k) stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mush[k]);
   }
@@ -1621,54 +1679,60 @@ __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) ff[v] = 0.0;
     if (kind == FV_FK_MIRROR) {
-      Rec5<NM> sL, sR;
-      side_record<NM>(P, S, SF, n, nb, fo, sL);
-      side_record<NM>(P, S, SF, n, i, f, sR);
-      double out[L::NEQ], un[3];
-      const bool g = sL.phi0 * sR.phi0 < 0.0;
-      ok &= g ? gfm_subface<NM>(P, G.frame[fo], sL, sR, out, un)
-              : hllc_subface<NM>(P, G.frame[fo], 0, sL, sR, S, n, nb, out, un);
+      Rec5<NM, R> sL, sR;
+      side_record<NM, R>(P, S, SF, n, nb, fo, sL);
+      side_record<NM, R>(P, S, SF, n, i, f, sR);
+      R out[L::NEQ], un[3], fr[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[fo][q];
+      const bool g = sL.phi0 * sR.phi0 < R(0.0);
+      ok &= g ? gfm_subface<NM, R>(P, fr, sL, sR, out, un)
+              : hllc_subface<NM, R>(P, fr, 0, sL, sR, S, n, nb, out, un);
       const double A = G.area[lvl];
 #pragma unroll
       for (int v = 0; v < L::NEQ; v++) {
         double t = 0.0;
-        t += (A) * out[v];
+        t += (A) * (double)out[v];
         ff[v] = -t;
       }
       // the star velocity is a global vector: the published code negates it with the flux
       const double us = P.fix ? 1.0 : -1.0;
-      uf0 = us * (un[0] / ((double)1)); uf1 = us * (un[1] / ((double)1)); uf2 = us * (un[2] / ((double)1));
+      uf0 = us * ((double)un[0] / ((double)1)); uf1 = us * ((double)un[1] / ((double)1));
+      uf2 = us * ((double)un[2] / ((double)1));
       Area_f += A;
     } else {
       const int Nneig = (kind == FV_FK_FINE) ? 4 : 1;
       const double A = (kind == FV_FK_FINE) ? G.area[lvl + 1] : G.area[lvl];
       for (int k = 0; k < Nneig; ++k) Area_f += A;
-      Rec5<NM> sL;
-      side_record<NM>(P, S, SF, n, i, f, sL);
+      R fr[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[f][q];
+      Rec5<NM, R> sL;
+      side_record<NM, R>(P, S, SF, n, i, f, sL);
       for (int k = 0; k < Nneig; ++k) {
-        Rec5<NM> sR;
+        Rec5<NM, R> sR;
         int bcode = 0;
         if (kind == FV_FK_BOUNDARY) {
           bcode = -nb;
-          side_record<NM>(P, S, SF, n, i, f, sR);
-          apply_bc5<NM>(bcode, G.frame[f], sL, sR);
+          side_record<NM, R>(P, S, SF, n, i, f, sR);
+          apply_bc5<NM, R>(bcode, fr, sL, sR);
         } else {
           const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
-          side_record<NM>(P, S, SF, n, j, fo, sR);
+          side_record<NM, R>(P, S, SF, n, j, fo, sR);
         }
-        double out[L::NEQ], un[3];
-        if (sL.phi0 * sR.phi0 < 0.0) {
+        R out[L::NEQ], un[3];
+        if (sL.phi0 * sR.phi0 < R(0.0)) {
           gbits |= (uint8_t)(1u << f);
-          Rec5<NM> gL = sL;   // the ghost rewrites only this sub-face's copies
-          ok &= gfm_subface<NM>(P, G.frame[f], gL, sR, out, un);
+          Rec5<NM, R> gL = sL;   // the ghost rewrites only this sub-face's copies
+          ok &= gfm_subface<NM, R>(P, fr, gL, sR, out, un);
         } else {
-          ok &= hllc_subface<NM>(P, G.frame[f], bcode, sL, sR, S, n, i, out, un);
+          ok &= hllc_subface<NM, R>(P, fr, bcode, sL, sR, S, n, i, out, un);
         }
 #pragma unroll
-        for (int v = 0; v < L::NEQ; v++) ff[v] += (A) * out[v];
-        uf0 = un[0] / ((double)Nneig);   // overwritten per sub-face (m5_residual.c:163-165)
-        uf1 = un[1] / ((double)Nneig);
-        uf2 = un[2] / ((double)Nneig);
+        for (int v = 0; v < L::NEQ; v++) ff[v] += (A) * (double)out[v];
+        uf0 = (double)un[0] / ((double)Nneig);   // overwritten per sub-face (m5_residual.c:163-165)
+        uf1 = (double)un[1] / ((double)Nneig);
+        uf2 = (double)un[2] / ((double)Nneig);
       }
     }
 #pragma unroll
@@ -1720,6 +1784,159 @@ __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__
     rhs[v5 + 6] += Area_f * ((C1 * u_nc * nx + C1 * v_nc * ny) + A1 * u_nc * nz - B1 * v_nc * nz);
     rhs[v5 + 7] += Area_f * ((C2 * u_nc * nx + C2 * v_nc * ny) + A2 * u_nc * nz - B2 * v_nc * nz);
     rhs[v5 + 8] += Area_f * ((C3 * u_nc * nx + C3 * v_nc * ny) + A3 * u_nc * nz - B3 * v_nc * nz);
+  }
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) {
+    ok &= !isnan(rhs[v]);
+    RHS[(size_t)v * n + i] = rhs[v];
+  }
+#pragma unroll
+  for (int f = 0; f < 6; f++) gfm[6 * i + f] = (gbits >> f) & 1u;
+  if (!ok) atomicOr(flag, 1);
+}
+
+// The mixed-precision residual: k_m5_residual's face loop with FP32 face
+// records, HLLC and GFM flux, the area-weighted face fluxes added in FP64
+// straight into the residual, and the non-conservative terms formed once after
+// the loop.  Every source term of m5_residual.c:176-214 (and of the consistent
+// mode) is linear in the face's u* n^T, so they follow from
+// T_ij = sum_f Area_f n_i u*_j and the cell record, which is built after the
+// loop and holds no registers during it.
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_residual_mx(int n, M5P P, const Geo *__restrict__ Gp,
+                                                        const double *__restrict__ S, const double *__restrict__ SF,
+                                                        const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+                                                        const int *__restrict__ fine4, double *__restrict__ RHS,
+                                                        uint8_t *__restrict__ gfm, int *__restrict__ flag) {
+  using L = L5<NM>;
+  using R = float;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Geo &G = *Gp;
+  const uint32_t inf = info[i];
+  const int lvl = fvk::cell_level(inf);
+  double rhs[L::NEQ], T[9];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) rhs[v] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 9; q++) T[q] = 0.0;
+  uint8_t gbits = 0;
+  bool ok = true;
+#pragma unroll 1
+  for (int f = 0; f < 6; f++) {
+    const int kind = fvk::face_kind(inf, f);
+    const int nb = nbr[(size_t)f * n + i];
+    const int fo = FCOPP5[f];
+    double uf0 = 0.0, uf1 = 0.0, uf2 = 0.0, Area_f = 0.0;
+    if (kind == FV_FK_MIRROR) {
+      Rec5<NM, R> sL, sR;
+      side_record<NM, R>(P, S, SF, n, nb, fo, sL);
+      side_record<NM, R>(P, S, SF, n, i, f, sR);
+      R out[L::NEQ], un[3], fr[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[fo][q];
+      const bool g = sL.phi0 * sR.phi0 < R(0.0);
+      ok &= g ? gfm_subface<NM, R>(P, fr, sL, sR, out, un)
+              : hllc_subface<NM, R>(P, fr, 0, sL, sR, S, n, nb, out, un);
+      const double A = G.area[lvl];
+#pragma unroll
+      for (int v = 0; v < L::NEQ; v++) rhs[v] -= A * (double)out[v];
+      const double us = P.fix ? 1.0 : -1.0;   // the published code negates the reused vector
+      uf0 = us * (double)un[0]; uf1 = us * (double)un[1]; uf2 = us * (double)un[2];
+      Area_f = A;
+    } else {
+      const int Nneig = (kind == FV_FK_FINE) ? 4 : 1;
+      const double A = (kind == FV_FK_FINE) ? G.area[lvl + 1] : G.area[lvl];
+      Area_f = (double)Nneig * A;
+      R fr[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[f][q];
+      Rec5<NM, R> sL;
+      side_record<NM, R>(P, S, SF, n, i, f, sL);
+      for (int k = 0; k < Nneig; ++k) {
+        Rec5<NM, R> sR;
+        int bcode = 0;
+        if (kind == FV_FK_BOUNDARY) {
+          bcode = -nb;
+          side_record<NM, R>(P, S, SF, n, i, f, sR);
+          apply_bc5<NM, R>(bcode, fr, sL, sR);
+        } else {
+          const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
+          side_record<NM, R>(P, S, SF, n, j, fo, sR);
+        }
+        R out[L::NEQ], un[3];
+        if (sL.phi0 * sR.phi0 < R(0.0)) {
+          gbits |= (uint8_t)(1u << f);
+          Rec5<NM, R> gL = sL;
+          ok &= gfm_subface<NM, R>(P, fr, gL, sR, out, un);
+        } else {
+          ok &= hllc_subface<NM, R>(P, fr, bcode, sL, sR, S, n, i, out, un);
+        }
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) rhs[v] += A * (double)out[v];
+        uf0 = (double)un[0] / (double)Nneig;   // the last sub-face's velocity (m5_residual.c:163-165)
+        uf1 = (double)un[1] / (double)Nneig;
+        uf2 = (double)un[2] / (double)Nneig;
+      }
+    }
+    const double nn[3] = {G.frame[f][0], G.frame[f][1], G.frame[f][2]}, uu[3] = {uf0, uf1, uf2};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) T[3 * a + b] += Area_f * nn[a] * uu[b];
+  }
+  // the cell record's source coefficients (m5_residual.c:31-59)
+  double src_avf[NM - 1], src_st[NM][9], Am[9];
+  {
+    double x[L::NEQ];
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + i];
+    Rec5<NM> sc;
+    record<NM>(P, x, sc);
+#pragma unroll
+    for (int k = 0; k < NM - 1; ++k) src_avf[k] = sc.avf[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Am[k] = sc.A[k];
+    double Gm[9], Gmag;
+    gmat(sc.A, Gm, Gmag);
+    const GPow<> gps = has_solid(P) ? gpow(Gmag) : GPow<>{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NM; ++k) stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mush[k]);
+  }
+  const double tr = T[0] + T[4] + T[8];   // sum_f Area_f (u*.n)
+  const double sgn = P.fix ? -1.0 : 1.0;
+#pragma unroll
+  for (int k = 0; k < NM - 1; ++k) rhs[L::I3 + k] += sgn * src_avf[k] * tr;
+#pragma unroll
+  for (int k = 0; k < NM; ++k) {
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc += src_st[k][q] * T[q];   // s[3a+b] n_a u_b
+    rhs[L::I4 + k] += sgn * acc;
+  }
+  if (P.fix) {
+    // A_ij: n_i (A_kj u_k) - A_ij (u.n)
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        rhs[L::I5 + 3 * a + j] += Am[j] * T[3 * a + 0] + Am[3 + j] * T[3 * a + 1] + Am[6 + j] * T[3 * a + 2] -
+                                  Am[3 * a + j] * tr;
+  } else {
+    // m5_residual.c:197-209 with T_ab = sum_f Area_f n_a u_b
+    const double A1 = Am[0], B1 = Am[3], C1 = Am[6];
+    const double A2 = Am[1], B2 = Am[4], C2 = Am[7];
+    const double A3 = Am[2], B3 = Am[5], C3 = Am[8];
+    const int v5 = L::I5;
+    rhs[v5 + 0] += A1 * (T[4] + T[8]) + B1 * T[1] - C1 * T[2];
+    rhs[v5 + 1] += A2 * (T[4] + T[8]) + B2 * T[1] - C2 * T[2];
+    rhs[v5 + 2] += A3 * (T[4] + T[8]) + B3 * T[1] - C3 * T[2];
+    rhs[v5 + 3] += B1 * (T[0] + T[8]) + A1 * T[3] - C1 * T[5];
+    rhs[v5 + 4] += B2 * (T[0] + T[8]) + A2 * T[3] - C2 * T[5];
+    rhs[v5 + 5] += B3 * (T[0] + T[8]) + A3 * T[3] - C3 * T[5];
+    rhs[v5 + 6] += C1 * (T[0] + T[4]) + A1 * T[6] - B1 * T[7];
+    rhs[v5 + 7] += C2 * (T[0] + T[4]) + A2 * T[6] - B2 * T[7];
+    rhs[v5 + 8] += C3 * (T[0] + T[4]) + A3 * T[6] - B3 * T[7];
   }
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) {
@@ -1972,6 +2189,7 @@ static hipError_t dalloc(T **p, size_t count) {
 struct fv_m5_dev {
   M5P P{};
   int n = 0, nfine = 0, neq = 0, rk_steps = 3;
+  int prec = FV_PREC_FP64;    // FV_PREC_MIXED: FP32 face records, HLLC and GFM flux (fv_m5_dev_set_precision)
   hipStream_t stream = nullptr;
   Geo *G = nullptr;
   int *nbr = nullptr, *fine4 = nullptr;
@@ -2034,8 +2252,12 @@ dim3 grid_of(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
 template <int NM>
 int m5_faceinterp_t(fv_m5_dev *c) {
-  hipLaunchKernelGGL(k_m5_recon<NM>, grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S, c->nbr,
-                     c->info, c->fine4, c->SF, c->flag);
+  if (c->prec == FV_PREC_MIXED)
+    hipLaunchKernelGGL((k_m5_recon<NM, float>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
+                       c->nbr, c->info, c->fine4, c->SF, c->flag);
+  else
+    hipLaunchKernelGGL((k_m5_recon<NM, double>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
+                       c->nbr, c->info, c->fine4, c->SF, c->flag);
   HIPCHK(hipGetLastError());
   if (c->P.prelax == 3) return m5_relax3<NM>(c, c->SF, 6L * c->n, nullptr);
   return FV_OK;
@@ -2043,8 +2265,12 @@ int m5_faceinterp_t(fv_m5_dev *c) {
 
 template <int NM>
 int m5_residual_t(fv_m5_dev *c) {
-  hipLaunchKernelGGL(k_m5_residual<NM>, grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S, c->SF,
-                     c->nbr, c->info, c->fine4, c->RHS, c->gfm, c->flag);
+  if (c->prec == FV_PREC_MIXED)
+    hipLaunchKernelGGL(k_m5_residual_mx<NM>, grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S, c->SF,
+                       c->nbr, c->info, c->fine4, c->RHS, c->gfm, c->flag);
+  else
+    hipLaunchKernelGGL((k_m5_residual<NM, double>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
+                       c->SF, c->nbr, c->info, c->fine4, c->RHS, c->gfm, c->flag);
   HIPCHK(hipGetLastError());
   return FV_OK;
 }
@@ -2221,6 +2447,14 @@ extern "C" int fv_m5_dev_advance(fv_m5_dev *c, int nsteps) {
     if (rc != FV_OK) return rc;
     if ((rc = m5_check(c)) != FV_OK) return rc;
   }
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_set_precision(fv_m5_dev *c, int prec) {
+  if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (prec != FV_PREC_FP64 && prec != FV_PREC_EXACT && prec != FV_PREC_MIXED)
+    return fv_set_err(FV_ERR_ARG, "model 5: precision must be FV_PREC_FP64, FV_PREC_EXACT or FV_PREC_MIXED");
+  c->prec = (prec == FV_PREC_MIXED) ? FV_PREC_MIXED : FV_PREC_FP64;
   return FV_OK;
 }
 

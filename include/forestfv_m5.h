@@ -20,6 +20,7 @@
 
 #include <stdint.h>
 #include "forestfv.h"
+#include "forestfv_dev.h"   /* FV_PREC_* */
 
 #ifdef __cplusplus
 extern "C" {
@@ -112,6 +113,13 @@ int fv_m5_dev_begin_step(fv_m5_dev *c);
 int fv_m5_dev_stage(fv_m5_dev *c, int irk);
 /* nsteps steps of stages 1..rk_steps (loop.c:21) */
 int fv_m5_dev_advance(fv_m5_dev *c, int nsteps);
+/* Precision of the stage (no counterpart in the reference, which is FP64):
+ * FV_PREC_FP64 (default; FV_PREC_EXACT is the same, the model-5 kernels keep
+ * the reference's operation order) or FV_PREC_MIXED: FP32 limited slopes in
+ * the MUSCL face states, FP32 face records, HLLC, star states and GFM ghost
+ * flux; the records are formed in FP64 (pressures carry p_inf up to 2e10 Pa),
+ * and the residual, RK accumulation and relaxation stay FP64. */
+int fv_m5_dev_set_precision(fv_m5_dev *c, int prec);
 /* the stage's pieces alone: face states of the current S (m5_faceinterp_V1),
  * and the residual of S and those face states (m5_residual) */
 int fv_m5_dev_faceinterp(fv_m5_dev *c);
