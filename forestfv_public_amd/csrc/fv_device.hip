@@ -993,9 +993,13 @@ FV_DEV int br_of_block(int nb) {
   return (int)b < nb ? (int)b : -1;
 }
 
-template <int NM>
+template <int NM, bool MIXED = false>
 struct BrickLds {
   using L = Lay<NM>;
+  // the face fluxes of the tasks: A F in FP64, or in mixed precision the FP32
+  // HLLC output F itself (A F is formed in FP64 when a cell gathers it, the
+  // same product the FP64 store would hold), which halves this array
+  using FT = typename std::conditional<MIXED, float, double>::type;
   // staged cell states [v][c], avf-limited as stencil values and the cell's
   // own reconstruction base use them: own cells c < 64, the neighbour across
   // surface slot s at 64 + s (that neighbour's stencil value beyond it stays in
@@ -1005,7 +1009,8 @@ struct BrickLds {
   double sn[L::NEQ * 64];               // S^n of the own cells (RK stages after the first)
   double araw[L::N3 * 64];              // the own cells' volume fractions before the limit
   double src[(L::N3 + NM) * 64];        // the own cells' source coefficients: alpha_k, then -alpha_k p_k
-  double ff[(L::NEQ + 1) * BR_TASKS];   // face fluxes A F and u*.n per face task
+  FT ff[L::NEQ * BR_TASKS];             // face fluxes per face task
+  double fu[BR_TASKS];                  // u*.n per face task
   uint32_t info[64];
   uint32_t infoh[BR_SLOTS];             // info word of the neighbour across each slot
   int nb[BR_SLOTS];                     // that neighbour (leaf index, or -BC code)
@@ -1014,10 +1019,10 @@ struct BrickLds {
   double idmh[BR_SLOTS];                // 1/dcc of the slot neighbour's face beyond it (any kind)
 };
 
-template <int NM>
-FV_DEV void br_state(const BrickLds<NM> &B, int c, double *X) {
+template <int NM, bool MIXED>
+FV_DEV void br_state(const BrickLds<NM, MIXED> &B, int c, double *X) {
 #pragma unroll
-  for (int v = 0; v < Lay<NM>::NEQ; v++) X[v] = B.st[v * BrickLds<NM>::NC + c];
+  for (int v = 0; v < Lay<NM>::NEQ; v++) X[v] = B.st[v * BrickLds<NM, MIXED>::NC + c];
 }
 
 // face state (FP64 vector, relaxed) or FP32 record of a same-level cell at a
@@ -1027,19 +1032,19 @@ FV_DEV void br_state(const BrickLds<NM> &B, int c, double *X) {
 // with the brick's one level (dcf, 1/dcc), and zero slope when either face is
 // a domain boundary (bnd)
 template <int NM, bool MIXED, bool XR>
-FV_DEV void br_face_state(const Params &P, const fvf::Mat<NM, double> &Md, double a, const BrickLds<NM> &B, int c,
+FV_DEV void br_face_state(const Params &P, const fvf::Mat<NM, double> &Md, double a, const BrickLds<NM, MIXED> &B, int c,
                           int cg, int co, const double *Xo, bool xo_reg, int limiter, bool bnd, double dcf,
                           double idp, double idm, double *F, float *rec) {
   using L = Lay<NM>;
   using R = typename std::conditional<MIXED, float, double>::type;
-  br_state<NM>(B, c, F);
+  br_state<NM, MIXED>(B, c, F);
   if (!bnd) {
     const R ip = (R)idp, im = (R)idm;
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) {
       const double sc = F[v];
-      const double xs = B.st[v * BrickLds<NM>::NC + co];
-      const double xg = B.st[v * BrickLds<NM>::NC + cg], xo = xo_reg ? Xo[v] : xs;
+      const double xs = B.st[v * BrickLds<NM, MIXED>::NC + co];
+      const double xg = B.st[v * BrickLds<NM, MIXED>::NC + cg], xo = xo_reg ? Xo[v] : xs;
       const R g = fvf::slope<R>(limiter, xg - sc, xo - sc, ip, im);
       F[v] = sc + dcf * (double)g;
     }
@@ -1069,7 +1074,7 @@ FV_DEV void br_write_sf(int n, int i, int f, const double *F, const float *rec, 
 }
 
 template <int NM, bool MIXED, bool XR>
-__global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, const Geo *__restrict__ Gp, int cs,
+__global__ __launch_bounds__(BR_THREADS, (MIXED && !XR) ? 5 : 4) void k_stage_brick(int n, Params P, const Geo *__restrict__ Gp, int cs,
                                                                const int *__restrict__ bricks, int nbricks,
                                                                const double *__restrict__ S, double *__restrict__ Sout,
                                                                double *__restrict__ SN, const int *__restrict__ nbr,
@@ -1081,7 +1086,7 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
                                                                int *__restrict__ nanflag) {
   using L = Lay<NM>;
   using R = typename std::conditional<MIXED, float, double>::type;
-  using LD = BrickLds<NM>;
+  using LD = BrickLds<NM, MIXED>;
   constexpr int NC = LD::NC;
   __shared__ LD B;
   const int br = br_of_block(nbricks);
@@ -1242,8 +1247,11 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
     R out[L::NEQ], un, vn, wn;
     fvf::hllc<NM, R>(M, P.fix, fr, bcode, sL, sR, out, un, vn, wn);
 #pragma unroll
-    for (int v = 0; v < L::NEQ; v++) B.ff[v * BR_TASKS + tid] = A * (double)out[v];
-    B.ff[L::NEQ * BR_TASKS + tid] =
+    for (int v = 0; v < L::NEQ; v++) {
+      if constexpr (MIXED) B.ff[v * BR_TASKS + tid] = out[v];
+      else B.ff[v * BR_TASKS + tid] = A * (double)out[v];
+    }
+    B.fu[tid] =
         (double)un * G.frame[fL][0] + (double)vn * G.frame[fL][1] + (double)wn * G.frame[fL][2];
   }
   __syncthreads();
@@ -1253,7 +1261,7 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
   const int l = tid, i = b0 + l;
   const uint32_t inf = B.info[l];
   double Sc[L::NEQ];
-  br_state<NM>(B, l, Sc);
+  br_state<NM, MIXED>(B, l, Sc);
 #pragma unroll
   for (int k = 0; k < L::N3; k++) Sc[L::I3 + k] = B.araw[k * 64 + l];   // the raw state
   double src_avf[NM], src_st[NM];
@@ -1278,11 +1286,14 @@ __global__ __launch_bounds__(BR_THREADS, 4) void k_stage_brick(int n, Params P, 
     else { t = BR_SLOTS + 48 * ax + 16 * (along - 1) + k; right = true; }
     const double sgn = right ? -1.0 : 1.0;
 #pragma unroll
-    for (int v = 0; v < L::NEQ; v++) RHS[v] += sgn * B.ff[v * BR_TASKS + t];
+    for (int v = 0; v < L::NEQ; v++) {
+      if constexpr (MIXED) RHS[v] += sgn * (A * (double)B.ff[v * BR_TASKS + t]);
+      else RHS[v] += sgn * B.ff[v * BR_TASKS + t];
+    }
     // u*.n of this leaf's face: the star velocity is shared by both sides; the
     // published code negates the reused global-frame vector on the mirrored side
     // (m4_residual.c:72-74), which FV_FIX_NONCONS leaves out
-    const double un = B.ff[L::NEQ * BR_TASKS + t];
+    const double un = B.fu[t];
     double und;
     if (right) und = (!fixnc && kind == FV_FK_MIRROR) ? un : -un;
     else und = (!fixnc && kind == FV_FK_MIRROR) ? -un : un;
