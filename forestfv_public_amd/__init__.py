@@ -266,6 +266,11 @@ def lib():
             "fv_m5_dev_get_stage_arrays": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_uint8)]),
             "fv_m5_dev_time_steps": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
             "fv_m5_dev_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+            "fv_m5_dev_set_dt": (C.c_int, [C.c_void_p, C.c_double]),
+            "fv_m5_dev_cfl_dt": (C.c_int, [C.c_void_p, C.c_double, P(C.c_double)]),
+            "fv_m5_init_slab_bubble": (C.c_int, [P(fv_mesh), P(fv_m5_params), P(C.c_double), C.c_double,
+                                                 P(C.c_double), C.c_double, C.c_double, C.c_double, C.c_double,
+                                                 P(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -347,6 +352,8 @@ def m5_neq(nmat):
 
 # fv_m5_params.fix: the consistent model-5 update (FV_M5_FIX_SIGNS of forestfv_m5.h)
 M5_FIX_SIGNS = 1
+# fv_m5_params.fix bit: the volume-fraction-weighted elastic mixture (FV_M5_FIX_MIXTURE)
+M5_FIX_MIXTURE = 2
 
 
 def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush=(2.6e10, 0.0, 0.0), amin=1e-6,
@@ -445,6 +452,16 @@ class M5Device:
 
     def stage(self, irk):
         _check(lib().fv_m5_dev_stage(self.h, irk), "fv_m5_dev_stage")
+
+    def set_dt(self, dt):
+        _check(lib().fv_m5_dev_set_dt(self.h, dt), "fv_m5_dev_set_dt")
+        self.params.dt = dt
+
+    def cfl_dt(self, cfl):
+        """cfl * min h / (max|u_i| + c) over the device state, c the mixture's longitudinal speed."""
+        d = C.c_double()
+        _check(lib().fv_m5_dev_cfl_dt(self.h, cfl, C.byref(d)), "fv_m5_dev_cfl_dt")
+        return d.value
 
     def advance(self, nsteps):
         _check(lib().fv_m5_dev_advance(self.h, nsteps), "fv_m5_dev_advance")
@@ -587,6 +604,16 @@ class Mesh:
         r = np.ascontiguousarray(rho_init, dtype=np.float64)
         p = np.ascontiguousarray(p_init, dtype=np.float64)
         _check(lib().fv_init_split_x(self._p, C.byref(params), _dp(r), _dp(p), xs, _dp(S)), "fv_init_split_x")
+        return S
+
+    def init_m5_slab_bubble(self, params, rho_init, z_slab, centre, radius, p_liquid, p_gas, smear=2.0):
+        """Model-5 state (materials solid, liquid, gas): a sharp solid slab below z_slab, a tanh
+        gas bubble in the liquid, A = I in the solid and 0 in the fluids, at rest."""
+        S = np.zeros((self.n, m5_neq(params.nmat)))
+        r = np.ascontiguousarray(rho_init, dtype=np.float64)
+        c = np.ascontiguousarray(centre, dtype=np.float64)
+        _check(lib().fv_m5_init_slab_bubble(self._p, C.byref(params), _dp(r), z_slab, _dp(c), radius, p_liquid,
+                                            p_gas, smear, _dp(S)), "fv_m5_init_slab_bubble")
         return S
 
     def init_bubble(self, params, rho_init, p_init, centre, radius, smear=2.0, seed=1234, perturb=0.0):

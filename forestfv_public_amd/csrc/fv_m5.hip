@@ -42,6 +42,7 @@ struct M5P {
   int prelax, order;
   double yield;   // von Mises yield stress of perfect plasticity, 0 = elastic
   int fix;        // 1: the consistent update (FV_M5_FIX_SIGNS), 0: the published one
+  int mix;        // 1: elastic energy and shear stress weighted by the volume fraction (FV_M5_FIX_MIXTURE)
   double dtu;     // signed step of the update: S = SN + dtu RHS / V, +dt published, -dt consistent
   double sg;      // sign of sigma in the momentum flux: +1 published (rho u u + sigma), -1 consistent
 };
@@ -270,13 +271,17 @@ __device__ void record(const M5P &P, const double *x, Rec5<NM> &r, double cavf0 
   const GPow<> gp = has_solid(P) ? gpow(Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < NM; ++i) {
-    const double are_elast = (P.mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, gp, P.mush[i]) : 0.0;
+    double are_elast = (P.mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, gp, P.mush[i]) : 0.0;
     const double g = P.gamma[i], pi = P.pinf[i];
     const double den = FACE ? r.avf[i] : avf_temp[i];
+    // mixture mode: phase i holds alpha_i of the elastic energy and of the shear stress
+    const double mu_i = P.mix ? r.avf[i] * P.mush[i] : P.mush[i];
+    if (P.mix) are_elast *= den;
     const double press = ((g - 1.0) * (r.are[i] - are_elast) / den) - g * pi;
     r.c2Y[i] = (den / r.r) * g * (press + pi);
+    if (P.mix) r.c2Y[i] += (4.0 / 3.0) * mu_i / r.r;   // longitudinal speed of the mixture
     r.ph[i] = press;
-    add_stress_v2(r.st, r.avf[i], r.ph[i], G, Gmag, gp, P.mush[i]);
+    add_stress_v2(r.st, r.avf[i], r.ph[i], G, Gmag, gp, mu_i);
   }
 }
 
@@ -378,7 +383,8 @@ __device__ void star_region(const M5P &P, const R *F, const Rec5<NM, R> &sL, con
     const R rm_S = (s.ra[v] / s.avf[v]) * (uK - SK) / (u_S - SK);
     const R p_k = P.fix ? s.ph[v] : -s.ph[v];   // published: -p_hydro
     const R p_S = (p_k + pi) * (((g - R(1.0)) * rm + (g + R(1.0)) * rm_S) / ((g - R(1.0)) * rm_S + (g + R(1.0)) * rm)) - pi;
-    const R are_el = elastic_energy_v1(G, (R)P.mush[v]);
+    R are_el = elastic_energy_v1(G, (R)P.mush[v]);
+    if (P.mix) are_el *= s.avf[v];   // mixture mode: alpha_v of the elastic energy
     const R e_el = are_el / (s.avf[v] * rm_S);
     e_S[v] = (p_S + g * pi) / (rm_S * (P.fix ? (g - R(1.0)) : (g + R(1.0)))) + e_el;   // published: gamma + 1
   }
@@ -533,6 +539,58 @@ __device__ void fallback_and_freeze(const M5P &P, int n, int i, const double *__
   if (P.zerou[0] == 1) { keep(L::I1 + 0); keep(L::I5 + 1); keep(L::I5 + 2); keep(L::I5 + 3); keep(L::I5 + 6); }
   if (P.zerou[1] == 1) { keep(L::I1 + 1); keep(L::I5 + 1); keep(L::I5 + 3); keep(L::I5 + 5); keep(L::I5 + 7); }
   if (P.zerou[2] == 1) { keep(L::I1 + 2); keep(L::I5 + 2); keep(L::I5 + 5); keep(L::I5 + 6); keep(L::I5 + 7); }
+}
+
+// CFL step of model 5 (new, the model-4 k_cfl's form): per-block minimum of
+// h / (max|u_i| + c) with c^2 = sum_k c2Y_k + (4/3) sum_k alpha_k mu_k / rho, the
+// longitudinal speed of the mixture (c2Y_k carries the shear part already in
+// the mixture mode), reduced with wavefront shuffles
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_cfl(int n, M5P P, const Geo *__restrict__ Gp,
+                                                const uint32_t *__restrict__ info, const double *__restrict__ S,
+                                                double *__restrict__ part) {
+  using L = L5<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d = INFINITY;
+  if (i < n) {
+    double x[L::NEQ];
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + i];
+    Rec5<NM> r;
+    record<NM>(P, x, r);
+    double c2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) c2 += r.c2Y[k];
+    if (!P.mix) {
+      double mu = 0.0;
+#pragma unroll
+      for (int k = 0; k < NM; ++k) mu += r.avf[k] * P.mush[k];
+      c2 += (4.0 / 3.0) * mu / r.r;
+    }
+    const double lam = maxr(maxr(fabs(r.u), fabs(r.v)), fabs(r.w)) + sqrt(c2);
+    d = Gp->h[fvk::cell_level(info[i])] / lam;
+  }
+  for (int off = 32; off > 0; off >>= 1) d = fmin(d, __shfl_xor(d, off, 64));
+  __shared__ double wmin[4];
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmin(fmin(wmin[0], wmin[1]), fmin(wmin[2], wmin[3]));
+}
+
+// minimum of m partial minima, times scale, into *out (one block of 1024 lanes)
+__global__ __launch_bounds__(1024) void k_m5_min_final(int m, const double *__restrict__ part,
+                                                       double *__restrict__ out, double scale) {
+  double d = INFINITY;
+  for (int k = threadIdx.x; k < m; k += blockDim.x) d = fmin(d, part[k]);
+  for (int off = 32; off > 0; off >>= 1) d = fmin(d, __shfl_xor(d, off, 64));
+  __shared__ double wmin[16];
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = wmin[0];
+    for (int w = 1; w < 16; w++) r = fmin(r, wmin[w]);
+    *out = scale * r;
+  }
 }
 
 template <int NM>
@@ -1028,9 +1086,19 @@ __device__ int relax_out(const M5P &P, const RelaxIn<NM> &R, double *are, double
   using L = L5<NM>;
   if (P.prelax >= 1 && P.prelax <= 3) {
     double e_elast = 0.0, are_el[NM];
+    // mixture mode: alpha_iv of the elastic energy of the relaxed A
+    double Gx[9], Gmx = R.Gmag;
+    GPow<> gpx = R.gp;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Gx[k] = R.G[k];
+    if (P.mix) {
+      gmat(R.A, Gx, Gmx);
+      gpx = gpow(Gmx);
+    }
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv) {
-      are_el[iv] = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0;
+      are_el[iv] = (P.mush[iv] != 0.0) ? elastic_energy_v2(Gx, Gmx, gpx, P.mush[iv]) : 0.0;
+      if (P.mix) are_el[iv] *= R.avf[iv];
       const double e_elastic_K = are_el[iv] / (R.ar[iv]);
       const double Ys = R.ar[iv] / R.rho;
       e_elast += Ys * e_elastic_K;
@@ -1077,7 +1145,8 @@ __device__ int r3_setup(const M5P &P, RelaxIn<NM> &R, double *pk, double &pr) {
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv) {
       if (R.avf[iv] > eps_sml) {
-        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0;
+        const double ael = ((P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0) *
+                           (P.mix ? R.avf[iv] : 1.0);
         pk[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
         if (pk[iv] <= -(1.0 - eps_big) * P.pinf[iv] + eps_big) pk[iv] = -(1.0 - eps_big) * P.pinf[iv] + eps_big;
       } else {
@@ -1168,7 +1237,8 @@ __device__ int relax5(const M5P &P, double *x) {
     if (P.prelax == 1) {   // m5_relaxationvec.c:89-190
 #pragma unroll
       for (int iv = 0; iv < NM; ++iv) {
-        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0;
+        const double ael = ((P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0) *
+                           (P.mix ? R.avf[iv] : 1.0);
         t0[iv] = 1.0 / (R.ar[iv] / R.avf[iv]);
         p0[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
       }
@@ -1182,7 +1252,8 @@ __device__ int relax5(const M5P &P, double *x) {
       int principal = 0;
 #pragma unroll
       for (int iv = 0; iv < NM; ++iv) {
-        const double ael = (P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0;
+        const double ael = ((P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0) *
+                           (P.mix ? R.avf[iv] : 1.0);
         t0[iv] = 1.0 / (R.ar[iv] / R.avf[iv]);
         p0[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
         if (p0[iv] < -(1.0 - eps_big) * P.pinf[iv] + eps_big) p0[iv] = -(1.0 - eps_big) * P.pinf[iv] + 1.0;
@@ -1660,7 +1731,8 @@ __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__
     gmat(sc.A, Gm, Gmag);
     const GPow<> gps = has_solid(P) ? gpow(Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < NM; ++k) stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mush[k]);
+    for (int k = 0; k < NM; ++k)
+      stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mix ? sc.avf[k] * P.mush[k] : P.mush[k]);
   }
   const double A1 = Am[0], B1 = Am[3], C1 = Am[6];
   const double A2 = Am[1], B2 = Am[4], C2 = Am[7];
@@ -1901,7 +1973,8 @@ __global__ __launch_bounds__(256) void k_m5_residual_mx(int n, M5P P, const Geo 
     gmat(sc.A, Gm, Gmag);
     const GPow<> gps = has_solid(P) ? gpow(Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < NM; ++k) stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mush[k]);
+    for (int k = 0; k < NM; ++k)
+      stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mix ? sc.avf[k] * P.mush[k] : P.mush[k]);
   }
   const double tr = T[0] + T[4] + T[8];   // sum_f Area_f (u*.n)
   const double sgn = P.fix ? -1.0 : 1.0;
@@ -1981,8 +2054,10 @@ int m5_params(const fv_m5_params *p, M5P &P) {
   P.prelax = p->prelax; P.order = p->order;
   if (!(p->yield >= 0.0)) return fv_set_err(FV_ERR_ARG, "model 5: the yield stress must be >= 0");
   P.yield = p->yield;
-  if (p->fix != 0 && p->fix != FV_M5_FIX_SIGNS) return fv_set_err(FV_ERR_ARG, "model 5: fix must be 0 or FV_M5_FIX_SIGNS");
-  P.fix = p->fix ? 1 : 0;
+  if (p->fix & ~(FV_M5_FIX_SIGNS | FV_M5_FIX_MIXTURE))
+    return fv_set_err(FV_ERR_ARG, "model 5: fix must be a combination of FV_M5_FIX_SIGNS and FV_M5_FIX_MIXTURE");
+  P.fix = (p->fix & FV_M5_FIX_SIGNS) ? 1 : 0;
+  P.mix = (p->fix & FV_M5_FIX_MIXTURE) ? 1 : 0;
   P.dtu = P.fix ? -p->dt : p->dt;
   P.sg = P.fix ? -1.0 : 1.0;
   return FV_OK;
@@ -2205,13 +2280,15 @@ struct fv_m5_dev {
   int *iters = nullptr;       // [6n]
   uint8_t *mark6 = nullptr;   // [6n]
   int *list6 = nullptr;       // [6n]
+  double *part = nullptr;     // [n/256 + 1] per-block minima of the CFL step, then the step
 };
 
 namespace {
 
 void m5_free(fv_m5_dev *c) {
   void *ptrs[] = {c->G, c->nbr, c->fine4, c->info, c->S, c->S2, c->SN, c->SF, c->RHS, c->Sb,
-                  c->gfm, c->mark, c->flag, c->list, c->count, c->cub_tmp, c->nst, c->iters, c->mark6, c->list6};
+                  c->gfm, c->mark, c->flag, c->list, c->count, c->cub_tmp, c->nst, c->iters, c->mark6, c->list6,
+                  c->part};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -2366,6 +2443,7 @@ extern "C" int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_
   chk(dalloc(&c->SF, 6 * sz));
   chk(dalloc(&c->gfm, 6 * n)); chk(dalloc(&c->mark, n));
   chk(dalloc(&c->flag, 2)); chk(dalloc(&c->list, n)); chk(dalloc(&c->count, 1));
+  chk(dalloc(&c->part, (n + 255) / 256 + 1));
   if (p->prelax == 3) {
     chk(dalloc(&c->nst, 18 * n)); chk(dalloc(&c->iters, 6 * n));
     chk(dalloc(&c->mark6, 6 * n)); chk(dalloc(&c->list6, 6 * n));
@@ -2447,6 +2525,30 @@ extern "C" int fv_m5_dev_advance(fv_m5_dev *c, int nsteps) {
     if (rc != FV_OK) return rc;
     if ((rc = m5_check(c)) != FV_OK) return rc;
   }
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_set_dt(fv_m5_dev *c, double dt) {
+  if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (!(dt > 0.0) || !isfinite(dt)) return fv_set_err(FV_ERR_ARG, "model 5: dt must be positive and finite");
+  c->P.dt = dt;
+  c->P.dtu = c->P.fix ? -dt : dt;
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_cfl_dt(fv_m5_dev *c, double cfl, double *dt) {
+  if (!c || !dt) return fv_set_err(FV_ERR_ARG, "null argument");
+  const int nb = (c->n + 255) / 256;
+  if (c->P.nmat == 2)
+    hipLaunchKernelGGL(k_m5_cfl<2>, dim3(nb), dim3(256), 0, c->stream, c->n, c->P, c->G, c->info, c->S, c->part);
+  else
+    hipLaunchKernelGGL(k_m5_cfl<3>, dim3(nb), dim3(256), 0, c->stream, c->n, c->P, c->G, c->info, c->S, c->part);
+  hipLaunchKernelGGL(k_m5_min_final, dim3(1), dim3(1024), 0, c->stream, nb, c->part, c->part + nb, cfl);
+  HIPCHK(hipGetLastError());
+  double h = 0.0;
+  HIPCHK(hipMemcpyAsync(&h, c->part + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *dt = h;
   return FV_OK;
 }
 

@@ -4,6 +4,7 @@
  * conditions.
  */
 #include "forestfv.h"
+#include "forestfv_m5.h"
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -246,6 +247,57 @@ int fv_init_bubble(const fv_mesh *m, const fv_params *p, const double *rho_init,
     s[2] = 0.0; s[3] = 0.0; s[4] = 0.0;
     s[5] = rE;
     s[6] = al;
+  }
+  return FV_OK;
+}
+
+/* Model-5 initial data of a solid slab under a liquid with a gas bubble (new;
+ * materials 0 solid, 1 liquid, 2 gas).  A cell of one material follows
+ * m5_init_S (init_S.c:205-266): that material holds 1 - (nmat-1) AMIN of the
+ * volume and the others AMIN, every material is at the cell's pressure, and
+ * A = I where the solid fraction exceeds 10 AMIN, else A = 0
+ * (compute_Amat.c:3-40), so that fluid cells carry no elastic energy or
+ * shear stress.  The slab z < z_slab is solid with a sharp face (the GFM
+ * level set phi = alpha_solid - AVF_LIM changes sign there); above it the gas
+ * fraction is the model-4 bubble's tanh profile over smear * hmin
+ * (fv_init_bubble) and the pressure mixes linearly with it.  At rest. */
+int fv_m5_init_slab_bubble(const fv_mesh *m, const fv_m5_params *p, const double *rho_init, double z_slab,
+                           const double centre[3], double radius, double p_liquid, double p_gas, double smear,
+                           double *S) {
+  if (p->nmat != 3 || !(radius > 0.0) || !(smear > 0.0)) return FV_ERR_ARG;
+  const int nm = 3, neq = 3 * nm + 12;
+  const int i1 = nm, i2 = nm + 3, i3 = nm + 4, i4 = i3 + nm - 1, i5 = i4 + nm;
+  const double amin = p->amin;
+  const double hmin = m->rootsize / (double)(1 << (m->maxlevel - 1));
+  for (int i = 0; i < m->n; i++) {
+    double c[3];
+    fv_mesh_centre(m, i, c);
+    double avf[3], pr;
+    if (c[2] < z_slab) {
+      avf[0] = 1.0 - (double)(nm - 1) * amin; avf[1] = amin; avf[2] = amin;
+      pr = p_liquid;
+    } else {
+      double r = sqrt(SQ(c[0] - centre[0]) + SQ(c[1] - centre[1]) + SQ(c[2] - centre[2]));
+      double ag = 0.5 * (1.0 - tanh((r - radius) / (smear * hmin)));
+      if (ag < amin) ag = amin;
+      if (ag > 1.0 - 2.0 * amin) ag = 1.0 - 2.0 * amin;
+      avf[0] = amin; avf[2] = ag; avf[1] = 1.0 - amin - ag;
+      pr = p_gas * ag + p_liquid * (1.0 - ag);
+    }
+    double *s = &S[(size_t)i * neq];
+    for (int v = 0; v < neq; v++) s[v] = 0.0;
+    double rE = 0.0;
+    for (int k = 0; k < nm; k++) {
+      s[k] = avf[k] * rho_init[k];
+      /* A is I or 0, so the elastic energy of every material is 0 */
+      double are = avf[k] * (pr + p->gamma[k] * p->pinf[k]) / (p->gamma[k] - 1.0);
+      s[i4 + k] = are;
+      rE += are;
+    }
+    s[i1 + 0] = 0.0; s[i1 + 1] = 0.0; s[i1 + 2] = 0.0;
+    s[i2] = rE;
+    for (int k = 0; k < nm - 1; k++) s[i3 + k] = avf[k];
+    if (avf[0] > 10.0 * amin) { s[i5 + 0] = 1.0; s[i5 + 4] = 1.0; s[i5 + 8] = 1.0; }
   }
   return FV_OK;
 }

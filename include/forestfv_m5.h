@@ -41,8 +41,9 @@ typedef struct {
   double yield;                /* von Mises yield stress of the solid (material 0) for perfect
                                   plasticity, 0 = elastic; new: the reference's plastic_rlx is
                                   commented out (m5_relaxation_solid.c:14) */
-  int    fix;                  /* FV_M5_FIX_SIGNS: the consistent model-5 update (new, below); 0 = the
-                                  published code */
+  int    fix;                  /* FV_M5_FIX_SIGNS: the consistent model-5 update (new, below), with
+                                  FV_M5_FIX_MIXTURE: the volume-fraction-weighted elastic mixture
+                                  (new, below); 0 = the published code */
 } fv_m5_params;
 
 /* Corrections of the published model-5 update, one mode (0 reproduces it):
@@ -60,8 +61,29 @@ typedef struct {
  * alpha), so that det A follows the solid's density (m5_relaxationvec.c
  * scales by alpha / alpha0). */
 #define FV_M5_FIX_SIGNS 1
+/* A mixture model (new, set as a bit of fix beside FV_M5_FIX_SIGNS): the
+ * reference gives every phase with a shear modulus the cell's whole elastic
+ * energy and shear stress (state_equation.c:143-316 carry no volume fraction),
+ * so a fluid cell holding a trace of solid shears with the solid's modulus
+ * while the wave speeds carry only the hydrodynamic sound speed.  This bit
+ * gives phase k the fraction alpha_k of both (mixture stress sum_k alpha_k
+ * sigma_k), adds (4/3) alpha_k mu_k / rho to each phase's squared sound speed
+ * in the HLLC and GFM wave speeds, and evaluates the relaxation's elastic
+ * energy on its relaxed A. */
+#define FV_M5_FIX_MIXTURE 2
 
 int fv_m5_neq(int nmat);
+
+/* Initial data (new; materials 0 solid, 1 liquid, 2 gas) in the form of
+ * m5_init_domain's solid cases (m5_init_domain.c:175-216, init_S.c:205-266,
+ * compute_Amat.c): a solid slab z < z_slab with a sharp face, a gas bubble of
+ * the given radius and centre smeared with tanh over smear * hmin in the
+ * liquid, every material at the cell's pressure (p_liquid, p_gas mixed with
+ * the bubble profile), A = I in the solid and 0 in the fluids, at rest.
+ * rho_init[3]: material densities; S: [n][NEQ]. */
+int fv_m5_init_slab_bubble(const fv_mesh *m, const fv_m5_params *p, const double *rho_init, double z_slab,
+                           const double centre[3], double radius, double p_liquid, double p_gas, double smear,
+                           double *S);
 
 /* GFM flux of nf sub-faces at a solid interface: replaces m5_hllc_gfm
  * (m5_hllc_gfm.c:3-100, with m5_wave_speeds_gfm.c, m5_flux_gfm.c and
@@ -120,6 +142,13 @@ int fv_m5_dev_advance(fv_m5_dev *c, int nsteps);
  * flux; the records are formed in FP64 (pressures carry p_inf up to 2e10 Pa),
  * and the residual, RK accumulation and relaxation stay FP64. */
 int fv_m5_dev_set_precision(fv_m5_dev *c, int prec);
+/* The time step of later stages (DT of the params at attach until then). */
+int fv_m5_dev_set_dt(fv_m5_dev *c, double dt);
+/* CFL step of the device state (new: the reference takes a fixed DT):
+ * cfl * min over leaves of h / (max|u_i| + c), with c^2 = sum_k c2Y_k +
+ * (4/3) sum_k alpha_k mu_k / rho, the longitudinal sound speed of the mixture
+ * (c2Y of m5_wave_speeds.c:20-40 is the hydrodynamic part). */
+int fv_m5_dev_cfl_dt(fv_m5_dev *c, double cfl, double *dt);
 /* the stage's pieces alone: face states of the current S (m5_faceinterp_V1),
  * and the residual of S and those face states (m5_residual) */
 int fv_m5_dev_faceinterp(fv_m5_dev *c);

@@ -188,14 +188,19 @@ static void m5_record_x(const lay5_t *L, const orc_m5_params *p, const double *S
   double Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) + G[2] * (G[3] * G[7] - G[4] * G[6]);
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) r->st[i][j] = 0.0;
+  const int mix = p->fix & ORC_M5_FIX_MIXTURE;
   for (int i = 0; i < L->n4; ++i) {
     double are_elast = (p->mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, p->mush[i]) : 0.0;
     double g = p->gamma[i], pi = p->pinf[i];
     const double den = face ? r->avf[i] : avf_temp[i];
+    /* mixture mode: phase k holds alpha_k of the elastic energy and of the shear stress */
+    const double mu_k = mix ? r->avf[i] * p->mush[i] : p->mush[i];
+    if (mix) are_elast *= den;
     double press = ((g - 1.0) * (r->are[i] - are_elast) / den) - g * pi;
     r->c2Y[i] = (den / r->r) * g * (press + pi);
+    if (mix) r->c2Y[i] += (4.0 / 3.0) * mu_k / r->r;   /* longitudinal speed of the mixture */
     r->p_hydro[i] = press;
-    eos_stress_v2(r->ast[i], r->avf[i], r->p_hydro[i], G, Gmag, p->mush[i]);
+    eos_stress_v2(r->ast[i], r->avf[i], r->p_hydro[i], G, Gmag, mu_k);
     for (int a1 = 0; a1 < 3; ++a1)
       for (int b = 0; b < 3; ++b) r->st[a1][b] += r->ast[i][a1][b];
   }
@@ -361,6 +366,7 @@ static int m5_star_region(const lay5_t *L, const orc_m5_params *p, const fr5_t *
     double p_k = fix ? s->p_hydro[v] : -s->p_hydro[v];
     double p_S = (p_k + pi) * (((g - 1.0) * rm + (g + 1.0) * rm_S) / ((g - 1.0) * rm_S + (g + 1.0) * rm)) - pi;
     double are_el = elastic_energy_v1(G, p->mush[v]);
+    if (p->fix & ORC_M5_FIX_MIXTURE) are_el *= s->avf[v];
     double e_el = are_el / (s->avf[v] * rm_S);
     e_S[v] = (p_S + g * pi) / (rm_S * (fix ? (g - 1.0) : (g + 1.0))) + e_el;
   }
@@ -754,6 +760,7 @@ static int m5_relaxvec(const lay5_t *L, const orc_m5_params *p, double *vec) {
   const double a = p->amin / 100.0, eps_sml = 1.0e-12, eps_big = 1.0e-8;
   const double *Gm = p->gamma, *Pi = p->pinf, *Mu = p->mush;
   const int nm = L->nmat;
+  const int mix = p->fix & ORC_M5_FIX_MIXTURE;
   double ar[ORC_MAXMAT], are[ORC_MAXMAT], avf[ORC_MAXMAT + 1], avf_init[ORC_MAXMAT + 1];
   double are_el[ORC_MAXMAT] = {0}, A[9], G[9], Gmag;
   double rho = 0.0;
@@ -771,6 +778,7 @@ static int m5_relaxvec(const lay5_t *L, const orc_m5_params *p, double *vec) {
     if (p->prelax == 1) {   /* m5_relaxationvec.c:89-190 */
       for (int iv = 0; iv < L->n0; ++iv) {
         double ael = (Mu[iv] != 0.0) ? elastic_energy_v2(G, Gmag, Mu[iv]) : 0.0;
+        if (mix) ael *= avf[iv];
         t0[iv] = 1.0 / (ar[iv] / avf[iv]);
         p0[iv] = ((Gm[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - Gm[iv] * Pi[iv];
       }
@@ -783,6 +791,7 @@ static int m5_relaxvec(const lay5_t *L, const orc_m5_params *p, double *vec) {
       int principal = 0;
       for (int iv = 0; iv < L->n0; ++iv) {
         double ael = (Mu[iv] != 0.0) ? elastic_energy_v2(G, Gmag, Mu[iv]) : 0.0;
+        if (mix) ael *= avf[iv];
         t0[iv] = 1.0 / (ar[iv] / avf[iv]);
         p0[iv] = ((Gm[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - Gm[iv] * Pi[iv];
         if (p0[iv] < -(1.0 - eps_big) * Pi[iv] + eps_big) p0[iv] = -(1.0 - eps_big) * Pi[iv] + 1.0;
@@ -826,6 +835,7 @@ static int m5_relaxvec(const lay5_t *L, const orc_m5_params *p, double *vec) {
       for (int iv = 0; iv < nm; ++iv) {
         if (avf[iv] > eps_sml) {
           double ael = (Mu[iv] != 0.0) ? elastic_energy_v2(G, Gmag, Mu[iv]) : 0.0;
+        if (mix) ael *= avf[iv];
           pk[iv] = ((Gm[iv] - 1.0) * (are[iv] - ael) / avf[iv]) - Gm[iv] * Pi[iv];
           if (pk[iv] <= -(1.0 - eps_big) * Pi[iv] + eps_big) pk[iv] = -(1.0 - eps_big) * Pi[iv] + eps_big;
         } else {
@@ -863,8 +873,10 @@ static int m5_relaxvec(const lay5_t *L, const orc_m5_params *p, double *vec) {
   if (p->prelax >= 1 && p->prelax <= 3) {
     /* re-initialisation: pressure from the total energy (m5_relaxationvec.c:157-189 and the same in each branch) */
     double e_elast = 0.0;
+    if (mix) gmat_of(A, G, &Gmag);   /* the elastic energy of the relaxed A */
     for (int iv = 0; iv < nm; ++iv) {
       are_el[iv] = (Mu[iv] != 0.0) ? elastic_energy_v2(G, Gmag, Mu[iv]) : 0.0;
+      if (mix) are_el[iv] *= avf[iv];
       double e_elastic_K = are_el[iv] / (ar[iv]);
       double Ys = ar[iv] / rho;
       e_elast += Ys * e_elastic_K;
@@ -1550,4 +1562,30 @@ int orc_m5_advance(const orc_mesh *m, const orc_cell *cells, const orc_m5_params
   free(SN);
   m5_stage_ws_free(&W);
   return rc;
+}
+
+/* CFL step of model 5 (new; the reference takes a fixed DT): cfl * min over
+ * leaves of h / (max|u_i| + c), c^2 = sum_k c2Y_k + (4/3) sum_k alpha_k mu_k /
+ * rho (c2Y_k of the cell record, m5_facevalues.c:140-150; in the mixture mode
+ * it carries the shear part already) */
+double orc_m5_cfl_dt(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, const double *S, double cfl) {
+  (void)cells;
+  lay5_t L = mk_lay5(p->nmat);
+  double dt = INFINITY;
+  for (int i = 0; i < m->n; i++) {
+    rec5_t r;
+    m5_record(&L, p, &S[(size_t)i * L.neq], &r);
+    double c2 = 0.0;
+    for (int k = 0; k < L.nmat; ++k) c2 += r.c2Y[k];
+    if (!(p->fix & ORC_M5_FIX_MIXTURE)) {
+      double mu = 0.0;
+      for (int k = 0; k < L.nmat; ++k) mu += r.avf[k] * p->mush[k];
+      c2 += (4.0 / 3.0) * mu / r.r;
+    }
+    const double lam = MAXF(MAXF(fabs(r.u), fabs(r.v)), fabs(r.w)) + sqrt(c2);
+    const double h = m->rootsize / (double)(1 << (m->level[i] - 1));
+    const double d = h / lam;
+    if (d < dt) dt = d;
+  }
+  return cfl * dt;
 }

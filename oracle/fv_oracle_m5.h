@@ -71,6 +71,16 @@ typedef struct {
  *  - the relaxation scales A by cbrt(alpha0 / alpha) of the solid, so that
  *    det A follows its density (m5_relaxationvec.c scales A by alpha / alpha0). */
 #define ORC_M5_FIX_SIGNS 1
+/* ORC_M5_FIX_MIXTURE (new, a mixture model rather than a correction): the
+ * reference gives every phase with a shear modulus the whole elastic energy
+ * and shear stress of the cell (state_equation.c:143-316 are not weighted by
+ * the volume fraction), so a fluid cell holding a trace of solid shears with
+ * the solid's modulus while the wave speeds carry only the hydrodynamic sound
+ * speed.  This mode gives phase k the fraction alpha_k of both (the mixture
+ * stress sum_k alpha_k sigma_k), adds (4/3) alpha_k mu_k / rho to the phase's
+ * squared sound speed in the HLLC and GFM wave speeds, and evaluates the
+ * relaxation's elastic energy on its relaxed A. */
+#define ORC_M5_FIX_MIXTURE 2
 
 int orc_m5_neq(int nmat);
 
@@ -142,6 +152,10 @@ int orc_m5_plastic(const orc_m5_params *p, double *vec);
 
 /* nsteps explicit steps: SN = S (RK.c case 0), then stages 1..rk_steps. */
 int orc_m5_advance(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, double *S, int nsteps);
+
+/* CFL step (new): cfl * min h / (max|u_i| + c), c^2 = sum_k c2Y_k + (4/3)
+ * sum_k alpha_k mu_k / rho, the longitudinal sound speed of the mixture. */
+double orc_m5_cfl_dt(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, const double *S, double cfl);
 
 /* One face of m5_residual.c:139-170 between two face states (m5_facercnstr
  * records: face state X with phi from the cell state C), by m5_hllc or, when

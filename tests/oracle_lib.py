@@ -147,6 +147,8 @@ def lib(variant="O2"):
         L.orc_m5_stage.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), C.c_int, dp, dp, dp, dp,
                                    P(C.c_uint8)]
         L.orc_m5_advance.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), dp, C.c_int]
+        L.orc_m5_cfl_dt.argtypes = [P(orc_mesh), P(orc_cell), P(orc_m5_params), dp, C.c_double]
+        L.orc_m5_cfl_dt.restype = C.c_double
         L.orc_m5_face.argtypes = [P(orc_m5_params), dp, dp, dp, dp, dp, C.c_int, C.c_double, dp, dp]
         L.orc_m5_plastic.argtypes = [P(orc_m5_params), dp]
         ip = P(C.c_int)
@@ -307,6 +309,10 @@ class OracleMesh:
         if rc != 0:
             raise RuntimeError(f"orc_m5_stage status {rc}")
         return S, SF, RHS, gf
+
+    def m5_cfl_dt(self, m5p, S, cfl):
+        return self.L.orc_m5_cfl_dt(C.byref(self.om), self.cells, C.byref(m5_from(m5p)),
+                                    _dp(np.ascontiguousarray(S, dtype=np.float64)), cfl)
 
     def m5_advance(self, m5p, S, nsteps):
         S = np.ascontiguousarray(S, dtype=np.float64).copy()
