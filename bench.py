@@ -438,134 +438,198 @@ C2_RHO = np.array([2700.0, 1000.0, 1.2])
 C2_YIELD = 0.3e9      # von Mises yield stress of the slab, perfect plasticity
 
 
-def c2_state(p, m, L=L_DOMAIN):
-    """Config 2 initial data on mesh m: a solid slab filling the bottom 25 % of the cube, an air
-    bubble of radius 0.4 mm whose centre stands 1.2 radii above the slab, water at 100 bar, air at
-    1 bar, both interfaces smeared with tanh over the cell size; A = I (undeformed solid)."""
-    c = m.centres()
-    n = m.n
-    h = L / (1 << (int(m.arrays()["level"].max()) - 1))
-    z_slab = 0.25 * L
-    centre = np.array([0.5 * L, 0.5 * L, z_slab + 1.2 * R_BUBBLE])
-    a_s = 0.5 * (1.0 - np.tanh((c[:, 2] - z_slab) / h))
-    bub = 0.5 * (1.0 - np.tanh((np.linalg.norm(c - centre, axis=1) - R_BUBBLE) / h))
-    a_a = bub * (1.0 - a_s)
-    alpha = np.stack([a_s, 1.0 - a_s - a_a, a_a], axis=1)
-    alpha = np.clip(alpha, 1e-6, None)
-    alpha /= alpha.sum(axis=1, keepdims=True)
-    pres = 1e5 + (1e7 - 1e5) * (1.0 - bub)
-    # a seeded smooth velocity of about 1 m/s: at rest the star velocity of every face is zero up to
-    # rounding, where the published star energy (denominator u* + S_K) changes sign with u*
-    rng = np.random.default_rng(1234)
-    k = rng.uniform(1.0, 3.0, size=(3, 3)) * (2.0 * np.pi / L)
-    ph = rng.uniform(0.0, 2.0 * np.pi, size=3)
-    u = np.stack([np.sin(c @ k[q] + ph[q]) for q in range(3)], axis=1)
-    g, pi = np.array(p.gamma[:3]), np.array(p.pinf[:3])
-    nq = F.m5_neq(3)
-    S = np.zeros((n, nq))
-    ra = alpha * C2_RHO
-    are = alpha * (pres[:, None] + g * pi) / (g - 1.0)
-    r = ra.sum(axis=1)
-    S[:, 0:3] = ra
-    S[:, 3:6] = r[:, None] * u
-    S[:, 6] = are.sum(axis=1) + 0.5 * r * (u * u).sum(axis=1)
-    S[:, 7:9] = alpha[:, :2]
-    S[:, 9:12] = are
-    S[:, 12:] = np.eye(3).ravel()
-    return S
+C2_Z_SLAB = 0.25 * L_DOMAIN                  # the slab fills the bottom quarter of the cube
+C2_STANDOFF = 1.2                            # bubble centre to slab face, in bubble radii
+C2_P = (1e7, 1e5)                            # water at 100 bar, air at 1 bar
+C2_FIX = F.M5_FIX_SIGNS | F.M5_FIX_MIXTURE   # the consistent update with the weighted elastic mixture
 
 
-def m5_leg(level=8, steps=3, check=True, log_fn=log):
-    """The whole model-5 stage (MUSCL, HLLC/GFM residual with A transport, m5_RK, the plastic
-    return map at 0.3 GPa, PRELAX 3 relaxation and the interface re-projections) with config 2's
-    materials on a uniform
-    128^3 grid, with the oracle timed on 32^3 and the device checked against it there."""
+def c2_params(prelax=3):
+    return F.make_m5_params(nmat=3, gamma=C2_TABLE["gamma"], pinf=C2_TABLE["pinf"], mush=C2_TABLE["mush"],
+                            amin=1e-6, avf_lim=0.5, prelax=prelax, order=2, rk_steps=3, fix=C2_FIX,
+                            yield_stress=C2_YIELD)
+
+
+def c2_crit(maxlevel, base, layers):
+    return F.make_crit(uniform_on=1, uniform_lvl=base, prs_on=1, prs_lvl=maxlevel, prs_thr=0.2,
+                       inter0_on=1, inter0_lvl=maxlevel, inter0_lrs=layers, inter1_on=0)
+
+
+def c2_state(p, m, dx=0.0):
+    """Config 2 initial data on mesh m: the solid slab below z = 1 mm with a sharp face, the
+    0.4 mm air bubble 1.2 radii above it smeared over two finest cells, water at 100 bar, air at
+    1 bar, A = I in the solid and 0 in the fluids, at rest (fv_m5_init_slab_bubble)."""
+    centre = (0.5 * L_DOMAIN + dx, 0.5 * L_DOMAIN, C2_Z_SLAB + C2_STANDOFF * R_BUBBLE)
+    return m.init_m5_slab_bubble(p, C2_RHO, C2_Z_SLAB, centre, R_BUBBLE, C2_P[0], C2_P[1], 2.0)
+
+
+def build_c2(maxlevel=9, base=5, layers=64, log_fn=log):
+    """Geometric pre-adaptation of config 2 on the device (main.c:107-134 order): initial data on
+    the current mesh, then the model-5 indicator, criterionsmth and refinement
+    (fv_m5_dev_regrid), repeated; the host mesh is rebuilt from the device's leaves each pass."""
+    p = c2_params()
+    m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), L_DOMAIN, base, [2] * 6)
+    crit = c2_crit(maxlevel, base, layers)
+    p.dt = 1e-12
+    dev = F.M5Device(p, m)
+    dev.set_forest(m)
+    for ipass in range(maxlevel - base + 1):
+        dev.set_state(c2_state(p, m))
+        dev.regrid(crit, maxlevel)
+        root, level, ci, SI = dev.get_forest()
+        m = F.Mesh.from_leaves((1, 1, 1), (0, 0, 0), L_DOMAIN, [2] * 6, root, level, ci)
+        m.set_SI(SI)
+        log_fn(f"  c2 geo pass {ipass}: {m.n} leaves, levels {np.bincount(level)[base:]}")
+    S = c2_state(p, m)
+    dev.set_state(S)
+    m.set_SI(dev.interface_location(crit))
+    return p, m, S, dev, crit
+
+
+def gas_volume(S, level, L=L_DOMAIN):
+    h = L / (1 << (level - 1))
+    return float(((1.0 - S[:, 7] - S[:, 8]) * h ** 3).sum())
+
+
+def c2_loop(dev, crit, maxlevel, nsteps, every=5, cfl=0.4, t_end=None, probe=0, log_fn=log):
+    """loop.c:12-30 for model 5: the CFL step, one SSP-RK3 step (timed with hipEvents per stage),
+    interface_location after it, and a device regrid every `every` steps; stops after nsteps or at
+    t_end.  probe > 0: the gas volume every `probe` steps."""
+    t = 0.0
+    ms_stage_tot = ms_res_tot = t_other = 0.0
+    cells = 0
+    leaves = [dev.n]
+    vols = []
+    step = 0
+    t_wall = time.perf_counter()
+    while step < nsteps and (t_end is None or t < t_end):
+        t0 = time.perf_counter()
+        if step > 0 and step % every == 0:
+            dev.regrid(crit, maxlevel)
+            leaves.append(dev.n)
+        dt = dev.cfl_dt(cfl)
+        dev.set_dt(dt)
+        t_other += time.perf_counter() - t0
+        ms_stage, ms_res = dev.time_steps(1)
+        ms_stage_tot += 3 * ms_stage
+        ms_res_tot += 3 * ms_res
+        cells += 3 * dev.n
+        t0 = time.perf_counter()
+        dev.interface_location(crit)
+        t_other += time.perf_counter() - t0
+        t += dt
+        step += 1
+        if probe and (step % probe == 0 or step == nsteps):
+            root, level, ci, SI = dev.get_forest()
+            vols.append((t, gas_volume(dev.get_state(), level)))
+            log_fn(f"  c2 step {step}: t {t:.3e} s, {dev.n} leaves, gas volume {vols[-1][1]:.4e} m^3")
+    wall = time.perf_counter() - t_wall
+    return {"steps": step, "t": t, "leaves": leaves, "ms_stage": ms_stage_tot / (3 * step),
+            "ms_residual": ms_res_tot / (3 * step), "stage_rate": cells / (ms_stage_tot * 1e-3) / 1e6,
+            "loop_rate": cells / wall / 1e6, "other_s": t_other, "wall_s": wall, "vols": vols}
+
+
+def c2_leg(maxlevel=9, base=5, layers=64, steps=15, every=5, check=True, log_fn=log):
+    """Config 2 at its size: the slab, water and bubble pre-adapted on the device to reference
+    levels 5-9 (config levels 4-8) with 64 interface layers on both interfaces, then the model-5
+    loop (CFL 0.4, SSP-RK3, PRELAX 3, von Mises 0.3 GPa, interface_location every step, a device
+    regrid every 5 steps) in FP64 and in mixed precision.  The oracle is timed and checked on the
+    32^3 uniform state."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    out = {}
-    for lvl, what in ((level, "device"), (6, "check")) if check else ((level, "device"),):
-        p = F.make_m5_params(nmat=3, gamma=C2_TABLE["gamma"], pinf=C2_TABLE["pinf"], mush=C2_TABLE["mush"],
-                             amin=1e-6, avf_lim=0.5, prelax=3, order=2, rk_steps=3, yield_stress=C2_YIELD)
-        m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), L_DOMAIN, lvl, [4] * 6)
-        h = L_DOMAIN / (1 << (lvl - 1))
-        # 1 % of the CFL step on the solid sound speed: the published model-5 update is not stable
-        # on this state (see below) and reaches NaN within one step at the CFL step itself
-        p.dt = 0.01 * 0.4 * h / np.sqrt(3.0 * (1e7 + 2.0e10) / 2700.0)
-        S = c2_state(p, m)
-        dev = F.M5Device(p, m)
-        dev.set_state(S)
-        if what == "device":
-            # The published model-5 update reaches NaN in its second step on this state (the GFM
-            # faces flux rho u u - sigma, the others rho u u + sigma, and the unlimited alpha-rho-e
-            # face states put pressures of order gamma pinf into sigma at the slab), so each timed
-            # step starts again from the initial state.
-            dev.time_steps(1)
-            ms = []
-            for _ in range(steps):
-                dev.set_state(S)
-                ms.append(dev.time_steps(1))
-            ms_stage = float(np.mean([x[0] for x in ms]))
-            ms_res = float(np.mean([x[1] for x in ms]))
-            Sg = dev.get_state()
-            out.update({"leaves": m.n, "ms_per_stage": round(ms_stage, 3), "ms_residual_kernel": round(ms_res, 3),
-                        "value": round(m.n / (ms_stage * 1e-3) / 1e6, 2), "finite": bool(np.all(np.isfinite(Sg)))})
-            B = algorithmic_bytes(F.m5_neq(3))
-            out["roofline"] = {"bound": "hbm", "algorithmic_bytes_per_leaf": B,
-                               "achieved": round(B * m.n / (ms_stage * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS}
-            out["roofline"]["frac"] = round(out["roofline"]["achieved"] / HBM_PEAK_GBPS, 4)
-            log_fn(f"[m5] {m.n} leaves: {ms_stage:.3f} ms/stage, residual {ms_res:.3f} ms")
-            # mixed precision: FP32 slopes, face records, HLLC and GFM flux; one step from the
-            # same state against the FP64 step, per conserved field
-            dev.set_precision(F.PREC_MIXED)
-            dev.set_state(S)
-            dev.time_steps(1)
-            mm = []
-            for _ in range(steps):
-                dev.set_state(S)
-                mm.append(dev.time_steps(1))
-            Sm = dev.get_state()
-            ms_mx = float(np.mean([x[0] for x in mm]))
-            nm = 3
-            names = [f"alpha_rho_{k}" for k in range(nm)] + ["rho_u", "rho_v", "rho_w", "rho_E"] + \
-                [f"alpha_{k}" for k in range(nm - 1)] + [f"alpha_rho_e_{k}" for k in range(nm)] + \
-                [f"A_{a}{b}" for a in range(3) for b in range(3)]
-            # momentum on the momentum magnitude, the deformation tensor on its Frobenius norm
-            # (its off-diagonal entries start at zero)
-            scale1 = np.abs(Sg).sum(0)
-            scalei = np.abs(Sg).max(0)
-            mom = np.sqrt((Sg[:, nm:nm + 3] ** 2).sum(1))
-            an = np.sqrt((Sg[:, -9:] ** 2).sum(1))
-            scale1[nm:nm + 3] = mom.sum(); scalei[nm:nm + 3] = mom.max()
-            scale1[-9:] = an.sum(); scalei[-9:] = an.max()
-            l1 = np.abs(Sm - Sg).sum(0) / np.maximum(scale1, 1e-300)
-            li = np.abs(Sm - Sg).max(0) / np.maximum(scalei, 1e-300)
-            out["mixed"] = {"ms_per_stage": round(ms_mx, 3), "ms_residual_kernel": round(float(np.mean([x[1] for x in mm])), 3),
-                            "value": round(m.n / (ms_mx * 1e-3) / 1e6, 2),
-                            "speedup_vs_fp64_stage": round(ms_stage / ms_mx, 3),
-                            "rel_l1_vs_fp64_1step": {k: float(v) for k, v in zip(names, l1)},
-                            "rel_linf_vs_fp64_1step": {k: float(v) for k, v in zip(names, li)},
-                            "rel_l1_vs_fp64_1step_max_field": float(l1.max()),
-                            "finite": bool(np.all(np.isfinite(Sm)))}
-            log_fn(f"[m5] mixed: {ms_mx:.3f} ms/stage, max field L1 vs FP64 {l1.max():.2e} ({names[int(l1.argmax())]})")
-        else:
-            dev.advance(1)
-            Sg = dev.get_state()
-            om = O.OracleMesh(m)
-            t0 = time.perf_counter()
-            So = om.m5_advance(p, S, 1)
-            t_cpu = time.perf_counter() - t0
-            num = np.abs(Sg - So).sum(axis=0)
-            den = np.abs(So).sum(axis=0) + 1e-300
-            out["rel_l1_vs_oracle_1step_32^3"] = float((num / den).max())
-            out["cpu_value_1core"] = round(m.n * 3 / t_cpu / 1e6, 4)
-            out["cpu_sample"] = f"32^3 config-2 state, 1 step (3 stages), {t_cpu:.1f} s, 1 thread, gcc -O2"
-        dev.close()
-    out["workload"] = ("config-2 materials (solid slab, water, air bubble), uniform 128^3, PRELAX 3, von Mises "
-                       "yield 0.3 GPa, SSP-RK3, "
-                       "dt = 1 % of the CFL-0.4 step on the solid sound speed, each timed step from the initial state")
+    t0 = time.perf_counter()
+    p, m, S, dev, crit = build_c2(maxlevel, base, layers, log_fn)
+    setup = time.perf_counter() - t0
+    out = {"workload": "c2_bubble_near_elastoplastic_slab_amr_L5-9", "levels": "5-9 (config 4-8)",
+           "interface_layers": layers, "leaves_start": m.n, "setup_s": round(setup, 1),
+           "model": "model 5, fix = FV_M5_FIX_SIGNS | FV_M5_FIX_MIXTURE, PRELAX 3, von Mises 0.3 GPa, SSP-RK3, CFL 0.4"}
+    dev.advance(1)                       # settle the kernels before timing
+    dev.set_state(S)
+    r = c2_loop(dev, crit, maxlevel, steps, every, log_fn=log_fn)
+    Sg = dev.get_state()
+    dev.close()
+    B = algorithmic_bytes(F.m5_neq(3))
+    out.update({"steps": r["steps"], "leaves": r["leaves"], "ms_per_stage": round(r["ms_stage"], 3),
+                "ms_residual_kernel": round(r["ms_residual"], 3), "value": round(r["stage_rate"], 2),
+                "loop_value": round(r["loop_rate"], 2), "finite": bool(np.all(np.isfinite(Sg))),
+                "unit": "Mcell-updates/s per RK stage; loop_value with CFL, interface_location and regrids"})
+    out["roofline"] = {"bound": "hbm", "algorithmic_bytes_per_leaf": B, "peak": HBM_PEAK_GBPS,
+                       "achieved": round(B * r["stage_rate"] * 1e6 / 1e9, 1)}
+    out["roofline"]["frac"] = round(out["roofline"]["achieved"] / HBM_PEAK_GBPS, 4)
+    log_fn(f"[c2] {r['leaves']} leaves: {r['ms_stage']:.3f} ms/stage, residual {r['ms_residual']:.3f} ms, "
+           f"{r['stage_rate']:.1f} Mcell-upd/s, loop {r['loop_rate']:.1f}")
+    # mixed precision against FP64 on the fixed start mesh, the same CFL step, 10 steps
+    mesh_res = {}
+    for prec in (F.PREC_FP64, F.PREC_MIXED):
+        d = F.M5Device(p, m)
+        d.set_precision(prec)
+        d.set_state(S)
+        p.dt = d.cfl_dt(0.4)
+        d.set_dt(p.dt)
+        d.advance(1)
+        d.set_state(S)
+        ms = [d.time_steps(1)[0] for _ in range(10)]
+        mesh_res[prec] = (d.get_state(), float(np.mean(ms)))
+        d.close()
+    S64, Smx = mesh_res[F.PREC_FP64][0], mesh_res[F.PREC_MIXED][0]
+    nm = 3
+    names = [f"alpha_rho_{k}" for k in range(nm)] + ["rho_u", "rho_v", "rho_w", "rho_E"] + \
+        [f"alpha_{k}" for k in range(nm - 1)] + [f"alpha_rho_e_{k}" for k in range(nm)] + \
+        [f"A_{a}{b}" for a in range(3) for b in range(3)]
+    scale = np.abs(S64).sum(0)
+    scale[nm:nm + 3] = np.sqrt((S64[:, nm:nm + 3] ** 2).sum(1)).sum()
+    scale[-9:] = np.sqrt((S64[:, -9:] ** 2).sum(1)).sum()
+    l1 = np.abs(Smx - S64).sum(0) / np.maximum(scale, 1e-300)
+    out["mixed"] = {"ms_per_stage": round(mesh_res[F.PREC_MIXED][1], 3),
+                    "ms_per_stage_fp64_same_mesh": round(mesh_res[F.PREC_FP64][1], 3),
+                    "speedup_vs_fp64_stage": round(mesh_res[F.PREC_FP64][1] / mesh_res[F.PREC_MIXED][1], 3),
+                    "rel_l1_vs_fp64_10steps": {k: float(v) for k, v in zip(names, l1)},
+                    "finite": bool(np.all(np.isfinite(Smx)))}
+    log_fn(f"[c2] mixed: {mesh_res[F.PREC_MIXED][1]:.3f} ms/stage against {mesh_res[F.PREC_FP64][1]:.3f}, "
+           f"max field L1 vs FP64 after 10 steps {l1.max():.2e} ({names[int(l1.argmax())]})")
     if check:
+        q = c2_params()
+        mm = F.Mesh.uniform((1, 1, 1), (0, 0, 0), L_DOMAIN, 6, [2] * 6)
+        S0 = c2_state(q, mm)
+        om = O.OracleMesh(mm)
+        q.dt = om.m5_cfl_dt(q, S0, 0.4)
+        d = F.M5Device(q, mm)
+        d.set_state(S0)
+        d.advance(1)
+        Sd = d.get_state()
+        d.close()
+        t1 = time.perf_counter()
+        So = om.m5_advance(q, S0, 1)
+        t_cpu = time.perf_counter() - t1
+        num = np.abs(Sd - So).sum(axis=0)
+        den = np.abs(So).sum(axis=0)
+        den[nm:nm + 3] = np.sqrt((So[:, nm:nm + 3] ** 2).sum(1)).sum()
+        den[-9:] = np.sqrt((So[:, -9:] ** 2).sum(1)).sum()
+        out["rel_l1_vs_oracle_1step_32^3"] = float((num / np.maximum(den, 1e-300)).max())
+        out["cpu_value_1core"] = round(mm.n * 3 / t_cpu / 1e6, 4)
+        out["cpu_sample"] = f"32^3 config-2 state, 1 step (3 stages), {t_cpu:.1f} s, 1 thread, gcc -O2"
         out["speedup_vs_cpu_1core"] = round(out["value"] / out["cpu_value_1core"], 1)
     return out
+
+
+def c2_collapse(maxlevel=8, base=5, layers=64, every=5, t_over_rayleigh=1.6, log_fn=log):
+    """Config 2 run past the bubble's collapse on reference levels 5-8: the model-5 loop as in
+    c2_leg, with the gas volume every 50 steps; the collapse is the first minimum of the gas volume."""
+    p, m, S, dev, crit = build_c2(maxlevel, base, layers, log_fn)
+    t_r = 0.915 * R_BUBBLE * np.sqrt(1000.0 / (C2_P[0] - C2_P[1]))
+    v0 = gas_volume(S, m.arrays()["level"])
+    r = c2_loop(dev, crit, maxlevel, 10 ** 7, every, t_end=t_over_rayleigh * t_r, probe=50, log_fn=log_fn)
+    dev.close()
+    ts = np.array([v[0] for v in r["vols"]])
+    vs = np.array([v[1] for v in r["vols"]])
+    k = int(np.argmax((vs[1:-1] <= vs[:-2]) & (vs[1:-1] < vs[2:]))) + 1 if len(vs) > 2 else int(np.argmin(vs))
+    return {"workload": f"c2_collapse_amr_L{base}-{maxlevel}", "steps": r["steps"], "t_end_s": r["t"],
+            "rayleigh_time_s": t_r, "t_collapse_over_rayleigh": float(ts[k] / t_r),
+            "gas_volume_min_over_v0": float(vs[k] / v0), "R_min_over_R0": float((vs[k] / v0) ** (1.0 / 3.0)),
+            "leaves_start": r["leaves"][0], "leaves_min": int(min(r["leaves"])), "leaves_max": int(max(r["leaves"])),
+            "regrids": len(r["leaves"]) - 1, "stage_rate": round(r["stage_rate"], 1),
+            "loop_rate": round(r["loop_rate"], 1), "wall_s": round(r["wall_s"], 1),
+            "gas_volume_over_v0": [[round(float(t / t_r), 4), round(float(v / v0), 4)] for t, v in zip(ts, vs)]}
 
 
 def uniform_leg(level=9, warmup=3, steps=3, log_fn=log):
@@ -655,9 +719,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed SSP-RK3 steps (3 stages each)")
     ap.add_argument("--warmup", type=int, default=3, help="untimed SSP-RK3 steps before the timed ones")
-    ap.add_argument("--workload", default="c1", choices=["c1", "u64", "u128", "u256", "u320", "c2", "c3"],
-                    help="c2: the model-5 stage alone (config-2 materials, uniform 128^3); c3: config 3, the "
-                         "bubble adapted to levels 5-10, FP64 against mixed precision")
+    ap.add_argument("--workload", default="c1",
+                    choices=["c1", "u64", "u128", "u256", "u320", "c2", "c2_collapse", "c3"],
+                    help="c2: config 2 (model 5, the bubble near an elastoplastic slab, AMR levels 5-9); "
+                         "c2_collapse: config 2 on levels 5-8 past the collapse; c3: config 3, the bubble adapted "
+                         "to levels 5-10, FP64 against mixed precision")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true")
     ap.add_argument("--layers", type=int, default=64)
@@ -666,7 +732,7 @@ def main():
     ap.add_argument("--no-mlp", action="store_true", help="skip the FP32 + learned-indicator leg")
     ap.add_argument("--no-regrid", action="store_true", help="skip the device / host regrid leg")
     ap.add_argument("--no-collapse", action="store_true", help="skip the run to the first collapse")
-    ap.add_argument("--no-m5", action="store_true", help="skip the model-5 (solid, GFM) stage leg")
+    ap.add_argument("--no-m5", action="store_true", help="skip the config-2 leg (model 5: solid, GFM, AMR)")
     ap.add_argument("--no-u256", action="store_true", help="skip the uniform 256^3 block-kernel roofline leg")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -675,14 +741,17 @@ def main():
         sys.exit(2)
 
     if a.workload == "c2":
-        leg = m5_leg(check=not a.no_cpu_baseline)
+        leg = c2_leg(steps=a.steps, check=not a.no_cpu_baseline)
         print(json.dumps({"metric": "Mcell-updates/s per RK stage (model 5: solid, GFM)", "value": leg["value"],
-                          "unit": "Mcell-updates/s per RK stage", "n_gpus": 1, "steps": 3 * 3, "warmup": 3,
+                          "unit": "Mcell-updates/s per RK stage", "n_gpus": 1, "steps": leg["steps"], "warmup": 1,
                           "ms_per_step": round(3 * leg["ms_per_stage"], 4), "ms_per_stage": leg["ms_per_stage"],
                           "higher_is_better": True, "scaling": "weak",
                           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-                          "config": {"workload": leg["workload"], "leaves": leg["leaves"]}, "m5_stage": leg}),
+                          "config": {"workload": leg["workload"], "leaves": leg["leaves"]}, "c2": leg}),
               flush=True)
+        return
+    if a.workload == "c2_collapse":
+        print(json.dumps(c2_collapse()), flush=True)
         return
 
     if a.workload == "c3":
@@ -763,7 +832,7 @@ def main():
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_collapse:
         out["collapse"] = collapse_leg(p, a.layers)
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_m5:
-        out["m5_stage"] = m5_leg()
+        out["c2"] = c2_leg()
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_u256:
         out["u256"] = uniform_leg(9)
     if not a.no_cpu_baseline:

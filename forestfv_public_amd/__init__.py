@@ -267,6 +267,11 @@ def lib():
             "fv_m5_dev_time_steps": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
             "fv_m5_dev_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
             "fv_m5_dev_set_dt": (C.c_int, [C.c_void_p, C.c_double]),
+            "fv_m5_dev_set_forest": (C.c_int, [C.c_void_p, P(fv_mesh)]),
+            "fv_m5_dev_get_forest": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]),
+            "fv_m5_dev_interface_location": (C.c_int, [C.c_void_p, P(fv_crit), P(C.c_int)]),
+            "fv_m5_dev_indicator": (C.c_int, [C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int)]),
+            "fv_m5_dev_regrid": (C.c_int, [C.c_void_p, P(fv_crit), C.c_int, P(C.c_int)]),
             "fv_m5_dev_cfl_dt": (C.c_int, [C.c_void_p, C.c_double, P(C.c_double)]),
             "fv_m5_init_slab_bubble": (C.c_int, [P(fv_mesh), P(fv_m5_params), P(C.c_double), C.c_double,
                                                  P(C.c_double), C.c_double, C.c_double, C.c_double, C.c_double,
@@ -456,6 +461,39 @@ class M5Device:
     def set_dt(self, dt):
         _check(lib().fv_m5_dev_set_dt(self.h, dt), "fv_m5_dev_set_dt")
         self.params.dt = dt
+
+    def set_forest(self, mesh):
+        """Leaf addresses of the attached mesh, for interface_location and regrids on the device."""
+        _check(lib().fv_m5_dev_set_forest(self.h, mesh._p), "fv_m5_dev_set_forest")
+
+    def get_forest(self):
+        """(root [n], level [n], ci [n][3], SI [n][2]) of the device mesh."""
+        root = np.zeros(self.n, np.int32)
+        level = np.zeros(self.n, np.int32)
+        ci = np.zeros((self.n, 3), np.int32)
+        SI = np.zeros((self.n, 2), np.int32)
+        _check(lib().fv_m5_dev_get_forest(self.h, _ip(root), _ip(level), _ip(ci), _ip(SI)), "fv_m5_dev_get_forest")
+        return root, level, ci, SI
+
+    def interface_location(self, crit):
+        """interface_location after a step: SI of the device state, face kinds refreshed; SI [n][2]."""
+        SI = np.zeros((self.n, 2), np.int32)
+        _check(lib().fv_m5_dev_interface_location(self.h, C.byref(crit), _ip(SI)), "fv_m5_dev_interface_location")
+        return SI
+
+    def indicator(self, crit):
+        """(SI [n][2], target levels [n]) of the device state; the mesh is unchanged."""
+        SI = np.zeros((self.n, 2), np.int32)
+        split = np.zeros(self.n, np.int32)
+        _check(lib().fv_m5_dev_indicator(self.h, C.byref(crit), _ip(SI), _ip(split)), "fv_m5_dev_indicator")
+        return SI, split
+
+    def regrid(self, crit, maxlevel):
+        """meshadaptation on the device; returns the new leaf count."""
+        n = C.c_int()
+        _check(lib().fv_m5_dev_regrid(self.h, C.byref(crit), maxlevel, C.byref(n)), "fv_m5_dev_regrid")
+        self.n = n.value
+        return self.n
 
     def cfl_dt(self, cfl):
         """cfl * min h / (max|u_i| + c) over the device state, c the mixture's longitudinal speed."""

@@ -1653,6 +1653,8 @@ int fv_dev_reserve(fv_dev *c, int n_leaves, int nfine) {
   };
   const int nw = (int)(sizeof want / sizeof want[0]);
   for (int k = 0; k < nw; k++) {
+    // a topology-only context (neq = 0, fv_dev_attach_topology) holds no state arrays
+    if (want[k].bytes == 0) { *want[k].p = nullptr; continue; }
     const hipError_t e = hipMalloc(want[k].p, want[k].bytes);
     if (e != hipSuccess) {
       for (int j = 0; j < k; j++) hipFree(*want[j].p);
@@ -1710,6 +1712,23 @@ extern "C" int fv_dev_attach(fv_dev **out, const fv_params *p, const fv_pack *pk
   c->nmat = p->nmat;
   c->neq = fv_neq(p->nmat);
   to_dev_params(p, c->P);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return set_err(FV_ERR_HIP, "stream creation failed");
+  }
+  int rc = upload_mesh(c, pk);
+  if (rc != FV_OK) { free_buffers(c); hipFree(c->Gdev); hipStreamDestroy(c->stream); delete c; return rc; }
+  *out = c;
+  return FV_OK;
+}
+
+// a context that holds a mesh's topology, forest and indicator buffers but no
+// state (neq = 0): the mesh side of a model-5 context's regrid
+int fv_dev_attach_topology(fv_dev **out, const fv_pack *pk, int nmat) {
+  if (!out || !pk) return set_err(FV_ERR_ARG, "null argument");
+  fv_dev *c = new fv_dev();
+  c->nmat = nmat;
+  c->neq = 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return set_err(FV_ERR_HIP, "stream creation failed");
@@ -2231,17 +2250,15 @@ __global__ __launch_bounds__(FV_BLOCK) void k_si_bfs_final(int n, int dmax, cons
 // interface_location (interface_location.c:7-86): interface leaves, then
 // imax-2 layers grown one face neighbour at a time; *out is the buffer that
 // holds the final SI [n][2].
-static int launch_interface_location(fv_dev *c, const fv_crit *cr, int **out) {
+// layers 2 .. ADAPT_INTER_MAX - 1 of interface_location (interface_location.c:52-86)
+// from the first layer in c->SIa; *out: the buffer that holds every layer
+int fv_dev_si_layers(fv_dev *c, const fv_crit *cr, int **out) {
   const int n = c->n;
   const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
   int imax = 0;   // ADAPT_INTER_MAX (m4_input.c:52-54)
   if (cr->inter0_on == 1 && cr->inter1_on == 1) imax = cr->inter0_lrs + cr->inter1_lrs;
   else if (cr->inter0_on == 1 && cr->inter1_on == 0) imax = cr->inter0_lrs;
   int *cur = c->SIa, *nxt = c->SIb;
-  if (c->nmat == 2)
-    hipLaunchKernelGGL(k_si_first<2>, grid, block, 0, c->stream, n, c->P, c->S, c->nbr, c->info, c->fine4, cur);
-  else
-    hipLaunchKernelGGL(k_si_first<3>, grid, block, 0, c->stream, n, c->P, c->S, c->nbr, c->info, c->fine4, cur);
   if (c->nmat == 2 && imax > 2) {
     const int dmax = imax - 1;
     int *X = nxt;                                   // [n] layer per leaf, in the spare SI buffer
@@ -2264,6 +2281,20 @@ static int launch_interface_location(fv_dev *c, const fv_crit *cr, int **out) {
   HIPCHK(hipGetLastError());
   *out = cur;
   return FV_OK;
+}
+
+static int launch_interface_location(fv_dev *c, const fv_crit *cr, int **out) {
+  const int n = c->n;
+  const dim3 grid((n + FV_BLOCK - 1) / FV_BLOCK), block(FV_BLOCK);
+  if (c->nmat == 2)
+    hipLaunchKernelGGL(k_si_first<2>, grid, block, 0, c->stream, n, c->P, c->S, c->nbr, c->info, c->fine4, c->SIa);
+  else
+    hipLaunchKernelGGL(k_si_first<3>, grid, block, 0, c->stream, n, c->P, c->S, c->nbr, c->info, c->fine4, c->SIa);
+  return fv_dev_si_layers(c, cr, out);
+}
+
+void fv_dev_refresh_kinds_launch(hipStream_t st, int n, const int *nbr, const int *SI, uint32_t *info) {
+  hipLaunchKernelGGL(k_refresh_kinds, dim3((n + FV_BLOCK - 1) / FV_BLOCK), dim3(FV_BLOCK), 0, st, n, nbr, SI, info);
 }
 
 int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev) {

@@ -77,3 +77,32 @@ int fv_copy_to_host(fv_dev *c, void *dst, const void *src, size_t bytes);
 // launches the indicator kernels (interface layers, target level) and leaves
 // SI in *SIdev and the targets in c->split; defined in fv_device.hip
 int fv_dev_indicator_raw(fv_dev *c, const fv_crit *cr, int **SIdev);
+
+// layers 2 .. ADAPT_INTER_MAX - 1 of the interface flags from the first layer in
+// c->SIa; *out: the device buffer ([n][2]) that holds them all (fv_device.hip)
+int fv_dev_si_layers(fv_dev *c, const fv_crit *cr, int **out);
+
+// the face kinds of info rewritten for the interface flags SI (the mirror rule of
+// m4_residual.c:62-85 and m5_residual.c:62-103), on stream st
+void fv_dev_refresh_kinds_launch(hipStream_t st, int n, const int *nbr, const int *SI, uint32_t *info);
+
+// a context with the topology, forest and indicator buffers of pk and no state
+// (neq = 0); freed with fv_dev_detach
+int fv_dev_attach_topology(fv_dev **out, const fv_pack *pk, int nmat);
+
+// the state side of a regrid run by a topology context (fv_dev_regrid_hook):
+// remap launches, on stream st, the state of the new leaves from the merge/split
+// plan of the n old leaves (kind: 0 kept, 1 merged -- the family's first kid
+// writes the parent --, 2 split into eight; off: the first new leaf; ci: the old
+// leaves' octant addresses); install takes the new state once the new topology
+// is in the topology context c
+struct fv_state_hook {
+  void *ctx;
+  int (*remap)(void *ctx, hipStream_t st, int n, int nn, const uint8_t *kind, const int *off, const int *ci);
+  int (*install)(void *ctx, fv_dev *c);
+};
+
+// criterionsmth of the targets split (device [n], changed in place), merge/split
+// with the hook's state remap, connectivity and install; SI (device [n][2]) are
+// the interface flags that become the mesh's (fv_regrid.hip)
+int fv_dev_regrid_hook(fv_dev *c, int *split, const int *SI, int maxlevel, const fv_state_hook *h, int *n_out);

@@ -2281,6 +2281,12 @@ struct fv_m5_dev {
   uint8_t *mark6 = nullptr;   // [6n]
   int *list6 = nullptr;       // [6n]
   double *part = nullptr;     // [n/256 + 1] per-block minima of the CFL step, then the step
+  int cap = 0, cap_fine = 0;  // leaves and 1:4 rows the per-leaf buffers hold
+  // mesh side of regrids and interface_location (fv_m5_dev_set_forest): a
+  // topology-only model-4 context over the same leaves
+  fv_dev *topo = nullptr;
+  double *Snew = nullptr;     // [NEQ][nn] remapped state of a regrid, until installed
+  size_t cap_new = 0;
 };
 
 namespace {
@@ -2288,9 +2294,10 @@ namespace {
 void m5_free(fv_m5_dev *c) {
   void *ptrs[] = {c->G, c->nbr, c->fine4, c->info, c->S, c->S2, c->SN, c->SF, c->RHS, c->Sb,
                   c->gfm, c->mark, c->flag, c->list, c->count, c->cub_tmp, c->nst, c->iters, c->mark6, c->list6,
-                  c->part};
+                  c->part, c->Snew};
   for (void *q : ptrs)
     if (q) hipFree(q);
+  if (c->topo) fv_dev_detach(c->topo);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -2412,6 +2419,266 @@ int m5_check(fv_m5_dev *c) {
   return FV_OK;
 }
 
+
+// ---- mesh adaptation of the model-5 state ----------------------------------
+// The forest side (criterionsmth, merge/split plan, connectivity) runs in the
+// context's topology-only model-4 context; this side gives it the model-5
+// indicator and remaps the model-5 state.
+
+// interface_location.c:7-50 with m5_facevalues (m5_facevalues.c:82-98,
+// n_solids = 0): SI[0] for the leaves with a face across which the solid's
+// phi[0] = clipped alpha_0 - AVF_LIM changes sign, SI[1] for the other fractions
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_si_first(int n, M5P P, const double *__restrict__ S,
+                                                     const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+                                                     const int *__restrict__ fine4, int *__restrict__ SI) {
+  using L = L5<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = P.amin / 100.0;
+  const uint32_t inf = info[i];
+  auto phi = [&](int c, int v) {
+    double x = S[(size_t)(L::I3 + v) * n + c];
+    if (x >= (1.0 - a)) x = 1.0 - a;
+    else if (x <= a) x = a;
+    return x - P.avf_lim;
+  };
+  int si0 = 0, si1 = 0, flag = 0;
+  for (int f = 0; f < 6 && !flag; f++) {
+    const int kind = fvk::face_kind(inf, f);
+    if (kind == FV_FK_BOUNDARY) continue;
+    const int nb = nbr[(size_t)f * n + i];
+    const int ns = (kind == FV_FK_FINE) ? 4 : 1;
+    for (int k = 0; k < ns; k++) {
+      const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
+#pragma unroll
+      for (int v = 0; v < L::N3; ++v) {
+        if (phi(i, v) * phi(j, v) < 0.0) {
+          if (v == 0) si0 = 1; else si1 = 1;
+          flag = 1;
+        }
+      }
+    }
+  }
+  SI[2 * i] = si0;
+  SI[2 * i + 1] = si1;
+}
+
+struct M5Crit {
+  int uniform_on, uniform_lvl, prs_on, prs_lvl, rho_on, rho_lvl, vel_on, vel_lvl;
+  int inter0_on, inter0_lvl, inter0_lrs, inter1_on, inter1_lvl, inter1_lrs;
+  double prs_thr, rho_thr, vel_thr;
+};
+
+__device__ __forceinline__ double m5_rel_jump(double x_cur, double x_nb) {   // criterionsplt.c:81-88
+  const double eps = 1.0e-8;
+  if ((fabs(x_nb) > eps) || (fabs(x_cur) > eps)) return fabs(x_nb - x_cur) / maxr(fabs(x_nb), fabs(x_cur));
+  return 0.0;
+}
+
+// criterionsplt.c:3-37 with criterion_grad (:40-140) on m5_facevalues cell
+// records: the mixture pressure sum_k alpha_k p_hydro_k, the density, and the
+// velocity criterion that reads the density jump (:113); then the interface bands
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_split(int n, M5P P, M5Crit C, const double *__restrict__ S,
+                                                  const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+                                                  const int *__restrict__ fine4, const int *__restrict__ SI,
+                                                  int *__restrict__ split) {
+  using L = L5<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t inf = info[i];
+  int lt = 1;
+  if (C.uniform_on == 1) lt = max(lt, C.uniform_lvl);
+  if (C.prs_on == 1 || C.rho_on == 1 || C.vel_on == 1) {
+    double x[L::NEQ];
+#pragma unroll
+    for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + i];
+    Rec5<NM> sL;
+    record<NM>(P, x, sL);
+    double pl = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) pl += sL.avf[k] * sL.ph[k];
+    double gp = 0.0, gr = 0.0, gv = 0.0, grho = 0.0, lg = 1;
+    for (int f = 0; f < 6; f++) {
+      const int kind = fvk::face_kind(inf, f);
+      if (kind == FV_FK_BOUNDARY) continue;
+      const int nb = nbr[(size_t)f * n + i];
+      const int ns = (kind == FV_FK_FINE) ? 4 : 1;
+      for (int k = 0; k < ns; k++) {
+        const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
+#pragma unroll
+        for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + j];
+        Rec5<NM> sR;
+        record<NM>(P, x, sR);
+        if (C.prs_on == 1) {
+          double pr = 0.0;
+#pragma unroll
+          for (int q = 0; q < NM; ++q) pr += sR.avf[q] * sR.ph[q];
+          gp = maxr(m5_rel_jump(pl, pr), gp);
+        }
+        if (C.rho_on == 1) { grho = m5_rel_jump(sL.r, sR.r); gr = maxr(grho, gr); }
+        if (C.vel_on == 1) gv = maxr(grho, gv);
+      }
+    }
+    if (C.prs_on == 1 && fabs(gp) >= C.prs_thr) lg = C.prs_lvl;
+    if (C.rho_on == 1 && fabs(gr) >= C.rho_thr) lg = C.rho_lvl;
+    if (C.vel_on == 1 && fabs(gv) >= C.vel_thr) lg = C.vel_lvl;
+    lt = (int)maxr((double)lt, lg);
+  }
+  if (C.inter0_on != 0) {
+    for (int v = 0; v < 2; ++v) {
+      const int s = SI[2 * i + v];
+      if (s >= 1 && s < C.inter0_lrs) lt = max(lt, C.inter0_lvl);
+      if (s >= C.inter0_lrs && s < C.inter1_lrs) lt = max(lt, C.inter1_lvl);
+    }
+  }
+  split[i] = lt;
+}
+
+// the state of the new leaves (merge.c:20-59 with m5_relaxationvec; split.c:425-434):
+// a merged parent is the mean of its eight kids in kid order, relaxed; a split
+// leaf's kids copy it; a kept leaf moves to its new position
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_adapt_state(int n, int nn, M5P P, const uint8_t *__restrict__ kind,
+                                                        const int *__restrict__ off, const int *__restrict__ ci,
+                                                        const double *__restrict__ S, double *__restrict__ S2,
+                                                        int *__restrict__ flag) {
+  using L = L5<NM>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = off[i];
+  if (kind[i] == 1) {
+    const int kid = (ci[3 * i] & 1) + 2 * (ci[3 * i + 1] & 1) + 4 * (ci[3 * i + 2] & 1);
+    if (kid != 0) return;   // the first kid writes the parent
+    double d[L::NEQ];
+#pragma unroll
+    for (int iv = 0; iv < L::NEQ; iv++) {
+      double s = 0.0;
+      for (int k = 0; k < 8; k++) s += S[(size_t)iv * n + i + k] / ((double)8);
+      d[iv] = s;
+    }
+    const int rc = relax5<NM>(P, d);
+    if (rc == -3) atomicOr(flag, 4);
+    else if (rc != 0) atomicOr(flag, 1);
+#pragma unroll
+    for (int iv = 0; iv < L::NEQ; iv++) S2[(size_t)iv * nn + o] = d[iv];
+  } else {
+    const int cnt = (kind[i] == 2) ? 8 : 1;
+    for (int k = 0; k < cnt; k++)
+#pragma unroll
+      for (int iv = 0; iv < L::NEQ; iv++) S2[(size_t)iv * nn + o + k] = S[(size_t)iv * n + i];
+  }
+}
+
+M5Crit m5_crit(const fv_crit *cr) {
+  M5Crit C;
+  C.uniform_on = cr->uniform_on; C.uniform_lvl = cr->uniform_lvl;
+  C.prs_on = cr->prs_on; C.prs_lvl = cr->prs_lvl; C.prs_thr = cr->prs_thr;
+  C.rho_on = cr->rho_on; C.rho_lvl = cr->rho_lvl; C.rho_thr = cr->rho_thr;
+  C.vel_on = cr->vel_on; C.vel_lvl = cr->vel_lvl; C.vel_thr = cr->vel_thr;
+  C.inter0_on = cr->inter0_on; C.inter0_lvl = cr->inter0_lvl; C.inter0_lrs = cr->inter0_lrs;
+  C.inter1_on = cr->inter1_on; C.inter1_lvl = cr->inter1_lvl; C.inter1_lrs = cr->inter1_lrs;
+  return C;
+}
+
+// per-leaf buffers for n leaves and nfine 1:4 rows: a new set, swapped in only
+// when every allocation succeeded (contents are not kept)
+int m5_reserve(fv_m5_dev *c, int n_leaves, int nfine, int prelax3) {
+  if (n_leaves <= c->cap && nfine <= c->cap_fine) return FV_OK;
+  const size_t n = (size_t)n_leaves, nf = (size_t)(nfine > 0 ? nfine : 1), sz = n * c->neq;
+  fv_m5_dev t;
+  std::vector<void *> got;
+  hipError_t e = hipSuccess;
+  auto take = [&](hipError_t x, void *p) { if (e == hipSuccess) e = x; if (x == hipSuccess) got.push_back(p); };
+  take(dalloc(&t.nbr, 6 * n), t.nbr); take(dalloc(&t.fine4, 4 * nf), t.fine4); take(dalloc(&t.info, n), t.info);
+  take(dalloc(&t.S, sz), t.S); take(dalloc(&t.S2, sz), t.S2); take(dalloc(&t.SN, sz), t.SN);
+  take(dalloc(&t.Sb, sz), t.Sb); take(dalloc(&t.RHS, sz), t.RHS); take(dalloc(&t.SF, 6 * sz), t.SF);
+  take(dalloc(&t.gfm, 6 * n), t.gfm); take(dalloc(&t.mark, n), t.mark); take(dalloc(&t.list, n), t.list);
+  take(dalloc(&t.part, (n + 255) / 256 + 1), t.part);
+  if (prelax3) {
+    take(dalloc(&t.nst, 18 * n), t.nst); take(dalloc(&t.iters, 6 * n), t.iters);
+    take(dalloc(&t.mark6, 6 * n), t.mark6); take(dalloc(&t.list6, 6 * n), t.list6);
+  }
+  size_t cub = 0;
+  if (e == hipSuccess) {
+    hipcub::CountingInputIterator<int> idx(0);
+    const int items = prelax3 ? 6 * n_leaves : n_leaves;
+    e = hipcub::DeviceSelect::Flagged(nullptr, cub, idx, t.mark, t.list, c->count, items, c->stream);
+    if (e == hipSuccess) take(dalloc((char **)&t.cub_tmp, cub), t.cub_tmp);
+  }
+  if (e != hipSuccess) {
+    for (void *q : got) hipFree(q);
+    return fv_set_err(e == hipErrorOutOfMemory ? FV_ERR_NOMEM : FV_ERR_HIP, hipGetErrorString(e));
+  }
+  void *old[] = {c->nbr, c->fine4, c->info, c->S, c->S2, c->SN, c->Sb, c->RHS, c->SF, c->gfm, c->mark, c->list,
+                 c->part, c->nst, c->iters, c->mark6, c->list6, c->cub_tmp};
+  for (void *q : old)
+    if (q) hipFree(q);
+  c->nbr = t.nbr; c->fine4 = t.fine4; c->info = t.info; c->S = t.S; c->S2 = t.S2; c->SN = t.SN; c->Sb = t.Sb;
+  c->RHS = t.RHS; c->SF = t.SF; c->gfm = t.gfm; c->mark = t.mark; c->list = t.list; c->part = t.part;
+  c->nst = t.nst; c->iters = t.iters; c->mark6 = t.mark6; c->list6 = t.list6; c->cub_tmp = t.cub_tmp;
+  c->cub_bytes = cub;
+  t = fv_m5_dev();   // its pointers now belong to c
+  c->cap = n_leaves;
+  c->cap_fine = (int)nf;
+  return FV_OK;
+}
+
+int m5_remap_hook(void *ctx, hipStream_t st, int n, int nn, const uint8_t *kind, const int *off, const int *ci) {
+  fv_m5_dev *c = (fv_m5_dev *)ctx;
+  const size_t need = (size_t)nn * c->neq;
+  if (need > c->cap_new) {
+    if (c->Snew) hipFree(c->Snew);
+    c->Snew = nullptr;
+    c->cap_new = 0;
+    HIPCHK(dalloc(&c->Snew, need + need / 4));
+    c->cap_new = need + need / 4;
+  }
+  if (c->P.nmat == 2)
+    hipLaunchKernelGGL(k_m5_adapt_state<2>, dim3((n + 255) / 256), dim3(256), 0, st, n, nn, c->P, kind, off, ci, c->S,
+                       c->Snew, c->flag);
+  else
+    hipLaunchKernelGGL(k_m5_adapt_state<3>, dim3((n + 255) / 256), dim3(256), 0, st, n, nn, c->P, kind, off, ci, c->S,
+                       c->Snew, c->flag);
+  HIPCHK(hipGetLastError());
+  return FV_OK;
+}
+
+int m5_install_hook(void *ctx, fv_dev *t) {
+  fv_m5_dev *c = (fv_m5_dev *)ctx;
+  const int nn = t->n, nfine = t->nfine;
+  // headroom, so that a slowly growing mesh does not reallocate at every regrid
+  if (nn > c->cap || nfine > c->cap_fine) {
+    const int rc = m5_reserve(c, nn + nn / 4, nfine + nfine / 4, c->P.prelax == 3);
+    if (rc != FV_OK) return rc;
+  }
+  c->n = nn;
+  c->nfine = nfine;
+  const size_t n = (size_t)nn;
+  HIPCHK(hipMemcpyAsync(c->S, c->Snew, sizeof(double) * n * c->neq, hipMemcpyDeviceToDevice, t->stream));
+  HIPCHK(hipMemcpyAsync(c->nbr, t->nbr, sizeof(int) * 6 * n, hipMemcpyDeviceToDevice, t->stream));
+  HIPCHK(hipMemcpyAsync(c->info, t->info, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, t->stream));
+  if (nfine > 0)
+    HIPCHK(hipMemcpyAsync(c->fine4, t->fine4, sizeof(int) * 4 * (size_t)nfine, hipMemcpyDeviceToDevice, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return FV_OK;
+}
+
+// the model-5 interface flags of the current state into the topology context
+// (first layer here, the layers there); *SIdev: the device buffer holding them
+int m5_interface_layers(fv_m5_dev *c, const fv_crit *cr, int **SIdev) {
+  fv_dev *t = c->topo;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const dim3 g((c->n + 255) / 256), b(256);
+  if (c->P.nmat == 2)
+    hipLaunchKernelGGL(k_m5_si_first<2>, g, b, 0, t->stream, c->n, c->P, c->S, c->nbr, c->info, c->fine4, t->SIa);
+  else
+    hipLaunchKernelGGL(k_m5_si_first<3>, g, b, 0, t->stream, c->n, c->P, c->S, c->nbr, c->info, c->fine4, t->SIa);
+  HIPCHK(hipGetLastError());
+  return fv_dev_si_layers(t, cr, SIdev);
+}
+
 }  // namespace
 
 extern "C" int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_dev **out) {
@@ -2431,6 +2698,8 @@ extern "C" int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_
   c->neq = 3 * P.nmat + 12;
   c->rk_steps = p->rk_steps;
   const size_t n = (size_t)c->n, nf = (size_t)(pk->nfine > 0 ? pk->nfine : 1), sz = n * c->neq;
+  c->cap = c->n;
+  c->cap_fine = (int)nf;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   chk(hipStreamCreate(&c->stream));
@@ -2627,6 +2896,82 @@ extern "C" int fv_m5_dev_time_steps(fv_m5_dev *c, int nsteps, double *ms_stage, 
   return m5_check(c);
 }
 
+
+extern "C" int fv_m5_dev_set_forest(fv_m5_dev *c, const fv_mesh *m) {
+  if (!c || !m) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (m->n != c->n) return fv_set_err(FV_ERR_ARG, "fv_m5_dev_set_forest: mesh and context differ in leaf count");
+  fv_pack pk;
+  memset(&pk, 0, sizeof pk);
+  if (fv_mesh_pack(m, &pk) != FV_OK) return fv_set_err(FV_ERR_MESH, "fv_m5_dev_set_forest: packing the mesh failed");
+  if (c->topo) { fv_dev_detach(c->topo); c->topo = nullptr; }
+  int rc = fv_dev_attach_topology(&c->topo, &pk, c->P.nmat);
+  fv_pack_free(&pk);
+  if (rc != FV_OK) return rc;
+  if ((rc = fv_dev_set_forest(c->topo, m)) != FV_OK) { fv_dev_detach(c->topo); c->topo = nullptr; return rc; }
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_get_forest(fv_m5_dev *c, int *root, int *level, int *ci, int *SI) {
+  if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (!c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
+  return fv_dev_get_forest(c->topo, root, level, ci, SI);
+}
+
+extern "C" int fv_m5_dev_interface_location(fv_m5_dev *c, const fv_crit *cr, int *SI) {
+  if (!c || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (!c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
+  fv_dev *t = c->topo;
+  int *cur = nullptr;
+  int rc = m5_interface_layers(c, cr, &cur);
+  if (rc != FV_OK) return rc;
+  // the new flags become the mesh's, and the face kinds of both contexts follow them
+  fv_dev_refresh_kinds_launch(t->stream, c->n, c->nbr, cur, c->info);
+  fv_dev_refresh_kinds_launch(t->stream, t->n, t->nbr, cur, t->info);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(t->lSI, cur, sizeof(int) * 2 * (size_t)c->n, hipMemcpyDeviceToDevice, t->stream));
+  if (SI) HIPCHK(hipMemcpyAsync(SI, cur, sizeof(int) * 2 * (size_t)c->n, hipMemcpyDeviceToHost, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_indicator(fv_m5_dev *c, const fv_crit *cr, int *SI, int *split) {
+  if (!c || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (!c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
+  fv_dev *t = c->topo;
+  int *cur = nullptr;
+  int rc = m5_interface_layers(c, cr, &cur);
+  if (rc != FV_OK) return rc;
+  const dim3 g((c->n + 255) / 256), b(256);
+  const M5Crit C = m5_crit(cr);
+  if (c->P.nmat == 2)
+    hipLaunchKernelGGL(k_m5_split<2>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
+  else
+    hipLaunchKernelGGL(k_m5_split<3>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
+  HIPCHK(hipGetLastError());
+  if (SI) HIPCHK(hipMemcpyAsync(SI, cur, sizeof(int) * 2 * (size_t)c->n, hipMemcpyDeviceToHost, t->stream));
+  if (split) HIPCHK(hipMemcpyAsync(split, t->split, sizeof(int) * (size_t)c->n, hipMemcpyDeviceToHost, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_regrid(fv_m5_dev *c, const fv_crit *cr, int maxlevel, int *n_out) {
+  if (!c || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (!c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
+  fv_dev *t = c->topo;
+  int *cur = nullptr;
+  int rc = m5_interface_layers(c, cr, &cur);
+  if (rc != FV_OK) return rc;
+  const dim3 g((c->n + 255) / 256), b(256);
+  const M5Crit C = m5_crit(cr);
+  if (c->P.nmat == 2)
+    hipLaunchKernelGGL(k_m5_split<2>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
+  else
+    hipLaunchKernelGGL(k_m5_split<3>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
+  HIPCHK(hipGetLastError());
+  fv_state_hook h{c, m5_remap_hook, m5_install_hook};
+  if ((rc = fv_dev_regrid_hook(t, t->split, cur, maxlevel, &h, n_out)) != FV_OK) return rc;
+  return m5_check(c);   // a merged parent whose relaxation failed
+}
 
 extern "C" int fv_m5_neq(int nmat) { return 3 * nmat + 12; }
 

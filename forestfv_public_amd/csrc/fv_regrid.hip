@@ -215,6 +215,44 @@ __global__ __launch_bounds__(BLK) void k_adapt_emit(int n, int nn, Params P, con
   }
 }
 
+// the forest side of k_adapt_emit alone (a regrid whose state is remapped by a
+// fv_state_hook): addresses and interface flags of the new leaves
+__global__ __launch_bounds__(BLK) void k_adapt_forest(int n, const int *__restrict__ root,
+                                                      const int *__restrict__ level, const int *__restrict__ ci,
+                                                      const int *__restrict__ SIold, const uint8_t *__restrict__ kind,
+                                                      const int *__restrict__ off, int *__restrict__ root2,
+                                                      int *__restrict__ level2, int *__restrict__ ci2,
+                                                      int *__restrict__ SI2) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const int l = level[i], o = off[i];
+  if (kind[i] == 1) {
+    if (kid_of(ci, i) != 0) return;
+    root2[o] = root[i];
+    level2[o] = l - 1;
+    for (int a = 0; a < 3; a++) ci2[3 * o + a] = ci[3 * i + a] >> 1;
+    SI2[2 * o] = 0;
+    SI2[2 * o + 1] = 0;
+  } else if (kind[i] == 2) {
+    for (int k = 0; k < 8; k++) {
+      const int q = o + k;
+      root2[q] = root[i];
+      level2[q] = l + 1;
+      ci2[3 * q] = 2 * ci[3 * i] + (k & 1);
+      ci2[3 * q + 1] = 2 * ci[3 * i + 1] + ((k >> 1) & 1);
+      ci2[3 * q + 2] = 2 * ci[3 * i + 2] + ((k >> 2) & 1);
+      SI2[2 * q] = SIold[2 * i];
+      SI2[2 * q + 1] = 0;
+    }
+  } else {
+    root2[o] = root[i];
+    level2[o] = l;
+    for (int a = 0; a < 3; a++) ci2[3 * o + a] = ci[3 * i + a];
+    SI2[2 * o] = SIold[2 * i];
+    SI2[2 * o + 1] = SIold[2 * i + 1];
+  }
+}
+
 // ---- connectivity ----
 
 __device__ __forceinline__ uint64_t spread3(uint32_t x) {
@@ -446,7 +484,7 @@ int smooth(fv_dev *c, Tmp &T, int LEVEL, int *split) {
 
 // merge/split, connectivity and pack of the mesh of c with targets split (device [n])
 // and mesh interface flags SIsrc (device [n][2]); installs the new mesh and state in c
-int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
+int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc, const fv_state_hook *h = nullptr) {
   const int n = c->n, neq = c->neq;
   int *cnt, *off, *err;
   uint8_t *kind;
@@ -466,13 +504,22 @@ int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
   double *S2;
   HIPCHK(T.get(&root2, nn)); HIPCHK(T.get(&level2, nn));
   HIPCHK(T.get(&ci2, 3 * (size_t)nn)); HIPCHK(T.get(&SI2, 2 * (size_t)nn));
-  HIPCHK(T.get(&S2, (size_t)nn * neq));
-  if (c->nmat == 2)
-    hipLaunchKernelGGL(k_adapt_emit<2>, grid_of(n), dim3(BLK), 0, c->stream, n, nn, c->P, c->lroot, c->llevel, c->lci,
-                       SIsrc, kind, off, c->S, root2, level2, ci2, SI2, S2);
-  else
-    hipLaunchKernelGGL(k_adapt_emit<3>, grid_of(n), dim3(BLK), 0, c->stream, n, nn, c->P, c->lroot, c->llevel, c->lci,
-                       SIsrc, kind, off, c->S, root2, level2, ci2, SI2, S2);
+  if (h) {
+    // the state is the hook's: the forest side here, the state remap there
+    S2 = nullptr;
+    hipLaunchKernelGGL(k_adapt_forest, grid_of(n), dim3(BLK), 0, c->stream, n, c->lroot, c->llevel, c->lci, SIsrc,
+                       kind, off, root2, level2, ci2, SI2);
+    HIPCHK(hipGetLastError());
+    if ((rc = h->remap(h->ctx, c->stream, n, nn, kind, off, c->lci)) != FV_OK) return rc;
+  } else {
+    HIPCHK(T.get(&S2, (size_t)nn * neq));
+    if (c->nmat == 2)
+      hipLaunchKernelGGL(k_adapt_emit<2>, grid_of(n), dim3(BLK), 0, c->stream, n, nn, c->P, c->lroot, c->llevel,
+                         c->lci, SIsrc, kind, off, c->S, root2, level2, ci2, SI2, S2);
+    else
+      hipLaunchKernelGGL(k_adapt_emit<3>, grid_of(n), dim3(BLK), 0, c->stream, n, nn, c->P, c->lroot, c->llevel,
+                         c->lci, SIsrc, kind, off, c->S, root2, level2, ci2, SI2, S2);
+  }
   HIPCHK(hipGetLastError());
   // connectivity of the new leaves
   Forest Fo;
@@ -504,7 +551,7 @@ int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
   c->brick_ok = 0;
   c->n = nn;
   c->nfine = (int)nfine;
-  HIPCHK(hipMemcpyAsync(c->S, S2, sizeof(double) * (size_t)nn * neq, hipMemcpyDeviceToDevice, c->stream));
+  if (!h) HIPCHK(hipMemcpyAsync(c->S, S2, sizeof(double) * (size_t)nn * neq, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->nbr, nbr2, sizeof(int) * (size_t)nn * 6, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->info, info2, sizeof(uint32_t) * nn, hipMemcpyDeviceToDevice, c->stream));
   if (nfine > 0)
@@ -523,6 +570,7 @@ int adapt_and_connect(fv_dev *c, Tmp &T, const int *split, const int *SIsrc) {
   c->cap_leaf = nn;
   T.keep(root2); T.keep(level2); T.keep(ci2); T.keep(SI2);
   c->broken = 0;
+  if (h && (rc = h->install(h->ctx, c)) != FV_OK) return rc;
   return FV_OK;
 }
 
@@ -621,6 +669,20 @@ extern "C" int fv_dev_regrid(fv_dev *c, const fv_crit *cr, int maxlevel, int *n_
   HIPCHK(hipMemcpyAsync(split, c->split, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
   if ((rc = smooth(c, T, maxlevel, split)) != FV_OK) return rc;
   if ((rc = adapt_and_connect(c, T, split, c->lSI)) != FV_OK) return rc;
+  if (n_out) *n_out = c->n;
+  return FV_OK;
+}
+
+int fv_dev_regrid_hook(fv_dev *c, int *split, const int *SI, int maxlevel, const fv_state_hook *h, int *n_out) {
+  FV_GUARD(c);
+  int rc = need_forest(c);
+  if (rc != FV_OK) return rc;
+  if (!split || !SI || !h) return fv_set_err(FV_ERR_ARG, "null argument");
+  Tmp T(c);
+  if (SI != c->lSI)
+    HIPCHK(hipMemcpyAsync(c->lSI, SI, sizeof(int) * 2 * (size_t)c->n, hipMemcpyDeviceToDevice, c->stream));
+  if ((rc = smooth(c, T, maxlevel, split)) != FV_OK) return rc;
+  if ((rc = adapt_and_connect(c, T, split, c->lSI, h)) != FV_OK) return rc;
   if (n_out) *n_out = c->n;
   return FV_OK;
 }

@@ -149,6 +149,28 @@ int fv_m5_dev_set_dt(fv_m5_dev *c, double dt);
  * (4/3) sum_k alpha_k mu_k / rho, the longitudinal sound speed of the mixture
  * (c2Y of m5_wave_speeds.c:20-40 is the hydrodynamic part). */
 int fv_m5_dev_cfl_dt(fv_m5_dev *c, double cfl, double *dt);
+
+/* ---- mesh adaptation of the model-5 context ----
+ * fv_m5_dev_set_forest gives the context the leaf addresses of the mesh it
+ * was attached with (m: the same leaves), after which:
+ *  - fv_m5_dev_interface_location is interface_location as loop() calls it
+ *    after every step (loop.c:23; interface_location.c:7-90 with
+ *    m5_facevalues): SI of the device state ([n][2] to SI when not NULL) and
+ *    the face kinds that follow it (the mirror rule of m5_residual.c:62-103);
+ *  - fv_m5_dev_indicator gives SI and the target levels (criterionsplt.c:3-140
+ *    on m5_facevalues records) without changing the mesh;
+ *  - fv_m5_dev_regrid is meshadaptation (meshadaptation.c:3-44) on the
+ *    device: indicator, criterionsmth (criterionsmth.c:13-56), merge with the
+ *    mean of the eight kids and m5_relaxationvec (merge.c:20-59), split with
+ *    copies (split.c:425-434), the new DFS order, connectivity and face kinds;
+ *    *n_out receives the new leaf count;
+ *  - fv_m5_dev_get_forest returns root [n], level [n], ci [n][3], SI [n][2]
+ *    of the device mesh (any may be NULL). */
+int fv_m5_dev_set_forest(fv_m5_dev *c, const fv_mesh *m);
+int fv_m5_dev_get_forest(fv_m5_dev *c, int *root, int *level, int *ci, int *SI);
+int fv_m5_dev_interface_location(fv_m5_dev *c, const fv_crit *cr, int *SI);
+int fv_m5_dev_indicator(fv_m5_dev *c, const fv_crit *cr, int *SI, int *split);
+int fv_m5_dev_regrid(fv_m5_dev *c, const fv_crit *cr, int maxlevel, int *n_out);
 /* the stage's pieces alone: face states of the current S (m5_faceinterp_V1),
  * and the residual of S and those face states (m5_residual) */
 int fv_m5_dev_faceinterp(fv_m5_dev *c);

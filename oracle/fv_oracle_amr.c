@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include "fv_oracle.h"
+#include "fv_oracle_m5.h"
 
 #define MAXI(a, b) ((a) > (b) ? (a) : (b))
 
@@ -111,10 +112,10 @@ int orc_criterionsmth(const orc_mesh *m, int LEVEL, int *split) {
  * SI[0] and split, with SI[1] = 0 (leafallocation.c:19).  Both passes keep
  * the DFS order, as the list surgery of merge() and split() does.
  * Outputs are sized by the caller for up to 8 n leaves; *nout the count. */
-int orc_refineh(const orc_mesh *m, const orc_params *p, const int *split, const double *S, int *nout,
-                int *root_o, int *level_o, int *ci_o, double *S_o, int *SI_o, int *split_o) {
+static int refineh_impl(const orc_mesh *m, int neq, int (*relax)(const void *, double *), const void *rp,
+                        const int *split, const double *S, int *nout, int *root_o, int *level_o, int *ci_o,
+                        double *S_o, int *SI_o, int *split_o) {
   const int n = m->n;
-  const int neq = orc_neq(p->nmat);
   int *fam = (int *)malloc(sizeof(int) * 8 * (size_t)(n > 0 ? n : 1));
   int *famn = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
   /* the list after the merge pass */
@@ -151,7 +152,7 @@ int orc_refineh(const orc_mesh *m, const orc_params *p, const int *split, const 
           d[iv] = 0.0;
           for (int q = 0; q < 8; q++) d[iv] += S[(size_t)(i + q) * neq + iv] / ((double)8);
         }
-        orc_relax(p, d);   /* m4_relaxationvec, whatever PRELAX is (merge.c:54-59) */
+        if (relax(rp, d) != 0) rc = -1;   /* m4/m5_relaxationvec, whatever PRELAX is (merge.c:54-59) */
         n1++;
         i += 8;
         merged = 1;
@@ -195,4 +196,21 @@ int orc_refineh(const orc_mesh *m, const orc_params *p, const int *split, const 
   *nout = (rc == 0) ? n2 : 0;
   free(fam); free(famn); free(r1); free(l1); free(c1); free(s1); free(si1); free(S1);
   return rc;
+}
+
+static int relax4(const void *p, double *d) {
+  orc_relax((const orc_params *)p, d);
+  return 0;
+}
+
+int orc_refineh(const orc_mesh *m, const orc_params *p, const int *split, const double *S, int *nout,
+                int *root_o, int *level_o, int *ci_o, double *S_o, int *SI_o, int *split_o) {
+  return refineh_impl(m, orc_neq(p->nmat), relax4, p, split, S, nout, root_o, level_o, ci_o, S_o, SI_o, split_o);
+}
+
+static int relax5(const void *p, double *d) { return orc_m5_relax((const orc_m5_params *)p, d); }
+
+int orc_m5_refineh(const orc_mesh *m, const orc_m5_params *p, const int *split, const double *S, int *nout,
+                   int *root_o, int *level_o, int *ci_o, double *S_o, int *SI_o, int *split_o) {
+  return refineh_impl(m, orc_m5_neq(p->nmat), relax5, p, split, S, nout, root_o, level_o, ci_o, S_o, SI_o, split_o);
 }

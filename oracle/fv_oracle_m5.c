@@ -1589,3 +1589,121 @@ double orc_m5_cfl_dt(const orc_mesh *m, const orc_cell *cells, const orc_m5_para
   }
   return cfl * dt;
 }
+
+/* ------------------------------------------------------------------ */
+/* indicator with MODEL 5 (interface_location.c, criterionsplt.c)      */
+/* ------------------------------------------------------------------ */
+/* phi[v] of a cell record (m5_facevalues.c:82-98 with n_solids = 0): the
+ * clipped cell fraction minus AVF_LIM */
+static double m5_phi(const lay5_t *L, const orc_m5_params *p, const double *S, int v) {
+  const double a = p->amin / 100.0;
+  double c = S[L->i3 + v];
+  if (c >= (1.0 - a)) c = 1.0 - a;
+  else if (c <= a) c = a;
+  return c - p->avf_lim;
+}
+
+static int m5_inter_max(const orc_crit *c) {   /* m5_input.c: ADAPT_INTER_MAX as in model 4 */
+  if (c->inter0_on == 1 && c->inter1_on == 1) return c->inter0_lrs + c->inter1_lrs;
+  if (c->inter0_on == 1 && c->inter1_on == 0) return c->inter0_lrs;
+  return 0;
+}
+
+/* interface_location.c:7-90 with m5_facevalues: SI[0] marks the leaves with
+ * a face across which phi[0] (the solid) changes sign, SI[1] those of the
+ * other fractions, then the layers 2 .. ADAPT_INTER_MAX-1 in leaf order */
+int orc_m5_interface_location(const orc_mesh *m, const orc_m5_params *p, const orc_crit *c, const double *S,
+                              int *SI) {
+  lay5_t L = mk_lay5(p->nmat);
+  const size_t neq = (size_t)L.neq;
+  for (int i = 0; i < m->n; i++) {
+    SI[2 * i] = 0; SI[2 * i + 1] = 0;
+    int flag = 0;
+    for (int f = 0; f < 6 && !flag; f++) {
+      if (m->bc[6 * i + f] != 0) continue;
+      const int lfc = m->lfc_neigs[6 * i + f];
+      const int nsfc = lfc > 1 ? lfc : 1;
+      for (int k = 0; k < nsfc; k++) {
+        const int j = m->neigtr[24 * i + 4 * f + k];
+        for (int v = 0; v < L.n3; ++v) {
+          if (m5_phi(&L, p, &S[i * neq], v) * m5_phi(&L, p, &S[j * neq], v) < 0.0) {
+            if (v == 0) SI[2 * i] = 1; else SI[2 * i + 1] = 1;
+            flag = 1;
+          }
+        }
+      }
+    }
+  }
+  for (int lvl = 2; lvl < m5_inter_max(c); lvl++) {
+    for (int i = 0; i < m->n; i++) {
+      int flag = 0;
+      for (int f = 0; f < 6 && !flag; f++) {
+        if (m->bc[6 * i + f] != 0) continue;
+        const int lfc = m->lfc_neigs[6 * i + f];
+        const int nsfc = lfc > 1 ? lfc : 1;
+        for (int k = 0; k < nsfc; k++) {
+          const int j = m->neigtr[24 * i + 4 * f + k];
+          for (int v = 0; v < 2; ++v)
+            if (SI[2 * j + v] == (lvl - 1) && SI[2 * i + v] == 0) { SI[2 * i + v] = lvl; flag = 1; }
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+static double m5_rel_jump(double x_cur, double x_nb) {   /* criterionsplt.c:81-88 */
+  const double eps = 1.0e-8;
+  if ((fabs(x_nb) > eps) || (fabs(x_cur) > eps)) return fabs(x_nb - x_cur) / MAXF(fabs(x_nb), fabs(x_cur));
+  return 0.0;
+}
+
+/* criterionsplt.c:3-37 with criterion_grad (:40-140) on m5_facevalues cell
+ * records: the mixture pressure sum_k alpha_k p_hydro_k, the density, and the
+ * velocity criterion that reads the density jump (:113) */
+int orc_m5_criterionsplt(const orc_mesh *m, const orc_m5_params *p, const orc_crit *c, const double *S,
+                         const int *SI, int *split) {
+  lay5_t L = mk_lay5(p->nmat);
+  const size_t neq = (size_t)L.neq;
+  for (int i = 0; i < m->n; i++) {
+    int lt = 1;
+    if (c->uniform_on == 1) lt = MAXF(lt, c->uniform_lvl);
+    if (c->prs_on == 1 || c->rho_on == 1 || c->vel_on == 1) {
+      double gp = 0.0, gr = 0.0, gv = 0.0, grho = 0.0, lg = 1;
+      for (int f = 0; f < 6; f++) {
+        if (m->bc[6 * i + f] != 0) continue;
+        const int lfc = m->lfc_neigs[6 * i + f];
+        const int nsfc = lfc > 1 ? lfc : 1;
+        for (int k = 0; k < nsfc; k++) {
+          const int j = m->neigtr[24 * i + 4 * f + k];
+          rec5_t sL, sR;
+          m5_record(&L, p, &S[i * neq], &sL);
+          m5_record(&L, p, &S[j * neq], &sR);
+          if (c->prs_on == 1) {
+            double pl = 0.0, pr = 0.0;
+            for (int iv = 0; iv < L.nmat; ++iv) { pl += sL.avf[iv] * sL.p_hydro[iv]; pr += sR.avf[iv] * sR.p_hydro[iv]; }
+            gp = MAXF(m5_rel_jump(pl, pr), gp);
+          }
+          if (c->rho_on == 1) {
+            grho = m5_rel_jump(sL.r, sR.r);
+            gr = MAXF(grho, gr);
+          }
+          if (c->vel_on == 1) gv = MAXF(grho, gv);
+        }
+      }
+      if (c->prs_on == 1 && fabs(gp) >= c->prs_thr) lg = c->prs_lvl;
+      if (c->rho_on == 1 && fabs(gr) >= c->rho_thr) lg = c->rho_lvl;
+      if (c->vel_on == 1 && fabs(gv) >= c->vel_thr) lg = c->vel_lvl;
+      lt = (int)MAXF((double)lt, lg);
+    }
+    if (c->inter0_on != 0) {
+      for (int v = 0; v < 2; ++v) {
+        const int s = SI[2 * i + v];
+        if (s >= 1 && s < c->inter0_lrs) lt = MAXF(lt, c->inter0_lvl);
+        if (s >= c->inter0_lrs && s < c->inter1_lrs) lt = MAXF(lt, c->inter1_lvl);
+      }
+    }
+    split[i] = lt;
+  }
+  return 0;
+}

@@ -153,6 +153,18 @@ int orc_m5_plastic(const orc_m5_params *p, double *vec);
 /* nsteps explicit steps: SN = S (RK.c case 0), then stages 1..rk_steps. */
 int orc_m5_advance(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, double *S, int nsteps);
 
+/* Indicator with MODEL 5: interface_location.c:7-90 and criterionsplt.c:3-140
+ * on m5_facevalues cell records (phi[v] = clipped alpha_v - AVF_LIM; the
+ * mixture pressure of the elastic EOS).  SI [n][2] and split [n] out. */
+int orc_m5_interface_location(const orc_mesh *m, const orc_m5_params *p, const orc_crit *c, const double *S,
+                              int *SI);
+int orc_m5_criterionsplt(const orc_mesh *m, const orc_m5_params *p, const orc_crit *c, const double *S,
+                         const int *SI, int *split);
+/* refineh with MODEL 5: orc_refineh's merge and split passes on NEQ = 3 nmat
+ * + 12 states, a merged parent relaxed with m5_relaxationvec (merge.c:54-59). */
+int orc_m5_refineh(const orc_mesh *m, const orc_m5_params *p, const int *split, const double *S, int *nout,
+                   int *root_o, int *level_o, int *ci_o, double *S_o, int *SI_o, int *split_o);
+
 /* CFL step (new): cfl * min h / (max|u_i| + c), c^2 = sum_k c2Y_k + (4/3)
  * sum_k alpha_k mu_k / rho, the longitudinal sound speed of the mixture. */
 double orc_m5_cfl_dt(const orc_mesh *m, const orc_cell *cells, const orc_m5_params *p, const double *S, double cfl);

@@ -154,6 +154,9 @@ def lib(variant="O2"):
         ip = P(C.c_int)
         L.orc_criterionsmth.argtypes = [P(orc_mesh), C.c_int, ip]
         L.orc_refineh.argtypes = [P(orc_mesh), P(orc_params), ip, dp, ip, ip, ip, ip, dp, ip, ip]
+        L.orc_m5_refineh.argtypes = [P(orc_mesh), P(orc_m5_params), ip, dp, ip, ip, ip, ip, dp, ip, ip]
+        L.orc_m5_interface_location.argtypes = [P(orc_mesh), P(orc_m5_params), P(orc_crit), dp, ip]
+        L.orc_m5_criterionsplt.argtypes = [P(orc_mesh), P(orc_m5_params), P(orc_crit), dp, ip, ip]
         L.orc_bc_ghost.argtypes = [P(orc_params), dp, dp, dp, dp, C.c_int, dp, dp]
         L.orc_bc_ghost.restype = None
         _libs[variant] = L
@@ -356,6 +359,35 @@ class OracleMesh:
                                _ip(out["split"]))
         if rc != 0:
             raise RuntimeError(f"orc_refineh status {rc}")
+        return {k: v[:nout.value].copy() for k, v in out.items()}
+
+    def m5_interface_location(self, m5p, crit, S):
+        SI = np.zeros((self.n, 2), dtype=np.int32)
+        self.L.orc_m5_interface_location(C.byref(self.om), C.byref(m5_from(m5p)), C.byref(crit_from(crit)),
+                                        _dp(np.ascontiguousarray(S, dtype=np.float64)), _ip(SI))
+        return SI
+
+    def m5_criterionsplt(self, m5p, crit, S, SI):
+        split = np.zeros(self.n, dtype=np.int32)
+        SI = np.ascontiguousarray(SI, dtype=np.int32)
+        self.L.orc_m5_criterionsplt(C.byref(self.om), C.byref(m5_from(m5p)), C.byref(crit_from(crit)),
+                                   _dp(np.ascontiguousarray(S, dtype=np.float64)), _ip(SI), _ip(split))
+        return split
+
+    def m5_refineh(self, m5p, split, S):
+        """orc_m5_refineh: dict of the new leaves (root, level, ci, S, SI, split) in DFS order."""
+        nq = 3 * m5p.nmat + 12
+        cap = 8 * self.n
+        sp = np.ascontiguousarray(split, dtype=np.int32)
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        out = {"root": np.zeros(cap, np.int32), "level": np.zeros(cap, np.int32), "ci": np.zeros((cap, 3), np.int32),
+               "S": np.zeros((cap, nq)), "SI": np.zeros((cap, 2), np.int32), "split": np.zeros(cap, np.int32)}
+        nout = C.c_int()
+        rc = self.L.orc_m5_refineh(C.byref(self.om), C.byref(m5_from(m5p)), _ip(sp), _dp(S), C.byref(nout),
+                                  _ip(out["root"]), _ip(out["level"]), _ip(out["ci"]), _dp(out["S"]), _ip(out["SI"]),
+                                  _ip(out["split"]))
+        if rc != 0:
+            raise RuntimeError(f"orc_m5_refineh status {rc}")
         return {k: v[:nout.value].copy() for k, v in out.items()}
 
     def cfl_dt(self, params, S, cfl):

@@ -245,8 +245,8 @@ def c2_mesh(level):
     return F.Mesh.uniform((1, 1, 1), (0, 0, 0), L_BOX, level, [2] * 6)
 
 
-def c2_init(p, m, z_slab=0.25 * L_BOX, standoff=1.2, p_liquid=1e7):
-    centre = (0.5 * L_BOX, 0.5 * L_BOX, z_slab + standoff * R_BUB)
+def c2_init(p, m, z_slab=0.25 * L_BOX, standoff=1.2, p_liquid=1e7, dx=0.0):
+    centre = (0.5 * L_BOX + dx, 0.5 * L_BOX, z_slab + standoff * R_BUB)
     return m.init_m5_slab_bubble(p, RHO, z_slab, centre, R_BUB, p_liquid, 1e5)
 
 
