@@ -135,6 +135,7 @@ class fv_m5_params(C.Structure):
         ("rk_steps", C.c_int),
         ("yield_", C.c_double),
         ("fix", C.c_int),
+        ("limiter", C.c_int),
     ]
 
 
@@ -267,11 +268,17 @@ def lib():
             "fv_m5_dev_time_steps": (C.c_int, [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]),
             "fv_m5_dev_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
             "fv_m5_dev_set_dt": (C.c_int, [C.c_void_p, C.c_double]),
+            "fv_m5_dev_set_limiter_flags": (C.c_int, [C.c_void_p, P(C.c_uint8)]),
+            "fv_m5_dev_get_limiter_flags": (C.c_int, [C.c_void_p, P(C.c_uint8)]),
             "fv_m5_dev_set_forest": (C.c_int, [C.c_void_p, P(fv_mesh)]),
             "fv_m5_dev_get_forest": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]),
             "fv_m5_dev_interface_location": (C.c_int, [C.c_void_p, P(fv_crit), P(C.c_int)]),
             "fv_m5_dev_indicator": (C.c_int, [C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int)]),
             "fv_m5_dev_regrid": (C.c_int, [C.c_void_p, P(fv_crit), C.c_int, P(C.c_int)]),
+            "fv_mlp_sample_m5": (C.c_int, [C.c_void_p, C.c_void_p, P(fv_crit)]),
+            "fv_m5_dev_indicator_mlp": (C.c_int, [C.c_void_p, C.c_void_p, P(fv_crit), P(C.c_int), P(C.c_int),
+                                                  P(C.c_uint8)]),
+            "fv_m5_dev_regrid_mlp": (C.c_int, [C.c_void_p, C.c_void_p, P(fv_crit), C.c_int, P(C.c_int)]),
             "fv_m5_dev_cfl_dt": (C.c_int, [C.c_void_p, C.c_double, P(C.c_double)]),
             "fv_m5_init_slab_bubble": (C.c_int, [P(fv_mesh), P(fv_m5_params), P(C.c_double), C.c_double,
                                                  P(C.c_double), C.c_double, C.c_double, C.c_double, C.c_double,
@@ -362,7 +369,8 @@ M5_FIX_MIXTURE = 2
 
 
 def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush=(2.6e10, 0.0, 0.0), amin=1e-6,
-                   avf_lim=0.5, dt=1e-9, zerou=(0, 0, 0), prelax=1, order=2, rk_steps=3, yield_stress=0.0, fix=0):
+                   avf_lim=0.5, dt=1e-9, zerou=(0, 0, 0), prelax=1, order=2, rk_steps=3, yield_stress=0.0, fix=0,
+                   limiter=0):
     """Model-5 material table (MATERGAMA, MATERPINF, MATERMUSH) and stage constants."""
     p = fv_m5_params()
     p.nmat = nmat
@@ -374,6 +382,7 @@ def make_m5_params(nmat=3, gamma=(4.4, 6.12, 1.4), pinf=(6e8, 3.43e8, 0.0), mush
     p.prelax, p.order, p.rk_steps = prelax, order, rk_steps
     p.yield_ = yield_stress
     p.fix = fix
+    p.limiter = limiter
     return p
 
 
@@ -462,6 +471,19 @@ class M5Device:
         _check(lib().fv_m5_dev_set_dt(self.h, dt), "fv_m5_dev_set_dt")
         self.params.dt = dt
 
+    def set_limiter_flags(self, lim):
+        """Per-leaf slope flags of limiter 2 (0 minmod, 1 MC)."""
+        a = np.ascontiguousarray(lim, dtype=np.uint8)
+        assert a.shape == (self.n,)
+        _check(lib().fv_m5_dev_set_limiter_flags(self.h, a.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "fv_m5_dev_set_limiter_flags")
+
+    def get_limiter_flags(self):
+        a = np.zeros(self.n, np.uint8)
+        _check(lib().fv_m5_dev_get_limiter_flags(self.h, a.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "fv_m5_dev_get_limiter_flags")
+        return a
+
     def set_forest(self, mesh):
         """Leaf addresses of the attached mesh, for interface_location and regrids on the device."""
         _check(lib().fv_m5_dev_set_forest(self.h, mesh._p), "fv_m5_dev_set_forest")
@@ -492,6 +514,26 @@ class M5Device:
         """meshadaptation on the device; returns the new leaf count."""
         n = C.c_int()
         _check(lib().fv_m5_dev_regrid(self.h, C.byref(crit), maxlevel, C.byref(n)), "fv_m5_dev_regrid")
+        self.n = n.value
+        return self.n
+
+    def sample_mlp(self, mlp, crit):
+        """Append this state's stencil features and plain-indicator labels to mlp's training set."""
+        _check(lib().fv_mlp_sample_m5(self.h, mlp._h, C.byref(crit)), "fv_mlp_sample_m5")
+
+    def indicator_mlp(self, mlp, crit):
+        """(SI, targets, smooth flags) of the learned indicator; the flags become limiter 2's."""
+        SI = np.zeros((self.n, 2), np.int32)
+        split = np.zeros(self.n, np.int32)
+        sm = np.zeros(self.n, np.uint8)
+        _check(lib().fv_m5_dev_indicator_mlp(self.h, mlp._h, C.byref(crit), _ip(SI), _ip(split),
+                                             sm.ctypes.data_as(C.POINTER(C.c_uint8))), "fv_m5_dev_indicator_mlp")
+        return SI, split, sm
+
+    def regrid_mlp(self, mlp, crit, maxlevel):
+        """meshadaptation on the learned targets; returns the new leaf count."""
+        n = C.c_int()
+        _check(lib().fv_m5_dev_regrid_mlp(self.h, mlp._h, C.byref(crit), maxlevel, C.byref(n)), "fv_m5_dev_regrid_mlp")
         self.n = n.value
         return self.n
 

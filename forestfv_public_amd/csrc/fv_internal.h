@@ -106,3 +106,26 @@ struct fv_state_hook {
 // with the hook's state remap, connectivity and install; SI (device [n][2]) are
 // the interface flags that become the mesh's (fv_regrid.hip)
 int fv_dev_regrid_hook(fv_dev *c, int *split, const int *SI, int maxlevel, const fv_state_hook *h, int *n_out);
+
+// ---- model-5 contexts seen by the learned indicator (fv_mlp.hip) ----
+struct fv_m5_dev;
+struct fv_m5_view {
+  int n;
+  const int *nbr;
+  const uint32_t *info;
+  const int *fine4;
+  uint8_t *lim;        // per-leaf slope flags of limiter 2
+  hipStream_t stream;  // the topology context's stream, where the indicator kernels run
+};
+int fv_m5_view_of(fv_m5_dev *c, fv_m5_view *v);
+// the model-5 indicator of the device state: interface flags (*SI) and plain
+// targets (*split), device buffers of the topology context (fv_m5_dev_set_forest)
+int fv_m5_indicator_raw(fv_m5_dev *c, const fv_crit *cr, int **SI, int **split);
+// per-leaf fields of the learned indicator from the model-5 state: (rho, mixture
+// pressure, liquid fraction alpha_1, |u|) and the longitudinal sound speed
+int fv_m5_mlp_fields(fv_m5_dev *c, float4 *fld, float *snd);
+// labels of rows row0.. from the targets split: class (0 refine, 1 keep, 2
+// coarsen) and the smooth flag (largest relative pressure jump below thr)
+int fv_m5_mlp_labels(fv_m5_dev *c, const int *split, float thr, int *cls, float *smooth, long row0);
+// regrid with the targets split (device [n], changed in place) and interface flags SI
+int fv_m5_regrid_targets(fv_m5_dev *c, int *split, const int *SI, int maxlevel, int *n_out);

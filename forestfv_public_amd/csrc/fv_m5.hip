@@ -43,6 +43,7 @@ struct M5P {
   double yield;   // von Mises yield stress of perfect plasticity, 0 = elastic
   int fix;        // 1: the consistent update (FV_M5_FIX_SIGNS), 0: the published one
   int mix;        // 1: elastic energy and shear stress weighted by the volume fraction (FV_M5_FIX_MIXTURE)
+  int limiter;    // MUSCL slope: 0 minmod, 1 monotonised central, 2 per leaf (fv_m5_dev.lim)
   double dtu;     // signed step of the update: S = SN + dtu RHS / V, +dt published, -dt consistent
   double sg;      // sign of sigma in the momentum flux: +1 published (rho u u + sigma), -1 consistent
 };
@@ -1599,17 +1600,41 @@ __device__ void dist5(const Geo &G, int kind, int lvl, double &dcf, double &dcc)
 // (PRELAX 3: left to k_m5_relax3_start / _finish).  R = float (mixed precision)
 // forms the limited slopes in FP32 from the FP64 differences and adds them to
 // the FP64 cell state; the relaxation stays FP64.
+// the limited slope of one variable: minmod as m5_reconctruct.c:110-140 has it,
+// or (new) the monotonised central slope minmod(2a, 2b, (a + b) / 2) with a the
+// backward and b the forward difference quotient
+template <typename R>
+__device__ __forceinline__ R m5_slope(bool mc, R df, R db, R dp, R dm) {
+  if (!mc) {
+    const R product = (-db / dm) * (df / dp);
+    if (product > R(0.0)) {
+      if (fabs(-db / dm) > fabs(df / dp)) return df / dp;
+      return -db / dm;
+    }
+    return R(0.0);
+  }
+  const R a = -db / dm, b = df / dp;
+  if (a * b > R(0.0)) {
+    const R c = R(0.5) * (a + b);
+    const R mag = minr(minr(R(2.0) * fabs(a), R(2.0) * fabs(b)), fabs(c));
+    return (c > R(0.0)) ? mag : -mag;
+  }
+  return R(0.0);
+}
+
 template <int NM, typename R>
 __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__restrict__ Gp,
                                                   const double *__restrict__ S, const int *__restrict__ nbr,
                                                   const uint32_t *__restrict__ info, const int *__restrict__ fine4,
-                                                  double *__restrict__ SF, int *__restrict__ flag) {
+                                                  const uint8_t *__restrict__ limf, double *__restrict__ SF,
+                                                  int *__restrict__ flag) {
   using L = L5<NM>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double a = P.amin / 100.0;
   const uint32_t inf = info[i];
   const int lvl = fvk::cell_level(inf);
+  const bool mc = (P.limiter == 1) || (P.limiter == 2 && limf[i] != 0);
   double Sc[L::NEQ];
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
@@ -1631,15 +1656,7 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
       const R lim = (v >= L::I4 && v < L::I5) ? R(0.0) : R(1.0);
       const R dpr = (R)dp, dmr = (R)dm;
       R g = R(0.0);
-      if (kf != FV_FK_BOUNDARY && fabs(df) > R(1.0e-10)) {
-        const R product = (-db / dmr) * (df / dpr);
-        if (product > R(0.0)) {
-          if (fabs(-db / dmr) > fabs(df / dpr)) g = lim * (df / dpr);
-          else g = lim * (-db / dmr);
-        } else {
-          g = R(0.0);
-        }
-      }
+      if (kf != FV_FK_BOUNDARY && fabs(df) > R(1.0e-10)) g = lim * m5_slope<R>(mc, df, db, dpr, dmr);
       Fv[v] = Sc[v] + dcf * (double)g;
     }
     // VF limit (m5_reconctruct.c:142-170)
@@ -2058,6 +2075,8 @@ int m5_params(const fv_m5_params *p, M5P &P) {
     return fv_set_err(FV_ERR_ARG, "model 5: fix must be a combination of FV_M5_FIX_SIGNS and FV_M5_FIX_MIXTURE");
   P.fix = (p->fix & FV_M5_FIX_SIGNS) ? 1 : 0;
   P.mix = (p->fix & FV_M5_FIX_MIXTURE) ? 1 : 0;
+  if (p->limiter < 0 || p->limiter > 2) return fv_set_err(FV_ERR_ARG, "model 5: limiter must be 0, 1 or 2");
+  P.limiter = p->limiter;
   P.dtu = P.fix ? -p->dt : p->dt;
   P.sg = P.fix ? -1.0 : 1.0;
   return FV_OK;
@@ -2281,6 +2300,7 @@ struct fv_m5_dev {
   uint8_t *mark6 = nullptr;   // [6n]
   int *list6 = nullptr;       // [6n]
   double *part = nullptr;     // [n/256 + 1] per-block minima of the CFL step, then the step
+  uint8_t *lim = nullptr;     // [n] per-leaf slope flags of limiter 2 (0 minmod, 1 MC)
   int cap = 0, cap_fine = 0;  // leaves and 1:4 rows the per-leaf buffers hold
   // mesh side of regrids and interface_location (fv_m5_dev_set_forest): a
   // topology-only model-4 context over the same leaves
@@ -2294,7 +2314,7 @@ namespace {
 void m5_free(fv_m5_dev *c) {
   void *ptrs[] = {c->G, c->nbr, c->fine4, c->info, c->S, c->S2, c->SN, c->SF, c->RHS, c->Sb,
                   c->gfm, c->mark, c->flag, c->list, c->count, c->cub_tmp, c->nst, c->iters, c->mark6, c->list6,
-                  c->part, c->Snew};
+                  c->part, c->Snew, c->lim};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->topo) fv_dev_detach(c->topo);
@@ -2338,10 +2358,10 @@ template <int NM>
 int m5_faceinterp_t(fv_m5_dev *c) {
   if (c->prec == FV_PREC_MIXED)
     hipLaunchKernelGGL((k_m5_recon<NM, float>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
-                       c->nbr, c->info, c->fine4, c->SF, c->flag);
+                       c->nbr, c->info, c->fine4, c->lim, c->SF, c->flag);
   else
     hipLaunchKernelGGL((k_m5_recon<NM, double>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
-                       c->nbr, c->info, c->fine4, c->SF, c->flag);
+                       c->nbr, c->info, c->fine4, c->lim, c->SF, c->flag);
   HIPCHK(hipGetLastError());
   if (c->P.prelax == 3) return m5_relax3<NM>(c, c->SF, 6L * c->n, nullptr);
   return FV_OK;
@@ -2571,6 +2591,69 @@ __global__ __launch_bounds__(256) void k_m5_adapt_state(int n, int nn, M5P P, co
   }
 }
 
+// fields of the learned indicator (the model-4 ones, fv_mlp.hip, with the
+// liquid as the fraction): density, mixture pressure sum_k alpha_k p_hydro_k,
+// alpha of material 1, |u|, and c^2 = sum_k c2Y_k + (4/3) sum_k alpha_k mu_k / rho
+template <int NM>
+__device__ void m5_cell_fields(const M5P &P, const double *__restrict__ S, int n, int j, double &rho, double &p,
+                               double &al, double &um, double &c) {
+  using L = L5<NM>;
+  double x[L::NEQ];
+#pragma unroll
+  for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + j];
+  Rec5<NM> r;
+  record<NM>(P, x, r);
+  double pm = 0.0, c2 = 0.0, mu = 0.0;
+#pragma unroll
+  for (int k = 0; k < NM; k++) { pm += r.avf[k] * r.ph[k]; c2 += r.c2Y[k]; mu += r.avf[k] * P.mush[k]; }
+  if (!P.mix) c2 += (4.0 / 3.0) * mu / r.r;
+  rho = r.r;
+  p = pm;
+  al = r.avf[1];
+  um = sqrt(r.u * r.u + r.v * r.v + r.w * r.w);
+  c = sqrt(c2);
+}
+
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_mlp_fields(int n, M5P P, const double *__restrict__ S,
+                                                       float4 *__restrict__ fld, float *__restrict__ snd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double rho, p, al, um, c;
+  m5_cell_fields<NM>(P, S, n, i, rho, p, al, um, c);
+  fld[i] = make_float4((float)rho, (float)p, (float)al, (float)um);
+  snd[i] = (float)c;
+}
+
+template <int NM>
+__global__ __launch_bounds__(256) void k_m5_mlp_labels(int n, M5P P, const double *__restrict__ S,
+                                                       const int *__restrict__ nbr, const uint32_t *__restrict__ info,
+                                                       const int *__restrict__ fine4, const int *__restrict__ split,
+                                                       float thr, int *__restrict__ cls, float *__restrict__ smooth,
+                                                       long row0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t inf = info[i];
+  const int lvl = fvk::cell_level(inf);
+  cls[row0 + i] = split[i] > lvl ? 0 : (split[i] < lvl ? 2 : 1);
+  double rho, p0, al, um, c;
+  m5_cell_fields<NM>(P, S, n, i, rho, p0, al, um, c);
+  double jmax = 0.0;
+  for (int f = 0; f < 6; f++) {
+    const int kind = fvk::face_kind(inf, f);
+    if (kind == FV_FK_BOUNDARY) continue;
+    const int nb = nbr[(size_t)f * n + i];
+    const int ns = (kind == FV_FK_FINE) ? 4 : 1;
+    for (int k = 0; k < ns; k++) {
+      const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
+      double pj;
+      m5_cell_fields<NM>(P, S, n, j, rho, pj, al, um, c);
+      jmax = fmax(jmax, fabs(pj - p0) / fmax(fmax(fabs(pj), fabs(p0)), 1e-30));
+    }
+  }
+  smooth[row0 + i] = (jmax < (double)thr) ? 1.f : 0.f;
+}
+
 M5Crit m5_crit(const fv_crit *cr) {
   M5Crit C;
   C.uniform_on = cr->uniform_on; C.uniform_lvl = cr->uniform_lvl;
@@ -2596,6 +2679,7 @@ int m5_reserve(fv_m5_dev *c, int n_leaves, int nfine, int prelax3) {
   take(dalloc(&t.Sb, sz), t.Sb); take(dalloc(&t.RHS, sz), t.RHS); take(dalloc(&t.SF, 6 * sz), t.SF);
   take(dalloc(&t.gfm, 6 * n), t.gfm); take(dalloc(&t.mark, n), t.mark); take(dalloc(&t.list, n), t.list);
   take(dalloc(&t.part, (n + 255) / 256 + 1), t.part);
+  take(dalloc(&t.lim, n), t.lim);
   if (prelax3) {
     take(dalloc(&t.nst, 18 * n), t.nst); take(dalloc(&t.iters, 6 * n), t.iters);
     take(dalloc(&t.mark6, 6 * n), t.mark6); take(dalloc(&t.list6, 6 * n), t.list6);
@@ -2612,11 +2696,11 @@ int m5_reserve(fv_m5_dev *c, int n_leaves, int nfine, int prelax3) {
     return fv_set_err(e == hipErrorOutOfMemory ? FV_ERR_NOMEM : FV_ERR_HIP, hipGetErrorString(e));
   }
   void *old[] = {c->nbr, c->fine4, c->info, c->S, c->S2, c->SN, c->Sb, c->RHS, c->SF, c->gfm, c->mark, c->list,
-                 c->part, c->nst, c->iters, c->mark6, c->list6, c->cub_tmp};
+                 c->part, c->lim, c->nst, c->iters, c->mark6, c->list6, c->cub_tmp};
   for (void *q : old)
     if (q) hipFree(q);
   c->nbr = t.nbr; c->fine4 = t.fine4; c->info = t.info; c->S = t.S; c->S2 = t.S2; c->SN = t.SN; c->Sb = t.Sb;
-  c->RHS = t.RHS; c->SF = t.SF; c->gfm = t.gfm; c->mark = t.mark; c->list = t.list; c->part = t.part;
+  c->RHS = t.RHS; c->SF = t.SF; c->gfm = t.gfm; c->mark = t.mark; c->list = t.list; c->part = t.part; c->lim = t.lim;
   c->nst = t.nst; c->iters = t.iters; c->mark6 = t.mark6; c->list6 = t.list6; c->cub_tmp = t.cub_tmp;
   c->cub_bytes = cub;
   t = fv_m5_dev();   // its pointers now belong to c
@@ -2661,6 +2745,7 @@ int m5_install_hook(void *ctx, fv_dev *t) {
   HIPCHK(hipMemcpyAsync(c->info, t->info, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, t->stream));
   if (nfine > 0)
     HIPCHK(hipMemcpyAsync(c->fine4, t->fine4, sizeof(int) * 4 * (size_t)nfine, hipMemcpyDeviceToDevice, t->stream));
+  HIPCHK(hipMemsetAsync(c->lim, 0, n, t->stream));   // minmod until the flags are set for the new leaves
   HIPCHK(hipStreamSynchronize(t->stream));
   return FV_OK;
 }
@@ -2713,6 +2798,7 @@ extern "C" int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_
   chk(dalloc(&c->gfm, 6 * n)); chk(dalloc(&c->mark, n));
   chk(dalloc(&c->flag, 2)); chk(dalloc(&c->list, n)); chk(dalloc(&c->count, 1));
   chk(dalloc(&c->part, (n + 255) / 256 + 1));
+  chk(dalloc(&c->lim, n));
   if (p->prelax == 3) {
     chk(dalloc(&c->nst, 18 * n)); chk(dalloc(&c->iters, 6 * n));
     chk(dalloc(&c->mark6, 6 * n)); chk(dalloc(&c->list6, 6 * n));
@@ -2740,6 +2826,7 @@ extern "C" int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_
     chk(hipMemset(c->S, 0, sizeof(double) * sz));
     chk(hipMemset(c->SN, 0, sizeof(double) * sz));
     chk(hipMemset(c->SF, 0, sizeof(double) * 6 * sz));
+    chk(hipMemset(c->lim, 0, n));
   }
   if (e != hipSuccess) {
     m5_free(c);
@@ -2821,6 +2908,20 @@ extern "C" int fv_m5_dev_cfl_dt(fv_m5_dev *c, double cfl, double *dt) {
   return FV_OK;
 }
 
+extern "C" int fv_m5_dev_set_limiter_flags(fv_m5_dev *c, const uint8_t *lim) {
+  if (!c || !lim) return fv_set_err(FV_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(c->lim, lim, (size_t)c->n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FV_OK;
+}
+
+extern "C" int fv_m5_dev_get_limiter_flags(fv_m5_dev *c, uint8_t *lim) {
+  if (!c || !lim) return fv_set_err(FV_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(lim, c->lim, (size_t)c->n, hipMemcpyDeviceToHost));
+  return FV_OK;
+}
+
 extern "C" int fv_m5_dev_set_precision(fv_m5_dev *c, int prec) {
   if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
   if (prec != FV_PREC_FP64 && prec != FV_PREC_EXACT && prec != FV_PREC_MIXED)
@@ -2897,6 +2998,65 @@ extern "C" int fv_m5_dev_time_steps(fv_m5_dev *c, int nsteps, double *ms_stage, 
 }
 
 
+int fv_m5_view_of(fv_m5_dev *c, fv_m5_view *v) {
+  if (!c || !v) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (!c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
+  v->n = c->n; v->nbr = c->nbr; v->info = c->info; v->fine4 = c->fine4; v->lim = c->lim;
+  v->stream = c->topo->stream;
+  return FV_OK;
+}
+
+int fv_m5_indicator_raw(fv_m5_dev *c, const fv_crit *cr, int **SI, int **split) {
+  if (!c || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
+  if (!c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
+  fv_dev *t = c->topo;
+  int *cur = nullptr;
+  int rc = m5_interface_layers(c, cr, &cur);
+  if (rc != FV_OK) return rc;
+  const dim3 g((c->n + 255) / 256), b(256);
+  const M5Crit C = m5_crit(cr);
+  if (c->P.nmat == 2)
+    hipLaunchKernelGGL(k_m5_split<2>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
+  else
+    hipLaunchKernelGGL(k_m5_split<3>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
+  HIPCHK(hipGetLastError());
+  *SI = cur;
+  *split = t->split;
+  return FV_OK;
+}
+
+int fv_m5_mlp_fields(fv_m5_dev *c, float4 *fld, float *snd) {
+  if (!c || !c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
+  const dim3 g((c->n + 255) / 256), b(256);
+  if (c->P.nmat == 2)
+    hipLaunchKernelGGL(k_m5_mlp_fields<2>, g, b, 0, c->topo->stream, c->n, c->P, c->S, fld, snd);
+  else
+    hipLaunchKernelGGL(k_m5_mlp_fields<3>, g, b, 0, c->topo->stream, c->n, c->P, c->S, fld, snd);
+  HIPCHK(hipGetLastError());
+  return FV_OK;
+}
+
+int fv_m5_mlp_labels(fv_m5_dev *c, const int *split, float thr, int *cls, float *smooth, long row0) {
+  if (!c || !c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
+  const dim3 g((c->n + 255) / 256), b(256);
+  if (c->P.nmat == 2)
+    hipLaunchKernelGGL(k_m5_mlp_labels<2>, g, b, 0, c->topo->stream, c->n, c->P, c->S, c->nbr, c->info, c->fine4,
+                       split, thr, cls, smooth, row0);
+  else
+    hipLaunchKernelGGL(k_m5_mlp_labels<3>, g, b, 0, c->topo->stream, c->n, c->P, c->S, c->nbr, c->info, c->fine4,
+                       split, thr, cls, smooth, row0);
+  HIPCHK(hipGetLastError());
+  return FV_OK;
+}
+
+int fv_m5_regrid_targets(fv_m5_dev *c, int *split, const int *SI, int maxlevel, int *n_out) {
+  if (!c || !c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
+  fv_state_hook h{c, m5_remap_hook, m5_install_hook};
+  int rc = fv_dev_regrid_hook(c->topo, split, SI, maxlevel, &h, n_out);
+  if (rc != FV_OK) return rc;
+  return m5_check(c);
+}
+
 extern "C" int fv_m5_dev_set_forest(fv_m5_dev *c, const fv_mesh *m) {
   if (!c || !m) return fv_set_err(FV_ERR_ARG, "null argument");
   if (m->n != c->n) return fv_set_err(FV_ERR_ARG, "fv_m5_dev_set_forest: mesh and context differ in leaf count");
@@ -2935,42 +3095,21 @@ extern "C" int fv_m5_dev_interface_location(fv_m5_dev *c, const fv_crit *cr, int
 }
 
 extern "C" int fv_m5_dev_indicator(fv_m5_dev *c, const fv_crit *cr, int *SI, int *split) {
-  if (!c || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
-  if (!c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
-  fv_dev *t = c->topo;
-  int *cur = nullptr;
-  int rc = m5_interface_layers(c, cr, &cur);
+  int *cur = nullptr, *sp = nullptr;
+  int rc = fv_m5_indicator_raw(c, cr, &cur, &sp);
   if (rc != FV_OK) return rc;
-  const dim3 g((c->n + 255) / 256), b(256);
-  const M5Crit C = m5_crit(cr);
-  if (c->P.nmat == 2)
-    hipLaunchKernelGGL(k_m5_split<2>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
-  else
-    hipLaunchKernelGGL(k_m5_split<3>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
-  HIPCHK(hipGetLastError());
-  if (SI) HIPCHK(hipMemcpyAsync(SI, cur, sizeof(int) * 2 * (size_t)c->n, hipMemcpyDeviceToHost, t->stream));
-  if (split) HIPCHK(hipMemcpyAsync(split, t->split, sizeof(int) * (size_t)c->n, hipMemcpyDeviceToHost, t->stream));
-  HIPCHK(hipStreamSynchronize(t->stream));
+  hipStream_t st = c->topo->stream;
+  if (SI) HIPCHK(hipMemcpyAsync(SI, cur, sizeof(int) * 2 * (size_t)c->n, hipMemcpyDeviceToHost, st));
+  if (split) HIPCHK(hipMemcpyAsync(split, sp, sizeof(int) * (size_t)c->n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return FV_OK;
 }
 
 extern "C" int fv_m5_dev_regrid(fv_m5_dev *c, const fv_crit *cr, int maxlevel, int *n_out) {
-  if (!c || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
-  if (!c->topo) return fv_set_err(FV_ERR_ARG, "no forest: call fv_m5_dev_set_forest after attach");
-  fv_dev *t = c->topo;
-  int *cur = nullptr;
-  int rc = m5_interface_layers(c, cr, &cur);
+  int *cur = nullptr, *sp = nullptr;
+  int rc = fv_m5_indicator_raw(c, cr, &cur, &sp);
   if (rc != FV_OK) return rc;
-  const dim3 g((c->n + 255) / 256), b(256);
-  const M5Crit C = m5_crit(cr);
-  if (c->P.nmat == 2)
-    hipLaunchKernelGGL(k_m5_split<2>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
-  else
-    hipLaunchKernelGGL(k_m5_split<3>, g, b, 0, t->stream, c->n, c->P, C, c->S, c->nbr, c->info, c->fine4, cur, t->split);
-  HIPCHK(hipGetLastError());
-  fv_state_hook h{c, m5_remap_hook, m5_install_hook};
-  if ((rc = fv_dev_regrid_hook(t, t->split, cur, maxlevel, &h, n_out)) != FV_OK) return rc;
-  return m5_check(c);   // a merged parent whose relaxation failed
+  return fv_m5_regrid_targets(c, sp, cur, maxlevel, n_out);
 }
 
 extern "C" int fv_m5_neq(int nmat) { return 3 * nmat + 12; }

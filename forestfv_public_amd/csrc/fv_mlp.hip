@@ -834,3 +834,94 @@ extern "C" int fv_dev_indicator_mlp(fv_dev *c, fv_mlp *m, const fv_crit *cr, int
   HIPCHK(hipStreamSynchronize(c->stream));
   return FV_OK;
 }
+
+// ---------------------------------------------------------------------
+// model-5 contexts (config 2): the same network on the model-5 state; the
+// fields and labels come from fv_m5.hip, the indicator kernels run on the
+// stream of the context's topology (fv_m5_dev_set_forest)
+// ---------------------------------------------------------------------
+
+// the smooth flags alone (the targets were used already): the per-leaf limiter
+__global__ void k_mlp_flags(int n, const float *__restrict__ out, uint8_t *__restrict__ lim,
+                            uint8_t *__restrict__ smooth) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint8_t s = out[(size_t)i * MLP_NO + 3] > 0.f ? 1 : 0;
+  lim[i] = s;
+  smooth[i] = s;
+}
+
+// network outputs of every leaf of the model-5 context into m->iout
+static int m5_infer(fv_m5_dev *c, fv_mlp *m, fv_m5_view &v) {
+  int rc = fv_m5_view_of(c, &v);
+  if (rc == FV_OK) rc = refresh_bf16(m);
+  if (rc == FV_OK) rc = ensure_infer(m, v.n, false);
+  if (rc == FV_OK) rc = ensure_fields(m, v.n);
+  if (rc == FV_OK) rc = fv_m5_mlp_fields(c, m->fld, m->snd);
+  if (rc != FV_OK) return rc;
+  hipLaunchKernelGGL(k_mlp_infer_stencil, dim3((unsigned)((v.n + 63) / 64)), dim3(256), 0, v.stream, v.n, m->fld,
+                     m->snd, v.nbr, v.info, v.fine4, m->w1t, m->w2t, m->w3t, m->w, m->iout);
+  HIPCHK(hipGetLastError());
+  return FV_OK;
+}
+
+extern "C" int fv_mlp_sample_m5(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr) {
+  if (!c || !m || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
+  int *SI = nullptr, *split = nullptr;
+  int rc = fv_m5_indicator_raw(c, cr, &SI, &split);
+  fv_m5_view v;
+  if (rc == FV_OK) rc = fv_m5_view_of(c, &v);
+  if (rc != FV_OK) return rc;
+  HIPCHK(hipStreamSynchronize(v.stream));
+  if ((rc = ensure_data(m, m->nd + v.n)) != FV_OK) return rc;
+  if ((rc = ensure_fields(m, v.n)) != FV_OK) return rc;
+  if ((rc = fv_m5_mlp_fields(c, m->fld, m->snd)) != FV_OK) return rc;
+  hipLaunchKernelGGL(k_mlp_features, g1(v.n), dim3(256), 0, v.stream, v.n, m->fld, m->snd, v.nbr, v.info, v.fine4,
+                     m->X, m->nd);
+  HIPCHK(hipGetLastError());
+  if ((rc = fv_m5_mlp_labels(c, split, m->smooth_thr, m->cls, m->smooth, m->nd)) != FV_OK) return rc;
+  HIPCHK(hipStreamSynchronize(v.stream));
+  m->nd += v.n;
+  return FV_OK;
+}
+
+// the learned indicator on a model-5 context: interface layers and plain
+// targets, then the network's targets with the plain refinement forced (left in
+// the topology context, where fv_m5_dev_regrid_mlp reads them) and the smooth
+// flag as the per-leaf limiter
+extern "C" int fv_m5_dev_indicator_mlp(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr, int *SI, int *split,
+                                       uint8_t *smooth) {
+  if (!c || !m || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
+  int *SIdev = nullptr, *sp = nullptr;
+  int rc = fv_m5_indicator_raw(c, cr, &SIdev, &sp);
+  fv_m5_view v;
+  if (rc == FV_OK) rc = m5_infer(c, m, v);
+  if (rc != FV_OK) return rc;
+  hipLaunchKernelGGL(k_mlp_decide, g1(v.n), dim3(256), 0, v.stream, v.n, m->iout, v.info, sp, v.lim, m->ismooth);
+  HIPCHK(hipGetLastError());
+  if (SI) HIPCHK(hipMemcpyAsync(SI, SIdev, sizeof(int) * 2 * (size_t)v.n, hipMemcpyDeviceToHost, v.stream));
+  if (split) HIPCHK(hipMemcpyAsync(split, sp, sizeof(int) * (size_t)v.n, hipMemcpyDeviceToHost, v.stream));
+  if (smooth) HIPCHK(hipMemcpyAsync(smooth, m->ismooth, (size_t)v.n, hipMemcpyDeviceToHost, v.stream));
+  HIPCHK(hipStreamSynchronize(v.stream));
+  return FV_OK;
+}
+
+// meshadaptation with the learned targets: the learned indicator, the regrid
+// (criterionsmth, merge/split with the model-5 remap, connectivity), then the
+// network's smooth flags of the new leaves as their limiter
+extern "C" int fv_m5_dev_regrid_mlp(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr, int maxlevel, int *n_out) {
+  if (!c || !m || !cr) return fv_set_err(FV_ERR_ARG, "null argument");
+  int *SIdev = nullptr, *sp = nullptr;
+  int rc = fv_m5_indicator_raw(c, cr, &SIdev, &sp);
+  fv_m5_view v;
+  if (rc == FV_OK) rc = m5_infer(c, m, v);
+  if (rc != FV_OK) return rc;
+  hipLaunchKernelGGL(k_mlp_decide, g1(v.n), dim3(256), 0, v.stream, v.n, m->iout, v.info, sp, v.lim, m->ismooth);
+  HIPCHK(hipGetLastError());
+  if ((rc = fv_m5_regrid_targets(c, sp, SIdev, maxlevel, n_out)) != FV_OK) return rc;
+  if ((rc = m5_infer(c, m, v)) != FV_OK) return rc;
+  hipLaunchKernelGGL(k_mlp_flags, g1(v.n), dim3(256), 0, v.stream, v.n, m->iout, v.lim, m->ismooth);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(v.stream));
+  return FV_OK;
+}

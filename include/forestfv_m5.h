@@ -44,6 +44,9 @@ typedef struct {
   int    fix;                  /* FV_M5_FIX_SIGNS: the consistent model-5 update (new, below), with
                                   FV_M5_FIX_MIXTURE: the volume-fraction-weighted elastic mixture
                                   (new, below); 0 = the published code */
+  int    limiter;              /* MUSCL slope (new): 0 minmod (m5_reconctruct.c:93-140), 1 monotonised
+                                  central, 2 per leaf from fv_m5_dev_set_limiter_flags (0 minmod,
+                                  1 MC; every leaf minmod after a regrid until set again) */
 } fv_m5_params;
 
 /* Corrections of the published model-5 update, one mode (0 reproduces it):
@@ -142,6 +145,10 @@ int fv_m5_dev_advance(fv_m5_dev *c, int nsteps);
  * flux; the records are formed in FP64 (pressures carry p_inf up to 2e10 Pa),
  * and the residual, RK accumulation and relaxation stay FP64. */
 int fv_m5_dev_set_precision(fv_m5_dev *c, int prec);
+/* Per-leaf slope flags of limiter 2 ([n], 0 minmod, 1 MC; new: the
+ * reference reconstructs with minmod only, m5_reconctruct.c:93-140). */
+int fv_m5_dev_set_limiter_flags(fv_m5_dev *c, const uint8_t *lim);
+int fv_m5_dev_get_limiter_flags(fv_m5_dev *c, uint8_t *lim);
 /* The time step of later stages (DT of the params at attach until then). */
 int fv_m5_dev_set_dt(fv_m5_dev *c, double dt);
 /* CFL step of the device state (new: the reference takes a fixed DT):
@@ -171,6 +178,22 @@ int fv_m5_dev_get_forest(fv_m5_dev *c, int *root, int *level, int *ci, int *SI);
 int fv_m5_dev_interface_location(fv_m5_dev *c, const fv_crit *cr, int *SI);
 int fv_m5_dev_indicator(fv_m5_dev *c, const fv_crit *cr, int *SI, int *split);
 int fv_m5_dev_regrid(fv_m5_dev *c, const fv_crit *cr, int maxlevel, int *n_out);
+
+/* ---- the learned indicator on a model-5 context (new; config 4) ----
+ * The network of fv_mlp_* (forestfv_dev.h) with the model-5 fields: density,
+ * mixture pressure, liquid fraction (material 1) and |u| over the sound
+ * speed, in the same 3x3x3 stencil features.  Need fv_m5_dev_set_forest.
+ *  - fv_mlp_sample_m5 appends the features and the plain indicator's labels
+ *    of every leaf to the training set;
+ *  - fv_m5_dev_indicator_mlp gives SI, the targets (the network's, with the
+ *    plain refinement forced) and the smooth flags, which become the per-leaf
+ *    slope flags of limiter 2;
+ *  - fv_m5_dev_regrid_mlp adapts the mesh to those targets and sets the new
+ *    leaves' slope flags from the network. */
+typedef struct fv_mlp fv_mlp;
+int fv_mlp_sample_m5(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr);
+int fv_m5_dev_indicator_mlp(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr, int *SI, int *split, uint8_t *smooth);
+int fv_m5_dev_regrid_mlp(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr, int maxlevel, int *n_out);
 /* the stage's pieces alone: face states of the current S (m5_faceinterp_V1),
  * and the residual of S and those face states (m5_residual) */
 int fv_m5_dev_faceinterp(fv_m5_dev *c);

@@ -1063,6 +1063,30 @@ static void m5_stencil(const lay5_t *L, const orc_m5_params *p, const orc_mesh *
   m5_avf_limit(L, p, X);
 }
 
+static const unsigned char *g_m5_cell_limiter = NULL;
+void orc_m5_set_cell_limiter(const unsigned char *lim) { g_m5_cell_limiter = lim; }
+
+/* the limited slope of one variable: minmod as m5_reconctruct.c:110-140 has it,
+ * or (new) the monotonised central slope minmod(2a, 2b, (a + b) / 2) of the
+ * model-4 restatement, a the backward and b the forward difference quotient */
+static double m5_slope(int mc, double df, double db, double dp, double dm) {
+  if (!mc) {
+    const double product = (-db / dm) * (df / dp);
+    if (product > 0.0) {
+      if (fabs(-db / dm) > fabs(df / dp)) return df / dp;
+      return -db / dm;
+    }
+    return 0.0;
+  }
+  const double a = -db / dm, b = df / dp;
+  if (a * b > 0.0) {
+    const double c = 0.5 * (a + b);
+    const double mag = MINF(MINF(2.0 * fabs(a), 2.0 * fabs(b)), fabs(c));
+    return (c > 0.0) ? mag : -mag;
+  }
+  return 0.0;
+}
+
 /* m5_faceinterp.c:60-93 for leaf i, with m5_deltavalues_V1 (:20-56) and
  * m5_reconctruct_V1 (:93-190); g_limitedvars is 0 on the alpha-rho-e block
  * (m5_input.c:196-204) */
@@ -1073,6 +1097,7 @@ static int m5_faceinterp_leaf(const lay5_t *L, const orc_m5_params *p, const orc
   double SLc[ORC_M5_MAXNEQ], Xf[ORC_M5_MAXNEQ], Xb[ORC_M5_MAXNEQ];
   for (int iv = 0; iv < neq; iv++) SLc[iv] = S[(size_t)i * neq + iv];
   m5_avf_limit(L, p, SLc);
+  const int mc = (p->limiter == 1) || (p->limiter == 2 && g_m5_cell_limiter && g_m5_cell_limiter[i]);
   for (int f = 0; f < 6; ++f) {
     const int fo = FCOPP5[f];
     m5_stencil(L, p, m, S, i, f, Xf);
@@ -1083,16 +1108,7 @@ static int m5_faceinterp_leaf(const lay5_t *L, const orc_m5_params *p, const orc
       const double df = Xf[iv] - SLc[iv], db = Xb[iv] - SLc[iv];
       const double lim = (iv >= L->i4 && iv < L->i5) ? 0.0 : 1.0;
       double g = 0.0;
-      if (bcf == 0 && fabs(df) > 1.0e-10) {
-        const double dp = cl->dist_cc[f], dm = cl->dist_cc[fo];
-        const double product = (-db / dm) * (df / dp);
-        if (product > 0.0) {
-          if (fabs(-db / dm) > fabs(df / dp)) g = lim * (df / dp);
-          else g = lim * (-db / dm);
-        } else {
-          g = 0.0;
-        }
-      }
+      if (bcf == 0 && fabs(df) > 1.0e-10) g = lim * m5_slope(mc, df, db, cl->dist_cc[f], cl->dist_cc[fo]);
       SF[iv] = SLc[iv] + cl->dist_cf[f] * g;
     }
     /* VF limit (m5_reconctruct.c:142-170) */

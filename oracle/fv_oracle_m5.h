@@ -39,7 +39,12 @@ typedef struct {
   int    rk_steps;             /* RK_STEPS: stages irk = 1..rk_steps of m5_RK */
   double yield;                /* von Mises yield stress of the solid, 0 = elastic (new) */
   int    fix;                  /* ORC_M5_FIX_SIGNS: the consistent model-5 update below; 0 = literal */
+  int    limiter;              /* MUSCL slope (new): 0 minmod (m5_reconctruct.c:93-140), 1 MC,
+                                  2 per leaf from orc_m5_set_cell_limiter (0 minmod, 1 MC) */
 } orc_m5_params;
+
+/* per-leaf slope choice of limiter 2 (a leaf-order array the caller keeps alive) */
+void orc_m5_set_cell_limiter(const unsigned char *lim);
 
 /* The published model-5 update is inconsistent with itself in several places;
  * ORC_M5_FIX_SIGNS takes the consistent form of each, all as one mode (0

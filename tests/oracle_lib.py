@@ -90,6 +90,7 @@ class orc_m5_params(C.Structure):
         ("rk_steps", C.c_int),
         ("yield_", C.c_double),
         ("fix", C.c_int),
+        ("limiter", C.c_int),
     ]
 
 
@@ -130,6 +131,8 @@ def lib(variant="O2"):
         L.orc_cfl_dt.restype = C.c_double
         L.orc_set_cell_limiter.argtypes = [C.c_void_p]
         L.orc_set_cell_limiter.restype = None
+        L.orc_m5_set_cell_limiter.argtypes = [C.c_void_p]
+        L.orc_m5_set_cell_limiter.restype = None
         L.orc_advance_par.argtypes = [P(orc_mesh), P(orc_cell), P(orc_params), dp, C.c_int, C.c_int]
         L.orc_num_threads.argtypes = []
         L.orc_cons2prim.argtypes = [P(orc_params), dp, dp]
@@ -416,6 +419,17 @@ def set_cell_limiter(lim):
     a = np.ascontiguousarray(lim, dtype=np.uint8)
     _keep["lim"] = a
     lib().orc_set_cell_limiter(a.ctypes.data)
+
+
+def m5_set_cell_limiter(lim):
+    """Per-leaf slope flags of model 5 for limiter == 2 (None clears)."""
+    if lim is None:
+        _keep.pop("lim5", None)
+        lib().orc_m5_set_cell_limiter(None)
+        return
+    a = np.ascontiguousarray(lim, dtype=np.uint8)
+    _keep["lim5"] = a
+    lib().orc_m5_set_cell_limiter(a.ctypes.data)
 
 
 def cons2prim(params, vec):
