@@ -220,7 +220,7 @@ def build_variant(p_liquid, centre, base=4, maxlevel=8, layers=16, seed=1234, ml
     return p, m, S, crit
 
 
-def train_on_variants(mlp, log_fn=log):
+def train_on_variants(mlp, log_fn=log, nvariants=12):
     """Config 4's training set: stencil/flag pairs of the plain indicator on seeded
     variants of the bubble case (driving pressure 20-200 bar, bubble centre moved by
     up to a radius), sampled at every pre-adaptation pass, at the start and after 15
@@ -228,6 +228,7 @@ def train_on_variants(mlp, log_fn=log):
     Lc = 0.5 * L_DOMAIN
     variants = [(pl, (Lc + dx, Lc + dy, Lc)) for pl in (2e6, 5e6, 1e7, 2e7)
                 for dx, dy in ((0.0, 0.0), (0.8 * R_BUBBLE, -0.5 * R_BUBBLE), (-0.6 * R_BUBBLE, 0.0))]
+    variants = variants[::len(variants) // nvariants][:nvariants]
     for k, (pl, c) in enumerate(variants):
         p, m, S, crit = build_variant(pl, c, seed=100 + k, mlp=mlp)
         dev = F.Device(p, m)
@@ -455,18 +456,20 @@ def c2_crit(maxlevel, base, layers):
                        inter0_on=1, inter0_lvl=maxlevel, inter0_lrs=layers, inter1_on=0)
 
 
-def c2_state(p, m, dx=0.0):
+def c2_state(p, m, dx=0.0, standoff=C2_STANDOFF, p_liquid=C2_P[0]):
     """Config 2 initial data on mesh m: the solid slab below z = 1 mm with a sharp face, the
-    0.4 mm air bubble 1.2 radii above it smeared over two finest cells, water at 100 bar, air at
-    1 bar, A = I in the solid and 0 in the fluids, at rest (fv_m5_init_slab_bubble)."""
-    centre = (0.5 * L_DOMAIN + dx, 0.5 * L_DOMAIN, C2_Z_SLAB + C2_STANDOFF * R_BUBBLE)
-    return m.init_m5_slab_bubble(p, C2_RHO, C2_Z_SLAB, centre, R_BUBBLE, C2_P[0], C2_P[1], 2.0)
+    0.4 mm air bubble `standoff` radii above it smeared over two finest cells, water at p_liquid
+    (100 bar), air at 1 bar, A = I in the solid and 0 in the fluids, at rest
+    (fv_m5_init_slab_bubble)."""
+    centre = (0.5 * L_DOMAIN + dx, 0.5 * L_DOMAIN, C2_Z_SLAB + standoff * R_BUBBLE)
+    return m.init_m5_slab_bubble(p, C2_RHO, C2_Z_SLAB, centre, R_BUBBLE, p_liquid, C2_P[1], 2.0)
 
 
-def build_c2(maxlevel=9, base=5, layers=64, log_fn=log):
+def build_c2(maxlevel=9, base=5, layers=64, log_fn=log, mlp=None, **init):
     """Geometric pre-adaptation of config 2 on the device (main.c:107-134 order): initial data on
     the current mesh, then the model-5 indicator, criterionsmth and refinement
-    (fv_m5_dev_regrid), repeated; the host mesh is rebuilt from the device's leaves each pass."""
+    (fv_m5_dev_regrid), repeated; the host mesh is rebuilt from the device's leaves each pass.
+    With mlp, every pass is sampled for the learned indicator's training set (gradient labels)."""
     p = c2_params()
     m = F.Mesh.uniform((1, 1, 1), (0, 0, 0), L_DOMAIN, base, [2] * 6)
     crit = c2_crit(maxlevel, base, layers)
@@ -474,13 +477,15 @@ def build_c2(maxlevel=9, base=5, layers=64, log_fn=log):
     dev = F.M5Device(p, m)
     dev.set_forest(m)
     for ipass in range(maxlevel - base + 1):
-        dev.set_state(c2_state(p, m))
+        dev.set_state(c2_state(p, m, **init))
+        if mlp is not None:
+            dev.sample_mlp(mlp, gradient_crit(crit))
         dev.regrid(crit, maxlevel)
         root, level, ci, SI = dev.get_forest()
         m = F.Mesh.from_leaves((1, 1, 1), (0, 0, 0), L_DOMAIN, [2] * 6, root, level, ci)
         m.set_SI(SI)
         log_fn(f"  c2 geo pass {ipass}: {m.n} leaves, levels {np.bincount(level)[base:]}")
-    S = c2_state(p, m)
+    S = c2_state(p, m, **init)
     dev.set_state(S)
     m.set_SI(dev.interface_location(crit))
     return p, m, S, dev, crit
@@ -491,10 +496,11 @@ def gas_volume(S, level, L=L_DOMAIN):
     return float(((1.0 - S[:, 7] - S[:, 8]) * h ** 3).sum())
 
 
-def c2_loop(dev, crit, maxlevel, nsteps, every=5, cfl=0.4, t_end=None, probe=0, log_fn=log):
+def c2_loop(dev, crit, maxlevel, nsteps, every=5, cfl=0.4, t_end=None, probe=0, log_fn=log, mlp=None):
     """loop.c:12-30 for model 5: the CFL step, one SSP-RK3 step (timed with hipEvents per stage),
-    interface_location after it, and a device regrid every `every` steps; stops after nsteps or at
-    t_end.  probe > 0: the gas volume every `probe` steps."""
+    interface_location after it, and a device regrid every `every` steps (on the learned targets
+    with their slope flags when mlp is given); stops after nsteps or at t_end.  probe > 0: the gas
+    volume every `probe` steps."""
     t = 0.0
     ms_stage_tot = ms_res_tot = t_other = 0.0
     cells = 0
@@ -505,7 +511,10 @@ def c2_loop(dev, crit, maxlevel, nsteps, every=5, cfl=0.4, t_end=None, probe=0, 
     while step < nsteps and (t_end is None or t < t_end):
         t0 = time.perf_counter()
         if step > 0 and step % every == 0:
-            dev.regrid(crit, maxlevel)
+            if mlp is not None:
+                dev.regrid_mlp(mlp, crit, maxlevel)
+            else:
+                dev.regrid(crit, maxlevel)
             leaves.append(dev.n)
         dt = dev.cfl_dt(cfl)
         dev.set_dt(dt)
@@ -524,7 +533,7 @@ def c2_loop(dev, crit, maxlevel, nsteps, every=5, cfl=0.4, t_end=None, probe=0, 
             vols.append((t, gas_volume(dev.get_state(), level)))
             log_fn(f"  c2 step {step}: t {t:.3e} s, {dev.n} leaves, gas volume {vols[-1][1]:.4e} m^3")
     wall = time.perf_counter() - t_wall
-    return {"steps": step, "t": t, "leaves": leaves, "ms_stage": ms_stage_tot / (3 * step),
+    return {"steps": step, "t": t, "leaves": leaves, "ms_stage": ms_stage_tot / (3 * step), "cells": cells,
             "ms_residual": ms_res_tot / (3 * step), "stage_rate": cells / (ms_stage_tot * 1e-3) / 1e6,
             "loop_rate": cells / wall / 1e6, "other_s": t_other, "wall_s": wall, "vols": vols}
 
@@ -632,6 +641,147 @@ def c2_collapse(maxlevel=8, base=5, layers=64, every=5, t_over_rayleigh=1.6, log
             "gas_volume_over_v0": [[round(float(t / t_r), 4), round(float(v / v0), 4)] for t, v in zip(ts, vs)]}
 
 
+def train_on_c2_variants(mlp, log_fn=log):
+    """The model-5 half of config 4's training set: stencil/flag pairs of the plain gradient
+    indicator on seeded variants of config 2 (stand-off 1.0-2.0 radii, driving pressure 20-200
+    bar) pre-adapted to reference levels 4-7, sampled at every pre-adaptation pass, at the start
+    and after 15 steps."""
+    nv = 0
+    for standoff in (1.0, 1.5, 2.0):
+        for pl in (2e6, 2e7):
+            p, m, S, dev, crit = build_c2(maxlevel=7, base=4, layers=16, log_fn=lambda *a: None, mlp=mlp,
+                                          standoff=standoff, p_liquid=pl)
+            g = gradient_crit(crit)
+            dev.sample_mlp(mlp, g)
+            for _ in range(15):
+                dev.set_dt(dev.cfl_dt(0.4))
+                dev.advance(1)
+            dev.sample_mlp(mlp, g)
+            dev.close()
+            nv += 1
+    return nv
+
+
+def train_c4_network(log_fn=log, seed=7):
+    """Config 4's network: trained on the device on about 5 M stencil/flag pairs of the plain
+    indicator from seeded variants of configs 1 and 2 (model 4 bubbles and model 5 slab-and-bubble
+    cases, 20-200 bar, bubble moved / stand-off 1.0-2.0 radii), two epochs of Adam."""
+    mlp = F.MLP(seed=seed)
+    t0 = time.perf_counter()
+    train_on_variants(mlp, log_fn, nvariants=6)
+    n1 = mlp.dataset_size()
+    nv = train_on_c2_variants(mlp, log_fn)
+    npairs = mlp.dataset_size()
+    X, cls, sm = mlp.dataset()
+    del X
+    freq = np.bincount(cls, minlength=3).astype(float)
+    mlp.set_options(class_weights=(freq.sum() / (3.0 * np.maximum(freq, 1.0))).astype(np.float32))
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = mlp.train(2, batch=4096, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    log_fn(f"  c4 training set: {npairs} pairs ({n1} from 6 config-1 variants, {npairs - n1} from {nv} config-2 "
+           f"variants), classes {freq.astype(int)}; 2 epochs, loss {loss:.4f}, {t_train:.2f} s")
+    return mlp, {"train_pairs": int(npairs), "pairs_config1": int(n1), "pairs_config2": int(npairs - n1),
+                 "class_frequencies": [int(v) for v in freq], "loss": round(float(loss), 4),
+                 "dataset_build_s": round(t_data, 1), "train_s": round(t_train, 2)}
+
+
+def c4_leg(maxlevel=10, base=5, layers=64, steps=10, every=5, log_fn=log):
+    """Config 4: config 2 on reference levels 5-10 (config 4-9) with the learned indicator driving
+    the regrids (the plain refinement forced, the smooth flag choosing MC per leaf) and the
+    mixed-precision stage, against the plain FP64 path (plain indicator, minmod) in the same
+    driver from the same mesh and state."""
+    mlp, train = train_c4_network(log_fn)
+    t0 = time.perf_counter()
+    p, m, S, dev, crit = build_c2(maxlevel, base, layers, log_fn)
+    setup = time.perf_counter() - t0
+    dev.close()
+    out = {"workload": "c4_learned_indicator_c2_amr_L5-10", "levels": "5-10 (config 4-9)", "interface_layers": layers,
+           "leaves_start": m.n, "setup_s": round(setup, 1), "training": train}
+    res = {}
+    for leg, prec, limiter, learned in (("plain", F.PREC_FP64, 0, False), ("learned", F.PREC_MIXED, 2, True)):
+        q = c2_params()
+        q.limiter = limiter
+        q.dt = 1e-12
+        d = F.M5Device(q, m)
+        d.set_precision(prec)
+        d.set_forest(m)
+        d.set_state(S)
+        d.advance(1)                      # settle the kernels
+        d.set_state(S)
+        if learned:
+            d.indicator_mlp(mlp, crit)    # the slope flags of the start mesh
+        r = c2_loop(d, crit, maxlevel, steps, every, log_fn=log_fn, mlp=mlp if learned else None)
+        res[leg] = r
+        res[leg + "_state"] = d.get_state()
+        d.close()
+        log_fn(f"[c4] {leg}: leaves {r['leaves']}, {r['ms_stage']:.2f} ms/stage, {r['stage_rate']:.1f} Mcell-upd/s "
+               f"per stage, loop {r['loop_rate']:.1f}")
+    mlp.close()
+    for leg in ("plain", "learned"):
+        r = res[leg]
+        out[leg] = {"leaves": r["leaves"], "steps": r["steps"], "ms_per_stage": round(r["ms_stage"], 3),
+                    "value": round(r["stage_rate"], 2), "loop_value": round(r["loop_rate"], 2),
+                    "finite": bool(np.all(np.isfinite(res[leg + "_state"])))}
+    out["speedup_stage"] = round(res["learned"]["stage_rate"] / res["plain"]["stage_rate"], 3)
+    out["speedup_loop"] = round(res["learned"]["loop_rate"] / res["plain"]["loop_rate"], 3)
+    out["unit"] = "Mcell-updates/s per RK stage; loop_value with CFL, interface_location and regrids"
+    return out
+
+
+def c4_ab(maxlevel=7, ref_maxlevel=8, base=5, layers=32, every=5, t_over_rayleigh=1.6, log_fn=log):
+    """What the learned indicator gains over the plain one (criterionsplt.c:40-140): config 2 to
+    1.6 Rayleigh times on reference levels 5-7 driven by the plain indicator (minmod) and by the
+    learned one (its targets with the plain refinement forced, MC where it flags smooth flow), in
+    FP64, against the plain run on levels 5-8 as the reference.  The error is the mean absolute
+    difference of the gas volume history (in V0) from the reference's; with the mean leaf count."""
+    mlp, train = train_c4_network(log_fn)
+    t_r = 0.915 * R_BUBBLE * np.sqrt(1000.0 / (C2_P[0] - C2_P[1]))
+    runs = {}
+    for leg, lvl, learned in (("reference", ref_maxlevel, False), ("plain", maxlevel, False),
+                              ("learned", maxlevel, True)):
+        p, m, S, dev, crit = build_c2(lvl, base, layers, log_fn=lambda *a: None)
+        dev.close()
+        q = c2_params()
+        q.limiter = 2 if learned else 0
+        q.dt = 1e-12
+        d = F.M5Device(q, m)
+        d.set_forest(m)
+        d.set_state(S)
+        if learned:
+            d.indicator_mlp(mlp, crit)
+        v0 = gas_volume(S, m.arrays()["level"])
+        r = c2_loop(d, crit, lvl, 10 ** 7, every, t_end=t_over_rayleigh * t_r, probe=25, log_fn=lambda *a: None,
+                    mlp=mlp if learned else None)
+        d.close()
+        ts = np.array([0.0] + [v[0] for v in r["vols"]]) / t_r
+        vs = np.array([v0] + [v[1] for v in r["vols"]]) / v0
+        runs[leg] = {"t": ts, "v": vs, "leaves_mean": float(np.mean(r["leaves"])), "steps": r["steps"],
+                     "wall": r["wall_s"]}
+        log_fn(f"[c4 a/b] {leg}: levels {base}-{lvl}, {r['steps']} steps, mean leaves {np.mean(r['leaves']):.0f}, "
+               f"min gas volume {vs.min():.4f} at {ts[int(np.argmin(vs))]:.3f} t_R")
+    mlp.close()
+    ref = runs["reference"]
+    grid = np.linspace(0.0, t_over_rayleigh, 161)
+    vr = np.interp(grid, ref["t"], ref["v"])
+    out = {"workload": f"c2 to {t_over_rayleigh} Rayleigh times, levels {base}-{maxlevel} against a "
+                       f"levels {base}-{ref_maxlevel} plain reference, FP64, interface layers {layers}",
+           "training": train}
+    for leg in ("plain", "learned"):
+        v = np.interp(grid, runs[leg]["t"], runs[leg]["v"])
+        out[leg] = {"gas_volume_error_mean": float(np.mean(np.abs(v - vr))),
+                    "gas_volume_min": float(runs[leg]["v"].min()),
+                    "t_min_over_rayleigh": float(runs[leg]["t"][int(np.argmin(runs[leg]["v"]))]),
+                    "leaves_mean": round(runs[leg]["leaves_mean"], 1), "steps": runs[leg]["steps"]}
+    out["reference"] = {"gas_volume_min": float(ref["v"].min()), "t_min_over_rayleigh": float(ref["t"][int(np.argmin(ref["v"]))]),
+                        "leaves_mean": round(ref["leaves_mean"], 1), "steps": ref["steps"]}
+    out["error_ratio_learned_over_plain"] = round(out["learned"]["gas_volume_error_mean"] /
+                                                  max(out["plain"]["gas_volume_error_mean"], 1e-300), 3)
+    out["leaves_ratio_learned_over_plain"] = round(out["learned"]["leaves_mean"] / out["plain"]["leaves_mean"], 3)
+    return out
+
+
 def uniform_leg(level=9, warmup=3, steps=3, log_fn=log):
     """The FP64 stage on the uniform 2^(level-1) cube of the same bubble (256^3 =
     16.8 M leaves at level 9, every leaf in a 4x4x4 block): the block kernel's
@@ -720,7 +870,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20, help="timed SSP-RK3 steps (3 stages each)")
     ap.add_argument("--warmup", type=int, default=3, help="untimed SSP-RK3 steps before the timed ones")
     ap.add_argument("--workload", default="c1",
-                    choices=["c1", "u64", "u128", "u256", "u320", "c2", "c2_collapse", "c3"],
+                    choices=["c1", "u64", "u128", "u256", "u320", "c2", "c2_collapse", "c3", "c4", "c4_ab"],
                     help="c2: config 2 (model 5, the bubble near an elastoplastic slab, AMR levels 5-9); "
                          "c2_collapse: config 2 on levels 5-8 past the collapse; c3: config 3, the bubble adapted "
                          "to levels 5-10, FP64 against mixed precision")
@@ -752,6 +902,17 @@ def main():
         return
     if a.workload == "c2_collapse":
         print(json.dumps(c2_collapse()), flush=True)
+        return
+    if a.workload == "c4":
+        leg = c4_leg(steps=a.steps)
+        print(json.dumps({"metric": METRIC, "value": leg["learned"]["value"], "unit": "Mcell-updates/s per RK stage",
+                          "n_gpus": 1, "steps": leg["learned"]["steps"], "warmup": 1, "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32 faces / f64 state", "data": "synthetic",
+                          "config": {"workload": leg["workload"], "leaves": leg["leaves_start"]}, "c4": leg}),
+              flush=True)
+        return
+    if a.workload == "c4_ab":
+        print(json.dumps(c4_ab()), flush=True)
         return
 
     if a.workload == "c3":

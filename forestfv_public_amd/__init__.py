@@ -969,6 +969,11 @@ class MLP:
     def sample(self, dev, crit):
         _check(lib().fv_mlp_sample(dev._h, self._h, C.byref(crit)), "fv_mlp_sample")
 
+    def dataset_size(self):
+        n = C.c_long()
+        _check(lib().fv_mlp_dataset(self._h, C.byref(n), None, None, None), "dataset")
+        return n.value
+
     def dataset(self):
         n = C.c_long()
         _check(lib().fv_mlp_dataset(self._h, C.byref(n), None, None, None), "dataset")

@@ -472,9 +472,13 @@ __global__ __launch_bounds__(256) void k_mlp_infer_stencil(int n, const float4 *
   mlp_infer_wave((long)n, r0, wv, lane, xa, w1t, w2t, w3t, wf, hs, out);
 }
 
-// target level from the network, with the plain indicator's refinement forced
+// target level from the network, with the plain indicator's refinement forced:
+// where the plain target is at or above the leaf's level (it refines or keeps
+// the leaf, as in its interface bands) it is a floor, so the network never
+// coarsens a leaf that the plain indicator would refine back at the next regrid
 __global__ void k_mlp_decide(int n, const float *__restrict__ out, const uint32_t *__restrict__ info,
-                             int *__restrict__ split, uint8_t *__restrict__ lim, uint8_t *__restrict__ smooth) {
+                             int *__restrict__ split, uint8_t *__restrict__ lim, uint8_t *__restrict__ smooth,
+                             int maxlevel) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float *o = out + (size_t)i * MLP_NO;
@@ -482,8 +486,9 @@ __global__ void k_mlp_decide(int n, const float *__restrict__ out, const uint32_
   const int c = (o[0] >= o[1] && o[0] >= o[2]) ? 0 : (o[1] >= o[2] ? 1 : 2);
   int t = (c == 0) ? lvl + 1 : (c == 2 ? lvl - 1 : lvl);
   if (t < 1) t = 1;
+  if (t > maxlevel) t = maxlevel;   // the network refines no further than the run's finest level
   const int plain = split[i];
-  if (plain > lvl && plain > t) t = plain;
+  if (plain >= lvl && plain > t) t = plain;
   split[i] = t;
   const uint8_t s = o[3] > 0.f ? 1 : 0;
   lim[i] = s;
@@ -826,7 +831,8 @@ extern "C" int fv_dev_indicator_mlp(fv_dev *c, fv_mlp *m, const fv_crit *cr, int
     hipLaunchKernelGGL(k_mlp_fields<3>, g1(c->n), dim3(256), 0, c->stream, c->n, c->P, c->S, m->fld, m->snd);
   hipLaunchKernelGGL(k_mlp_infer_stencil, dim3((unsigned)((c->n + 63) / 64)), dim3(256), 0, c->stream, c->n, m->fld,
                      m->snd, c->nbr, c->info, c->fine4, m->w1t, m->w2t, m->w3t, m->w, m->iout);
-  hipLaunchKernelGGL(k_mlp_decide, g1(c->n), dim3(256), 0, c->stream, c->n, m->iout, c->info, c->split, c->lim, m->ismooth);
+  hipLaunchKernelGGL(k_mlp_decide, g1(c->n), dim3(256), 0, c->stream, c->n, m->iout, c->info, c->split, c->lim, m->ismooth,
+                     FV_MAXLEVEL);
   HIPCHK(hipGetLastError());
   if (SI && (rc = fv_copy_to_host(c, SI, SIdev, sizeof(int) * 2 * (size_t)c->n)) != FV_OK) return rc;
   if (split && (rc = fv_copy_to_host(c, split, c->split, sizeof(int) * (size_t)c->n)) != FV_OK) return rc;
@@ -897,7 +903,8 @@ extern "C" int fv_m5_dev_indicator_mlp(fv_m5_dev *c, fv_mlp *m, const fv_crit *c
   fv_m5_view v;
   if (rc == FV_OK) rc = m5_infer(c, m, v);
   if (rc != FV_OK) return rc;
-  hipLaunchKernelGGL(k_mlp_decide, g1(v.n), dim3(256), 0, v.stream, v.n, m->iout, v.info, sp, v.lim, m->ismooth);
+  hipLaunchKernelGGL(k_mlp_decide, g1(v.n), dim3(256), 0, v.stream, v.n, m->iout, v.info, sp, v.lim, m->ismooth,
+                     FV_MAXLEVEL);
   HIPCHK(hipGetLastError());
   if (SI) HIPCHK(hipMemcpyAsync(SI, SIdev, sizeof(int) * 2 * (size_t)v.n, hipMemcpyDeviceToHost, v.stream));
   if (split) HIPCHK(hipMemcpyAsync(split, sp, sizeof(int) * (size_t)v.n, hipMemcpyDeviceToHost, v.stream));
@@ -916,7 +923,8 @@ extern "C" int fv_m5_dev_regrid_mlp(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr, 
   fv_m5_view v;
   if (rc == FV_OK) rc = m5_infer(c, m, v);
   if (rc != FV_OK) return rc;
-  hipLaunchKernelGGL(k_mlp_decide, g1(v.n), dim3(256), 0, v.stream, v.n, m->iout, v.info, sp, v.lim, m->ismooth);
+  hipLaunchKernelGGL(k_mlp_decide, g1(v.n), dim3(256), 0, v.stream, v.n, m->iout, v.info, sp, v.lim, m->ismooth,
+                     maxlevel);
   HIPCHK(hipGetLastError());
   if ((rc = fv_m5_regrid_targets(c, sp, SIdev, maxlevel, n_out)) != FV_OK) return rc;
   if ((rc = m5_infer(c, m, v)) != FV_OK) return rc;

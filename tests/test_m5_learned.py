@@ -135,7 +135,7 @@ def test_m5_learned_indicator_features_labels_and_forced_refinement():
     mlp.train(1, batch=256, lr=2e-3)
     SI2, learned, smooth = dev.indicator_mlp(mlp, crit)
     assert np.array_equal(SI2, SI)
-    assert np.all(learned[plain > lvl] >= plain[plain > lvl])
+    assert np.all(learned[plain >= lvl] >= plain[plain >= lvl])
     assert np.array_equal(dev.get_limiter_flags(), smooth)
     dev.close()
     mlp.close()

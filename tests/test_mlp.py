@@ -166,7 +166,7 @@ def test_learned_indicator_training_and_gain():
     _, plain = dev.indicator(crit)
     _, learned, smooth = dev.indicator_mlp(mlp, crit)
     lvl = m.arrays()["level"]
-    assert np.all(learned[plain > lvl] >= plain[plain > lvl])
+    assert np.all(learned[plain >= lvl] >= plain[plain >= lvl])
     assert np.array_equal(dev.get_limiter_flags(), smooth)
     dev.close()
     mlp.close()
@@ -258,7 +258,7 @@ def test_fused_stencil_inference_matches_feature_rows():
     o0, o1, o2 = out[:, 0], out[:, 1], out[:, 2]
     c = np.where((o0 >= o1) & (o0 >= o2), 0, np.where(o1 >= o2, 1, 2))
     t = np.maximum(np.where(c == 0, lvl + 1, np.where(c == 2, lvl - 1, lvl)), 1)
-    t = np.where((plain > lvl) & (plain > t), plain, t)
+    t = np.where((plain >= lvl) & (plain > t), plain, t)
     say(f"[mlp stencil] classes {np.bincount(c, minlength=3)}, smooth {int((out[:, 3] > 0).sum())}")
     assert np.array_equal(split, t)
     assert np.array_equal(sm, (out[:, 3] > 0).astype(np.uint8))
