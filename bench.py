@@ -179,6 +179,35 @@ def mixture_pressure(p, S):
     return ((g[0] - 1.0) * S[:, 7] - a0 * g[0] * pi[0]) + ((g[1] - 1.0) * S[:, 8] - (1.0 - a0) * g[1] * pi[1])
 
 
+def m5_mixture_pressure(p, S, chunk=1 << 22):
+    """sum_k alpha_k p_k of model-5 states in the mixture mode: p_k = (gamma_k - 1) (alpha rho e_k
+    - alpha_k E_k(G)) / alpha_k - gamma_k pinf_k with E_k = mu_k / (4 |G|^(1/6)) tr((G - |G|^(1/3)
+    I)^2), G = A A^T (state_equation.c:269-316 weighted by alpha_k); fractions as stored (the
+    record's clip at AMIN / 100 does not change them in these states)."""
+    nm = p.nmat
+    out = np.empty(S.shape[0])
+    for a0 in range(0, S.shape[0], chunk):
+        X = S[a0:a0 + chunk]
+        al = np.empty((X.shape[0], nm))
+        al[:, :nm - 1] = X[:, nm + 4:2 * nm + 3]
+        al[:, nm - 1] = 1.0 - al[:, :nm - 1].sum(1)
+        are = X[:, 2 * nm + 3:3 * nm + 3]
+        A = X[:, 3 * nm + 3:].reshape(-1, 3, 3)
+        G = A @ np.transpose(A, (0, 2, 1))
+        d = np.abs(np.linalg.det(G))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            M = G - np.cbrt(d)[:, None, None] * np.eye(3)
+            tr = np.einsum("nij,nji->n", M, M)
+            E1 = np.where(d != 0.0, tr / (4.0 * d ** (1.0 / 6.0)), 0.0)
+        pm = np.zeros(X.shape[0])
+        for k in range(nm):
+            g, pi, mu = p.gamma[k], p.pinf[k], p.mush[k]
+            pk = (g - 1.0) * (are[:, k] - al[:, k] * mu * E1) / al[:, k] - g * pi
+            pm += al[:, k] * pk
+        out[a0:a0 + chunk] = pm
+    return out
+
+
 def rel_l1(a, b):
     return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
 
@@ -589,7 +618,11 @@ def c2_leg(maxlevel=9, base=5, layers=64, steps=15, every=5, check=True, log_fn=
     scale[nm:nm + 3] = np.sqrt((S64[:, nm:nm + 3] ** 2).sum(1)).sum()
     scale[-9:] = np.sqrt((S64[:, -9:] ** 2).sum(1)).sum()
     l1 = np.abs(Smx - S64).sum(0) / np.maximum(scale, 1e-300)
-    out["mixed"] = {"ms_per_stage": round(mesh_res[F.PREC_MIXED][1], 3),
+    rho64, rhomx = S64[:, :nm].sum(1), Smx[:, :nm].sum(1)
+    out_l1_rho = rel_l1(rhomx, rho64)
+    out_l1_p = rel_l1(m5_mixture_pressure(p, Smx), m5_mixture_pressure(p, S64))
+    out["mixed"] = {"rel_l1_rho_vs_fp64_10steps": out_l1_rho, "rel_l1_p_vs_fp64_10steps": out_l1_p,
+                    "ms_per_stage": round(mesh_res[F.PREC_MIXED][1], 3),
                     "ms_per_stage_fp64_same_mesh": round(mesh_res[F.PREC_FP64][1], 3),
                     "speedup_vs_fp64_stage": round(mesh_res[F.PREC_FP64][1] / mesh_res[F.PREC_MIXED][1], 3),
                     "rel_l1_vs_fp64_10steps": {k: float(v) for k, v in zip(names, l1)},
@@ -711,7 +744,8 @@ def c4_leg(maxlevel=10, base=5, layers=64, steps=10, every=5, log_fn=log):
         d.advance(1)                      # settle the kernels
         d.set_state(S)
         if learned:
-            d.indicator_mlp(mlp, crit)    # the slope flags of the start mesh
+            _, _, flags = d.indicator_mlp(mlp, crit)    # the slope flags of the start mesh
+            res["mlp_flags"] = flags
         r = c2_loop(d, crit, maxlevel, steps, every, log_fn=log_fn, mlp=mlp if learned else None)
         res[leg] = r
         res[leg + "_state"] = d.get_state()
@@ -724,6 +758,31 @@ def c4_leg(maxlevel=10, base=5, layers=64, steps=10, every=5, log_fn=log):
         out[leg] = {"leaves": r["leaves"], "steps": r["steps"], "ms_per_stage": round(r["ms_stage"], 3),
                     "value": round(r["stage_rate"], 2), "loop_value": round(r["loop_rate"], 2),
                     "finite": bool(np.all(np.isfinite(res[leg + "_state"])))}
+    # the learned leg's precision: FP64 and mixed stages with the same network's slope flags, ten
+    # steps on the start mesh at the CFL step
+    mlp2 = res.pop("mlp_flags")
+    fin = {}
+    for prec in (F.PREC_FP64, F.PREC_MIXED):
+        q = c2_params()
+        q.limiter = 2
+        q.dt = 1e-12
+        d = F.M5Device(q, m)
+        d.set_precision(prec)
+        d.set_state(S)
+        d.set_limiter_flags(mlp2)
+        d.set_dt(d.cfl_dt(0.4))
+        d.advance(10)
+        fin[prec] = d.get_state()
+        d.close()
+    nm = 3
+    out["mixed_vs_fp64_10steps"] = {
+        "rel_l1_rho": rel_l1(fin[F.PREC_MIXED][:, :nm].sum(1), fin[F.PREC_FP64][:, :nm].sum(1)),
+        "rel_l1_p": rel_l1(m5_mixture_pressure(c2_params(), fin[F.PREC_MIXED]),
+                           m5_mixture_pressure(c2_params(), fin[F.PREC_FP64])),
+        "case": "the start mesh, the learned slope flags, ten steps at the CFL-0.4 step"}
+    del fin
+    log_fn(f"[c4] mixed vs FP64 after 10 steps: rho L1 {out['mixed_vs_fp64_10steps']['rel_l1_rho']:.2e}, "
+           f"p L1 {out['mixed_vs_fp64_10steps']['rel_l1_p']:.2e}")
     out["speedup_stage"] = round(res["learned"]["stage_rate"] / res["plain"]["stage_rate"], 3)
     out["speedup_loop"] = round(res["learned"]["loop_rate"] / res["plain"]["loop_rate"], 3)
     out["unit"] = "Mcell-updates/s per RK stage; loop_value with CFL, interface_location and regrids"
@@ -884,6 +943,7 @@ def main():
     ap.add_argument("--no-collapse", action="store_true", help="skip the run to the first collapse")
     ap.add_argument("--no-m5", action="store_true", help="skip the config-2 leg (model 5: solid, GFM, AMR)")
     ap.add_argument("--no-u256", action="store_true", help="skip the uniform 256^3 block-kernel roofline leg")
+    ap.add_argument("--no-c4", action="store_true", help="skip the config-4 leg (learned indicator on config 2)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU halo exchange is "
@@ -994,6 +1054,8 @@ def main():
         out["collapse"] = collapse_leg(p, a.layers)
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_m5:
         out["c2"] = c2_leg()
+    if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_c4:
+        out["c4"] = c4_leg()
     if a.workload == "c1" and a.prec == "fp64" and not a.fused and not a.no_u256:
         out["u256"] = uniform_leg(9)
     if not a.no_cpu_baseline:
