@@ -425,7 +425,7 @@ class M5Device:
     """The whole model-5 stage on the device (fv_m5_dev_*): MUSCL face states, the residual with
     its HLLC and GFM fluxes, m5_RK and the relaxation, on the leaves and face kinds of `mesh`."""
 
-    def __init__(self, params, mesh):
+    def __init__(self, params, mesh, prec=None):
         self.params = params
         self.pack = mesh.pack()
         self.n = mesh.n
@@ -434,6 +434,8 @@ class M5Device:
         _check(lib().fv_m5_dev_attach(C.byref(params), C.byref(self.pack.pk), C.byref(h)), "fv_m5_dev_attach")
         self.h = h
         _track(self)
+        if prec is not None:
+            self.set_precision(prec)
 
     def close(self):
         if getattr(self, "h", None):
@@ -461,7 +463,8 @@ class M5Device:
         _check(lib().fv_m5_dev_begin_step(self.h), "fv_m5_dev_begin_step")
 
     def set_precision(self, prec):
-        """PREC_FP64 (default) or PREC_MIXED: FP32 slopes, face records, HLLC and GFM flux."""
+        """PREC_EXACT (the reference's operation order), PREC_FP64 (default: the non-conservative
+        sources formed once per leaf) or PREC_MIXED (FP32 slopes, face records, HLLC and GFM flux)."""
         _check(lib().fv_m5_dev_set_precision(self.h, prec), "fv_m5_dev_set_precision")
 
     def stage(self, irk):

@@ -1884,21 +1884,21 @@ __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__
   if (!ok) atomicOr(flag, 1);
 }
 
-// The mixed-precision residual: k_m5_residual's face loop with FP32 face
-// records, HLLC and GFM flux, the area-weighted face fluxes added in FP64
-// straight into the residual, and the non-conservative terms formed once after
-// the loop.  Every source term of m5_residual.c:176-214 (and of the consistent
-// mode) is linear in the face's u* n^T, so they follow from
-// T_ij = sum_f Area_f n_i u*_j and the cell record, which is built after the
-// loop and holds no registers during it.
-template <int NM>
+// The residual with the non-conservative terms formed once after the face loop
+// (R = float: the mixed-precision residual with FP32 face records, HLLC and GFM
+// flux; R = double: the default FP64 residual).  k_m5_residual's face loop, the
+// area-weighted face fluxes added in FP64 straight into the residual.  Every
+// source term of m5_residual.c:176-214 (and of the consistent mode) is linear
+// in the face's u* n^T, so they follow from T_ij = sum_f Area_f n_i u*_j and the
+// cell record, which is built after the loop and holds no registers during it.
+// The sums differ from k_m5_residual's face-by-face source terms by rounding.
+template <int NM, typename R>
 __global__ __launch_bounds__(256) void k_m5_residual_mx(int n, M5P P, const Geo *__restrict__ Gp,
                                                         const double *__restrict__ S, const double *__restrict__ SF,
                                                         const int *__restrict__ nbr, const uint32_t *__restrict__ info,
                                                         const int *__restrict__ fine4, double *__restrict__ RHS,
                                                         uint8_t *__restrict__ gfm, int *__restrict__ flag) {
   using L = L5<NM>;
-  using R = float;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Geo &G = *Gp;
@@ -2283,7 +2283,10 @@ static hipError_t dalloc(T **p, size_t count) {
 struct fv_m5_dev {
   M5P P{};
   int n = 0, nfine = 0, neq = 0, rk_steps = 3;
-  int prec = FV_PREC_FP64;    // FV_PREC_MIXED: FP32 face records, HLLC and GFM flux (fv_m5_dev_set_precision)
+  // FV_PREC_EXACT: the reference's face-by-face source terms (bit for bit where
+  // no pow runs); FV_PREC_FP64: the sources formed once per leaf; FV_PREC_MIXED:
+  // that with FP32 face records, HLLC and GFM flux (fv_m5_dev_set_precision)
+  int prec = FV_PREC_FP64;
   hipStream_t stream = nullptr;
   Geo *G = nullptr;
   int *nbr = nullptr, *fine4 = nullptr;
@@ -2370,8 +2373,11 @@ int m5_faceinterp_t(fv_m5_dev *c) {
 template <int NM>
 int m5_residual_t(fv_m5_dev *c) {
   if (c->prec == FV_PREC_MIXED)
-    hipLaunchKernelGGL(k_m5_residual_mx<NM>, grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S, c->SF,
-                       c->nbr, c->info, c->fine4, c->RHS, c->gfm, c->flag);
+    hipLaunchKernelGGL((k_m5_residual_mx<NM, float>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
+                       c->SF, c->nbr, c->info, c->fine4, c->RHS, c->gfm, c->flag);
+  else if (c->prec == FV_PREC_FP64)
+    hipLaunchKernelGGL((k_m5_residual_mx<NM, double>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
+                       c->SF, c->nbr, c->info, c->fine4, c->RHS, c->gfm, c->flag);
   else
     hipLaunchKernelGGL((k_m5_residual<NM, double>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
                        c->SF, c->nbr, c->info, c->fine4, c->RHS, c->gfm, c->flag);
@@ -2926,7 +2932,7 @@ extern "C" int fv_m5_dev_set_precision(fv_m5_dev *c, int prec) {
   if (!c) return fv_set_err(FV_ERR_ARG, "null argument");
   if (prec != FV_PREC_FP64 && prec != FV_PREC_EXACT && prec != FV_PREC_MIXED)
     return fv_set_err(FV_ERR_ARG, "model 5: precision must be FV_PREC_FP64, FV_PREC_EXACT or FV_PREC_MIXED");
-  c->prec = (prec == FV_PREC_MIXED) ? FV_PREC_MIXED : FV_PREC_FP64;
+  c->prec = prec;
   return FV_OK;
 }
 

@@ -139,11 +139,13 @@ int fv_m5_dev_stage(fv_m5_dev *c, int irk);
 /* nsteps steps of stages 1..rk_steps (loop.c:21) */
 int fv_m5_dev_advance(fv_m5_dev *c, int nsteps);
 /* Precision of the stage (no counterpart in the reference, which is FP64):
- * FV_PREC_FP64 (default; FV_PREC_EXACT is the same, the model-5 kernels keep
- * the reference's operation order) or FV_PREC_MIXED: FP32 limited slopes in
- * the MUSCL face states, FP32 face records, HLLC, star states and GFM ghost
- * flux; the records are formed in FP64 (pressures carry p_inf up to 2e10 Pa),
- * and the residual, RK accumulation and relaxation stay FP64. */
+ * FV_PREC_EXACT keeps the reference's operation order (the non-conservative
+ * sources added face by face, m5_residual.c:176-214); FV_PREC_FP64 (default)
+ * forms them once per leaf from the face velocities' sum over faces, equal to
+ * rounding; FV_PREC_MIXED adds FP32 limited slopes in the MUSCL face states,
+ * FP32 face records, HLLC, star states and GFM ghost flux; the records are
+ * formed in FP64 (pressures carry p_inf up to 2e10 Pa), and the residual, RK
+ * accumulation and relaxation stay FP64. */
 int fv_m5_dev_set_precision(fv_m5_dev *c, int prec);
 /* Per-leaf slope flags of limiter 2 ([n], 0 minmod, 1 MC; new: the
  * reference reconstructs with minmod only, m5_reconctruct.c:93-140). */

@@ -174,7 +174,7 @@ def fixed_case(seed, amr, prelax, table_name):
 def test_m5_device_consistent_face_states_and_residual_match_oracle(table, prelax, amr):
     p, m, om, S = fixed_case(140 + prelax, amr, prelax, table)
     say(f"[m5 fix dev] {table} prelax={prelax} amr={amr} n={m.n}")
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     dev.faceinterp()
     dev.residual()
@@ -202,7 +202,7 @@ def test_m5_device_consistent_steps_match_oracle(prelax):
     p, m, om, S = fixed_case(150 + prelax, True, prelax, "solid")
     p.dt = 2e-9
     say(f"[m5 fix dev steps] prelax={prelax} n={m.n}")
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     dev.advance(10)
     Sd = dev.get_state()
@@ -224,7 +224,7 @@ def test_m5_device_consistent_solid_pulse_matches_oracle():
     p.dt = 0.3 * (L / N) / cL
     S0 = pulse(p, m)
     say(f"[m5 fix dev pulse] n={m.n} dt={p.dt:.3e}")
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S0)
     dev.advance(30)
     Sd = dev.get_state()

@@ -250,4 +250,4 @@ def test_m5_device_interface_location_refreshes_face_kinds():
     m.set_SI(SI)
     So = O.OracleMesh(m).m5_advance(p, S3, 1)
     scale = np.abs(So).max(axis=0) + 1e-300
-    assert (np.abs(Sd - So).max(axis=0) / scale).max() <= 1e-11
+    assert (np.abs(Sd - So).max(axis=0) / scale).max() <= 1e-10   # four steps with PRELAX 3 and the elastic pow

@@ -523,7 +523,7 @@ def test_m5_device_face_states_and_residual_match_oracle(name, amr):
     table, nmat, prelax, bitwise = DEVICE_CASES[name]
     p, m, om, S = m5_case(100 + nmat + prelax, amr=amr, nmat=nmat, table=table, prelax=prelax)
     say(f"[m5 dev] {name} amr={amr} n={m.n}")
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     dev.faceinterp()
     dev.residual()
@@ -543,7 +543,7 @@ def test_m5_device_stages_match_oracle(name):
     table, nmat, prelax, bitwise = DEVICE_CASES[name]
     p, m, om, S = m5_case(110 + nmat + prelax, nmat=nmat, table=table, prelax=prelax)
     p.dt = 2e-9
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     dev.begin_step()
     SN = S.copy()
@@ -571,7 +571,7 @@ def test_m5_device_steps_match_oracle(name):
     table, nmat, prelax, bitwise = DEVICE_CASES[name]
     p, m, om, S = m5_case(120 + nmat + prelax, nmat=nmat, table=table, prelax=prelax)
     p.dt = 2e-9
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     dev.advance(3)
     Sd = dev.get_state()
@@ -590,7 +590,7 @@ def test_m5_device_boundary_codes_match_oracle(bc):
     """Symmetry faces average the mirrored cell fluxes (m5_hllc.c:66-88), walls force u* = 0 or
     mirror all velocity components, outlets copy the face state."""
     p, m, om, S = m5_case(130 + bc[0], nmat=3, table=FLUIDS, prelax=0, bc=bc)
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     dev.faceinterp()
     dev.residual()
@@ -605,7 +605,7 @@ def test_m5_device_boundary_codes_match_oracle(bc):
 @pytest.mark.gpu
 def test_m5_device_first_order_matches_oracle():
     p, m, om, S = m5_case(140, nmat=3, table=FLUIDS, prelax=0, order=1)
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     dev.residual()
     _, RHS, gf = dev.stage_arrays()
@@ -624,7 +624,7 @@ def test_m5_device_and_oracle_fail_on_the_same_run():
     p.dt = 2e-9
     with pytest.raises(RuntimeError):
         om.m5_advance(p, S, 3)
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     with pytest.raises(F.FVError) as e:
         dev.advance(3)
@@ -639,7 +639,7 @@ def test_m5_device_reports_nan_as_a_status():
     p, m, om, S = m5_case(141, nmat=3, table=FLUIDS, prelax=0)
     S[5, :3] = 0.0
     S[5, 3:6] = 0.0
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     with pytest.raises(F.FVError) as e:
         dev.advance(1)
@@ -658,7 +658,7 @@ def test_m5_device_single_leaf_and_root_bricks_match_oracle(nroot, level):
     rng = np.random.default_rng(150 + level)
     S = smooth_field(p, m, rng, 2.0 ** -6 * nroot[0])
     om = O.OracleMesh(m)
-    dev = F.M5Device(p, m)
+    dev = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev.set_state(S)
     dev.faceinterp()
     dev.residual()
@@ -668,9 +668,40 @@ def test_m5_device_single_leaf_and_root_bricks_match_oracle(nroot, level):
     say(f"[m5 dev small] nroot={nroot} level={level} n={m.n}")
     assert np.array_equal(SF, SFo) and np.array_equal(RHS, RHSo) and np.array_equal(gf, gfo)
     p.dt = 1e-10
-    dev2 = F.M5Device(p, m)
+    dev2 = F.M5Device(p, m, prec=F.PREC_EXACT)
     dev2.set_state(S)
     dev2.advance(1)
     assert np.array_equal(dev2.get_state(), om.m5_advance(p, S, 1))
     dev.close()
     dev2.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("table,prelax,amr", [("fluids", 0, True), ("solid", 1, False), ("solid", 3, True)])
+def test_m5_device_fp64_sources_per_leaf_equal_exact_to_rounding(table, prelax, amr):
+    """The default FP64 residual forms the non-conservative terms once per leaf from the sum of
+    the face velocities (linear in each face's u* n^T); the exact path adds them face by face as
+    m5_residual.c:176-214 does.  The two agree to rounding per stage and over three steps."""
+    tab = SOLID_WATER_AIR if table == "solid" else FLUIDS
+    p, m, om, S = m5_case(170 + prelax, amr=amr, nmat=3, table=tab, prelax=prelax)
+    p.fix = F.M5_FIX_SIGNS | F.M5_FIX_MIXTURE   # a runnable update for the three steps
+    p.dt = 1e-9
+    out = {}
+    for prec in (F.PREC_EXACT, F.PREC_FP64):
+        dev = F.M5Device(p, m, prec=prec)
+        dev.set_state(S)
+        dev.faceinterp()
+        dev.residual()
+        _, RHS, gf = dev.stage_arrays()
+        dev.set_state(S)
+        dev.advance(3)
+        out[prec] = (RHS, gf, dev.get_state())
+        dev.close()
+    (R0, g0, S0), (R1, g1, S1) = out[F.PREC_EXACT], out[F.PREC_FP64]
+    assert np.array_equal(g0, g1)
+    scale = np.abs(R0).max(axis=0) + 1e-300
+    assert (np.abs(R1 - R0).max(axis=0) / scale).max() <= 1e-12
+    scale = np.abs(S0).max(axis=0) + 1e-300
+    err = (np.abs(S1 - S0).max(axis=0) / scale).max()
+    say(f"[m5 fp64 vs exact] {table} prelax={prelax}: three steps, largest difference {err:.2e}")
+    assert err <= 1e-12
