@@ -1457,6 +1457,31 @@ __device__ void store_state(double *X, int n, long q, const double *x) {
   for (int v = 0; v < L::NEQ; v++) X[(size_t)f * L::NEQ * n + (size_t)v * n + i] = x[v];
 }
 
+// the first R3_CAP iterations of state q held in x: true when the loop ended
+// and x is relaxed; false when x keeps its unrelaxed values (marked and saved
+// for the resume passes, or NaN, which raises the flag)
+template <int NM>
+__device__ bool r3_first(const M5P &P, double *x, long q, double *__restrict__ nst, int *__restrict__ iters,
+                         uint8_t *__restrict__ mark, int *__restrict__ flag) {
+  RelaxIn<NM> R;
+  relax_in<NM>(P, x, R);
+  double pk[NM], pr, Fv = 1.0e-9, dF = 1.0e9;
+  int iter = 0;
+  const int irelax = r3_setup<NM>(P, R, pk, pr);
+  int rc = 1;
+  if (irelax == 1) rc = r3_newton<NM>(P, R, pk, pr, Fv, dF, iter, R3_CAP);
+  if (rc < 0) { atomicOr(flag, 1); mark[q] = 0; return false; }
+  if (rc == 0) {
+    nst[3 * q + 0] = pr; nst[3 * q + 1] = Fv; nst[3 * q + 2] = dF;
+    iters[q] = iter;
+    mark[q] = 1;
+    return false;
+  }
+  mark[q] = 0;
+  if (r3_finish<NM>(P, R, pk, irelax, pr, x) < 0) atomicOr(flag, 1);
+  return true;
+}
+
 template <int NM>
 __global__ __launch_bounds__(256) void k_m5_relax3_start(long count, int n, M5P P, double *__restrict__ X,
                                                          double *__restrict__ Sb, double *__restrict__ nst,
@@ -1468,23 +1493,7 @@ __global__ __launch_bounds__(256) void k_m5_relax3_start(long count, int n, M5P 
   double x[L::NEQ];
   load_state<NM>(X, n, q, x);
   if (Sb) store_state<NM>(Sb, n, q, x);
-  RelaxIn<NM> R;
-  relax_in<NM>(P, x, R);
-  double pk[NM], pr, Fv = 1.0e-9, dF = 1.0e9;
-  int iter = 0;
-  const int irelax = r3_setup<NM>(P, R, pk, pr);
-  int rc = 1;
-  if (irelax == 1) rc = r3_newton<NM>(P, R, pk, pr, Fv, dF, iter, R3_CAP);
-  if (rc < 0) { atomicOr(flag, 1); mark[q] = 0; return; }
-  if (rc == 0) {
-    nst[3 * q + 0] = pr; nst[3 * q + 1] = Fv; nst[3 * q + 2] = dF;
-    iters[q] = iter;
-    mark[q] = 1;
-    return;
-  }
-  mark[q] = 0;
-  if (r3_finish<NM>(P, R, pk, irelax, pr, x) < 0) atomicOr(flag, 1);
-  store_state<NM>(X, n, q, x);
+  if (r3_first<NM>(P, x, q, nst, iters, mark, flag)) store_state<NM>(X, n, q, x);
 }
 
 // resumes the listed states for at most cap more iterations; those still
@@ -1597,7 +1606,9 @@ __device__ void dist5(const Geo &G, int kind, int lvl, double &dcf, double &dcc)
 // m5_faceinterp_V1 (m5_faceinterp.c:60-93) for one leaf per thread: MUSCL face
 // states with minmod (m5_reconctruct.c:93-190; g_limitedvars is 0 on the
 // alpha-rho-e block, m5_input.c:196-204), the VF limit and m5_relaxationvec
-// (PRELAX 3: left to k_m5_relax3_start / _finish).  R = float (mixed precision)
+// (RX3: PRELAX 3's first R3_CAP Newton iterations of each face state, the
+// states still iterating left marked for k_m5_relax3_resume; the face states
+// then go to HBM once).  R = float (mixed precision)
 // forms the limited slopes in FP32 from the FP64 differences and adds them to
 // the FP64 cell state; the relaxation stays FP64.
 // the limited slope of one variable: minmod as m5_reconctruct.c:110-140 has it,
@@ -1622,12 +1633,13 @@ __device__ __forceinline__ R m5_slope(bool mc, R df, R db, R dp, R dm) {
   return R(0.0);
 }
 
-template <int NM, typename R>
+template <int NM, typename R, bool RX3>
 __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__restrict__ Gp,
                                                   const double *__restrict__ S, const int *__restrict__ nbr,
                                                   const uint32_t *__restrict__ info, const int *__restrict__ fine4,
                                                   const uint8_t *__restrict__ limf, double *__restrict__ SF,
-                                                  int *__restrict__ flag) {
+                                                  double *__restrict__ nst, int *__restrict__ iters,
+                                                  uint8_t *__restrict__ mark, int *__restrict__ flag) {
   using L = L5<NM>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1688,9 +1700,12 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
         Fv[L::I4 + k] = Sc[L::I4 + k] * (af / ac);
       }
     }
-    // PRELAX 3 relaxes the face states in k_m5_relax3_start / _finish
-    const int rc = (P.prelax == 3) ? 0 : relax5<NM>(P, Fv);
-    if (rc) atomicOr(flag, rc == -3 ? 4 : 1);
+    if constexpr (RX3) {
+      r3_first<NM>(P, Fv, (long)f * n + i, nst, iters, mark, flag);
+    } else {
+      const int rc = relax5<NM>(P, Fv);
+      if (rc) atomicOr(flag, rc == -3 ? 4 : 1);
+    }
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) SF[((size_t)f * L::NEQ + v) * n + i] = Fv[v];
   }
@@ -2334,12 +2349,15 @@ int m5_compact(fv_m5_dev *c, const uint8_t *mark, int *list, int items) {
 }
 
 // PRELAX 3 on `count` states of X (see k_m5_relax3_start)
+// (started: the first pass ran inside the producing kernel)
 template <int NM>
-int m5_relax3(fv_m5_dev *c, double *X, long count, double *Sb) {
+int m5_relax3(fv_m5_dev *c, double *X, long count, double *Sb, bool started = false) {
   const dim3 g((unsigned)((count + 255) / 256)), b(256);
-  hipLaunchKernelGGL(k_m5_relax3_start<NM>, g, b, 0, c->stream, count, c->n, c->P, X, Sb, c->nst, c->iters, c->mark6,
-                     c->flag);
-  HIPCHK(hipGetLastError());
+  if (!started) {
+    hipLaunchKernelGGL(k_m5_relax3_start<NM>, g, b, 0, c->stream, count, c->n, c->P, X, Sb, c->nst, c->iters,
+                       c->mark6, c->flag);
+    HIPCHK(hipGetLastError());
+  }
   // the states still iterating: 24 more iterations for all of them, then the
   // few that stagnate run to the reference's limit of 500 in full waves
   const dim3 gr((unsigned)std::min<long>(4096, (count + 255) / 256));
@@ -2357,16 +2375,21 @@ int m5_relax3(fv_m5_dev *c, double *X, long count, double *Sb) {
 
 dim3 grid_of(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
+template <int NM, typename R, bool RX3>
+void m5_recon_launch(fv_m5_dev *c) {
+  hipLaunchKernelGGL((k_m5_recon<NM, R, RX3>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
+                     c->nbr, c->info, c->fine4, c->lim, c->SF, c->nst, c->iters, c->mark6, c->flag);
+}
+
 template <int NM>
 int m5_faceinterp_t(fv_m5_dev *c) {
+  const bool rx3 = c->P.prelax == 3;
   if (c->prec == FV_PREC_MIXED)
-    hipLaunchKernelGGL((k_m5_recon<NM, float>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
-                       c->nbr, c->info, c->fine4, c->lim, c->SF, c->flag);
+    rx3 ? m5_recon_launch<NM, float, true>(c) : m5_recon_launch<NM, float, false>(c);
   else
-    hipLaunchKernelGGL((k_m5_recon<NM, double>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
-                       c->nbr, c->info, c->fine4, c->lim, c->SF, c->flag);
+    rx3 ? m5_recon_launch<NM, double, true>(c) : m5_recon_launch<NM, double, false>(c);
   HIPCHK(hipGetLastError());
-  if (c->P.prelax == 3) return m5_relax3<NM>(c, c->SF, 6L * c->n, nullptr);
+  if (rx3) return m5_relax3<NM>(c, c->SF, 6L * c->n, nullptr, true);
   return FV_OK;
 }
 

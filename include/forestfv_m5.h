@@ -192,7 +192,6 @@ int fv_m5_dev_regrid(fv_m5_dev *c, const fv_crit *cr, int maxlevel, int *n_out);
  *    slope flags of limiter 2;
  *  - fv_m5_dev_regrid_mlp adapts the mesh to those targets and sets the new
  *    leaves' slope flags from the network. */
-typedef struct fv_mlp fv_mlp;
 int fv_mlp_sample_m5(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr);
 int fv_m5_dev_indicator_mlp(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr, int *SI, int *split, uint8_t *smooth);
 int fv_m5_dev_regrid_mlp(fv_m5_dev *c, fv_mlp *m, const fv_crit *cr, int maxlevel, int *n_out);
