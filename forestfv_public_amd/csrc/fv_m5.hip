@@ -46,6 +46,7 @@ struct M5P {
   int limiter;    // MUSCL slope: 0 minmod, 1 monotonised central, 2 per leaf (fv_m5_dev.lim)
   double dtu;     // signed step of the update: S = SN + dtu RHS / V, +dt published, -dt consistent
   double sg;      // sign of sigma in the momentum flux: +1 published (rho u u + sigma), -1 consistent
+  int fast;       // 1 (FP64 and mixed stages): the powers below in cheaper forms within a few ulp
 };
 
 template <int NM>
@@ -71,6 +72,16 @@ struct GPow {
 };
 template <typename R>
 __device__ GPow<R> gpow(R Gmag) { return GPow<R>{pow(fabs(Gmag), R(1.0) / R(3.0)), pow(fabs(Gmag), R(1.0) / R(6.0))}; }
+// P.fast: |G|^(1/3) = cbrt |G| and |G|^(1/6) = sqrt(cbrt |G|), each within two
+// ulp of pow, at a fraction of its cost (the exact stage keeps pow)
+template <typename R>
+__device__ GPow<R> gpow(const M5P &P, R Gmag) {
+  if (P.fast) {
+    const R c = cbrt(fabs(Gmag));
+    return GPow<R>{c, sqrt(c)};
+  }
+  return gpow(Gmag);
+}
 
 // m5_elastic_energy_solid_V2, state_equation.c:269-316
 template <typename R>
@@ -269,7 +280,7 @@ __device__ void record(const M5P &P, const double *x, Rec5<NM> &r, double cavf0 
   G[8] = A[6] * A[6] + A[7] * A[7] + A[8] * A[8];
   const double Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
                       G[2] * (G[3] * G[7] - G[4] * G[6]);
-  const GPow<> gp = has_solid(P) ? gpow(Gmag) : GPow<>{0.0, 0.0};
+  const GPow<> gp = has_solid(P) ? gpow(P, Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < NM; ++i) {
     double are_elast = (P.mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, gp, P.mush[i]) : 0.0;
@@ -1077,7 +1088,7 @@ __device__ void relax_in(const M5P &P, const double *x, RelaxIn<NM> &R) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) R.A[k] = x[L::I5 + k];
   gmat(R.A, R.G, R.Gmag);
-  R.gp = has_solid(P) ? gpow(R.Gmag) : GPow<>{0.0, 0.0};
+  R.gp = has_solid(P) ? gpow(P, R.Gmag) : GPow<>{0.0, 0.0};
 }
 
 // the re-initialisation after PRELAX 1, 2, 3 (m5_relaxationvec.c:157-189 and
@@ -1094,7 +1105,7 @@ __device__ int relax_out(const M5P &P, const RelaxIn<NM> &R, double *are, double
     for (int k = 0; k < 9; ++k) Gx[k] = R.G[k];
     if (P.mix) {
       gmat(R.A, Gx, Gmx);
-      gpx = gpow(Gmx);
+      gpx = gpow(P, Gmx);
     }
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv) {
@@ -1161,7 +1172,10 @@ __device__ int r3_setup(const M5P &P, RelaxIn<NM> &R, double *pk, double &pr) {
 
 template <int NM>
 __device__ double r3_rhoK(const M5P &P, const RelaxIn<NM> &R, const double *pk, double pr, int iv) {
-  return R.ar[iv] / maxr(R.avf[iv], 1.0e-12) * pow((pr + P.pinf[iv]) / (pk[iv] + P.pinf[iv]), 1.0 / P.gamma[iv]);
+  const double ratio = (pr + P.pinf[iv]) / (pk[iv] + P.pinf[iv]);
+  // P.fast: x^y as exp(y ln x), a few ulp from pow for the pressure ratios here
+  const double pw = P.fast ? exp(log(ratio) * (1.0 / P.gamma[iv])) : pow(ratio, 1.0 / P.gamma[iv]);
+  return R.ar[iv] / maxr(R.avf[iv], 1.0e-12) * pw;
 }
 
 // at most cap iterations of the Newton loop (m5_relaxationvec.c:409-470)
@@ -1761,7 +1775,7 @@ __global__ __launch_bounds__(256) void k_m5_residual(int n, M5P P, const Geo *__
     for (int k = 0; k < 9; ++k) Am[k] = sc.A[k];
     double Gm[9], Gmag;
     gmat(sc.A, Gm, Gmag);
-    const GPow<> gps = has_solid(P) ? gpow(Gmag) : GPow<>{0.0, 0.0};
+    const GPow<> gps = has_solid(P) ? gpow(P, Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < NM; ++k)
       stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mix ? sc.avf[k] * P.mush[k] : P.mush[k]);
@@ -2003,7 +2017,7 @@ __global__ __launch_bounds__(256) void k_m5_residual_mx(int n, M5P P, const Geo 
     for (int k = 0; k < 9; ++k) Am[k] = sc.A[k];
     double Gm[9], Gmag;
     gmat(sc.A, Gm, Gmag);
-    const GPow<> gps = has_solid(P) ? gpow(Gmag) : GPow<>{0.0, 0.0};
+    const GPow<> gps = has_solid(P) ? gpow(P, Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < NM; ++k)
       stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mix ? sc.avf[k] * P.mush[k] : P.mush[k]);
@@ -2094,6 +2108,7 @@ int m5_params(const fv_m5_params *p, M5P &P) {
   P.limiter = p->limiter;
   P.dtu = P.fix ? -p->dt : p->dt;
   P.sg = P.fix ? -1.0 : 1.0;
+  P.fast = 0;
   return FV_OK;
 }
 
@@ -2807,6 +2822,7 @@ extern "C" int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_
   if (!pk || pk->n <= 0 || !pk->nbr || !pk->info) return fv_set_err(FV_ERR_ARG, "model 5: empty mesh");
   fv_m5_dev *c = new fv_m5_dev();
   c->P = P;
+  c->P.fast = c->prec != FV_PREC_EXACT;
   c->n = pk->n;
   c->nfine = pk->nfine;
   c->neq = 3 * P.nmat + 12;
@@ -2956,6 +2972,7 @@ extern "C" int fv_m5_dev_set_precision(fv_m5_dev *c, int prec) {
   if (prec != FV_PREC_FP64 && prec != FV_PREC_EXACT && prec != FV_PREC_MIXED)
     return fv_set_err(FV_ERR_ARG, "model 5: precision must be FV_PREC_FP64, FV_PREC_EXACT or FV_PREC_MIXED");
   c->prec = prec;
+  c->P.fast = prec != FV_PREC_EXACT;
   return FV_OK;
 }
 
