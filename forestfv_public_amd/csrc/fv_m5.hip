@@ -1471,12 +1471,25 @@ __device__ void store_state(double *X, int n, long q, const double *x) {
   for (int v = 0; v < L::NEQ; v++) X[(size_t)f * L::NEQ * n + (size_t)v * n + i] = x[v];
 }
 
-// the first R3_CAP iterations of state q held in x: true when the loop ended
-// and x is relaxed; false when x keeps its unrelaxed values (marked and saved
-// for the resume passes, or NaN, which raises the flag)
+// appends val to list at *cnt for the lanes with want, one atomic per wave
+// (the list's order is free: every listed state is resumed on its own)
+__device__ __forceinline__ void wave_append(int *cnt, int *list, int val, bool want) {
+  const unsigned long long m = __ballot(want);
+  if (m == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(cnt, __popcll(m));
+  base = __shfl(base, leader);
+  if (want) list[base + __popcll(m & ((1ull << lane) - 1ull))] = val;
+}
+
+// the first R3_CAP iterations of state q held in x: 1 when the loop ended and
+// x is relaxed; 0 when x keeps its unrelaxed values and the Newton state is
+// saved for the resume passes; -1 on NaN (x unrelaxed, the flag raised)
 template <int NM>
-__device__ bool r3_first(const M5P &P, double *x, long q, double *__restrict__ nst, int *__restrict__ iters,
-                         uint8_t *__restrict__ mark, int *__restrict__ flag) {
+__device__ int r3_first(const M5P &P, double *x, long q, double *__restrict__ nst, int *__restrict__ iters,
+                        int *__restrict__ flag) {
   RelaxIn<NM> R;
   relax_in<NM>(P, x, R);
   double pk[NM], pr, Fv = 1.0e-9, dF = 1.0e9;
@@ -1484,40 +1497,40 @@ __device__ bool r3_first(const M5P &P, double *x, long q, double *__restrict__ n
   const int irelax = r3_setup<NM>(P, R, pk, pr);
   int rc = 1;
   if (irelax == 1) rc = r3_newton<NM>(P, R, pk, pr, Fv, dF, iter, R3_CAP);
-  if (rc < 0) { atomicOr(flag, 1); mark[q] = 0; return false; }
+  if (rc < 0) { atomicOr(flag, 1); return -1; }
   if (rc == 0) {
     nst[3 * q + 0] = pr; nst[3 * q + 1] = Fv; nst[3 * q + 2] = dF;
     iters[q] = iter;
-    mark[q] = 1;
-    return false;
+    return 0;
   }
-  mark[q] = 0;
   if (r3_finish<NM>(P, R, pk, irelax, pr, x) < 0) atomicOr(flag, 1);
-  return true;
+  return 1;
 }
 
 template <int NM>
 __global__ __launch_bounds__(256) void k_m5_relax3_start(long count, int n, M5P P, double *__restrict__ X,
                                                          double *__restrict__ Sb, double *__restrict__ nst,
-                                                         int *__restrict__ iters, uint8_t *__restrict__ mark,
-                                                         int *__restrict__ flag) {
+                                                         int *__restrict__ iters, int *__restrict__ cnt,
+                                                         int *__restrict__ list, int *__restrict__ flag) {
   using L = L5<NM>;
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= count) return;
   double x[L::NEQ];
   load_state<NM>(X, n, q, x);
   if (Sb) store_state<NM>(Sb, n, q, x);
-  if (r3_first<NM>(P, x, q, nst, iters, mark, flag)) store_state<NM>(X, n, q, x);
+  const int st = r3_first<NM>(P, x, q, nst, iters, flag);
+  if (st == 1) store_state<NM>(X, n, q, x);
+  wave_append(cnt, list, (int)q, st == 0);
 }
 
 // resumes the listed states for at most cap more iterations; those still
-// iterating save their Newton state again and are marked (mark != nullptr)
+// iterating save their Newton state again and go to olist (when given)
 template <int NM>
 __global__ __launch_bounds__(256) void k_m5_relax3_resume(const int *__restrict__ nlist_p,
                                                           const int *__restrict__ list, int n, M5P P,
                                                           double *__restrict__ X, double *__restrict__ nst,
-                                                          int *__restrict__ iters, uint8_t *__restrict__ mark, int cap,
-                                                          int *__restrict__ flag) {
+                                                          int *__restrict__ iters, int *__restrict__ ocnt,
+                                                          int *__restrict__ olist, int cap, int *__restrict__ flag) {
   using L = L5<NM>;
   const int nlist = *nlist_p;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist; t += gridDim.x * blockDim.x) {
@@ -1532,17 +1545,21 @@ __global__ __launch_bounds__(256) void k_m5_relax3_resume(const int *__restrict_
     double Fv = nst[3 * q + 1], dF = nst[3 * q + 2];
     int iter = iters[q];
     const int rc = r3_newton<NM>(P, R, pk, pr, Fv, dF, iter, cap);
-    if (rc < 0) { atomicOr(flag, 1); if (mark) mark[q] = 0; continue; }
-    if (rc == 0) {
+    if (rc < 0) {
+      atomicOr(flag, 1);
+    } else if (rc == 0) {
       nst[3 * q + 0] = pr; nst[3 * q + 1] = Fv; nst[3 * q + 2] = dF;
       iters[q] = iter;
-      if (mark) mark[q] = 1;
-      continue;
+    } else {
+      if (r3_finish<NM>(P, R, pk, irelax, pr, x) < 0) atomicOr(flag, 1);
+      store_state<NM>(X, n, q, x);
     }
-    if (mark) mark[q] = 0;
-    if (r3_finish<NM>(P, R, pk, irelax, pr, x) < 0) atomicOr(flag, 1);
-    store_state<NM>(X, n, q, x);
+    if (olist) wave_append(ocnt, olist, (int)q, rc == 0);
   }
+}
+
+__global__ void k_zero_ints(int *p, int k) {
+  if ((int)threadIdx.x < k) p[threadIdx.x] = 0;
 }
 
 // avf_limit.c:3-21 with max_limit 100 on the fractions of X
@@ -1653,7 +1670,8 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
                                                   const uint32_t *__restrict__ info, const int *__restrict__ fine4,
                                                   const uint8_t *__restrict__ limf, double *__restrict__ SF,
                                                   double *__restrict__ nst, int *__restrict__ iters,
-                                                  uint8_t *__restrict__ mark, int *__restrict__ flag) {
+                                                  int *__restrict__ cnt, int *__restrict__ list,
+                                                  int *__restrict__ flag) {
   using L = L5<NM>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1715,7 +1733,8 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
       }
     }
     if constexpr (RX3) {
-      r3_first<NM>(P, Fv, (long)f * n + i, nst, iters, mark, flag);
+      const int st = r3_first<NM>(P, Fv, (long)f * n + i, nst, iters, flag);
+      wave_append(cnt, list, f * n + i, st == 0);
     } else {
       const int rc = relax5<NM>(P, Fv);
       if (rc) atomicOr(flag, rc == -3 ? 4 : 1);
@@ -2324,14 +2343,14 @@ struct fv_m5_dev {
   double *S = nullptr, *S2 = nullptr, *SN = nullptr, *SF = nullptr, *RHS = nullptr, *Sb = nullptr;
   uint8_t *gfm = nullptr, *mark = nullptr;
   int *flag = nullptr;        // [0] NaN (1) / Newton failure (4) bits, [1] GFM flag on a boundary face
-  int *list = nullptr, *count = nullptr;
+  int *list = nullptr, *count = nullptr;   // count [3]: compacted leaves, the two PRELAX 3 lists
   void *cub_tmp = nullptr;
   size_t cub_bytes = 0;
   // PRELAX 3 over the 6n face states (and the n cell states): saved Newton state of the slow ones
   double *nst = nullptr;      // [6n][3] pr, F, dF
   int *iters = nullptr;       // [6n]
-  uint8_t *mark6 = nullptr;   // [6n]
-  int *list6 = nullptr;       // [6n]
+  int *list6 = nullptr;       // [6n] states still iterating after the first pass (count[1] of them)
+  int *list6b = nullptr;      // [6n] those still iterating after the second (count[2])
   double *part = nullptr;     // [n/256 + 1] per-block minima of the CFL step, then the step
   uint8_t *lim = nullptr;     // [n] per-leaf slope flags of limiter 2 (0 minmod, 1 MC)
   int cap = 0, cap_fine = 0;  // leaves and 1:4 rows the per-leaf buffers hold
@@ -2346,7 +2365,7 @@ namespace {
 
 void m5_free(fv_m5_dev *c) {
   void *ptrs[] = {c->G, c->nbr, c->fine4, c->info, c->S, c->S2, c->SN, c->SF, c->RHS, c->Sb,
-                  c->gfm, c->mark, c->flag, c->list, c->count, c->cub_tmp, c->nst, c->iters, c->mark6, c->list6,
+                  c->gfm, c->mark, c->flag, c->list, c->count, c->cub_tmp, c->nst, c->iters, c->list6b, c->list6,
                   c->part, c->Snew, c->lim};
   for (void *q : ptrs)
     if (q) hipFree(q);
@@ -2369,21 +2388,19 @@ template <int NM>
 int m5_relax3(fv_m5_dev *c, double *X, long count, double *Sb, bool started = false) {
   const dim3 g((unsigned)((count + 255) / 256)), b(256);
   if (!started) {
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c->stream, c->count + 1, 2);
     hipLaunchKernelGGL(k_m5_relax3_start<NM>, g, b, 0, c->stream, count, c->n, c->P, X, Sb, c->nst, c->iters,
-                       c->mark6, c->flag);
+                       c->count + 1, c->list6, c->flag);
     HIPCHK(hipGetLastError());
   }
   // the states still iterating: 24 more iterations for all of them, then the
   // few that stagnate run to the reference's limit of 500 in full waves
   const dim3 gr((unsigned)std::min<long>(4096, (count + 255) / 256));
-  int rc = m5_compact(c, c->mark6, c->list6, (int)count);
-  if (rc != FV_OK) return rc;
-  hipLaunchKernelGGL(k_m5_relax3_resume<NM>, gr, b, 0, c->stream, c->count, c->list6, c->n, c->P, X, c->nst,
-                     c->iters, c->mark6, 24, c->flag);
+  hipLaunchKernelGGL(k_m5_relax3_resume<NM>, gr, b, 0, c->stream, c->count + 1, c->list6, c->n, c->P, X, c->nst,
+                     c->iters, c->count + 2, c->list6b, 24, c->flag);
   HIPCHK(hipGetLastError());
-  if ((rc = m5_compact(c, c->mark6, c->list6, (int)count)) != FV_OK) return rc;
-  hipLaunchKernelGGL(k_m5_relax3_resume<NM>, gr, b, 0, c->stream, c->count, c->list6, c->n, c->P, X, c->nst,
-                     c->iters, (uint8_t *)nullptr, 1 << 30, c->flag);
+  hipLaunchKernelGGL(k_m5_relax3_resume<NM>, gr, b, 0, c->stream, c->count + 2, c->list6b, c->n, c->P, X, c->nst,
+                     c->iters, (int *)nullptr, (int *)nullptr, 1 << 30, c->flag);
   HIPCHK(hipGetLastError());
   return FV_OK;
 }
@@ -2392,8 +2409,9 @@ dim3 grid_of(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
 template <int NM, typename R, bool RX3>
 void m5_recon_launch(fv_m5_dev *c) {
+  if (RX3) hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c->stream, c->count + 1, 2);
   hipLaunchKernelGGL((k_m5_recon<NM, R, RX3>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
-                     c->nbr, c->info, c->fine4, c->lim, c->SF, c->nst, c->iters, c->mark6, c->flag);
+                     c->nbr, c->info, c->fine4, c->lim, c->SF, c->nst, c->iters, c->count + 1, c->list6, c->flag);
 }
 
 template <int NM>
@@ -2726,13 +2744,12 @@ int m5_reserve(fv_m5_dev *c, int n_leaves, int nfine, int prelax3) {
   take(dalloc(&t.lim, n), t.lim);
   if (prelax3) {
     take(dalloc(&t.nst, 18 * n), t.nst); take(dalloc(&t.iters, 6 * n), t.iters);
-    take(dalloc(&t.mark6, 6 * n), t.mark6); take(dalloc(&t.list6, 6 * n), t.list6);
+    take(dalloc(&t.list6b, 6 * n), t.list6b); take(dalloc(&t.list6, 6 * n), t.list6);
   }
   size_t cub = 0;
   if (e == hipSuccess) {
     hipcub::CountingInputIterator<int> idx(0);
-    const int items = prelax3 ? 6 * n_leaves : n_leaves;
-    e = hipcub::DeviceSelect::Flagged(nullptr, cub, idx, t.mark, t.list, c->count, items, c->stream);
+    e = hipcub::DeviceSelect::Flagged(nullptr, cub, idx, t.mark, t.list, c->count, n_leaves, c->stream);
     if (e == hipSuccess) take(dalloc((char **)&t.cub_tmp, cub), t.cub_tmp);
   }
   if (e != hipSuccess) {
@@ -2740,12 +2757,12 @@ int m5_reserve(fv_m5_dev *c, int n_leaves, int nfine, int prelax3) {
     return fv_set_err(e == hipErrorOutOfMemory ? FV_ERR_NOMEM : FV_ERR_HIP, hipGetErrorString(e));
   }
   void *old[] = {c->nbr, c->fine4, c->info, c->S, c->S2, c->SN, c->Sb, c->RHS, c->SF, c->gfm, c->mark, c->list,
-                 c->part, c->lim, c->nst, c->iters, c->mark6, c->list6, c->cub_tmp};
+                 c->part, c->lim, c->nst, c->iters, c->list6b, c->list6, c->cub_tmp};
   for (void *q : old)
     if (q) hipFree(q);
   c->nbr = t.nbr; c->fine4 = t.fine4; c->info = t.info; c->S = t.S; c->S2 = t.S2; c->SN = t.SN; c->Sb = t.Sb;
   c->RHS = t.RHS; c->SF = t.SF; c->gfm = t.gfm; c->mark = t.mark; c->list = t.list; c->part = t.part; c->lim = t.lim;
-  c->nst = t.nst; c->iters = t.iters; c->mark6 = t.mark6; c->list6 = t.list6; c->cub_tmp = t.cub_tmp;
+  c->nst = t.nst; c->iters = t.iters; c->list6b = t.list6b; c->list6 = t.list6; c->cub_tmp = t.cub_tmp;
   c->cub_bytes = cub;
   t = fv_m5_dev();   // its pointers now belong to c
   c->cap = n_leaves;
@@ -2841,18 +2858,17 @@ extern "C" int fv_m5_dev_attach(const fv_m5_params *p, const fv_pack *pk, fv_m5_
   chk(dalloc(&c->RHS, sz));
   chk(dalloc(&c->SF, 6 * sz));
   chk(dalloc(&c->gfm, 6 * n)); chk(dalloc(&c->mark, n));
-  chk(dalloc(&c->flag, 2)); chk(dalloc(&c->list, n)); chk(dalloc(&c->count, 1));
+  chk(dalloc(&c->flag, 2)); chk(dalloc(&c->list, n)); chk(dalloc(&c->count, 3));
   chk(dalloc(&c->part, (n + 255) / 256 + 1));
   chk(dalloc(&c->lim, n));
   if (p->prelax == 3) {
     chk(dalloc(&c->nst, 18 * n)); chk(dalloc(&c->iters, 6 * n));
-    chk(dalloc(&c->mark6, 6 * n)); chk(dalloc(&c->list6, 6 * n));
+    chk(dalloc(&c->list6b, 6 * n)); chk(dalloc(&c->list6, 6 * n));
   }
   if (e == hipSuccess) {
-    // temporary storage of the largest compaction (6n face states under PRELAX 3, else n leaves)
+    // temporary storage of the compaction of n leaves
     hipcub::CountingInputIterator<int> idx(0);
-    const int items = (p->prelax == 3) ? 6 * c->n : c->n;
-    chk(hipcub::DeviceSelect::Flagged(nullptr, c->cub_bytes, idx, c->mark, c->list, c->count, items, c->stream));
+    chk(hipcub::DeviceSelect::Flagged(nullptr, c->cub_bytes, idx, c->mark, c->list, c->count, c->n, c->stream));
     chk(dalloc((char **)&c->cub_tmp, c->cub_bytes));
   }
   if (e == hipSuccess) {
