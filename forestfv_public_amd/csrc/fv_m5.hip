@@ -83,8 +83,36 @@ __device__ GPow<R> gpow(const M5P &P, R Gmag) {
   return gpow(Gmag);
 }
 
+// FAST (the FP64 and mixed residual): a / b as a times a reciprocal from
+// v_rcp and two Newton steps (FP64, within an ulp or two of the quotient) or
+// v_rcp_f32, with one reciprocal shared by the quotients of one denominator;
+// the denominators it is used on are nonzero in every admissible state
+__device__ __forceinline__ double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+template <bool FAST, typename R>
+__device__ __forceinline__ R dv(R a, R b) {
+  if constexpr (FAST) return a * frcp(b);
+  else return a / b;
+}
+// the powers of FAST: cbrt forms whatever P.fast says
+template <bool FAST, typename R>
+__device__ __forceinline__ GPow<R> gpow_t(const M5P &P, R Gmag) {
+  if constexpr (FAST) {
+    const R c = cbrt(fabs(Gmag));
+    return GPow<R>{c, sqrt(c)};
+  } else {
+    return gpow(P, Gmag);
+  }
+}
+
 // m5_elastic_energy_solid_V2, state_equation.c:269-316
-template <typename R>
+template <typename R, bool FAST = false>
 __device__ R elastic_energy_v2(const R *G, R Gmag, const GPow<R> &gp, R mu) {
   R eints;
   if (fabs(Gmag) != R(0.0)) {
@@ -101,7 +129,7 @@ __device__ R elastic_energy_v2(const R *G, R Gmag, const GPow<R> &gp, R mu) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) M2[i * 3 + j] += M1[i * 3 + k] * M1[k * 3 + j];
     const R tr1 = M2[0] + M2[4] + M2[8];
-    eints = mu / (R(4.0) * gp.g16) * tr1;
+    eints = dv<FAST>(mu, R(4.0) * gp.g16) * tr1;
   } else {
     eints = R(0.0);
   }
@@ -109,16 +137,16 @@ __device__ R elastic_energy_v2(const R *G, R Gmag, const GPow<R> &gp, R mu) {
 }
 
 // G_det (state_equation.c:206-215) and m5_elastic_energy_solid (:217-267)
-template <typename R>
-__device__ R elastic_energy_v1(const R *G, R mu) {
+template <typename R, bool FAST = false>
+__device__ R elastic_energy_v1(const M5P &P, const R *G, R mu) {
   const R Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
                       G[2] * (G[3] * G[7] - G[4] * G[6]);
-  if ((fabs(Gmag) != R(0.0)) && (mu != R(0.0))) return elastic_energy_v2(G, Gmag, gpow(Gmag), mu);
+  if ((fabs(Gmag) != R(0.0)) && (mu != R(0.0))) return elastic_energy_v2<R, FAST>(G, Gmag, gpow_t<FAST>(P, Gmag), mu);
   return R(0.0);
 }
 
 // m5_eos_stress_V2, state_equation.c:143-204: the stress s of one material
-template <typename R>
+template <typename R, bool FAST = false>
 __device__ void stress_v2(R *s, R avf, R pr, const R *G, R Gmag, const GPow<R> &gp, R mu) {
   if ((fabs(Gmag) != R(0.0)) && (mu != R(0.0))) {
     R G2[9], ss[9];
@@ -133,16 +161,17 @@ __device__ void stress_v2(R *s, R avf, R pr, const R *G, R Gmag, const GPow<R> &
     const R J1 = G[0] + G[4] + G[8];
     const R J2 = G2[0] + G2[4] + G2[8];
     const R g13 = gp.g13;
-    ss[0] = G2[0] - J2 / R(3.0) - g13 * (G[0] - J1 / R(3.0));
+    const R J2t = FAST ? J2 * R(1.0 / 3.0) : J2 / R(3.0), J1t = FAST ? J1 * R(1.0 / 3.0) : J1 / R(3.0);
+    ss[0] = G2[0] - J2t - g13 * (G[0] - J1t);
     ss[1] = G2[1] - g13 * (G[1]);
     ss[2] = G2[2] - g13 * (G[2]);
     ss[3] = G2[3] - g13 * (G[3]);
-    ss[4] = G2[4] - J2 / R(3.0) - g13 * (G[4] - J1 / R(3.0));
+    ss[4] = G2[4] - J2t - g13 * (G[4] - J1t);
     ss[5] = G2[5] - g13 * (G[5]);
     ss[6] = G2[6] - g13 * (G[6]);
     ss[7] = G2[7] - g13 * (G[7]);
-    ss[8] = G2[8] - J2 / R(3.0) - g13 * (G[8] - J1 / R(3.0));
-    const R k = mu / gp.g16;
+    ss[8] = G2[8] - J2t - g13 * (G[8] - J1t);
+    const R k = dv<FAST>(mu, gp.g16);
 #pragma unroll
     for (int q = 0; q < 9; q++) s[q] = (q == 0 || q == 4 || q == 8) ? (-avf * pr - k * (ss[q])) : (-k * (ss[q]));
   } else {
@@ -153,11 +182,11 @@ __device__ void stress_v2(R *s, R avf, R pr, const R *G, R Gmag, const GPow<R> &
 }
 
 // the sum of the materials' stresses (m5_facercnstr.c:150-173)
-template <typename R>
+template <typename R, bool FAST = false>
 __device__ void add_stress_v2(R *st, R avf, R pr, const R *G, R Gmag, const GPow<R> &gp,
                               R mu) {
   R s[9];
-  stress_v2(s, avf, pr, G, Gmag, gp, mu);
+  stress_v2<R, FAST>(s, avf, pr, G, Gmag, gp, mu);
 #pragma unroll
   for (int q = 0; q < 9; q++) st[q] += s[q];
 }
@@ -223,7 +252,7 @@ __device__ void narrow(const Rec5<NM, double> &a, Rec5<NM, R> &b) {
 // comes from the cell's raw solid fraction cavf0, and whose pressure and c2Y
 // divide by the re-summed fraction r.avf[i] (the two forms differ only for the
 // last material after a renormalisation).
-template <int NM, bool FACE = false>
+template <int NM, bool FACE = false, bool FAST = false>
 __device__ void record(const M5P &P, const double *x, Rec5<NM> &r, double cavf0 = 0.0) {
   using L = L5<NM>;
   const double a = P.amin / 100.0;
@@ -231,11 +260,12 @@ __device__ void record(const M5P &P, const double *x, Rec5<NM> &r, double cavf0 
   r.r = 0.0;
 #pragma unroll
   for (int i = 0; i < NM; ++i) { r.r += x[i]; r.ra[i] = x[i]; }
-  r.u = x[L::I1 + 0] / r.r;
-  r.v = x[L::I1 + 1] / r.r;
-  r.w = x[L::I1 + 2] / r.r;
+  const double ir = FAST ? frcp(r.r) : 0.0;
+  r.u = FAST ? x[L::I1 + 0] * ir : x[L::I1 + 0] / r.r;
+  r.v = FAST ? x[L::I1 + 1] * ir : x[L::I1 + 1] / r.r;
+  r.w = FAST ? x[L::I1 + 2] * ir : x[L::I1 + 2] / r.r;
   r.vm[0] = x[L::I1 + 0]; r.vm[1] = x[L::I1 + 1]; r.vm[2] = x[L::I1 + 2];
-  r.e = x[L::I2] / r.r;
+  r.e = FAST ? x[L::I2] * ir : x[L::I2] / r.r;
   r.vE = x[L::I2];
   double avfsum = 0.0;
 #pragma unroll
@@ -280,20 +310,20 @@ __device__ void record(const M5P &P, const double *x, Rec5<NM> &r, double cavf0 
   G[8] = A[6] * A[6] + A[7] * A[7] + A[8] * A[8];
   const double Gmag = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) +
                       G[2] * (G[3] * G[7] - G[4] * G[6]);
-  const GPow<> gp = has_solid(P) ? gpow(P, Gmag) : GPow<>{0.0, 0.0};
+  const GPow<> gp = has_solid(P) ? gpow_t<FAST>(P, Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < NM; ++i) {
-    double are_elast = (P.mush[i] != 0.0) ? elastic_energy_v2(G, Gmag, gp, P.mush[i]) : 0.0;
+    double are_elast = (P.mush[i] != 0.0) ? elastic_energy_v2<double, FAST>(G, Gmag, gp, P.mush[i]) : 0.0;
     const double g = P.gamma[i], pi = P.pinf[i];
     const double den = FACE ? r.avf[i] : avf_temp[i];
     // mixture mode: phase i holds alpha_i of the elastic energy and of the shear stress
     const double mu_i = P.mix ? r.avf[i] * P.mush[i] : P.mush[i];
     if (P.mix) are_elast *= den;
-    const double press = ((g - 1.0) * (r.are[i] - are_elast) / den) - g * pi;
-    r.c2Y[i] = (den / r.r) * g * (press + pi);
-    if (P.mix) r.c2Y[i] += (4.0 / 3.0) * mu_i / r.r;   // longitudinal speed of the mixture
+    const double press = dv<FAST>((g - 1.0) * (r.are[i] - are_elast), den) - g * pi;
+    r.c2Y[i] = (FAST ? den * ir : den / r.r) * g * (press + pi);
+    if (P.mix) r.c2Y[i] += FAST ? (4.0 / 3.0) * mu_i * ir : (4.0 / 3.0) * mu_i / r.r;   // longitudinal speed of the mixture
     r.ph[i] = press;
-    add_stress_v2(r.st, r.avf[i], r.ph[i], G, Gmag, gp, mu_i);
+    add_stress_v2<double, FAST>(r.st, r.avf[i], r.ph[i], G, Gmag, gp, mu_i);
   }
 }
 
@@ -330,7 +360,7 @@ struct Star5 {
 };
 
 // m5_star_region.c:3-157 for side s (flag 1: left, 2: right)
-template <int NM, typename R>
+template <int NM, typename R, bool FAST = false>
 __device__ void star_region(const M5P &P, const R *F, const Rec5<NM, R> &sL, const Rec5<NM, R> &sR,
                             const Rec5<NM, R> &s, int flag_ifs, R u_S, R SL, R SR, R uL,
                             R vL, R wL, R uR, R vR, R wR, R s11L, R s12L,
@@ -339,41 +369,51 @@ __device__ void star_region(const M5P &P, const R *F, const Rec5<NM, R> &sL, con
   R SK, uK, vK, wK, s11K, s12K, s13K;
   if (flag_ifs == 1) { SK = SL; uK = uL; vK = vL; wK = wL; s11K = s11L; s12K = s12L; s13K = s13L; }
   else { SK = SR; uK = uR; vK = vR; wK = wR; s11K = s11R; s12K = s12R; s13K = s13R; }
+  // FAST: the shared denominators u* - S_K, the stress jump's and (u_K - S_K) rho_K
+  const R iUS = FAST ? frcp(u_S - SK) : R(0.0);
+  const R Dst = (uR - SR) * sR.r - (uL - SL) * sL.r;
+  const R iDst = FAST ? frcp(Dst) : R(0.0);
+  const R iUK = FAST ? frcp((uK - SK) * s.r) : R(0.0);
   R ar_S[NM], r_S = R(0.0);
 #pragma unroll
-  for (int v = 0; v < NM; ++v) { ar_S[v] = s.ra[v] * (SK - uK) / (SK - u_S); r_S += ar_S[v]; }
-  const R s11_S = ((uR - SR) * sR.r * (s11L) - (uL - SL) * sL.r * (s11R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (uR - uL)) /
-                       ((uR - SR) * sR.r - (uL - SL) * sL.r);
+  for (int v = 0; v < NM; ++v) {
+    ar_S[v] = FAST ? s.ra[v] * ((uK - SK) * iUS) : s.ra[v] * (SK - uK) / (SK - u_S);
+    r_S += ar_S[v];
+  }
+  const R n11 = (uR - SR) * sR.r * (s11L) - (uL - SL) * sL.r * (s11R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (uR - uL);
+  const R s11_S = FAST ? n11 * iDst : n11 / Dst;
   R s12_S, s13_S;
   if ((sL.phi0 > R(0.0)) && (sR.phi0 > R(0.0))) {
-    s12_S = ((uR - SR) * sR.r * (s12L) - (uL - SL) * sL.r * (s12R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (vR - vL)) /
-            ((uR - SR) * sR.r - (uL - SL) * sL.r);
-    s13_S = ((uR - SR) * sR.r * (s13L) - (uL - SL) * sL.r * (s13R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (wR - wL)) /
-            ((uR - SR) * sR.r - (uL - SL) * sL.r);
+    const R n12 = (uR - SR) * sR.r * (s12L) - (uL - SL) * sL.r * (s12R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (vR - vL);
+    const R n13 = (uR - SR) * sR.r * (s13L) - (uL - SL) * sL.r * (s13R) + (uL - SL) * sL.r * (uR - SR) * sR.r * (wR - wL);
+    s12_S = FAST ? n12 * iDst : n12 / Dst;
+    s13_S = FAST ? n13 * iDst : n13 / Dst;
   } else {
     s12_S = R(0.0);
     s13_S = R(0.0);
   }
-  const R v_S = vK + (s12_S - s12K) / ((uK - SK) * s.r);
-  const R w_S = wK + (s13_S - s13K) / ((uK - SK) * s.r);
+  const R v_S = vK + (FAST ? (s12_S - s12K) * iUK : (s12_S - s12K) / ((uK - SK) * s.r));
+  const R w_S = wK + (FAST ? (s13_S - s13K) * iUK : (s13_S - s13K) / ((uK - SK) * s.r));
   R vel[3];
   vel[0] = u_S * n[0] + v_S * m[0] + w_S * l[0];   // rotate_vector_ltg_v2
   vel[1] = u_S * n[1] + v_S * m[1] + w_S * l[1];
   vel[2] = u_S * n[2] + v_S * m[2] + w_S * l[2];
   const R re = s.r * s.e;
   // consistent: - s11K uK over r* (u* - S_K); published (m5_star_region.c:73): + s11K uK over r* (u* + S_K)
-  const R Em_S = P.fix ? (re * (uK - SK) - s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
-                                  (r_S * (u_S - SK))
-                            : (re * (uK - SK) + s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S) /
-                                  (r_S * (u_S + SK));
+  const R Em_S = P.fix ? dv<FAST>(re * (uK - SK) - s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S,
+                                  r_S * (u_S - SK))
+                            : dv<FAST>(re * (uK - SK) + s11K * uK - s12K * vK - s13K * wK + s11_S * u_S + s12_S * v_S + s13_S * w_S,
+                                  r_S * (u_S + SK));
   R A[9], t[9], AS[9];
   rot_tensor(t, s.A, n, m, l);
 #pragma unroll
   for (int k = 0; k < 9; k++) A[k] = t[k];
 #pragma unroll
-  for (int j = 0; j < 3; ++j)
-    AS[j] = P.fix ? (A[j] * (uK - SK) + A[3 + j] * (vK - v_S) + A[6 + j] * (wK - w_S)) / (u_S - SK)
-                  : (A[j] * (uK - SK) - A[3 + j] * (vK - v_S) + A[6 + j] * (wK - w_S)) / (u_S - SK);
+  for (int j = 0; j < 3; ++j) {
+    const R na = P.fix ? (A[j] * (uK - SK) + A[3 + j] * (vK - v_S) + A[6 + j] * (wK - w_S))
+                       : (A[j] * (uK - SK) - A[3 + j] * (vK - v_S) + A[6 + j] * (wK - w_S));
+    AS[j] = FAST ? na * iUS : na / (u_S - SK);
+  }
 #pragma unroll
   for (int j = 0; j < 3; ++j) { AS[3 + j] = A[3 + j]; AS[6 + j] = A[6 + j]; }
   {
@@ -391,14 +431,14 @@ __device__ void star_region(const M5P &P, const R *F, const Rec5<NM, R> &sL, con
 #pragma unroll
   for (int v = 0; v < NM; ++v) {
     const R g = (R)P.gamma[v], pi = (R)P.pinf[v];
-    const R rm = (s.ra[v] / s.avf[v]);
-    const R rm_S = (s.ra[v] / s.avf[v]) * (uK - SK) / (u_S - SK);
+    const R rm = dv<FAST>(s.ra[v], s.avf[v]);
+    const R rm_S = FAST ? rm * ((uK - SK) * iUS) : (s.ra[v] / s.avf[v]) * (uK - SK) / (u_S - SK);
     const R p_k = P.fix ? s.ph[v] : -s.ph[v];   // published: -p_hydro
-    const R p_S = (p_k + pi) * (((g - R(1.0)) * rm + (g + R(1.0)) * rm_S) / ((g - R(1.0)) * rm_S + (g + R(1.0)) * rm)) - pi;
-    R are_el = elastic_energy_v1(G, (R)P.mush[v]);
+    const R p_S = (p_k + pi) * dv<FAST>((g - R(1.0)) * rm + (g + R(1.0)) * rm_S, (g - R(1.0)) * rm_S + (g + R(1.0)) * rm) - pi;
+    R are_el = elastic_energy_v1<R, FAST>(P, G, (R)P.mush[v]);
     if (P.mix) are_el *= s.avf[v];   // mixture mode: alpha_v of the elastic energy
-    const R e_el = are_el / (s.avf[v] * rm_S);
-    e_S[v] = (p_S + g * pi) / (rm_S * (P.fix ? (g - R(1.0)) : (g + R(1.0)))) + e_el;   // published: gamma + 1
+    const R e_el = dv<FAST>(are_el, s.avf[v] * rm_S);
+    e_S[v] = dv<FAST>(p_S + g * pi, rm_S * (P.fix ? (g - R(1.0)) : (g + R(1.0)))) + e_el;   // published: gamma + 1
   }
 #pragma unroll
   for (int v = 0; v < NM; ++v) W.Us[v] = ar_S[v];
@@ -418,7 +458,7 @@ __device__ void star_region(const M5P &P, const R *F, const Rec5<NM, R> &sL, con
 // m5_flux_gfm.c): out[iv] = F.n + F_hllc as m5_residual.c:166-170 forms it
 // before the area, and the non-conservative face velocity.  sL and sR are
 // rewritten with the ghost.  Returns false where the reference prints NaN and exits.
-template <int NM, typename R>
+template <int NM, typename R, bool FAST = false>
 __device__ bool gfm_subface(const M5P &P, const R *F, Rec5<NM, R> &sL, Rec5<NM, R> &sR, R *out, R *unc) {
   using L = L5<NM>;
   const R *n = F, *m = F + 3, *l = F + 6;
@@ -447,8 +487,8 @@ __device__ bool gfm_subface(const M5P &P, const R *F, Rec5<NM, R> &sL, Rec5<NM, 
   const R s11L = t[0], s12L = t[1], s13L = t[2];
   rot_tensor(t, sR.st, n, m, l);
   const R s11R = t[0], s12R = t[1], s13R = t[2];
-  const R u_S = ((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR)) /
-                     (ruL - ruR - ((SL)*sL.r) + ((SR)*sR.r));
+  const R u_S = dv<FAST>((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR),
+                         ruL - ruR - ((SL)*sL.r) + ((SR)*sR.r));
   if (isnan(u_S) || isnan(SR) || isnan(SL)) return false;
   int flag_ifs;
   if (R(0.0) <= SL) flag_ifs = 0;
@@ -463,7 +503,7 @@ __device__ bool gfm_subface(const M5P &P, const R *F, Rec5<NM, R> &sL, Rec5<NM, 
   R SK = R(0.0);
   if (flag_ifs == 1 || flag_ifs == 2) {
     SK = (flag_ifs == 1) ? SL : SR;
-    star_region<NM, R>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
+    star_region<NM, R, FAST>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
 #pragma unroll
     for (int iv = 0; iv < L::NCONS; ++iv)
       if (isnan(W.Us[iv])) return false;
@@ -867,7 +907,7 @@ __device__ void phys_rows(const Rec5<NM, R> &s, const R *n, R sg, R *adv) {
 // non-conservative face velocity.  bc 4 forces u* = 0; bc 1 (symmetry)
 // replaces the flux by the mean of the physical fluxes of the cell record of
 // xc and its mirror.  Returns false where the reference prints NaN and exits.
-template <int NM, typename R>
+template <int NM, typename R, bool FAST = false>
 __device__ bool hllc_subface(const M5P &P, const R *F, int bc, const Rec5<NM, R> &sL, const Rec5<NM, R> &sR,
                              const double *__restrict__ S, int n, int cell, R *out, R *unc) {
   using L = L5<NM>;
@@ -890,8 +930,8 @@ __device__ bool hllc_subface(const M5P &P, const R *F, int bc, const Rec5<NM, R>
   const R s11L = t[0], s12L = t[1], s13L = t[2];
   rot_tensor(t, sR.st, nv, m, l);
   const R s11R = t[0], s12R = t[1], s13R = t[2];
-  R u_S = ((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR)) /
-               (ruL - ruR - ((SL)*sL.r) + ((SR)*sR.r));
+  R u_S = dv<FAST>((sL.r * sq((uL)) - (s11L)) - (sR.r * sq((uR)) - (s11R)) - ((SL)*ruL) + ((SR)*ruR),
+                   ruL - ruR - ((SL)*sL.r) + ((SR)*sR.r));
   if (isnan(u_S) || isnan(SR) || isnan(SL)) return false;
   if (bc == 4) u_S = R(0.0);
   int flag_ifs;
@@ -908,7 +948,7 @@ __device__ bool hllc_subface(const M5P &P, const R *F, int bc, const Rec5<NM, R>
   const bool star = (flag_ifs == 1 || flag_ifs == 2);
   if (star) {
     SK = (flag_ifs == 1) ? SL : SR;
-    star_region<NM, R>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
+    star_region<NM, R, FAST>(P, F, sL, sR, s, flag_ifs, u_S, SL, SR, uL, vL, wL, uR, vR, wR, s11L, s12L, s13L, s11R, s12R, s13R, W);
 #pragma unroll
     for (int iv = 0; iv < L::NCONS; ++iv)
       if (isnan(W.Us[iv])) return false;
@@ -932,7 +972,7 @@ __device__ bool hllc_subface(const M5P &P, const R *F, int bc, const Rec5<NM, R>
     Rec5<NM, R> cl, cr;
     {
       Rec5<NM> c64;
-      record<NM>(P, x, c64);
+      record<NM, false, FAST>(P, x, c64);
       narrow<NM, R>(c64, cl);
     }
     cr = cl;
@@ -1172,10 +1212,15 @@ __device__ int r3_setup(const M5P &P, RelaxIn<NM> &R, double *pk, double &pr) {
 
 template <int NM>
 __device__ double r3_rhoK(const M5P &P, const RelaxIn<NM> &R, const double *pk, double pr, int iv) {
-  const double ratio = (pr + P.pinf[iv]) / (pk[iv] + P.pinf[iv]);
-  // P.fast: x^y as exp(y ln x), a few ulp from pow for the pressure ratios here
-  const double pw = P.fast ? exp(log(ratio) * (1.0 / P.gamma[iv])) : pow(ratio, 1.0 / P.gamma[iv]);
-  return R.ar[iv] / maxr(R.avf[iv], 1.0e-12) * pw;
+  return R.ar[iv] / maxr(R.avf[iv], 1.0e-12) * pow((pr + P.pinf[iv]) / (pk[iv] + P.pinf[iv]), 1.0 / P.gamma[iv]);
+}
+
+// P.fast: ar_k / rho_k = max(alpha_k, 1e-12) x^(-1/gamma_k) with x the ratio
+// (pr + pinf) / (pk + pinf) formed with the reciprocal ipk of its denominator,
+// and x^y as exp(y ln x): a few ulp from the quotient of r3_rhoK
+template <int NM>
+__device__ double r3_q_fast(const M5P &P, const RelaxIn<NM> &R, double ipk, double pr, int iv) {
+  return maxr(R.avf[iv], 1.0e-12) * exp(log((pr + P.pinf[iv]) * ipk) * (-1.0 / P.gamma[iv]));
 }
 
 // at most cap iterations of the Newton loop (m5_relaxationvec.c:409-470)
@@ -1185,6 +1230,9 @@ template <int NM>
 __device__ int r3_newton(const M5P &P, const RelaxIn<NM> &R, const double *pk, double &pr, double &Fv, double &dF,
                          int &iter, int cap) {
   const double eps_sml = 1.0e-12, eps_big = 1.0e-8;
+  double ipk[NM];
+#pragma unroll
+  for (int iv = 0; iv < NM; ++iv) ipk[iv] = P.fast ? frcp(pk[iv] + P.pinf[iv]) : 0.0;
   for (int k = 0; k < cap; k++) {
     if (!((fabs(Fv) > 1.0e-12) && (iter < 500))) return 1;
     const double pr0 = pr, F0 = Fv, dF0 = dF;
@@ -1198,9 +1246,15 @@ __device__ int r3_newton(const M5P &P, const RelaxIn<NM> &R, const double *pk, d
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv) {
       if (R.avf[iv] > eps_sml) {
-        const double rk = r3_rhoK<NM>(P, R, pk, pr, iv);
-        Fv = Fv + R.ar[iv] / rk;
-        dF = dF - R.ar[iv] / (P.gamma[iv] * rk * (pr + P.pinf[iv]));
+        if (P.fast) {
+          const double q = r3_q_fast<NM>(P, R, ipk[iv], pr, iv);
+          Fv = Fv + q;
+          dF = dF - q * frcp(P.gamma[iv] * (pr + P.pinf[iv]));
+        } else {
+          const double rk = r3_rhoK<NM>(P, R, pk, pr, iv);
+          Fv = Fv + R.ar[iv] / rk;
+          dF = dF - R.ar[iv] / (P.gamma[iv] * rk * (pr + P.pinf[iv]));
+        }
       }
     }
     if (isnan(pr) || isnan(Fv) || isnan(dF)) return -1;
@@ -1217,7 +1271,9 @@ __device__ int r3_finish(const M5P &P, RelaxIn<NM> &R, const double *pk, int ire
   if (irelax == 1) {
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv)
-      if (R.avf[iv] > 1.0e-12) R.avf[iv] = R.ar[iv] / r3_rhoK<NM>(P, R, pk, pr, iv);
+      if (R.avf[iv] > 1.0e-12)
+        R.avf[iv] = P.fast ? r3_q_fast<NM>(P, R, frcp(pk[iv] + P.pinf[iv]), pr, iv)
+                           : R.ar[iv] / r3_rhoK<NM>(P, R, pk, pr, iv);
     vf_limit1<NM>(P.amin / 100.0, R.avf);
   }
   const int i_as = ((R.avf[0] < R.avf[1]) && (P.mush[1] != 0.0)) ? 1 : 0;
@@ -1746,7 +1802,7 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
 
 // the record a face flux reads: the face state (ORDER 2, m5_facercnstr) or
 // the cell state (ORDER 1, m5_facevalues) of leaf j across its face fj
-template <int NM, typename R>
+template <int NM, typename R, bool FAST = false>
 __device__ void side_record(const M5P &P, const double *__restrict__ S, const double *__restrict__ SF, int n, int j,
                             int fj, Rec5<NM, R> &r) {
   using L = L5<NM>;
@@ -1755,11 +1811,11 @@ __device__ void side_record(const M5P &P, const double *__restrict__ S, const do
   if (P.order == 2) {
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) x[v] = SF[((size_t)fj * L::NEQ + v) * n + j];
-    record<NM, true>(P, x, r64, S[(size_t)L::I3 * n + j]);
+    record<NM, true, FAST>(P, x, r64, S[(size_t)L::I3 * n + j]);
   } else {
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + j];
-    record<NM>(P, x, r64);
+    record<NM, false, FAST>(P, x, r64);
   }
   narrow<NM, R>(r64, r);
 }
@@ -1967,14 +2023,14 @@ __global__ __launch_bounds__(256) void k_m5_residual_mx(int n, M5P P, const Geo 
     double uf0 = 0.0, uf1 = 0.0, uf2 = 0.0, Area_f = 0.0;
     if (kind == FV_FK_MIRROR) {
       Rec5<NM, R> sL, sR;
-      side_record<NM, R>(P, S, SF, n, nb, fo, sL);
-      side_record<NM, R>(P, S, SF, n, i, f, sR);
+      side_record<NM, R, true>(P, S, SF, n, nb, fo, sL);
+      side_record<NM, R, true>(P, S, SF, n, i, f, sR);
       R out[L::NEQ], un[3], fr[9];
 #pragma unroll
       for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[fo][q];
       const bool g = sL.phi0 * sR.phi0 < R(0.0);
-      ok &= g ? gfm_subface<NM, R>(P, fr, sL, sR, out, un)
-              : hllc_subface<NM, R>(P, fr, 0, sL, sR, S, n, nb, out, un);
+      ok &= g ? gfm_subface<NM, R, true>(P, fr, sL, sR, out, un)
+              : hllc_subface<NM, R, true>(P, fr, 0, sL, sR, S, n, nb, out, un);
       const double A = G.area[lvl];
 #pragma unroll
       for (int v = 0; v < L::NEQ; v++) rhs[v] -= A * (double)out[v];
@@ -1989,25 +2045,25 @@ __global__ __launch_bounds__(256) void k_m5_residual_mx(int n, M5P P, const Geo 
 #pragma unroll
       for (int q = 0; q < 9; q++) fr[q] = (R)G.frame[f][q];
       Rec5<NM, R> sL;
-      side_record<NM, R>(P, S, SF, n, i, f, sL);
+      side_record<NM, R, true>(P, S, SF, n, i, f, sL);
       for (int k = 0; k < Nneig; ++k) {
         Rec5<NM, R> sR;
         int bcode = 0;
         if (kind == FV_FK_BOUNDARY) {
           bcode = -nb;
-          side_record<NM, R>(P, S, SF, n, i, f, sR);
+          side_record<NM, R, true>(P, S, SF, n, i, f, sR);
           apply_bc5<NM, R>(bcode, fr, sL, sR);
         } else {
           const int j = (kind == FV_FK_FINE) ? fine4[4 * nb + k] : nb;
-          side_record<NM, R>(P, S, SF, n, j, fo, sR);
+          side_record<NM, R, true>(P, S, SF, n, j, fo, sR);
         }
         R out[L::NEQ], un[3];
         if (sL.phi0 * sR.phi0 < R(0.0)) {
           gbits |= (uint8_t)(1u << f);
           Rec5<NM, R> gL = sL;
-          ok &= gfm_subface<NM, R>(P, fr, gL, sR, out, un);
+          ok &= gfm_subface<NM, R, true>(P, fr, gL, sR, out, un);
         } else {
-          ok &= hllc_subface<NM, R>(P, fr, bcode, sL, sR, S, n, i, out, un);
+          ok &= hllc_subface<NM, R, true>(P, fr, bcode, sL, sR, S, n, i, out, un);
         }
 #pragma unroll
         for (int v = 0; v < L::NEQ; v++) rhs[v] += A * (double)out[v];
@@ -2029,17 +2085,17 @@ __global__ __launch_bounds__(256) void k_m5_residual_mx(int n, M5P P, const Geo 
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) x[v] = S[(size_t)v * n + i];
     Rec5<NM> sc;
-    record<NM>(P, x, sc);
+    record<NM, false, true>(P, x, sc);
 #pragma unroll
     for (int k = 0; k < NM - 1; ++k) src_avf[k] = sc.avf[k];
 #pragma unroll
     for (int k = 0; k < 9; ++k) Am[k] = sc.A[k];
     double Gm[9], Gmag;
     gmat(sc.A, Gm, Gmag);
-    const GPow<> gps = has_solid(P) ? gpow(P, Gmag) : GPow<>{0.0, 0.0};
+    const GPow<> gps = has_solid(P) ? gpow_t<true>(P, Gmag) : GPow<>{0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < NM; ++k)
-      stress_v2(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mix ? sc.avf[k] * P.mush[k] : P.mush[k]);
+      stress_v2<double, true>(src_st[k], sc.avf[k], sc.ph[k], Gm, Gmag, gps, P.mix ? sc.avf[k] * P.mush[k] : P.mush[k]);
   }
   const double tr = T[0] + T[4] + T[8];   // sum_f Area_f (u*.n)
   const double sgn = P.fix ? -1.0 : 1.0;
