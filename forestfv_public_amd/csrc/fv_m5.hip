@@ -1118,7 +1118,12 @@ __device__ void relax_in(const M5P &P, const double *x, RelaxIn<NM> &R) {
   R.rho = 0.0;
 #pragma unroll
   for (int iv = 0; iv < NM; ++iv) { R.ar[iv] = x[L::I0 + iv]; R.are[iv] = x[L::I4 + iv]; R.rho += R.ar[iv]; }
-  R.u = x[L::I1 + 0] / R.rho; R.v = x[L::I1 + 1] / R.rho; R.w = x[L::I1 + 2] / R.rho;
+  if (P.fast) {
+    const double ir = frcp(R.rho);
+    R.u = x[L::I1 + 0] * ir; R.v = x[L::I1 + 1] * ir; R.w = x[L::I1 + 2] * ir;
+  } else {
+    R.u = x[L::I1 + 0] / R.rho; R.v = x[L::I1 + 1] / R.rho; R.w = x[L::I1 + 2] / R.rho;
+  }
   R.rE = x[L::I2];
 #pragma unroll
   for (int iv = 0; iv < NM - 1; ++iv) R.avf[iv] = x[L::I3 + iv];
@@ -1147,26 +1152,29 @@ __device__ int relax_out(const M5P &P, const RelaxIn<NM> &R, double *are, double
       gmat(R.A, Gx, Gmx);
       gpx = gpow(P, Gmx);
     }
+    const double irho = P.fast ? frcp(R.rho) : 0.0;
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv) {
       are_el[iv] = (P.mush[iv] != 0.0) ? elastic_energy_v2(Gx, Gmx, gpx, P.mush[iv]) : 0.0;
       if (P.mix) are_el[iv] *= R.avf[iv];
-      const double e_elastic_K = are_el[iv] / (R.ar[iv]);
-      const double Ys = R.ar[iv] / R.rho;
-      e_elast += Ys * e_elastic_K;
+      // Ys e_K = (ar / rho) (are_el / ar) = are_el / rho (P.fast)
+      e_elast += P.fast ? are_el[iv] * irho : (R.ar[iv] / R.rho) * (are_el[iv] / (R.ar[iv]));
     }
-    const double e_hyd = R.rE / R.rho - 0.5 * (sq((R.u)) + sq((R.v)) + sq((R.w))) - e_elast;
-    double num = 0.0, den = 0.0;
+    const double e_hyd = (P.fast ? R.rE * irho : R.rE / R.rho) - 0.5 * (sq((R.u)) + sq((R.v)) + sq((R.w))) - e_elast;
+    double num = 0.0, den = 0.0, igm[NM];
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv) {
-      num += R.avf[iv] * P.gamma[iv] * P.pinf[iv] / (P.gamma[iv] - 1.0);
-      den += R.avf[iv] / (P.gamma[iv] - 1.0);
+      igm[iv] = P.fast ? frcp(P.gamma[iv] - 1.0) : 0.0;
+      num += P.fast ? R.avf[iv] * P.gamma[iv] * P.pinf[iv] * igm[iv] : R.avf[iv] * P.gamma[iv] * P.pinf[iv] / (P.gamma[iv] - 1.0);
+      den += P.fast ? R.avf[iv] * igm[iv] : R.avf[iv] / (P.gamma[iv] - 1.0);
     }
-    const double p_relaxed = (R.rho * e_hyd - num) / den;
+    const double p_relaxed = P.fast ? (R.rho * e_hyd - num) * frcp(den) : (R.rho * e_hyd - num) / den;
     if (isnan(p_relaxed)) return -1;
 #pragma unroll
     for (int iv = 0; iv < NM; ++iv)
-      are[iv] = (R.avf[iv] * (p_relaxed + P.gamma[iv] * P.pinf[iv]) / (P.gamma[iv] - 1.0)) + are_el[iv];
+      are[iv] = (P.fast ? R.avf[iv] * (p_relaxed + P.gamma[iv] * P.pinf[iv]) * igm[iv]
+                        : R.avf[iv] * (p_relaxed + P.gamma[iv] * P.pinf[iv]) / (P.gamma[iv] - 1.0)) +
+                are_el[iv];
   }
   bool bad = false;
 #pragma unroll
@@ -1199,7 +1207,9 @@ __device__ int r3_setup(const M5P &P, RelaxIn<NM> &R, double *pk, double &pr) {
       if (R.avf[iv] > eps_sml) {
         const double ael = ((P.mush[iv] != 0.0) ? elastic_energy_v2(R.G, R.Gmag, R.gp, P.mush[iv]) : 0.0) *
                            (P.mix ? R.avf[iv] : 1.0);
-        pk[iv] = ((P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) - P.gamma[iv] * P.pinf[iv];
+        pk[iv] = (P.fast ? (P.gamma[iv] - 1.0) * (R.are[iv] - ael) * frcp(R.avf[iv])
+                         : (P.gamma[iv] - 1.0) * (R.are[iv] - ael) / R.avf[iv]) -
+                 P.gamma[iv] * P.pinf[iv];
         if (pk[iv] <= -(1.0 - eps_big) * P.pinf[iv] + eps_big) pk[iv] = -(1.0 - eps_big) * P.pinf[iv] + eps_big;
       } else {
         pk[iv] = 0.0;
@@ -1701,6 +1711,22 @@ __device__ void dist5(const Geo &G, int kind, int lvl, double &dcf, double &dcc)
 // the limited slope of one variable: minmod as m5_reconctruct.c:110-140 has it,
 // or (new) the monotonised central slope minmod(2a, 2b, (a + b) / 2) with a the
 // backward and b the forward difference quotient
+// FAST (the FP64 and mixed stages): the two difference quotients with the
+// reciprocals of the face's distances, formed once per face
+template <typename R>
+__device__ __forceinline__ R m5_slope_q(bool mc, R a, R b) {
+  if (!mc) {
+    if (a * b > R(0.0)) return (fabs(a) > fabs(b)) ? b : a;
+    return R(0.0);
+  }
+  if (a * b > R(0.0)) {
+    const R c = R(0.5) * (a + b);
+    const R mag = minr(minr(R(2.0) * fabs(a), R(2.0) * fabs(b)), fabs(c));
+    return (c > R(0.0)) ? mag : -mag;
+  }
+  return R(0.0);
+}
+
 template <typename R>
 __device__ __forceinline__ R m5_slope(bool mc, R df, R db, R dp, R dm) {
   if (!mc) {
@@ -1720,7 +1746,7 @@ __device__ __forceinline__ R m5_slope(bool mc, R df, R db, R dp, R dm) {
   return R(0.0);
 }
 
-template <int NM, typename R, bool RX3>
+template <int NM, typename R, bool RX3, bool FAST>
 __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__restrict__ Gp,
                                                   const double *__restrict__ S, const int *__restrict__ nbr,
                                                   const uint32_t *__restrict__ info, const int *__restrict__ fine4,
@@ -1739,24 +1765,34 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
   avf_limit5<NM>(a, Sc);
+  // per axis, the two stencil values across its faces are gathered once and
+  // serve both faces (each face's forward value is the other's backward one)
 #pragma unroll 1
-  for (int f = 0; f < 6; f++) {
-    const int fo = FCOPP5[f];
-    const int kf = fvk::face_kind(inf, f), kb = fvk::face_kind(inf, fo);
-    double Xf[L::NEQ], Xb[L::NEQ];
-    stencil5<NM>(P, S, n, i, kf, nbr[(size_t)f * n + i], fine4, Xf);
-    stencil5<NM>(P, S, n, i, kb, nbr[(size_t)fo * n + i], fine4, Xb);
-    double dcf, dp, dcf_b, dm;
-    dist5(*Gp, kf, lvl, dcf, dp);
-    dist5(*Gp, kb, lvl, dcf_b, dm);
+  for (int ax = 0; ax < 3; ax++) {
+  const int fpl = (ax == 0) ? 1 : (ax == 1 ? 2 : 5), fmi = FCOPP5[fpl];   // the + and - faces
+  const int kpl = fvk::face_kind(inf, fpl), kmi = fvk::face_kind(inf, fmi);
+  double Xp[L::NEQ], Xm[L::NEQ];
+  stencil5<NM>(P, S, n, i, kpl, nbr[(size_t)fpl * n + i], fine4, Xp);
+  stencil5<NM>(P, S, n, i, kmi, nbr[(size_t)fmi * n + i], fine4, Xm);
+  double dcf_p, dcc_p, dcf_m, dcc_m;
+  dist5(*Gp, kpl, lvl, dcf_p, dcc_p);
+  dist5(*Gp, kmi, lvl, dcf_m, dcc_m);
+#pragma unroll 1
+  for (int side = 0; side < 2; side++) {
+    const int f = side ? fmi : fpl;
+    const int kf = side ? kmi : kpl;
+    const double dcf = side ? dcf_m : dcf_p, dp = side ? dcc_m : dcc_p, dm = side ? dcc_p : dcc_m;
     double Fv[L::NEQ];
+    const R idp = FAST ? frcp((R)dp) : R(0.0), idm = FAST ? frcp((R)dm) : R(0.0);
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) {
-      const R df = (R)(Xf[v] - Sc[v]), db = (R)(Xb[v] - Sc[v]);
+      const double xf = side ? Xm[v] : Xp[v], xb = side ? Xp[v] : Xm[v];
+      const R df = (R)(xf - Sc[v]), db = (R)(xb - Sc[v]);
       const R lim = (v >= L::I4 && v < L::I5) ? R(0.0) : R(1.0);
       const R dpr = (R)dp, dmr = (R)dm;
       R g = R(0.0);
-      if (kf != FV_FK_BOUNDARY && fabs(df) > R(1.0e-10)) g = lim * m5_slope<R>(mc, df, db, dpr, dmr);
+      if (kf != FV_FK_BOUNDARY && fabs(df) > R(1.0e-10))
+        g = lim * (FAST ? m5_slope_q<R>(mc, -db * idm, df * idp) : m5_slope<R>(mc, df, db, dpr, dmr));
       Fv[v] = Sc[v] + dcf * (double)g;
     }
     // VF limit (m5_reconctruct.c:142-170)
@@ -1797,6 +1833,7 @@ __global__ __launch_bounds__(256) void k_m5_recon(int n, M5P P, const Geo *__res
     }
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) SF[((size_t)f * L::NEQ + v) * n + i] = Fv[v];
+  }
   }
 }
 
@@ -2466,8 +2503,12 @@ dim3 grid_of(int n) { return dim3((unsigned)((n + 255) / 256)); }
 template <int NM, typename R, bool RX3>
 void m5_recon_launch(fv_m5_dev *c) {
   if (RX3) hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c->stream, c->count + 1, 2);
-  hipLaunchKernelGGL((k_m5_recon<NM, R, RX3>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
-                     c->nbr, c->info, c->fine4, c->lim, c->SF, c->nst, c->iters, c->count + 1, c->list6, c->flag);
+  if (c->P.fast)
+    hipLaunchKernelGGL((k_m5_recon<NM, R, RX3, true>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
+                       c->nbr, c->info, c->fine4, c->lim, c->SF, c->nst, c->iters, c->count + 1, c->list6, c->flag);
+  else
+    hipLaunchKernelGGL((k_m5_recon<NM, R, RX3, false>), grid_of(c->n), dim3(256), 0, c->stream, c->n, c->P, c->G, c->S,
+                       c->nbr, c->info, c->fine4, c->lim, c->SF, c->nst, c->iters, c->count + 1, c->list6, c->flag);
 }
 
 template <int NM>
