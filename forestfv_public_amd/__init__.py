@@ -216,6 +216,7 @@ def lib():
             "fv_dev_selftest_guard": (C.c_int, []),
             "fv_dev_set_bricks": (C.c_int, [C.c_void_p, C.c_int]),
             "fv_dev_brick_count": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
+            "fv_dev_brick_kinds": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
             "fv_cfl_dt_noalloc": (C.c_int, [P(fv_params), P(fv_geo), C.c_int, C.c_void_p, C.c_void_p, C.c_double,
                                             C.c_void_p, C.c_void_p]),
             "fv_dev_stream_copy": (C.c_int, [C.c_size_t, C.c_int, P(C.c_double)]),
@@ -867,13 +868,22 @@ class Device:
         _check(lib().fv_dev_set_fused(self._h, 1 if on else 0), "fv_dev_set_fused")
 
     def set_bricks(self, on):
-        """4x4x4 same-level blocks through the LDS block kernel (default on) or every leaf per leaf."""
-        _check(lib().fv_dev_set_bricks(self._h, 1 if on else 0), "fv_dev_set_bricks")
+        """4x4x4 same-level blocks through the LDS block kernel: True or 2 (the default) plain and
+        graded blocks (2:1 or 1:4 faces on the surface), 1 plain blocks only, False or 0 every leaf
+        per leaf."""
+        mode = (2 if on else 0) if isinstance(on, bool) else int(on)
+        _check(lib().fv_dev_set_bricks(self._h, mode), "fv_dev_set_bricks")
 
     def brick_count(self):
         """(blocks, leaves outside blocks) of the current mesh."""
         a, b = C.c_int(), C.c_int()
         _check(lib().fv_dev_brick_count(self._h, C.byref(a), C.byref(b)), "fv_dev_brick_count")
+        return a.value, b.value
+
+    def brick_kinds(self):
+        """(plain blocks, graded blocks) of the current mesh."""
+        a, b = C.c_int(), C.c_int()
+        _check(lib().fv_dev_brick_kinds(self._h, C.byref(a), C.byref(b)), "fv_dev_brick_kinds")
         return a.value, b.value
 
     def cfl_dt(self, cfl):

@@ -52,7 +52,7 @@ struct fv_dev {
   int broken = 0;
   // 4x4x4 same-level blocks of the mesh for the block stage kernel (FP64 and
   // mixed stages): their first leaves, the leaves outside them, a per-leaf flag
-  int use_bricks = 1, brick_ok = 0, brick_cap = 0, nbricks = 0, nrest = 0;
+  int use_bricks = 2, brick_ok = 0, brick_cap = 0, nbricks = 0, nplain = 0, ngraded = 0, nrest = 0;
   int *bricks = nullptr, *rest = nullptr, *bmark = nullptr, *bcnt = nullptr;
   uint8_t *inbrick = nullptr, *notbrick = nullptr;
   void *bcub = nullptr;

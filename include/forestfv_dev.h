@@ -71,15 +71,20 @@ int  fv_dev_set_precision(fv_dev *ctx, int prec);
 int  fv_dev_set_fused(fv_dev *ctx, int on);
 
 /* FV_PREC_FP64 / FV_PREC_MIXED stages with stored face states (fused off):
- * 1 (default) every 4x4x4 block of 64 consecutive same-level leaves whose
+ * with on = 1 every 4x4x4 block of 64 consecutive same-level leaves whose
  * surface faces have same-level neighbours or boundaries runs the LDS block
  * kernel (each face evaluated once, no face states in HBM), the other leaves
- * the per-leaf kernels; 0 every leaf runs the per-leaf kernels.  Both give
- * the same state up to rounding (within 1e-13 relative per stage).
+ * the per-leaf kernels; on = 2 (the default) also takes the blocks with 2:1
+ * or 1:4 faces on their surface whose neighbours across those faces are
+ * outside blocks ("graded" blocks, evaluated as the per-leaf flux evaluates
+ * those faces); 0 every leaf runs the per-leaf kernels.  All give the same
+ * state up to rounding (within 1e-13 relative per stage).
  * fv_dev_brick_count: the blocks of the current mesh and the leaves outside
- * them (no counterpart in the reference). */
+ * them; fv_dev_brick_kinds: how many of the blocks are plain and graded (no
+ * counterpart in the reference). */
 int  fv_dev_set_bricks(fv_dev *ctx, int on);
 int  fv_dev_brick_count(fv_dev *ctx, int *nbricks, int *nrest);
+int  fv_dev_brick_kinds(fv_dev *ctx, int *nplain, int *ngraded);
 
 /* Face states of the last stage, SF as [n][6][NEQ] (LEAF.SF); written by the
  * stages of FV_PREC_EXACT and of the unfused throughput path with the block

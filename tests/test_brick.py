@@ -1,7 +1,8 @@
 """The LDS block stage kernel (k_stage_brick) against the per-leaf kernels.
 
 A brick is 64 consecutive leaves in DFS order forming a 4x4x4 block of one
-level whose surface faces have same-level neighbours or boundaries.  The
+level whose surface faces have same-level neighbours or boundaries (plain), or
+also 2:1 and 1:4 faces whose neighbours are outside bricks (graded).  The
 block kernel evaluates every face once and keeps face states in LDS; the
 per-leaf kernels evaluate each face from both sides.  The two give the same
 state to rounding (the HLLC flux is antisymmetric under exchange of sides
@@ -23,11 +24,14 @@ def say(*a):
     print(*a, flush=True)
 
 
-def expected_bricks(m, params):
+def expected_bricks(m, params, graded=True):
     """Brick starts by integer geometry: 64 consecutive same-level leaves at global positions
-    base + Morton(l), whose surface faces touch one same-level leaf or the domain boundary."""
+    base + Morton(l).  Plain: every surface face touches one same-level leaf or the domain
+    boundary.  Graded (when on): the others, less those with a leaf in any candidate across one
+    of their 1:4 faces.  Returns (plain starts, graded starts)."""
     a = m.arrays()
     lvl, ci, root = a["level"], a["ci"], a["root"]
+    lfc, ntr = a["lfc_neigs"], a["neigtr"]
     nr = m.m.nroot
     n = m.n
     L = int(lvl.max())
@@ -37,25 +41,30 @@ def expected_bricks(m, params):
     g = rx * (D // 1) + ci * size[:, None]          # global position of each leaf at the finest level
     mort = np.array([[(l & 1) | (((l >> 3) & 1) << 1), ((l >> 1) & 1) | (((l >> 4) & 1) << 1),
                       ((l >> 2) & 1) | (((l >> 5) & 1) << 1)] for l in range(64)])
-    starts = []
+    plain, cand = [], []
     i = 0
     while i + 63 < n:
-        if np.all(lvl[i:i + 64] == lvl[i]):
-            want = g[i] + mort * size[i]
-            if np.array_equal(g[i:i + 64], want) and np.all(a["lfc_neigs"][i:i + 64] <= 1):
-                # surface faces: same-level neighbours or boundary (lfc 0); no coarser neighbour
-                ok = True
-                for k in range(64):
-                    for f in range(6):
-                        j = a["neigtr"][i + k, f, 0]
-                        if a["lfc_neigs"][i + k, f] == 1 and lvl[j] != lvl[i]:
-                            ok = False
-                if ok:
-                    starts.append(i)
-                    i += 64
-                    continue
+        if np.all(lvl[i:i + 64] == lvl[i]) and np.array_equal(g[i:i + 64], g[i] + mort * size[i]):
+            # surface faces: same-level neighbours or boundary (lfc 0) make a plain brick
+            ok = True
+            for k in range(64):
+                for f in range(6):
+                    j = ntr[i + k, f, 0]
+                    if lfc[i + k, f] == 4 or (lfc[i + k, f] == 1 and lvl[j] != lvl[i]):
+                        ok = False
+            (plain if ok else cand).append(i)
+            i += 64
+            continue
         i += 1
-    return starts
+    if not graded:
+        return plain, []
+    member = np.zeros(n, bool)
+    for b in plain + cand:
+        member[b:b + 64] = True
+    kept = [b for b in cand
+            if not any(member[ntr[b + k, f, q]] for k in range(64) for f in range(6) if lfc[b + k, f] == 4
+                       for q in range(4))]
+    return plain, kept
 
 
 def compare_paths(p, m, S, nstage=3, prec=F.PREC_FP64, SI=None, tol=1e-11):
@@ -97,9 +106,17 @@ def test_brick_list_matches_brute_force_on_amr_mesh():
         dev = F.Device(p, m)
         nb, nrest = dev.brick_count()
         dev.close()
-        want = expected_bricks(m, p)
-        say(f"[bricks] n={m.n} device {nb} bricks, brute force {len(want)}")
-        assert nb == len(want) and nrest == m.n - 64 * nb
+        plain, graded = expected_bricks(m, p)
+        dev = F.Device(p, m)
+        npl, ngr = dev.brick_kinds()
+        dev.set_bricks(1)
+        nb1, nrest1 = dev.brick_count()
+        dev.close()
+        say(f"[bricks] n={m.n} device {nb} bricks ({npl} plain, {ngr} graded), brute force {len(plain)} + "
+            f"{len(graded)}")
+        assert (npl, ngr) == (len(plain), len(graded))
+        assert nb == npl + ngr and nrest == m.n - 64 * nb
+        assert nb1 == npl and nrest1 == m.n - 64 * npl
         assert nb > 0 and nrest > 0
 
 
@@ -147,6 +164,71 @@ def test_brick_path_on_amr_with_interface_flags(fix):
     SI[rng.random(m.n) < 0.3, 0] = 1
     nb, nrest, worst = compare_paths(p, m, S, nstage=3, SI=SI)
     assert nb > 0 and nrest > 0
+
+
+def graded_face_counts(m, graded):
+    """(2:1 faces, 1:4 faces) on the surfaces of the graded blocks: the block finer or coarser."""
+    a = m.arrays()
+    fine_side = coarse_side = 0
+    for b in graded:
+        for k in range(64):
+            for f in range(6):
+                if a["lfc_neigs"][b + k, f] == 4:
+                    coarse_side += 1
+                elif a["lfc_neigs"][b + k, f] == 1 and a["level"][a["neigtr"][b + k, f, 0]] < a["level"][b]:
+                    fine_side += 1
+    return fine_side, coarse_side
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("prelax", [1, 3])
+def test_graded_bricks_match_per_leaf_path(prec, prelax):
+    """Blocks with 2:1 faces (level-5 blocks beside level-4 leaves) and 1:4 faces (level-4 blocks
+    beside level-5 families) on their surface against the per-leaf path, and the graded mode
+    against the plain-only mode, on two AMR meshes of the bubble."""
+    L = 4e-3
+    meshes = ((3, 5, (1, 1, 1), 0.6 * L, L, (1.6e-3, 2e-3, 2.2e-3)),
+              (4, 5, (1, 1, 1), 0.25 * L, L, (1.1e-3, 1.3e-3, 2.9e-3)))
+    pr = {"fp64": F.PREC_FP64, "mixed": F.PREC_MIXED}[prec]
+    tol = 1e-11 if prec == "fp64" else 1e-5
+    kinds = [0, 0]
+    for args in meshes:
+        p = water_air_params(prelax=prelax)
+        m = sphere_amr_mesh(p, *args)
+        S = m.init_bubble(p, (1000.0, 1.2), (5e6, 1e5), (2e-3, 2e-3, 2e-3), 0.8e-3, 2.0, 9, 0.01)
+        p.dt = 0.2 * (L / 32) / 1900.0
+        dev = F.Device(p, m)
+        npl, ngr = dev.brick_kinds()
+        dev.close()
+        plain, graded = expected_bricks(m, p)
+        assert (npl, ngr) == (len(plain), len(graded))
+        fs, cs = graded_face_counts(m, graded)
+        kinds[0] += fs
+        kinds[1] += cs
+        say(f"[graded] {prec} PRELAX {prelax} n={m.n}: {npl} plain, {ngr} graded blocks; {fs} 2:1 and {cs} 1:4 "
+            f"block faces")
+        assert ngr > 0
+        compare_paths(p, m, S, prec=pr, tol=tol)
+        # graded (2) against plain-only (1): the faces the graded blocks take are the per-leaf ones
+        devs = []
+        for mode in (2, 1):
+            d = F.Device(p, m, prec=pr)
+            d.set_bricks(mode)
+            d.set_state(S)
+            devs.append(d)
+        for irk in (1, 2, 3):
+            for d in devs:
+                d.stage(irk)
+        A, B = devs[0].get_state(), devs[1].get_state()
+        for d in devs:
+            d.close()
+        scale = np.abs(B).max(0)
+        scale[2:5] = np.sqrt((B[:, 2:5] ** 2).sum(1)).max()
+        scale[scale == 0] = 1.0
+        err = (np.abs(A - B) / scale).max()
+        say(f"  graded vs plain-only: {err:.3e}")
+        assert np.all(np.isfinite(A)) and err <= tol
+    assert kinds[0] > 0 and kinds[1] > 0
 
 
 def test_brick_path_three_materials():
