@@ -1601,29 +1601,6 @@ __global__ __launch_bounds__(FV_BLOCK) void k_si_first(int n, Params P, const do
   SI[2 * i + 1] = si1;
 }
 
-__global__ __launch_bounds__(FV_BLOCK) void k_si_layer(int n, int lvl, const int *__restrict__ nbr,
-                                                       const uint32_t *__restrict__ info, const int *__restrict__ fine4,
-                                                       const int *__restrict__ SIin, int *__restrict__ SIout) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t inf = info[i];
-  int si[2] = {SIin[2 * i], SIin[2 * i + 1]};
-  int flag = 0;
-  for (int f = 0; f < 6 && !flag; f++) {
-    const int kind = face_kind(inf, f);
-    if (kind == FV_FK_BOUNDARY) continue;
-    const int nb = nbr[(size_t)f * n + i];
-    const int ns = (kind == FV_FK_FINE) ? 4 : 1;
-    for (int k = 0; k < ns; k++) {
-      const int j = sub_neighbour(kind, nb, k, fine4);
-      for (int v = 0; v < 2; ++v)
-        if (SIin[2 * j + v] == (lvl - 1) && si[v] == 0) { si[v] = lvl; flag = 1; }
-    }
-  }
-  SIout[2 * i] = si[0];
-  SIout[2 * i + 1] = si[1];
-}
-
 // New SI: the info word's interface bit and the mirror / same choice of
 // every equal-level face, by the rule of the host packer (a face is computed
 // once, by the later leaf, only when neither leaf is an interface leaf).
@@ -1764,7 +1741,7 @@ int fv_dev_reserve(fv_dev *c, int n_leaves, int nfine) {
       {(void **)&t.SN, sizeof(double) * n * c->neq},     {(void **)&t.SF, sizeof(double) * n * 6 * c->neq},
       {(void **)&t.part, sizeof(double) * (nb + 1)},     {(void **)&t.dtmp, sizeof(double) * 4},
       {(void **)&t.nbr, sizeof(int) * n * 6},            {(void **)&t.fine4, sizeof(int) * nf * 4},
-      {(void **)&t.nanflag, sizeof(int)},                {(void **)&t.iflag, 4 * sizeof(int)},
+      {(void **)&t.nanflag, sizeof(int)},                {(void **)&t.iflag, 8 * sizeof(int)},
       {(void **)&t.bfsq, 2 * n * sizeof(int)},           {(void **)&t.info, sizeof(uint32_t) * n},
       {(void **)&t.SIa, sizeof(int) * n * 2},            {(void **)&t.SIb, sizeof(int) * n * 2},
       {(void **)&t.split, sizeof(int) * n},              {(void **)&t.lim, n},
@@ -2428,6 +2405,105 @@ __global__ __launch_bounds__(FV_BLOCK) void k_si_bfs_final(int n, int dmax, cons
   SI[2 * i] = (x <= dmax) ? x : 0;
 }
 
+// Two volume fractions (three materials).  The reference's face loop stops at
+// the first face where either fraction's layer is set (interface_location.c:
+// 60-80), so the two layers are coupled and a leaf is decided by its faces in
+// order.  Layer lvl still changes only leaves with a face neighbour in layer
+// lvl-1 of some fraction, and such a neighbour is in the frontier of the
+// previous layer (the leaves that took a layer there).  So per layer: the
+// frontier pushes its neighbours with an unset fraction that the frontier leaf
+// has at lvl-1 into a candidate list (each leaf once, by an atomic max on a
+// stamp), then each candidate applies the reference's face-ordered rule in
+// place, and the ones that took a layer form the next frontier.  In place is
+// exact: the rule reads its neighbours' values equal to lvl-1, which no lane
+// writes in this layer, and a value being written goes from 0 to lvl.
+__global__ __launch_bounds__(FV_BLOCK) void k_si3_seed(int n, const int *__restrict__ SI, int *__restrict__ stamp,
+                                                       int *__restrict__ Q, int *__restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  const bool seed = live && (SI[2 * i] == 1 || SI[2 * i + 1] == 1);
+  if (live) stamp[i] = 0;
+  bfs_append(seed, i, Q, cnt);
+}
+
+// the frontier (layer lvl-1) pushes candidates for layer lvl
+__global__ __launch_bounds__(FV_BLOCK) void k_si3_push(int n, int lvl, const int *__restrict__ nbr,
+                                                       const uint32_t *__restrict__ info,
+                                                       const int *__restrict__ fine4, const int *__restrict__ SI,
+                                                       int *__restrict__ stamp, const int *__restrict__ Qin,
+                                                       const int *__restrict__ cin, int *__restrict__ C,
+                                                       int *__restrict__ cc, int *__restrict__ cnext) {
+  const int total = *cin;
+  const int stride = gridDim.x * blockDim.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *cnext = 0;   // the frontier counter two layers on is free again
+  const int rounds = (total + stride - 1) / stride;       // the same for every lane: full waves append
+  for (int r = 0; r < rounds; r++) {
+    const int t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < total;
+    const int j = live ? Qin[t] : 0;
+    const uint32_t inf = live ? info[j] : 0u;
+    const bool w0 = live && SI[2 * j] == lvl - 1, w1 = live && SI[2 * j + 1] == lvl - 1;
+    for (int f = 0; f < 6; f++) {
+      const int kind = face_kind(inf, f);
+      const bool on = live && kind != FV_FK_BOUNDARY;
+      const int nb = on ? nbr[(size_t)f * n + j] : 0;
+      const int ns = (on && kind == FV_FK_FINE) ? 4 : 1;
+      for (int k = 0; k < 4; k++) {
+        bool take = false;
+        int c = 0;
+        if (on && k < ns) {
+          c = sub_neighbour(kind, nb, k, fine4);
+          const bool want = (w0 && SI[2 * c] == 0) || (w1 && SI[2 * c + 1] == 0);
+          take = want && atomicMax(&stamp[c], lvl) < lvl;
+        }
+        bfs_append(take, c, C, cc);
+      }
+    }
+  }
+}
+
+// each candidate applies the rule of interface_location.c:60-80 in place; the
+// ones that took a layer are the next frontier
+__global__ __launch_bounds__(FV_BLOCK) void k_si3_pull(int n, int lvl, const int *__restrict__ nbr,
+                                                       const uint32_t *__restrict__ info,
+                                                       const int *__restrict__ fine4, int *__restrict__ SI,
+                                                       const int *__restrict__ C, const int *__restrict__ cc,
+                                                       int *__restrict__ Qout, int *__restrict__ cout,
+                                                       int *__restrict__ ccnext) {
+  const int total = *cc;
+  const int stride = gridDim.x * blockDim.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ccnext = 0;   // the other candidate counter is free again
+  const int rounds = (total + stride - 1) / stride;
+  for (int r = 0; r < rounds; r++) {
+    const int t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < total;
+    const int i = live ? C[t] : 0;
+    bool took = false;
+    if (live) {
+      const uint32_t inf = info[i];
+      int si[2] = {SI[2 * i], SI[2 * i + 1]};
+      int flag = 0;
+      for (int f = 0; f < 6 && !flag; f++) {
+        const int kind = face_kind(inf, f);
+        if (kind == FV_FK_BOUNDARY) continue;
+        const int nb = nbr[(size_t)f * n + i];
+        const int ns = (kind == FV_FK_FINE) ? 4 : 1;
+        for (int k = 0; k < ns; k++) {
+          const int j = sub_neighbour(kind, nb, k, fine4);
+          for (int v = 0; v < 2; ++v)
+            if (SI[2 * j + v] == (lvl - 1) && si[v] == 0) { si[v] = lvl; flag = 1; }
+        }
+      }
+      if (flag) {
+        SI[2 * i] = si[0];
+        SI[2 * i + 1] = si[1];
+        took = true;
+      }
+    }
+    bfs_append(took, i, Qout, cout);
+  }
+}
+
 // interface_location (interface_location.c:7-86): interface leaves, then
 // imax-2 layers grown one face neighbour at a time; *out is the buffer that
 // holds the final SI [n][2].
@@ -2453,10 +2529,19 @@ int fv_dev_si_layers(fv_dev *c, const fv_crit *cr, int **out) {
                          Q[lvl & 1], cnt + a, Q[(lvl + 1) & 1], cnt + b, cnt + z);
     }
     hipLaunchKernelGGL(k_si_bfs_final, grid, block, 0, c->stream, n, dmax, X, cur);
-  } else {
+  } else if (imax > 2) {
+    // three materials: frontier, candidates and in-place layers (k_si3_*)
+    int *stamp = nxt, *C = nxt + n;                 // the spare SI buffer holds 2n ints
+    int *Q[2] = {c->bfsq, c->bfsq + n};
+    int *fc = c->iflag, *ccnt = c->iflag + 3;       // three rotating frontier and two candidate counters
+    HIPCHK(hipMemsetAsync(c->iflag, 0, 5 * sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_si3_seed, grid, block, 0, c->stream, n, cur, stamp, Q[0], fc);
     for (int lvl = 2; lvl < imax; lvl++) {
-      hipLaunchKernelGGL(k_si_layer, grid, block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur, nxt);
-      int *t = cur; cur = nxt; nxt = t;
+      const int a = (lvl - 2) % 3, b = (lvl - 1) % 3, z = lvl % 3;
+      hipLaunchKernelGGL(k_si3_push, dim3(BFS_GRID), block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur,
+                         stamp, Q[lvl & 1], fc + a, C, ccnt + (lvl & 1), fc + z);
+      hipLaunchKernelGGL(k_si3_pull, dim3(BFS_GRID), block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur, C,
+                         ccnt + (lvl & 1), Q[(lvl + 1) & 1], fc + b, ccnt + ((lvl + 1) & 1));
     }
   }
   HIPCHK(hipGetLastError());

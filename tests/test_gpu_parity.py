@@ -14,7 +14,8 @@ import pytest
 
 import forestfv_public_amd as F
 import oracle_lib as O
-from cases import bubble_case, random_amr_mesh, sod_case, sod_params, three_material_case, water_air_params
+from cases import (THREE_RHO, bubble_case, random_amr_mesh, sod_case, sod_params, sphere_amr_mesh,
+                   three_material_case, three_material_params, water_air_params)
 
 pytestmark = pytest.mark.gpu
 
@@ -284,6 +285,53 @@ def test_interface_layers_with_frontiers_larger_than_one_push_round():
     dev.set_state(S)
     for _ in range(3):   # the queues and counters are reused from call to call
         assert np.array_equal(dev.interface_location(crit), SIo)
+    dev.close()
+
+
+def three_material_blend(p, m, weights):
+    """Cell states blended from the three pure materials at 1e6 Pa by per-leaf weights [n][3]."""
+    pure = np.stack([F.init_pure(p, THREE_RHO, k, 1.0e6) for k in range(3)])
+    w = weights / weights.sum(1, keepdims=True)
+    return w @ pure
+
+
+@pytest.mark.parametrize("case", ["slabs", "spheres_amr", "blocks_64"])
+def test_three_material_interface_layers_both_fractions_match_oracle(case):
+    """Three materials with both interface indicators on: the two fractions' layers grow
+    into each other, and the reference's face loop, which stops at the first face where
+    either fraction takes a layer (interface_location.c:60-80), couples them.  The
+    frontier search (k_si3_*) against the oracle's full passes, three calls in a row
+    (queues, stamps and counters are reused); on 64^3 with 8^3 blocks of the three
+    materials the frontiers hold more leaves than one round of the fixed push grid."""
+    crit = F.make_crit(inter0_on=1, inter0_lvl=7, inter0_lrs=10, inter1_on=1, inter1_lvl=6, inter1_lrs=9)
+    if case == "slabs":
+        p, m, S = three_material_case(level=5, prelax=0)
+    elif case == "spheres_amr":
+        p = three_material_params(prelax=0, level=5)
+        m = sphere_amr_mesh(p, 3, 5, (1, 1, 1), 0.3, 1.0, (0.45, 0.5, 0.55))
+        c = m.centres()
+        r0 = np.linalg.norm(c - np.array([0.4, 0.5, 0.5]), axis=1)
+        r1 = np.linalg.norm(c - np.array([0.62, 0.45, 0.55]), axis=1)
+        w0 = (r0 < 0.18).astype(float)
+        w1 = ((r1 < 0.15) & (w0 == 0)).astype(float)
+        S = three_material_blend(p, m, np.stack([w0, w1, 1.0 - w0 - w1], 1) + 1e-3)
+    else:
+        p = three_material_params(prelax=0, level=7)
+        m = F.Mesh.uniform((1, 1, 1), (0.0, 0.0, 0.0), 1.0, 7, list(p.bc))
+        blk = np.floor(m.centres() * 8).astype(int).sum(axis=1) % 3
+        S = three_material_blend(p, m, np.eye(3)[blk] + 1e-3)
+    SIo = O.OracleMesh(m).interface_location(p, crit, S)
+    h0, h1 = np.bincount(SIo[:, 0]), np.bincount(SIo[:, 1])
+    say(f"[three-material layers] {case} n={m.n} layers 0: {h0[:6]}.. max {SIo[:, 0].max()}, "
+        f"1: {h1[:6]}.. max {SIo[:, 1].max()}")
+    assert SIo[:, 0].max() > 3 and SIo[:, 1].max() > 3
+    if case == "blocks_64":
+        assert h0[1] + h1[1] > 65536
+    dev = F.Device(p, m)
+    dev.set_state(S)
+    for _ in range(3):
+        SIg = dev.interface_location(crit)
+        assert np.array_equal(SIg, SIo)
     dev.close()
 
 
