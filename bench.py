@@ -553,7 +553,7 @@ def c2_loop(dev, crit, maxlevel, nsteps, every=5, cfl=0.4, t_end=None, probe=0, 
         ms_res_tot += 3 * ms_res
         cells += 3 * dev.n
         t0 = time.perf_counter()
-        dev.interface_location(crit)
+        dev.interface_location(crit, copy=False)   # SI stays on the device, as loop.c needs it
         t_other += time.perf_counter() - t0
         t += dt
         step += 1
@@ -589,7 +589,11 @@ def c2_leg(maxlevel=9, base=5, layers=64, steps=15, every=5, check=True, log_fn=
     B = algorithmic_bytes(F.m5_neq(3))
     out.update({"steps": r["steps"], "leaves": r["leaves"], "ms_per_stage": round(r["ms_stage"], 3),
                 "ms_residual_kernel": round(r["ms_residual"], 3), "value": round(r["stage_rate"], 2),
-                "loop_value": round(r["loop_rate"], 2), "finite": bool(np.all(np.isfinite(Sg))),
+                "loop_value": round(r["loop_rate"], 2),
+                "loop_ms_per_step": {"wall": round(1e3 * r["wall_s"] / r["steps"], 2),
+                                     "stages": round(3 * r["ms_stage"], 2),
+                                     "cfl_regrid_interface_location": round(1e3 * r["other_s"] / r["steps"], 2)},
+                "finite": bool(np.all(np.isfinite(Sg))),
                 "unit": "Mcell-updates/s per RK stage; loop_value with CFL, interface_location and regrids"})
     out["roofline"] = {"bound": "hbm", "algorithmic_bytes_per_leaf": B, "peak": HBM_PEAK_GBPS,
                        "achieved": round(B * r["stage_rate"] * 1e6 / 1e9, 1)}

@@ -501,10 +501,12 @@ class M5Device:
         _check(lib().fv_m5_dev_get_forest(self.h, _ip(root), _ip(level), _ip(ci), _ip(SI)), "fv_m5_dev_get_forest")
         return root, level, ci, SI
 
-    def interface_location(self, crit):
-        """interface_location after a step: SI of the device state, face kinds refreshed; SI [n][2]."""
-        SI = np.zeros((self.n, 2), np.int32)
-        _check(lib().fv_m5_dev_interface_location(self.h, C.byref(crit), _ip(SI)), "fv_m5_dev_interface_location")
+    def interface_location(self, crit, copy=True):
+        """interface_location after a step: SI of the device state, face kinds refreshed; SI [n][2]
+        (copy=False: SI stays on the device, as the time loop needs it, and None is returned)."""
+        SI = np.zeros((self.n, 2), np.int32) if copy else None
+        _check(lib().fv_m5_dev_interface_location(self.h, C.byref(crit), _ip(SI) if copy else None),
+               "fv_m5_dev_interface_location")
         return SI
 
     def indicator(self, crit):
@@ -901,10 +903,12 @@ class Device:
         _check(lib().fv_dev_indicator(self._h, C.byref(crit), _ip(SI), _ip(split)), "fv_dev_indicator")
         return SI, split
 
-    def interface_location(self, crit):
-        """SI from the current state; the following stages use it (loop.c:23)."""
-        SI = np.zeros((self.n, 2), dtype=np.int32)
-        _check(lib().fv_dev_interface_location(self._h, C.byref(crit), _ip(SI)), "fv_dev_interface_location")
+    def interface_location(self, crit, copy=True):
+        """SI from the current state; the following stages use it (loop.c:23).  copy=False: SI stays
+        on the device and None is returned."""
+        SI = np.zeros((self.n, 2), dtype=np.int32) if copy else None
+        _check(lib().fv_dev_interface_location(self._h, C.byref(crit), _ip(SI) if copy else None),
+               "fv_dev_interface_location")
         return SI
 
     def set_limiter_flags(self, lim):

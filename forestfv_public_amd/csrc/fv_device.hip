@@ -2313,6 +2313,7 @@ extern "C" int fv_dev_cfl_dt(fv_dev *c, double cfl, double *dt) {
 // queue order varies from run to run; the layer values do not.
 constexpr int BFS_INF = 1 << 28;
 constexpr int BFS_GRID = 256;
+constexpr int BFS3_GRID = 1024;   // the three-material candidates: one round of lanes
 
 // wave-aggregated append of leaf i (when take) to queue Q with counter *cnt
 __device__ __forceinline__ void bfs_append(bool take, int i, int *__restrict__ Q, int *__restrict__ cnt) {
@@ -2540,7 +2541,7 @@ int fv_dev_si_layers(fv_dev *c, const fv_crit *cr, int **out) {
       const int a = (lvl - 2) % 3, b = (lvl - 1) % 3, z = lvl % 3;
       hipLaunchKernelGGL(k_si3_push, dim3(BFS_GRID), block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur,
                          stamp, Q[lvl & 1], fc + a, C, ccnt + (lvl & 1), fc + z);
-      hipLaunchKernelGGL(k_si3_pull, dim3(BFS_GRID), block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur, C,
+      hipLaunchKernelGGL(k_si3_pull, dim3(BFS3_GRID), block, 0, c->stream, n, lvl, c->nbr, c->info, c->fine4, cur, C,
                          ccnt + (lvl & 1), Q[(lvl + 1) & 1], fc + b, ccnt + ((lvl + 1) & 1));
     }
   }
