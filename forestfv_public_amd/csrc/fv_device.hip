@@ -293,7 +293,10 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_rk(int n, Params P, Geo G, in
 // pressure recovery and RK accumulation)
 // ---------------------------------------------------------------------
 #pragma clang fp contract(fast)
-template <int NM, bool MIXED, bool XR>
+// AX: one lane per (leaf, axis) instead of per leaf, for the short list of
+// leaves outside blocks, whose launch holds too few waves to hide the latency
+// of a lane's three axes in turn; the arithmetic of each face is the same
+template <int NM, bool MIXED, bool XR, bool AX>
 __global__ __launch_bounds__(FV_BLOCK, 2) void k_recon_fast(int n, Params P, Geo G, const double *__restrict__ S,
                                                          const int *__restrict__ nbr, const uint32_t *__restrict__ info,
                                                          const int *__restrict__ fine4, const uint8_t *__restrict__ lim,
@@ -302,8 +305,10 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_recon_fast(int n, Params P, Geo
   using L = Lay<NM>;
   using R = typename std::conditional<MIXED, float, double>::type;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnt) return;
-  const int i = list ? list[t] : t;   // every leaf, or the leaves of a list
+  if (t >= (AX ? 3 * cnt : cnt)) return;
+  const int li = AX ? t / 3 : t;
+  const int ax0 = AX ? t - 3 * li : 0, ax1 = AX ? ax0 + 1 : 3;
+  const int i = list ? list[li] : li;   // every leaf, or the leaves of a list
   const fvf::Mat<NM, double> M = fvf::make_mat<NM, double>(P);
   const double a = P.amin / 100.0;
   const uint32_t inf = info[i];
@@ -315,17 +320,24 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_recon_fast(int n, Params P, Geo
 #pragma unroll
   for (int k = 0; k < L::N3; k++) Araw[k] = Sc[L::I3 + k];
   avf_limit<NM, double>(a, Sc);
-  // the six neighbour ids are requested together with the cell state, so each
+  // the neighbour ids are requested together with the cell state, so each
   // axis waits on one round trip (the stencil states) instead of two
-  const int nb0 = nbr[i], nb1 = nbr[(size_t)n + i], nb2 = nbr[(size_t)2 * n + i];
-  const int nb3 = nbr[(size_t)3 * n + i], nb4 = nbr[(size_t)4 * n + i], nb5 = nbr[(size_t)5 * n + i];
+  int nb0 = 0, nb1 = 0, nb2 = 0, nb3 = 0, nb4 = 0, nb5 = 0;
+  if constexpr (AX) {
+    const int fa = (ax0 == 0) ? 0 : (ax0 == 1 ? 1 : 4);
+    nb0 = nbr[(size_t)fa * n + i];
+    nb2 = nbr[(size_t)FCOPP_D[fa] * n + i];
+  } else {
+    nb0 = nbr[i]; nb1 = nbr[(size_t)n + i]; nb2 = nbr[(size_t)2 * n + i];
+    nb3 = nbr[(size_t)3 * n + i]; nb4 = nbr[(size_t)4 * n + i]; nb5 = nbr[(size_t)5 * n + i];
+  }
 #pragma unroll 1
-  for (int ax = 0; ax < 3; ax++) {
+  for (int ax = ax0; ax < ax1; ax++) {
     const int fa = (ax == 0) ? 0 : (ax == 1 ? 1 : 4);
     const int fb = FCOPP_D[fa];
     const int ka = face_kind(inf, fa), kb = face_kind(inf, fb);
-    const int na = (ax == 0) ? nb0 : (ax == 1 ? nb1 : nb4);
-    const int nbb = (ax == 0) ? nb2 : (ax == 1 ? nb3 : nb5);
+    const int na = AX ? nb0 : ((ax == 0) ? nb0 : (ax == 1 ? nb1 : nb4));
+    const int nbb = AX ? nb2 : ((ax == 0) ? nb2 : (ax == 1 ? nb3 : nb5));
     double Xa[L::NEQ], Xb[L::NEQ];
     stencil_value<NM>(S, n, i, ka, na, fine4, Sc, Araw, a, Xa);
     stencil_value<NM>(S, n, i, kb, nbb, fine4, Sc, Araw, a, Xb);
@@ -2117,12 +2129,20 @@ static int launch_stage_nm(fv_dev *c, int cs, hipEvent_t *ev_flux0, hipEvent_t *
     const int *list = bricks ? c->rest : nullptr;
     const dim3 gl((cnt + FV_BLOCK - 1) / FV_BLOCK);
     if (cnt > 0) {
-      if (mixed)
-        hipLaunchKernelGGL((k_recon_fast<NM, true, XR>), gl, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
-                           c->fine4, c->lim, c->SF, c->SFm, list, cnt);
+      // the leaves outside blocks: one lane per (leaf, axis); every leaf: one per leaf
+      const dim3 ga((3 * cnt + FV_BLOCK - 1) / FV_BLOCK);
+      if (mixed && bricks)
+        hipLaunchKernelGGL((k_recon_fast<NM, true, XR, true>), ga, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr,
+                           c->info, c->fine4, c->lim, c->SF, c->SFm, list, cnt);
+      else if (mixed)
+        hipLaunchKernelGGL((k_recon_fast<NM, true, XR, false>), gl, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr,
+                           c->info, c->fine4, c->lim, c->SF, c->SFm, list, cnt);
+      else if (bricks)
+        hipLaunchKernelGGL((k_recon_fast<NM, false, XR, true>), ga, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr,
+                           c->info, c->fine4, c->lim, c->SF, c->SFm, list, cnt);
       else
-        hipLaunchKernelGGL((k_recon_fast<NM, false, XR>), gl, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr, c->info,
-                           c->fine4, c->lim, c->SF, c->SFm, list, cnt);
+        hipLaunchKernelGGL((k_recon_fast<NM, false, XR, false>), gl, block, 0, c->stream, n, c->P, c->G, c->S, c->nbr,
+                           c->info, c->fine4, c->lim, c->SF, c->SFm, list, cnt);
     }
     if (ev_flux0) HIPCHK(hipEventRecord(*ev_flux0, c->stream));
     if (bricks && c->nbricks > 0) {
