@@ -243,14 +243,26 @@ def test_brick_path_fixed_pressure_boundaries():
     compare_paths(p, m, S)
 
 
-def test_brick_path_100_steps_against_the_oracle():
-    """The default FP64 stage (block kernel on) over 100 steps of the bubble on an AMR
-    mesh: relative L1 <= 1e-10 per conserved field against the CPU restatement."""
+@pytest.mark.parametrize("mesh", ["fine_blocks", "coarse_blocks"])
+def test_brick_path_100_steps_against_the_oracle(mesh):
+    """The default FP64 stage (plain and graded blocks on) over 100 steps of the bubble on an
+    AMR mesh: relative L1 <= 1e-10 per conserved field against the CPU restatement.  The
+    first mesh's graded blocks have 2:1 faces (finer than their neighbours), the second's
+    1:4 faces (coarser than them)."""
     p = water_air_params(prelax=1)
-    m = sphere_amr_mesh(p, 3, 5, (1, 1, 1), 0.6 * 2.0 ** -8, 2.0 ** -8)
-    S = m.init_bubble(p, (1000.0, 1.2), (5e6, 1e5), (2.0 ** -9,) * 3, 0.2 * 2.0 ** -8, 2.0, 5, 0.01)
-    p.dt = 0.3 * (2.0 ** -8 / 16) / 1900.0
+    L = 2.0 ** -8
+    if mesh == "fine_blocks":
+        m = sphere_amr_mesh(p, 3, 5, (1, 1, 1), 0.6 * L, L, (0.4 * L, 0.5 * L, 0.55 * L))
+    else:
+        m = sphere_amr_mesh(p, 4, 5, (1, 1, 1), 0.25 * L, L, (0.275 * L, 0.325 * L, 0.725 * L))
+    S = m.init_bubble(p, (1000.0, 1.2), (5e6, 1e5), (2.0 ** -9,) * 3, 0.2 * L, 2.0, 5, 0.01)
+    p.dt = 0.3 * (L / 16) / 1900.0
+    plain, graded = expected_bricks(m, p)
+    fs, cs = graded_face_counts(m, graded)
+    say(f"[brick 100 steps] {mesh}: {len(graded)} graded blocks, {fs} 2:1 and {cs} 1:4 block faces")
+    assert len(graded) > 0 and (fs if mesh == "fine_blocks" else cs) > 0
     dev = F.Device(p, m)
+    assert dev.brick_kinds() == (len(plain), len(graded))
     nb, nrest = dev.brick_count()
     dev.set_state(S)
     dev.advance(100)
