@@ -1018,7 +1018,9 @@ struct BrickLds {
   // the registers of the slot's lane)
   static constexpr int NC = 64 + BR_SLOTS;
   double st[L::NEQ * NC];
-  double sn[L::NEQ * 64];               // S^n of the own cells (RK stages after the first)
+  // S^n of the own cells (RK stages after the first); the mixed kernel reads it
+  // from HBM in the cell phase instead, which leaves room for a sixth workgroup
+  double sn[MIXED ? 1 : L::NEQ * 64];
   double araw[L::N3 * 64];              // the own cells' volume fractions before the limit
   double src[(L::N3 + NM) * 64];        // the own cells' source coefficients: alpha_k, then -alpha_k p_k
   FT ff[L::NEQ * BR_TASKS];             // face fluxes per face task
@@ -1105,7 +1107,7 @@ FV_DEV void br_face_dist(const Geo &G, uint32_t inf, int f, int lvl, double &dcf
 // this cell: one HLLC against the coarser leaf's face state, or four with the
 // finer area against the finer leaves' face states.
 template <int NM, bool MIXED, bool XR, bool GR>
-__global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 5 : 4)) void k_stage_brick(int n, Params P, const Geo *__restrict__ Gp, int cs,
+__global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void k_stage_brick(int n, Params P, const Geo *__restrict__ Gp, int cs,
                                                                const int *__restrict__ bricks, int nbricks,
                                                                const double *__restrict__ S, double *__restrict__ Sout,
                                                                double *__restrict__ SN, const int *__restrict__ nbr,
@@ -1134,7 +1136,7 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 5 : 4)) void
   // (lanes 0..95) the neighbour across it, whose stencil value beyond it the
   // slot's lane keeps in registers for its face task.  No load waits for
   // another lane's.
-  if (cs != 1) {
+  if (!MIXED && cs != 1) {
     for (int q = tid; q < L::NEQ * 64; q += BR_THREADS) {
       const int v = q >> 6, l = q & 63;
       B.sn[v * 64 + l] = SN[(size_t)v * n + b0 + l];
@@ -1416,6 +1418,7 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 5 : 4)) void
   for (int v = 0; v < L::NEQ; v++) {
     double sn;
     if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
+    else if constexpr (MIXED) sn = SN[(size_t)v * n + i];
     else sn = B.sn[v * 64 + l];
     Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
   }
