@@ -279,7 +279,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_rk(int n, Params P, Geo G, in
   for (int v = 0; v < L::NEQ; v++) {
     double sn;
     if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
-    else sn = SN[(size_t)v * n + i];
+    else sn = SN[(size_t)v * n + i];   // S^n is read here, not staged: less LDS, and no staging load waits on it
     Sn[v] = rk_combine(cs, P.dt, Vol, sn, Sc[v], RHS[v]);
   }
   relax<NM>(P, Sn);
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_fast(int n, Params P, Geo G, 
   for (int v = 0; v < L::NEQ; v++) {
     double sn;
     if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
-    else sn = SN[(size_t)v * n + i];
+    else sn = SN[(size_t)v * n + i];   // S^n is read here, not staged: less LDS, and no staging load waits on it
     Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
   }
   fvf::relax_fast<NM, XR>(P, Md, Sn);
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(FV_BLOCK) void k_flux_pipe(int n, Params P, Geo G, 
   for (int v = 0; v < L::NEQ; v++) {
     double sn;
     if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
-    else sn = SN[(size_t)v * n + i];
+    else sn = SN[(size_t)v * n + i];   // S^n is read here, not staged: less LDS, and no staging load waits on it
     Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
   }
   fvf::relax_fast<NM, XR>(P, Md, Sn);
@@ -930,7 +930,7 @@ __global__ __launch_bounds__(FV_BLOCK, 2) void k_stage_fused(int n, Params P, co
     const double s0 = S[(size_t)v * n + i];
     double sn;
     if (cs == 1) { sn = s0; SN[(size_t)v * n + i] = sn; }
-    else sn = SN[(size_t)v * n + i];
+    else sn = SN[(size_t)v * n + i];   // S^n is read here, not staged: less LDS, and no staging load waits on it
     Sn[v] = wa * sn + wb * (s0 - dtv * RHS[v]);
   }
   fvf::relax_fast<NM, XR>(P, Md, Sn);
@@ -1018,9 +1018,6 @@ struct BrickLds {
   // the registers of the slot's lane)
   static constexpr int NC = 64 + BR_SLOTS;
   double st[L::NEQ * NC];
-  // S^n of the own cells (RK stages after the first); the mixed kernel reads it
-  // from HBM in the cell phase instead, which leaves room for a sixth workgroup
-  double sn[MIXED ? 1 : L::NEQ * 64];
   double araw[L::N3 * 64];              // the own cells' volume fractions before the limit
   double src[(L::N3 + NM) * 64];        // the own cells' source coefficients: alpha_k, then -alpha_k p_k
   FT ff[L::NEQ * BR_TASKS];             // face fluxes per face task
@@ -1130,18 +1127,12 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void
   const fvf::Mat<NM, double> Md = fvf::make_mat<NM, double>(P);
   const double a = P.amin / 100.0;
 
-  // ---- one staging phase: S^n (every lane), the own states (lanes 96..159,
+  // ---- one staging phase: the own states (lanes 96..159,
   // one cell each: raw fractions, source coefficients and the avf-limited
   // state), the info words and limiters (lanes 160..223), and per surface slot
   // (lanes 0..95) the neighbour across it, whose stencil value beyond it the
   // slot's lane keeps in registers for its face task.  No load waits for
   // another lane's.
-  if (!MIXED && cs != 1) {
-    for (int q = tid; q < L::NEQ * 64; q += BR_THREADS) {
-      const int v = q >> 6, l = q & 63;
-      B.sn[v * 64 + l] = SN[(size_t)v * n + b0 + l];
-    }
-  }
   double X2[L::NEQ];   // slot lanes: the slot neighbour's stencil value beyond it
 #pragma unroll
   for (int v = 0; v < L::NEQ; v++) X2[v] = 0.0;
@@ -1418,8 +1409,7 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void
   for (int v = 0; v < L::NEQ; v++) {
     double sn;
     if (cs == 1) { sn = Sc[v]; SN[(size_t)v * n + i] = sn; }
-    else if constexpr (MIXED) sn = SN[(size_t)v * n + i];
-    else sn = B.sn[v * 64 + l];
+    else sn = SN[(size_t)v * n + i];   // S^n is read here, not staged: less LDS, and no staging load waits on it
     Sn[v] = wa * sn + wb * (Sc[v] - dtv * RHS[v]);
   }
   fvf::relax_fast<NM, XR>(P, Md, Sn);
