@@ -1018,16 +1018,20 @@ struct BrickLds {
   // the registers of the slot's lane)
   static constexpr int NC = 64 + BR_SLOTS;
   double st[L::NEQ * NC];
-  double araw[L::N3 * 64];              // the own cells' volume fractions before the limit
-  double src[(L::N3 + NM) * 64];        // the own cells' source coefficients: alpha_k, then -alpha_k p_k
+  // the own cells' raw volume fractions and source coefficients (alpha_k, then
+  // -alpha_k p_k); the mixed kernel forms them in the cell phase from the raw
+  // state read there, which with fu in FP32 and idmh formed where it is used
+  // leaves 22.4 KB of LDS, room for a seventh workgroup
+  double araw[MIXED ? 1 : L::N3 * 64];
+  double src[MIXED ? 1 : (L::N3 + NM) * 64];
   FT ff[L::NEQ * BR_TASKS];             // face fluxes per face task
-  double fu[BR_TASKS];                  // u*.n per face task
+  FT fu[BR_TASKS];                      // u*.n per face task
   uint32_t info[64];
   uint32_t infoh[BR_SLOTS];             // info word of the neighbour across each slot
   int nb[BR_SLOTS];                     // that neighbour (leaf index, or -BC code)
   uint8_t need[BR_SLOTS];               // it is no brick leaf: its flux reads this face state from HBM
   uint8_t lim[64 + BR_SLOTS];           // per-leaf limiter of the own cells and the slot neighbours
-  double idmh[BR_SLOTS];                // 1/dcc of the slot neighbour's face beyond it (any kind)
+  double idmh[MIXED ? 1 : BR_SLOTS];    // 1/dcc of the slot neighbour's face beyond it (any kind)
 };
 
 template <int NM, bool MIXED>
@@ -1104,7 +1108,7 @@ FV_DEV void br_face_dist(const Geo &G, uint32_t inf, int f, int lvl, double &dcf
 // this cell: one HLLC against the coarser leaf's face state, or four with the
 // finer area against the finer leaves' face states.
 template <int NM, bool MIXED, bool XR, bool GR>
-__global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void k_stage_brick(int n, Params P, const Geo *__restrict__ Gp, int cs,
+__global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 7 : 4)) void k_stage_brick(int n, Params P, const Geo *__restrict__ Gp, int cs,
                                                                const int *__restrict__ bricks, int nbricks,
                                                                const double *__restrict__ S, double *__restrict__ Sout,
                                                                double *__restrict__ SN, const int *__restrict__ nbr,
@@ -1141,9 +1145,9 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void
     double X[L::NEQ];
 #pragma unroll
     for (int v = 0; v < L::NEQ; v++) X[v] = S[(size_t)v * n + b0 + l];
+    if constexpr (!MIXED) {
 #pragma unroll
-    for (int k = 0; k < L::N3; k++) B.araw[k * 64 + l] = X[L::I3 + k];
-    {
+      for (int k = 0; k < L::N3; k++) B.araw[k * 64 + l] = X[L::I3 + k];
       // the sources of the cell update take the raw state (k_flux_fast's fill_cons of S)
       fvf::SolF<NM, double> sc;
       fvf::fill_cons<NM, double>(Md, a, X, sc);
@@ -1175,7 +1179,7 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void
       B.need[s] = 1;
       B.infoh[s] = 0;
       B.lim[64 + s] = 0;
-      B.idmh[s] = 0.0;
+      if constexpr (!MIXED) B.idmh[s] = 0.0;
     } else if (kf == FV_FK_BOUNDARY) {
       // no state across a boundary face: its slope is zero and its ghost comes from the face state
 #pragma unroll
@@ -1183,7 +1187,7 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void
       B.need[s] = 0;
       B.infoh[s] = 0;
       B.lim[64 + s] = 0;
-      B.idmh[s] = 0.0;
+      if constexpr (!MIXED) B.idmh[s] = 0.0;
     } else {
       const uint32_t infj = info[nb];
       B.infoh[s] = infj;
@@ -1197,7 +1201,7 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void
 #pragma unroll
         for (int k = 0; k < L::N3; k++) Araw[k] = Xj[L::I3 + k];
       }
-      {
+      if constexpr (!MIXED) {
         double dcf_j, dcc_j;
         dist_of(G, kj, cell_level(infj), dcf_j, dcc_j);
         B.idmh[s] = (kj == FV_FK_BOUNDARY) ? 0.0 : fvf::rcp(dcc_j);
@@ -1265,7 +1269,14 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void
         cB = 64 + s; gB = l; oB = l; limB = B.lim[64 + s];   // the column beyond is X2
         oreg = true;
         bndB = face_kind(infj, fo) == FV_FK_BOUNDARY || face_kind(infj, f) == FV_FK_BOUNDARY;
-        idmB = B.idmh[s];
+        if constexpr (MIXED) {
+          const int kj = face_kind(infj, f);
+          double dcf_j, dcc_j;
+          dist_of(G, kj, cell_level(infj), dcf_j, dcc_j);
+          idmB = (kj == FV_FK_BOUNDARY) ? 0.0 : fvf::rcp(dcc_j);
+        } else {
+          idmB = B.idmh[s];
+        }
       }
     } else {
       const int q = tid - BR_SLOTS, ax = q / 48, p = (q % 48) >> 4, k = q & 15;
@@ -1347,15 +1358,26 @@ __global__ __launch_bounds__(BR_THREADS, GR ? 3 : ((MIXED && !XR) ? 6 : 4)) void
   if (tid >= 64) return;
   const int l = tid, i = b0 + l;
   const uint32_t inf = B.info[l];
-  double Sc[L::NEQ];
-  br_state<NM, MIXED>(B, l, Sc);
-#pragma unroll
-  for (int k = 0; k < L::N3; k++) Sc[L::I3 + k] = B.araw[k * 64 + l];   // the raw state
+  double Sc[L::NEQ];   // the raw state
   double src_avf[NM], src_st[NM];
+  if constexpr (MIXED) {
 #pragma unroll
-  for (int k = 0; k < L::N3; ++k) src_avf[k] = B.src[k * 64 + l];
+    for (int v = 0; v < L::NEQ; v++) Sc[v] = S[(size_t)v * n + i];
+    fvf::SolF<NM, double> sc;
+    fvf::fill_cons<NM, double>(Md, a, Sc, sc);
 #pragma unroll
-  for (int k = 0; k < NM; ++k) src_st[k] = B.src[(L::N3 + k) * 64 + l];
+    for (int k = 0; k < L::N3; ++k) src_avf[k] = sc.avf[k];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) src_st[k] = -sc.avf[k] * sc.p[k];
+  } else {
+    br_state<NM, MIXED>(B, l, Sc);
+#pragma unroll
+    for (int k = 0; k < L::N3; k++) Sc[L::I3 + k] = B.araw[k * 64 + l];
+#pragma unroll
+    for (int k = 0; k < L::N3; ++k) src_avf[k] = B.src[k * 64 + l];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) src_st[k] = B.src[(L::N3 + k) * 64 + l];
+  }
   const bool fixnc = (P.fix & 1) != 0;
   const BrPos pos = br_pos(l);
   double RHS[L::NEQ];
